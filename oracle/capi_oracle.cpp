@@ -1,0 +1,142 @@
+// ORACLE — test infrastructure only.
+// extern "C" wrappers so tests can call the oracle through ctypes.  Arrays of
+// dd values are passed as interleaved (hi, lo) doubles, the dd_real layout.
+#include <cstring>
+#include <string>
+
+#include "sdpa_oracle.hpp"
+
+using namespace orc;
+
+static dd_real *D(double *p) { return reinterpret_cast<dd_real *>(p); }
+static const dd_real *D(const double *p) { return reinterpret_cast<const dd_real *>(p); }
+
+extern "C" {
+
+void orc_set_dot_threads(int t) { g_dot_threads = t < 1 ? 1 : t; }
+int orc_get_dot_threads(void) { return g_dot_threads; }
+
+// Elementwise dd ops on n values: op 0 add (a+b), 1 sub, 2 mul, 3 div, 4 sqrt(a),
+// 5 in-place add (a+=b), 6 in-place sub, 7 in-place mul, 8 a*double(b.hi),
+// 9 a/double(b.hi), 10 a+double(b.hi), 11 double(a.hi)/b.
+void orc_dd_op(int op, long n, const double *a, const double *b, double *out) {
+    for (long i = 0; i < n; i++) {
+        dd_real x = D(a)[i], y = D(b)[i], r;
+        switch (op) {
+        case 0: r = x + y; break;
+        case 1: r = x - y; break;
+        case 2: r = x * y; break;
+        case 3: r = x / y; break;
+        case 4: r = sqrt(x); break;
+        case 5: r = x; r += y; break;
+        case 6: r = x; r -= y; break;
+        case 7: r = x; r *= y; break;
+        case 8: r = x * y.x[0]; break;
+        case 9: r = x / y.x[0]; break;
+        case 10: r = x + y.x[0]; break;
+        case 11: r = x.x[0] / y; break;
+        default: r = dd_real(0.0);
+        }
+        D(out)[i] = r;
+    }
+}
+
+int orc_read_dd(const char *s, double *out) {
+    dd_real v;
+    int rc = read_dd(s, v);
+    out[0] = v.x[0];
+    out[1] = v.x[1];
+    return rc;
+}
+
+void orc_Rgemm(char ta, char tb, long m, long n, long k, const double *alpha, const double *A, long lda,
+               const double *B, long ldb, const double *beta, double *C, long ldc) {
+    Rgemm(ta, tb, m, n, k, D(alpha)[0], D(A), lda, D(B), ldb, D(beta)[0], D(C), ldc);
+}
+void orc_Rsyrk_LN(long n, long k, const double *alpha, const double *A, long lda, const double *beta, double *C, long ldc) {
+    Rsyrk_LN(n, k, D(alpha)[0], D(A), lda, D(beta)[0], D(C), ldc);
+}
+void orc_Rtrsm_RLTN(long m, long n, const double *alpha, const double *A, long lda, double *B, long ldb) {
+    Rtrsm_RLTN(m, n, D(alpha)[0], D(A), lda, D(B), ldb);
+}
+void orc_Rtrsm_LLNN(long m, long n, const double *alpha, const double *A, long lda, double *B, long ldb) {
+    Rtrsm_LLNN(m, n, D(alpha)[0], D(A), lda, D(B), ldb);
+}
+void orc_Rtrmm_LLTN(long m, long n, const double *alpha, const double *A, long lda, double *B, long ldb) {
+    Rtrmm_LLTN(m, n, D(alpha)[0], D(A), lda, D(B), ldb);
+}
+void orc_Rtrsv(char trans, long n, const double *A, long lda, double *x) {
+    if (trans == 'N') Rtrsv_LN(n, D(A), lda, D(x));
+    else Rtrsv_LT(n, D(A), lda, D(x));
+}
+void orc_Rtrmv(char trans, long n, const double *A, long lda, double *x) {
+    if (trans == 'N') Rtrmv_LN(n, D(A), lda, D(x));
+    else Rtrmv_LT(n, D(A), lda, D(x));
+}
+void orc_Rgemv_N(long m, long n, const double *alpha, const double *A, long lda, const double *x, const double *beta, double *y) {
+    Rgemv_N(m, n, D(alpha)[0], D(A), lda, D(x), D(beta)[0], D(y));
+}
+void orc_Rdot(long n, const double *x, long incx, const double *y, long incy, double *out) {
+    dd_real r = Rdot(n, D(x), incx, D(y), incy);
+    out[0] = r.x[0];
+    out[1] = r.x[1];
+}
+long orc_Rpotrf_L(long n, double *A, long lda) {
+    mint info = 0;
+    Rpotrf_L(n, D(A), lda, info);
+    return (long)info;
+}
+long orc_Rsteqr_I(long n, double *d, double *e, double *z, long ldz) {
+    std::vector<dd_real> work(2 * (n > 1 ? n : 1));
+    mint info = 0;
+    Rsteqr('I', n, D(d), D(e), D(z), ldz, work.data(), info);
+    return (long)info;
+}
+void orc_lanczos(long n, const double *L, const double *M, double *out) {
+    DenseMatrix l, mm;
+    l.nRow = l.nCol = (int)n;
+    mm.nRow = mm.nCol = (int)n;
+    l.de_ele.assign(D(L), D(L) + n * n);
+    mm.de_ele.assign(D(M), D(M) + n * n);
+    dd_real r = lanczos_min_eigen(l, mm);
+    out[0] = r.x[0];
+    out[1] = r.x[1];
+}
+
+// Loads a .dat-s file, returns m and the number of blocks; block sizes into bs.
+static InputData g_data;
+int orc_load(const char *path, int *m, int *nblock, int *bs, int bs_cap, char *err, int errcap) {
+    g_data = InputData();
+    std::string e;
+    if (!read_dat_s(path, g_data, e)) {
+        std::strncpy(err, e.c_str(), errcap - 1);
+        err[errcap - 1] = 0;
+        return -1;
+    }
+    *m = g_data.m;
+    *nblock = (int)g_data.blockStruct.size();
+    for (int i = 0; i < *nblock && i < bs_cap; i++) bs[i] = g_data.blockStruct[i];
+    return 0;
+}
+// useFormula table (m*nBlock, index i*nBlock+l) of the loaded problem.
+void orc_formula(int *out) {
+    auto f = compute_formula(g_data, dd_real(1.2));
+    for (size_t i = 0; i < f.size(); i++) out[i] = f[i];
+}
+// Schur matrix for the loaded problem with given X and invZ (blocks concatenated).
+void orc_schur(const double *X, const double *invZ, double *B) {
+    DenseLinearSpace x, z;
+    x.initialize(g_data.blockStruct);
+    z.initialize(g_data.blockStruct);
+    size_t off = 0;
+    for (size_t l = 0; l < g_data.blockStruct.size(); l++) {
+        size_t len = x.SDP_block[l].de_ele.size();
+        std::memcpy(x.SDP_block[l].de_ele.data(), X + 2 * off, len * 16);
+        std::memcpy(z.SDP_block[l].de_ele.data(), invZ + 2 * off, len * 16);
+        off += len;
+    }
+    std::vector<dd_real> b((size_t)g_data.m * g_data.m, dd_real(0.0));
+    schur_dense(g_data, x, z, b, compute_formula(g_data, dd_real(1.2)));
+    std::memcpy(B, b.data(), b.size() * 16);
+}
+}

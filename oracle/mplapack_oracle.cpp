@@ -1,0 +1,788 @@
+// ORACLE — test infrastructure only.
+// CPU restatement of the MPLAPACK 2.0.1 dd routines on the SDPA-DD Newton
+// step.  Each function cites the reference routine it restates; loop orders
+// and the form of every dd update (`c += t*a`, `b = b - t*c`, `t = one/a`
+// then `t*b`, ...) follow the reference because the rounding depends on them.
+#include "mplapack_oracle.hpp"
+
+#include <algorithm>
+
+namespace orc {
+
+int g_dot_threads = 1;
+
+// mplapack/Rgemm_omp.cpp:41-127, Rgemm_NN_omp.cpp:36-65, Rgemm_NT_omp.cpp:31-57.
+// Only the NN and NT forms are reached on the path.
+void Rgemm(char ta, char tb, mint m, mint n, mint k, dd_real alpha, const dd_real *A, mint lda, const dd_real *B,
+           mint ldb, dd_real beta, dd_real *C, mint ldc) {
+    const dd_real zero = 0.0, one = 1.0;
+    if (m == 0 || n == 0 || (((alpha == zero) || (k == 0)) && (beta == one))) return;
+    if (alpha == zero) {
+        for (mint j = 0; j < n; j++)
+            for (mint i = 0; i < m; i++) C[i + j * ldc] = (beta == zero) ? zero : beta * C[i + j * ldc];
+        return;
+    }
+    if (ta != 'N' && ta != 'n') std::abort();  // transposed A is never reached on this path
+    bool notb = (tb == 'N' || tb == 'n');
+    for (mint j = 0; j < n; j++) {
+        if (beta == 0.0) {
+            for (mint i = 0; i < m; i++) C[i + j * ldc] = 0.0;
+        } else if (beta != 1.0) {
+            for (mint i = 0; i < m; i++) C[i + j * ldc] = beta * C[i + j * ldc];
+        }
+    }
+    for (mint j = 0; j < n; j++) {
+        for (mint l = 0; l < k; l++) {
+            dd_real temp = alpha * (notb ? B[l + j * ldb] : B[j + l * ldb]);
+            for (mint i = 0; i < m; i++) C[i + j * ldc] += temp * A[i + l * lda];
+        }
+    }
+}
+
+// mplapack/Rsyrk.cpp:158-176 (Lower, No transpose).
+void Rsyrk_LN(mint n, mint k, dd_real alpha, const dd_real *A, mint lda, dd_real beta, dd_real *C, mint ldc) {
+    const dd_real zero = 0.0, one = 1.0;
+    if (n == 0 || (((alpha == zero) || (k == 0)) && (beta == one))) return;
+    if (alpha == zero) {
+        for (mint j = 0; j < n; j++)
+            for (mint i = j; i < n; i++) C[i + j * ldc] = (beta == zero) ? zero : beta * C[i + j * ldc];
+        return;
+    }
+    for (mint j = 0; j < n; j++) {
+        if (beta == zero) {
+            for (mint i = j; i < n; i++) C[i + j * ldc] = zero;
+        } else if (beta != one) {
+            for (mint i = j; i < n; i++) C[i + j * ldc] = beta * C[i + j * ldc];
+        }
+        for (mint l = 0; l < k; l++) {
+            if (A[j + l * lda] != zero) {
+                dd_real temp = alpha * A[j + l * lda];
+                for (mint i = j; i < n; i++) C[i + j * ldc] += temp * A[i + l * lda];
+            }
+        }
+    }
+}
+
+// mplapack/Rtrsm.cpp:256-283 (Right, Lower, Transpose, Non-unit):
+// B := alpha * B * inv(A**T).
+void Rtrsm_RLTN(mint m, mint n, dd_real alpha, const dd_real *A, mint lda, dd_real *B, mint ldb) {
+    const dd_real zero = 0.0, one = 1.0;
+    if (m == 0 || n == 0) return;
+    if (alpha == zero) {
+        for (mint j = 0; j < n; j++)
+            for (mint i = 0; i < m; i++) B[i + j * ldb] = zero;
+        return;
+    }
+    for (mint k = 0; k < n; k++) {
+        dd_real temp = one / A[k + k * lda];
+        for (mint i = 0; i < m; i++) B[i + k * ldb] = temp * B[i + k * ldb];
+        for (mint j = k + 1; j < n; j++) {
+            if (A[j + k * lda] != zero) {
+                temp = A[j + k * lda];
+                for (mint i = 0; i < m; i++) B[i + j * ldb] = B[i + j * ldb] - temp * B[i + k * ldb];
+            }
+        }
+        if (alpha != one)
+            for (mint i = 0; i < m; i++) B[i + k * ldb] = alpha * B[i + k * ldb];
+    }
+}
+
+// mplapack/Rtrsm.cpp:128-142 (Left, Lower, No transpose, Non-unit):
+// B := alpha * inv(A) * B.
+void Rtrsm_LLNN(mint m, mint n, dd_real alpha, const dd_real *A, mint lda, dd_real *B, mint ldb) {
+    const dd_real zero = 0.0, one = 1.0;
+    if (m == 0 || n == 0) return;
+    if (alpha == zero) {
+        for (mint j = 0; j < n; j++)
+            for (mint i = 0; i < m; i++) B[i + j * ldb] = zero;
+        return;
+    }
+    for (mint j = 0; j < n; j++) {
+        if (alpha != one)
+            for (mint i = 0; i < m; i++) B[i + j * ldb] = alpha * B[i + j * ldb];
+        for (mint k = 0; k < m; k++) {
+            if (B[k + j * ldb] != zero) {
+                B[k + j * ldb] = B[k + j * ldb] / A[k + k * lda];
+                for (mint i = k + 1; i < m; i++) B[i + j * ldb] = B[i + j * ldb] - B[k + j * ldb] * A[i + k * lda];
+            }
+        }
+    }
+}
+
+// mplapack/Rtrmm.cpp:157-169 (Left, Lower, Transpose, Non-unit):
+// B := alpha * A**T * B.
+void Rtrmm_LLTN(mint m, mint n, dd_real alpha, const dd_real *A, mint lda, dd_real *B, mint ldb) {
+    const dd_real zero = 0.0;
+    if (m == 0 || n == 0) return;
+    if (alpha == zero) {
+        for (mint j = 0; j < n; j++)
+            for (mint i = 0; i < m; i++) B[i + j * ldb] = zero;
+        return;
+    }
+    for (mint j = 0; j < n; j++) {
+        for (mint i = 0; i < m; i++) {
+            dd_real temp = B[i + j * ldb];
+            temp = temp * A[i + i * lda];
+            for (mint k = i + 1; k < m; k++) temp += A[k + i * lda] * B[k + j * ldb];
+            B[i + j * ldb] = alpha * temp;
+        }
+    }
+}
+
+// mplapack/Rtrsv.cpp:134-145 (Lower, No transpose, Non-unit, incx=1).
+void Rtrsv_LN(mint n, const dd_real *A, mint lda, dd_real *x) {
+    const dd_real zero = 0.0;
+    for (mint j = 0; j < n; j++) {
+        if (x[j] != zero) {
+            x[j] = x[j] / A[j + j * lda];
+            dd_real temp = x[j];
+            for (mint i = j + 1; i < n; i++) x[i] = x[i] - temp * A[i + j * lda];
+        }
+    }
+}
+
+// mplapack/Rtrsv.cpp:200-211 (Lower, Transpose, Non-unit, incx=1).
+void Rtrsv_LT(mint n, const dd_real *A, mint lda, dd_real *x) {
+    for (mint j = n - 1; j >= 0; j--) {
+        dd_real temp = x[j];
+        for (mint i = n - 1; i >= j + 1; i--) temp = temp - A[i + j * lda] * x[i];
+        temp = temp / A[j + j * lda];
+        x[j] = temp;
+    }
+}
+
+// mplapack/Rtrmv.cpp:134-145 (Lower, No transpose, Non-unit, incx=1).
+void Rtrmv_LN(mint n, const dd_real *A, mint lda, dd_real *x) {
+    const dd_real zero = 0.0;
+    for (mint j = n - 1; j >= 0; j--) {
+        if (x[j] != zero) {
+            dd_real temp = x[j];
+            for (mint i = n - 1; i >= j + 1; i--) x[i] += temp * A[i + j * lda];
+            x[j] = x[j] * A[j + j * lda];
+        }
+    }
+}
+
+// mplapack/Rtrmv.cpp:203-213 (Lower, Transpose, Non-unit, incx=1).
+void Rtrmv_LT(mint n, const dd_real *A, mint lda, dd_real *x) {
+    for (mint j = 0; j < n; j++) {
+        dd_real temp = x[j];
+        temp = temp * A[j + j * lda];
+        for (mint i = j + 1; i < n; i++) temp += A[i + j * lda] * x[i];
+        x[j] = temp;
+    }
+}
+
+// mplapack/Rgemv.cpp:118-171 (No transpose, incx=incy=1).
+void Rgemv_N(mint m, mint n, dd_real alpha, const dd_real *A, mint lda, const dd_real *x, dd_real beta, dd_real *y) {
+    const dd_real zero = 0.0, one = 1.0;
+    if (m == 0 || n == 0 || ((alpha == zero) && (beta == one))) return;
+    if (beta != one) {
+        for (mint i = 0; i < m; i++) y[i] = (beta == zero) ? zero : beta * y[i];
+    }
+    if (alpha == zero) return;
+    for (mint j = 0; j < n; j++) {
+        dd_real temp = alpha * x[j];
+        for (mint i = 0; i < m; i++) y[i] += temp * A[i + j * lda];
+    }
+}
+
+// mplapack/Rdot_omp.cpp:80-120.  Unit strides: an OpenMP static partition
+// of [0,n) over g_dot_threads chunks (q = n/T, the first n%T chunks one
+// longer), each summed in order from zero, the partial sums then added in
+// chunk order.  Other strides: one sequential sum.
+dd_real Rdot(mint n, const dd_real *x, mint incx, const dd_real *y, mint incy) {
+    dd_real temp = 0.0;
+    if (incx == 1 && incy == 1) {
+        mint T = g_dot_threads;
+        mint q = n / T, r = n % T;
+        for (mint t = 0; t < T; t++) {
+            mint cnt = q + (t < r ? 1 : 0);
+            mint start = t * q + (t < r ? t : r);
+            dd_real templ = 0.0;
+            for (mint i = start; i < start + cnt; i++) templ += x[i] * y[i];
+            temp += templ;
+        }
+        return temp;
+    }
+    mint ix = 0, iy = 0;
+    if (incx < 0) ix = (-n + 1) * incx;
+    if (incy < 0) iy = (-n + 1) * incy;
+    for (mint i = 0; i < n; i++) {
+        temp += x[ix] * y[iy];
+        ix += incx;
+        iy += incy;
+    }
+    return temp;
+}
+
+// mplapack/Raxpy.cpp:38-53 + Raxpy_omp.cpp:79-112.
+void Raxpy(mint n, dd_real a, const dd_real *x, mint incx, dd_real *y, mint incy) {
+    const dd_real zero = 0.0;
+    if (n <= 0) return;
+    if (a == zero) return;
+    mint ix = 0, iy = 0;
+    if (incx < 0) ix = (-n + 1) * incx;
+    if (incy < 0) iy = (-n + 1) * incy;
+    for (mint i = 0; i < n; i++) {
+        y[iy] += a * x[ix];
+        ix += incx;
+        iy += incy;
+    }
+}
+
+// mplapack/Rscal.cpp: dx = da * dx elementwise.
+void Rscal(mint n, dd_real a, dd_real *x, mint incx) {
+    if (n <= 0 || incx <= 0) return;
+    for (mint i = 0; i < n; i++) x[i * incx] = a * x[i * incx];
+}
+
+// mplapack/Rcopy_omp.cpp.
+void Rcopy(mint n, const dd_real *x, mint incx, dd_real *y, mint incy) {
+    mint ix = 0, iy = 0;
+    if (incx < 0) ix = (-n + 1) * incx;
+    if (incy < 0) iy = (-n + 1) * incy;
+    for (mint i = 0; i < n; i++) {
+        y[iy] = x[ix];
+        ix += incx;
+        iy += incy;
+    }
+}
+
+// mplapack/Rswap.cpp.
+void Rswap(mint n, dd_real *x, mint incx, dd_real *y, mint incy) {
+    mint ix = 0, iy = 0;
+    if (incx < 0) ix = (-n + 1) * incx;
+    if (incy < 0) iy = (-n + 1) * incy;
+    for (mint i = 0; i < n; i++) {
+        dd_real t = x[ix];
+        x[ix] = y[iy];
+        y[iy] = t;
+        ix += incx;
+        iy += incy;
+    }
+}
+
+// mplapack/Rpotrf2.cpp:94-146 (Lower): recursive halving, sqrt pivots.
+void Rpotrf2_L(mint n, dd_real *a, mint lda, mint &info) {
+    info = 0;
+    if (n == 0) return;
+    const dd_real zero = 0.0, one = 1.0;
+    if (n == 1) {
+        if (a[0] <= zero || isnan_dd(a[0])) {
+            info = 1;
+            return;
+        }
+        a[0] = sqrt(a[0]);
+        return;
+    }
+    mint n1 = n / 2, n2 = n - n1, iinfo = 0;
+    Rpotrf2_L(n1, a, lda, iinfo);
+    if (iinfo != 0) {
+        info = iinfo;
+        return;
+    }
+    Rtrsm_RLTN(n2, n1, one, a, lda, &a[n1], lda);
+    Rsyrk_LN(n2, n1, -one, &a[n1], lda, one, &a[n1 + n1 * lda], lda);
+    Rpotrf2_L(n2, &a[n1 + n1 * lda], lda, iinfo);
+    if (iinfo != 0) info = iinfo + n1;
+}
+
+// mplapack/Rpotrf.cpp:75-151 (Lower), block size 64 (iMlaenv.cpp:222-229).
+void Rpotrf_L(mint n, dd_real *a, mint lda, mint &info) {
+    info = 0;
+    if (n == 0) return;
+    const mint nb = 64;
+    const dd_real one = 1.0;
+    if (nb <= 1 || nb >= n) {
+        Rpotrf2_L(n, a, lda, info);
+        return;
+    }
+    for (mint j = 1; j <= n; j += nb) {
+        mint jb = std::min(nb, n - j + 1);
+        Rsyrk_LN(jb, j - 1, -one, &a[j - 1], lda, one, &a[(j - 1) + (j - 1) * lda], lda);
+        Rpotrf2_L(jb, &a[(j - 1) + (j - 1) * lda], lda, info);
+        if (info != 0) {
+            info += j - 1;
+            return;
+        }
+        if (j + jb <= n) {
+            Rgemm('N', 'T', n - j - jb + 1, jb, j - 1, -one, &a[(j + jb) - 1], lda, &a[j - 1], lda, one,
+                  &a[((j + jb) - 1) + (j - 1) * lda], lda);
+            Rtrsm_RLTN(n - j - jb + 1, jb, one, &a[(j - 1) + (j - 1) * lda], lda, &a[((j + jb) - 1) + (j - 1) * lda], lda);
+        }
+    }
+}
+
+// ---- tridiagonal eigen helpers (mplapack/Rlapy2.cpp, Rlartg.cpp, Rlae2.cpp,
+// Rlaev2.cpp, Rlasr.cpp, Rlanst.cpp, Rlascl.cpp, Rlaset.cpp) ----------------
+static inline dd_real pow2(const dd_real &a) { return a * a; }  // mplapack_utils_dd.h: pow2
+static inline dd_real sign(const dd_real &a, const dd_real &b) {
+    dd_real t = abs(a);
+    if (b < 0.0) t = -t;
+    return t;
+}
+static inline dd_real lamchE() { return dd_real(dd_eps); }
+static inline dd_real lamchS() { return dd_real(dd_min_normalized); }
+
+dd_real Rlapy2(dd_real x, dd_real y) {
+    bool xn = x != x, yn = y != y;
+    dd_real rv = 0.0;
+    if (xn) rv = x;
+    if (yn) rv = y;
+    if (!(xn || yn)) {
+        dd_real xa = abs(x), ya = abs(y);
+        dd_real w = max(xa, ya), z = min(xa, ya);
+        if (z == dd_real(0.0))
+            rv = w;
+        else
+            rv = w * sqrt(dd_real(1.0) + pow2(z / w));
+    }
+    return rv;
+}
+
+// Rlartg computes its scaling threshold as
+// 2**castINTEGER(log(safmin/eps)/log(2)/2) with safmin = 2^-969, eps = 2^-104:
+// the dd quotient is -432.5 to ~30 digits, its high word is exactly -432.5 and
+// the cast truncates to -432, so the threshold is 2^-432 exactly.
+void Rlartg(dd_real f, dd_real g, dd_real &cs, dd_real &sn, dd_real &r) {
+    const dd_real zero = 0.0, one = 1.0;
+    const dd_real safmn2 = npwr(dd_real(2.0), -432);
+    const dd_real safmx2 = one / safmn2;
+    if (g == zero) {
+        cs = one;
+        sn = zero;
+        r = f;
+        return;
+    }
+    if (f == zero) {
+        cs = zero;
+        sn = one;
+        r = g;
+        return;
+    }
+    dd_real f1 = f, g1 = g;
+    dd_real scale = max(abs(f1), abs(g1));
+    if (scale >= safmx2) {
+        mint count = 0;
+        do {
+            count++;
+            f1 = f1 * safmn2;
+            g1 = g1 * safmn2;
+            scale = max(abs(f1), abs(g1));
+        } while (scale >= safmx2 && count < 20);
+        r = sqrt(pow2(f1) + pow2(g1));
+        cs = f1 / r;
+        sn = g1 / r;
+        for (mint i = 1; i <= count; i++) r = r * safmx2;
+    } else if (scale <= safmn2) {
+        mint count = 0;
+        do {
+            count++;
+            f1 = f1 * safmx2;
+            g1 = g1 * safmx2;
+            scale = max(abs(f1), abs(g1));
+        } while (scale <= safmn2);
+        r = sqrt(pow2(f1) + pow2(g1));
+        cs = f1 / r;
+        sn = g1 / r;
+        for (mint i = 1; i <= count; i++) r = r * safmn2;
+    } else {
+        r = sqrt(pow2(f1) + pow2(g1));
+        cs = f1 / r;
+        sn = g1 / r;
+    }
+    if (abs(f) > abs(g) && cs < zero) {
+        cs = -cs;
+        sn = -sn;
+        r = -r;
+    }
+}
+
+static void lae2_core(const dd_real &a, const dd_real &b, const dd_real &c, dd_real &rt1, dd_real &rt2, dd_real &df,
+                      dd_real &rt, dd_real &tb, dd_real &ab, int &sgn1) {
+    const dd_real one = 1.0, two = 2.0, zero = 0.0, half = 0.5;
+    dd_real sm = a + c;
+    df = a - c;
+    dd_real adf = abs(df);
+    tb = b + b;
+    ab = abs(tb);
+    dd_real acmx, acmn;
+    if (abs(a) > abs(c)) {
+        acmx = a;
+        acmn = c;
+    } else {
+        acmx = c;
+        acmn = a;
+    }
+    if (adf > ab)
+        rt = adf * sqrt(one + pow2(ab / adf));
+    else if (adf < ab)
+        rt = ab * sqrt(one + pow2(adf / ab));
+    else
+        rt = ab * sqrt(two);
+    if (sm < zero) {
+        rt1 = half * (sm - rt);
+        sgn1 = -1;
+        rt2 = (acmx / rt1) * acmn - (b / rt1) * b;
+    } else if (sm > zero) {
+        rt1 = half * (sm + rt);
+        sgn1 = 1;
+        rt2 = (acmx / rt1) * acmn - (b / rt1) * b;
+    } else {
+        rt1 = half * rt;
+        rt2 = -half * rt;
+        sgn1 = 1;
+    }
+}
+
+void Rlae2(dd_real a, dd_real b, dd_real c, dd_real &rt1, dd_real &rt2) {
+    dd_real df, rt, tb, ab;
+    int sgn1;
+    lae2_core(a, b, c, rt1, rt2, df, rt, tb, ab, sgn1);
+}
+
+void Rlaev2(dd_real a, dd_real b, dd_real c, dd_real &rt1, dd_real &rt2, dd_real &cs1, dd_real &sn1) {
+    const dd_real one = 1.0, zero = 0.0;
+    dd_real df, rt, tb, ab;
+    int sgn1;
+    lae2_core(a, b, c, rt1, rt2, df, rt, tb, ab, sgn1);
+    int sgn2;
+    dd_real cs;
+    if (df >= zero) {
+        cs = df + rt;
+        sgn2 = 1;
+    } else {
+        cs = df - rt;
+        sgn2 = -1;
+    }
+    dd_real acs = abs(cs);
+    if (acs > ab) {
+        dd_real ct = -tb / cs;
+        sn1 = one / sqrt(one + ct * ct);
+        cs1 = ct * sn1;
+    } else {
+        if (ab == zero) {
+            cs1 = one;
+            sn1 = zero;
+        } else {
+            dd_real tn = -cs / tb;
+            cs1 = one / sqrt(one + tn * tn);
+            sn1 = tn * cs1;
+        }
+    }
+    if (sgn1 == sgn2) {
+        dd_real tn = cs1;
+        cs1 = -sn1;
+        sn1 = tn;
+    }
+}
+
+// Rlasr with side='R', pivot='V': plane rotations applied to columns j, j+1
+// of the m x n matrix a, forward (F) or backward (B) over j.
+static void Rlasr_RV(bool forward, mint m, mint n, const dd_real *c, const dd_real *s, dd_real *a, mint lda) {
+    const dd_real one = 1.0, zero = 0.0;
+    if (m == 0 || n == 0) return;
+    for (mint jj = 0; jj < n - 1; jj++) {
+        mint j = forward ? jj : (n - 2 - jj);
+        dd_real ct = c[j], st = s[j];
+        if ((ct != one) || (st != zero)) {
+            for (mint i = 0; i < m; i++) {
+                dd_real temp = a[i + (j + 1) * lda];
+                a[i + (j + 1) * lda] = ct * temp - st * a[i + j * lda];
+                a[i + j * lda] = st * temp + ct * a[i + j * lda];
+            }
+        }
+    }
+}
+
+static dd_real Rlanst_M(mint n, const dd_real *d, const dd_real *e) {
+    if (n <= 0) return dd_real(0.0);
+    dd_real anorm = abs(d[n - 1]);
+    for (mint i = 0; i < n - 1; i++) {
+        dd_real sum = abs(d[i]);
+        if (anorm < sum || isnan_dd(sum)) anorm = sum;
+        sum = abs(e[i]);
+        if (anorm < sum || isnan_dd(sum)) anorm = sum;
+    }
+    return anorm;
+}
+
+// Rlascl type 'G' on an m x 1 column.
+static void Rlascl_G(dd_real cfrom, dd_real cto, mint m, dd_real *a) {
+    const dd_real zero = 0.0, one = 1.0;
+    if (m == 0) return;
+    dd_real smlnum = lamchS();
+    dd_real bignum = one / smlnum;
+    dd_real cfromc = cfrom, ctoc = cto, mul;
+    bool done = false;
+    while (!done) {
+        dd_real cfrom1 = cfromc * smlnum;
+        if (cfrom1 == cfromc) {
+            mul = ctoc / cfromc;
+            done = true;
+        } else {
+            dd_real cto1 = ctoc / bignum;
+            if (cto1 == ctoc) {
+                mul = ctoc;
+                done = true;
+                cfromc = one;
+            } else if (abs(cfrom1) > abs(ctoc) && ctoc != zero) {
+                mul = smlnum;
+                done = false;
+                cfromc = cfrom1;
+            } else if (abs(cto1) > abs(cfromc)) {
+                mul = bignum;
+                done = false;
+                ctoc = cto1;
+            } else {
+                mul = ctoc / cfromc;
+                done = true;
+            }
+        }
+        for (mint i = 0; i < m; i++) a[i] = a[i] * mul;
+    }
+}
+
+// mplapack/Rsteqr.cpp:32-503, compz = 'I' or 'N'.  Indices below are the
+// reference's 1-based ones; d(i) is d[i-1].
+void Rsteqr(char compz, mint n, dd_real *d, dd_real *e, dd_real *z, mint ldz, dd_real *work, mint &info) {
+    const dd_real one = 1.0, zero = 0.0, two = 2.0, three = 3.0;
+    const mint maxit = 30;
+    int icompz = (compz == 'I' || compz == 'i') ? 2 : 0;
+    info = 0;
+    if (n == 0) return;
+    if (n == 1) {
+        if (icompz == 2) z[0] = one;
+        return;
+    }
+#define D(i) d[(i) - 1]
+#define E(i) e[(i) - 1]
+    dd_real eps = lamchE();
+    dd_real eps2 = pow2(eps);
+    dd_real safmin = lamchS();
+    dd_real safmax = one / safmin;
+    dd_real ssfmax = sqrt(safmax) / three;
+    dd_real ssfmin = sqrt(safmin) / eps2;
+    if (icompz == 2) {
+        for (mint j = 0; j < n; j++)
+            for (mint i = 0; i < n; i++) z[i + j * ldz] = zero;
+        for (mint i = 0; i < n; i++) z[i + i * ldz] = one;
+    }
+    mint nmaxit = n * maxit, jtot = 0, l1 = 1, nm1 = n - 1;
+    mint m = 0, l = 0, lsv = 0, lend = 0, lendsv = 0, iscale = 0;
+    dd_real anorm, p, rt1, rt2, c, s, g, r, f, b, tst;
+    for (;;) {  // statement 10
+        if (l1 > n) goto sorting;
+        if (l1 > 1) E(l1 - 1) = zero;
+        m = n;
+        if (l1 <= nm1) {
+            for (mint mm = l1; mm <= nm1; mm++) {
+                tst = abs(E(mm));
+                if (tst == zero) {
+                    m = mm;
+                    break;
+                }
+                if (tst <= (sqrt(abs(D(mm))) * sqrt(abs(D(mm + 1)))) * eps) {
+                    E(mm) = zero;
+                    m = mm;
+                    break;
+                }
+            }
+        }
+        l = l1;
+        lsv = l;
+        lend = m;
+        lendsv = lend;
+        l1 = m + 1;
+        if (lend == l) continue;
+        anorm = Rlanst_M(lend - l + 1, &D(l), &E(l));
+        iscale = 0;
+        if (anorm == zero) continue;
+        if (anorm > ssfmax) {
+            iscale = 1;
+            Rlascl_G(anorm, ssfmax, lend - l + 1, &D(l));
+            Rlascl_G(anorm, ssfmax, lend - l, &E(l));
+        } else if (anorm < ssfmin) {
+            iscale = 2;
+            Rlascl_G(anorm, ssfmin, lend - l + 1, &D(l));
+            Rlascl_G(anorm, ssfmin, lend - l, &E(l));
+        }
+        if (abs(D(lend)) < abs(D(l))) {
+            lend = lsv;
+            l = lendsv;
+        }
+        if (lend > l) {
+            // QL iteration: look for a small subdiagonal element (statement 40).
+            for (;;) {
+                m = lend;
+                if (l != lend) {
+                    for (mint mm = l; mm <= lend - 1; mm++) {
+                        tst = pow2(abs(E(mm)));
+                        if (tst <= (eps2 * abs(D(mm))) * abs(D(mm + 1)) + safmin) {
+                            m = mm;
+                            break;
+                        }
+                    }
+                }
+                if (m < lend) E(m) = zero;
+                p = D(l);
+                if (m == l) {  // statement 80
+                    D(l) = p;
+                    l++;
+                    if (l <= lend) continue;
+                    break;
+                }
+                if (m == l + 1) {
+                    if (icompz > 0) {
+                        Rlaev2(D(l), E(l), D(l + 1), rt1, rt2, c, s);
+                        work[l - 1] = c;
+                        work[n - 1 + l - 1] = s;
+                        Rlasr_RV(false, n, 2, &work[l - 1], &work[n - 1 + l - 1], &z[(l - 1) * ldz], ldz);
+                    } else {
+                        Rlae2(D(l), E(l), D(l + 1), rt1, rt2);
+                    }
+                    D(l) = rt1;
+                    D(l + 1) = rt2;
+                    E(l) = zero;
+                    l += 2;
+                    if (l <= lend) continue;
+                    break;
+                }
+                if (jtot == nmaxit) break;
+                jtot++;
+                g = (D(l + 1) - p) / (two * E(l));
+                r = Rlapy2(g, one);
+                g = D(m) - p + (E(l) / (g + sign(r, g)));
+                s = one;
+                c = one;
+                p = zero;
+                for (mint i = m - 1; i >= l; i--) {
+                    f = s * E(i);
+                    b = c * E(i);
+                    Rlartg(g, f, c, s, r);
+                    if (i != m - 1) E(i + 1) = r;
+                    g = D(i + 1) - p;
+                    r = (D(i) - g) * s + two * c * b;
+                    p = s * r;
+                    D(i + 1) = g + p;
+                    g = c * r - b;
+                    if (icompz > 0) {
+                        work[i - 1] = c;
+                        work[n - 1 + i - 1] = -s;
+                    }
+                }
+                if (icompz > 0) {
+                    mint mm = m - l + 1;
+                    Rlasr_RV(false, n, mm, &work[l - 1], &work[n - 1 + l - 1], &z[(l - 1) * ldz], ldz);
+                }
+                D(l) = D(l) - p;
+                E(l) = g;
+            }
+        } else {
+            // QR iteration (statement 90).
+            for (;;) {
+                m = lend;
+                if (l != lend) {
+                    for (mint mm = l; mm >= lend + 1; mm--) {
+                        tst = pow2(abs(E(mm - 1)));
+                        if (tst <= (eps2 * abs(D(mm))) * abs(D(mm - 1)) + safmin) {
+                            m = mm;
+                            break;
+                        }
+                    }
+                }
+                if (m > lend) E(m - 1) = zero;
+                p = D(l);
+                if (m == l) {  // statement 130
+                    D(l) = p;
+                    l--;
+                    if (l >= lend) continue;
+                    break;
+                }
+                if (m == l - 1) {
+                    if (icompz > 0) {
+                        Rlaev2(D(l - 1), E(l - 1), D(l), rt1, rt2, c, s);
+                        work[m - 1] = c;
+                        work[n - 1 + m - 1] = s;
+                        Rlasr_RV(true, n, 2, &work[m - 1], &work[n - 1 + m - 1], &z[(l - 2) * ldz], ldz);
+                    } else {
+                        Rlae2(D(l - 1), E(l - 1), D(l), rt1, rt2);
+                    }
+                    D(l - 1) = rt1;
+                    D(l) = rt2;
+                    E(l - 1) = zero;
+                    l -= 2;
+                    if (l >= lend) continue;
+                    break;
+                }
+                if (jtot == nmaxit) break;
+                jtot++;
+                g = (D(l - 1) - p) / (two * E(l - 1));
+                r = Rlapy2(g, one);
+                g = D(m) - p + (E(l - 1) / (g + sign(r, g)));
+                s = one;
+                c = one;
+                p = zero;
+                mint lm1 = l - 1;
+                for (mint i = m; i <= lm1; i++) {
+                    f = s * E(i);
+                    b = c * E(i);
+                    Rlartg(g, f, c, s, r);
+                    if (i != m) E(i - 1) = r;
+                    g = D(i) - p;
+                    r = (D(i + 1) - g) * s + two * c * b;
+                    p = s * r;
+                    D(i) = g + p;
+                    g = c * r - b;
+                    if (icompz > 0) {
+                        work[i - 1] = c;
+                        work[n - 1 + i - 1] = s;
+                    }
+                }
+                if (icompz > 0) {
+                    mint mm = l - m + 1;
+                    Rlasr_RV(true, n, mm, &work[m - 1], &work[n - 1 + m - 1], &z[(m - 1) * ldz], ldz);
+                }
+                D(l) = D(l) - p;
+                E(lm1) = g;
+            }
+        }
+        // statement 140: undo scaling.
+        if (iscale == 1) {
+            Rlascl_G(ssfmax, anorm, lendsv - lsv + 1, &D(lsv));
+            Rlascl_G(ssfmax, anorm, lendsv - lsv, &E(lsv));
+        } else if (iscale == 2) {
+            Rlascl_G(ssfmin, anorm, lendsv - lsv + 1, &D(lsv));
+            Rlascl_G(ssfmin, anorm, lendsv - lsv, &E(lsv));
+        }
+        if (jtot < nmaxit) continue;
+        for (mint i = 1; i <= n - 1; i++)
+            if (E(i) != zero) info++;
+        return;
+    }
+sorting:
+    // statement 160: selection sort of eigenvalues, swapping vectors.
+    if (icompz == 0) {
+        std::abort();  // not reached: the path always asks for vectors
+    }
+    for (mint ii = 2; ii <= n; ii++) {
+        mint i = ii - 1, k = i;
+        p = D(i);
+        for (mint j = ii; j <= n; j++) {
+            if (D(j) < p) {
+                k = j;
+                p = D(j);
+            }
+        }
+        if (k != i) {
+            D(k) = D(i);
+            D(i) = p;
+            Rswap(n, &z[(i - 1) * ldz], 1, &z[(k - 1) * ldz], 1);
+        }
+    }
+#undef D
+#undef E
+}
+
+}  // namespace orc

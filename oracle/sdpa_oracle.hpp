@@ -1,0 +1,102 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of SDPA-DD 7.1.3's dense-Schur SDP path: the .dat-s reader,
+// the parameter file, and the Mehrotra predictor-corrector loop with its HKM
+// Newton step, Schur assembly (F1/F2/F3), Cholesky solve, Lanczos step length
+// and phase logic.  Sparse-Schur ordering, LP/SOCP blocks and blocks of size
+// <= 20 (which use an eigenvalue routine instead of Lanczos) are not restated;
+// inputs needing them are rejected.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "mplapack_oracle.hpp"
+
+namespace orc {
+
+// sdpa_struct.h:71-108
+struct SparseMatrix {
+    int nRow = 0, nCol = 0;
+    enum Type { SPARSE, DENSE } type = SPARSE;
+    int NonZeroNumber = 0, NonZeroCount = 0, NonZeroEffect = 0;
+    std::vector<dd_real> de_ele;
+    std::vector<int> row_index, column_index;
+    std::vector<dd_real> sp_ele;
+    void changeToDense();
+};
+struct DenseMatrix {
+    int nRow = 0, nCol = 0;
+    std::vector<dd_real> de_ele;
+    void initialize(int n) {
+        nRow = nCol = n;
+        de_ele.assign((size_t)n * n, dd_real(0.0));
+    }
+    void setIdentity(dd_real s) {
+        for (auto &v : de_ele) v = 0.0;
+        for (int i = 0; i < nRow; i++) de_ele[(size_t)i * nRow + i] = s;
+    }
+};
+struct SparseLinearSpace {
+    std::vector<int> SDP_sp_index;
+    std::vector<SparseMatrix> SDP_sp_block;
+};
+struct DenseLinearSpace {
+    std::vector<DenseMatrix> SDP_block;
+    void initialize(const std::vector<int> &bs) {
+        SDP_block.resize(bs.size());
+        for (size_t l = 0; l < bs.size(); l++) SDP_block[l].initialize(bs[l]);
+    }
+    void setZero() {
+        for (auto &b : SDP_block)
+            for (auto &v : b.de_ele) v = 0.0;
+    }
+};
+
+struct Parameter {
+    int maxIteration = 200;
+    double epsilonStar = 1.0e-30, lambdaStar = 1.0e+4, omegaStar = 2.0, lowerBound = -1.0e+5, upperBound = 1.0e+5,
+           betaStar = 0.1, betaBar = 0.3, gammaStar = 0.9, epsilonDash = 1.0e-30;
+    bool readFile(const std::string &path);
+};
+
+struct InputData {
+    int m = 0;
+    std::vector<int> blockStruct;  // SDP blocks only
+    std::vector<dd_real> b;
+    SparseLinearSpace C;
+    std::vector<SparseLinearSpace> A;
+    std::vector<std::vector<int>> SDP_constraint, SDP_blockIndex;
+    void initialize_index();
+};
+
+// Reads an SDPA sparse-format (.dat-s) file (sdpa_io.cpp:124-150, 429-786).
+// Returns false with a message in err on unsupported or malformed input.
+bool read_dat_s(const std::string &path, InputData &data, std::string &err);
+
+enum PhaseType { noINFO, pFEAS, dFEAS, pdFEAS, pdINF, pFEAS_dINF, pINF_dFEAS, pdOPT, pUNBD, dUNBD };
+
+struct IterRecord {
+    int it;
+    dd_real mu, thetaP, thetaD, objP, objD, alphaP, alphaD, beta;
+};
+
+struct SolveResult {
+    int iterations = 0;
+    int phase = noINFO;  // internal phase before the primal/dual swap
+    int phase_reported = noINFO;  // after Phase::reverse (what the result file prints)
+    dd_real mu, objValPrimal, objValDual, normPrimalVec, normDualMat, relgap;
+    std::vector<dd_real> xMat, zMat, yVec;  // concatenated blocks, column-major
+    std::vector<IterRecord> trace;
+    std::string message;
+};
+
+// Runs the solver; maxIter < 0 keeps the parameter's value.
+bool solve(const InputData &data, const Parameter &param, SolveResult &res);
+
+// Pieces exposed to tests.
+void schur_dense(const InputData &data, const DenseLinearSpace &X, const DenseLinearSpace &invZ,
+                 std::vector<dd_real> &B, const std::vector<int> &useFormula);
+std::vector<int> compute_formula(const InputData &data, dd_real kappa);
+dd_real lanczos_min_eigen(const DenseMatrix &L, const DenseMatrix &M);
+
+}  // namespace orc
