@@ -1,0 +1,161 @@
+/* sdpadd.h — C ABI of the MI355X double-double Newton-step library
+ * (libsdpadd_hip.so).
+ *
+ * Conventions
+ *   - A dd value is two doubles {hi, lo}: the memory layout of QD's dd_real,
+ *     so a dd_real* from SDPA-DD's host code can be passed as double*.
+ *   - Matrices are column-major with a leading dimension, as in MPLAPACK.
+ *   - Arguments named d_* are device pointers (from sdpadd_malloc).
+ *   - Every function returns int: 0 = success, < 0 = argument or HIP error
+ *     (message from sdpadd_last_error), > 0 = LAPACK-style info where
+ *     documented.  Nothing in the library calls exit().
+ *   - Calls on one handle are stream-ordered and return after completion.
+ *     One host thread per handle.
+ */
+#ifndef SDPADD_H
+#define SDPADD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdpadd_ctx *sdpadd_handle;
+typedef struct sdpadd_problem_s *sdpadd_problem;
+
+/* ---- context and memory ------------------------------------------------ */
+int sdpadd_create(int device, sdpadd_handle *h);
+int sdpadd_destroy(sdpadd_handle h);
+const char *sdpadd_last_error(sdpadd_handle h);
+/* Number of chunks of the static partition that unit-stride dot products
+ * sum over (mplapack/Rdot_omp.cpp:80-120 with that many OpenMP threads,
+ * partial sums added in thread order).  Default 1024. */
+int sdpadd_set_dot_partition(sdpadd_handle h, int chunks);
+int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr);
+int sdpadd_free(sdpadd_handle h, void *d_ptr);
+int sdpadd_upload(sdpadd_handle h, void *d_dst, const void *src, size_t bytes);
+int sdpadd_download(sdpadd_handle h, void *dst, const void *d_src, size_t bytes);
+
+/* Kernel timing with HIP events on the library's stream, per family
+ * ("gemm", "potrf", "trsm", "trmm", "trsv", "mv", "dot"): launches, summed
+ * kernel milliseconds and algorithmic dd-flops since the last reset.  No
+ * counterpart in the reference (its ComputeTime buckets are host wall
+ * clocks, sdpa_parts.h:46-83). */
+int sdpadd_stats_enable(sdpadd_handle h, int on);
+int sdpadd_stats_reset(sdpadd_handle h);
+int sdpadd_stats_get(sdpadd_handle h, const char *family, int64_t *launches, double *ms, double *ddflops);
+
+/* ---- dd BLAS/LAPACK on device arrays ------------------------------------
+ * Each replaces the MPLAPACK routine of the same name (C++ signatures in
+ * mplapack/mpblas_dd.h:38-56 and mplapack/mplapack_dd.h:38-92) for the
+ * option combinations SDPA-DD's Newton step uses; other combinations return
+ * -1.  alpha and beta point to host dd scalars. */
+
+/* Rgemm: transa 'N'; transb 'N' or 'T' (mplapack/Rgemm_omp.cpp:41-127). */
+int sdpadd_Rgemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n, int64_t k, const double *alpha,
+                 const double *d_A, int64_t lda, const double *d_B, int64_t ldb, const double *beta, double *d_C,
+                 int64_t ldc);
+/* Rsyrk: uplo 'L', trans 'N' (mplapack/Rsyrk.cpp:31-215). */
+int sdpadd_Rsyrk(sdpadd_handle h, char uplo, char trans, int64_t n, int64_t k, const double *alpha, const double *d_A,
+                 int64_t lda, const double *beta, double *d_C, int64_t ldc);
+/* Rtrsm: (side,uplo,transa,diag) = (R,L,T,N) or (L,L,N,N), alpha = 1
+ * (mplapack/Rtrsm.cpp:31-291). */
+int sdpadd_Rtrsm(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n,
+                 const double *alpha, const double *d_A, int64_t lda, double *d_B, int64_t ldb);
+/* Rtrmm: (L,L,T,N), alpha = 1 (mplapack/Rtrmm.cpp:31-275). */
+int sdpadd_Rtrmm(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n,
+                 const double *alpha, const double *d_A, int64_t lda, double *d_B, int64_t ldb);
+/* Rtrsv / Rtrmv: uplo 'L', trans 'N' or 'T', diag 'N', incx 1
+ * (mplapack/Rtrsv.cpp:31-234, Rtrmv.cpp:31-234). */
+int sdpadd_Rtrsv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, const double *d_A, int64_t lda,
+                 double *d_x, int64_t incx);
+int sdpadd_Rtrmv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, const double *d_A, int64_t lda,
+                 double *d_x, int64_t incx);
+/* Rgemv: trans 'N', unit strides (mplapack/Rgemv.cpp:31-204). */
+int sdpadd_Rgemv(sdpadd_handle h, char trans, int64_t m, int64_t n, const double *alpha, const double *d_A,
+                 int64_t lda, const double *d_x, int64_t incx, const double *beta, double *d_y, int64_t incy);
+/* Rdot: unit strides; result to host dd *result (mplapack/Rdot.cpp:38-47). */
+int sdpadd_Rdot(sdpadd_handle h, int64_t n, const double *d_x, int64_t incx, const double *d_y, int64_t incy,
+                double *result);
+/* Rpotrf: uplo 'L'; *info = 0 or the first non-positive leading minor
+ * (mplapack/Rpotrf.cpp:32-154). */
+int sdpadd_Rpotrf(sdpadd_handle h, char uplo, int64_t n, double *d_A, int64_t lda, int64_t *info);
+/* Rsteqr on HOST arrays, compz 'I' (mplapack/Rsteqr.cpp:32-503).
+ * work: 2*n dd. */
+int sdpadd_Rsteqr(sdpadd_handle h, char compz, int64_t n, double *d, double *e, double *z, int64_t ldz, double *work,
+                  int64_t *info);
+
+/* ---- Newton-step pieces --------------------------------------------------- */
+/* Truncated-Lanczos estimate of lambda_min(L M L^T) for one block, as
+ * Lal::getMinEigen computes it (sdpa_linear.cpp:27-122).  d_L, d_M device
+ * n x n (lda = n); result to host dd *out. */
+int sdpadd_lanczos_min_eig(sdpadd_handle h, int64_t n, const double *d_L, const double *d_M, double *out);
+/* Inverse Cholesky pieces of Solutions::computeInverse (sdpa_dataset.cpp:96-118,
+ * sdpa_jordan.cpp:123-170): d_invChol = chol(A)^{-1}; if d_inv != NULL also
+ * d_inv = L^{-T} L^{-1}.  d_work: n*n dd.  Returns > 0 (failing pivot) if A is
+ * not positive definite. */
+int sdpadd_inv_chol(sdpadd_handle h, int64_t n, const double *d_A, double *d_work, double *d_invChol, double *d_inv);
+
+/* ---- Problems ------------------------------------------------------------- */
+/* Reads an SDPA sparse-format file (.dat-s) the way IO::read does
+ * (sdpa_io.cpp:124-150, 429-786): C = -F0, A_k = F_k, b = c. */
+int sdpadd_problem_load(sdpadd_handle h, const char *path, sdpadd_problem *p);
+/* Builds a problem in SDPA's internal form: block sizes (all > 20, SDP),
+ * then entries of matrix k (0 = C, 1..m = A_k) block l (0-based), given as
+ * 0-based (row, col) pairs with dd values in file order
+ * (SparseLinearSpace::setElement_SDP, sdpa_struct.cpp:1217-1249). */
+int sdpadd_problem_create(sdpadd_handle h, int m, int nblock, const int *block_struct, sdpadd_problem *p);
+int sdpadd_problem_set_entries(sdpadd_problem p, int k, int l, int64_t nnz, const int *rows, const int *cols,
+                               const double *vals);
+int sdpadd_problem_set_b(sdpadd_problem p, const double *b);
+/* Dense conversion (>= 20% stored entries), formula table, device image. */
+int sdpadd_problem_finalize(sdpadd_problem p);
+int sdpadd_problem_destroy(sdpadd_problem p);
+/* m, number of blocks, block sizes (up to cap). */
+int sdpadd_problem_dims(sdpadd_problem p, int *m, int *nblock, int *block_struct, int cap);
+/* useFormula table of Newton::computeFormula_SDP (sdpa_newton.cpp:694-819):
+ * out[i*nblock + l] in {0: F1, 1: F2, 2: F3}. */
+int sdpadd_problem_formula(sdpadd_problem p, int *out);
+/* Schur complement of Newton::compute_bMat_dense_SDP (sdpa_newton.cpp:979-1068)
+ * for device X and Zinv (blocks concatenated, each n_l x n_l): d_B (m x m)
+ * is overwritten. */
+int sdpadd_schur_build(sdpadd_problem p, const double *d_X, const double *d_Zinv, double *d_B);
+/* g_i = A_i . W (InputData::multi_InnerProductToA, sdpa_dataset.cpp:477-485). */
+int sdpadd_inner_A(sdpadd_problem p, const double *d_W, double *d_out);
+/* R = sum_i y_i A_i (InputData::multi_plusToA, sdpa_dataset.cpp:488-493). */
+int sdpadd_plus_A(sdpadd_problem p, const double *d_y, double *d_R);
+
+/* ---- Solver --------------------------------------------------------------- */
+/* param.sdpa (sdpa_parts.cpp:136-183). */
+typedef struct {
+    int maxIteration;
+    double epsilonStar, lambdaStar, omegaStar, lowerBound, upperBound, betaStar, betaBar, gammaStar, epsilonDash;
+} sdpadd_params;
+void sdpadd_default_params(sdpadd_params *prm);
+int sdpadd_read_params(const char *path, sdpadd_params *prm);
+
+typedef struct {
+    int iterations;
+    int phase;          /* SolveInfo::phaseType after Phase::reverse (what the result file prints) */
+    int phase_internal; /* before the primal/dual swap */
+    double mu[2], objValPrimal[2], objValDual[2], normPrimalVec[2], normDualMat[2], relgap[2];
+    double seconds;     /* main loop wall time */
+    double t_schur, t_chol, t_lanczos, t_inverse;
+} sdpadd_result;
+
+/* Runs the predictor-corrector loop of sdpa_main.cpp:294-377 on the device.
+ * Internal convention: objValPrimal = C.X, objValDual = b.y. */
+int sdpadd_solve(sdpadd_problem p, const sdpadd_params *prm, sdpadd_result *res);
+/* Per-iteration records of the last solve: 17 doubles each
+ * (it, then dd pairs mu, thetaP, thetaD, objP, objD, alphaP, alphaD, beta).
+ * Returns the number of records (copies at most cap). */
+int sdpadd_trace(sdpadd_problem p, double *out, int cap);
+/* Final iterate of the last solve (host arrays: X and Z nn dd, y m dd). */
+int sdpadd_solution(sdpadd_problem p, double *X, double *y, double *Z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDPADD_H */

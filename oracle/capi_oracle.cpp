@@ -124,7 +124,10 @@ void orc_formula(int *out) {
     for (size_t i = 0; i < f.size(); i++) out[i] = f[i];
 }
 // Schur matrix for the loaded problem with given X and invZ (blocks concatenated).
-void orc_schur(const double *X, const double *invZ, double *B) {
+// forced >= 0 overrides the formula table (F3 only where A_i is sparse).
+void orc_schur_forced(const double *X, const double *invZ, double *B, int forced);
+void orc_schur(const double *X, const double *invZ, double *B) { orc_schur_forced(X, invZ, B, -1); }
+void orc_schur_forced(const double *X, const double *invZ, double *B, int forced) {
     DenseLinearSpace x, z;
     x.initialize(g_data.blockStruct);
     z.initialize(g_data.blockStruct);
@@ -136,7 +139,17 @@ void orc_schur(const double *X, const double *invZ, double *B) {
         off += len;
     }
     std::vector<dd_real> b((size_t)g_data.m * g_data.m, dd_real(0.0));
-    schur_dense(g_data, x, z, b, compute_formula(g_data, dd_real(1.2)));
+    std::vector<int> f = compute_formula(g_data, dd_real(1.2));
+    if (forced >= 0) {
+        const int nB = (int)g_data.blockStruct.size();
+        for (int l = 0; l < nB; l++)
+            for (size_t k = 0; k < g_data.SDP_constraint[l].size(); k++) {
+                int i = g_data.SDP_constraint[l][k];
+                bool dense = g_data.A[i].SDP_sp_block[g_data.SDP_blockIndex[l][k]].type == SparseMatrix::DENSE;
+                f[i * nB + l] = (forced == 2 && dense) ? 1 : forced;
+            }
+    }
+    schur_dense(g_data, x, z, b, f);
     std::memcpy(B, b.data(), b.size() * 16);
 }
 }

@@ -31,6 +31,8 @@ void Rgemm(char ta, char tb, mint m, mint n, mint k, dd_real alpha, const dd_rea
             for (mint i = 0; i < m; i++) C[i + j * ldc] = beta * C[i + j * ldc];
         }
     }
+    // Columns are independent (Rgemm_NN_omp.cpp:54 parallelises over them too).
+#pragma omp parallel for schedule(static)
     for (mint j = 0; j < n; j++) {
         for (mint l = 0; l < k; l++) {
             dd_real temp = alpha * (notb ? B[l + j * ldb] : B[j + l * ldb]);
@@ -221,6 +223,11 @@ void Raxpy(mint n, dd_real a, const dd_real *x, mint incx, dd_real *y, mint incy
     const dd_real zero = 0.0;
     if (n <= 0) return;
     if (a == zero) return;
+    if (incx == 1 && incy == 1) {  // elementwise (Raxpy_omp.cpp:96-101 runs this in parallel)
+#pragma omp parallel for schedule(static) if (n > 100000)
+        for (mint i = 0; i < n; i++) y[i] += a * x[i];
+        return;
+    }
     mint ix = 0, iy = 0;
     if (incx < 0) ix = (-n + 1) * incx;
     if (incy < 0) iy = (-n + 1) * incy;

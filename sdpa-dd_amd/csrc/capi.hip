@@ -1,0 +1,424 @@
+// extern "C" entry points of libsdpadd_hip.so (declared in include/sdpadd.h).
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/sdpadd.h"
+#include "context.hpp"
+#include "solver.hpp"
+#include "steqr.hpp"
+
+using sdk::idx_t;
+
+struct sdpadd_ctx {
+    sdp::Context c;
+};
+struct sdpadd_problem_s {
+    sdpadd_ctx *h = nullptr;
+    sdp::HostProblem hp;
+    sdp::DeviceProblem dp;
+    bool finalized = false;
+    std::vector<sdp::IterRecord> trace;
+    std::vector<dd> X, y, Z;
+};
+
+namespace {
+inline dd D(const double *p) { return dd{p[0], p[1]}; }
+inline dd *DP(double *p) { return reinterpret_cast<dd *>(p); }
+inline const dd *DP(const double *p) { return reinterpret_cast<const dd *>(p); }
+inline bool up(char c, char u) { return c == u || c == (char)(u + 32); }
+int check(sdpadd_handle h) {
+    hipError_t e = hipStreamSynchronize(h->c.stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        h->c.set_error(hipGetErrorString(e));
+        return -3;
+    }
+    return 0;
+}
+int bad(sdpadd_handle h, const char *msg) {
+    if (h) h->c.set_error(msg);
+    return -1;
+}
+}  // namespace
+
+extern "C" {
+
+int sdpadd_create(int device, sdpadd_handle *h) {
+    if (!h) return -1;
+    auto *ctx = new sdpadd_ctx();
+    if (ctx->c.init(device) != 0) {
+        delete ctx;
+        *h = nullptr;
+        return -3;
+    }
+    *h = ctx;
+    return 0;
+}
+int sdpadd_destroy(sdpadd_handle h) {
+    if (!h) return -1;
+    h->c.destroy();
+    delete h;
+    return 0;
+}
+const char *sdpadd_last_error(sdpadd_handle h) { return h ? h->c.last_error.c_str() : "null handle"; }
+int sdpadd_set_dot_partition(sdpadd_handle h, int chunks) {
+    if (!h || chunks < 1 || chunks > 4096) return bad(h, "dot partition must be in [1, 4096]");
+    h->c.dot_T = chunks;
+    return 0;
+}
+int sdpadd_stats_enable(sdpadd_handle h, int on) {
+    if (!h) return -1;
+    sdk::stats_enable(on != 0);
+    return 0;
+}
+int sdpadd_stats_reset(sdpadd_handle h) {
+    if (!h) return -1;
+    sdk::stats_reset();
+    return 0;
+}
+int sdpadd_stats_get(sdpadd_handle h, const char *family, int64_t *launches, double *ms, double *ddflops) {
+    if (!h || !family) return -1;
+    sdk::stats_get(family, launches, ms, ddflops);
+    return 0;
+}
+int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr) {
+    if (!h || !d_ptr) return -1;
+    if (hipMalloc(d_ptr, bytes ? bytes : 16) != hipSuccess) return bad(h, "hipMalloc failed");
+    return 0;
+}
+int sdpadd_free(sdpadd_handle h, void *d_ptr) {
+    if (!h) return -1;
+    return hipFree(d_ptr) == hipSuccess ? 0 : bad(h, "hipFree failed");
+}
+int sdpadd_upload(sdpadd_handle h, void *d_dst, const void *src, size_t bytes) {
+    if (!h) return -1;
+    if (hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return bad(h, "upload failed");
+    return 0;
+}
+int sdpadd_download(sdpadd_handle h, void *dst, const void *d_src, size_t bytes) {
+    if (!h) return -1;
+    if (hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost) != hipSuccess) return bad(h, "download failed");
+    return 0;
+}
+
+int sdpadd_Rgemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n, int64_t k, const double *alpha,
+                 const double *d_A, int64_t lda, const double *d_B, int64_t ldb, const double *beta, double *d_C,
+                 int64_t ldc) {
+    if (!h || !up(transa, 'N') || !(up(transb, 'N') || up(transb, 'T'))) return bad(h, "Rgemm: unsupported op");
+    if (m < 0 || n < 0 || k < 0 || lda < std::max<int64_t>(1, m) || ldc < std::max<int64_t>(1, m) ||
+        ldb < std::max<int64_t>(1, up(transb, 'N') ? k : n))
+        return bad(h, "Rgemm: bad dimension");
+    sdk::gemm(h->c.stream, up(transb, 'T') ? 'T' : 'N', m, n, k, D(alpha), DP(d_A), lda, DP(d_B), ldb, D(beta), DP(d_C), ldc);
+    return check(h);
+}
+int sdpadd_Rsyrk(sdpadd_handle h, char uplo, char trans, int64_t n, int64_t k, const double *alpha, const double *d_A,
+                 int64_t lda, const double *beta, double *d_C, int64_t ldc) {
+    if (!h || !up(uplo, 'L') || !up(trans, 'N')) return bad(h, "Rsyrk: unsupported op");
+    if (n < 0 || k < 0 || lda < std::max<int64_t>(1, n) || ldc < std::max<int64_t>(1, n)) return bad(h, "Rsyrk: bad dimension");
+    // Rsyrk skips columns with a[j,l] == 0; adding those zero products leaves
+    // every value unchanged, so the GEMM order gives the same bits.
+    sdk::syrk_ln(h->c.stream, n, k, D(alpha), DP(d_A), lda, D(beta), DP(d_C), ldc);
+    return check(h);
+}
+int sdpadd_Rtrsm(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n, const double *alpha,
+                 const double *d_A, int64_t lda, double *d_B, int64_t ldb) {
+    if (!h || !(alpha[0] == 1.0 && alpha[1] == 0.0) || !up(uplo, 'L') || !up(diag, 'N')) return bad(h, "Rtrsm: unsupported op");
+    if (m < 0 || n < 0 || ldb < std::max<int64_t>(1, m)) return bad(h, "Rtrsm: bad dimension");
+    if (up(side, 'R') && up(transa, 'T'))
+        sdk::trsm_rltn(h->c.stream, m, n, DP(d_A), lda, DP(d_B), ldb);
+    else if (up(side, 'L') && up(transa, 'N'))
+        sdk::trsm_llnn(h->c.stream, m, n, DP(d_A), lda, DP(d_B), ldb);
+    else
+        return bad(h, "Rtrsm: unsupported op");
+    return check(h);
+}
+int sdpadd_Rtrmm(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n, const double *alpha,
+                 const double *d_A, int64_t lda, double *d_B, int64_t ldb) {
+    if (!h || !(alpha[0] == 1.0 && alpha[1] == 0.0) || !up(side, 'L') || !up(uplo, 'L') || !up(transa, 'T') || !up(diag, 'N'))
+        return bad(h, "Rtrmm: unsupported op");
+    if (m < 0 || n < 0 || ldb < std::max<int64_t>(1, m)) return bad(h, "Rtrmm: bad dimension");
+    dd *tmp = nullptr;
+    if (hipMalloc((void **)&tmp, sizeof(dd) * std::max<int64_t>(1, m * n)) != hipSuccess) return bad(h, "hipMalloc failed");
+    sdk::trmm_lltn(h->c.stream, m, n, DP(d_A), lda, DP(d_B), ldb, tmp, m);
+    (void)hipMemcpy2DAsync(d_B, sizeof(dd) * ldb, tmp, sizeof(dd) * m, sizeof(dd) * m, n, hipMemcpyDeviceToDevice,
+                           h->c.stream);
+    int rc = check(h);
+    (void)hipFree(tmp);
+    return rc;
+}
+int sdpadd_Rtrsv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, const double *d_A, int64_t lda, double *d_x,
+                 int64_t incx) {
+    if (!h || !up(uplo, 'L') || !up(diag, 'N') || incx != 1 || !(up(trans, 'N') || up(trans, 'T')))
+        return bad(h, "Rtrsv: unsupported op");
+    sdk::trsv_lower(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x));
+    return check(h);
+}
+int sdpadd_Rtrmv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, const double *d_A, int64_t lda, double *d_x,
+                 int64_t incx) {
+    if (!h || !up(uplo, 'L') || !up(diag, 'N') || incx != 1 || !(up(trans, 'N') || up(trans, 'T')))
+        return bad(h, "Rtrmv: unsupported op");
+    dd *tmp = nullptr;
+    if (hipMalloc((void **)&tmp, sizeof(dd) * std::max<int64_t>(1, n)) != hipSuccess) return bad(h, "hipMalloc failed");
+    sdk::trmv_lower(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x), tmp);
+    sdk::copy(h->c.stream, n, tmp, DP(d_x));
+    int rc = check(h);
+    (void)hipFree(tmp);
+    return rc;
+}
+int sdpadd_Rgemv(sdpadd_handle h, char trans, int64_t m, int64_t n, const double *alpha, const double *d_A, int64_t lda,
+                 const double *d_x, int64_t incx, const double *beta, double *d_y, int64_t incy) {
+    if (!h || !up(trans, 'N') || incx != 1 || incy != 1) return bad(h, "Rgemv: unsupported op");
+    sdk::gemv_n(h->c.stream, m, n, D(alpha), DP(d_A), lda, DP(d_x), D(beta), DP(d_y));
+    return check(h);
+}
+int sdpadd_Rdot(sdpadd_handle h, int64_t n, const double *d_x, int64_t incx, const double *d_y, int64_t incy,
+                double *result) {
+    if (!h || incx != 1 || incy != 1) return bad(h, "Rdot: unsupported op");
+    h->c.dot(n, DP(d_x), DP(d_y), 0);
+    (void)hipMemcpyAsync(result, h->c.d_scal, sizeof(dd), hipMemcpyDeviceToHost, h->c.stream);
+    return check(h);
+}
+int sdpadd_Rpotrf(sdpadd_handle h, char uplo, int64_t n, double *d_A, int64_t lda, int64_t *info) {
+    if (!h || !up(uplo, 'L')) return bad(h, "Rpotrf: unsupported op");
+    if (n < 0 || lda < std::max<int64_t>(1, n)) return bad(h, "Rpotrf: bad dimension");
+    sdk::potrf_lower(h->c.stream, n, DP(d_A), lda, h->c.d_info, h->c.d_rinv);
+    (void)hipMemcpyAsync(info, h->c.d_info, sizeof(int64_t), hipMemcpyDeviceToHost, h->c.stream);
+    return check(h);
+}
+int sdpadd_Rsteqr(sdpadd_handle h, char compz, int64_t n, double *d, double *e, double *z, int64_t ldz, double *work,
+                  int64_t *info) {
+    if (!up(compz, 'I') || n < 0 || ldz < std::max<int64_t>(1, n)) return bad(h, "Rsteqr: unsupported op");
+    static_assert(sizeof(ddr) == 16, "ddr layout");
+    *info = sdp::steqr_rows(n, reinterpret_cast<ddr *>(d), reinterpret_cast<ddr *>(e), reinterpret_cast<ddr *>(z), ldz,
+                            reinterpret_cast<ddr *>(work), 0, n);
+    return 0;
+}
+
+int sdpadd_lanczos_min_eig(sdpadd_handle h, int64_t n, const double *d_L, const double *d_M, double *out) {
+    if (!h || n < 1) return bad(h, "lanczos: bad dimension");
+    dd r = sdp::lanczos_min_eigen(h->c, (int)n, DP(d_L), DP(d_M));
+    out[0] = r.hi;
+    out[1] = r.lo;
+    return check(h);
+}
+int sdpadd_inv_chol(sdpadd_handle h, int64_t n, const double *d_A, double *d_work, double *d_invChol, double *d_inv) {
+    if (!h || n < 1) return bad(h, "inv_chol: bad dimension");
+    hipStream_t s = h->c.stream;
+    sdk::copy(s, n * n, DP(d_A), DP(d_work));
+    sdk::potrf_lower(s, n, DP(d_work), n, h->c.d_info, h->c.d_rinv);
+    int64_t info = 0;
+    (void)hipMemcpyAsync(&info, h->c.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
+    int rc = check(h);
+    if (rc) return rc;
+    if (info != 0) return (int)info;
+    sdk::zero_upper(s, n, DP(d_work));
+    sdk::inv_lower(s, n, DP(d_work), n, DP(d_invChol), n);
+    if (d_inv) sdk::trmm_lltn_self(s, n, DP(d_invChol), n, DP(d_inv), n);
+    return check(h);
+}
+
+// ---- problems ----
+int sdpadd_problem_load(sdpadd_handle h, const char *path, sdpadd_problem *p) {
+    if (!h || !path || !p) return -1;
+    auto *pr = new sdpadd_problem_s();
+    pr->h = h;
+    std::string err;
+    if (!sdp::read_dat_s(path, pr->hp, err)) {
+        h->c.set_error(err);
+        delete pr;
+        return -1;
+    }
+    if (sdp::upload_problem(pr->hp, pr->dp, h->c.stream, err) != 0) {
+        h->c.set_error(err);
+        delete pr;
+        return -3;
+    }
+    pr->finalized = true;
+    *p = pr;
+    return 0;
+}
+int sdpadd_problem_create(sdpadd_handle h, int m, int nblock, const int *bs, sdpadd_problem *p) {
+    if (!h || m < 1 || nblock < 1 || !bs || !p) return bad(h, "problem_create: bad arguments");
+    for (int l = 0; l < nblock; l++)
+        if (bs[l] <= 20) return bad(h, "problem_create: SDP blocks must be larger than 20");
+    auto *pr = new sdpadd_problem_s();
+    pr->h = h;
+    pr->hp.m = m;
+    pr->hp.bs.assign(bs, bs + nblock);
+    pr->hp.b.assign(m, mk(0.0));
+    pr->hp.A.assign(m, sdp::Space());
+    *p = pr;
+    return 0;
+}
+int sdpadd_problem_set_entries(sdpadd_problem p, int k, int l, int64_t nnz, const int *rows, const int *cols,
+                               const double *vals) {
+    if (!p || p->finalized || k < 0 || k > p->hp.m || l < 0 || l >= (int)p->hp.bs.size() || nnz < 0)
+        return bad(p ? p->h : nullptr, "set_entries: bad arguments");
+    sdp::Space &sp = k ? p->hp.A[k - 1] : p->hp.C;
+    int slot = sp.find(l);
+    if (slot < 0) {
+        // keep the block list sorted by block number
+        size_t pos = 0;
+        while (pos < sp.index.size() && sp.index[pos] < l) pos++;
+        sp.index.insert(sp.index.begin() + pos, l);
+        sdp::Block b;
+        b.n = p->hp.bs[l];
+        sp.blk.insert(sp.blk.begin() + pos, b);
+        slot = (int)pos;
+    }
+    sdp::Block &b = sp.blk[slot];
+    for (int64_t q = 0; q < nnz; q++) {
+        if (rows[q] < 0 || cols[q] < 0 || rows[q] >= b.n || cols[q] >= b.n) return bad(p->h, "set_entries: index out of range");
+        b.row.push_back(rows[q]);
+        b.col.push_back(cols[q]);
+        b.val.push_back(D(vals + 2 * q));
+        b.nnz_count++;
+        b.nnz_effect += (rows[q] == cols[q]) ? 1 : 2;
+    }
+    return 0;
+}
+int sdpadd_problem_set_b(sdpadd_problem p, const double *b) {
+    if (!p || p->finalized) return -1;
+    for (int i = 0; i < p->hp.m; i++) p->hp.b[i] = D(b + 2 * i);
+    return 0;
+}
+int sdpadd_problem_finalize(sdpadd_problem p) {
+    if (!p || p->finalized) return -1;
+    for (auto &b : p->hp.C.blk) b.change_to_dense();
+    for (auto &a : p->hp.A)
+        for (auto &b : a.blk) b.change_to_dense();
+    p->hp.layout();
+    p->hp.index();
+    p->hp.formula = sdp::compute_formula(p->hp, ddr(1.2));
+    std::string err;
+    if (sdp::upload_problem(p->hp, p->dp, p->h->c.stream, err) != 0) {
+        p->h->c.set_error(err);
+        return -3;
+    }
+    p->finalized = true;
+    return 0;
+}
+int sdpadd_problem_destroy(sdpadd_problem p) {
+    if (!p) return -1;
+    sdp::free_problem(p->dp);
+    delete p;
+    return 0;
+}
+int sdpadd_problem_dims(sdpadd_problem p, int *m, int *nblock, int *bs, int cap) {
+    if (!p) return -1;
+    *m = p->hp.m;
+    *nblock = (int)p->hp.bs.size();
+    for (int l = 0; l < *nblock && l < cap; l++) bs[l] = p->hp.bs[l];
+    return 0;
+}
+int sdpadd_problem_formula(sdpadd_problem p, int *out) {
+    if (!p || !p->finalized) return -1;
+    std::memcpy(out, p->hp.formula.data(), sizeof(int) * p->hp.formula.size());
+    return 0;
+}
+int sdpadd_schur_build(sdpadd_problem p, const double *d_X, const double *d_Zinv, double *d_B) {
+    if (!p || !p->finalized) return -1;
+    sdp::schur(p->h->c.stream, p->dp, DP(d_X), DP(d_Zinv), DP(d_B), p->h->c.dot_T);
+    return check(p->h);
+}
+int sdpadd_inner_A(sdpadd_problem p, const double *d_W, double *d_out) {
+    if (!p || !p->finalized) return -1;
+    sdp::inner_A(p->h->c.stream, p->dp, DP(d_W), DP(d_out), p->h->c.dot_T);
+    return check(p->h);
+}
+int sdpadd_plus_A(sdpadd_problem p, const double *d_y, double *d_R) {
+    if (!p || !p->finalized) return -1;
+    sdp::plus_A(p->h->c.stream, p->dp, DP(d_y), DP(d_R));
+    return check(p->h);
+}
+
+void sdpadd_default_params(sdpadd_params *prm) {
+    sdp::Params d;
+    prm->maxIteration = d.maxIteration;
+    prm->epsilonStar = d.epsilonStar;
+    prm->lambdaStar = d.lambdaStar;
+    prm->omegaStar = d.omegaStar;
+    prm->lowerBound = d.lowerBound;
+    prm->upperBound = d.upperBound;
+    prm->betaStar = d.betaStar;
+    prm->betaBar = d.betaBar;
+    prm->gammaStar = d.gammaStar;
+    prm->epsilonDash = d.epsilonDash;
+}
+int sdpadd_read_params(const char *path, sdpadd_params *prm) {
+    sdp::Params d;
+    if (!sdp::read_params(path, d)) return -1;
+    prm->maxIteration = d.maxIteration;
+    prm->epsilonStar = d.epsilonStar;
+    prm->lambdaStar = d.lambdaStar;
+    prm->omegaStar = d.omegaStar;
+    prm->lowerBound = d.lowerBound;
+    prm->upperBound = d.upperBound;
+    prm->betaStar = d.betaStar;
+    prm->betaBar = d.betaBar;
+    prm->gammaStar = d.gammaStar;
+    prm->epsilonDash = d.epsilonDash;
+    return 0;
+}
+int sdpadd_solve(sdpadd_problem p, const sdpadd_params *prm, sdpadd_result *res) {
+    if (!p || !p->finalized || !prm || !res) return -1;
+    sdp::Params d;
+    d.maxIteration = prm->maxIteration;
+    d.epsilonStar = prm->epsilonStar;
+    d.lambdaStar = prm->lambdaStar;
+    d.omegaStar = prm->omegaStar;
+    d.lowerBound = prm->lowerBound;
+    d.upperBound = prm->upperBound;
+    d.betaStar = prm->betaStar;
+    d.betaBar = prm->betaBar;
+    d.gammaStar = prm->gammaStar;
+    d.epsilonDash = prm->epsilonDash;
+    sdp::SolveOutput out;
+    int rc = sdp::solve(p->h->c, p->hp, p->dp, d, out, &p->X, &p->y, &p->Z);
+    if (rc != 0) return rc;
+    p->trace = out.trace;
+    res->iterations = out.iterations;
+    res->phase = out.phase_reported;
+    res->phase_internal = out.phase;
+    auto put = [](double *dst, dd v) {
+        dst[0] = v.hi;
+        dst[1] = v.lo;
+    };
+    put(res->mu, out.mu);
+    put(res->objValPrimal, out.objValPrimal);
+    put(res->objValDual, out.objValDual);
+    put(res->normPrimalVec, out.normPrimalVec);
+    put(res->normDualMat, out.normDualMat);
+    put(res->relgap, out.relgap);
+    res->seconds = out.t.total;
+    res->t_schur = out.t.schur;
+    res->t_chol = out.t.chol;
+    res->t_lanczos = out.t.lanczos;
+    res->t_inverse = out.t.inverse;
+    return 0;
+}
+int sdpadd_trace(sdpadd_problem p, double *out, int cap) {
+    if (!p) return -1;
+    int n = (int)p->trace.size();
+    for (int r = 0; r < n && r < cap; r++) {
+        const sdp::IterRecord &t = p->trace[r];
+        double *o = out + 17 * r;
+        o[0] = t.it;
+        const dd v[8] = {t.mu, t.thetaP, t.thetaD, t.objP, t.objD, t.alphaP, t.alphaD, t.beta};
+        for (int q = 0; q < 8; q++) {
+            o[1 + 2 * q] = v[q].hi;
+            o[2 + 2 * q] = v[q].lo;
+        }
+    }
+    return n;
+}
+int sdpadd_solution(sdpadd_problem p, double *X, double *y, double *Z) {
+    if (!p || p->X.empty()) return -1;
+    if (X) std::memcpy(X, p->X.data(), sizeof(dd) * p->X.size());
+    if (y) std::memcpy(y, p->y.data(), sizeof(dd) * p->y.size());
+    if (Z) std::memcpy(Z, p->Z.data(), sizeof(dd) * p->Z.size());
+    return 0;
+}
+
+}  // extern "C"

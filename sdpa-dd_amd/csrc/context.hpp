@@ -1,0 +1,33 @@
+// Library handle internals: device, stream and reusable scratch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sdp {
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int dot_T = 1024;           // chunks of the static dot partition
+    dd *d_dot_work = nullptr;   // >= dot_T dd
+    unsigned *d_counter = nullptr;
+    dd *d_scal = nullptr;       // scalar slots for dot results
+    int64_t *d_info = nullptr;
+    dd *d_rinv = nullptr;       // Cholesky panel reciprocals (>= 64)
+    dd *d_max_work = nullptr;   // max-abs partials (>= 256)
+    std::string last_error;
+
+    int init(int dev);
+    void destroy();
+    // Fixed-partition dot into scalar slot `slot` (device).
+    void dot(sdk::idx_t n, const dd *x, const dd *y, int slot) {
+        sdk::dot(stream, n, x, y, d_scal + slot, dot_T, d_dot_work, d_counter);
+    }
+    void set_error(const std::string &e) { last_error = e; }
+};
+
+}  // namespace sdp

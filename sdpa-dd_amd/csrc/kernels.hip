@@ -1,0 +1,823 @@
+// Dense dd kernels for CDNA4 (gfx950): GEMM, Cholesky, triangular solves,
+// inverse of a lower triangle, TRMM, mat-vecs, dots and elementwise ops.
+//
+// Bit-exactness contract: every output element goes through the same
+// sequence of dd operations as in the column-major MPLAPACK loops the CPU
+// solver runs (mplapack/Rgemm_NN_omp.cpp, Rsyrk.cpp, Rtrsm.cpp, Rtrmm.cpp,
+// Rtrsv.cpp, Rtrmv.cpp, Rgemv.cpp, Rpotrf.cpp, Rpotrf2.cpp, Rdot_omp.cpp).
+// Parallelism comes from distributing independent elements; the k order of
+// each element's accumulation is never split or reassociated.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sdk {
+
+// ------------------------------------------------------------- statistics --
+namespace {
+struct StatEntry {
+    int64_t launches = 0;
+    double flops = 0, ms = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+bool g_stats_on = false;
+std::map<std::string, StatEntry> g_stats;
+std::vector<hipEvent_t> g_event_pool;
+std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> g_open;
+hipEvent_t get_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+void resolve(StatEntry &st) {
+    for (auto &pr : st.pending) {
+        (void)hipEventSynchronize(pr.second);
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) st.ms += ms;
+        g_event_pool.push_back(pr.first);
+        g_event_pool.push_back(pr.second);
+    }
+    st.pending.clear();
+}
+}  // namespace
+
+void stats_enable(bool on) { g_stats_on = on; }
+void stats_reset() {
+    for (auto &kv : g_stats) resolve(kv.second);
+    g_stats.clear();
+}
+bool stats_get(const char *family, int64_t *launches, double *ms, double *ddflops) {
+    auto it = g_stats.find(family);
+    if (it == g_stats.end()) {
+        *launches = 0;
+        *ms = 0;
+        *ddflops = 0;
+        return false;
+    }
+    resolve(it->second);
+    *launches = it->second.launches;
+    *ms = it->second.ms;
+    *ddflops = it->second.flops;
+    return true;
+}
+StatScope::StatScope(const char *family, hipStream_t stream, double ddflops) {
+    if (!g_stats_on) return;
+    s = stream;
+    StatEntry &st = g_stats[family];
+    st.launches++;
+    st.flops += ddflops;
+    hipEvent_t a = get_event(), b = get_event();
+    (void)hipEventRecord(a, s);
+    g_open.push_back({family, {a, b}});
+    slot = (int)g_open.size() - 1;
+}
+StatScope::~StatScope() {
+    if (slot < 0) return;
+    auto pr = g_open[slot];
+    (void)hipEventRecord(pr.second.second, s);
+    g_stats[pr.first].pending.push_back(pr.second);
+    g_open.pop_back();
+}
+
+static constexpr int TB = 64;   // GEMM tile (rows and columns)
+static constexpr int TK = 16;   // GEMM k-chunk staged in LDS
+static constexpr int NT = 256;  // threads per GEMM workgroup
+
+static inline unsigned cdiv(idx_t a, idx_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ GEMM --
+// alpha_kind: 0 general, 1 alpha == 1 (alpha*b == b exactly), 2 alpha == -1.
+// beta_kind: 0 beta == 0, 1 beta == 1, 2 general.
+template <bool TRANSB, bool PTRS>
+__global__ __launch_bounds__(NT) void k_gemm(idx_t m, idx_t n, idx_t k, dd alpha, int alpha_kind, const dd *__restrict__ A,
+                                             idx_t lda, const dd *__restrict__ B, idx_t ldb, dd beta, int beta_kind,
+                                             dd *__restrict__ C, idx_t ldc, idx_t sA, idx_t sB, idx_t sC, int lower_only,
+                                             idx_t diag_offset, const dd *const *Ap, const dd *const *Bp, dd *const *Cp) {
+    __shared__ dd As[TK][TB];
+    __shared__ dd Bs[TK][TB];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const idx_t row0 = (idx_t)blockIdx.x * TB, col0 = (idx_t)blockIdx.y * TB;
+    if (PTRS) {
+        A = Ap[blockIdx.z];
+        B = Bp[blockIdx.z];
+        C = Cp[blockIdx.z];
+    } else {
+        A += (idx_t)blockIdx.z * sA;
+        B += (idx_t)blockIdx.z * sB;
+        C += (idx_t)blockIdx.z * sC;
+    }
+    // element (i,j) is "lower" when i + diag_offset >= j
+    if (lower_only && row0 + TB - 1 + diag_offset < col0) return;
+
+    dd acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            idx_t i = row0 + tx + 16 * r, j = col0 + ty + 16 * c;
+            dd v = mk(0.0);
+            if (i < m && j < n && beta_kind != 0) {
+                v = C[i + j * ldc];
+                if (beta_kind == 2) v = dd_mul(beta, v);
+            }
+            acc[r][c] = v;
+        }
+    }
+    for (idx_t k0 = 0; k0 < k; k0 += TK) {
+        const int kc = (int)((k - k0) < TK ? (k - k0) : TK);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int e = tid + NT * q;
+            int row = e & 63, kk = e >> 6;
+            idx_t i = row0 + row;
+            As[kk][row] = (i < m && kk < kc) ? A[i + (k0 + kk) * lda] : mk(0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int e = tid + NT * q;
+            int kk, col;
+            if (TRANSB) {
+                col = e & 63;
+                kk = e >> 6;
+            } else {
+                kk = e & 15;
+                col = e >> 4;
+            }
+            idx_t j = col0 + col;
+            dd b = mk(0.0);
+            if (j < n && kk < kc) {
+                b = TRANSB ? B[j + (k0 + kk) * ldb] : B[(k0 + kk) + j * ldb];
+                if (alpha_kind == 0)
+                    b = dd_mul(alpha, b);
+                else if (alpha_kind == 2)
+                    b = dd_neg(b);
+            }
+            Bs[kk][col] = b;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; kk++) {
+            dd a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) a[r] = As[kk][tx + 16 * r];
+#pragma unroll
+            for (int c = 0; c < 4; c++) b[c] = Bs[kk][ty + 16 * c];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[r][c] = dd_add_eq(acc[r][c], dd_mul(b[c], a[r]));
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            idx_t i = row0 + tx + 16 * r, j = col0 + ty + 16 * c;
+            if (i < m && j < n && (!lower_only || i + diag_offset >= j)) C[i + j * ldc] = acc[r][c];
+        }
+    }
+}
+
+static int kind_alpha(dd a) {
+    if (a.hi == 1.0 && a.lo == 0.0) return 1;
+    if (a.hi == -1.0 && a.lo == 0.0) return 2;
+    return 0;
+}
+static int kind_beta(dd b) {
+    if (b.hi == 0.0 && b.lo == 0.0) return 0;
+    if (b.hi == 1.0 && b.lo == 0.0) return 1;
+    return 2;
+}
+
+__global__ void k_gemm_scale_only(idx_t m, idx_t n, dd beta, int beta_kind, dd *C, idx_t ldc, idx_t sC) {
+    C += (idx_t)blockIdx.z * sC;
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= m || j >= n) return;
+    dd &c = C[i + j * ldc];
+    c = (beta_kind == 0) ? mk(0.0) : dd_mul(beta, c);
+}
+
+void gemm(hipStream_t s, char tb, idx_t m, idx_t n, idx_t k, dd alpha, const dd *A, idx_t lda, const dd *B, idx_t ldb,
+          dd beta, dd *C, idx_t ldc, int batch, idx_t sA, idx_t sB, idx_t sC, bool lower_only, idx_t diag_offset) {
+    if (m <= 0 || n <= 0 || batch <= 0) return;
+    int ak = kind_alpha(alpha), bk = kind_beta(beta);
+    bool alpha_zero = (alpha.hi == 0.0 && alpha.lo == 0.0);
+    // Rgemm_omp.cpp:87-106: quick returns and the alpha == 0 path.
+    if ((alpha_zero || k == 0) && bk == 1) return;
+    if (alpha_zero) {
+        dim3 g(cdiv(m, 256), (unsigned)n, (unsigned)batch);
+        k_gemm_scale_only<<<g, 256, 0, s>>>(m, n, beta, bk, C, ldc, sC);
+        return;
+    }
+    dim3 grid(cdiv(m, TB), cdiv(n, TB), (unsigned)batch);
+    StatScope st("gemm", s, 2.0 * (double)m * n * k * batch * (lower_only ? 0.5 : 1.0));
+    if (tb == 'T' || tb == 't')
+        k_gemm<true, false><<<grid, NT, 0, s>>>(m, n, k, alpha, ak, A, lda, B, ldb, beta, bk, C, ldc, sA, sB, sC,
+                                                 lower_only ? 1 : 0, diag_offset, nullptr, nullptr, nullptr);
+    else
+        k_gemm<false, false><<<grid, NT, 0, s>>>(m, n, k, alpha, ak, A, lda, B, ldb, beta, bk, C, ldc, sA, sB, sC,
+                                                  lower_only ? 1 : 0, diag_offset, nullptr, nullptr, nullptr);
+}
+
+void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx_t lda, const dd *const *B, idx_t ldb,
+               dd *const *C, idx_t ldc, int batch) {
+    if (m <= 0 || n <= 0 || batch <= 0) return;
+    dim3 grid(cdiv(m, TB), cdiv(n, TB), (unsigned)batch);
+    StatScope st("gemm", s, 2.0 * (double)m * n * k * batch);
+    k_gemm<false, true><<<grid, NT, 0, s>>>(m, n, k, mk(1.0), 1, nullptr, lda, nullptr, ldb, mk(0.0), 0, nullptr, ldc, 0, 0,
+                                             0, 0, 0, A, B, C);
+}
+
+// -------------------------------------------------------------- Cholesky --
+static constexpr int PB = 64;  // Cholesky panel width
+
+// Factor the nb x nb diagonal block at (K,K) in LDS; right-looking within the
+// block (each element still sees its updates in ascending column order).
+// Stores one/L_kk for the panel solve.
+__global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
+    __shared__ dd S[PB][PB + 1];
+    __shared__ dd rl[PB];
+    __shared__ int failed;
+    const int tid = threadIdx.x;
+    if (*info != 0) return;
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e % nb, j = e / nb;
+        if (i >= j) S[i][j] = A[(K + i) + (K + j) * lda];
+    }
+    if (tid == 0) failed = 0;
+    __syncthreads();
+    for (int k = 0; k < nb; k++) {
+        if (tid == 0) {
+            dd a = S[k][k];
+            if (dd_le(a, mk(0.0)) || dd_isnan(a)) {
+                failed = 1;
+                *info = K + k + 1;
+            } else {
+                dd l = dd_sqrt(a);
+                S[k][k] = l;
+                rl[k] = dd_div(mk(1.0), l);
+            }
+        }
+        __syncthreads();
+        if (failed) return;
+        dd r = rl[k];
+        for (int i = k + 1 + tid; i < nb; i += 256) S[i][k] = dd_mul(r, S[i][k]);
+        __syncthreads();
+        // rank-1 update of the remaining lower triangle
+        const int rem = nb - k - 1;
+        for (int e = tid; e < rem * rem; e += 256) {
+            int i = k + 1 + e % rem, j = k + 1 + e / rem;
+            if (i >= j) {
+                dd ljk = S[j][k];
+                if (!dd_is_zero(ljk)) S[i][j] = dd_sub(S[i][j], dd_mul(ljk, S[i][k]));
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e % nb, j = e / nb;
+        if (i >= j) A[(K + i) + (K + j) * lda] = S[i][j];
+    }
+    for (int k = tid; k < nb; k += 256) rinv[k] = rl[k];
+}
+
+// Panel solve: rows [r0, r0+nrows) of columns K..K+nb-1 against the factored
+// diagonal block (Rtrsm 'R','L','T','N' order).  32 rows per workgroup.
+__global__ __launch_bounds__(256) void k_potrf_trsm(dd *A, idx_t lda, idx_t K, int nb, idx_t r0, idx_t nrows,
+                                                   const int64_t *info, const dd *rinv) {
+    __shared__ dd X[32][PB + 1];
+    __shared__ dd L[PB][PB + 1];
+    if (*info != 0) return;
+    const int tid = threadIdx.x;
+    const idx_t rb = r0 + (idx_t)blockIdx.x * 32;
+    const int nr = (int)((r0 + nrows - rb) < 32 ? (r0 + nrows - rb) : 32);
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e % nb, j = e / nb;
+        L[i][j] = (i >= j) ? A[(K + i) + (K + j) * lda] : mk(0.0);
+    }
+    for (int e = tid; e < 32 * nb; e += 256) {
+        int r = e % 32, j = e / 32;
+        if (r < nr) X[r][j] = A[(rb + r) + (K + j) * lda];
+    }
+    __syncthreads();
+    for (int k = 0; k < nb; k++) {
+        if (tid < nr) X[tid][k] = dd_mul(rinv[k], X[tid][k]);
+        __syncthreads();
+        const int rem = nb - k - 1;
+        for (int e = tid; e < nr * rem; e += 256) {
+            int r = e % nr, j = k + 1 + e / nr;
+            dd ljk = L[j][k];
+            if (!dd_is_zero(ljk)) X[r][j] = dd_sub(X[r][j], dd_mul(ljk, X[r][k]));
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < 32 * nb; e += 256) {
+        int r = e % 32, j = e / 32;
+        if (r < nr) A[(rb + r) + (K + j) * lda] = X[r][j];
+    }
+}
+
+__global__ void k_zero_i64(int64_t *p) { *p = 0; }
+
+void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch) {
+    k_zero_i64<<<1, 1, 0, s>>>(d_info);
+    for (idx_t K = 0; K < n; K += PB) {
+        int nb = (int)std::min<idx_t>(PB, n - K);
+        idx_t below = n - K - nb;
+        {
+            StatScope st("potrf", s, (double)nb * nb * nb / 3.0 + (double)below * nb * nb);
+            k_potrf_diag<<<1, 256, 0, s>>>(A, lda, K, nb, d_info, scratch);
+            if (below > 0) k_potrf_trsm<<<cdiv(below, 32), 256, 0, s>>>(A, lda, K, nb, K + nb, below, d_info, scratch);
+        }
+        if (below > 0) {
+            // trailing lower triangle: A22 += (-L21) * L21^T, k ascending
+            dd *A21 = A + (K + nb) + K * lda;
+            dd *A22 = A + (K + nb) + (K + nb) * lda;
+            gemm(s, 'T', below, below, nb, mk(-1.0), A21, lda, A21, lda, mk(1.0), A22, lda, 1, 0, 0, 0, true, 0);
+        }
+    }
+}
+
+// ------------------------------------------------------ triangular solves --
+__global__ __launch_bounds__(1024) void k_trsv_ln(idx_t n, const dd *L, idx_t lda, dd *x) {
+    __shared__ dd xj;
+    __shared__ int nz;
+    const int tid = threadIdx.x;
+    for (idx_t j = 0; j < n; j++) {
+        if (tid == 0) {
+            dd v = x[j];
+            nz = !dd_is_zero(v);
+            if (nz) {
+                v = dd_div(v, L[j + j * lda]);
+                x[j] = v;
+            }
+            xj = v;
+        }
+        __syncthreads();
+        if (nz) {
+            dd t = xj;
+            for (idx_t i = j + 1 + tid; i < n; i += 1024) x[i] = dd_sub(x[i], dd_mul(t, L[i + j * lda]));
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_trsv_lt(idx_t n, const dd *L, idx_t lda, dd *x) {
+    __shared__ dd xj;
+    const int tid = threadIdx.x;
+    for (idx_t j = n - 1; j >= 0; j--) {
+        if (tid == 0) {
+            dd v = dd_div(x[j], L[j + j * lda]);
+            x[j] = v;
+            xj = v;
+        }
+        __syncthreads();
+        dd t = xj;
+        for (idx_t k = tid; k < j; k += 1024) x[k] = dd_sub(x[k], dd_mul(L[j + k * lda], t));
+        __syncthreads();
+    }
+}
+
+void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x) {
+    if (n <= 0) return;
+    StatScope st("trsv", s, (double)n * n);
+    if (trans == 'N' || trans == 'n')
+        k_trsv_ln<<<1, 1024, 0, s>>>(n, L, lda, x);
+    else
+        k_trsv_lt<<<1, 1024, 0, s>>>(n, L, lda, x);
+}
+
+// ---------------------------------------------- inverse of a lower triangle --
+__global__ void k_set_identity(idx_t n, dd v, dd *A, idx_t lda) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < n) A[i + j * lda] = (i == j) ? v : mk(0.0);
+}
+
+// Rows [rb, rb+nb) of B := L^{-1} B for the column range [c0, c0+32), with
+// the updates from rows above already applied (Rtrsm 'L','L','N','N' order:
+// b_kj = b_kj / a_kk, then b_ij -= b_kj * a_ik for i > k).
+__global__ __launch_bounds__(256) void k_trsm_llnn_diag(idx_t rb, int nb, idx_t ncols, const dd *L, idx_t lda, dd *B,
+                                                        idx_t ldb) {
+    __shared__ dd X[PB][33];
+    const int tid = threadIdx.x;
+    const idx_t c0 = (idx_t)blockIdx.x * 32;
+    const int nc = (int)((ncols - c0) < 32 ? (ncols - c0) : 32);
+    for (int e = tid; e < nb * 32; e += 256) {
+        int r = e % nb, c = e / nb;
+        if (c < nc) X[r][c] = B[(rb + r) + (c0 + c) * ldb];
+    }
+    __syncthreads();
+    for (int k = 0; k < nb; k++) {
+        if (tid < nc) {
+            dd v = X[k][tid];
+            if (!dd_is_zero(v)) X[k][tid] = dd_div(v, L[(rb + k) + (rb + k) * lda]);
+        }
+        __syncthreads();
+        const int rem = nb - k - 1;
+        for (int e = tid; e < rem * nc; e += 256) {
+            int i = k + 1 + e % rem, c = e / rem;
+            dd bk = X[k][c];
+            if (!dd_is_zero(bk)) X[i][c] = dd_sub(X[i][c], dd_mul(bk, L[(rb + i) + (rb + k) * lda]));
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < nb * 32; e += 256) {
+        int r = e % nb, c = e / nb;
+        if (c < nc) B[(rb + r) + (c0 + c) * ldb] = X[r][c];
+    }
+}
+
+// General B := L^{-1} B (m x n B); for the inverse, B starts as identity and
+// only columns < rb+nb can be non-zero in the rows being solved.
+static void trsm_llnn_impl(hipStream_t s, idx_t m, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb, bool identity_rhs) {
+    for (idx_t rb = 0; rb < m; rb += PB) {
+        int nb = (int)std::min<idx_t>(PB, m - rb);
+        idx_t ncols = identity_rhs ? std::min<idx_t>(n, rb + nb) : n;
+        {
+            StatScope st("trsm", s, (double)ncols * nb * nb);
+            k_trsm_llnn_diag<<<cdiv(ncols, 32), 256, 0, s>>>(rb, nb, ncols, L, lda, B, ldb);
+        }
+        idx_t below = m - rb - nb;
+        if (below > 0) {
+            // B[below, cols] += (-B[rb block, cols]) * L[below, rb block]: b_ij - b_kj*a_ik, k ascending
+            gemm(s, 'N', below, ncols, nb, mk(-1.0), L + (rb + nb) + rb * lda, lda, B + rb, ldb, mk(1.0), B + (rb + nb), ldb);
+        }
+    }
+}
+
+void inv_lower(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb) {
+    if (n <= 0) return;
+    k_set_identity<<<dim3(cdiv(n, 256), (unsigned)n), 256, 0, s>>>(n, mk(1.0), B, ldb);
+    trsm_llnn_impl(s, n, n, L, lda, B, ldb, true);
+}
+
+void trsm_llnn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, dd *B, idx_t ldb) {
+    if (m <= 0 || n <= 0) return;
+    trsm_llnn_impl(s, m, n, A, lda, B, ldb, false);
+}
+
+// B := B * inv(A^T) for lower A (Rtrsm 'R','L','T','N', alpha = 1): the same
+// panel kernels as the Cholesky, with the reciprocals one/a_kk.
+__global__ void k_recips(idx_t n, const dd *A, idx_t lda, dd *r) {
+    idx_t k = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) r[k] = dd_div(mk(1.0), A[k + k * lda]);
+}
+// Panel solve for a general m x n right-hand side, columns [K, K+nb).
+__global__ __launch_bounds__(256) void k_trsm_rltn_panel(idx_t m, idx_t K, int nb, const dd *A, idx_t lda, dd *B, idx_t ldb,
+                                                         const dd *rinv) {
+    __shared__ dd X[32][PB + 1];
+    const int tid = threadIdx.x;
+    const idx_t rb = (idx_t)blockIdx.x * 32;
+    const int nr = (int)((m - rb) < 32 ? (m - rb) : 32);
+    for (int e = tid; e < 32 * nb; e += 256) {
+        int r = e % 32, j = e / 32;
+        if (r < nr) X[r][j] = B[(rb + r) + (K + j) * ldb];
+    }
+    __syncthreads();
+    for (int k = 0; k < nb; k++) {
+        if (tid < nr) X[tid][k] = dd_mul(rinv[K + k], X[tid][k]);
+        __syncthreads();
+        const int rem = nb - k - 1;
+        for (int e = tid; e < nr * rem; e += 256) {
+            int r = e % nr, j = k + 1 + e / nr;
+            dd ljk = A[(K + j) + (K + k) * lda];
+            if (!dd_is_zero(ljk)) X[r][j] = dd_sub(X[r][j], dd_mul(ljk, X[r][k]));
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < 32 * nb; e += 256) {
+        int r = e % 32, j = e / 32;
+        if (r < nr) B[(rb + r) + (K + j) * ldb] = X[r][j];
+    }
+}
+void trsm_rltn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, dd *B, idx_t ldb) {
+    if (m <= 0 || n <= 0) return;
+    dd *rinv = nullptr;
+    hipMallocAsync((void **)&rinv, sizeof(dd) * n, s);
+    k_recips<<<cdiv(n, 256), 256, 0, s>>>(n, A, lda, rinv);
+    for (idx_t K = 0; K < n; K += PB) {
+        int nb = (int)std::min<idx_t>(PB, n - K);
+        // columns left of the panel were finished earlier: B[:, panel] += (-B[:, done]) * A[panel, done]^T
+        if (K > 0) gemm(s, 'T', m, nb, K, mk(-1.0), B, ldb, A + K, lda, mk(1.0), B + K * ldb, ldb);
+        k_trsm_rltn_panel<<<cdiv(m, 32), 256, 0, s>>>(m, K, nb, A, lda, B, ldb, rinv);
+    }
+    hipFreeAsync(rinv, s);
+}
+
+// ------------------------------------------------------------------ TRMM --
+// C[i,j] = b_ij*a_ii + sum_{k>i} a_ki*b_kj (k ascending) -- Rtrmm 'L','L','T','N'.
+// When B is lower triangular too, terms with k < j vanish and are skipped.
+__global__ __launch_bounds__(NT) void k_trmm_lltn(idx_t m, idx_t n, const dd *__restrict__ A, idx_t lda,
+                                                 const dd *__restrict__ B, idx_t ldb, dd *__restrict__ C, idx_t ldc,
+                                                 int b_lower) {
+    __shared__ dd As[TK][TB + 1];
+    __shared__ dd Bs[TK][TB + 1];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const idx_t row0 = (idx_t)blockIdx.x * TB, col0 = (idx_t)blockIdx.y * TB;
+    dd acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            idx_t i = row0 + tx + 16 * r, j = col0 + ty + 16 * c;
+            acc[r][c] = (i < m && j < n) ? dd_mul(B[i + j * ldb], A[i + i * lda]) : mk(0.0);
+        }
+    idx_t kbeg = row0 + 1;
+    if (b_lower && col0 > kbeg) kbeg = col0;
+    for (idx_t k0 = kbeg; k0 < m; k0 += TK) {
+        const int kc = (int)((m - k0) < TK ? (m - k0) : TK);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int e = tid + NT * q;
+            int kk = e & 15, rr = e >> 4;
+            idx_t i = row0 + rr, j = col0 + rr;
+            As[kk][rr] = (i < m && kk < kc) ? A[(k0 + kk) + i * lda] : mk(0.0);
+            Bs[kk][rr] = (j < n && kk < kc) ? B[(k0 + kk) + j * ldb] : mk(0.0);
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; kk++) {
+            idx_t k = k0 + kk;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                idx_t i = row0 + tx + 16 * r;
+                if (k <= i) continue;
+                dd a = As[kk][tx + 16 * r];
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[r][c] = dd_add_eq(acc[r][c], dd_mul(a, Bs[kk][ty + 16 * c]));
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            idx_t i = row0 + tx + 16 * r, j = col0 + ty + 16 * c;
+            if (i < m && j < n) C[i + j * ldc] = acc[r][c];
+        }
+}
+
+void trmm_lltn_self(hipStream_t s, idx_t n, const dd *A, idx_t lda, dd *C, idx_t ldc) {
+    if (n <= 0) return;
+    StatScope st("trmm", s, (double)n * n * n / 3.0);
+    k_trmm_lltn<<<dim3(cdiv(n, TB), cdiv(n, TB)), NT, 0, s>>>(n, n, A, lda, A, lda, C, ldc, 1);
+}
+void trmm_lltn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *B, idx_t ldb, dd *C, idx_t ldc) {
+    if (m <= 0 || n <= 0) return;
+    k_trmm_lltn<<<dim3(cdiv(m, TB), cdiv(n, TB)), NT, 0, s>>>(m, n, A, lda, B, ldb, C, ldc, 0);
+}
+
+// SYRK lower/no-transpose: C := alpha*A*A^T + beta*C on the lower triangle.
+void syrk_ln(hipStream_t s, idx_t n, idx_t k, dd alpha, const dd *A, idx_t lda, dd beta, dd *C, idx_t ldc) {
+    gemm(s, 'T', n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1, 0, 0, 0, true, 0);
+}
+
+// ---------------------------------------------------------------- mat-vec --
+// w_j = x_j*L_jj + sum_{i>j} L_ij*x_i, i ascending (Rtrmv 'L','T').
+__global__ void k_trmv_lt(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    idx_t j = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    dd t = dd_mul(x[j], L[j + j * lda]);
+    const dd *col = L + j * lda;
+    for (idx_t i = j + 1; i < n; i++) t = dd_add_eq(t, dd_mul(col[i], x[i]));
+    w[j] = t;
+}
+// w_i = x_i*L_ii, then += x_j*L_ij for j = i-1 down to 0, skipping x_j == 0 (Rtrmv 'L','N').
+__global__ void k_trmv_ln(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dd xi = x[i];
+    dd t = dd_is_zero(xi) ? xi : dd_mul(xi, L[i + i * lda]);
+    for (idx_t j = i - 1; j >= 0; j--) {
+        dd xj = x[j];
+        if (!dd_is_zero(xj)) t = dd_add_eq(t, dd_mul(xj, L[i + j * lda]));
+    }
+    w[i] = t;
+}
+void trmv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, const dd *x, dd *out) {
+    if (n <= 0) return;
+    StatScope st("mv", s, (double)n * n);
+    if (trans == 'T' || trans == 't')
+        k_trmv_lt<<<cdiv(n, 64), 64, 0, s>>>(n, L, lda, x, out);
+    else
+        k_trmv_ln<<<cdiv(n, 64), 64, 0, s>>>(n, L, lda, x, out);
+}
+// y_i = (beta handling) then += (alpha*x_j)*A_ij, j ascending (Rgemv 'N').
+__global__ void k_gemv_n(idx_t m, idx_t n, dd alpha, int ak, const dd *A, idx_t lda, const dd *x, dd beta, int bk, dd *y) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    dd t = (bk == 0) ? mk(0.0) : (bk == 1 ? y[i] : dd_mul(beta, y[i]));
+    for (idx_t j = 0; j < n; j++) {
+        dd xj = x[j];
+        dd tmp = (ak == 1) ? xj : (ak == 2 ? dd_neg(xj) : dd_mul(alpha, xj));
+        t = dd_add_eq(t, dd_mul(tmp, A[i + j * lda]));
+    }
+    y[i] = t;
+}
+void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y) {
+    if (m <= 0) return;
+    int ak = kind_alpha(alpha), bk = kind_beta(beta);
+    bool az = (alpha.hi == 0.0 && alpha.lo == 0.0);
+    if (n == 0 || (az && bk == 1)) return;
+    if (az) {
+        k_gemm_scale_only<<<dim3(cdiv(m, 256), 1, 1), 256, 0, s>>>(m, 1, beta, bk, y, m, 0);
+        return;
+    }
+    StatScope st("mv", s, 2.0 * (double)m * n);
+    k_gemv_n<<<cdiv(m, 64), 64, 0, s>>>(m, n, alpha, ak, A, lda, x, beta, bk, y);
+}
+
+// ----------------------------------------------------------- elementwise --
+__global__ void k_fill(idx_t len, dd v, dd *x) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) x[i] = v;
+}
+__global__ void k_copy(idx_t len, const dd *x, dd *y) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) y[i] = x[i];
+}
+__global__ void k_scal(idx_t len, dd a, const dd *pa, dd *x) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pa) a = *pa;
+    if (i < len) x[i] = dd_mul(a, x[i]);
+}
+__global__ void k_axpy(idx_t len, dd a, const dd *pa, int negate, const dd *x, dd *y) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pa) a = negate ? dd_neg(*pa) : *pa;
+    if (dd_is_zero(a)) return;  // Raxpy: da == 0 returns
+    if (i < len) y[i] = dd_add_eq(y[i], dd_mul(a, x[i]));
+}
+__global__ void k_zero_upper(idx_t n, dd *A) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < j && i < n) A[i + j * n] = mk(0.0);
+}
+// lower(i,j) = 0.5*(lower + 1*upper); upper = lower  (Lal::getSymmetrize)
+__global__ void k_symmetrize(idx_t n, dd *A) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= n || i <= j) return;
+    dd v = dd_add_eq(A[i + j * n], dd_mul(mk(1.0), A[j + i * n]));
+    v = dd_mul(mk(0.5), v);
+    A[i + j * n] = v;
+    A[j + i * n] = v;
+}
+
+void fill(hipStream_t s, idx_t len, dd v, dd *x) {
+    if (len > 0) k_fill<<<cdiv(len, 256), 256, 0, s>>>(len, v, x);
+}
+void copy(hipStream_t s, idx_t len, const dd *x, dd *y) {
+    if (len > 0 && x != y) k_copy<<<cdiv(len, 256), 256, 0, s>>>(len, x, y);
+}
+void scal(hipStream_t s, idx_t len, dd a, dd *x) {
+    if (len > 0) k_scal<<<cdiv(len, 256), 256, 0, s>>>(len, a, nullptr, x);
+}
+void scal_dev(hipStream_t s, idx_t len, const dd *a, dd *x) {
+    if (len > 0) k_scal<<<cdiv(len, 256), 256, 0, s>>>(len, mk(0.0), a, x);
+}
+void axpy(hipStream_t s, idx_t len, dd a, const dd *x, dd *y) {
+    if (len > 0 && !(a.hi == 0.0 && a.lo == 0.0)) k_axpy<<<cdiv(len, 256), 256, 0, s>>>(len, a, nullptr, 0, x, y);
+}
+void axpy_dev(hipStream_t s, idx_t len, const dd *a, bool negate, const dd *x, dd *y) {
+    if (len > 0) k_axpy<<<cdiv(len, 256), 256, 0, s>>>(len, mk(0.0), a, negate ? 1 : 0, x, y);
+}
+void set_identity(hipStream_t s, idx_t n, dd v, dd *A) {
+    if (n > 0) k_set_identity<<<dim3(cdiv(n, 256), (unsigned)n), 256, 0, s>>>(n, v, A, n);
+}
+void zero_upper(hipStream_t s, idx_t n, dd *A) {
+    if (n > 0) k_zero_upper<<<dim3(cdiv(n, 256), (unsigned)n), 256, 0, s>>>(n, A);
+}
+void symmetrize(hipStream_t s, idx_t n, dd *A) {
+    if (n > 1) k_symmetrize<<<dim3(cdiv(n, 256), (unsigned)n), 256, 0, s>>>(n, A);
+}
+
+// ------------------------------------------------------------------ dots --
+// Chunk t of the static partition of [0,n) into T chunks.
+__device__ __forceinline__ void chunk_range(idx_t n, int T, int t, idx_t &start, idx_t &cnt) {
+    idx_t q = n / T, r = n % T;
+    cnt = q + (t < r ? 1 : 0);
+    start = (idx_t)t * q + (t < r ? t : r);
+}
+__device__ __forceinline__ dd chunk_sum(const dd *x, const dd *y, idx_t start, idx_t cnt) {
+    dd s = mk(0.0);
+    for (idx_t i = start; i < start + cnt; i++) s = dd_add_eq(s, dd_mul(x[i], y[i]));
+    return s;
+}
+
+// One dot over T chunks spread across workgroups; the last workgroup to
+// finish adds the partial sums in chunk order.
+__global__ __launch_bounds__(256) void k_dot_big(idx_t n, const dd *x, const dd *y, dd *out, int T, dd *partial,
+                                                 unsigned *counter) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < T) {
+        idx_t st, cnt;
+        chunk_range(n, T, t, st, cnt);
+        partial[t] = chunk_sum(x, y, st, cnt);
+    }
+    __threadfence();
+    __shared__ unsigned ticket;
+    __syncthreads();
+    if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
+    __syncthreads();
+    if (ticket != gridDim.x - 1) return;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        dd acc = mk(0.0);
+        for (int u = 0; u < T; u++) acc = dd_add_eq(acc, partial[u]);
+        *out = acc;
+        *counter = 0u;
+    }
+}
+
+void dot(hipStream_t s, idx_t n, const dd *x, const dd *y, dd *out, int T, dd *work, unsigned *counter) {
+    StatScope st("dot", s, 2.0 * (double)n);
+    k_dot_big<<<cdiv(T, 256), 256, 0, s>>>(n, x, y, out, T, work, counter);
+}
+
+// Batched dots: one workgroup per task, partial sums in LDS (T <= 4096).
+__global__ __launch_bounds__(256) void k_dot_batch(const DotTask *tasks, int T) {
+    extern __shared__ dd part[];
+    const DotTask tk = tasks[blockIdx.x];
+    for (int t = threadIdx.x; t < T; t += 256) {
+        idx_t st, cnt;
+        chunk_range(tk.n, T, t, st, cnt);
+        part[t] = chunk_sum(tk.x, tk.y, st, cnt);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        dd acc = mk(0.0);
+        for (int t = 0; t < T; t++) acc = dd_add_eq(acc, part[t]);
+        *tk.out = acc;
+    }
+}
+void dot_batch(hipStream_t s, const DotTask *d_tasks, int ntasks, int T) {
+    if (ntasks > 0) k_dot_batch<<<ntasks, 256, sizeof(dd) * T, s>>>(d_tasks, T);
+}
+__global__ __launch_bounds__(256) void k_dot_batch_off(const DotTaskOff *tasks, const dd *ybase, int T) {
+    extern __shared__ dd part[];
+    const DotTaskOff tk = tasks[blockIdx.x];
+    const dd *y = ybase + tk.yoff;
+    for (int t = threadIdx.x; t < T; t += 256) {
+        idx_t st, cnt;
+        chunk_range(tk.n, T, t, st, cnt);
+        part[t] = chunk_sum(tk.x, y, st, cnt);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        dd acc = mk(0.0);
+        for (int t = 0; t < T; t++) acc = dd_add_eq(acc, part[t]);
+        *tk.out = acc;
+    }
+}
+void dot_batch_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T) {
+    if (ntasks > 0) k_dot_batch_off<<<ntasks, 256, sizeof(dd) * T, s>>>(d_tasks, ybase, T);
+}
+
+// Max of |x_i|: per-thread running max, tree merge, last workgroup merges.
+__global__ __launch_bounds__(256) void k_max_abs(idx_t len, const dd *x, dd *out, dd *partial, unsigned *counter) {
+    __shared__ dd sm[256];
+    dd best = mk(0.0);
+    for (idx_t i = (idx_t)blockIdx.x * 256 + threadIdx.x; i < len; i += (idx_t)gridDim.x * 256) {
+        dd t = dd_abs(x[i]);
+        if (dd_gt(t, best)) best = t;
+    }
+    sm[threadIdx.x] = best;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w && dd_gt(sm[threadIdx.x + w], sm[threadIdx.x])) sm[threadIdx.x] = sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+    __threadfence();
+    __shared__ unsigned ticket;
+    if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
+    __syncthreads();
+    if (ticket != gridDim.x - 1) return;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        dd r = mk(0.0);
+        for (unsigned b = 0; b < gridDim.x; b++)
+            if (dd_gt(partial[b], r)) r = partial[b];
+        *out = r;
+        *counter = 0u;
+    }
+}
+void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned *counter) {
+    unsigned g = std::min<unsigned>(cdiv(len, 256), 256u);
+    if (g == 0) g = 1;
+    k_max_abs<<<g, 256, 0, s>>>(len, x, out, work, counter);
+}
+
+}  // namespace sdk

@@ -1,0 +1,99 @@
+// Internal launcher API for the dd kernels (host side).  All pointers are
+// device pointers to dd values (16 bytes, {hi, lo}), matrices column-major.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "dd.hpp"
+
+namespace sdk {
+
+typedef int64_t idx_t;
+
+// C := alpha*op(A)*op(B) + beta*C with op(B) = B ('N') or B**T ('T'); A is
+// never transposed.  Batched with element strides sA, sB, sC over `batch`.
+// Each C element accumulates its k terms in ascending order as
+// c += (alpha*b_lj) * a_il, the order of Rgemm_NN/NT.
+void gemm(hipStream_t s, char tb, idx_t m, idx_t n, idx_t k, dd alpha, const dd *A, idx_t lda, const dd *B, idx_t ldb,
+          dd beta, dd *C, idx_t ldc, int batch = 1, idx_t sA = 0, idx_t sB = 0, idx_t sC = 0,
+          bool lower_only = false, idx_t diag_offset = 0);
+
+// Lower Cholesky in place.  info (device int64) receives 0 or the 1-based
+// index of the first non-positive pivot.  scratch: >= n dd of device memory.
+void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch);
+
+// x := L^{-1} x ('N') or L^{-T} x ('T'), L lower non-unit.
+void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x);
+
+// B := L^{-1} (B enters as anything; set to identity then solved like
+// Rtrsm('L','L','N','N') on the identity).
+void inv_lower(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb);
+
+// C := A**T * A for lower triangular A (Rtrmm('L','L','T','N') applied to a
+// copy of A); C full n x n.
+void trmm_lltn_self(hipStream_t s, idx_t n, const dd *A, idx_t lda, dd *C, idx_t ldc);
+
+// General Rtrsm('R','L','T','N', alpha = 1) and Rtrsm('L','L','N','N', alpha = 1)
+// and Rtrmm('L','L','T','N', alpha = 1) on arbitrary operands.
+void trsm_rltn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, dd *B, idx_t ldb);
+void trsm_llnn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, dd *B, idx_t ldb);
+void trmm_lltn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *B, idx_t ldb, dd *C, idx_t ldc);
+void syrk_ln(hipStream_t s, idx_t n, idx_t k, dd alpha, const dd *A, idx_t lda, dd beta, dd *C, idx_t ldc);
+
+// x := L**T x ('T') or L x ('N'), Rtrmv lower non-unit.  out may differ from x.
+void trmv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, const dd *x, dd *out);
+// y := alpha*A*x (+ beta*y), Rgemv 'N'.
+void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y);
+
+// Elementwise (length len, flat).
+void fill(hipStream_t s, idx_t len, dd v, dd *x);
+void copy(hipStream_t s, idx_t len, const dd *x, dd *y);
+void scal(hipStream_t s, idx_t len, dd a, dd *x);                    // x = a*x
+void scal_dev(hipStream_t s, idx_t len, const dd *a, dd *x);         // x = (*a)*x, scalar on device
+void axpy(hipStream_t s, idx_t len, dd a, const dd *x, dd *y);        // y += a*x (skipped if a == 0)
+void axpy_dev(hipStream_t s, idx_t len, const dd *a, bool negate, const dd *x, dd *y);
+void set_identity(hipStream_t s, idx_t n, dd v, dd *A);
+void zero_upper(hipStream_t s, idx_t n, dd *A);
+void symmetrize(hipStream_t s, idx_t n, dd *A);
+
+// Dot products with the static T-way partition (Rdot_omp order).
+// Result written to *out (device).  work: >= T dd + counter space.
+struct DotTask {
+    const dd *x;
+    const dd *y;
+    idx_t n;
+    dd *out;
+};
+void dot(hipStream_t s, idx_t n, const dd *x, const dd *y, dd *out, int T, dd *work, unsigned *counter);
+// Many independent dots, one workgroup each (tasks on device).
+void dot_batch(hipStream_t s, const DotTask *d_tasks, int ntasks, int T);
+// Same with y = ybase + yoff per task.
+struct DotTaskOff {
+    const dd *x;
+    idx_t yoff;
+    idx_t n;
+    dd *out;
+};
+void dot_batch_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T);
+// Batched GEMM over pointer arrays (device arrays of batch pointers):
+// C[b] := A[b]*B[b] (alpha = 1, beta = 0), order as gemm().
+void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx_t lda, const dd *const *B, idx_t ldb,
+               dd *const *C, idx_t ldc, int batch);
+// Sequential strided dot (Rdot with non-unit stride): one thread per task.
+// Max |x_i| over len values into *out.
+void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned *counter);
+
+// Per-kernel-family timing with HIP events on the launch stream (off by
+// default).  Families: "gemm", "potrf", "trsm", "trmm", "trsv", "mv", "dot",
+// "schur", "elementwise".  flops are algorithmic dd operations.
+void stats_enable(bool on);
+void stats_reset();
+bool stats_get(const char *family, int64_t *launches, double *ms, double *ddflops);
+struct StatScope {
+    int slot = -1;
+    hipStream_t s = nullptr;
+    StatScope(const char *family, hipStream_t stream, double ddflops);
+    ~StatScope();
+};
+
+}  // namespace sdk

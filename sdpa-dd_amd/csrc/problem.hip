@@ -1,0 +1,869 @@
+// Problem input (SDPA sparse format reader, param.sdpa), the Schur formula
+// table, the device image of the problem and the kernels that touch the
+// constraint matrices: A_i . W inner products, sum_i y_i A_i, and the Schur
+// complement assembly with formulas F1/F2/F3.
+#include "problem.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace sdp {
+
+using sdk::DotTask;
+
+// ------------------------------------------------------------------ input --
+// QD's decimal reader (qd/src/dd_real.cpp read()): r = r*10 + digit in dd,
+// then r *= 10^e with binary powering.
+static dd npwr10(int n) {
+    dd r = mk(10.0), s = mk(1.0);
+    int N = n < 0 ? -n : n;
+    if (N > 1) {
+        while (N > 0) {
+            if (N % 2 == 1) s = dd_mul_eq(s, r);
+            N /= 2;
+            if (N > 0) r = dd_sqr(r);
+        }
+    } else {
+        s = r;
+    }
+    if (n < 0) return dd_div(mk(1.0), s);
+    return s;
+}
+bool qd_read(const char *s, dd &out) {
+    const char *p = s;
+    int sign = 0, point = -1, nd = 0, e = 0;
+    bool done = false;
+    dd r = mk(0.0);
+    while (*p == ' ') p++;
+    char ch;
+    while (!done && (ch = *p) != '\0') {
+        if (ch >= '0' && ch <= '9') {
+            r = dd_mul_d(r, 10.0);
+            r = dd_add_d(r, (double)(ch - '0'));
+            nd++;
+        } else if (ch == '.') {
+            if (point >= 0) return false;
+            point = nd;
+        } else if (ch == '-' || ch == '+') {
+            if (sign != 0 || nd > 0) return false;
+            sign = (ch == '-') ? -1 : 1;
+        } else if (ch == 'e' || ch == 'E') {
+            if (std::sscanf(p + 1, "%d", &e) != 1) return false;
+            done = true;
+        } else {
+            return false;
+        }
+        p++;
+    }
+    if (point >= 0) e -= (nd - point);
+    if (e != 0) r = dd_mul_eq(r, npwr10(e));
+    out = (sign == -1) ? dd_neg(r) : r;
+    return true;
+}
+
+void Block::change_to_dense() {
+    if (dense) return;
+    if (nnz_count < (n * n) * 0.20) return;  // sdpa_struct.cpp:465
+    dense = true;
+    dval.assign((size_t)n * n, mk(0.0));
+    for (int k = 0; k < nnz_count; k++) {
+        int i = row[k], j = col[k];
+        dval[i + (size_t)n * j] = val[k];
+        if (i != j) dval[j + (size_t)n * i] = val[k];
+    }
+    nnz_count = nnz_effect = n * n;
+    row.clear();
+    col.clear();
+    val.clear();
+}
+
+namespace {
+bool is_numch(char c) { return (c >= '0' && c <= '9') || c == '+' || c == '-'; }
+struct Cursor {
+    const std::string &s;
+    size_t p = 0;
+    explicit Cursor(const std::string &x) : s(x) {}
+    // fscanf "%*[^0-9+-]": consume one or more characters outside the set
+    bool sep() {
+        size_t q = p;
+        while (q < s.size() && !is_numch(s[q])) q++;
+        if (q == p || q >= s.size()) return false;
+        p = q;
+        return true;
+    }
+    bool integer(int &v) {
+        while (p < s.size() && std::isspace((unsigned char)s[p])) p++;
+        size_t q = p;
+        if (q < s.size() && (s[q] == '+' || s[q] == '-')) q++;
+        size_t d = q;
+        while (q < s.size() && std::isdigit((unsigned char)s[q])) q++;
+        if (q == d) return false;
+        v = std::atoi(s.substr(p, q - p).c_str());
+        p = q;
+        return true;
+    }
+    // "%[^,} \t\n]"
+    bool token(std::string &t) {
+        size_t q = p;
+        while (q < s.size() && s[q] != ',' && s[q] != '}' && s[q] != ' ' && s[q] != '\t' && s[q] != '\n') q++;
+        if (q == p) return false;
+        t = s.substr(p, q - p);
+        p = q;
+        return true;
+    }
+};
+dd value_of(const std::string &t) {
+    dd v;
+    if (!qd_read(t.c_str(), v)) v = dd{NAN, NAN};
+    return v;
+}
+}  // namespace
+
+bool read_dat_s(const std::string &path, HostProblem &P, std::string &err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        err = "cannot open data file " + path;
+        return false;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string buf = ss.str();
+    Cursor c(buf);
+    int m = 0;
+    for (;;) {  // sdpa_io.cpp:124-135: skip '*' and '"' comment lines
+        if (c.p >= buf.size()) {
+            err = "missing mDIM";
+            return false;
+        }
+        size_t e = buf.find('\n', c.p);
+        size_t end = (e == std::string::npos) ? buf.size() : e + 1;
+        std::string line = buf.substr(c.p, end - c.p);
+        c.p = end;
+        if (!line.empty() && (line[0] == '*' || line[0] == '"')) continue;
+        if (std::sscanf(line.c_str(), "%d", &m) != 1 || m <= 0) {
+            err = "bad mDIM";
+            return false;
+        }
+        break;
+    }
+    int nb = 0;
+    if (!c.integer(nb) || nb <= 0) {
+        err = "bad nBLOCK";
+        return false;
+    }
+    std::vector<int> bs(nb);
+    for (int l = 0; l < nb; l++) {
+        if (!c.sep() || !c.integer(bs[l])) {
+            err = "bad bLOCKsTRUCT";
+            return false;
+        }
+        if (bs[l] < 2) {
+            err = "LP blocks are not supported by this path";
+            return false;
+        }
+        if (bs[l] <= 20) {
+            err = "SDP blocks of size <= 20 (eigenvalue path) are not supported by this path";
+            return false;
+        }
+    }
+    P.m = m;
+    P.bs = bs;
+    P.b.assign(m, mk(0.0));
+    for (int k = 0; k < m; k++) {
+        std::string t;
+        if (!c.sep() || !c.token(t)) {
+            err = "bad b vector";
+            return false;
+        }
+        P.b[k] = value_of(t);
+    }
+    struct E {
+        int k, l, i, j;
+        dd v;
+    };
+    std::vector<E> ent;
+    for (;;) {
+        E e;
+        std::string t;
+        if (!c.sep() || !c.integer(e.k)) break;
+        if (!c.sep() || !c.integer(e.l)) break;
+        if (!c.sep() || !c.integer(e.i)) break;
+        if (!c.sep() || !c.integer(e.j)) break;
+        if (!c.sep() || !c.token(t)) break;
+        double chk;
+        if (std::sscanf(t.c_str(), "%lf", &chk) <= 0) break;
+        e.v = value_of(t);
+        if (e.k < 0 || e.k > m || e.l < 1 || e.l > nb || e.i < 1 || e.j < 1 || e.i > bs[e.l - 1] || e.j > bs[e.l - 1]) {
+            err = "entry out of range in input data";
+            return false;
+        }
+        ent.push_back(e);
+    }
+    std::vector<int> cnt((size_t)(m + 1) * nb, 0);
+    for (auto &e : ent) cnt[(size_t)e.k * nb + e.l - 1]++;
+    auto init = [&](Space &sp, int k) {
+        for (int l = 0; l < nb; l++) {
+            int c0 = cnt[(size_t)k * nb + l];
+            if (!c0) continue;
+            sp.index.push_back(l);
+            Block b;
+            b.n = bs[l];
+            b.row.reserve(c0);
+            b.col.reserve(c0);
+            b.val.reserve(c0);
+            sp.blk.push_back(std::move(b));
+        }
+    };
+    init(P.C, 0);
+    P.A.assign(m, Space());
+    for (int k = 0; k < m; k++) init(P.A[k], k + 1);
+    for (auto &e : ent) {
+        Space &sp = e.k ? P.A[e.k - 1] : P.C;
+        Block &b = sp.blk[sp.find(e.l - 1)];
+        b.row.push_back(e.i - 1);
+        b.col.push_back(e.j - 1);
+        b.val.push_back(e.k ? e.v : dd_neg(e.v));  // C = -F0 (sdpa_io.cpp:692)
+        b.nnz_count++;
+        b.nnz_effect += (e.i == e.j) ? 1 : 2;
+    }
+    for (auto &b : P.C.blk) b.change_to_dense();
+    for (auto &a : P.A)
+        for (auto &b : a.blk) b.change_to_dense();
+    P.layout();
+    P.index();
+    P.formula = compute_formula(P, ddr(1.2));
+    return true;
+}
+
+void HostProblem::layout() {
+    boff.assign(bs.size(), 0);
+    nn = 0;
+    for (size_t l = 0; l < bs.size(); l++) {
+        boff[l] = nn;
+        nn += (idx_t)bs[l] * bs[l];
+    }
+}
+void HostProblem::index() {
+    cons.assign(bs.size(), {});
+    cidx.assign(bs.size(), {});
+    for (int i = 0; i < m; i++)
+        for (size_t k = 0; k < A[i].index.size(); k++) {
+            cons[A[i].index[k]].push_back(i);
+            cidx[A[i].index[k]].push_back((int)k);
+        }
+}
+
+bool read_params(const std::string &path, Params &p) {
+    FILE *fp = std::fopen(path.c_str(), "r");
+    if (!fp) return false;
+    int r = 0;
+    r += std::fscanf(fp, "%d%*[^\n]", &p.maxIteration);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.epsilonStar);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.lambdaStar);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.omegaStar);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.lowerBound);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.upperBound);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.betaStar);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.betaBar);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.gammaStar);
+    r += std::fscanf(fp, "%lf%*[^\n]", &p.epsilonDash);
+    std::fclose(fp);
+    return r == 10;
+}
+
+// Newton::computeFormula_SDP (sdpa_newton.cpp:694-819), evaluated in dd.
+std::vector<int> compute_formula(const HostProblem &P, ddr Kappa) {
+    const int m = P.m, nB = (int)P.bs.size();
+    std::vector<int> up((size_t)m * nB, 0), f((size_t)m * nB, 0);
+    for (int l = 0; l < nB; l++) {
+        const auto &cs = P.cons[l];
+        const auto &ci = P.cidx[l];
+        for (size_t k1 = 0; k1 < cs.size(); k1++) {
+            int i = cs[k1], inz = P.A[i].blk[ci[k1]].nnz_effect, u = inz;
+            for (size_t k2 = 0; k2 < cs.size(); k2++) {
+                int j = cs[k2], jnz = P.A[j].blk[ci[k2]].nnz_effect;
+                if (jnz < inz || (jnz == inz && j < i)) u += jnz;
+            }
+            up[(size_t)i * nB + l] = u;
+        }
+    }
+    for (int l = 0; l < nB; l++) {
+        const auto &cs = P.cons[l];
+        const auto &ci = P.cidx[l];
+        for (size_t k = 0; k < cs.size(); k++) {
+            int i = cs[k];
+            const Block &Ai = P.A[i].blk[ci[k]];
+            ddr inz = Ai.nnz_effect, n = Ai.n, u = up[(size_t)i * nB + l];
+            ddr f1 = Kappa * n * inz + n * n * n + Kappa * u;
+            ddr f2 = Kappa * n * inz + Kappa * (n + 1) * u;
+            ddr f3 = Kappa * (2 * Kappa * inz + 1) * u / Kappa;
+            int r;
+            if (Ai.dense)
+                r = (f1 < f2) ? 0 : 1;
+            else if (f1 < f2 && f1 < f3)
+                r = 0;
+            else if (f2 < f3)
+                r = 1;
+            else
+                r = 2;
+            f[(size_t)i * nB + l] = r;
+        }
+    }
+    return f;
+}
+
+// ----------------------------------------------------------------- upload --
+template <typename T>
+static T *dev_copy(DeviceProblem &dp, const std::vector<T> &h, hipStream_t s) {
+    if (h.empty()) return nullptr;
+    T *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(T) * h.size()) != hipSuccess) return nullptr;
+    dp.allocs.push_back(d);
+    (void)s;
+    if (hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return d;
+}
+template <typename T>
+static T *dev_alloc(DeviceProblem &dp, size_t n) {
+    if (n == 0) return nullptr;
+    T *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(T) * n) != hipSuccess) return nullptr;
+    dp.allocs.push_back(d);
+    return d;
+}
+
+int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::string &err) {
+    const int m = P.m, nB = (int)P.bs.size();
+    dp.m = m;
+    dp.bs = P.bs;
+    dp.boff = P.boff;
+    dp.nn = P.nn;
+    dp.d_b = dev_copy(dp, P.b, s);
+    // Shared sparse-entry and dense-value pools.
+    std::vector<int> sp_row, sp_col;
+    std::vector<dd> sp_val, dense;
+    auto add_block = [&](const Block &b, int &type, int &nnz, idx_t &off) {
+        if (b.dense) {
+            type = 1;
+            nnz = b.n * b.n;
+            off = (idx_t)dense.size();
+            dense.insert(dense.end(), b.dval.begin(), b.dval.end());
+        } else {
+            type = 0;
+            nnz = b.nnz_count;
+            off = (idx_t)sp_row.size();
+            sp_row.insert(sp_row.end(), b.row.begin(), b.row.end());
+            sp_col.insert(sp_col.end(), b.col.begin(), b.col.end());
+            sp_val.insert(sp_val.end(), b.val.begin(), b.val.end());
+        }
+    };
+    // Inner-product tasks: constraint i, its blocks in SDP_sp_index order.
+    std::vector<idx_t> tptr(m + 1, 0);
+    std::vector<int> ttype, tblock, tnnz;
+    std::vector<idx_t> toff;
+    // Remember each (i, block-slot) pool location for the Schur data.
+    std::vector<std::vector<std::pair<int, idx_t>>> loc(m);
+    for (int i = 0; i < m; i++) {
+        tptr[i] = (idx_t)ttype.size();
+        for (size_t k = 0; k < P.A[i].index.size(); k++) {
+            int type, nnz;
+            idx_t off;
+            add_block(P.A[i].blk[k], type, nnz, off);
+            ttype.push_back(type);
+            tblock.push_back(P.A[i].index[k]);
+            tnnz.push_back(nnz);
+            toff.push_back(off);
+            loc[i].push_back({type, off});
+        }
+    }
+    tptr[m] = (idx_t)ttype.size();
+    std::vector<idx_t> cptr(2, 0);
+    std::vector<int> cty, cbl, cnz;
+    std::vector<idx_t> coff;
+    for (size_t k = 0; k < P.C.index.size(); k++) {
+        int type, nnz;
+        idx_t off;
+        add_block(P.C.blk[k], type, nnz, off);
+        cty.push_back(type);
+        cbl.push_back(P.C.index[k]);
+        cnz.push_back(nnz);
+        coff.push_back(off);
+    }
+    cptr[1] = (idx_t)cty.size();
+    dp.ntask = (int)ttype.size();
+    dp.h_task_type = ttype;
+    dp.d_task_ptr = dev_copy(dp, tptr, s);
+    dp.d_task_type = dev_copy(dp, ttype, s);
+    dp.d_task_block = dev_copy(dp, tblock, s);
+    dp.d_task_nnz = dev_copy(dp, tnnz, s);
+    dp.d_task_off = dev_copy(dp, toff, s);
+    dp.nctask = (int)cty.size();
+    dp.h_ctask_type = cty;
+    dp.d_ctask_ptr = dev_copy(dp, cptr, s);
+    dp.d_ctask_type = dev_copy(dp, cty, s);
+    dp.d_ctask_block = dev_copy(dp, cbl, s);
+    dp.d_ctask_nnz = dev_copy(dp, cnz, s);
+    dp.d_ctask_off = dev_copy(dp, coff, s);
+    dp.d_sp_row = dev_copy(dp, sp_row, s);
+    dp.d_sp_col = dev_copy(dp, sp_col, s);
+    dp.d_sp_val = dev_copy(dp, sp_val, s);
+    dp.d_dense = dev_copy(dp, dense, s);
+
+    // Element update lists for sum_i y_i A_i (multi_plusToA order: i
+    // ascending, entries in file order, off-diagonals to both positions).
+    {
+        std::vector<std::vector<std::pair<int, dd>>> el(P.nn);
+        for (int i = 0; i < m; i++)
+            for (size_t k = 0; k < P.A[i].index.size(); k++) {
+                int l = P.A[i].index[k], n = P.bs[l];
+                idx_t o = P.boff[l];
+                const Block &b = P.A[i].blk[k];
+                if (b.dense) {
+                    for (idx_t e = 0; e < (idx_t)n * n; e++)
+                        if (!dd_is_zero(b.dval[e])) el[o + e].push_back({i, b.dval[e]});
+                } else {
+                    for (int q = 0; q < b.nnz_count; q++) {
+                        int r = b.row[q], cc = b.col[q];
+                        el[o + r + (idx_t)n * cc].push_back({i, b.val[q]});
+                        if (r != cc) el[o + cc + (idx_t)n * r].push_back({i, b.val[q]});
+                    }
+                }
+            }
+        std::vector<idx_t> ptr(P.nn + 1, 0);
+        std::vector<int> idx;
+        std::vector<dd> val;
+        for (idx_t e = 0; e < P.nn; e++) {
+            ptr[e] = (idx_t)idx.size();
+            for (auto &pr : el[e]) {
+                idx.push_back(pr.first);
+                val.push_back(pr.second);
+            }
+        }
+        ptr[P.nn] = (idx_t)idx.size();
+        dp.d_el_ptr = dev_copy(dp, ptr, s);
+        dp.d_el_idx = dev_copy(dp, idx, s);
+        dp.d_el_val = dev_copy(dp, val, s);
+        std::vector<std::vector<dd>> cel(P.nn);
+        for (size_t k = 0; k < P.C.index.size(); k++) {
+            int l = P.C.index[k], n = P.bs[l];
+            idx_t o = P.boff[l];
+            const Block &b = P.C.blk[k];
+            if (b.dense) {
+                for (idx_t e = 0; e < (idx_t)n * n; e++)
+                    if (!dd_is_zero(b.dval[e])) cel[o + e].push_back(b.dval[e]);
+            } else {
+                for (int q = 0; q < b.nnz_count; q++) {
+                    int r = b.row[q], cc = b.col[q];
+                    cel[o + r + (idx_t)n * cc].push_back(b.val[q]);
+                    if (r != cc) cel[o + cc + (idx_t)n * r].push_back(b.val[q]);
+                }
+            }
+        }
+        std::vector<idx_t> cp(P.nn + 1, 0);
+        std::vector<dd> cv;
+        for (idx_t e = 0; e < P.nn; e++) {
+            cp[e] = (idx_t)cv.size();
+            for (auto &v : cel[e]) cv.push_back(v);
+        }
+        cp[P.nn] = (idx_t)cv.size();
+        dp.d_cel_ptr = dev_copy(dp, cp, s);
+        dp.d_cel_val = dev_copy(dp, cv, s);
+    }
+
+    // Dense inner-product tasks (A_i then C), y offset = block offset.
+    {
+        std::vector<sdk::DotTaskOff> dt, dct;
+        dp.d_task_val = dev_alloc<dd>(dp, std::max(1, dp.ntask));
+        dp.d_ctask_val = dev_alloc<dd>(dp, std::max(1, dp.nctask));
+        dd *dense_base = dp.d_dense;
+        for (int t = 0; t < dp.ntask; t++)
+            if (ttype[t] == 1) dt.push_back({dense_base + toff[t], P.boff[tblock[t]], (idx_t)tnnz[t], dp.d_task_val + t});
+        for (int t = 0; t < dp.nctask; t++)
+            if (cty[t] == 1) dct.push_back({dense_base + coff[t], P.boff[cbl[t]], (idx_t)cnz[t], dp.d_ctask_val + t});
+        dp.n_dense_tasks = (int)dt.size();
+        dp.n_dense_ctasks = (int)dct.size();
+        dp.d_dense_tasks = dev_copy(dp, dt, s);
+        dp.d_dense_ctasks = dev_copy(dp, dct, s);
+        dp.d_boff = dev_copy(dp, P.boff, s);
+        dp.d_bsize = dev_copy(dp, P.bs, s);
+    }
+
+    // Schur data per block.
+    size_t work_need = 0;
+    int max_pairs = 0;
+    dp.schur.assign(nB, SchurBlock());
+    struct PendingTasks {
+        int l;
+        std::vector<int> psi, psj;
+    };
+    std::vector<PendingTasks> pending;
+    std::vector<std::vector<int>> wslots(nB);
+    std::vector<std::vector<idx_t>> offs(nB);
+    for (int l = 0; l < nB; l++) {
+        SchurBlock &S = dp.schur[l];
+        const int n = P.bs[l];
+        S.n = n;
+        const auto &cs = P.cons[l];
+        const auto &ci = P.cidx[l];
+        const int ns = (int)cs.size();
+        std::vector<int> type(ns), nnz(ns), eff(ns);
+        std::vector<idx_t> off(ns);
+        std::vector<int> wslot(ns, -1);
+        int nw = 0;
+        for (int k = 0; k < ns; k++) {
+            int i = cs[k];
+            const Block &b = P.A[i].blk[ci[k]];
+            type[k] = b.dense ? 1 : 0;
+            nnz[k] = b.nnz_count;
+            eff[k] = b.nnz_effect;
+            off[k] = loc[i][ci[k]].second;
+            int fm = P.formula[(size_t)i * nB + l];
+            S.ci.push_back(i);
+            S.form.push_back(fm);
+            if (fm == 0)
+                S.slots_f1.push_back(k);
+            else if (fm == 1)
+                S.slots_f2.push_back(k);
+            else
+                S.slots_f3.push_back(k);
+            if (fm != 2) {
+                wslot[k] = nw++;
+                (type[k] ? S.de_slots : S.sp_slots).push_back(k);
+            }
+        }
+        // pairs in the order of compute_bMat_dense_SDP's loops (sdpa_newton.cpp:989-1026)
+        std::vector<int> pi, pj, psi, psj, pf;
+        std::vector<int> need_g(ns, 0);
+        for (int k1 = 0; k1 < ns; k1++)
+            for (int k2 = 0; k2 < ns; k2++) {
+                int i = cs[k1], j = cs[k2];
+                if (eff[k1] < eff[k2] || (eff[k1] == eff[k2] && i < j)) continue;
+                pi.push_back(i);
+                pj.push_back(j);
+                psi.push_back(k1);
+                psj.push_back(k2);
+                pf.push_back(S.form[k1]);
+                if (S.form[k1] == 0 || (S.form[k1] == 1 && type[k2] == 1)) need_g[k1] = 1;
+            }
+        for (int k = 0; k < ns; k++)
+            if (need_g[k] && S.form[k] != 2) S.g_slots.push_back(k);
+        S.npairs = (int)pi.size();
+        max_pairs = std::max(max_pairs, S.npairs);
+        S.h_pair_form = pf;
+        S.d_pair_i = dev_copy(dp, pi, s);
+        S.d_pair_j = dev_copy(dp, pj, s);
+        S.d_pair_slot_i = dev_copy(dp, psi, s);
+        S.d_pair_slot_j = dev_copy(dp, psj, s);
+        S.d_pair_form = dev_copy(dp, pf, s);
+        S.d_type = dev_copy(dp, type, s);
+        S.d_nnz = dev_copy(dp, nnz, s);
+        S.d_off = dev_copy(dp, off, s);
+        S.d_wslot = dev_copy(dp, wslot, s);
+        S.d_sp_slots = dev_copy(dp, S.sp_slots, s);
+        S.d_de_slots = dev_copy(dp, S.de_slots, s);
+        S.d_g_slots = dev_copy(dp, S.g_slots, s);
+        pending.push_back({l, psi, psj});
+        wslots[l] = wslot;
+        offs[l] = off;
+        // row lists of A_i * Zinv for sparse F1/F2 constraints (Raxpy order)
+        std::vector<idx_t> rptr((size_t)ns * (n + 1), 0);
+        std::vector<int> rsrc;
+        std::vector<dd> rval;
+        for (int k = 0; k < ns; k++) {
+            const Block &b = P.A[cs[k]].blk[ci[k]];
+            std::vector<std::vector<std::pair<int, dd>>> rows(n);
+            if (!b.dense && S.form[k] != 2) {
+                for (int q = 0; q < b.nnz_count; q++) {
+                    int r = b.row[q], cc = b.col[q];
+                    if (r != cc) {
+                        rows[r].push_back({cc, b.val[q]});
+                        rows[cc].push_back({r, b.val[q]});
+                    } else {
+                        rows[cc].push_back({cc, b.val[q]});
+                    }
+                }
+            }
+            for (int r = 0; r < n; r++) {
+                rptr[(size_t)k * (n + 1) + r] = (idx_t)rsrc.size();
+                for (auto &pr : rows[r]) {
+                    rsrc.push_back(pr.first);
+                    rval.push_back(pr.second);
+                }
+            }
+            rptr[(size_t)k * (n + 1) + n] = (idx_t)rsrc.size();
+        }
+        S.d_rl_ptr = dev_copy(dp, rptr, s);
+        S.d_rl_src = dev_copy(dp, rsrc, s);
+        S.d_rl_val = dev_copy(dp, rval, s);
+        work_need = std::max(work_need, (size_t)nw * n * n);
+    }
+    dp.work_elems = work_need;
+    dp.d_work1 = dev_alloc<dd>(dp, work_need);
+    dp.d_work2 = dev_alloc<dd>(dp, work_need);
+    dp.d_pair_val = dev_alloc<dd>(dp, std::max(1, max_pairs));
+    // Dot tasks for pairs with a dense A_j, now that work2 has an address.
+    for (auto &pt : pending) {
+        SchurBlock &S = dp.schur[pt.l];
+        const idx_t nsq = (idx_t)S.n * S.n;
+        const auto &cs = P.cons[pt.l];
+        const auto &ci = P.cidx[pt.l];
+        std::vector<sdk::DotTask> tasks;
+        for (int p = 0; p < S.npairs; p++) {
+            int sj = pt.psj[p], si = pt.psi[p];
+            if (!P.A[cs[sj]].blk[ci[sj]].dense) continue;
+            tasks.push_back({dp.d_dense + offs[pt.l][sj], dp.d_work2 + (idx_t)wslots[pt.l][si] * nsq, nsq, dp.d_pair_val + p});
+        }
+        S.ndense_pairs = (int)tasks.size();
+        S.d_pair_tasks = dev_copy(dp, tasks, s);
+    }
+    size_t maxslots = 1;
+    for (auto &S : dp.schur) maxslots = std::max(maxslots, std::max(S.de_slots.size(), S.g_slots.size()));
+    dp.ptr_cap = maxslots;
+    dp.d_ptrs = dev_alloc<char>(dp, 3 * sizeof(void *) * maxslots);
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        err = hipGetErrorString(e);
+        return -1;
+    }
+    return 0;
+}
+
+void free_problem(DeviceProblem &dp) {
+    for (void *p : dp.allocs) hipFree(p);
+    dp.allocs.clear();
+}
+
+// ------------------------------------------------------- inner products --
+// One thread per sparse task: the 4-way grouped sum of getInnerProduct
+// (sdpa_linear.cpp:202-290); dense tasks are left to the chunked dot.
+__device__ dd sparse_inner(const int *row, const int *col, const dd *val, int nnz, const dd *W, int n) {
+    auto term = [&](int q) {
+        int i = row[q], j = col[q];
+        if (i == j) return dd_mul(val[q], W[i + (idx_t)n * j]);
+        return dd_mul(val[q], dd_add(W[i + (idx_t)n * j], W[j + (idx_t)n * i]));
+    };
+    dd ret = mk(0.0);
+    int amari = nnz % 4, shou = nnz / 4;
+    for (int q = 0; q < amari; q++) ret = dd_add_eq(ret, term(q));
+    for (int c = 0, q = amari; c < shou; c++, q += 4) {
+        dd r1 = term(q), r2 = term(q + 1), r3 = term(q + 2), r4 = term(q + 3);
+        ret = dd_add_eq(ret, dd_add(dd_add(dd_add(r1, r2), r3), r4));
+    }
+    return ret;
+}
+
+__global__ void k_sparse_tasks(int ntask, const int *type, const int *block, const int *nnz, const idx_t *off,
+                               const int *row, const int *col, const dd *val, const idx_t *boff, const int *bsize,
+                               const dd *W, dd *out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntask || type[t] != 0) return;
+    int l = block[t];
+    idx_t o = off[t];
+    out[t] = sparse_inner(row + o, col + o, val + o, nnz[t], W + boff[l], bsize[l]);
+}
+// out[i] = 0 + sum of its task values in block order
+__global__ void k_combine_tasks(int m, const idx_t *ptr, const dd *tv, dd *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    dd r = mk(0.0);
+    for (idx_t t = ptr[i]; t < ptr[i + 1]; t++) r = dd_add_eq(r, tv[t]);
+    out[i] = r;
+}
+
+void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
+    if (dp.ntask > 0)
+        k_sparse_tasks<<<(dp.ntask + 127) / 128, 128, 0, s>>>(dp.ntask, dp.d_task_type, dp.d_task_block, dp.d_task_nnz,
+                                                              dp.d_task_off, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val,
+                                                              dp.d_boff, dp.d_bsize, W, dp.d_task_val);
+    if (dp.n_dense_tasks > 0) sdk::dot_batch_off(s, dp.d_dense_tasks, dp.n_dense_tasks, W, T);
+    k_combine_tasks<<<(dp.m + 255) / 256, 256, 0, s>>>(dp.m, dp.d_task_ptr, dp.d_task_val, out);
+}
+void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
+    if (dp.nctask > 0)
+        k_sparse_tasks<<<(dp.nctask + 127) / 128, 128, 0, s>>>(dp.nctask, dp.d_ctask_type, dp.d_ctask_block,
+                                                               dp.d_ctask_nnz, dp.d_ctask_off, dp.d_sp_row, dp.d_sp_col,
+                                                               dp.d_sp_val, dp.d_boff, dp.d_bsize, W, dp.d_ctask_val);
+    if (dp.n_dense_ctasks > 0) sdk::dot_batch_off(s, dp.d_dense_ctasks, dp.n_dense_ctasks, W, T);
+    k_combine_tasks<<<1, 32, 0, s>>>(1, dp.d_ctask_ptr, dp.d_ctask_val, out);
+}
+
+// ------------------------------------------------------------ sum y_i A_i --
+__global__ void k_plus_A(idx_t nn, const idx_t *ptr, const int *idx, const dd *val, const dd *y, dd *R) {
+    idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nn) return;
+    dd acc = mk(0.0);
+    for (idx_t q = ptr[e]; q < ptr[e + 1]; q++) acc = dd_add_eq(acc, dd_mul(val[q], y[idx[q]]));
+    R[e] = acc;
+}
+__global__ void k_plus_C(idx_t nn, const idx_t *ptr, const dd *val, dd *R) {
+    idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nn) return;
+    idx_t q0 = ptr[e], q1 = ptr[e + 1];
+    if (q0 == q1) return;
+    dd acc = R[e];
+    for (idx_t q = q0; q < q1; q++) acc = dd_add_eq(acc, val[q]);
+    R[e] = acc;
+}
+void plus_A(hipStream_t s, DeviceProblem &dp, const dd *y, dd *R) {
+    k_plus_A<<<(unsigned)((dp.nn + 255) / 256), 256, 0, s>>>(dp.nn, dp.d_el_ptr, dp.d_el_idx, dp.d_el_val, y, R);
+}
+void plus_C(hipStream_t s, DeviceProblem &dp, dd *R) {
+    k_plus_C<<<(unsigned)((dp.nn + 255) / 256), 256, 0, s>>>(dp.nn, dp.d_cel_ptr, dp.d_cel_val, R);
+}
+
+// ------------------------------------------------------------------ Schur --
+// work1[w] = A_i * Zinv for sparse A_i via row lists: row r accumulates
+// v * Zinv[src, :] in Raxpy order (sdpa_linear.cpp:843-866).
+__global__ void k_rowlist_mul(int n, int ns, const int *slot_list, int nslots, const int *wslot, const idx_t *rl_ptr,
+                              const int *rl_src, const dd *rl_val, const dd *Zinv, dd *work1) {
+    idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int which = blockIdx.y;
+    if (which >= nslots || e >= (idx_t)n * n) return;
+    int k = slot_list[which];
+    int r = (int)(e % n), t = (int)(e / n);
+    const idx_t *p = rl_ptr + (idx_t)k * (n + 1);
+    dd acc = mk(0.0);
+    for (idx_t q = p[r]; q < p[r + 1]; q++) {
+        dd v = rl_val[q];
+        if (dd_is_zero(v)) continue;  // Raxpy returns when da == 0
+        acc = dd_add_eq(acc, dd_mul(v, Zinv[rl_src[q] + (idx_t)n * t]));
+    }
+    work1[(idx_t)wslot[k] * n * n + e] = acc;
+}
+
+// Pair values computed by one thread: F1 with sparse A_j, F2 with sparse
+// A_j, and F3 (sdpa_newton.cpp:895-977).
+__global__ void k_pair_thread(int npairs, int n, const int *pform, const int *psi, const int *psj, const int *type,
+                              const int *nnz, const idx_t *off, const int *wslot, const int *row, const int *col,
+                              const dd *val, const dd *X, const dd *Zinv, const dd *work1, const dd *work2, dd *pv) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    int f = pform[p], si = psi[p], sj = psj[p];
+    if (type[sj] != 0) return;  // dense A_j: chunked dot kernel
+    const int *rj = row + off[sj], *cj = col + off[sj];
+    const dd *vj = val + off[sj];
+    const int nj = nnz[sj];
+    const idx_t nsq = (idx_t)n * n;
+    dd ret = mk(0.0);
+    if (f == 0) {
+        ret = sparse_inner(rj, cj, vj, nj, work2 + (idx_t)wslot[si] * nsq, n);
+    } else if (f == 1) {
+        const dd *F = work1 + (idx_t)wslot[si] * nsq;
+        for (int q = 0; q < nj; q++) {
+            int a = rj[q], b = cj[q];
+            dd v1 = vj[q];
+            dd v2 = mk(0.0);
+            for (int t = 0; t < n; t++) v2 = dd_add_eq(v2, dd_mul(X[a + (idx_t)n * t], F[t + (idx_t)n * b]));
+            ret = dd_add_eq(ret, dd_mul(v1, v2));
+            if (a != b) {
+                v2 = mk(0.0);
+                for (int t = 0; t < n; t++) v2 = dd_add_eq(v2, dd_mul(X[b + (idx_t)n * t], F[t + (idx_t)n * a]));
+                ret = dd_add_eq(ret, dd_mul(v1, v2));
+            }
+        }
+    } else {
+        const int *ri = row + off[si], *cii = col + off[si];
+        const dd *vi = val + off[si];
+        const int ni = nnz[si];
+        for (int q1 = 0; q1 < nj; q1++) {
+            int alpha = rj[q1], beta = cj[q1];
+            dd value1 = vj[q1];
+            dd sum = mk(0.0);
+            for (int q2 = 0; q2 < ni; q2++) {
+                int gamma = ri[q2], delta = cii[q2];
+                dd value2 = vi[q2];
+                sum = dd_add_eq(sum, dd_mul(dd_mul(value2, Zinv[delta + (idx_t)n * beta]), X[alpha + (idx_t)n * gamma]));
+                if (gamma != delta)
+                    sum = dd_add_eq(sum, dd_mul(dd_mul(value2, Zinv[gamma + (idx_t)n * beta]), X[alpha + (idx_t)n * delta]));
+            }
+            ret = dd_add_eq(ret, dd_mul(value1, sum));
+            if (alpha == beta) continue;
+            sum = mk(0.0);
+            for (int q2 = 0; q2 < ni; q2++) {
+                int gamma = ri[q2], delta = cii[q2];
+                dd value2 = vi[q2];
+                sum = dd_add_eq(sum, dd_mul(dd_mul(value2, Zinv[delta + (idx_t)n * alpha]), X[beta + (idx_t)n * gamma]));
+                if (gamma != delta)
+                    sum = dd_add_eq(sum, dd_mul(dd_mul(value2, Zinv[gamma + (idx_t)n * alpha]), X[beta + (idx_t)n * delta]));
+            }
+            ret = dd_add_eq(ret, dd_mul(value1, sum));
+        }
+    }
+    pv[p] = ret;
+}
+__global__ void k_scatter_pairs(int npairs, int m, const int *pi, const int *pj, const dd *pv, dd *B) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    int i = pi[p], j = pj[p];
+    dd v = pv[p];
+    if (i != j) {
+        B[i + (idx_t)m * j] = dd_add_eq(B[i + (idx_t)m * j], v);
+        B[j + (idx_t)m * i] = dd_add_eq(B[j + (idx_t)m * i], v);
+    } else {
+        B[i + (idx_t)m * i] = dd_add_eq(B[i + (idx_t)m * i], v);
+    }
+}
+// Pointer-list GEMM support: fill pointer arrays for batched products.
+__global__ void k_fill_ptrs(int cnt, const int *slots, const int *wslot, const int *type, const idx_t *off,
+                            const dd *dense, dd *work1, dd *work2, idx_t nsq, int mode, const dd **A, const dd **B, dd **C,
+                            const dd *X, const dd *Zinv) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cnt) return;
+    int k = slots[q];
+    if (mode == 0) {  // work1 = Adense * Zinv
+        A[q] = dense + off[k];
+        B[q] = Zinv;
+        C[q] = work1 + (idx_t)wslot[k] * nsq;
+    } else {  // work2 = X * work1
+        A[q] = X;
+        B[q] = work1 + (idx_t)wslot[k] * nsq;
+        C[q] = work2 + (idx_t)wslot[k] * nsq;
+    }
+}
+
+void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B, int T) {
+    const int m = dp.m;
+    sdk::fill(s, (idx_t)m * m, mk(0.0), B);  // Newton::Make_bMat: bMat.setZero()
+    for (size_t l = 0; l < dp.schur.size(); l++) {
+        SchurBlock &S = dp.schur[l];
+        const int n = S.n;
+        const idx_t nsq = (idx_t)n * n;
+        const dd *Xl = X + dp.boff[l];
+        const dd *Zl = invZ + dp.boff[l];
+        const dd **pA = (const dd **)dp.d_ptrs;
+        const dd **pB = pA + dp.ptr_cap;
+        dd **pC = (dd **)(pB + dp.ptr_cap);
+        // work1 = A_i * Zinv
+        if (!S.sp_slots.empty()) {
+            dim3 g((unsigned)((nsq + 255) / 256), (unsigned)S.sp_slots.size());
+            k_rowlist_mul<<<g, 256, 0, s>>>(n, (int)S.ci.size(), S.d_sp_slots, (int)S.sp_slots.size(), S.d_wslot,
+                                            S.d_rl_ptr, S.d_rl_src, S.d_rl_val, Zl, dp.d_work1);
+        }
+        if (!S.de_slots.empty()) {
+            int c = (int)S.de_slots.size();
+            k_fill_ptrs<<<(c + 63) / 64, 64, 0, s>>>(c, S.d_de_slots, S.d_wslot, S.d_type, S.d_off, dp.d_dense, dp.d_work1,
+                                                     dp.d_work2, nsq, 0, pA, pB, pC, Xl, Zl);
+            sdk::gemm_ptrs(s, n, n, n, pA, n, pB, n, pC, n, c);
+        }
+        // work2 = X * work1 (F1, and F2 constraints paired with dense A_j)
+        if (!S.g_slots.empty()) {
+            int c = (int)S.g_slots.size();
+            k_fill_ptrs<<<(c + 63) / 64, 64, 0, s>>>(c, S.d_g_slots, S.d_wslot, S.d_type, S.d_off, dp.d_dense, dp.d_work1,
+                                                     dp.d_work2, nsq, 1, pA, pB, pC, Xl, Zl);
+            sdk::gemm_ptrs(s, n, n, n, pA, n, pB, n, pC, n, c);
+        }
+        if (S.npairs > 0) {
+            k_pair_thread<<<(S.npairs + 127) / 128, 128, 0, s>>>(S.npairs, n, S.d_pair_form, S.d_pair_slot_i,
+                                                                  S.d_pair_slot_j, S.d_type, S.d_nnz, S.d_off, S.d_wslot,
+                                                                  dp.d_sp_row, dp.d_sp_col, dp.d_sp_val, Xl, Zl,
+                                                                  dp.d_work1, dp.d_work2, dp.d_pair_val);
+            if (S.ndense_pairs > 0) sdk::dot_batch(s, S.d_pair_tasks, S.ndense_pairs, T);
+            k_scatter_pairs<<<(S.npairs + 255) / 256, 256, 0, s>>>(S.npairs, m, S.d_pair_i, S.d_pair_j, dp.d_pair_val, B);
+        }
+    }
+}
+
+}  // namespace sdp

@@ -1,0 +1,656 @@
+// Interior-point main loop over device-resident iterates.
+//
+// The scalar control (phase test, step lengths, centering parameter,
+// complementarity, residual ratios) is the reference's host logic evaluated
+// in dd on the CPU; every n x n and m x m operation runs on the GPU.  Only
+// scalars and block diagonals cross the PCIe boundary during the loop.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+
+#include "context.hpp"
+#include "solver.hpp"
+#include "steqr.hpp"
+
+namespace sdp {
+
+using sdk::idx_t;
+
+int Context::init(int dev) {
+    device = dev;
+    if (hipSetDevice(dev) != hipSuccess) return -1;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_dot_work, sizeof(dd) * 8192) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_counter, sizeof(unsigned) * 4) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_scal, sizeof(dd) * 64) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_info, sizeof(int64_t)) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_rinv, sizeof(dd) * 64) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_max_work, sizeof(dd) * 256) != hipSuccess) return -1;
+    if (hipMemset(d_counter, 0, sizeof(unsigned) * 4) != hipSuccess) return -1;
+    return 0;
+}
+void Context::destroy() {
+    if (d_dot_work) (void)hipFree(d_dot_work);
+    if (d_counter) (void)hipFree(d_counter);
+    if (d_scal) (void)hipFree(d_scal);
+    if (d_info) (void)hipFree(d_info);
+    if (d_rinv) (void)hipFree(d_rinv);
+    if (d_max_work) (void)hipFree(d_max_work);
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+namespace {
+// Lal::getMinEigen (sdpa_linear.cpp:27-122): Lanczos on L M L^T with start
+// vector of ones; the (k+1)-tridiagonal eigenproblem is solved on the host,
+// forming only the last row of its eigenvector matrix.
+static ddr lanczos_run(Context &ctx, int nDim, const dd *L, const dd *M, dd *lz_r, dd *lz_q, dd *lz_qold, dd *lz_w,
+                       dd *lz_tmp) {
+    hipStream_t s = ctx.stream;
+    ddr alpha, beta, value;
+    ddr min = 1.0e+51, min_old = 1.0e+52, min_min = 1.0e+50;
+    ddr error = 1.0e+10;
+    int k = 0, kk = 0;
+    std::vector<ddr> diagVec(nDim, min_min), diagVec2(nDim, ddr(0.0)), out(nDim), bv(nDim), work(2 * nDim + 2);
+    std::vector<ddr> Q((size_t)nDim + 1);
+    sdk::fill(s, nDim, mk(0.0), lz_q);
+    sdk::fill(s, nDim, mk(1.0), lz_r);
+    beta = sqrt((ddr)nDim);
+    while (k < nDim && k < sqrt((ddr)nDim) + 10 && beta > 1.0e-16 &&
+           (abs(min - min_old) > (1.0e-5) * abs(min) + (1.0e-8) || abs(error * beta) > (1.0e-2) * abs(min) + (1.0e-4))) {
+        sdk::copy(s, nDim, lz_q, lz_qold);
+        value = ddr(1.0) / beta;
+        sdk::copy(s, nDim, lz_r, lz_q);
+        sdk::scal(s, nDim, value.v, lz_q);
+        sdk::trmv_lower(s, 'T', nDim, L, nDim, lz_q, lz_w);
+        sdk::gemv_n(s, nDim, nDim, mk(1.0), M, nDim, lz_w, mk(0.0), lz_tmp);
+        sdk::trmv_lower(s, 'N', nDim, L, nDim, lz_tmp, lz_w);
+        ctx.dot(nDim, lz_q, lz_w, 0);  // alpha, kept on the device for the update
+        sdk::copy(s, nDim, lz_w, lz_r);
+        sdk::axpy_dev(s, nDim, ctx.d_scal + 0, true, lz_q, lz_r);
+        sdk::axpy(s, nDim, dd_neg(beta.v), lz_qold, lz_r);
+        ctx.dot(nDim, lz_r, lz_r, 1);
+        dd h[2];
+        (void)hipMemcpyAsync(h, ctx.d_scal, sizeof(dd) * 2, hipMemcpyDeviceToHost, s);
+        (void)hipStreamSynchronize(s);
+        alpha = ddr(h[0]);
+        diagVec[k] = alpha;
+        if (kk >= sqrt((ddr)k) || k == nDim - 1 || k > sqrt((ddr)nDim + 9)) {
+            kk = 0;
+            out = diagVec;
+            bv = diagVec2;
+            out[nDim - 1] = diagVec[k];
+            bv[nDim - 1] = 0.0;
+            int kp1 = k + 1;
+            // only row k of the eigenvector matrix is needed: Q[k + 0*ldz]
+            std::vector<ddr> z((size_t)kp1 * kp1);
+            int64_t info = steqr_rows(kp1, out.data(), bv.data(), z.data(), kp1, work.data(), k, k + 1);
+            if (info > 0) break;
+            min_old = min;
+            min = out[0];
+            error = z[k];
+        }
+        value = ddr(h[1]);
+        beta = sqrt(value);
+        diagVec2[k] = beta;
+        ++k;
+        ++kk;
+    }
+    return min - abs(error * beta);
+}
+
+}  // namespace
+
+namespace {
+
+struct DevBuf {
+    dd *p = nullptr;
+    idx_t n = 0;
+};
+
+class Solver {
+  public:
+    Solver(Context &c, const HostProblem &h, DeviceProblem &d, const Params &p) : ctx(c), hp(h), dp(d), prm(p) {}
+    ~Solver() {
+        for (void *q : mem) (void)hipFree(q);
+    }
+    int run(SolveOutput &out, std::vector<dd> *xo, std::vector<dd> *yo, std::vector<dd> *zo);
+
+  private:
+    Context &ctx;
+    const HostProblem &hp;
+    DeviceProblem &dp;
+    const Params &prm;
+    std::vector<void *> mem;
+    hipStream_t s = nullptr;
+    int m = 0, nB = 0, nDim = 0;
+    idx_t nn = 0;
+    // iterates and work spaces (concatenated SDP blocks)
+    dd *X, *Z, *invCholX, *invCholZ, *invZ, *DxMat, *DzMat, *r_zinv, *x_rd_zinv, *DLS1, *DLS2, *dualMat;
+    dd *y, *dy, *g, *primalVec, *bMat;
+    // Lanczos vectors
+    dd *lz_r, *lz_q, *lz_qold, *lz_w, *lz_tmp;
+    std::vector<dd> diag_buf;
+    double t_schur = 0, t_chol = 0, t_solve = 0, t_lanczos = 0, t_inverse = 0;
+
+    dd *alloc(idx_t n) {
+        dd *p = nullptr;
+        if (n <= 0) n = 1;
+        if (hipMalloc((void **)&p, sizeof(dd) * n) != hipSuccess) return nullptr;
+        mem.push_back(p);
+        return p;
+    }
+    dd *blk(dd *base, int l) { return base + dp.boff[l]; }
+    int bsz(int l) const { return hp.bs[l]; }
+    void sync() { (void)hipStreamSynchronize(s); }
+    static double now() {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
+    // Download `cnt` scalar slots.
+    void get_scalars(dd *h, int cnt) {
+        (void)hipMemcpyAsync(h, ctx.d_scal, sizeof(dd) * cnt, hipMemcpyDeviceToHost, s);
+        sync();
+    }
+
+    // ---- dense linear-space helpers (sdpa_linear.cpp Lal::*) ----
+    // DenseLinearSpace inner product: per-block Rdot, summed in block order.
+    ddr inner_dense(const dd *a, const dd *b) {
+        for (int l = 0; l < nB; l++) ctx.dot((idx_t)bsz(l) * bsz(l), a + dp.boff[l], b + dp.boff[l], l);
+        std::vector<dd> h(nB);
+        get_scalars(h.data(), nB);
+        ddr r = 0.0;
+        for (int l = 0; l < nB; l++) r += ddr(h[l]);
+        return r;
+    }
+    // Three dense inner products with one synchronisation.
+    void inner_dense3(const dd *a0, const dd *b0, const dd *a1, const dd *b1, const dd *a2, const dd *b2, ddr &r0, ddr &r1,
+                      ddr &r2) {
+        const dd *as[3] = {a0, a1, a2}, *bs2[3] = {b0, b1, b2};
+        for (int q = 0; q < 3; q++)
+            for (int l = 0; l < nB; l++)
+                ctx.dot((idx_t)bsz(l) * bsz(l), as[q] + dp.boff[l], bs2[q] + dp.boff[l], q * nB + l);
+        std::vector<dd> h(3 * nB);
+        get_scalars(h.data(), 3 * nB);
+        ddr r[3];
+        for (int q = 0; q < 3; q++) {
+            r[q] = 0.0;
+            for (int l = 0; l < nB; l++) r[q] += ddr(h[q * nB + l]);
+        }
+        r0 = r[0];
+        r1 = r[1];
+        r2 = r[2];
+    }
+    ddr inner_C(const dd *W) {
+        sdp::inner_C(s, dp, W, ctx.d_scal + 40, ctx.dot_T);
+        dd h;
+        (void)hipMemcpyAsync(&h, ctx.d_scal + 40, sizeof(dd), hipMemcpyDeviceToHost, s);
+        sync();
+        return ddr(h);
+    }
+    ddr inner_b(const dd *v) {
+        ctx.dot(m, dp.d_b, v, 41);
+        dd h;
+        (void)hipMemcpyAsync(&h, ctx.d_scal + 41, sizeof(dd), hipMemcpyDeviceToHost, s);
+        sync();
+        return ddr(h);
+    }
+    ddr max_abs(const dd *x, idx_t len) {
+        sdk::max_abs(s, len, x, ctx.d_scal + 42, ctx.d_max_work, ctx.d_counter + 1);
+        dd h;
+        (void)hipMemcpyAsync(&h, ctx.d_scal + 42, sizeof(dd), hipMemcpyDeviceToHost, s);
+        sync();
+        return ddr(h);
+    }
+    // ret = a*b*c per block (Jal::ns_jordan_triple_product)
+    void triple(dd *ret, const dd *a, const dd *b, const dd *c, dd *work) {
+        for (int l = 0; l < nB; l++) {
+            int n = bsz(l);
+            idx_t o = dp.boff[l];
+            sdk::gemm(s, 'N', n, n, n, mk(1.0), a + o, n, b + o, n, mk(0.0), work + o, n);
+        }
+        for (int l = 0; l < nB; l++) {
+            int n = bsz(l);
+            idx_t o = dp.boff[l];
+            sdk::gemm(s, 'N', n, n, n, mk(1.0), work + o, n, c + o, n, mk(0.0), ret + o, n);
+        }
+    }
+    // ret = a + s*b (Lal::plus) over the whole space
+    void plus(dd *ret, const dd *a, const dd *b, dd sc) {
+        sdk::copy(s, nn, a, ret);
+        sdk::axpy(s, nn, sc, b, ret);
+    }
+
+    // ---- pieces of the algorithm ----
+    void residuals(ddr &normP, ddr &normD) {
+        // sdpa_dataset.cpp:625-645
+        sdp::inner_A(s, dp, X, primalVec, ctx.dot_T);
+        sdk::scal(s, m, mk(-1.0), primalVec);
+        sdk::axpy(s, m, mk(1.0), dp.d_b, primalVec);
+        sdp::plus_A(s, dp, y, dualMat);
+        sdk::scal(s, nn, mk(-1.0), dualMat);
+        sdp::plus_C(s, dp, dualMat);
+        sdk::axpy(s, nn, mk(-1.0), Z, dualMat);
+        normP = max_abs(primalVec, m);
+        normD = max_abs(dualMat, nn);
+    }
+    bool inv_chol_all(dd *A, dd *invChol, dd *work, dd *inv) {
+        double t0 = now();
+        for (int l = 0; l < nB; l++) {
+            int n = bsz(l);
+            idx_t o = dp.boff[l];
+            sdk::copy(s, (idx_t)n * n, A + o, work + o);
+            sdk::potrf_lower(s, n, work + o, n, ctx.d_info, ctx.d_rinv);
+            int64_t info = 0;
+            (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
+            sync();
+            if (info != 0) {
+                t_inverse += now() - t0;
+                return false;
+            }
+            sdk::zero_upper(s, n, work + o);
+            sdk::inv_lower(s, n, work + o, n, invChol + o, n);
+        }
+        if (inv)
+            for (int l = 0; l < nB; l++) {
+                int n = bsz(l);
+                idx_t o = dp.boff[l];
+                sdk::trmm_lltn_self(s, n, invChol + o, n, inv + o, n);
+            }
+        sync();
+        t_inverse += now() - t0;
+        return true;
+    }
+    ddr lanczos(int n, const dd *L, const dd *M) { return lanczos_run(ctx, n, L, M, lz_r, lz_q, lz_qold, lz_w, lz_tmp); }
+    ddr min_eigen(const dd *L, const dd *M) {
+        double t0 = now();
+        ddr mn = 1.0E50;
+        for (int l = 0; l < nB; l++) {
+            ddr v = lanczos(bsz(l), L + dp.boff[l], M + dp.boff[l]);
+            if (v < mn) mn = v;
+        }
+        t_lanczos += now() - t0;
+        return mn;
+    }
+    void diag_sums(double lam, ddr &x0z, ddr &xz0) {
+        // x0zMat += lambda * Z_ii, xMatz0 += lambda * X_ii, blocks in order
+        for (int l = 0; l < nB; l++) {
+            int n = bsz(l);
+            diag_buf.resize((size_t)2 * n);
+            (void)hipMemcpy2DAsync(diag_buf.data(), sizeof(dd), Z + dp.boff[l], sizeof(dd) * (n + 1), sizeof(dd), n,
+                                   hipMemcpyDeviceToHost, s);
+            (void)hipMemcpy2DAsync(diag_buf.data() + n, sizeof(dd), X + dp.boff[l], sizeof(dd) * (n + 1), sizeof(dd), n,
+                                   hipMemcpyDeviceToHost, s);
+            sync();
+            for (int i = 0; i < n; i++) {
+                x0z += lam * ddr(diag_buf[i]);
+                xz0 += lam * ddr(diag_buf[n + i]);
+            }
+        }
+    }
+};
+
+int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std::vector<dd> *zo) {
+    s = ctx.stream;
+    m = hp.m;
+    nB = (int)hp.bs.size();
+    nn = hp.nn;
+    nDim = 0;
+    for (int n : hp.bs) nDim += n;
+    int maxn = 0;
+    for (int n : hp.bs) maxn = std::max(maxn, n);
+    dd **bufs[] = {&X, &Z, &invCholX, &invCholZ, &invZ, &DxMat, &DzMat, &r_zinv, &x_rd_zinv, &DLS1, &DLS2, &dualMat};
+    for (dd **b : bufs)
+        if (!(*b = alloc(nn))) return -2;
+    if (!(y = alloc(m)) || !(dy = alloc(m)) || !(g = alloc(m)) || !(primalVec = alloc(m)) ||
+        !(bMat = alloc((idx_t)m * m)))
+        return -2;
+    if (!(lz_r = alloc(maxn)) || !(lz_q = alloc(maxn)) || !(lz_qold = alloc(maxn)) || !(lz_w = alloc(maxn)) ||
+        !(lz_tmp = alloc(maxn)))
+        return -2;
+    double t_start = now();
+    for (dd **b : bufs) sdk::fill(s, nn, mk(0.0), *b);
+
+    // Solutions::initialize (sdpa_dataset.cpp:35-60)
+    ddr lambda = prm.lambdaStar;
+    for (int l = 0; l < nB; l++) {
+        int n = bsz(l);
+        sdk::set_identity(s, n, lambda.v, blk(X, l));
+        sdk::set_identity(s, n, lambda.v, blk(Z, l));
+        sdk::set_identity(s, n, (1.0 / sqrt(lambda)).v, blk(invCholX, l));
+        sdk::set_identity(s, n, (1.0 / sqrt(lambda)).v, blk(invCholZ, l));
+        sdk::set_identity(s, n, (1.0 / lambda).v, blk(invZ, l));
+    }
+    sdk::fill(s, m, mk(0.0), y);
+
+    ddr initNormP, initNormD, curNormP, curNormD;
+    residuals(initNormP, initNormD);
+    curNormP = initNormP;
+    curNormD = initNormD;
+
+    ddr alphaP = 0.0, alphaD = 0.0, betaV = prm.betaStar;
+    bool switchOn = true;
+    ddr muInitial = ddr(prm.lambdaStar) * ddr(prm.lambdaStar), muCurrent = muInitial;
+    ddr thetaP, thetaD;
+    {
+        ddr accuracy = prm.epsilonDash;
+        thetaP = (initNormP < accuracy) ? ddr(0.0) : ddr(1.0);
+        thetaD = (initNormD < accuracy) ? ddr(0.0) : ddr(1.0);
+    }
+    ddr rho = 1.0, etaPrimal, etaDual, objValPrimal, objValDual;
+    {
+        ddr omegaStar = prm.omegaStar;
+        etaPrimal = omegaStar * nDim * muInitial;
+        etaDual = omegaStar * nDim * muInitial;
+        objValPrimal = inner_C(X);
+        objValDual = inner_b(y);
+    }
+    int phase = noINFO;
+    auto updateCheck = [&](const ddr &nP, const ddr &nD) -> bool {
+        // Phase::updateCheck (sdpa_parts.cpp:875-954)
+        const ddr NONZERO = 1.0e-6;
+        ddr accuracy = prm.epsilonDash;
+        phase = noINFO;
+        if (nP <= accuracy) phase = (nD <= accuracy) ? pdFEAS : pFEAS;
+        if (phase == noINFO && nD <= accuracy) phase = dFEAS;
+        if (phase == pdFEAS) {
+            ddr mean = (abs(objValPrimal) + abs(objValDual)) / 2.0;
+            ddr PDgap = abs(objValPrimal - objValDual);
+            ddr dominator = (mean < 1.0) ? ddr(1.0) : mean;
+            if (PDgap / dominator <= prm.epsilonStar) {
+                phase = pdOPT;
+                return false;
+            }
+        }
+        if (phase == noINFO && rho > 1.0 + NONZERO) {
+            phase = pdINF;
+            return false;
+        }
+        if (phase == pFEAS) {
+            if (objValPrimal <= -prm.upperBound) {
+                phase = pUNBD;
+                return false;
+            }
+            if (rho > 1.0 + NONZERO) {
+                phase = pFEAS_dINF;
+                return false;
+            }
+        }
+        if (phase == dFEAS) {
+            if (objValDual >= -prm.lowerBound) {
+                phase = dUNBD;
+                return false;
+            }
+            if (rho > 1.0 + NONZERO) {
+                phase = pINF_dFEAS;
+                return false;
+            }
+        }
+        return true;
+    };
+    updateCheck(initNormP, initNormD);
+
+    // Newton::Mehrotra (sdpa_newton.cpp:1345-1384)
+    auto mehrotra = [&](bool predictor) -> bool {
+        ddr target = betaV * muCurrent;
+        sdk::copy(s, nn, invZ, r_zinv);
+        sdk::scal(s, nn, target.v, r_zinv);
+        sdk::axpy(s, nn, mk(-1.0), X, r_zinv);
+        if (!predictor) {
+            triple(DLS1, DxMat, DzMat, invZ, DLS2);
+            sdk::axpy(s, nn, mk(-1.0), DLS1, r_zinv);
+        }
+        if (phase == pFEAS || phase == noINFO) {
+            if (predictor) triple(x_rd_zinv, X, dualMat, invZ, DLS2);
+            plus(DLS1, r_zinv, x_rd_zinv, mk(-1.0));
+        } else {
+            sdk::copy(s, nn, r_zinv, DLS1);
+        }
+        sdp::inner_A(s, dp, DLS1, g, ctx.dot_T);
+        sdk::scal(s, m, mk(-1.0), g);
+        sdk::axpy(s, m, mk(1.0), primalVec, g);
+        if (predictor) {
+            sync();
+            double t0 = now();
+            sdp::schur(s, dp, X, invZ, bMat, ctx.dot_T);
+            sync();
+            double t1 = now();
+            sdk::potrf_lower(s, m, bMat, m, ctx.d_info, ctx.d_rinv);
+            int64_t info = 0;
+            (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
+            sync();
+            double t2 = now();
+            t_schur += t1 - t0;
+            t_chol += t2 - t1;
+            if (info != 0) return false;
+        }
+        sdk::copy(s, m, g, dy);
+        sdk::trsv_lower(s, 'N', m, bMat, m, dy);
+        sdk::trsv_lower(s, 'T', m, bMat, m, dy);
+        sdp::plus_A(s, dp, dy, DzMat);
+        sdk::scal(s, nn, mk(-1.0), DzMat);
+        if (phase == pFEAS || phase == noINFO) sdk::axpy(s, nn, mk(1.0), dualMat, DzMat);
+        triple(DLS1, X, DzMat, invZ, DLS2);
+        plus(DxMat, r_zinv, DLS1, mk(-1.0));
+        for (int l = 0; l < nB; l++) sdk::symmetrize(s, bsz(l), blk(DxMat, l));
+        return true;
+    };
+
+    int pIteration = 0;
+    while (updateCheck(curNormP, curNormD) && pIteration < prm.maxIteration) {
+        switchOn = (phase == noINFO || phase == pFEAS || phase == dFEAS);
+        if (phase == pdFEAS) {
+            betaV = 0.0;
+        } else {
+            betaV = prm.betaBar;
+            if (!switchOn) betaV = ddr(2.0);
+        }
+        if (!mehrotra(true)) {
+            res.message = "Schur Cholesky failed";
+            break;
+        }
+        // StepLength::MehrotraPredictor (sdpa_parts.cpp:307-354)
+        alphaP = alphaD = 0.9;
+        if (phase == noINFO || phase == dFEAS) {
+            if (alphaP > 1.0) alphaP = 1.0;
+        } else {
+            ddr inc = inner_C(DxMat);
+            if (inc > 0.0) {
+                if (alphaP > alphaD) alphaP = alphaD;
+                if (alphaP > 1.0) alphaP = 1.0;
+            }
+        }
+        if (phase == noINFO || phase == pFEAS) {
+            if (alphaD > 1.0) alphaD = 1.0;
+        } else {
+            ddr inc = inner_b(dy);
+            if (inc < 0.0) {
+                if (alphaD > alphaP) alphaD = alphaP;
+                if (alphaD > 1.0) alphaD = 1.0;
+            }
+        }
+        // DirectionParameter::MehrotraCorrector (sdpa_parts.cpp:499-529)
+        {
+            ddr xv, uz, uv;
+            inner_dense3(X, DzMat, DxMat, Z, DxMat, DzMat, xv, uz, uv);
+            ddr muTarget = muCurrent + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nDim;
+            betaV = muTarget / muCurrent;
+            if (betaV < 1.0) betaV = betaV * betaV;
+            if (phase == pdFEAS) {
+                if (betaV < prm.betaStar) betaV = prm.betaStar;
+                if (betaV > 1.0) betaV = 1.0;
+            } else {
+                if (betaV < prm.betaBar) betaV = prm.betaBar;
+            }
+        }
+        if ((phase == pdFEAS) && ((betaV > 5) || (objValPrimal < objValDual))) break;
+        mehrotra(false);
+        // StepLength::MehrotraCorrector (sdpa_parts.cpp:356-466)
+        {
+            const ddr xi = 3.0;
+            ddr alphaBD = 100.0;
+            ddr mx = min_eigen(invCholX, DxMat);
+            alphaP = (-mx > 1.0 / alphaBD) ? -1.0 / mx : alphaBD;
+            ddr mz = min_eigen(invCholZ, DzMat);
+            alphaD = (-mz > 1.0 / alphaBD) ? -1.0 / mz : alphaBD;
+            alphaP = prm.gammaStar * alphaP;
+            alphaD = prm.gammaStar * alphaD;
+            if (phase == noINFO || phase == dFEAS) {
+                if (alphaP > 1.0) alphaP = 1.0;
+            } else {
+                ddr inc = inner_C(DxMat);
+                if (inc > 0.0) {
+                    if (alphaP > alphaD) alphaP = alphaD;
+                    if (alphaP > 1.0) alphaP = 1.0;
+                }
+            }
+            if (phase == noINFO || phase == pFEAS) {
+                if (alphaD > 1.0) alphaD = 1.0;
+            } else {
+                ddr inc = inner_b(dy);
+                if (inc < 0.0) {
+                    if (alphaD > alphaP) alphaD = alphaP;
+                    if (alphaD > 1.0) alphaD = 1.0;
+                }
+            }
+            if (switchOn && (phase == noINFO || phase == pFEAS || phase == dFEAS)) {
+                ddr xv, uz, uv;
+                inner_dense3(X, DzMat, DxMat, Z, DxMat, DzMat, xv, uz, uv);
+                ddr thetaMax = max((1.0 - alphaP) * thetaP, (1.0 - alphaD) * thetaD);
+                ddr muNew = muCurrent + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nDim;
+                while (thetaMax * thetaMax * muInitial > xi * muNew) {
+                    ddr alphaMax = 0.95 * max(alphaP, alphaD);
+                    alphaP = min(alphaP, alphaMax);
+                    alphaD = min(alphaD, alphaMax);
+                    thetaMax = max((1.0 - alphaP) * thetaP, (1.0 - alphaD) * thetaD);
+                    muNew = muCurrent + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nDim;
+                    if (alphaP < 1.0e-6 && alphaD < 1.0e-6) break;
+                }
+            }
+            if (phase == pdFEAS) {
+                ddr oD = inner_b(y);
+                ddr oP = inner_C(X);
+                ddr incD = inner_b(dy);
+                incD *= alphaD;
+                ddr incP = inner_C(DxMat);
+                incP *= alphaP;
+                ddr maxRatio = (oD - oP) / (incP - incD);
+                if ((maxRatio > 0.0) && (maxRatio < 1.0)) {
+                    alphaP *= maxRatio;
+                    alphaD *= maxRatio;
+                }
+            }
+        }
+        res.trace.push_back({pIteration, muCurrent.v, thetaP.v, thetaD.v, objValPrimal.v, objValDual.v, alphaP.v, alphaD.v,
+                             betaV.v});
+        // Solutions::update (sdpa_dataset.cpp:120-143)
+        sdk::axpy(s, nn, alphaP.v, DxMat, X);
+        sdk::axpy(s, m, alphaD.v, dy, y);
+        sdk::axpy(s, nn, alphaD.v, DzMat, Z);
+        const ddr cannot_move = 1.0e-4;
+        if (alphaP < cannot_move && alphaD < cannot_move) {
+            res.message = "cannot move";
+            break;
+        }
+        bool okx = inv_chol_all(X, invCholX, DLS1, nullptr);
+        bool okz = inv_chol_all(Z, invCholZ, DLS2, invZ);
+        if (!okx || !okz) {
+            res.message = "cannot move";
+            break;
+        }
+        if (switchOn) {
+            thetaP = abs((1.0 - alphaP) * thetaP);
+            thetaD = abs((1.0 - alphaD) * thetaD);
+        }
+        muCurrent = inner_dense(X, Z);
+        muCurrent /= nDim;
+        residuals(curNormP, curNormD);
+        thetaP = (initNormP > 1.0e-10) ? curNormP / initNormP : ddr(0.0);
+        thetaD = (initNormD > 1.0e-10) ? curNormD / initNormD : ddr(0.0);
+        {
+            // SolveInfo::update without an initial point (sdpa_parts.cpp:743-790)
+            objValPrimal = inner_C(X);
+            objValDual = inner_b(y);
+            ddr primal = thetaP, dual = thetaD, omega = prm.omegaStar;
+            rho = 0.0;
+            ddr x0z0 = nDim * muInitial, xMatzMat = nDim * muCurrent, x0zMat = 0.0, xMatz0 = 0.0;
+            diag_sums(prm.lambdaStar, x0zMat, xMatz0);
+            ddr accuracy = prm.epsilonDash;
+            if (curNormP <= accuracy && xMatz0 < etaPrimal) etaPrimal = xMatz0;
+            if (curNormD <= accuracy && x0zMat < etaDual) etaDual = x0zMat;
+            if (curNormP > accuracy && curNormD <= accuracy)
+                rho = primal * x0zMat / ((primal + (1.0 - primal) * omega) * etaDual + xMatzMat);
+            if (curNormP <= accuracy && curNormD > accuracy)
+                rho = dual * xMatz0 / ((dual + (1.0 - dual) * omega) * etaPrimal + xMatzMat);
+            if (curNormP > accuracy && curNormD > accuracy)
+                rho = (dual * xMatz0 + primal * x0zMat) /
+                      ((primal * dual + omega * (primal * (1.0 - dual) + (1.0 - primal) * dual)) * x0z0 + xMatzMat);
+        }
+        pIteration++;
+    }
+    residuals(curNormP, curNormD);
+    sync();
+    res.t.total = now() - t_start;
+    res.t.schur = t_schur;
+    res.t.chol = t_chol;
+    res.t.lanczos = t_lanczos;
+    res.t.inverse = t_inverse;
+    res.iterations = pIteration;
+    res.phase = phase;
+    int rep = phase;
+    switch (phase) {  // Phase::reverse
+    case pFEAS: rep = dFEAS; break;
+    case dFEAS: rep = pFEAS; break;
+    case pFEAS_dINF: rep = pINF_dFEAS; break;
+    case pINF_dFEAS: rep = pFEAS_dINF; break;
+    case pUNBD: rep = dUNBD; break;
+    case dUNBD: rep = pUNBD; break;
+    default: break;
+    }
+    res.phase_reported = rep;
+    res.mu = muCurrent.v;
+    res.objValPrimal = objValPrimal.v;
+    res.objValDual = objValDual.v;
+    res.normPrimalVec = curNormP.v;
+    res.normDualMat = curNormD.v;
+    {
+        ddr mean = (abs(objValPrimal) + abs(objValDual)) / 2.0;
+        ddr PDgap = abs(objValPrimal - objValDual);
+        res.relgap = ((mean < 1.0) ? PDgap : PDgap / mean).v;
+    }
+    if (xo) {
+        xo->resize(nn);
+        (void)hipMemcpy(xo->data(), X, sizeof(dd) * nn, hipMemcpyDeviceToHost);
+    }
+    if (zo) {
+        zo->resize(nn);
+        (void)hipMemcpy(zo->data(), Z, sizeof(dd) * nn, hipMemcpyDeviceToHost);
+    }
+    if (yo) {
+        yo->resize(m);
+        (void)hipMemcpy(yo->data(), y, sizeof(dd) * m, hipMemcpyDeviceToHost);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx.set_error(hipGetErrorString(e));
+        return -3;
+    }
+    return 0;
+}
+
+}  // namespace
+
+int solve(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const Params &prm, SolveOutput &out, std::vector<dd> *xMat,
+          std::vector<dd> *yVec, std::vector<dd> *zMat) {
+    Solver sv(ctx, hp, dp, prm);
+    return sv.run(out, xMat, yVec, zMat);
+}
+
+dd lanczos_min_eigen(Context &ctx, int n, const dd *d_L, const dd *d_M) {
+    dd *v = nullptr;
+    if (hipMalloc((void **)&v, sizeof(dd) * 5 * (size_t)n) != hipSuccess) return dd{NAN, NAN};
+    ddr r = lanczos_run(ctx, n, d_L, d_M, v, v + n, v + 2 * (size_t)n, v + 3 * (size_t)n, v + 4 * (size_t)n);
+    (void)hipFree(v);
+    return r.v;
+}
+
+}  // namespace sdp

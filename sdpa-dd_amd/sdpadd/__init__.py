@@ -1,0 +1,288 @@
+"""Python front end of the MI355X double-double Newton-step library.
+
+The compute path is the C ABI of ``libsdpadd_hip.so`` (``include/sdpadd.h``);
+this module only loads it with ctypes and moves numpy arrays in and out.
+A dd array is a float64 array whose last axis has length 2 (hi, lo), the
+memory layout of QD's ``dd_real``.  Matrices are column-major.
+
+There is no CPU fallback: if the library is missing or no GPU is present,
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libsdpadd_hip.so")
+CLI_PATH = os.path.join(os.path.dirname(_HERE), "sdpa_dd_gpu")
+
+PHASES = ["noINFO", "pFEAS", "dFEAS", "pdFEAS", "pdINF", "pFEAS_dINF", "pINF_dFEAS", "pdOPT", "pUNBD", "dUNBD"]
+
+_lib = None
+
+
+class SdpaddError(RuntimeError):
+    pass
+
+
+def load_library(path=None):
+    """Loads libsdpadd_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise SdpaddError("libsdpadd_hip.so not built (run sdpa-dd_amd/build.py): %s" % path)
+    lib = ctypes.CDLL(path)
+    c_dp = ctypes.POINTER(ctypes.c_double)
+    c_i64 = ctypes.c_int64
+    vp = ctypes.c_void_p
+    sig = {
+        "sdpadd_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+        "sdpadd_destroy": (ctypes.c_int, [vp]),
+        "sdpadd_last_error": (ctypes.c_char_p, [vp]),
+        "sdpadd_set_dot_partition": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sdpadd_malloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
+        "sdpadd_stats_enable": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sdpadd_stats_reset": (ctypes.c_int, [vp]),
+        "sdpadd_stats_get": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(c_i64), c_dp, c_dp]),
+        "sdpadd_free": (ctypes.c_int, [vp, vp]),
+        "sdpadd_upload": (ctypes.c_int, [vp, vp, vp, ctypes.c_size_t]),
+        "sdpadd_download": (ctypes.c_int, [vp, vp, vp, ctypes.c_size_t]),
+        "sdpadd_Rgemm": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_i64, c_dp, vp, c_i64, vp, c_i64,
+                                        c_dp, vp, c_i64]),
+        "sdpadd_Rsyrk": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_dp, vp, c_i64, c_dp, vp, c_i64]),
+        "sdpadd_Rtrsm": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_dp,
+                                        vp, c_i64, vp, c_i64]),
+        "sdpadd_Rtrmm": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_dp,
+                                        vp, c_i64, vp, c_i64]),
+        "sdpadd_Rtrsv": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, vp, c_i64, vp, c_i64]),
+        "sdpadd_Rtrmv": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, vp, c_i64, vp, c_i64]),
+        "sdpadd_Rgemv": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_i64, c_dp, vp, c_i64, vp, c_i64, c_dp, vp, c_i64]),
+        "sdpadd_Rdot": (ctypes.c_int, [vp, c_i64, vp, c_i64, vp, c_i64, c_dp]),
+        "sdpadd_Rpotrf": (ctypes.c_int, [vp, ctypes.c_char, c_i64, vp, c_i64, ctypes.POINTER(c_i64)]),
+        "sdpadd_Rsteqr": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_dp, c_dp, c_dp, c_i64, c_dp, ctypes.POINTER(c_i64)]),
+        "sdpadd_lanczos_min_eig": (ctypes.c_int, [vp, c_i64, vp, vp, c_dp]),
+        "sdpadd_inv_chol": (ctypes.c_int, [vp, c_i64, vp, vp, vp, vp]),
+        "sdpadd_problem_load": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(vp)]),
+        "sdpadd_problem_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(vp)]),
+        "sdpadd_problem_set_entries": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, c_i64, ctypes.POINTER(ctypes.c_int),
+                                                      ctypes.POINTER(ctypes.c_int), c_dp]),
+        "sdpadd_problem_set_b": (ctypes.c_int, [vp, c_dp]),
+        "sdpadd_problem_finalize": (ctypes.c_int, [vp]),
+        "sdpadd_problem_destroy": (ctypes.c_int, [vp]),
+        "sdpadd_problem_dims": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
+        "sdpadd_problem_formula": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
+        "sdpadd_schur_build": (ctypes.c_int, [vp, vp, vp, vp]),
+        "sdpadd_inner_A": (ctypes.c_int, [vp, vp, vp]),
+        "sdpadd_plus_A": (ctypes.c_int, [vp, vp, vp]),
+        "sdpadd_read_params": (ctypes.c_int, [ctypes.c_char_p, vp]),
+        "sdpadd_default_params": (None, [vp]),
+        "sdpadd_solve": (ctypes.c_int, [vp, vp, vp]),
+        "sdpadd_trace": (ctypes.c_int, [vp, c_dp, ctypes.c_int]),
+        "sdpadd_solution": (ctypes.c_int, [vp, c_dp, c_dp, c_dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("maxIteration", ctypes.c_int)] + [(n, ctypes.c_double) for n in (
+        "epsilonStar", "lambdaStar", "omegaStar", "lowerBound", "upperBound", "betaStar", "betaBar", "gammaStar",
+        "epsilonDash")]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("phase", ctypes.c_int), ("phase_internal", ctypes.c_int)] + [
+        (n, ctypes.c_double * 2) for n in ("mu", "objValPrimal", "objValDual", "normPrimalVec", "normDualMat", "relgap")
+    ] + [(n, ctypes.c_double) for n in ("seconds", "t_schur", "t_chol", "t_lanczos", "t_inverse")]
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def dd_scalar(hi, lo=0.0):
+    return np.array([hi, lo], dtype=np.float64)
+
+
+class Device:
+    """A library handle (one GPU, one stream)."""
+
+    def __init__(self, device=0, dot_partition=1024):
+        self.lib = load_library()
+        self.h = ctypes.c_void_p()
+        rc = self.lib.sdpadd_create(device, ctypes.byref(self.h))
+        if rc != 0:
+            raise SdpaddError("sdpadd_create failed (%d): no usable GPU" % rc)
+        self.set_dot_partition(dot_partition)
+
+    def close(self):
+        if self.h:
+            self.lib.sdpadd_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, rc, what):
+        if rc < 0:
+            raise SdpaddError("%s failed (%d): %s" % (what, rc, self.lib.sdpadd_last_error(self.h).decode()))
+        return rc
+
+    def set_dot_partition(self, t):
+        self.dot_partition = t
+        self.check(self.lib.sdpadd_set_dot_partition(self.h, t), "set_dot_partition")
+
+    def stats(self, on=True):
+        self.check(self.lib.sdpadd_stats_enable(self.h, 1 if on else 0), "stats_enable")
+        self.check(self.lib.sdpadd_stats_reset(self.h), "stats_reset")
+
+    def stats_get(self, family):
+        n = ctypes.c_int64()
+        ms = np.zeros(1)
+        fl = np.zeros(1)
+        self.check(self.lib.sdpadd_stats_get(self.h, family.encode(), ctypes.byref(n), _dp(ms), _dp(fl)), "stats_get")
+        return {"launches": n.value, "ms": float(ms[0]), "ddflops": float(fl[0])}
+
+    # device memory -----------------------------------------------------
+    def to_device(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        p = ctypes.c_void_p()
+        self.check(self.lib.sdpadd_malloc(self.h, max(a.nbytes, 16), ctypes.byref(p)), "malloc")
+        if a.nbytes:
+            self.check(self.lib.sdpadd_upload(self.h, p, a.ctypes.data, a.nbytes), "upload")
+        return DeviceArray(self, p, a.shape)
+
+    def empty(self, shape):
+        return self.to_device(np.zeros(shape))
+
+
+class DeviceArray:
+    def __init__(self, dev, ptr, shape):
+        self.dev, self.ptr, self.shape = dev, ptr, tuple(shape)
+
+    @property
+    def nbytes(self):
+        return int(np.prod(self.shape)) * 8
+
+    def get(self):
+        out = np.empty(self.shape, dtype=np.float64)
+        if out.nbytes:
+            self.dev.check(self.dev.lib.sdpadd_download(self.dev.h, out.ctypes.data, self.ptr, out.nbytes), "download")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.dev.lib.sdpadd_free(self.dev.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---- BLAS-level helpers on host arrays (upload, run, download) ------------
+def cm(a):
+    """(rows, cols, 2) dd matrix -> column-major (rows*cols, 2) buffer."""
+    a = np.asarray(a, dtype=np.float64)
+    return np.ascontiguousarray(a.transpose(1, 0, 2)).reshape(-1, 2)
+
+
+def from_cm(buf, rows, cols):
+    return np.ascontiguousarray(np.asarray(buf).reshape(cols, rows, 2).transpose(1, 0, 2))
+
+
+def rgemm(dev, transb, A, B, C, alpha=(1.0, 0.0), beta=(0.0, 0.0)):
+    """C := alpha*A*op(B) + beta*C on dd matrices given as (r, c, 2) arrays."""
+    m, k = A.shape[:2]
+    n = B.shape[1] if transb == "N" else B.shape[0]
+    dA, dB, dC = dev.to_device(cm(A)), dev.to_device(cm(B)), dev.to_device(cm(C))
+    al, be = np.array(alpha, dtype=np.float64), np.array(beta, dtype=np.float64)
+    dev.check(dev.lib.sdpadd_Rgemm(dev.h, b"N", transb.encode(), m, n, k, _dp(al), dA.ptr, max(1, m), dB.ptr,
+                                   max(1, B.shape[0]), _dp(be), dC.ptr, max(1, m)), "Rgemm")
+    return from_cm(dC.get(), m, n)
+
+
+class Problem:
+    """A problem on the device, loaded from a .dat-s file or built in memory."""
+
+    def __init__(self, dev, path=None):
+        self.dev = dev
+        self.p = ctypes.c_void_p()
+        if path is not None:
+            dev.check(dev.lib.sdpadd_problem_load(dev.h, path.encode(), ctypes.byref(self.p)), "problem_load")
+        m, nb = ctypes.c_int(), ctypes.c_int()
+        bs = (ctypes.c_int * 64)()
+        dev.check(dev.lib.sdpadd_problem_dims(self.p, ctypes.byref(m), ctypes.byref(nb), bs, 64), "dims")
+        self.m, self.block_struct = m.value, [bs[i] for i in range(nb.value)]
+
+    @property
+    def nn(self):
+        return sum(n * n for n in self.block_struct)
+
+    def formula(self):
+        out = (ctypes.c_int * (self.m * len(self.block_struct)))()
+        self.dev.check(self.dev.lib.sdpadd_problem_formula(self.p, out), "formula")
+        return np.array(out[:], dtype=np.int32)
+
+    def schur(self, X, Zinv):
+        dX, dZ = self.dev.to_device(X), self.dev.to_device(Zinv)
+        dB = self.dev.empty((self.m * self.m, 2))
+        self.dev.check(self.dev.lib.sdpadd_schur_build(self.p, dX.ptr, dZ.ptr, dB.ptr), "schur_build")
+        return dB.get()
+
+    def solve(self, params=None):
+        prm = Params()
+        self.dev.lib.sdpadd_default_params(ctypes.byref(prm))
+        if isinstance(params, str):
+            if self.dev.lib.sdpadd_read_params(params.encode(), ctypes.byref(prm)) != 0:
+                raise SdpaddError("cannot read parameter file %s" % params)
+        elif isinstance(params, dict):
+            for k, v in params.items():
+                setattr(prm, k, v)
+        res = Result()
+        self.dev.check(self.dev.lib.sdpadd_solve(self.p, ctypes.byref(prm), ctypes.byref(res)), "solve")
+        out = {
+            "iterations": res.iterations, "phase": PHASES[res.phase], "phase_internal": PHASES[res.phase_internal],
+            "seconds": res.seconds, "t_schur": res.t_schur, "t_chol": res.t_chol, "t_lanczos": res.t_lanczos,
+            "t_inverse": res.t_inverse,
+        }
+        for k in ("mu", "objValPrimal", "objValDual", "normPrimalVec", "normDualMat", "relgap"):
+            out[k] = [getattr(res, k)[0], getattr(res, k)[1]]
+        cap = 2000
+        buf = np.zeros(17 * cap)
+        n = self.dev.lib.sdpadd_trace(self.p, _dp(buf), cap)
+        tr = []
+        keys = ["mu", "thetaP", "thetaD", "objP", "objD", "alphaP", "alphaD", "beta"]
+        for i in range(min(n, cap)):
+            rec = {"it": int(buf[17 * i])}
+            for q, k in enumerate(keys):
+                rec[k] = [buf[17 * i + 1 + 2 * q], buf[17 * i + 2 + 2 * q]]
+            tr.append(rec)
+        out["trace"] = tr
+        return out
+
+    def solution(self):
+        X = np.zeros((self.nn, 2))
+        Z = np.zeros((self.nn, 2))
+        y = np.zeros((self.m, 2))
+        self.dev.check(self.dev.lib.sdpadd_solution(self.p, _dp(X), _dp(y), _dp(Z)), "solution")
+        return X, y, Z
+
+    def close(self):
+        if self.p:
+            self.dev.lib.sdpadd_problem_destroy(self.p)
+            self.p = ctypes.c_void_p()
