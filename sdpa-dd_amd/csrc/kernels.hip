@@ -99,16 +99,19 @@ static inline unsigned cdiv(idx_t a, idx_t b) { return (unsigned)((a + b - 1) / 
 // ------------------------------------------------------------------ GEMM --
 // alpha_kind: 0 general, 1 alpha == 1 (alpha*b == b exactly), 2 alpha == -1.
 // beta_kind: 0 beta == 0, 1 beta == 1, 2 general.
-template <bool TRANSB, bool PTRS>
+// Tile (16*MT) x (16*MT) per 256-thread workgroup, MT x MT outputs per
+// thread strided by 16, k staged through LDS in chunks of TK.
+template <bool TRANSB, bool PTRS, int MT>
 __global__ __launch_bounds__(NT) void k_gemm(idx_t m, idx_t n, idx_t k, dd alpha, int alpha_kind, const dd *__restrict__ A,
                                              idx_t lda, const dd *__restrict__ B, idx_t ldb, dd beta, int beta_kind,
                                              dd *__restrict__ C, idx_t ldc, idx_t sA, idx_t sB, idx_t sC, int lower_only,
                                              idx_t diag_offset, const dd *const *Ap, const dd *const *Bp, dd *const *Cp) {
-    __shared__ dd As[TK][TB];
-    __shared__ dd Bs[TK][TB];
+    constexpr int T = 16 * MT;
+    __shared__ dd As[TK][T];
+    __shared__ dd Bs[TK][T];
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
-    const idx_t row0 = (idx_t)blockIdx.x * TB, col0 = (idx_t)blockIdx.y * TB;
+    const idx_t row0 = (idx_t)blockIdx.x * T, col0 = (idx_t)blockIdx.y * T;
     if (PTRS) {
         A = Ap[blockIdx.z];
         B = Bp[blockIdx.z];
@@ -119,13 +122,13 @@ __global__ __launch_bounds__(NT) void k_gemm(idx_t m, idx_t n, idx_t k, dd alpha
         C += (idx_t)blockIdx.z * sC;
     }
     // element (i,j) is "lower" when i + diag_offset >= j
-    if (lower_only && row0 + TB - 1 + diag_offset < col0) return;
+    if (lower_only && row0 + T - 1 + diag_offset < col0) return;
 
-    dd acc[4][4];
+    dd acc[MT][MT];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < MT; r++) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < MT; c++) {
             idx_t i = row0 + tx + 16 * r, j = col0 + ty + 16 * c;
             dd v = mk(0.0);
             if (i < m && j < n && beta_kind != 0) {
@@ -138,19 +141,19 @@ __global__ __launch_bounds__(NT) void k_gemm(idx_t m, idx_t n, idx_t k, dd alpha
     for (idx_t k0 = 0; k0 < k; k0 += TK) {
         const int kc = (int)((k - k0) < TK ? (k - k0) : TK);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < MT; q++) {
             int e = tid + NT * q;
-            int row = e & 63, kk = e >> 6;
+            int row = e % T, kk = e / T;
             idx_t i = row0 + row;
             As[kk][row] = (i < m && kk < kc) ? A[i + (k0 + kk) * lda] : mk(0.0);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < MT; q++) {
             int e = tid + NT * q;
             int kk, col;
             if (TRANSB) {
-                col = e & 63;
-                kk = e >> 6;
+                col = e % T;
+                kk = e / T;
             } else {
                 kk = e & 15;
                 col = e >> 4;
@@ -168,27 +171,32 @@ __global__ __launch_bounds__(NT) void k_gemm(idx_t m, idx_t n, idx_t k, dd alpha
         }
         __syncthreads();
         for (int kk = 0; kk < kc; kk++) {
-            dd a[4], b[4];
+            dd a[MT], b[MT];
 #pragma unroll
-            for (int r = 0; r < 4; r++) a[r] = As[kk][tx + 16 * r];
+            for (int r = 0; r < MT; r++) a[r] = As[kk][tx + 16 * r];
 #pragma unroll
-            for (int c = 0; c < 4; c++) b[c] = Bs[kk][ty + 16 * c];
+            for (int c = 0; c < MT; c++) b[c] = Bs[kk][ty + 16 * c];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
+            for (int r = 0; r < MT; r++) {
 #pragma unroll
-                for (int c = 0; c < 4; c++) acc[r][c] = dd_add_eq(acc[r][c], dd_mul(b[c], a[r]));
+                for (int c = 0; c < MT; c++) acc[r][c] = dd_add_eq(acc[r][c], dd_mul(b[c], a[r]));
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < MT; r++) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < MT; c++) {
             idx_t i = row0 + tx + 16 * r, j = col0 + ty + 16 * c;
             if (i < m && j < n && (!lower_only || i + diag_offset >= j)) C[i + j * ldc] = acc[r][c];
         }
     }
+}
+
+// 32x32 tiles when 64x64 tiles would leave most of the 256 CUs idle.
+static bool use_small_tiles(idx_t m, idx_t n, int batch) {
+    return (idx_t)cdiv(m, TB) * cdiv(n, TB) * batch < 1024;
 }
 
 static int kind_alpha(dd a) {
@@ -222,23 +230,41 @@ void gemm(hipStream_t s, char tb, idx_t m, idx_t n, idx_t k, dd alpha, const dd 
         k_gemm_scale_only<<<g, 256, 0, s>>>(m, n, beta, bk, C, ldc, sC);
         return;
     }
-    dim3 grid(cdiv(m, TB), cdiv(n, TB), (unsigned)batch);
     StatScope st("gemm", s, 2.0 * (double)m * n * k * batch * (lower_only ? 0.5 : 1.0));
-    if (tb == 'T' || tb == 't')
-        k_gemm<true, false><<<grid, NT, 0, s>>>(m, n, k, alpha, ak, A, lda, B, ldb, beta, bk, C, ldc, sA, sB, sC,
-                                                 lower_only ? 1 : 0, diag_offset, nullptr, nullptr, nullptr);
+    const int lo = lower_only ? 1 : 0;
+    const bool tr = (tb == 'T' || tb == 't');
+    if (use_small_tiles(m, n, batch)) {
+        dim3 grid(cdiv(m, 32), cdiv(n, 32), (unsigned)batch);
+        if (tr)
+            k_gemm<true, false, 2><<<grid, NT, 0, s>>>(m, n, k, alpha, ak, A, lda, B, ldb, beta, bk, C, ldc, sA, sB, sC, lo,
+                                                        diag_offset, nullptr, nullptr, nullptr);
+        else
+            k_gemm<false, false, 2><<<grid, NT, 0, s>>>(m, n, k, alpha, ak, A, lda, B, ldb, beta, bk, C, ldc, sA, sB, sC, lo,
+                                                         diag_offset, nullptr, nullptr, nullptr);
+        return;
+    }
+    dim3 grid(cdiv(m, TB), cdiv(n, TB), (unsigned)batch);
+    if (tr)
+        k_gemm<true, false, 4><<<grid, NT, 0, s>>>(m, n, k, alpha, ak, A, lda, B, ldb, beta, bk, C, ldc, sA, sB, sC, lo,
+                                                    diag_offset, nullptr, nullptr, nullptr);
     else
-        k_gemm<false, false><<<grid, NT, 0, s>>>(m, n, k, alpha, ak, A, lda, B, ldb, beta, bk, C, ldc, sA, sB, sC,
-                                                  lower_only ? 1 : 0, diag_offset, nullptr, nullptr, nullptr);
+        k_gemm<false, false, 4><<<grid, NT, 0, s>>>(m, n, k, alpha, ak, A, lda, B, ldb, beta, bk, C, ldc, sA, sB, sC, lo,
+                                                     diag_offset, nullptr, nullptr, nullptr);
 }
 
 void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx_t lda, const dd *const *B, idx_t ldb,
                dd *const *C, idx_t ldc, int batch) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
-    dim3 grid(cdiv(m, TB), cdiv(n, TB), (unsigned)batch);
     StatScope st("gemm", s, 2.0 * (double)m * n * k * batch);
-    k_gemm<false, true><<<grid, NT, 0, s>>>(m, n, k, mk(1.0), 1, nullptr, lda, nullptr, ldb, mk(0.0), 0, nullptr, ldc, 0, 0,
-                                             0, 0, 0, A, B, C);
+    if (use_small_tiles(m, n, batch)) {
+        dim3 grid(cdiv(m, 32), cdiv(n, 32), (unsigned)batch);
+        k_gemm<false, true, 2><<<grid, NT, 0, s>>>(m, n, k, mk(1.0), 1, nullptr, lda, nullptr, ldb, mk(0.0), 0, nullptr, ldc,
+                                                    0, 0, 0, 0, 0, A, B, C);
+        return;
+    }
+    dim3 grid(cdiv(m, TB), cdiv(n, TB), (unsigned)batch);
+    k_gemm<false, true, 4><<<grid, NT, 0, s>>>(m, n, k, mk(1.0), 1, nullptr, lda, nullptr, ldb, mk(0.0), 0, nullptr, ldc, 0, 0,
+                                                0, 0, 0, A, B, C);
 }
 
 // -------------------------------------------------------------- Cholesky --
@@ -716,34 +742,58 @@ __device__ __forceinline__ dd chunk_sum(const dd *x, const dd *y, idx_t start, i
     return s;
 }
 
-// One dot over T chunks spread across workgroups; the last workgroup to
-// finish adds the partial sums in chunk order.
-__global__ __launch_bounds__(256) void k_dot_big(idx_t n, const dd *x, const dd *y, dd *out, int T, dd *partial,
-                                                 unsigned *counter) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t < T) {
-        idx_t st, cnt;
-        chunk_range(n, T, t, st, cnt);
-        partial[t] = chunk_sum(x, y, st, cnt);
+// Long dots: one workgroup per (chunk, task).  All lanes form the products
+// x_i*y_i of a tile into LDS; lane 0 adds them in index order (the chunk's
+// running sum), so the per-chunk result is exactly Rdot_omp's thread-local
+// sum.  A second kernel adds the T partial sums of each task in chunk order.
+static constexpr int DOT_TILE = 1024;
+__global__ __launch_bounds__(256) void k_dot_chunks(const DotTask *tasks, int T, dd *partial) {
+    __shared__ dd prod[DOT_TILE];
+    const DotTask tk = tasks[blockIdx.y];
+    const int chunk = blockIdx.x;
+    idx_t st, cnt;
+    chunk_range(tk.n, T, chunk, st, cnt);
+    dd acc = mk(0.0);
+    for (idx_t t0 = 0; t0 < cnt; t0 += DOT_TILE) {
+        const int tl = (int)((cnt - t0) < DOT_TILE ? (cnt - t0) : DOT_TILE);
+        for (int e = threadIdx.x; e < tl; e += 256) prod[e] = dd_mul(tk.x[st + t0 + e], tk.y[st + t0 + e]);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int e = 0; e < tl; e++) acc = dd_add_eq(acc, prod[e]);
+        __syncthreads();
     }
-    __threadfence();
-    __shared__ unsigned ticket;
-    __syncthreads();
-    if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
-    __syncthreads();
-    if (ticket != gridDim.x - 1) return;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        dd acc = mk(0.0);
-        for (int u = 0; u < T; u++) acc = dd_add_eq(acc, partial[u]);
-        *out = acc;
-        *counter = 0u;
-    }
+    if (threadIdx.x == 0) partial[(idx_t)blockIdx.y * T + chunk] = acc;
+}
+__global__ void k_dot_combine(const DotTask *tasks, int ntasks, int T, const dd *partial) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ntasks) return;
+    dd acc = mk(0.0);
+    for (int c = 0; c < T; c++) acc = dd_add_eq(acc, partial[(idx_t)q * T + c]);
+    *tasks[q].out = acc;
+}
+void dot_tasks_long(hipStream_t s, const DotTask *d_tasks, int ntasks, int T, dd *partial) {
+    if (ntasks <= 0) return;
+    k_dot_chunks<<<dim3((unsigned)T, (unsigned)ntasks), 256, 0, s>>>(d_tasks, T, partial);
+    k_dot_combine<<<(ntasks + 63) / 64, 64, 0, s>>>(d_tasks, ntasks, T, partial);
+}
+__global__ void k_set_task(DotTask *t, const dd *x, const dd *y, idx_t n, dd *out) {
+    t->x = x;
+    t->y = y;
+    t->n = n;
+    t->out = out;
 }
 
 void dot(hipStream_t s, idx_t n, const dd *x, const dd *y, dd *out, int T, dd *work, unsigned *counter) {
     StatScope st("dot", s, 2.0 * (double)n);
-    k_dot_big<<<cdiv(T, 256), 256, 0, s>>>(n, x, y, out, T, work, counter);
+    // work holds the task record followed by T partial sums
+    DotTask *task = reinterpret_cast<DotTask *>(work);
+    dd *partial = work + 4;
+    (void)counter;
+    k_set_task<<<1, 1, 0, s>>>(task, x, y, n, out);
+    if (n / T <= 64)
+        dot_batch(s, task, 1, T);  // short chunks: one workgroup, a lane per chunk
+    else
+        dot_tasks_long(s, task, 1, T, partial);
 }
 
 // Batched dots: one workgroup per task, partial sums in LDS (T <= 4096).

@@ -41,62 +41,173 @@ void Context::destroy() {
 }
 
 namespace {
-// Lal::getMinEigen (sdpa_linear.cpp:27-122): Lanczos on L M L^T with start
-// vector of ones; the (k+1)-tridiagonal eigenproblem is solved on the host,
-// forming only the last row of its eigenvector matrix.
-static ddr lanczos_run(Context &ctx, int nDim, const dd *L, const dd *M, dd *lz_r, dd *lz_q, dd *lz_qold, dd *lz_w,
-                       dd *lz_tmp) {
-    hipStream_t s = ctx.stream;
-    ddr alpha, beta, value;
-    ddr min = 1.0e+51, min_old = 1.0e+52, min_min = 1.0e+50;
-    ddr error = 1.0e+10;
+// Lal::getMinEigen (sdpa_linear.cpp:27-122): Lanczos on L M L^T with a start
+// vector of ones.  Chains for different (block, X/Z) pairs are independent,
+// so they run lock-stepped on their own streams with one host
+// synchronisation per step for all of them.  The (k+1)-tridiagonal
+// eigenproblem is solved on the host, forming only the last row of its
+// eigenvector matrix (the one entry the estimate reads).
+struct Chain {
+    int n = 0, cap = 0;
+    const dd *L = nullptr, *M = nullptr;
+    dd *r = nullptr, *q = nullptr, *qold = nullptr, *w = nullptr, *tmp = nullptr;
+    dd *scal = nullptr;        // device: [alpha, r.r]
+    dd *dot_work = nullptr;    // partial sums of this chain's dots
+    unsigned *counter = nullptr;
+    hipStream_t s = nullptr;
+    dd *h = nullptr;           // pinned host copy of scal
+    // host state
+    ddr alpha, beta, value, min = 1.0e+51, min_old = 1.0e+52, error = 1.0e+10;
     int k = 0, kk = 0;
-    std::vector<ddr> diagVec(nDim, min_min), diagVec2(nDim, ddr(0.0)), out(nDim), bv(nDim), work(2 * nDim + 2);
-    std::vector<ddr> Q((size_t)nDim + 1);
-    sdk::fill(s, nDim, mk(0.0), lz_q);
-    sdk::fill(s, nDim, mk(1.0), lz_r);
-    beta = sqrt((ddr)nDim);
-    while (k < nDim && k < sqrt((ddr)nDim) + 10 && beta > 1.0e-16 &&
-           (abs(min - min_old) > (1.0e-5) * abs(min) + (1.0e-8) || abs(error * beta) > (1.0e-2) * abs(min) + (1.0e-4))) {
-        sdk::copy(s, nDim, lz_q, lz_qold);
-        value = ddr(1.0) / beta;
-        sdk::copy(s, nDim, lz_r, lz_q);
-        sdk::scal(s, nDim, value.v, lz_q);
-        sdk::trmv_lower(s, 'T', nDim, L, nDim, lz_q, lz_w);
-        sdk::gemv_n(s, nDim, nDim, mk(1.0), M, nDim, lz_w, mk(0.0), lz_tmp);
-        sdk::trmv_lower(s, 'N', nDim, L, nDim, lz_tmp, lz_w);
-        ctx.dot(nDim, lz_q, lz_w, 0);  // alpha, kept on the device for the update
-        sdk::copy(s, nDim, lz_w, lz_r);
-        sdk::axpy_dev(s, nDim, ctx.d_scal + 0, true, lz_q, lz_r);
-        sdk::axpy(s, nDim, dd_neg(beta.v), lz_qold, lz_r);
-        ctx.dot(nDim, lz_r, lz_r, 1);
-        dd h[2];
-        (void)hipMemcpyAsync(h, ctx.d_scal, sizeof(dd) * 2, hipMemcpyDeviceToHost, s);
-        (void)hipStreamSynchronize(s);
-        alpha = ddr(h[0]);
-        diagVec[k] = alpha;
-        if (kk >= sqrt((ddr)k) || k == nDim - 1 || k > sqrt((ddr)nDim + 9)) {
-            kk = 0;
-            out = diagVec;
-            bv = diagVec2;
-            out[nDim - 1] = diagVec[k];
-            bv[nDim - 1] = 0.0;
-            int kp1 = k + 1;
-            // only row k of the eigenvector matrix is needed: Q[k + 0*ldz]
-            std::vector<ddr> z((size_t)kp1 * kp1);
-            int64_t info = steqr_rows(kp1, out.data(), bv.data(), z.data(), kp1, work.data(), k, k + 1);
-            if (info > 0) break;
-            min_old = min;
-            min = out[0];
-            error = z[k];
-        }
-        value = ddr(h[1]);
-        beta = sqrt(value);
-        diagVec2[k] = beta;
-        ++k;
-        ++kk;
+    bool done = false;
+    std::vector<ddr> diagVec, diagVec2, out, bv, work;
+    ddr result;
+};
+
+struct ChainPool {
+    std::vector<Chain> chains;
+    std::vector<void *> dmem, hmem;
+    std::vector<hipStream_t> streams;
+    ~ChainPool() {
+        for (void *p : dmem) (void)hipFree(p);
+        for (void *p : hmem) (void)hipHostFree(p);
+        for (hipStream_t s : streams) (void)hipStreamDestroy(s);
     }
-    return min - abs(error * beta);
+    // make sure chains [0, cnt) exist with vectors of length >= n
+    bool ensure(int cnt, int n) {
+        while ((int)chains.size() < cnt) {
+            Chain c;
+            dd *v = nullptr;
+            if (hipMalloc((void **)&v, sizeof(dd) * (5 * (size_t)n + 2 + 4 + 4096)) != hipSuccess) return false;
+            dmem.push_back(v);
+            c.r = v;
+            c.q = v + n;
+            c.qold = v + 2 * (size_t)n;
+            c.w = v + 3 * (size_t)n;
+            c.tmp = v + 4 * (size_t)n;
+            c.scal = v + 5 * (size_t)n;
+            c.dot_work = v + 5 * (size_t)n + 2;
+            if (hipMalloc((void **)&c.counter, sizeof(unsigned) * 2) != hipSuccess) return false;
+            dmem.push_back(c.counter);
+            (void)hipMemset(c.counter, 0, sizeof(unsigned) * 2);
+            if (hipHostMalloc((void **)&c.h, sizeof(dd) * 2) != hipSuccess) return false;
+            hmem.push_back(c.h);
+            hipStream_t s;
+            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return false;
+            streams.push_back(s);
+            c.s = s;
+            c.n = c.cap = n;
+            chains.push_back(c);
+        }
+        for (int i = 0; i < cnt; i++)
+            if (chains[i].cap < n) return false;
+        return true;
+    }
+};
+
+bool chain_active(Chain &c) {
+    const int nDim = c.n;
+    return c.k < nDim && c.k < sqrt((ddr)nDim) + 10 && c.beta > 1.0e-16 &&
+           (abs(c.min - c.min_old) > (1.0e-5) * abs(c.min) + (1.0e-8) ||
+            abs(c.error * c.beta) > (1.0e-2) * abs(c.min) + (1.0e-4));
+}
+
+void chain_enqueue(Chain &c, int T) {
+    hipStream_t s = c.s;
+    const int nDim = c.n;
+    sdk::copy(s, nDim, c.q, c.qold);
+    c.value = ddr(1.0) / c.beta;
+    sdk::copy(s, nDim, c.r, c.q);
+    sdk::scal(s, nDim, c.value.v, c.q);
+    sdk::trmv_lower(s, 'T', nDim, c.L, nDim, c.q, c.w);
+    sdk::gemv_n(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
+    sdk::trmv_lower(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
+    sdk::dot(s, nDim, c.q, c.w, c.scal, T, c.dot_work, c.counter);  // alpha
+    sdk::copy(s, nDim, c.w, c.r);
+    sdk::axpy_dev(s, nDim, c.scal, true, c.q, c.r);                  // r -= alpha q
+    sdk::axpy(s, nDim, dd_neg(c.beta.v), c.qold, c.r);              // r -= beta qold
+    sdk::dot(s, nDim, c.r, c.r, c.scal + 1, T, c.dot_work, c.counter);
+    (void)hipMemcpyAsync(c.h, c.scal, sizeof(dd) * 2, hipMemcpyDeviceToHost, s);
+}
+
+void chain_finish_step(Chain &c) {
+    const int nDim = c.n, k = c.k;
+    c.alpha = ddr(c.h[0]);
+    c.diagVec[k] = c.alpha;
+    if (c.kk >= sqrt((ddr)k) || k == nDim - 1 || k > sqrt((ddr)nDim + 9)) {
+        c.kk = 0;
+        c.out = c.diagVec;
+        c.bv = c.diagVec2;
+        c.out[nDim - 1] = c.diagVec[k];
+        c.bv[nDim - 1] = 0.0;
+        const int kp1 = k + 1;
+        std::vector<ddr> z((size_t)kp1 * kp1);
+        int64_t info = steqr_rows(kp1, c.out.data(), c.bv.data(), z.data(), kp1, c.work.data(), k, k + 1);
+        if (info > 0) {  // "cannot converge": leave the loop with beta unchanged
+            c.done = true;
+            return;
+        }
+        c.min_old = c.min;
+        c.min = c.out[0];
+        c.error = z[k];
+    }
+    c.value = ddr(c.h[1]);
+    c.beta = sqrt(c.value);
+    c.diagVec2[k] = c.beta;
+    ++c.k;
+    ++c.kk;
+}
+
+// Runs chains[0..cnt) to completion; chain i computes lambda_min(L_i M_i L_i^T).
+void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent) {
+    hipEvent_t ready;
+    (void)hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+    (void)hipEventRecord(ready, parent);
+    for (int i = 0; i < cnt; i++) {
+        Chain &c = pool.chains[i];
+        (void)hipStreamWaitEvent(c.s, ready, 0);
+        const int nDim = c.n;
+        c.min = 1.0e+51;
+        c.min_old = 1.0e+52;
+        c.error = 1.0e+10;
+        c.k = c.kk = 0;
+        c.done = false;
+        c.diagVec.assign(nDim, ddr(1.0e+50));
+        c.diagVec2.assign(nDim, ddr(0.0));
+        c.out.assign(nDim, ddr(0.0));
+        c.bv.assign(nDim, ddr(0.0));
+        c.work.assign(2 * (size_t)nDim + 2, ddr(0.0));
+        sdk::fill(c.s, nDim, mk(0.0), c.q);
+        sdk::fill(c.s, nDim, mk(1.0), c.r);
+        c.beta = sqrt((ddr)nDim);
+    }
+    for (;;) {
+        int launched = 0;
+        for (int i = 0; i < cnt; i++) {
+            Chain &c = pool.chains[i];
+            if (c.done) continue;
+            if (!chain_active(c)) {
+                c.done = true;
+                continue;
+            }
+            chain_enqueue(c, T);
+            launched++;
+        }
+        if (!launched) break;
+        for (int i = 0; i < cnt; i++) {
+            Chain &c = pool.chains[i];
+            if (c.done) continue;
+            (void)hipStreamSynchronize(c.s);
+            chain_finish_step(c);
+        }
+    }
+    for (int i = 0; i < cnt; i++) {
+        Chain &c = pool.chains[i];
+        c.result = c.min - abs(c.error * c.beta);
+        (void)hipEventRecord(ready, c.s);
+        (void)hipStreamWaitEvent(parent, ready, 0);
+    }
+    (void)hipEventDestroy(ready);
 }
 
 }  // namespace
@@ -128,8 +239,6 @@ class Solver {
     // iterates and work spaces (concatenated SDP blocks)
     dd *X, *Z, *invCholX, *invCholZ, *invZ, *DxMat, *DzMat, *r_zinv, *x_rd_zinv, *DLS1, *DLS2, *dualMat;
     dd *y, *dy, *g, *primalVec, *bMat;
-    // Lanczos vectors
-    dd *lz_r, *lz_q, *lz_qold, *lz_w, *lz_tmp;
     std::vector<dd> diag_buf;
     double t_schur = 0, t_chol = 0, t_solve = 0, t_lanczos = 0, t_inverse = 0;
 
@@ -260,16 +369,36 @@ class Solver {
         t_inverse += now() - t0;
         return true;
     }
-    ddr lanczos(int n, const dd *L, const dd *M) { return lanczos_run(ctx, n, L, M, lz_r, lz_q, lz_qold, lz_w, lz_tmp); }
-    ddr min_eigen(const dd *L, const dd *M) {
+    ChainPool pool;
+    // Jal::getMinEigen (sdpa_jordan.cpp:55-88) for (L_X^{-1}, dX) and (L_Z^{-1}, dZ):
+    // all blocks of both pairs at once.
+    void min_eigen_xz(ddr &mx, ddr &mz) {
         double t0 = now();
-        ddr mn = 1.0E50;
+        int maxn = 0;
+        for (int l = 0; l < nB; l++) maxn = std::max(maxn, bsz(l));
+        if (!pool.ensure(2 * nB, maxn)) {
+            ctx.set_error("Lanczos workspace allocation failed");
+            mx = mz = ddr(NAN);
+            return;
+        }
         for (int l = 0; l < nB; l++) {
-            ddr v = lanczos(bsz(l), L + dp.boff[l], M + dp.boff[l]);
-            if (v < mn) mn = v;
+            Chain &cx = pool.chains[l];
+            cx.n = bsz(l);
+            cx.L = invCholX + dp.boff[l];
+            cx.M = DxMat + dp.boff[l];
+            Chain &cz = pool.chains[nB + l];
+            cz.n = bsz(l);
+            cz.L = invCholZ + dp.boff[l];
+            cz.M = DzMat + dp.boff[l];
+        }
+        run_chains(pool, 2 * nB, ctx.dot_T, s);
+        mx = 1.0E50;
+        mz = 1.0E50;
+        for (int l = 0; l < nB; l++) {
+            if (pool.chains[l].result < mx) mx = pool.chains[l].result;
+            if (pool.chains[nB + l].result < mz) mz = pool.chains[nB + l].result;
         }
         t_lanczos += now() - t0;
-        return mn;
     }
     void diag_sums(double lam, ddr &x0z, ddr &xz0) {
         // x0zMat += lambda * Z_ii, xMatz0 += lambda * X_ii, blocks in order
@@ -303,9 +432,6 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         if (!(*b = alloc(nn))) return -2;
     if (!(y = alloc(m)) || !(dy = alloc(m)) || !(g = alloc(m)) || !(primalVec = alloc(m)) ||
         !(bMat = alloc((idx_t)m * m)))
-        return -2;
-    if (!(lz_r = alloc(maxn)) || !(lz_q = alloc(maxn)) || !(lz_qold = alloc(maxn)) || !(lz_w = alloc(maxn)) ||
-        !(lz_tmp = alloc(maxn)))
         return -2;
     double t_start = now();
     for (dd **b : bufs) sdk::fill(s, nn, mk(0.0), *b);
@@ -488,9 +614,9 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         {
             const ddr xi = 3.0;
             ddr alphaBD = 100.0;
-            ddr mx = min_eigen(invCholX, DxMat);
+            ddr mx, mz;
+            min_eigen_xz(mx, mz);
             alphaP = (-mx > 1.0 / alphaBD) ? -1.0 / mx : alphaBD;
-            ddr mz = min_eigen(invCholZ, DzMat);
             alphaD = (-mz > 1.0 / alphaBD) ? -1.0 / mz : alphaBD;
             alphaP = prm.gammaStar * alphaP;
             alphaD = prm.gammaStar * alphaD;
@@ -646,11 +772,14 @@ int solve(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const Params &
 }
 
 dd lanczos_min_eigen(Context &ctx, int n, const dd *d_L, const dd *d_M) {
-    dd *v = nullptr;
-    if (hipMalloc((void **)&v, sizeof(dd) * 5 * (size_t)n) != hipSuccess) return dd{NAN, NAN};
-    ddr r = lanczos_run(ctx, n, d_L, d_M, v, v + n, v + 2 * (size_t)n, v + 3 * (size_t)n, v + 4 * (size_t)n);
-    (void)hipFree(v);
-    return r.v;
+    ChainPool pool;
+    if (!pool.ensure(1, n)) return dd{NAN, NAN};
+    pool.chains[0].n = n;
+    pool.chains[0].L = d_L;
+    pool.chains[0].M = d_M;
+    run_chains(pool, 1, ctx.dot_T, ctx.stream);
+    (void)hipStreamSynchronize(ctx.stream);
+    return pool.chains[0].result.v;
 }
 
 }  // namespace sdp

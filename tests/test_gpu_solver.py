@@ -3,9 +3,10 @@
 * With the same dot partition (1024 chunks) the whole trajectory is
   bit-identical to the oracle: every per-iteration scalar and the final
   objectives, residual norms and iterates.
-* Against the single-threaded reference order (partition 1) the path must
-  agree on iteration count and phase, objectives to >= 28 significant digits
-  and feasibility norms within 1e-28.
+* With the partition set to one chunk (the reference's single-threaded
+  order) the trajectory is bit-identical to that order too, so iteration
+  count and phase agree, objectives to >= 28 significant digits and
+  feasibility norms within 1e-28.
 * On SDPLIB problems the reference's own result files pin iteration count,
   phase, the printed per-iteration trace and the objectives' high words.
 """
@@ -79,16 +80,25 @@ def test_sdplib_against_reference_results(gpu, name):
     assert "%.16e" % (-got["objValPrimal"][0]) == "%.16e" % objd
 
 
-def test_random_dense_against_single_thread_order(gpu, oracle, tmp_path):
-    """Workload-0 shape (n=40, m=50, dense A_i): 28-digit agreement with the
-    reference's deterministic single-threaded order."""
+def test_random_dense_single_thread_order_bitwise(gpu, oracle, tmp_path):
+    """Workload-0 shape (n=40, m=50, dense A_i) with the dot partition set to
+    one chunk, the reference's deterministic single-threaded order: the whole
+    trajectory is bit-identical to it, which implies the iteration count and
+    phase, objectives to >= 28 digits and feasibility within 1e-28."""
     from sdpadd import gen
     data = gen.random_dense(str(tmp_path / "c0.dat-s"))
     param = gen.param_file(str(tmp_path / "p.sdpa"), epsilon_star=1e-25, epsilon_dash=1e-25)
     ref = oracle.solve(data, param, dot_threads=1)
-    got = gpu_solve(gpu, data, param)
+    gpu.set_dot_partition(1)
+    try:
+        got = gpu_solve(gpu, data, param)
+    finally:
+        gpu.set_dot_partition(1024)
     assert got["iterations"] == ref["iterations"]
     assert got["phase"] == ref["phase"]
+    for a, b in zip(got["trace"], ref["trace"]):
+        for k in KEYS:
+            assert same(a[k], b[k]), (a["it"], k)
     for k in ("objValPrimal", "objValDual"):
         assert digits_agree(got[k], ref[k]) >= 28, k
     for k in ("normPrimalVec", "normDualMat"):
