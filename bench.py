@@ -58,7 +58,9 @@ def cpu_baseline(data, param, iterations):
     cores = min(16, os.cpu_count() or 1)
     env = dict(os.environ, OMP_NUM_THREADS=str(cores))
     t0 = time.time()
-    r = subprocess.run([exe, data, param, "--max-iter", "1"], capture_output=True, text=True, env=env, timeout=900)
+    import sdpadd
+    r = subprocess.run([exe, data, param, "--max-iter", "1", "--dot-threads", str(sdpadd.DEFAULT_DOT_PARTITION)],
+                       capture_output=True, text=True, env=env, timeout=900)
     wall = time.time() - t0
     out = json.loads(r.stdout)
     per_it = out["seconds"]

@@ -31,7 +31,7 @@ int sdpadd_destroy(sdpadd_handle h);
 const char *sdpadd_last_error(sdpadd_handle h);
 /* Number of chunks of the static partition that unit-stride dot products
  * sum over (mplapack/Rdot_omp.cpp:80-120 with that many OpenMP threads,
- * partial sums added in thread order).  Default 1024. */
+ * partial sums added in thread order).  1 <= chunks <= 2048; default 64. */
 int sdpadd_set_dot_partition(sdpadd_handle h, int chunks);
 int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr);
 int sdpadd_free(sdpadd_handle h, void *d_ptr);

@@ -63,7 +63,7 @@ int sdpadd_destroy(sdpadd_handle h) {
 }
 const char *sdpadd_last_error(sdpadd_handle h) { return h ? h->c.last_error.c_str() : "null handle"; }
 int sdpadd_set_dot_partition(sdpadd_handle h, int chunks) {
-    if (!h || chunks < 1 || chunks > 4096) return bad(h, "dot partition must be in [1, 4096]");
+    if (!h || chunks < 1 || chunks > 2048) return bad(h, "dot partition must be in [1, 2048]");
     h->c.dot_T = chunks;
     return 0;
 }

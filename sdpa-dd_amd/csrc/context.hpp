@@ -12,7 +12,7 @@ namespace sdp {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
-    int dot_T = 1024;           // chunks of the static dot partition
+    int dot_T = 64;             // chunks of the static dot partition
     dd *d_dot_work = nullptr;   // >= dot_T dd
     unsigned *d_counter = nullptr;
     dd *d_scal = nullptr;       // scalar slots for dot results

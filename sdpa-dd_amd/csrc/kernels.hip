@@ -378,41 +378,103 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
 }
 
 // ------------------------------------------------------ triangular solves --
-__global__ __launch_bounds__(1024) void k_trsv_ln(idx_t n, const dd *L, idx_t lda, dd *x) {
-    __shared__ dd xj;
-    __shared__ int nz;
+// Triangular solves with one 256-thread workgroup, blocked by SB rows.  Per
+// block, wave 0 solves the diagonal block with one row per lane and the
+// solved value broadcast by lane shuffles (no barriers), then every thread
+// applies the block's SB updates to one of the remaining rows, in the
+// reference's column order.  Each element sees exactly the operation
+// sequence of the column loop of Rtrsv.
+static constexpr int SB = 32;
+__device__ __forceinline__ dd shfl_dd(dd v, int src) {
+    dd r;
+    r.hi = __shfl(v.hi, src, 64);
+    r.lo = __shfl(v.lo, src, 64);
+    return r;
+}
+
+// x := L^{-1} x ('L','N','N'): for j ascending, if x_j != 0 { x_j /= L_jj;
+// x_i -= x_j * L_ij for i > j }.
+__global__ __launch_bounds__(256) void k_trsv_ln(idx_t n, const dd *L, idx_t lda, dd *x) {
+    __shared__ dd D[SB][SB + 1];
+    __shared__ dd v[SB];
+    __shared__ int nzv[SB];
     const int tid = threadIdx.x;
-    for (idx_t j = 0; j < n; j++) {
-        if (tid == 0) {
-            dd v = x[j];
-            nz = !dd_is_zero(v);
-            if (nz) {
-                v = dd_div(v, L[j + j * lda]);
-                x[j] = v;
-            }
-            xj = v;
+    for (idx_t b = 0; b < n; b += SB) {
+        const int nb = (int)((n - b) < SB ? (n - b) : SB);
+        for (int e = tid; e < SB * SB; e += 256) {
+            int r = e % SB, c = e / SB;
+            if (r < nb && c < nb && r >= c) D[r][c] = L[(b + r) + (b + c) * lda];
         }
         __syncthreads();
-        if (nz) {
-            dd t = xj;
-            for (idx_t i = j + 1 + tid; i < n; i += 1024) x[i] = dd_sub(x[i], dd_mul(t, L[i + j * lda]));
+        if (tid < 64) {
+            const int lane = tid;
+            dd xr = lane < nb ? x[b + lane] : mk(0.0);
+            for (int jj = 0; jj < nb; jj++) {
+                int nz = !dd_is_zero(xr);
+                if (lane == jj && nz) xr = dd_div(xr, D[jj][jj]);
+                const dd t = shfl_dd(xr, jj);
+                nz = __shfl(nz, jj, 64);
+                if (nz && lane > jj && lane < nb) xr = dd_sub(xr, dd_mul(t, D[lane][jj]));
+                if (lane == jj) {
+                    v[jj] = xr;
+                    nzv[jj] = nz;
+                }
+            }
+            if (lane < nb) x[b + lane] = xr;
+        }
+        __syncthreads();
+        for (idx_t i = b + nb + tid; i < n; i += 256) {
+            dd a[SB];
+#pragma unroll
+            for (int jj = 0; jj < SB; jj++) a[jj] = L[i + (b + (jj < nb ? jj : 0)) * lda];
+            dd xi = x[i];
+#pragma unroll
+            for (int jj = 0; jj < SB; jj++)
+                if (jj < nb && nzv[jj]) xi = dd_sub(xi, dd_mul(v[jj], a[jj]));
+            x[i] = xi;
         }
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(1024) void k_trsv_lt(idx_t n, const dd *L, idx_t lda, dd *x) {
-    __shared__ dd xj;
+// x := L^{-T} x ('L','T','N'): for j descending, x_j = (x_j - sum over
+// i > j, descending, of L_ij x_i) / L_jj.
+__global__ __launch_bounds__(256) void k_trsv_lt(idx_t n, const dd *L, idx_t lda, dd *x) {
+    __shared__ dd D[SB][SB + 1];
+    __shared__ dd v[SB];
     const int tid = threadIdx.x;
-    for (idx_t j = n - 1; j >= 0; j--) {
-        if (tid == 0) {
-            dd v = dd_div(x[j], L[j + j * lda]);
-            x[j] = v;
-            xj = v;
+    for (idx_t top = n; top > 0; top -= SB) {
+        const idx_t b = top > SB ? top - SB : 0;
+        const int nb = (int)(top - b);
+        for (int e = tid; e < SB * SB; e += 256) {
+            int r = e % SB, c = e / SB;
+            if (r < nb && c < nb && r >= c) D[r][c] = L[(b + r) + (b + c) * lda];
         }
         __syncthreads();
-        dd t = xj;
-        for (idx_t k = tid; k < j; k += 1024) x[k] = dd_sub(x[k], dd_mul(L[j + k * lda], t));
+        if (tid < 64) {
+            const int lane = tid;
+            dd xr = lane < nb ? x[b + lane] : mk(0.0);
+            for (int jj = nb - 1; jj >= 0; jj--) {
+                if (lane == jj) xr = dd_div(xr, D[jj][jj]);
+                const dd t = shfl_dd(xr, jj);
+                if (lane < jj) xr = dd_sub(xr, dd_mul(D[jj][lane], t));
+            }
+            if (lane < nb) {
+                x[b + lane] = xr;
+                v[lane] = xr;
+            }
+        }
+        __syncthreads();
+        for (idx_t k = tid; k < b; k += 256) {
+            dd a[SB];
+#pragma unroll
+            for (int jj = 0; jj < SB; jj++) a[jj] = L[(b + (jj < nb ? jj : 0)) + k * lda];
+            dd xk = x[k];
+#pragma unroll
+            for (int jj = SB - 1; jj >= 0; jj--)
+                if (jj < nb) xk = dd_sub(xk, dd_mul(a[jj], v[jj]));
+            x[k] = xk;
+        }
         __syncthreads();
     }
 }
@@ -421,9 +483,9 @@ void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *
     if (n <= 0) return;
     StatScope st("trsv", s, (double)n * n);
     if (trans == 'N' || trans == 'n')
-        k_trsv_ln<<<1, 1024, 0, s>>>(n, L, lda, x);
+        k_trsv_ln<<<1, 256, 0, s>>>(n, L, lda, x);
     else
-        k_trsv_lt<<<1, 1024, 0, s>>>(n, L, lda, x);
+        k_trsv_lt<<<1, 256, 0, s>>>(n, L, lda, x);
 }
 
 // ---------------------------------------------- inverse of a lower triangle --
@@ -613,26 +675,117 @@ void syrk_ln(hipStream_t s, idx_t n, idx_t k, dd alpha, const dd *A, idx_t lda, 
 }
 
 // ---------------------------------------------------------------- mat-vec --
-// w_j = x_j*L_jj + sum_{i>j} L_ij*x_i, i ascending (Rtrmv 'L','T').
-__global__ void k_trmv_lt(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
-    idx_t j = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    dd t = dd_mul(x[j], L[j + j * lda]);
-    const dd *col = L + j * lda;
-    for (idx_t i = j + 1; i < n; i++) t = dd_add_eq(t, dd_mul(col[i], x[i]));
-    w[j] = t;
+// Mat-vecs inside Lanczos.  Each output is one dependent chain of dd
+// additions in the reference's order, so a lane owns one output and the
+// kernel is latency-bound on that chain: the matrix tile for the next U
+// terms is loaded (and its products formed) while the current U additions
+// run.  The vector entries of a tile are the same for every lane; lanes
+// 0..U-1 load them one tile ahead with vector loads and publish them in
+// LDS.  One wave per workgroup.
+static constexpr int MV_U = 16;
+
+// w_j = x_j*L_jj, then += L_ij*x_i for i = j+1 .. n-1 ascending (Rtrmv 'L','T').
+// Lane j owns column j; rows [i0, i0+16) x columns [j0, j0+64) are staged
+// through LDS so that global reads are 256-byte column segments.
+__global__ __launch_bounds__(64) void k_trmv_lt(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    __shared__ dd sm[64 * (MV_U + 1)];
+    __shared__ dd xs[MV_U];
+    const int lane = threadIdx.x;
+    const idx_t j0 = (idx_t)blockIdx.x * 64, j = j0 + lane;
+    const idx_t jc = j < n ? j : n - 1;
+    dd t = dd_mul(x[jc], L[jc + jc * lda]);
+    // staging pattern: row r = lane % 16, columns lane / 16 + 4 q
+    const int r_ld = lane % MV_U, c_ld = lane / MV_U;
+    dd g[MV_U];
+    dd xn = mk(0.0);
+    auto fetch = [&](idx_t i0) {
+        idx_t ir = i0 + r_ld;
+        ir = ir < n ? ir : n - 1;
+#pragma unroll
+        for (int q = 0; q < MV_U; q++) {
+            idx_t c = j0 + c_ld + 4 * q;
+            c = c < n ? c : n - 1;
+            g[q] = L[ir + c * lda];
+        }
+        if (lane < MV_U) xn = x[ir];
+    };
+    // fetches are unconditional (indices clamped): the next tile is always
+    // in flight while the current chain runs
+    idx_t i0 = j0 + 1;
+    fetch(i0);
+    while (i0 < n) {
+#pragma unroll
+        for (int q = 0; q < MV_U; q++) sm[(c_ld + 4 * q) * (MV_U + 1) + r_ld] = g[q];
+        if (lane < MV_U) xs[lane] = xn;
+        __syncthreads();
+        const idx_t i1 = i0 + MV_U;
+        fetch(i1);
+        dd p[MV_U];
+#pragma unroll
+        for (int r = 0; r < MV_U; r++) p[r] = dd_mul(sm[lane * (MV_U + 1) + r], xs[r]);
+        if (i0 > j && i1 <= n) {
+#pragma unroll
+            for (int r = 0; r < MV_U; r++) t = dd_add_eq(t, p[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < MV_U; r++) {
+                idx_t i = i0 + r;
+                if (i < n && i > j) t = dd_add_eq(t, p[r]);
+            }
+        }
+        __syncthreads();
+        i0 = i1;
+    }
+    if (j < n) w[j] = t;
 }
 // w_i = x_i*L_ii, then += x_j*L_ij for j = i-1 down to 0, skipping x_j == 0 (Rtrmv 'L','N').
-__global__ void k_trmv_ln(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
-    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    dd xi = x[i];
-    dd t = dd_is_zero(xi) ? xi : dd_mul(xi, L[i + i * lda]);
-    for (idx_t j = i - 1; j >= 0; j--) {
-        dd xj = x[j];
-        if (!dd_is_zero(xj)) t = dd_add_eq(t, dd_mul(xj, L[i + j * lda]));
+__global__ __launch_bounds__(64) void k_trmv_ln(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    __shared__ dd xs[MV_U];
+    const int lane = threadIdx.x;
+    const idx_t i0 = (idx_t)blockIdx.x * 64, i = i0 + lane;
+    const idx_t ic = i < n ? i : n - 1;
+    const dd *Li = L + ic;
+    dd xi = x[ic];
+    dd t = dd_is_zero(xi) ? xi : dd_mul(xi, Li[ic * lda]);
+    // columns jt, jt-1, ..., jt-U+1 per tile, from the wave's largest row - 1 down to 0
+    idx_t jt = (i0 + 63 < n ? i0 + 63 : n - 1) - 1;
+    dd a[MV_U];
+    dd xn = mk(0.0);
+    auto fetch = [&](idx_t top) {
+#pragma unroll
+        for (int u = 0; u < MV_U; u++) {
+            idx_t jj = top - u;
+            a[u] = Li[(jj >= 0 ? jj : 0) * lda];
+        }
+        if (lane < MV_U) xn = x[top - lane >= 0 ? top - lane : 0];
+    };
+    fetch(jt);  // unconditional (indices clamped)
+    while (jt >= 0) {
+        if (lane < MV_U) xs[lane] = xn;
+        __syncthreads();
+        dd p[MV_U], xv[MV_U];
+#pragma unroll
+        for (int u = 0; u < MV_U; u++) {
+            xv[u] = xs[u];
+            p[u] = dd_mul(xv[u], a[u]);
+        }
+        const idx_t jn = jt - MV_U;
+        fetch(jn);
+        if (jt < i && jn >= -1) {
+#pragma unroll
+            for (int u = 0; u < MV_U; u++)
+                if (!dd_is_zero(xv[u])) t = dd_add_eq(t, p[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < MV_U; u++) {
+                idx_t jj = jt - u;
+                if (jj >= 0 && jj < i && !dd_is_zero(xv[u])) t = dd_add_eq(t, p[u]);
+            }
+        }
+        __syncthreads();
+        jt = jn;
     }
-    w[i] = t;
+    if (i < n) w[i] = t;
 }
 void trmv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, const dd *x, dd *out) {
     if (n <= 0) return;
@@ -643,16 +796,46 @@ void trmv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, cons
         k_trmv_ln<<<cdiv(n, 64), 64, 0, s>>>(n, L, lda, x, out);
 }
 // y_i = (beta handling) then += (alpha*x_j)*A_ij, j ascending (Rgemv 'N').
-__global__ void k_gemv_n(idx_t m, idx_t n, dd alpha, int ak, const dd *A, idx_t lda, const dd *x, dd beta, int bk, dd *y) {
-    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    dd t = (bk == 0) ? mk(0.0) : (bk == 1 ? y[i] : dd_mul(beta, y[i]));
-    for (idx_t j = 0; j < n; j++) {
-        dd xj = x[j];
-        dd tmp = (ak == 1) ? xj : (ak == 2 ? dd_neg(xj) : dd_mul(alpha, xj));
-        t = dd_add_eq(t, dd_mul(tmp, A[i + j * lda]));
+__global__ __launch_bounds__(64) void k_gemv_n(idx_t m, idx_t n, dd alpha, int ak, const dd *A, idx_t lda, const dd *x, dd beta, int bk,
+                                               dd *y) {
+    __shared__ dd xs[MV_U];
+    const int lane = threadIdx.x;
+    const idx_t i = (idx_t)blockIdx.x * 64 + lane;
+    const idx_t ic = i < m ? i : m - 1;
+    const dd *Ai = A + ic;
+    dd t = (bk == 0) ? mk(0.0) : (bk == 1 ? y[ic] : dd_mul(beta, y[ic]));
+    dd a[MV_U];
+    dd xn = mk(0.0);
+    auto fetch = [&](idx_t j0) {
+#pragma unroll
+        for (int u = 0; u < MV_U; u++) {
+            idx_t jj = j0 + u;
+            a[u] = Ai[(jj < n ? jj : n - 1) * lda];
+        }
+        if (lane < MV_U) xn = x[j0 + lane < n ? j0 + lane : n - 1];
+    };
+    idx_t j0 = 0;
+    fetch(0);
+    while (j0 < n) {
+        if (lane < MV_U) xs[lane] = (ak == 1) ? xn : (ak == 2 ? dd_neg(xn) : dd_mul(alpha, xn));
+        __syncthreads();
+        dd p[MV_U];
+#pragma unroll
+        for (int u = 0; u < MV_U; u++) p[u] = dd_mul(xs[u], a[u]);
+        const idx_t j1 = j0 + MV_U;
+        fetch(j1);  // unconditional (indices clamped)
+        if (j1 <= n) {
+#pragma unroll
+            for (int u = 0; u < MV_U; u++) t = dd_add_eq(t, p[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < MV_U; u++)
+                if (j0 + u < n) t = dd_add_eq(t, p[u]);
+        }
+        __syncthreads();
+        j0 = j1;
     }
-    y[i] = t;
+    if (i < m) y[i] = t;
 }
 void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y) {
     if (m <= 0) return;
@@ -833,6 +1016,76 @@ __global__ __launch_bounds__(256) void k_dot_batch_off(const DotTaskOff *tasks, 
 }
 void dot_batch_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T) {
     if (ntasks > 0) k_dot_batch_off<<<ntasks, 256, sizeof(dd) * T, s>>>(d_tasks, ybase, T);
+}
+
+// out_i = A_ii for an n x n column-major block.
+__global__ void k_gather_diag(idx_t n, const dd *A, dd *out) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = A[i + i * n];
+}
+void gather_diag(hipStream_t s, idx_t n, const dd *A, dd *out) {
+    if (n > 0) k_gather_diag<<<cdiv(n, 256), 256, 0, s>>>(n, A, out);
+}
+
+// Lanczos vector updates (Lal::getMinEigen, sdpa_linear.cpp:64-82), fused:
+// qold = q; q = v*r (Rcopy, Rcopy, Rscal).
+__global__ void k_lanczos_q(idx_t n, dd v, const dd *r, dd *q, dd *qold) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        qold[i] = q[i];
+        q[i] = dd_mul(v, r[i]);
+    }
+}
+void lanczos_q(hipStream_t s, idx_t n, dd v, const dd *r, dd *q, dd *qold) {
+    if (n > 0) k_lanczos_q<<<cdiv(n, 256), 256, 0, s>>>(n, v, r, q, qold);
+}
+// alpha = q.w; r = w; r += (-alpha) q; r += (-beta) qold (Raxpy skips a
+// zero scalar); rr = r.r.  Dots in the T-chunk order; out = {alpha, rr}.
+// One workgroup: thread t owns chunk t of both dots and the elements of
+// that chunk in the update.
+__global__ __launch_bounds__(256) void k_lanczos_tail(idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd nbeta,
+                                                      dd *out) {
+    extern __shared__ dd part[];
+    __shared__ dd s_alpha;
+    for (int t = threadIdx.x; t < T; t += 256) {
+        idx_t st, cnt;
+        chunk_range(n, T, t, st, cnt);
+        part[t] = chunk_sum(q, w, st, cnt);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        dd acc = mk(0.0);
+        for (int t = 0; t < T; t++) acc = dd_add_eq(acc, part[t]);
+        s_alpha = acc;
+        out[0] = acc;
+    }
+    __syncthreads();
+    const dd na = dd_neg(s_alpha);
+    const bool za = dd_is_zero(na), zb = dd_is_zero(nbeta);
+    __syncthreads();
+    for (int t = threadIdx.x; t < T; t += 256) {
+        idx_t st, cnt;
+        chunk_range(n, T, t, st, cnt);
+        dd acc = mk(0.0);
+        for (idx_t i = st; i < st + cnt; i++) {
+            dd ri = w[i];
+            if (!za) ri = dd_add_eq(ri, dd_mul(na, q[i]));
+            if (!zb) ri = dd_add_eq(ri, dd_mul(nbeta, qold[i]));
+            r[i] = ri;
+            acc = dd_add_eq(acc, dd_mul(ri, ri));
+        }
+        part[t] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        dd acc = mk(0.0);
+        for (int t = 0; t < T; t++) acc = dd_add_eq(acc, part[t]);
+        out[1] = acc;
+    }
+}
+void lanczos_tail(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd nbeta, dd *out) {
+    StatScope st("dot", s, 6.0 * (double)n);
+    k_lanczos_tail<<<1, 256, sizeof(dd) * T, s>>>(n, T, q, qold, w, r, nbeta, out);
 }
 
 // Max of |x_i|: per-thread running max, tree merge, last workgroup merges.

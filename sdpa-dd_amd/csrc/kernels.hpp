@@ -75,6 +75,12 @@ struct DotTaskOff {
     dd *out;
 };
 void dot_batch_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T);
+// out_i = A_ii of an n x n column-major block.
+void gather_diag(hipStream_t s, idx_t n, const dd *A, dd *out);
+// Lanczos step pieces (Lal::getMinEigen): qold = q, q = v*r; and the tail
+// alpha = q.w, r = w - alpha q - beta qold, rr = r.r with out = {alpha, rr}.
+void lanczos_q(hipStream_t s, idx_t n, dd v, const dd *r, dd *q, dd *qold);
+void lanczos_tail(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd nbeta, dd *out);
 // Batched GEMM over pointer arrays (device arrays of batch pointers):
 // C[b] := A[b]*B[b] (alpha = 1, beta = 0), order as gemm().
 void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx_t lda, const dd *const *B, idx_t ldb,

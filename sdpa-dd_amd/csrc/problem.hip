@@ -655,14 +655,43 @@ __device__ dd sparse_inner(const int *row, const int *col, const dd *val, int nn
     return ret;
 }
 
-__global__ void k_sparse_tasks(int ntask, const int *type, const int *block, const int *nnz, const idx_t *off,
-                               const int *row, const int *col, const dd *val, const idx_t *boff, const int *bsize,
-                               const dd *W, dd *out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntask || type[t] != 0) return;
-    int l = block[t];
-    idx_t o = off[t];
-    out[t] = sparse_inner(row + o, col + o, val + o, nnz[t], W + boff[l], bsize[l]);
+// One wave per task.  The terms (and the four-term groups of
+// getInnerProduct's unrolled loop) are independent, so the lanes form them;
+// lane 0 then adds the leading single terms and the group sums in order.
+__global__ __launch_bounds__(64) void k_sparse_tasks(int ntask, const int *type, const int *block, const int *nnz,
+                                                     const idx_t *off, const int *row, const int *col, const dd *val,
+                                                     const idx_t *boff, const int *bsize, const dd *W, dd *out) {
+    __shared__ dd g[256];
+    const int t = blockIdx.x, lane = threadIdx.x;
+    if (type[t] != 0) return;
+    const int l = block[t], nz = nnz[t], n = bsize[l];
+    const idx_t o = off[t];
+    const int *rw = row + o, *cl = col + o;
+    const dd *vl = val + o, *Wl = W + boff[l];
+    auto term = [&](int q) {
+        int i = rw[q], j = cl[q];
+        if (i == j) return dd_mul(vl[q], Wl[i + (idx_t)n * j]);
+        return dd_mul(vl[q], dd_add(Wl[i + (idx_t)n * j], Wl[j + (idx_t)n * i]));
+    };
+    const int amari = nz % 4, shou = nz / 4, nitems = amari + shou;
+    dd ret = mk(0.0);
+    for (int base = 0; base < nitems; base += 256) {
+        const int cnt = nitems - base < 256 ? nitems - base : 256;
+        for (int k = lane; k < cnt; k += 64) {
+            const int it = base + k;
+            if (it < amari) {
+                g[k] = term(it);
+            } else {
+                const int q = amari + 4 * (it - amari);
+                g[k] = dd_add(dd_add(dd_add(term(q), term(q + 1)), term(q + 2)), term(q + 3));
+            }
+        }
+        __syncthreads();
+        if (lane == 0)
+            for (int k = 0; k < cnt; k++) ret = dd_add_eq(ret, g[k]);
+        __syncthreads();
+    }
+    if (lane == 0) out[t] = ret;
 }
 // out[i] = 0 + sum of its task values in block order
 __global__ void k_combine_tasks(int m, const idx_t *ptr, const dd *tv, dd *out) {
@@ -675,7 +704,7 @@ __global__ void k_combine_tasks(int m, const idx_t *ptr, const dd *tv, dd *out) 
 
 void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
     if (dp.ntask > 0)
-        k_sparse_tasks<<<(dp.ntask + 127) / 128, 128, 0, s>>>(dp.ntask, dp.d_task_type, dp.d_task_block, dp.d_task_nnz,
+        k_sparse_tasks<<<dp.ntask, 64, 0, s>>>(dp.ntask, dp.d_task_type, dp.d_task_block, dp.d_task_nnz,
                                                               dp.d_task_off, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val,
                                                               dp.d_boff, dp.d_bsize, W, dp.d_task_val);
     if (dp.n_dense_tasks > 0) sdk::dot_batch_off(s, dp.d_dense_tasks, dp.n_dense_tasks, W, T);
@@ -683,7 +712,7 @@ void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
 }
 void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
     if (dp.nctask > 0)
-        k_sparse_tasks<<<(dp.nctask + 127) / 128, 128, 0, s>>>(dp.nctask, dp.d_ctask_type, dp.d_ctask_block,
+        k_sparse_tasks<<<dp.nctask, 64, 0, s>>>(dp.nctask, dp.d_ctask_type, dp.d_ctask_block,
                                                                dp.d_ctask_nnz, dp.d_ctask_off, dp.d_sp_row, dp.d_sp_col,
                                                                dp.d_sp_val, dp.d_boff, dp.d_bsize, W, dp.d_ctask_val);
     if (dp.n_dense_ctasks > 0) sdk::dot_batch_off(s, dp.d_dense_ctasks, dp.n_dense_ctasks, W, T);

@@ -115,18 +115,13 @@ bool chain_active(Chain &c) {
 void chain_enqueue(Chain &c, int T) {
     hipStream_t s = c.s;
     const int nDim = c.n;
-    sdk::copy(s, nDim, c.q, c.qold);
     c.value = ddr(1.0) / c.beta;
-    sdk::copy(s, nDim, c.r, c.q);
-    sdk::scal(s, nDim, c.value.v, c.q);
+    sdk::lanczos_q(s, nDim, c.value.v, c.r, c.q, c.qold);   // qold = q; q = r / beta
     sdk::trmv_lower(s, 'T', nDim, c.L, nDim, c.q, c.w);
     sdk::gemv_n(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
     sdk::trmv_lower(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
-    sdk::dot(s, nDim, c.q, c.w, c.scal, T, c.dot_work, c.counter);  // alpha
-    sdk::copy(s, nDim, c.w, c.r);
-    sdk::axpy_dev(s, nDim, c.scal, true, c.q, c.r);                  // r -= alpha q
-    sdk::axpy(s, nDim, dd_neg(c.beta.v), c.qold, c.r);              // r -= beta qold
-    sdk::dot(s, nDim, c.r, c.r, c.scal + 1, T, c.dot_work, c.counter);
+    // alpha = q.w; r = w - alpha q - beta qold; r.r
+    sdk::lanczos_tail(s, nDim, T, c.q, c.qold, c.w, c.r, dd_neg(c.beta.v), c.scal);
     (void)hipMemcpyAsync(c.h, c.scal, sizeof(dd) * 2, hipMemcpyDeviceToHost, s);
 }
 
@@ -224,6 +219,8 @@ class Solver {
     Solver(Context &c, const HostProblem &h, DeviceProblem &d, const Params &p) : ctx(c), hp(h), dp(d), prm(p) {}
     ~Solver() {
         for (void *q : mem) (void)hipFree(q);
+        if (h_sc) (void)hipHostFree(h_sc);
+        if (h_diag) (void)hipHostFree(h_diag);
     }
     int run(SolveOutput &out, std::vector<dd> *xo, std::vector<dd> *yo, std::vector<dd> *zo);
 
@@ -239,7 +236,11 @@ class Solver {
     // iterates and work spaces (concatenated SDP blocks)
     dd *X, *Z, *invCholX, *invCholZ, *invZ, *DxMat, *DzMat, *r_zinv, *x_rd_zinv, *DLS1, *DLS2, *dualMat;
     dd *y, *dy, *g, *primalVec, *bMat;
-    std::vector<dd> diag_buf;
+    // scalar slots: [0, 3 nB) block inner products, then C, b and max-norm
+    // results; h_sc is their pinned host mirror.  d_diag gathers the X and Z
+    // diagonals (2 nDim) for SolveInfo.
+    dd *d_sc = nullptr, *h_sc = nullptr, *d_diag = nullptr, *h_diag = nullptr;
+    int slot_C = 0, slot_b = 0, slot_max = 0;
     double t_schur = 0, t_chol = 0, t_solve = 0, t_lanczos = 0, t_inverse = 0;
 
     dd *alloc(idx_t n) {
@@ -255,18 +256,21 @@ class Solver {
     static double now() {
         return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     }
-    // Download `cnt` scalar slots.
-    void get_scalars(dd *h, int cnt) {
-        (void)hipMemcpyAsync(h, ctx.d_scal, sizeof(dd) * cnt, hipMemcpyDeviceToHost, s);
+    void dot(idx_t n, const dd *x, const dd *y, int slot) {
+        sdk::dot(s, n, x, y, d_sc + slot, ctx.dot_T, ctx.d_dot_work, ctx.d_counter);
+    }
+    // Download scalar slots [first, first + cnt) into h_sc.
+    const dd *get_scalars(int first, int cnt) {
+        (void)hipMemcpyAsync(h_sc + first, d_sc + first, sizeof(dd) * cnt, hipMemcpyDeviceToHost, s);
         sync();
+        return h_sc + first;
     }
 
     // ---- dense linear-space helpers (sdpa_linear.cpp Lal::*) ----
     // DenseLinearSpace inner product: per-block Rdot, summed in block order.
     ddr inner_dense(const dd *a, const dd *b) {
-        for (int l = 0; l < nB; l++) ctx.dot((idx_t)bsz(l) * bsz(l), a + dp.boff[l], b + dp.boff[l], l);
-        std::vector<dd> h(nB);
-        get_scalars(h.data(), nB);
+        for (int l = 0; l < nB; l++) dot((idx_t)bsz(l) * bsz(l), a + dp.boff[l], b + dp.boff[l], l);
+        const dd *h = get_scalars(0, nB);
         ddr r = 0.0;
         for (int l = 0; l < nB; l++) r += ddr(h[l]);
         return r;
@@ -277,9 +281,8 @@ class Solver {
         const dd *as[3] = {a0, a1, a2}, *bs2[3] = {b0, b1, b2};
         for (int q = 0; q < 3; q++)
             for (int l = 0; l < nB; l++)
-                ctx.dot((idx_t)bsz(l) * bsz(l), as[q] + dp.boff[l], bs2[q] + dp.boff[l], q * nB + l);
-        std::vector<dd> h(3 * nB);
-        get_scalars(h.data(), 3 * nB);
+                dot((idx_t)bsz(l) * bsz(l), as[q] + dp.boff[l], bs2[q] + dp.boff[l], q * nB + l);
+        const dd *h = get_scalars(0, 3 * nB);
         ddr r[3];
         for (int q = 0; q < 3; q++) {
             r[q] = 0.0;
@@ -290,25 +293,16 @@ class Solver {
         r2 = r[2];
     }
     ddr inner_C(const dd *W) {
-        sdp::inner_C(s, dp, W, ctx.d_scal + 40, ctx.dot_T);
-        dd h;
-        (void)hipMemcpyAsync(&h, ctx.d_scal + 40, sizeof(dd), hipMemcpyDeviceToHost, s);
-        sync();
-        return ddr(h);
+        sdp::inner_C(s, dp, W, d_sc + slot_C, ctx.dot_T);
+        return ddr(*get_scalars(slot_C, 1));
     }
     ddr inner_b(const dd *v) {
-        ctx.dot(m, dp.d_b, v, 41);
-        dd h;
-        (void)hipMemcpyAsync(&h, ctx.d_scal + 41, sizeof(dd), hipMemcpyDeviceToHost, s);
-        sync();
-        return ddr(h);
+        dot(m, dp.d_b, v, slot_b);
+        return ddr(*get_scalars(slot_b, 1));
     }
     ddr max_abs(const dd *x, idx_t len) {
-        sdk::max_abs(s, len, x, ctx.d_scal + 42, ctx.d_max_work, ctx.d_counter + 1);
-        dd h;
-        (void)hipMemcpyAsync(&h, ctx.d_scal + 42, sizeof(dd), hipMemcpyDeviceToHost, s);
-        sync();
-        return ddr(h);
+        sdk::max_abs(s, len, x, d_sc + slot_max, ctx.d_max_work, ctx.d_counter + 1);
+        return ddr(*get_scalars(slot_max, 1));
     }
     // ret = a*b*c per block (Jal::ns_jordan_triple_product)
     void triple(dd *ret, const dd *a, const dd *b, const dd *c, dd *work) {
@@ -402,18 +396,18 @@ class Solver {
     }
     void diag_sums(double lam, ddr &x0z, ddr &xz0) {
         // x0zMat += lambda * Z_ii, xMatz0 += lambda * X_ii, blocks in order
+        // gather both diagonals on the device, one contiguous download
+        idx_t off = 0;
         for (int l = 0; l < nB; l++) {
-            int n = bsz(l);
-            diag_buf.resize((size_t)2 * n);
-            (void)hipMemcpy2DAsync(diag_buf.data(), sizeof(dd), Z + dp.boff[l], sizeof(dd) * (n + 1), sizeof(dd), n,
-                                   hipMemcpyDeviceToHost, s);
-            (void)hipMemcpy2DAsync(diag_buf.data() + n, sizeof(dd), X + dp.boff[l], sizeof(dd) * (n + 1), sizeof(dd), n,
-                                   hipMemcpyDeviceToHost, s);
-            sync();
-            for (int i = 0; i < n; i++) {
-                x0z += lam * ddr(diag_buf[i]);
-                xz0 += lam * ddr(diag_buf[n + i]);
-            }
+            sdk::gather_diag(s, bsz(l), blk(Z, l), d_diag + off);
+            sdk::gather_diag(s, bsz(l), blk(X, l), d_diag + nDim + off);
+            off += bsz(l);
+        }
+        (void)hipMemcpyAsync(h_diag, d_diag, sizeof(dd) * 2 * (size_t)nDim, hipMemcpyDeviceToHost, s);
+        sync();
+        for (int i = 0; i < nDim; i++) {
+            x0z += lam * ddr(h_diag[i]);
+            xz0 += lam * ddr(h_diag[nDim + i]);
         }
     }
 };
@@ -433,6 +427,12 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     if (!(y = alloc(m)) || !(dy = alloc(m)) || !(g = alloc(m)) || !(primalVec = alloc(m)) ||
         !(bMat = alloc((idx_t)m * m)))
         return -2;
+    slot_C = 3 * nB;
+    slot_b = slot_C + 1;
+    slot_max = slot_C + 2;
+    if (!(d_sc = alloc(slot_C + 3)) || !(d_diag = alloc(2 * (idx_t)nDim))) return -2;
+    if (hipHostMalloc((void **)&h_sc, sizeof(dd) * (slot_C + 3)) != hipSuccess) return -2;
+    if (hipHostMalloc((void **)&h_diag, sizeof(dd) * 2 * (size_t)nDim) != hipSuccess) return -2;
     double t_start = now();
     for (dd **b : bufs) sdk::fill(s, nn, mk(0.0), *b);
 

@@ -22,6 +22,11 @@ PHASES = ["noINFO", "pFEAS", "dFEAS", "pdFEAS", "pdINF", "pFEAS_dINF", "pINF_dFE
 _lib = None
 
 
+# Chunks of the static dot-product partition (the reference's Rdot_omp with
+# this many OpenMP threads); the library's default.
+DEFAULT_DOT_PARTITION = 64
+
+
 class SdpaddError(RuntimeError):
     pass
 
@@ -116,13 +121,13 @@ def dd_scalar(hi, lo=0.0):
 class Device:
     """A library handle (one GPU, one stream)."""
 
-    def __init__(self, device=0, dot_partition=1024):
+    def __init__(self, device=0, dot_partition=None):
         self.lib = load_library()
         self.h = ctypes.c_void_p()
         rc = self.lib.sdpadd_create(device, ctypes.byref(self.h))
         if rc != 0:
             raise SdpaddError("sdpadd_create failed (%d): no usable GPU" % rc)
-        self.set_dot_partition(dot_partition)
+        self.set_dot_partition(DEFAULT_DOT_PARTITION if dot_partition is None else dot_partition)
 
     def close(self):
         if self.h:

@@ -130,7 +130,7 @@ def test_rtrmm(gpu, oracle, m, n):
     assert same(dB.get(), ref)
 
 
-@pytest.mark.parametrize("n", [1, 50, 257, 1100])
+@pytest.mark.parametrize("n", [1, 2, 17, 50, 64, 65, 257, 1100])
 @pytest.mark.parametrize("trans", ["N", "T"])
 def test_rtrsv_rtrmv(gpu, oracle, n, trans):
     rng = np.random.default_rng(n)
@@ -147,7 +147,7 @@ def test_rtrsv_rtrmv(gpu, oracle, n, trans):
     assert same(dx.get(), ref)
 
 
-@pytest.mark.parametrize("m,n", [(1, 1), (33, 17), (300, 300)])
+@pytest.mark.parametrize("m,n", [(1, 1), (33, 17), (64, 16), (65, 33), (130, 801), (300, 300)])
 def test_rgemv(gpu, oracle, m, n):
     rng = np.random.default_rng(m + n)
     A = rand_dd((m * n,), rng)
@@ -161,7 +161,7 @@ def test_rgemv(gpu, oracle, m, n):
 
 
 @pytest.mark.parametrize("n", [0, 1, 5, 1023, 1024, 1025, 40000, 640000])
-@pytest.mark.parametrize("T", [1, 7, 1024])
+@pytest.mark.parametrize("T", [1, 7, 64, 1024, 2048])
 def test_rdot_partition(gpu, oracle, n, T):
     """Rdot_omp's static partition into T chunks, partials added in order."""
     rng = np.random.default_rng(n + T)
@@ -178,7 +178,7 @@ def test_rdot_partition(gpu, oracle, n, T):
         out = np.zeros(2)
         gpu.check(gpu.lib.sdpadd_Rdot(gpu.h, n, dx.ptr, 1, dy.ptr, 1, dp(out)), "Rdot")
     finally:
-        gpu.set_dot_partition(1024)
+        gpu.set_dot_partition(sdpadd.DEFAULT_DOT_PARTITION)
     assert same(out, ref)
 
 
@@ -189,7 +189,7 @@ def test_lanczos(gpu, oracle, n):
     g = rng.standard_normal((n, n))
     Mh = (g + g.T) / 2
     M = np.ascontiguousarray(np.stack([Mh, np.zeros_like(Mh)], -1).transpose(1, 0, 2)).reshape(-1, 2)
-    oracle.set_dot_threads(1024)
+    oracle.set_dot_threads(sdpadd.DEFAULT_DOT_PARTITION)
     try:
         ref = oracle.lanczos(n, L, M)
     finally:

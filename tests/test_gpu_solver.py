@@ -1,6 +1,6 @@
 """End-to-end parity of the GPU Newton-step path against the CPU oracle.
 
-* With the same dot partition (1024 chunks) the whole trajectory is
+* With the same dot partition (the library's default) the whole trajectory is
   bit-identical to the oracle: every per-iteration scalar and the final
   objectives, residual norms and iterates.
 * With the partition set to one chunk (the reference's single-threaded
@@ -49,7 +49,7 @@ def gpu_solve(gpu, path, param=None):
 @pytest.mark.parametrize("name", ["theta1", "mcp100"])
 def test_sdplib_bitwise_vs_oracle(gpu, oracle, name):
     path = golden_sdplib(name)
-    ref = oracle.solve(path, SDPLIB_PARAM, dot_threads=1024)
+    ref = oracle.solve(path, SDPLIB_PARAM, dot_threads=sdpadd.DEFAULT_DOT_PARTITION)
     got = gpu_solve(gpu, path, SDPLIB_PARAM)
     assert got["iterations"] == ref["iterations"]
     assert got["phase"] == ref["phase"]
@@ -93,7 +93,7 @@ def test_random_dense_single_thread_order_bitwise(gpu, oracle, tmp_path):
     try:
         got = gpu_solve(gpu, data, param)
     finally:
-        gpu.set_dot_partition(1024)
+        gpu.set_dot_partition(sdpadd.DEFAULT_DOT_PARTITION)
     assert got["iterations"] == ref["iterations"]
     assert got["phase"] == ref["phase"]
     for a, b in zip(got["trace"], ref["trace"]):
@@ -109,7 +109,7 @@ def test_random_dense_bitwise(gpu, oracle, tmp_path):
     from sdpadd import gen
     data = gen.random_dense(str(tmp_path / "c0.dat-s"))
     param = gen.param_file(str(tmp_path / "p.sdpa"), epsilon_star=1e-25, epsilon_dash=1e-25)
-    ref = oracle.solve(data, param, dot_threads=1024)
+    ref = oracle.solve(data, param, dot_threads=sdpadd.DEFAULT_DOT_PARTITION)
     got = gpu_solve(gpu, data, param)
     assert got["iterations"] == ref["iterations"]
     for a, b in zip(got["trace"], ref["trace"]):
@@ -123,7 +123,7 @@ def test_small_workloads_bitwise(gpu, oracle, tmp_path, builder, kw):
     from sdpadd import gen
     data = getattr(gen, builder)(str(tmp_path / "w.dat-s"), **kw)
     param = gen.param_file(str(tmp_path / "p.sdpa"), epsilon_star=1e-25, epsilon_dash=1e-25)
-    ref = oracle.solve(data, param, dot_threads=1024)
+    ref = oracle.solve(data, param, dot_threads=sdpadd.DEFAULT_DOT_PARTITION)
     got = gpu_solve(gpu, data, param)
     assert got["iterations"] == ref["iterations"] and got["phase"] == ref["phase"]
     for a, b in zip(got["trace"], ref["trace"]):
