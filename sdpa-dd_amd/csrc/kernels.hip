@@ -270,11 +270,37 @@ void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx
 // -------------------------------------------------------------- Cholesky --
 static constexpr int PB = 64;  // Cholesky panel width
 
-// Factor the nb x nb diagonal block at (K,K) in LDS; right-looking within the
-// block (each element still sees its updates in ascending column order).
-// Stores one/L_kk for the panel solve.
+// Lane broadcast of a dd from a wave-uniform lane index.
+__device__ __forceinline__ double readlane_d(double v, int k) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)b, k), hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ dd readlane_dd(dd v, int k) {
+    dd r;
+    r.hi = readlane_d(v.hi, k);
+    r.lo = readlane_d(v.lo, k);
+    return r;
+}
+// Packed column-major lower triangle of a PB x PB block: (j, k), j >= k.
+__device__ __forceinline__ int lpack(int j, int k) { return k * PB - (k * (k - 1)) / 2 + (j - k); }
+// Row r and column c (c <= r) of element e of a lower triangle, row-wise.
+__device__ __forceinline__ void tri_rc(int e, int &r, int &c) {
+    r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+    while ((r + 1) * (r + 2) / 2 <= e) r++;
+    while (r * (r + 1) / 2 > e) r--;
+    c = e - r * (r + 1) / 2;
+}
+
+// Diagonal block of the blocked Cholesky (Rpotrf2 order per element:
+// a_ij -= l_jk * l_ik for k ascending, skipping l_jk == 0; l_kk =
+// sqrt(a_kk); l_ik = (1/l_kk) * a_ik).  One workgroup.  Thread 0 is the
+// pivot thread: during step k it applies column k to a_{k+1,k+1} and forms
+// l_{k+1,k+1} and its reciprocal, while threads 1..255 apply column k to
+// the rest of the trailing triangle.  Two barriers per column.
 __global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
     __shared__ dd S[PB][PB + 1];
+    __shared__ dd Lk[PB];
     __shared__ dd rl[PB];
     __shared__ int failed;
     const int tid = threadIdx.x;
@@ -283,32 +309,47 @@ __global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, i
         int i = e % nb, j = e / nb;
         if (i >= j) S[i][j] = A[(K + i) + (K + j) * lda];
     }
-    if (tid == 0) failed = 0;
+    __syncthreads();
+    auto pivot = [&](int k) {
+        dd a = S[k][k];
+        if (dd_le(a, mk(0.0)) || dd_isnan(a)) {
+            failed = 1;
+            *info = K + k + 1;
+        } else {
+            dd l = dd_sqrt(a);
+            S[k][k] = l;
+            rl[k] = dd_div(mk(1.0), l);
+        }
+    };
+    if (tid == 0) {
+        failed = 0;
+        pivot(0);
+    }
     __syncthreads();
     for (int k = 0; k < nb; k++) {
-        if (tid == 0) {
-            dd a = S[k][k];
-            if (dd_le(a, mk(0.0)) || dd_isnan(a)) {
-                failed = 1;
-                *info = K + k + 1;
-            } else {
-                dd l = dd_sqrt(a);
-                S[k][k] = l;
-                rl[k] = dd_div(mk(1.0), l);
-            }
+        if (failed) return;
+        const dd r = rl[k];
+        for (int i = k + 1 + tid; i < nb; i += 256) {
+            dd v = dd_mul(r, S[i][k]);
+            Lk[i] = v;
+            S[i][k] = v;
         }
         __syncthreads();
-        if (failed) return;
-        dd r = rl[k];
-        for (int i = k + 1 + tid; i < nb; i += 256) S[i][k] = dd_mul(r, S[i][k]);
-        __syncthreads();
-        // rank-1 update of the remaining lower triangle
         const int rem = nb - k - 1;
-        for (int e = tid; e < rem * rem; e += 256) {
-            int i = k + 1 + e % rem, j = k + 1 + e / rem;
-            if (i >= j) {
-                dd ljk = S[j][k];
-                if (!dd_is_zero(ljk)) S[i][j] = dd_sub(S[i][j], dd_mul(ljk, S[i][k]));
+        if (tid == 0) {
+            if (rem > 0) {
+                dd l = Lk[k + 1];
+                if (!dd_is_zero(l)) S[k + 1][k + 1] = dd_sub(S[k + 1][k + 1], dd_mul(l, l));
+                pivot(k + 1);
+            }
+        } else {
+            const int cnt = rem * (rem + 1) / 2;
+            for (int e = tid; e < cnt; e += 255) {
+                int ri, ci;
+                tri_rc(e, ri, ci);
+                const int i = k + 1 + ri, j = k + 1 + ci;
+                dd ljk = Lk[j];
+                if (!dd_is_zero(ljk)) S[i][j] = dd_sub(S[i][j], dd_mul(ljk, Lk[i]));
             }
         }
         __syncthreads();
@@ -320,39 +361,59 @@ __global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, i
     for (int k = tid; k < nb; k += 256) rinv[k] = rl[k];
 }
 
-// Panel solve: rows [r0, r0+nrows) of columns K..K+nb-1 against the factored
-// diagonal block (Rtrsm 'R','L','T','N' order).  32 rows per workgroup.
-__global__ __launch_bounds__(256) void k_potrf_trsm(dd *A, idx_t lda, idx_t K, int nb, idx_t r0, idx_t nrows,
-                                                   const int64_t *info, const dd *rinv) {
-    __shared__ dd X[32][PB + 1];
-    __shared__ dd L[PB][PB + 1];
-    if (*info != 0) return;
-    const int tid = threadIdx.x;
-    const idx_t rb = r0 + (idx_t)blockIdx.x * 32;
-    const int nr = (int)((r0 + nrows - rb) < 32 ? (r0 + nrows - rb) : 32);
+// B := B * L^{-T} on an nb-column panel (Rtrsm 'R','L','T','N' order with
+// reciprocal scaling: b_rk *= rl_k, then b_rj -= l_jk * b_rk for j > k,
+// skipping l_jk == 0), rows [r0, r0 + nrows).  L is the nb x nb lower block
+// at Lb.  16 rows per workgroup are staged through LDS; each wave owns 4
+// rows, lane j column j of each, and the solved column is broadcast with
+// readlane.
+static constexpr int PR = 16;
+__global__ __launch_bounds__(256) void k_panel_rltn(dd *B, idx_t ldb, idx_t r0, idx_t nrows, const dd *Lb, idx_t lda, int nb,
+                                                    const dd *rinv, const int64_t *info) {
+    __shared__ dd Lc[PB * (PB + 1) / 2];
+    __shared__ dd rl[PB];
+    __shared__ dd Tt[PR][PB + 1];
+    if (info && *info != 0) return;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const idx_t rb = r0 + (idx_t)blockIdx.x * PR;
+    const int nr = (int)((r0 + nrows - rb) < PR ? (r0 + nrows - rb) : PR);
     for (int e = tid; e < nb * nb; e += 256) {
-        int i = e % nb, j = e / nb;
-        L[i][j] = (i >= j) ? A[(K + i) + (K + j) * lda] : mk(0.0);
+        int j = e % nb, k = e / nb;
+        if (j >= k) Lc[lpack(j, k)] = Lb[j + (idx_t)k * lda];
     }
-    for (int e = tid; e < 32 * nb; e += 256) {
-        int r = e % 32, j = e / 32;
-        if (r < nr) X[r][j] = A[(rb + r) + (K + j) * lda];
+    for (int k = tid; k < nb; k += 256) rl[k] = rinv[k];
+    for (int e = tid; e < PR * nb; e += 256) {
+        int r = e % PR, j = e / PR;
+        Tt[r][j] = r < nr ? B[(rb + r) + (idx_t)j * ldb] : mk(0.0);
     }
     __syncthreads();
+    dd x[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) x[q] = lane < nb ? Tt[4 * wave + q][lane] : mk(0.0);
     for (int k = 0; k < nb; k++) {
-        if (tid < nr) X[tid][k] = dd_mul(rinv[k], X[tid][k]);
-        __syncthreads();
-        const int rem = nb - k - 1;
-        for (int e = tid; e < nr * rem; e += 256) {
-            int r = e % nr, j = k + 1 + e / nr;
-            dd ljk = L[j][k];
-            if (!dd_is_zero(ljk)) X[r][j] = dd_sub(X[r][j], dd_mul(ljk, X[r][k]));
+        if (lane == k) {
+            const dd r = rl[k];
+#pragma unroll
+            for (int q = 0; q < 4; q++) x[q] = dd_mul(r, x[q]);
         }
-        __syncthreads();
+        dd xk[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) xk[q] = readlane_dd(x[q], k);
+        if (lane > k && lane < nb) {
+            const dd l = Lc[lpack(lane, k)];
+            if (!dd_is_zero(l)) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) x[q] = dd_sub(x[q], dd_mul(l, xk[q]));
+            }
+        }
     }
-    for (int e = tid; e < 32 * nb; e += 256) {
-        int r = e % 32, j = e / 32;
-        if (r < nr) A[(rb + r) + (K + j) * lda] = X[r][j];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (lane < nb) Tt[4 * wave + q][lane] = x[q];
+    __syncthreads();
+    for (int e = tid; e < PR * nb; e += 256) {
+        int r = e % PR, j = e / PR;
+        if (r < nr) B[(rb + r) + (idx_t)j * ldb] = Tt[r][j];
     }
 }
 
@@ -366,7 +427,9 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
         {
             StatScope st("potrf", s, (double)nb * nb * nb / 3.0 + (double)below * nb * nb);
             k_potrf_diag<<<1, 256, 0, s>>>(A, lda, K, nb, d_info, scratch);
-            if (below > 0) k_potrf_trsm<<<cdiv(below, 32), 256, 0, s>>>(A, lda, K, nb, K + nb, below, d_info, scratch);
+            if (below > 0)
+                k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda, nb,
+                                                             scratch, d_info);
         }
         if (below > 0) {
             // trailing lower triangle: A22 += (-L21) * L21^T, k ascending
@@ -494,37 +557,30 @@ __global__ void k_set_identity(idx_t n, dd v, dd *A, idx_t lda) {
     if (i < n) A[i + j * lda] = (i == j) ? v : mk(0.0);
 }
 
-// Rows [rb, rb+nb) of B := L^{-1} B for the column range [c0, c0+32), with
-// the updates from rows above already applied (Rtrsm 'L','L','N','N' order:
-// b_kj = b_kj / a_kk, then b_ij -= b_kj * a_ik for i > k).
+// Rows [rb, rb+nb) of B := L^{-1} B for columns [0, ncols), with the
+// updates from rows above already applied (Rtrsm 'L','L','N','N' order: if
+// b_kj != 0 { b_kj = b_kj / a_kk; b_ij -= b_kj * a_ik for i > k }).  A wave
+// owns a column, lane i its row i; the diagonal block is staged in LDS.
 __global__ __launch_bounds__(256) void k_trsm_llnn_diag(idx_t rb, int nb, idx_t ncols, const dd *L, idx_t lda, dd *B,
                                                         idx_t ldb) {
-    __shared__ dd X[PB][33];
-    const int tid = threadIdx.x;
-    const idx_t c0 = (idx_t)blockIdx.x * 32;
-    const int nc = (int)((ncols - c0) < 32 ? (ncols - c0) : 32);
-    for (int e = tid; e < nb * 32; e += 256) {
-        int r = e % nb, c = e / nb;
-        if (c < nc) X[r][c] = B[(rb + r) + (c0 + c) * ldb];
+    __shared__ dd Lc[PB * (PB + 1) / 2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int e = tid; e < nb * nb; e += 256) {
+        int j = e % nb, k = e / nb;
+        if (j >= k) Lc[lpack(j, k)] = L[(rb + j) + (rb + k) * lda];
     }
     __syncthreads();
-    for (int k = 0; k < nb; k++) {
-        if (tid < nc) {
-            dd v = X[k][tid];
-            if (!dd_is_zero(v)) X[k][tid] = dd_div(v, L[(rb + k) + (rb + k) * lda]);
+    for (idx_t c = (idx_t)blockIdx.x * 4 + wave; c < ncols; c += (idx_t)gridDim.x * 4) {
+        dd* col = B + rb + c * ldb;
+        dd b = lane < nb ? col[lane] : mk(0.0);
+        for (int k = 0; k < nb; k++) {
+            const int nz = __builtin_amdgcn_readlane(!dd_is_zero(b), k);
+            if (!nz) continue;
+            if (lane == k) b = dd_div(b, Lc[lpack(k, k)]);
+            const dd bk = readlane_dd(b, k);
+            if (lane > k && lane < nb) b = dd_sub(b, dd_mul(bk, Lc[lpack(lane, k)]));
         }
-        __syncthreads();
-        const int rem = nb - k - 1;
-        for (int e = tid; e < rem * nc; e += 256) {
-            int i = k + 1 + e % rem, c = e / rem;
-            dd bk = X[k][c];
-            if (!dd_is_zero(bk)) X[i][c] = dd_sub(X[i][c], dd_mul(bk, L[(rb + i) + (rb + k) * lda]));
-        }
-        __syncthreads();
-    }
-    for (int e = tid; e < nb * 32; e += 256) {
-        int r = e % nb, c = e / nb;
-        if (c < nc) B[(rb + r) + (c0 + c) * ldb] = X[r][c];
+        if (lane < nb) col[lane] = b;
     }
 }
 
@@ -536,7 +592,7 @@ static void trsm_llnn_impl(hipStream_t s, idx_t m, idx_t n, const dd *L, idx_t l
         idx_t ncols = identity_rhs ? std::min<idx_t>(n, rb + nb) : n;
         {
             StatScope st("trsm", s, (double)ncols * nb * nb);
-            k_trsm_llnn_diag<<<cdiv(ncols, 32), 256, 0, s>>>(rb, nb, ncols, L, lda, B, ldb);
+            k_trsm_llnn_diag<<<std::min<idx_t>(cdiv(ncols, 4), 1024), 256, 0, s>>>(rb, nb, ncols, L, lda, B, ldb);
         }
         idx_t below = m - rb - nb;
         if (below > 0) {
@@ -563,34 +619,6 @@ __global__ void k_recips(idx_t n, const dd *A, idx_t lda, dd *r) {
     idx_t k = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) r[k] = dd_div(mk(1.0), A[k + k * lda]);
 }
-// Panel solve for a general m x n right-hand side, columns [K, K+nb).
-__global__ __launch_bounds__(256) void k_trsm_rltn_panel(idx_t m, idx_t K, int nb, const dd *A, idx_t lda, dd *B, idx_t ldb,
-                                                         const dd *rinv) {
-    __shared__ dd X[32][PB + 1];
-    const int tid = threadIdx.x;
-    const idx_t rb = (idx_t)blockIdx.x * 32;
-    const int nr = (int)((m - rb) < 32 ? (m - rb) : 32);
-    for (int e = tid; e < 32 * nb; e += 256) {
-        int r = e % 32, j = e / 32;
-        if (r < nr) X[r][j] = B[(rb + r) + (K + j) * ldb];
-    }
-    __syncthreads();
-    for (int k = 0; k < nb; k++) {
-        if (tid < nr) X[tid][k] = dd_mul(rinv[K + k], X[tid][k]);
-        __syncthreads();
-        const int rem = nb - k - 1;
-        for (int e = tid; e < nr * rem; e += 256) {
-            int r = e % nr, j = k + 1 + e / nr;
-            dd ljk = A[(K + j) + (K + k) * lda];
-            if (!dd_is_zero(ljk)) X[r][j] = dd_sub(X[r][j], dd_mul(ljk, X[r][k]));
-        }
-        __syncthreads();
-    }
-    for (int e = tid; e < 32 * nb; e += 256) {
-        int r = e % 32, j = e / 32;
-        if (r < nr) B[(rb + r) + (K + j) * ldb] = X[r][j];
-    }
-}
 void trsm_rltn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, dd *B, idx_t ldb) {
     if (m <= 0 || n <= 0) return;
     dd *rinv = nullptr;
@@ -600,7 +628,7 @@ void trsm_rltn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, dd *B, i
         int nb = (int)std::min<idx_t>(PB, n - K);
         // columns left of the panel were finished earlier: B[:, panel] += (-B[:, done]) * A[panel, done]^T
         if (K > 0) gemm(s, 'T', m, nb, K, mk(-1.0), B, ldb, A + K, lda, mk(1.0), B + K * ldb, ldb);
-        k_trsm_rltn_panel<<<cdiv(m, 32), 256, 0, s>>>(m, K, nb, A, lda, B, ldb, rinv);
+        k_panel_rltn<<<cdiv(m, PR), 256, 0, s>>>(B + K * ldb, ldb, 0, m, A + K + K * lda, lda, nb, rinv + K, nullptr);
     }
     hipFreeAsync(rinv, s);
 }
