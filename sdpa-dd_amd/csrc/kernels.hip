@@ -295,9 +295,10 @@ __device__ __forceinline__ void tri_rc(int e, int &r, int &c) {
 // Diagonal block of the blocked Cholesky (Rpotrf2 order per element:
 // a_ij -= l_jk * l_ik for k ascending, skipping l_jk == 0; l_kk =
 // sqrt(a_kk); l_ik = (1/l_kk) * a_ik).  One workgroup.  Thread 0 is the
-// pivot thread: during step k it applies column k to a_{k+1,k+1} and forms
-// l_{k+1,k+1} and its reciprocal, while threads 1..255 apply column k to
-// the rest of the trailing triangle.  Two barriers per column.
+// pivot thread (lane 0 of wave 0, the rest of that wave idles): during step
+// k it applies column k to a_{k+1,k+1} and forms l_{k+1,k+1} and its
+// reciprocal, while waves 1-3 scale column k and apply it to the rest of
+// the trailing triangle.  Two barriers per column.
 __global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
     __shared__ dd S[PB][PB + 1];
     __shared__ dd Lk[PB];
@@ -329,22 +330,23 @@ __global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, i
     for (int k = 0; k < nb; k++) {
         if (failed) return;
         const dd r = rl[k];
-        for (int i = k + 1 + tid; i < nb; i += 256) {
-            dd v = dd_mul(r, S[i][k]);
-            Lk[i] = v;
-            S[i][k] = v;
-        }
+        if (tid >= 64)
+            for (int i = k + 1 + (tid - 64); i < nb; i += 192) {
+                dd v = dd_mul(r, S[i][k]);
+                Lk[i] = v;
+                S[i][k] = v;
+            }
         __syncthreads();
         const int rem = nb - k - 1;
-        if (tid == 0) {
-            if (rem > 0) {
+        if (tid < 64) {
+            if (tid == 0 && rem > 0) {
                 dd l = Lk[k + 1];
                 if (!dd_is_zero(l)) S[k + 1][k + 1] = dd_sub(S[k + 1][k + 1], dd_mul(l, l));
                 pivot(k + 1);
             }
         } else {
             const int cnt = rem * (rem + 1) / 2;
-            for (int e = tid; e < cnt; e += 255) {
+            for (int e = tid - 63; e < cnt; e += 192) {
                 int ri, ci;
                 tri_rc(e, ri, ci);
                 const int i = k + 1 + ri, j = k + 1 + ci;
@@ -704,166 +706,214 @@ void syrk_ln(hipStream_t s, idx_t n, idx_t k, dd alpha, const dd *A, idx_t lda, 
 
 // ---------------------------------------------------------------- mat-vec --
 // Mat-vecs inside Lanczos.  Each output is one dependent chain of dd
-// additions in the reference's order, so a lane owns one output and the
-// kernel is latency-bound on that chain: the matrix tile for the next U
-// terms is loaded (and its products formed) while the current U additions
-// run.  The vector entries of a tile are the same for every lane; lanes
-// 0..U-1 load them one tile ahead with vector loads and publish them in
-// LDS.  One wave per workgroup.
-static constexpr int MV_U = 16;
+// additions in the reference's order, so the kernels are latency-bound on
+// that chain.  A workgroup has two waves for 64 outputs: wave 1 (producer)
+// loads the matrix and vector tiles and forms the products of tile t+1 into
+// an LDS ring while wave 0 (consumer) adds the products of tile t into the
+// 64 chains, one per lane.  The producer issues its loads one tile ahead.
+static constexpr int MU = 32;
+__device__ __forceinline__ bool mv_valid(unsigned mask, int u) { return (mask >> u) & 1u; }
 
 // w_j = x_j*L_jj, then += L_ij*x_i for i = j+1 .. n-1 ascending (Rtrmv 'L','T').
-// Lane j owns column j; rows [i0, i0+16) x columns [j0, j0+64) are staged
-// through LDS so that global reads are 256-byte column segments.
-__global__ __launch_bounds__(64) void k_trmv_lt(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
-    __shared__ dd sm[64 * (MV_U + 1)];
-    __shared__ dd xs[MV_U];
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(128) void k_trmv_lt(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    __shared__ dd P[2][MU][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const idx_t j0 = (idx_t)blockIdx.x * 64, j = j0 + lane;
     const idx_t jc = j < n ? j : n - 1;
-    dd t = dd_mul(x[jc], L[jc + jc * lda]);
-    // staging pattern: row r = lane % 16, columns lane / 16 + 4 q
-    const int r_ld = lane % MV_U, c_ld = lane / MV_U;
-    dd g[MV_U];
-    dd xn = mk(0.0);
-    auto fetch = [&](idx_t i0) {
-        idx_t ir = i0 + r_ld;
-        ir = ir < n ? ir : n - 1;
+    const idx_t ibeg = j0 + 1;
+    const int nt = ibeg < n ? (int)((n - ibeg + MU - 1) / MU) : 0;
+    if (wave == 1) {
+        const dd *col = L + jc * lda;
+        dd a[MU], xv;
+        auto fetch = [&](int t) {
+            const idx_t i0 = ibeg + (idx_t)t * MU;
 #pragma unroll
-        for (int q = 0; q < MV_U; q++) {
-            idx_t c = j0 + c_ld + 4 * q;
-            c = c < n ? c : n - 1;
-            g[q] = L[ir + c * lda];
-        }
-        if (lane < MV_U) xn = x[ir];
-    };
-    // fetches are unconditional (indices clamped): the next tile is always
-    // in flight while the current chain runs
-    idx_t i0 = j0 + 1;
-    fetch(i0);
-    while (i0 < n) {
-#pragma unroll
-        for (int q = 0; q < MV_U; q++) sm[(c_ld + 4 * q) * (MV_U + 1) + r_ld] = g[q];
-        if (lane < MV_U) xs[lane] = xn;
-        __syncthreads();
-        const idx_t i1 = i0 + MV_U;
-        fetch(i1);
-        dd p[MV_U];
-#pragma unroll
-        for (int r = 0; r < MV_U; r++) p[r] = dd_mul(sm[lane * (MV_U + 1) + r], xs[r]);
-        if (i0 > j && i1 <= n) {
-#pragma unroll
-            for (int r = 0; r < MV_U; r++) t = dd_add_eq(t, p[r]);
-        } else {
-#pragma unroll
-            for (int r = 0; r < MV_U; r++) {
-                idx_t i = i0 + r;
-                if (i < n && i > j) t = dd_add_eq(t, p[r]);
+            for (int u = 0; u < MU; u++) {
+                idx_t i = i0 + u;
+                a[u] = col[i < n ? i : n - 1];
             }
-        }
+            idx_t il = i0 + lane;
+            xv = x[il < n ? il : n - 1];
+        };
+        auto produce = [&](int buf) {
+#pragma unroll
+            for (int u = 0; u < MU; u++) P[buf][u][lane] = dd_mul(a[u], readlane_dd(xv, u));
+        };
+        fetch(0);
+        if (nt > 0) produce(0);
+        fetch(1);
         __syncthreads();
-        i0 = i1;
+        for (int t = 0; t < nt; t++) {
+            if (t + 1 < nt) {
+                produce((t + 1) & 1);
+                fetch(t + 2);
+            }
+            __syncthreads();
+        }
+    } else {
+        dd acc = dd_mul(x[jc], L[jc + jc * lda]);
+        __syncthreads();
+        for (int t = 0; t < nt; t++) {
+            const idx_t i0 = ibeg + (idx_t)t * MU;
+            const int buf = t & 1;
+            dd pv[MU];
+#pragma unroll
+            for (int u = 0; u < MU; u++) pv[u] = P[buf][u][lane];
+            __builtin_amdgcn_sched_barrier(0);  // all LDS reads issued before the chain
+            if (i0 > j && i0 + MU <= n) {
+#pragma unroll
+                for (int u = 0; u < MU; u++) acc = dd_add_eq(acc, pv[u]);
+            } else {
+#pragma unroll
+                for (int u = 0; u < MU; u++) {
+                    idx_t i = i0 + u;
+                    if (i < n && i > j) acc = dd_add_eq(acc, pv[u]);
+                }
+            }
+            __syncthreads();
+        }
+        if (j < n) w[j] = acc;
     }
-    if (j < n) w[j] = t;
 }
+
 // w_i = x_i*L_ii, then += x_j*L_ij for j = i-1 down to 0, skipping x_j == 0 (Rtrmv 'L','N').
-__global__ __launch_bounds__(64) void k_trmv_ln(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
-    __shared__ dd xs[MV_U];
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(128) void k_trmv_ln(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    __shared__ dd P[2][MU][64];
+    __shared__ unsigned M[2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const idx_t i0 = (idx_t)blockIdx.x * 64, i = i0 + lane;
     const idx_t ic = i < n ? i : n - 1;
-    const dd *Li = L + ic;
-    dd xi = x[ic];
-    dd t = dd_is_zero(xi) ? xi : dd_mul(xi, Li[ic * lda]);
-    // columns jt, jt-1, ..., jt-U+1 per tile, from the wave's largest row - 1 down to 0
-    idx_t jt = (i0 + 63 < n ? i0 + 63 : n - 1) - 1;
-    dd a[MV_U];
-    dd xn = mk(0.0);
-    auto fetch = [&](idx_t top) {
+    // columns jtop - t*MU - u, from the workgroup's largest row - 1 down to 0
+    const idx_t jtop = (i0 + 63 < n ? i0 + 63 : n - 1) - 1;
+    const int nt = jtop >= 0 ? (int)((jtop + MU) / MU) : 0;
+    if (wave == 1) {
+        const dd *Li = L + ic;
+        dd a[MU], xv;
+        auto fetch = [&](int t) {
+            const idx_t jt = jtop - (idx_t)t * MU;
 #pragma unroll
-        for (int u = 0; u < MV_U; u++) {
-            idx_t jj = top - u;
-            a[u] = Li[(jj >= 0 ? jj : 0) * lda];
-        }
-        if (lane < MV_U) xn = x[top - lane >= 0 ? top - lane : 0];
-    };
-    fetch(jt);  // unconditional (indices clamped)
-    while (jt >= 0) {
-        if (lane < MV_U) xs[lane] = xn;
-        __syncthreads();
-        dd p[MV_U], xv[MV_U];
-#pragma unroll
-        for (int u = 0; u < MV_U; u++) {
-            xv[u] = xs[u];
-            p[u] = dd_mul(xv[u], a[u]);
-        }
-        const idx_t jn = jt - MV_U;
-        fetch(jn);
-        if (jt < i && jn >= -1) {
-#pragma unroll
-            for (int u = 0; u < MV_U; u++)
-                if (!dd_is_zero(xv[u])) t = dd_add_eq(t, p[u]);
-        } else {
-#pragma unroll
-            for (int u = 0; u < MV_U; u++) {
+            for (int u = 0; u < MU; u++) {
                 idx_t jj = jt - u;
-                if (jj >= 0 && jj < i && !dd_is_zero(xv[u])) t = dd_add_eq(t, p[u]);
+                a[u] = Li[(jj >= 0 ? jj : 0) * lda];
             }
-        }
+            idx_t jl = jt - lane;
+            xv = x[jl >= 0 ? jl : 0];
+        };
+        auto produce = [&](int t, int buf) {
+            const idx_t jt = jtop - (idx_t)t * MU;
+            unsigned mask = 0;
+#pragma unroll
+            for (int u = 0; u < MU; u++) {
+                const dd xj = readlane_dd(xv, u);
+                P[buf][u][lane] = dd_mul(xj, a[u]);
+                if (jt - u >= 0 && !dd_is_zero(xj)) mask |= 1u << u;
+            }
+            if (lane == 0) M[buf] = mask;
+        };
+        fetch(0);
+        if (nt > 0) produce(0, 0);
+        fetch(1);
         __syncthreads();
-        jt = jn;
+        for (int t = 0; t < nt; t++) {
+            if (t + 1 < nt) {
+                produce(t + 1, (t + 1) & 1);
+                fetch(t + 2);
+            }
+            __syncthreads();
+        }
+    } else {
+        const dd xi = x[ic];
+        dd acc = dd_is_zero(xi) ? xi : dd_mul(xi, L[ic + ic * lda]);
+        __syncthreads();
+        for (int t = 0; t < nt; t++) {
+            const idx_t jt = jtop - (idx_t)t * MU;
+            const int buf = t & 1;
+            const unsigned mask = M[buf];
+            dd pv[MU];
+#pragma unroll
+            for (int u = 0; u < MU; u++) pv[u] = P[buf][u][lane];
+            __builtin_amdgcn_sched_barrier(0);  // all LDS reads issued before the chain
+            if (jt < i && mask == 0xffffffffu) {
+#pragma unroll
+                for (int u = 0; u < MU; u++) acc = dd_add_eq(acc, pv[u]);
+            } else {
+#pragma unroll
+                for (int u = 0; u < MU; u++)
+                    if (mv_valid(mask, u) && jt - u < i) acc = dd_add_eq(acc, pv[u]);
+            }
+            __syncthreads();
+        }
+        if (i < n) w[i] = acc;
     }
-    if (i < n) w[i] = t;
 }
 void trmv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, const dd *x, dd *out) {
     if (n <= 0) return;
     StatScope st("mv", s, (double)n * n);
     if (trans == 'T' || trans == 't')
-        k_trmv_lt<<<cdiv(n, 64), 64, 0, s>>>(n, L, lda, x, out);
+        k_trmv_lt<<<cdiv(n, 64), 128, 0, s>>>(n, L, lda, x, out);
     else
-        k_trmv_ln<<<cdiv(n, 64), 64, 0, s>>>(n, L, lda, x, out);
+        k_trmv_ln<<<cdiv(n, 64), 128, 0, s>>>(n, L, lda, x, out);
 }
 // y_i = (beta handling) then += (alpha*x_j)*A_ij, j ascending (Rgemv 'N').
-__global__ __launch_bounds__(64) void k_gemv_n(idx_t m, idx_t n, dd alpha, int ak, const dd *A, idx_t lda, const dd *x, dd beta, int bk,
-                                               dd *y) {
-    __shared__ dd xs[MV_U];
-    const int lane = threadIdx.x;
+// Same producer/consumer structure as the triangular mat-vecs.
+__global__ __launch_bounds__(128) void k_gemv_n(idx_t m, idx_t n, dd alpha, int ak, const dd *A, idx_t lda, const dd *x, dd beta, int bk,
+                                                dd *y) {
+    __shared__ dd P[2][MU][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const idx_t i = (idx_t)blockIdx.x * 64 + lane;
     const idx_t ic = i < m ? i : m - 1;
-    const dd *Ai = A + ic;
-    dd t = (bk == 0) ? mk(0.0) : (bk == 1 ? y[ic] : dd_mul(beta, y[ic]));
-    dd a[MV_U];
-    dd xn = mk(0.0);
-    auto fetch = [&](idx_t j0) {
+    const int nt = (int)((n + MU - 1) / MU);
+    if (wave == 1) {
+        const dd *Ai = A + ic;
+        dd a[MU], xv;
+        auto fetch = [&](int t) {
+            const idx_t j0 = (idx_t)t * MU;
 #pragma unroll
-        for (int u = 0; u < MV_U; u++) {
-            idx_t jj = j0 + u;
-            a[u] = Ai[(jj < n ? jj : n - 1) * lda];
-        }
-        if (lane < MV_U) xn = x[j0 + lane < n ? j0 + lane : n - 1];
-    };
-    idx_t j0 = 0;
-    fetch(0);
-    while (j0 < n) {
-        if (lane < MV_U) xs[lane] = (ak == 1) ? xn : (ak == 2 ? dd_neg(xn) : dd_mul(alpha, xn));
+            for (int u = 0; u < MU; u++) {
+                idx_t jj = j0 + u;
+                a[u] = Ai[(jj < n ? jj : n - 1) * lda];
+            }
+            idx_t jl = j0 + lane;
+            xv = x[jl < n ? jl : n - 1];
+            xv = (ak == 1) ? xv : (ak == 2 ? dd_neg(xv) : dd_mul(alpha, xv));
+        };
+        auto produce = [&](int buf) {
+#pragma unroll
+            for (int u = 0; u < MU; u++) P[buf][u][lane] = dd_mul(readlane_dd(xv, u), a[u]);
+        };
+        fetch(0);
+        if (nt > 0) produce(0);
+        fetch(1);
         __syncthreads();
-        dd p[MV_U];
-#pragma unroll
-        for (int u = 0; u < MV_U; u++) p[u] = dd_mul(xs[u], a[u]);
-        const idx_t j1 = j0 + MV_U;
-        fetch(j1);  // unconditional (indices clamped)
-        if (j1 <= n) {
-#pragma unroll
-            for (int u = 0; u < MV_U; u++) t = dd_add_eq(t, p[u]);
-        } else {
-#pragma unroll
-            for (int u = 0; u < MV_U; u++)
-                if (j0 + u < n) t = dd_add_eq(t, p[u]);
+        for (int t = 0; t < nt; t++) {
+            if (t + 1 < nt) {
+                produce((t + 1) & 1);
+                fetch(t + 2);
+            }
+            __syncthreads();
         }
+    } else {
+        dd acc = (bk == 0) ? mk(0.0) : (bk == 1 ? y[ic] : dd_mul(beta, y[ic]));
         __syncthreads();
-        j0 = j1;
+        for (int t = 0; t < nt; t++) {
+            const idx_t j0 = (idx_t)t * MU;
+            const int buf = t & 1;
+            dd pv[MU];
+#pragma unroll
+            for (int u = 0; u < MU; u++) pv[u] = P[buf][u][lane];
+            __builtin_amdgcn_sched_barrier(0);  // all LDS reads issued before the chain
+            if (j0 + MU <= n) {
+#pragma unroll
+                for (int u = 0; u < MU; u++) acc = dd_add_eq(acc, pv[u]);
+            } else {
+#pragma unroll
+                for (int u = 0; u < MU; u++)
+                    if (j0 + u < n) acc = dd_add_eq(acc, pv[u]);
+            }
+            __syncthreads();
+        }
+        if (i < m) y[i] = acc;
     }
-    if (i < m) y[i] = t;
 }
 void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y) {
     if (m <= 0) return;
@@ -875,7 +925,7 @@ void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, c
         return;
     }
     StatScope st("mv", s, 2.0 * (double)m * n);
-    k_gemv_n<<<cdiv(m, 64), 64, 0, s>>>(m, n, alpha, ak, A, lda, x, beta, bk, y);
+    k_gemv_n<<<cdiv(m, 64), 128, 0, s>>>(m, n, alpha, ak, A, lda, x, beta, bk, y);
 }
 
 // ----------------------------------------------------------- elementwise --
@@ -1005,6 +1055,17 @@ void dot(hipStream_t s, idx_t n, const dd *x, const dd *y, dd *out, int T, dd *w
         dot_batch(s, task, 1, T);  // short chunks: one workgroup, a lane per chunk
     else
         dot_tasks_long(s, task, 1, T, partial);
+}
+
+// Many dots in one launch; maxn is the longest.  Short chunks: one
+// workgroup per task; long ones: one workgroup per (chunk, task).
+void dot_tasks(hipStream_t s, const DotTask *d_tasks, int ntasks, idx_t maxn, int T, dd *partial) {
+    if (ntasks <= 0) return;
+    StatScope st("dot", s, 2.0 * (double)maxn * ntasks);
+    if (maxn / T <= 64)
+        dot_batch(s, d_tasks, ntasks, T);
+    else
+        dot_tasks_long(s, d_tasks, ntasks, T, partial);
 }
 
 // Batched dots: one workgroup per task, partial sums in LDS (T <= 4096).

@@ -67,6 +67,8 @@ struct DotTask {
 void dot(hipStream_t s, idx_t n, const dd *x, const dd *y, dd *out, int T, dd *work, unsigned *counter);
 // Many independent dots, one workgroup each (tasks on device).
 void dot_batch(hipStream_t s, const DotTask *d_tasks, int ntasks, int T);
+// Many dots in one launch (maxn = longest task); partial: >= T * ntasks dd.
+void dot_tasks(hipStream_t s, const DotTask *d_tasks, int ntasks, idx_t maxn, int T, dd *partial);
 // Same with y = ybase + yoff per task.
 struct DotTaskOff {
     const dd *x;

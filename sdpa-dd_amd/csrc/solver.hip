@@ -218,9 +218,14 @@ class Solver {
   public:
     Solver(Context &c, const HostProblem &h, DeviceProblem &d, const Params &p) : ctx(c), hp(h), dp(d), prm(p) {}
     ~Solver() {
+        if (s2) (void)hipStreamSynchronize(s2);
         for (void *q : mem) (void)hipFree(q);
         if (h_sc) (void)hipHostFree(h_sc);
         if (h_diag) (void)hipHostFree(h_diag);
+        if (h_infos) (void)hipHostFree(h_infos);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (s2) (void)hipStreamDestroy(s2);
     }
     int run(SolveOutput &out, std::vector<dd> *xo, std::vector<dd> *yo, std::vector<dd> *zo);
 
@@ -241,6 +246,16 @@ class Solver {
     // diagonals (2 nDim) for SolveInfo.
     dd *d_sc = nullptr, *h_sc = nullptr, *d_diag = nullptr, *h_diag = nullptr;
     int slot_C = 0, slot_b = 0, slot_max = 0;
+    // Fixed dense inner-product task lists: (X, Z) per block, and (X, dZ),
+    // (dX, Z), (dX, dZ) per block; partial sums of their chunks.
+    sdk::DotTask *d_tasks_xz = nullptr, *d_tasks3 = nullptr;
+    dd *d_partial = nullptr;
+    idx_t max_blk_elems = 0;
+    // second stream for the Z factorisation, per-block Cholesky info slots
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int64_t *d_infos = nullptr, *h_infos = nullptr;
+    dd *d_rinv2 = nullptr;
     double t_schur = 0, t_chol = 0, t_solve = 0, t_lanczos = 0, t_inverse = 0;
 
     dd *alloc(idx_t n) {
@@ -267,21 +282,18 @@ class Solver {
     }
 
     // ---- dense linear-space helpers (sdpa_linear.cpp Lal::*) ----
-    // DenseLinearSpace inner product: per-block Rdot, summed in block order.
-    ddr inner_dense(const dd *a, const dd *b) {
-        for (int l = 0; l < nB; l++) dot((idx_t)bsz(l) * bsz(l), a + dp.boff[l], b + dp.boff[l], l);
+    // DenseLinearSpace inner product X . Z: per-block Rdot (all blocks in one
+    // launch), summed in block order.
+    ddr inner_xz() {
+        sdk::dot_tasks(s, d_tasks_xz, nB, max_blk_elems, ctx.dot_T, d_partial);
         const dd *h = get_scalars(0, nB);
         ddr r = 0.0;
         for (int l = 0; l < nB; l++) r += ddr(h[l]);
         return r;
     }
-    // Three dense inner products with one synchronisation.
-    void inner_dense3(const dd *a0, const dd *b0, const dd *a1, const dd *b1, const dd *a2, const dd *b2, ddr &r0, ddr &r1,
-                      ddr &r2) {
-        const dd *as[3] = {a0, a1, a2}, *bs2[3] = {b0, b1, b2};
-        for (int q = 0; q < 3; q++)
-            for (int l = 0; l < nB; l++)
-                dot((idx_t)bsz(l) * bsz(l), as[q] + dp.boff[l], bs2[q] + dp.boff[l], q * nB + l);
+    // X . dZ, dX . Z and dX . dZ with one launch and one synchronisation.
+    void inner_dirs(ddr &r0, ddr &r1, ddr &r2) {
+        sdk::dot_tasks(s, d_tasks3, 3 * nB, max_blk_elems, ctx.dot_T, d_partial);
         const dd *h = get_scalars(0, 3 * nB);
         ddr r[3];
         for (int q = 0; q < 3; q++) {
@@ -336,31 +348,41 @@ class Solver {
         normP = max_abs(primalVec, m);
         normD = max_abs(dualMat, nn);
     }
-    bool inv_chol_all(dd *A, dd *invChol, dd *work, dd *inv) {
-        double t0 = now();
+    // Jal::getInvChol / getInvCholAndInv (sdpa_jordan.cpp:123-170) for every
+    // block of A, enqueued on `st` without synchronisation; block l's
+    // Cholesky info goes to infos[l].
+    void inv_chol_async(hipStream_t st, dd *A, dd *invChol, dd *work, dd *inv, int64_t *infos, dd *rinv) {
         for (int l = 0; l < nB; l++) {
             int n = bsz(l);
             idx_t o = dp.boff[l];
-            sdk::copy(s, (idx_t)n * n, A + o, work + o);
-            sdk::potrf_lower(s, n, work + o, n, ctx.d_info, ctx.d_rinv);
-            int64_t info = 0;
-            (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
-            sync();
-            if (info != 0) {
-                t_inverse += now() - t0;
-                return false;
-            }
-            sdk::zero_upper(s, n, work + o);
-            sdk::inv_lower(s, n, work + o, n, invChol + o, n);
+            sdk::copy(st, (idx_t)n * n, A + o, work + o);
+            sdk::potrf_lower(st, n, work + o, n, infos + l, rinv);
+            sdk::zero_upper(st, n, work + o);
+            sdk::inv_lower(st, n, work + o, n, invChol + o, n);
         }
         if (inv)
             for (int l = 0; l < nB; l++) {
                 int n = bsz(l);
                 idx_t o = dp.boff[l];
-                sdk::trmm_lltn_self(s, n, invChol + o, n, inv + o, n);
+                sdk::trmm_lltn_self(st, n, invChol + o, n, inv + o, n);
             }
+    }
+    // Solutions::update's factorisations: X on the main stream and Z on the
+    // second one, concurrently.  Returns false when either Cholesky fails.
+    // (The results after a failed block are never used: the loop ends.)
+    bool inv_chol_xz() {
+        double t0 = now();
+        (void)hipEventRecord(ev_fork, s);
+        (void)hipStreamWaitEvent(s2, ev_fork, 0);
+        inv_chol_async(s, X, invCholX, DLS1, nullptr, d_infos, ctx.d_rinv);
+        inv_chol_async(s2, Z, invCholZ, DLS2, invZ, d_infos + nB, d_rinv2);
+        (void)hipEventRecord(ev_join, s2);
+        (void)hipStreamWaitEvent(s, ev_join, 0);
+        (void)hipMemcpyAsync(h_infos, d_infos, sizeof(int64_t) * 2 * nB, hipMemcpyDeviceToHost, s);
         sync();
         t_inverse += now() - t0;
+        for (int q = 0; q < 2 * nB; q++)
+            if (h_infos[q] != 0) return false;
         return true;
     }
     ChainPool pool;
@@ -433,6 +455,36 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     if (!(d_sc = alloc(slot_C + 3)) || !(d_diag = alloc(2 * (idx_t)nDim))) return -2;
     if (hipHostMalloc((void **)&h_sc, sizeof(dd) * (slot_C + 3)) != hipSuccess) return -2;
     if (hipHostMalloc((void **)&h_diag, sizeof(dd) * 2 * (size_t)nDim) != hipSuccess) return -2;
+    if (hipHostMalloc((void **)&h_infos, sizeof(int64_t) * 2 * nB) != hipSuccess) return -2;
+    if (hipMalloc((void **)&d_infos, sizeof(int64_t) * 2 * nB) != hipSuccess) return -2;
+    mem.push_back(d_infos);
+    if (!(d_rinv2 = alloc(64))) return -2;
+    if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess) return -2;
+    if (hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
+        return -2;
+    {
+        std::vector<sdk::DotTask> txz, t3;
+        const dd *a3[3] = {X, DxMat, DxMat}, *b3[3] = {DzMat, Z, DzMat};
+        for (int l = 0; l < nB; l++) {
+            idx_t ne = (idx_t)bsz(l) * bsz(l), o = dp.boff[l];
+            max_blk_elems = std::max(max_blk_elems, ne);
+            txz.push_back({X + o, Z + o, ne, d_sc + l});
+        }
+        for (int q = 0; q < 3; q++)
+            for (int l = 0; l < nB; l++) {
+                idx_t ne = (idx_t)bsz(l) * bsz(l), o = dp.boff[l];
+                t3.push_back({a3[q] + o, b3[q] + o, ne, d_sc + q * nB + l});
+            }
+        if (hipMalloc((void **)&d_tasks_xz, sizeof(sdk::DotTask) * txz.size()) != hipSuccess) return -2;
+        mem.push_back(d_tasks_xz);
+        if (hipMalloc((void **)&d_tasks3, sizeof(sdk::DotTask) * t3.size()) != hipSuccess) return -2;
+        mem.push_back(d_tasks3);
+        if (hipMemcpy(d_tasks_xz, txz.data(), sizeof(sdk::DotTask) * txz.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d_tasks3, t3.data(), sizeof(sdk::DotTask) * t3.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return -2;
+        if (!(d_partial = alloc((idx_t)3 * nB * ctx.dot_T))) return -2;
+    }
     double t_start = now();
     for (dd **b : bufs) sdk::fill(s, nn, mk(0.0), *b);
 
@@ -597,7 +649,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         // DirectionParameter::MehrotraCorrector (sdpa_parts.cpp:499-529)
         {
             ddr xv, uz, uv;
-            inner_dense3(X, DzMat, DxMat, Z, DxMat, DzMat, xv, uz, uv);
+            inner_dirs(xv, uz, uv);
             ddr muTarget = muCurrent + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nDim;
             betaV = muTarget / muCurrent;
             if (betaV < 1.0) betaV = betaV * betaV;
@@ -640,7 +692,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             }
             if (switchOn && (phase == noINFO || phase == pFEAS || phase == dFEAS)) {
                 ddr xv, uz, uv;
-                inner_dense3(X, DzMat, DxMat, Z, DxMat, DzMat, xv, uz, uv);
+                inner_dirs(xv, uz, uv);
                 ddr thetaMax = max((1.0 - alphaP) * thetaP, (1.0 - alphaD) * thetaD);
                 ddr muNew = muCurrent + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nDim;
                 while (thetaMax * thetaMax * muInitial > xi * muNew) {
@@ -677,9 +729,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             res.message = "cannot move";
             break;
         }
-        bool okx = inv_chol_all(X, invCholX, DLS1, nullptr);
-        bool okz = inv_chol_all(Z, invCholZ, DLS2, invZ);
-        if (!okx || !okz) {
+        if (!inv_chol_xz()) {
             res.message = "cannot move";
             break;
         }
@@ -687,7 +737,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             thetaP = abs((1.0 - alphaP) * thetaP);
             thetaD = abs((1.0 - alphaD) * thetaD);
         }
-        muCurrent = inner_dense(X, Z);
+        muCurrent = inner_xz();
         muCurrent /= nDim;
         residuals(curNormP, curNormD);
         thetaP = (initNormP > 1.0e-10) ? curNormP / initNormP : ddr(0.0);
