@@ -269,6 +269,9 @@ void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx
 
 // -------------------------------------------------------------- Cholesky --
 static constexpr int PB = 64;  // Cholesky panel width
+static constexpr int PD_NT = 512;                                  // threads of the diagonal-block kernel
+static constexpr int PD_UPD = PD_NT - 64;                          // update threads (waves 1..)
+static constexpr int PD_OWN = (PB * (PB + 1) / 2 + PD_UPD - 1) / PD_UPD;  // elements per update thread
 
 // Lane broadcast of a dd from a wave-uniform lane index.
 __device__ __forceinline__ double readlane_d(double v, int k) {
@@ -297,18 +300,37 @@ __device__ __forceinline__ void tri_rc(int e, int &r, int &c) {
 // sqrt(a_kk); l_ik = (1/l_kk) * a_ik).  One workgroup.  Thread 0 is the
 // pivot thread (lane 0 of wave 0, the rest of that wave idles): during step
 // k it applies column k to a_{k+1,k+1} and forms l_{k+1,k+1} and its
-// reciprocal, while waves 1-3 scale column k and apply it to the rest of
-// the trailing triangle.  Two barriers per column.
-__global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
+// reciprocal, while waves 1-7 scale column k and apply it to the rest of
+// the trailing triangle (each update thread owns a fixed set of elements;
+// it issues all their LDS reads before the arithmetic).  Two barriers per
+// column.
+__global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
     __shared__ dd S[PB][PB + 1];
     __shared__ dd Lk[PB];
     __shared__ dd rl[PB];
     __shared__ int failed;
     const int tid = threadIdx.x;
     if (*info != 0) return;
-    for (int e = tid; e < nb * nb; e += 256) {
+    for (int e = tid; e < nb * nb; e += PD_NT) {
         int i = e % nb, j = e / nb;
         if (i >= j) S[i][j] = A[(K + i) + (K + j) * lda];
+    }
+    // Waves 1-7 own the lower-triangle elements round-robin (row-major
+    // enumeration); own[q] packs (i, j) as i | j << 8.  Unused slots hold
+    // (0, 0), which the update skips like every element with j <= k.
+    int own[PD_OWN];
+    {
+        const int t = tid - 64;
+#pragma unroll
+        for (int q = 0; q < PD_OWN; q++) {
+            own[q] = 0;
+            const int e = t + PD_UPD * q;
+            if (t >= 0 && e < nb * (nb + 1) / 2) {
+                int ri, ci;
+                tri_rc(e, ri, ci);
+                own[q] = ri | (ci << 8);
+            }
+        }
     }
     __syncthreads();
     auto pivot = [&](int k) {
@@ -331,13 +353,14 @@ __global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, i
         if (failed) return;
         const dd r = rl[k];
         if (tid >= 64)
-            for (int i = k + 1 + (tid - 64); i < nb; i += 192) {
+            for (int i = k + 1 + (tid - 64); i < nb; i += PD_UPD) {
                 dd v = dd_mul(r, S[i][k]);
                 Lk[i] = v;
                 S[i][k] = v;
             }
         __syncthreads();
         const int rem = nb - k - 1;
+        const int own_pivot = (k + 1) | ((k + 1) << 8);
         if (tid < 64) {
             if (tid == 0 && rem > 0) {
                 dd l = Lk[k + 1];
@@ -345,22 +368,28 @@ __global__ __launch_bounds__(256) void k_potrf_diag(dd *A, idx_t lda, idx_t K, i
                 pivot(k + 1);
             }
         } else {
-            const int cnt = rem * (rem + 1) / 2;
-            for (int e = tid - 63; e < cnt; e += 192) {
-                int ri, ci;
-                tri_rc(e, ri, ci);
-                const int i = k + 1 + ri, j = k + 1 + ci;
-                dd ljk = Lk[j];
-                if (!dd_is_zero(ljk)) S[i][j] = dd_sub(S[i][j], dd_mul(ljk, Lk[i]));
+            dd sv[PD_OWN], lj[PD_OWN], li[PD_OWN];
+#pragma unroll
+            for (int q = 0; q < PD_OWN; q++) {
+                const int i = own[q] & 0xff, j = own[q] >> 8;
+                sv[q] = S[i][j];
+                lj[q] = Lk[j];
+                li[q] = Lk[i];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < PD_OWN; q++) {
+                const int i = own[q] & 0xff, j = own[q] >> 8;
+                if (j > k && own[q] != own_pivot && !dd_is_zero(lj[q])) S[i][j] = dd_sub(sv[q], dd_mul(lj[q], li[q]));
             }
         }
         __syncthreads();
     }
-    for (int e = tid; e < nb * nb; e += 256) {
+    for (int e = tid; e < nb * nb; e += PD_NT) {
         int i = e % nb, j = e / nb;
         if (i >= j) A[(K + i) + (K + j) * lda] = S[i][j];
     }
-    for (int k = tid; k < nb; k += 256) rinv[k] = rl[k];
+    for (int k = tid; k < nb; k += PD_NT) rinv[k] = rl[k];
 }
 
 // B := B * L^{-T} on an nb-column panel (Rtrsm 'R','L','T','N' order with
@@ -428,7 +457,7 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
         idx_t below = n - K - nb;
         {
             StatScope st("potrf", s, (double)nb * nb * nb / 3.0 + (double)below * nb * nb);
-            k_potrf_diag<<<1, 256, 0, s>>>(A, lda, K, nb, d_info, scratch);
+            k_potrf_diag<<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
             if (below > 0)
                 k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda, nb,
                                                              scratch, d_info);
@@ -443,12 +472,12 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
 }
 
 // ------------------------------------------------------ triangular solves --
-// Triangular solves with one 256-thread workgroup, blocked by SB rows.  Per
-// block, wave 0 solves the diagonal block with one row per lane and the
-// solved value broadcast by lane shuffles (no barriers), then every thread
-// applies the block's SB updates to one of the remaining rows, in the
-// reference's column order.  Each element sees exactly the operation
-// sequence of the column loop of Rtrsv.
+// Triangular solves with one 256-thread workgroup, in blocks of SB rows
+// with one block of look-ahead.  In round R, wave 0 applies block R-1's
+// solved values to block R (one row per lane) and then solves block R's
+// diagonal part, broadcasting each solved value with readlane, while waves
+// 1-3 apply block R-1's values to every later row.  Each element still sees
+// exactly the operation sequence of the column loop of Rtrsv.
 static constexpr int SB = 32;
 __device__ __forceinline__ dd shfl_dd(dd v, int src) {
     dd r;
@@ -461,84 +490,112 @@ __device__ __forceinline__ dd shfl_dd(dd v, int src) {
 // x_i -= x_j * L_ij for i > j }.
 __global__ __launch_bounds__(256) void k_trsv_ln(idx_t n, const dd *L, idx_t lda, dd *x) {
     __shared__ dd D[SB][SB + 1];
-    __shared__ dd v[SB];
-    __shared__ int nzv[SB];
-    const int tid = threadIdx.x;
-    for (idx_t b = 0; b < n; b += SB) {
+    __shared__ dd v[2][SB];
+    __shared__ int nzv[2][SB];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int nblk = (int)((n + SB - 1) / SB);
+    for (int R = 0; R < nblk; R++) {
+        const idx_t b = (idx_t)R * SB, bp = b - SB;  // this block and the previous (full) one
         const int nb = (int)((n - b) < SB ? (n - b) : SB);
-        for (int e = tid; e < SB * SB; e += 256) {
-            int r = e % SB, c = e / SB;
-            if (r < nb && c < nb && r >= c) D[r][c] = L[(b + r) + (b + c) * lda];
-        }
-        __syncthreads();
+        const int cur = R & 1, prv = cur ^ 1;
         if (tid < 64) {
-            const int lane = tid;
-            dd xr = lane < nb ? x[b + lane] : mk(0.0);
-            for (int jj = 0; jj < nb; jj++) {
-                int nz = !dd_is_zero(xr);
-                if (lane == jj && nz) xr = dd_div(xr, D[jj][jj]);
-                const dd t = shfl_dd(xr, jj);
-                nz = __shfl(nz, jj, 64);
-                if (nz && lane > jj && lane < nb) xr = dd_sub(xr, dd_mul(t, D[lane][jj]));
-                if (lane == jj) {
-                    v[jj] = xr;
-                    nzv[jj] = nz;
+            if (nb > 0) {
+                for (int e = lane; e < SB * SB; e += 64) {
+                    int r = e % SB, c = e / SB;
+                    if (r < nb && c < nb && r >= c) D[r][c] = L[(b + r) + (b + c) * lda];
                 }
+                dd xr = lane < nb ? x[b + lane] : mk(0.0);
+                if (R > 0 && lane < nb) {
+                    // block R-1 (full, SB columns) applied to this row
+                    dd a[SB];
+#pragma unroll
+                    for (int jj = 0; jj < SB; jj++) a[jj] = L[(b + lane) + (bp + jj) * lda];
+#pragma unroll
+                    for (int jj = 0; jj < SB; jj++)
+                        if (nzv[prv][jj]) xr = dd_sub(xr, dd_mul(v[prv][jj], a[jj]));
+                }
+                __builtin_amdgcn_wave_barrier();
+                for (int jj = 0; jj < nb; jj++) {
+                    int nz = !dd_is_zero(xr);
+                    if (lane == jj && nz) xr = dd_div(xr, D[jj][jj]);
+                    const dd t = shfl_dd(xr, jj);
+                    nz = __shfl(nz, jj, 64);
+                    if (nz && lane > jj && lane < nb) xr = dd_sub(xr, dd_mul(t, D[lane][jj]));
+                    if (lane == jj) {
+                        v[cur][jj] = xr;
+                        nzv[cur][jj] = nz;
+                    }
+                }
+                if (lane < nb) x[b + lane] = xr;
             }
-            if (lane < nb) x[b + lane] = xr;
-        }
-        __syncthreads();
-        for (idx_t i = b + nb + tid; i < n; i += 256) {
-            dd a[SB];
+        } else if (R > 0) {
+            // rows after this block get block R-1's update
+            for (idx_t i = b + nb + (tid - 64); i < n; i += 192) {
+                dd a[SB];
 #pragma unroll
-            for (int jj = 0; jj < SB; jj++) a[jj] = L[i + (b + (jj < nb ? jj : 0)) * lda];
-            dd xi = x[i];
+                for (int jj = 0; jj < SB; jj++) a[jj] = L[i + (bp + jj) * lda];
+                dd xi = x[i];
 #pragma unroll
-            for (int jj = 0; jj < SB; jj++)
-                if (jj < nb && nzv[jj]) xi = dd_sub(xi, dd_mul(v[jj], a[jj]));
-            x[i] = xi;
+                for (int jj = 0; jj < SB; jj++)
+                    if (nzv[prv][jj]) xi = dd_sub(xi, dd_mul(v[prv][jj], a[jj]));
+                x[i] = xi;
+            }
         }
         __syncthreads();
     }
 }
 
 // x := L^{-T} x ('L','T','N'): for j descending, x_j = (x_j - sum over
-// i > j, descending, of L_ij x_i) / L_jj.
+// i > j, descending, of L_ij x_i) / L_jj.  Blocks from the bottom.
 __global__ __launch_bounds__(256) void k_trsv_lt(idx_t n, const dd *L, idx_t lda, dd *x) {
     __shared__ dd D[SB][SB + 1];
-    __shared__ dd v[SB];
-    const int tid = threadIdx.x;
-    for (idx_t top = n; top > 0; top -= SB) {
-        const idx_t b = top > SB ? top - SB : 0;
-        const int nb = (int)(top - b);
-        for (int e = tid; e < SB * SB; e += 256) {
-            int r = e % SB, c = e / SB;
-            if (r < nb && c < nb && r >= c) D[r][c] = L[(b + r) + (b + c) * lda];
-        }
-        __syncthreads();
+    __shared__ dd v[2][SB];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int nblk = (int)((n + SB - 1) / SB);
+    // block R covers rows [lo(R), hi(R)); R = 0 is the bottom block
+    auto blo = [&](int R) -> idx_t { idx_t t = n - (idx_t)(R + 1) * SB; return t > 0 ? t : 0; };
+    auto bhi = [&](int R) -> idx_t { return n - (idx_t)R * SB; };
+    for (int R = 0; R < nblk; R++) {
+        const idx_t b = blo(R);
+        const int nb = (int)(bhi(R) - b);
+        const idx_t bp = R > 0 ? blo(R - 1) : 0;  // previous (lower, full) block: rows [bp, bp + SB)
+        const int cur = R & 1, prv = cur ^ 1;
         if (tid < 64) {
-            const int lane = tid;
-            dd xr = lane < nb ? x[b + lane] : mk(0.0);
-            for (int jj = nb - 1; jj >= 0; jj--) {
-                if (lane == jj) xr = dd_div(xr, D[jj][jj]);
-                const dd t = shfl_dd(xr, jj);
-                if (lane < jj) xr = dd_sub(xr, dd_mul(D[jj][lane], t));
-            }
-            if (lane < nb) {
-                x[b + lane] = xr;
-                v[lane] = xr;
-            }
-        }
-        __syncthreads();
-        for (idx_t k = tid; k < b; k += 256) {
-            dd a[SB];
+            if (nb > 0) {
+                for (int e = lane; e < SB * SB; e += 64) {
+                    int r = e % SB, c = e / SB;
+                    if (r < nb && c < nb && r >= c) D[r][c] = L[(b + r) + (b + c) * lda];
+                }
+                dd xr = lane < nb ? x[b + lane] : mk(0.0);
+                if (R > 0 && lane < nb) {
+                    dd a[SB];
 #pragma unroll
-            for (int jj = 0; jj < SB; jj++) a[jj] = L[(b + (jj < nb ? jj : 0)) + k * lda];
-            dd xk = x[k];
+                    for (int jj = 0; jj < SB; jj++) a[jj] = L[(bp + jj) + (b + lane) * lda];
 #pragma unroll
-            for (int jj = SB - 1; jj >= 0; jj--)
-                if (jj < nb) xk = dd_sub(xk, dd_mul(a[jj], v[jj]));
-            x[k] = xk;
+                    for (int jj = SB - 1; jj >= 0; jj--) xr = dd_sub(xr, dd_mul(a[jj], v[prv][jj]));
+                }
+                __builtin_amdgcn_wave_barrier();
+                for (int jj = nb - 1; jj >= 0; jj--) {
+                    if (lane == jj) xr = dd_div(xr, D[jj][jj]);
+                    const dd t = shfl_dd(xr, jj);
+                    if (lane < jj) xr = dd_sub(xr, dd_mul(D[jj][lane], t));
+                }
+                if (lane < nb) {
+                    x[b + lane] = xr;
+                    v[cur][lane] = xr;
+                }
+            }
+        } else if (R > 0) {
+            // rows above this block get block R-1's update
+            for (idx_t k = tid - 64; k < b; k += 192) {
+                dd a[SB];
+#pragma unroll
+                for (int jj = 0; jj < SB; jj++) a[jj] = L[(bp + jj) + k * lda];
+                dd xk = x[k];
+#pragma unroll
+                for (int jj = SB - 1; jj >= 0; jj--) xk = dd_sub(xk, dd_mul(a[jj], v[prv][jj]));
+                x[k] = xk;
+            }
         }
         __syncthreads();
     }
@@ -707,213 +764,166 @@ void syrk_ln(hipStream_t s, idx_t n, idx_t k, dd alpha, const dd *A, idx_t lda, 
 // ---------------------------------------------------------------- mat-vec --
 // Mat-vecs inside Lanczos.  Each output is one dependent chain of dd
 // additions in the reference's order, so the kernels are latency-bound on
-// that chain.  A workgroup has two waves for 64 outputs: wave 1 (producer)
-// loads the matrix and vector tiles and forms the products of tile t+1 into
-// an LDS ring while wave 0 (consumer) adds the products of tile t into the
-// 64 chains, one per lane.  The producer issues its loads one tile ahead.
-static constexpr int MU = 32;
-__device__ __forceinline__ bool mv_valid(unsigned mask, int u) { return (mask >> u) & 1u; }
+// that chain.  A workgroup serves 64 outputs with four waves: wave 0
+// (consumer) adds the products of tile t into the 64 chains, one per lane,
+// while waves 1-3 (producers) form the products of tile t+1 into an LDS
+// ring, each taking every third term of the tile, with their matrix loads
+// issued two tiles ahead.
+static constexpr int MU = 32;   // terms per tile
+static constexpr int MP = 3;    // producer waves
+static constexpr int MQ = (MU + MP - 1) / MP;
 
-// w_j = x_j*L_jj, then += L_ij*x_i for i = j+1 .. n-1 ascending (Rtrmv 'L','T').
-__global__ __launch_bounds__(128) void k_trmv_lt(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
-    __shared__ dd P[2][MU][64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const idx_t j0 = (idx_t)blockIdx.x * 64, j = j0 + lane;
-    const idx_t jc = j < n ? j : n - 1;
-    const idx_t ibeg = j0 + 1;
-    const int nt = ibeg < n ? (int)((n - ibeg + MU - 1) / MU) : 0;
-    if (wave == 1) {
-        const dd *col = L + jc * lda;
-        dd a[MU], xv;
-        auto fetch = [&](int t) {
-            const idx_t i0 = ibeg + (idx_t)t * MU;
-#pragma unroll
-            for (int u = 0; u < MU; u++) {
-                idx_t i = i0 + u;
-                a[u] = col[i < n ? i : n - 1];
-            }
-            idx_t il = i0 + lane;
-            xv = x[il < n ? il : n - 1];
-        };
-        auto produce = [&](int buf) {
-#pragma unroll
-            for (int u = 0; u < MU; u++) P[buf][u][lane] = dd_mul(a[u], readlane_dd(xv, u));
-        };
-        fetch(0);
-        if (nt > 0) produce(0);
-        fetch(1);
-        __syncthreads();
-        for (int t = 0; t < nt; t++) {
-            if (t + 1 < nt) {
-                produce((t + 1) & 1);
-                fetch(t + 2);
-            }
-            __syncthreads();
-        }
-    } else {
-        dd acc = dd_mul(x[jc], L[jc + jc * lda]);
-        __syncthreads();
-        for (int t = 0; t < nt; t++) {
-            const idx_t i0 = ibeg + (idx_t)t * MU;
-            const int buf = t & 1;
-            dd pv[MU];
-#pragma unroll
-            for (int u = 0; u < MU; u++) pv[u] = P[buf][u][lane];
-            __builtin_amdgcn_sched_barrier(0);  // all LDS reads issued before the chain
-            if (i0 > j && i0 + MU <= n) {
-#pragma unroll
-                for (int u = 0; u < MU; u++) acc = dd_add_eq(acc, pv[u]);
-            } else {
-#pragma unroll
-                for (int u = 0; u < MU; u++) {
-                    idx_t i = i0 + u;
-                    if (i < n && i > j) acc = dd_add_eq(acc, pv[u]);
-                }
-            }
-            __syncthreads();
-        }
-        if (j < n) w[j] = acc;
+// The three mat-vecs differ only in how term u of tile t maps to a matrix
+// element, a vector entry and the product's operand order; the consumer
+// adds term u when mask bit u is set and keep(t, u) holds for its lane.
+struct MvGemvN {  // y_i += (alpha*x_j)*A_ij, j ascending (Rgemv 'N')
+    idx_t m, n, lda;
+    const dd *A, *x;
+    dd alpha;
+    int ak;
+    __device__ idx_t row(int lane, idx_t base) const { idx_t i = base + lane; return i < m ? i : m - 1; }
+    __device__ idx_t vidx(int t, int u) const { idx_t j = (idx_t)t * MU + u; return j < n ? j : n - 1; }
+    __device__ dd mat(idx_t r, int t, int u) const { return A[r + vidx(t, u) * lda]; }
+    __device__ dd vec(int t, int u) const {
+        dd xv = x[vidx(t, u)];
+        return (ak == 1) ? xv : (ak == 2 ? dd_neg(xv) : dd_mul(alpha, xv));
     }
-}
+    __device__ dd prod(dd a, dd v) const { return dd_mul(v, a); }
+    __device__ bool term_ok(int t, int u, const dd &) const { return (idx_t)t * MU + u < n; }
+    __device__ bool keep(idx_t, int, int) const { return true; }
+    __device__ bool full(idx_t, int t) const { return (idx_t)(t + 1) * MU <= n; }
+};
+struct MvTrmvLT {  // w_j = x_j*L_jj, += L_ij*x_i for i > j ascending (Rtrmv 'L','T')
+    idx_t n, lda, ibeg;
+    const dd *L, *x;
+    __device__ idx_t row(int lane, idx_t base) const { idx_t j = base + lane; return j < n ? j : n - 1; }
+    __device__ idx_t vidx(int t, int u) const { idx_t i = ibeg + (idx_t)t * MU + u; return i < n ? i : n - 1; }
+    __device__ dd mat(idx_t r, int t, int u) const { return L[vidx(t, u) + r * lda]; }
+    __device__ dd vec(int t, int u) const { return x[vidx(t, u)]; }
+    __device__ dd prod(dd a, dd v) const { return dd_mul(a, v); }
+    __device__ bool term_ok(int t, int u, const dd &) const { return ibeg + (idx_t)t * MU + u < n; }
+    __device__ bool keep(idx_t j, int t, int u) const { return ibeg + (idx_t)t * MU + u > j; }
+    __device__ bool full(idx_t j, int t) const { return ibeg + (idx_t)t * MU > j && ibeg + (idx_t)(t + 1) * MU <= n; }
+};
+struct MvTrmvLN {  // w_i = x_i*L_ii, += x_j*L_ij for j = i-1 down to 0, x_j != 0 (Rtrmv 'L','N')
+    idx_t n, lda, jtop;
+    const dd *L, *x;
+    __device__ idx_t row(int lane, idx_t base) const { idx_t i = base + lane; return i < n ? i : n - 1; }
+    __device__ idx_t vidx(int t, int u) const { idx_t j = jtop - (idx_t)t * MU - u; return j >= 0 ? j : 0; }
+    __device__ dd mat(idx_t r, int t, int u) const { return L[r + vidx(t, u) * lda]; }
+    __device__ dd vec(int t, int u) const { return x[vidx(t, u)]; }
+    __device__ dd prod(dd a, dd v) const { return dd_mul(v, a); }
+    __device__ bool term_ok(int t, int u, const dd &v) const { return jtop - (idx_t)t * MU - u >= 0 && !dd_is_zero(v); }
+    __device__ bool keep(idx_t i, int t, int u) const { return jtop - (idx_t)t * MU - u < i; }
+    __device__ bool full(idx_t i, int t) const { return jtop - (idx_t)t * MU < i; }
+};
 
-// w_i = x_i*L_ii, then += x_j*L_ij for j = i-1 down to 0, skipping x_j == 0 (Rtrmv 'L','N').
-__global__ __launch_bounds__(128) void k_trmv_ln(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+template <class K>
+__device__ void mv_run(const K &k, int nt, idx_t base, dd acc, dd *out, idx_t nout) {
     __shared__ dd P[2][MU][64];
-    __shared__ unsigned M[2];
+    __shared__ unsigned M[2][MP];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const idx_t i0 = (idx_t)blockIdx.x * 64, i = i0 + lane;
-    const idx_t ic = i < n ? i : n - 1;
-    // columns jtop - t*MU - u, from the workgroup's largest row - 1 down to 0
-    const idx_t jtop = (i0 + 63 < n ? i0 + 63 : n - 1) - 1;
-    const int nt = jtop >= 0 ? (int)((jtop + MU) / MU) : 0;
-    if (wave == 1) {
-        const dd *Li = L + ic;
-        dd a[MU], xv;
-        auto fetch = [&](int t) {
-            const idx_t jt = jtop - (idx_t)t * MU;
+    const idx_t r = k.row(lane, base);
+    if (wave > 0) {
+        const int pw = wave - 1;
+        dd a[2][MQ];
+        dd xv[2];
+        auto fetch = [&](int t, int slot) {
 #pragma unroll
-            for (int u = 0; u < MU; u++) {
-                idx_t jj = jt - u;
-                a[u] = Li[(jj >= 0 ? jj : 0) * lda];
+            for (int q = 0; q < MQ; q++) {
+                const int u = pw + MP * q;
+                if (u < MU) a[slot][q] = k.mat(r, t, u);
             }
-            idx_t jl = jt - lane;
-            xv = x[jl >= 0 ? jl : 0];
+            xv[slot] = lane < MU ? k.vec(t, lane) : mk(0.0);
         };
-        auto produce = [&](int t, int buf) {
-            const idx_t jt = jtop - (idx_t)t * MU;
+        auto produce = [&](int t, int slot, int buf) {
             unsigned mask = 0;
 #pragma unroll
-            for (int u = 0; u < MU; u++) {
-                const dd xj = readlane_dd(xv, u);
-                P[buf][u][lane] = dd_mul(xj, a[u]);
-                if (jt - u >= 0 && !dd_is_zero(xj)) mask |= 1u << u;
+            for (int q = 0; q < MQ; q++) {
+                const int u = pw + MP * q;
+                if (u < MU) {
+                    const dd v = readlane_dd(xv[slot], u);
+                    P[buf][u][lane] = k.prod(a[slot][q], v);
+                    if (k.term_ok(t, u, v)) mask |= 1u << u;
+                }
             }
-            if (lane == 0) M[buf] = mask;
+            if (lane == 0) M[buf][pw] = mask;
         };
-        fetch(0);
-        if (nt > 0) produce(0, 0);
-        fetch(1);
+        // tile t lives in register slot t & 1 and LDS buffer t & 1; the loop
+        // is unrolled by two so that slots are compile-time indices
+        fetch(0, 0);
+        fetch(1, 1);
+        if (nt > 0) produce(0, 0, 0);
+        fetch(2, 0);
         __syncthreads();
-        for (int t = 0; t < nt; t++) {
+        for (int t = 0; t < nt; t += 2) {
             if (t + 1 < nt) {
-                produce(t + 1, (t + 1) & 1);
-                fetch(t + 2);
+                produce(t + 1, 1, 1);
+                fetch(t + 3, 1);
             }
             __syncthreads();
+            if (t + 1 < nt) {
+                if (t + 2 < nt) {
+                    produce(t + 2, 0, 0);
+                    fetch(t + 4, 0);
+                }
+                __syncthreads();
+            }
         }
     } else {
-        const dd xi = x[ic];
-        dd acc = dd_is_zero(xi) ? xi : dd_mul(xi, L[ic + ic * lda]);
         __syncthreads();
         for (int t = 0; t < nt; t++) {
-            const idx_t jt = jtop - (idx_t)t * MU;
             const int buf = t & 1;
-            const unsigned mask = M[buf];
+            const unsigned mask = M[buf][0] | M[buf][1] | M[buf][2];
             dd pv[MU];
 #pragma unroll
             for (int u = 0; u < MU; u++) pv[u] = P[buf][u][lane];
             __builtin_amdgcn_sched_barrier(0);  // all LDS reads issued before the chain
-            if (jt < i && mask == 0xffffffffu) {
+            if (mask == 0xffffffffu && k.full(base + lane, t)) {
 #pragma unroll
                 for (int u = 0; u < MU; u++) acc = dd_add_eq(acc, pv[u]);
             } else {
 #pragma unroll
                 for (int u = 0; u < MU; u++)
-                    if (mv_valid(mask, u) && jt - u < i) acc = dd_add_eq(acc, pv[u]);
+                    if (((mask >> u) & 1u) && k.keep(base + lane, t, u)) acc = dd_add_eq(acc, pv[u]);
             }
             __syncthreads();
         }
-        if (i < n) w[i] = acc;
+        if (base + lane < nout) out[base + lane] = acc;
     }
+}
+
+__global__ __launch_bounds__(256) void k_trmv_lt(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    const idx_t j0 = (idx_t)blockIdx.x * 64;
+    MvTrmvLT k{n, lda, j0 + 1, L, x};
+    const int nt = k.ibeg < n ? (int)((n - k.ibeg + MU - 1) / MU) : 0;
+    const idx_t jc = k.row(threadIdx.x & 63, j0);
+    mv_run(k, nt, j0, dd_mul(x[jc], L[jc + jc * lda]), w, n);
+}
+__global__ __launch_bounds__(256) void k_trmv_ln(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    const idx_t i0 = (idx_t)blockIdx.x * 64;
+    MvTrmvLN k{n, lda, (i0 + 63 < n ? i0 + 63 : n - 1) - 1, L, x};
+    const int nt = k.jtop >= 0 ? (int)((k.jtop + MU) / MU) : 0;
+    const idx_t ic = k.row(threadIdx.x & 63, i0);
+    const dd xi = x[ic];
+    mv_run(k, nt, i0, dd_is_zero(xi) ? xi : dd_mul(xi, L[ic + ic * lda]), w, n);
 }
 void trmv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, const dd *x, dd *out) {
     if (n <= 0) return;
     StatScope st("mv", s, (double)n * n);
     if (trans == 'T' || trans == 't')
-        k_trmv_lt<<<cdiv(n, 64), 128, 0, s>>>(n, L, lda, x, out);
+        k_trmv_lt<<<cdiv(n, 64), 256, 0, s>>>(n, L, lda, x, out);
     else
-        k_trmv_ln<<<cdiv(n, 64), 128, 0, s>>>(n, L, lda, x, out);
+        k_trmv_ln<<<cdiv(n, 64), 256, 0, s>>>(n, L, lda, x, out);
 }
 // y_i = (beta handling) then += (alpha*x_j)*A_ij, j ascending (Rgemv 'N').
-// Same producer/consumer structure as the triangular mat-vecs.
-__global__ __launch_bounds__(128) void k_gemv_n(idx_t m, idx_t n, dd alpha, int ak, const dd *A, idx_t lda, const dd *x, dd beta, int bk,
+__global__ __launch_bounds__(256) void k_gemv_n(idx_t m, idx_t n, dd alpha, int ak, const dd *A, idx_t lda, const dd *x, dd beta, int bk,
                                                 dd *y) {
-    __shared__ dd P[2][MU][64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const idx_t i = (idx_t)blockIdx.x * 64 + lane;
-    const idx_t ic = i < m ? i : m - 1;
-    const int nt = (int)((n + MU - 1) / MU);
-    if (wave == 1) {
-        const dd *Ai = A + ic;
-        dd a[MU], xv;
-        auto fetch = [&](int t) {
-            const idx_t j0 = (idx_t)t * MU;
-#pragma unroll
-            for (int u = 0; u < MU; u++) {
-                idx_t jj = j0 + u;
-                a[u] = Ai[(jj < n ? jj : n - 1) * lda];
-            }
-            idx_t jl = j0 + lane;
-            xv = x[jl < n ? jl : n - 1];
-            xv = (ak == 1) ? xv : (ak == 2 ? dd_neg(xv) : dd_mul(alpha, xv));
-        };
-        auto produce = [&](int buf) {
-#pragma unroll
-            for (int u = 0; u < MU; u++) P[buf][u][lane] = dd_mul(readlane_dd(xv, u), a[u]);
-        };
-        fetch(0);
-        if (nt > 0) produce(0);
-        fetch(1);
-        __syncthreads();
-        for (int t = 0; t < nt; t++) {
-            if (t + 1 < nt) {
-                produce((t + 1) & 1);
-                fetch(t + 2);
-            }
-            __syncthreads();
-        }
-    } else {
-        dd acc = (bk == 0) ? mk(0.0) : (bk == 1 ? y[ic] : dd_mul(beta, y[ic]));
-        __syncthreads();
-        for (int t = 0; t < nt; t++) {
-            const idx_t j0 = (idx_t)t * MU;
-            const int buf = t & 1;
-            dd pv[MU];
-#pragma unroll
-            for (int u = 0; u < MU; u++) pv[u] = P[buf][u][lane];
-            __builtin_amdgcn_sched_barrier(0);  // all LDS reads issued before the chain
-            if (j0 + MU <= n) {
-#pragma unroll
-                for (int u = 0; u < MU; u++) acc = dd_add_eq(acc, pv[u]);
-            } else {
-#pragma unroll
-                for (int u = 0; u < MU; u++)
-                    if (j0 + u < n) acc = dd_add_eq(acc, pv[u]);
-            }
-            __syncthreads();
-        }
-        if (i < m) y[i] = acc;
-    }
+    const idx_t i0 = (idx_t)blockIdx.x * 64;
+    MvGemvN k{m, n, lda, A, x, alpha, ak};
+    const idx_t ic = k.row(threadIdx.x & 63, i0);
+    const dd init = (bk == 0) ? mk(0.0) : (bk == 1 ? y[ic] : dd_mul(beta, y[ic]));
+    mv_run(k, (int)((n + MU - 1) / MU), i0, init, y, m);
 }
 void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y) {
     if (m <= 0) return;
@@ -925,7 +935,7 @@ void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, c
         return;
     }
     StatScope st("mv", s, 2.0 * (double)m * n);
-    k_gemv_n<<<cdiv(m, 64), 128, 0, s>>>(m, n, alpha, ak, A, lda, x, beta, bk, y);
+    k_gemv_n<<<cdiv(m, 64), 256, 0, s>>>(m, n, alpha, ak, A, lda, x, beta, bk, y);
 }
 
 // ----------------------------------------------------------- elementwise --
