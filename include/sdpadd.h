@@ -33,6 +33,15 @@ const char *sdpadd_last_error(sdpadd_handle h);
  * sum over (mplapack/Rdot_omp.cpp:80-120 with that many OpenMP threads,
  * partial sums added in thread order).  1 <= chunks <= 2048; default 64. */
 int sdpadd_set_dot_partition(sdpadd_handle h, int chunks);
+/* Form of the Newton step used by sdpadd_solve (no counterpart in the
+ * reference, which has only the first form): path 0 = plain, every dd
+ * operation in the reference's order (bit-identical to it for the same dot
+ * partition); path 1 = Ozaki, the n x n GEMMs and the Cholesky trailing
+ * updates run on INT8 MFMA with `slices` 7-bit digit planes (8, 12, 14 or
+ * 16) and the Schur solve is followed by `refine_steps` (0..8) sweeps of dd
+ * iterative refinement against the unfactored Schur matrix.  Default: 0, 16,
+ * 2. */
+int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps);
 int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr);
 int sdpadd_free(sdpadd_handle h, void *d_ptr);
 int sdpadd_upload(sdpadd_handle h, void *d_dst, const void *src, size_t bytes);
@@ -57,6 +66,17 @@ int sdpadd_stats_get(sdpadd_handle h, const char *family, int64_t *launches, dou
 int sdpadd_Rgemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n, int64_t k, const double *alpha,
                  const double *d_A, int64_t lda, const double *d_B, int64_t ldb, const double *beta, double *d_C,
                  int64_t ldc);
+/* Ozaki form of Rgemm (the products of mplapack/Rgemm_omp.cpp:41-127 on
+ * INT8 MFMA): C = beta*C + alpha*(op(A) op(B)) with alpha = +1 or -1, beta
+ * = 0 or 1, op(X) = X ('N') or X^T ('T'); lower != 0 writes only entries
+ * i >= j.  Each row of A and column of op(B) is split into `slices`
+ * (8, 12, 14 or 16) signed 7-bit digit planes under its own power-of-two
+ * scale; plane products accumulate exactly in int32 and are recombined in
+ * dd.  The error of C_ij is about k * 2^(-7*slices) * max_l |A_il| *
+ * max_l |B_lj| (16 slices: dd accuracy normwise).  k <= 8192. */
+int sdpadd_ozaki_gemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n, int64_t k, int alpha,
+                      const double *d_A, int64_t lda, const double *d_B, int64_t ldb, int beta, double *d_C, int64_t ldc,
+                      int lower, int slices);
 /* Rsyrk: uplo 'L', trans 'N' (mplapack/Rsyrk.cpp:31-215). */
 int sdpadd_Rsyrk(sdpadd_handle h, char uplo, char trans, int64_t n, int64_t k, const double *alpha, const double *d_A,
                  int64_t lda, const double *beta, double *d_C, int64_t ldc);

@@ -67,6 +67,16 @@ int sdpadd_set_dot_partition(sdpadd_handle h, int chunks) {
     h->c.dot_T = chunks;
     return 0;
 }
+int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps) {
+    if (!h) return -1;
+    if (path != 0 && path != 1) return bad(h, "path must be 0 (plain) or 1 (Ozaki)");
+    if (slices != 8 && slices != 12 && slices != 14 && slices != 16) return bad(h, "slices must be 8, 12, 14 or 16");
+    if (refine_steps < 0 || refine_steps > 8) return bad(h, "refine_steps must be in [0, 8]");
+    h->c.path = path;
+    h->c.oz_slices = slices;
+    h->c.refine_steps = refine_steps;
+    return 0;
+}
 int sdpadd_stats_enable(sdpadd_handle h, int on) {
     if (!h) return -1;
     sdk::stats_enable(on != 0);
@@ -110,6 +120,19 @@ int sdpadd_Rgemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n
         ldb < std::max<int64_t>(1, up(transb, 'N') ? k : n))
         return bad(h, "Rgemm: bad dimension");
     sdk::gemm(h->c.stream, up(transb, 'T') ? 'T' : 'N', m, n, k, D(alpha), DP(d_A), lda, DP(d_B), ldb, D(beta), DP(d_C), ldc);
+    return check(h);
+}
+int sdpadd_ozaki_gemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n, int64_t k, int alpha,
+                      const double *d_A, int64_t lda, const double *d_B, int64_t ldb, int beta, double *d_C, int64_t ldc,
+                      int lower, int slices) {
+    if (!h) return -1;
+    if (m < 0 || n < 0 || k < 0 || lda < std::max<int64_t>(1, up(transa, 'N') ? m : k) || ldc < std::max<int64_t>(1, m) ||
+        ldb < std::max<int64_t>(1, up(transb, 'N') ? k : n))
+        return bad(h, "ozaki_gemm: bad dimension");
+    int rc = oz::gemm(h->c.stream, h->c.ozw, transa, transb, m, n, k, alpha, DP(d_A), lda, DP(d_B), ldb, beta, DP(d_C),
+                      ldc, lower != 0, slices);
+    if (rc == -1) return bad(h, "ozaki_gemm: unsupported argument (alpha +-1, beta 0/1, k <= 8192, slices 8/12/14/16)");
+    if (rc != 0) return bad(h, "ozaki_gemm: out of device memory");
     return check(h);
 }
 int sdpadd_Rsyrk(sdpadd_handle h, char uplo, char trans, int64_t n, int64_t k, const double *alpha, const double *d_A,

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "ozaki.hpp"
 
 namespace sdp {
 
@@ -19,6 +20,13 @@ struct Context {
     int64_t *d_info = nullptr;
     dd *d_rinv = nullptr;       // Cholesky panel reciprocals (>= 64)
     dd *d_max_work = nullptr;   // max-abs partials (>= 256)
+    oz::Work ozw;               // digit planes of the Ozaki products
+    // Solver form: 0 = plain (the reference's dd operation order), 1 = Ozaki
+    // (INT8-MFMA GEMMs and Cholesky trailing updates with oz_slices digit
+    // planes, refine_steps sweeps of dd iterative refinement of dy).
+    int path = 0;
+    int oz_slices = 16;
+    int refine_steps = 2;
     std::string last_error;
 
     int init(int dev);

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "ozaki.hpp"
 
 namespace sdk {
 
@@ -469,6 +470,46 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
             gemm(s, 'T', below, below, nb, mk(-1.0), A21, lda, A21, lda, mk(1.0), A22, lda, 1, 0, 0, 0, true, 0);
         }
     }
+}
+
+// Cholesky with Ozaki trailing updates (the precision path): panels of
+// OZ_NB columns are factored with the dd kernels above (diagonal blocks,
+// panel solves and in-panel dd GEMM updates); the lower triangle right of a
+// panel is updated with the INT8-MFMA product A22 -= L21 L21^T (k = panel
+// width).  Not bit-identical to Rpotrf; its accuracy is that of the Ozaki
+// product (normwise dd for 16 digit planes).
+static constexpr idx_t OZ_NB = 256;
+int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch, oz::Work &w, int S) {
+    k_zero_i64<<<1, 1, 0, s>>>(d_info);
+    for (idx_t P = 0; P < n; P += OZ_NB) {
+        const idx_t pw = std::min<idx_t>(OZ_NB, n - P);
+        for (idx_t K = P; K < P + pw; K += PB) {
+            const int nb = (int)std::min<idx_t>(PB, P + pw - K);
+            const idx_t below = n - K - nb;
+            {
+                StatScope st("potrf", s, (double)nb * nb * nb / 3.0 + (double)below * nb * nb);
+                k_potrf_diag<<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
+                if (below > 0)
+                    k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda,
+                                                                 nb, scratch, d_info);
+            }
+            const idx_t rem = P + pw - K - nb;  // panel columns right of this block
+            if (rem > 0) {
+                dd *A21 = A + (K + nb) + K * lda;
+                gemm(s, 'T', below, rem, nb, mk(-1.0), A21, lda, A21, lda, mk(1.0), A + (K + nb) + (K + nb) * lda, lda, 1,
+                     0, 0, 0, true, 0);
+            }
+        }
+        const idx_t below = n - P - pw;
+        if (below > 0) {
+            const dd *L21 = A + (P + pw) + P * lda;
+            StatScope st("ozaki", s, (double)below * below * pw);
+            int rc = oz::gemm(s, w, 'N', 'T', below, below, pw, -1, L21, lda, L21, lda, 1, A + (P + pw) * (1 + lda), lda,
+                              true, S);
+            if (rc != 0) return rc;
+        }
+    }
+    return 0;
 }
 
 // ------------------------------------------------------ triangular solves --

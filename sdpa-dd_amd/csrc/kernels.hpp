@@ -21,6 +21,12 @@ void gemm(hipStream_t s, char tb, idx_t m, idx_t n, idx_t k, dd alpha, const dd 
 // Lower Cholesky in place.  info (device int64) receives 0 or the 1-based
 // index of the first non-positive pivot.  scratch: >= n dd of device memory.
 void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch);
+// Cholesky of the precision path: dd panels, Ozaki INT8-MFMA trailing
+// updates with S digit planes.  Returns 0 or the Ozaki product's error code.
+}  // namespace sdk
+namespace oz { struct Work; }
+namespace sdk {
+int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch, oz::Work &w, int S);
 
 // x := L^{-1} x ('N') or L^{-T} x ('T'), L lower non-unit.
 void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x);

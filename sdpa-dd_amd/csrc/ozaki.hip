@@ -1,0 +1,192 @@
+// Ozaki-scheme dd products on INT8 MFMA: digit-plane slicing, the
+// level-accumulating MFMA kernel and the dd recombination.
+#include "ozaki.hpp"
+
+#include <algorithm>
+
+namespace oz {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+static inline unsigned cdiv(idx_t a, idx_t b) { return (unsigned)((a + b - 1) / b); }
+static inline idx_t roundup(idx_t a, idx_t b) { return (a + b - 1) / b * b; }
+
+bool Work::reserve(idx_t mpad, idx_t npad, idx_t kpad, int S) {
+    auto grow = [](void **p, size_t &cap, size_t want) {
+        if (want <= cap) return true;
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+        cap = 0;
+        if (hipMalloc(p, want) != hipSuccess) return false;
+        cap = want;
+        return true;
+    };
+    return grow((void **)&sa, cap_a, (size_t)S * mpad * kpad) && grow((void **)&sb, cap_b, (size_t)S * npad * kpad) &&
+           grow((void **)&ea, cap_ea, sizeof(int) * mpad) && grow((void **)&eb, cap_eb, sizeof(int) * npad);
+}
+void Work::release() {
+    if (sa) (void)hipFree(sa);
+    if (sb) (void)hipFree(sb);
+    if (ea) (void)hipFree(ea);
+    if (eb) (void)hipFree(eb);
+    *this = Work();
+}
+
+// exps[r] = e with max_k |M(r,k).hi| = f 2^e, f in [0.5, 1) (0 for a zero
+// row); element (r, k) is M[r*rs + k*cs].  Then |M(r,k)| 2^-e < 1.
+__global__ void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, int *exps) {
+    idx_t r = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double mx = 0.0;
+    for (idx_t kk = 0; kk < k; kk++) mx = fmax(mx, fabs(M[r * rs + kk * cs].hi));
+    int e = 0;
+    if (mx > 0.0) (void)frexp(mx, &e);
+    exps[r] = e;
+}
+
+// Digit planes of rows [64 bx, 64 bx + 64) over k chunk by: plane s of row r,
+// chunk kb lives at out[((s nkb + kb) rows_pad + r) 64 + (k mod 64)].
+// x = a 2^-e (|x| < 1); digit d = trunc(128 x) in [-127, 127], x <- 128 x - d
+// (exact: the remainder is formed with a TwoSum).  Padded rows and k are 0.
+template <int S>
+__global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const int *exps,
+                                               int8_t *out, idx_t rows_pad, int nkb) {
+    const int t = threadIdx.x, kq = t >> 6, kb = blockIdx.y;
+    const idx_t r = (idx_t)blockIdx.x * 64 + (t & 63);
+    unsigned wd[S][4];
+#pragma unroll
+    for (int s = 0; s < S; s++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) wd[s][q] = 0u;
+    const int e = r < rows ? exps[r] : 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const idx_t kk = (idx_t)kb * KC + kq * 16 + q;
+        const dd a = (r < rows && kk < k) ? M[r * rs + kk * cs] : mk(0.0);
+        double xh = ldexp(a.hi, -e), xl = ldexp(a.lo, -e);
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const double th = xh * 128.0, tl = xl * 128.0;
+            const double d = trunc(th);
+            const double rh = th - d;
+            const double sm = rh + tl, bb = sm - rh;
+            xl = (rh - (sm - bb)) + (tl - bb);
+            xh = sm;
+            wd[s][q >> 2] |= ((unsigned)((int)d & 0xff)) << (8 * (q & 3));
+        }
+    }
+    if (r >= rows_pad) return;
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        v4i v = {(int)wd[s][0], (int)wd[s][1], (int)wd[s][2], (int)wd[s][3]};
+        *(v4i *)(out + ((((idx_t)s * nkb + kb) * rows_pad + r) * KC + kq * 16)) = v;
+    }
+}
+
+// 64 x 64 output tile per 512-thread workgroup; wave w computes rows
+// 16 (w % 4) .. +16 and columns 32 (w / 4) .. +32 (two 16 x 16 MFMA tiles).
+// Per 64-wide k chunk the planes of both operands are staged in LDS (16-byte
+// chunks XOR-swizzled by row so that a fragment read is conflict-free);
+// level d = s + t of the product accumulates in acc[d] (int32, exact).
+template <int S>
+__global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const int8_t *SA, idx_t mpad, const int *EA,
+                                              const int8_t *SB, idx_t npad, const int *EB, dd *C, idx_t ldc, int alpha,
+                                              int beta, int lower) {
+    __shared__ v4i LA[S][64][4];
+    __shared__ v4i LB[S][64][4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const idx_t r0 = (idx_t)blockIdx.x * 64, c0 = (idx_t)blockIdx.y * 64;
+    if (lower && r0 + 63 < c0) return;
+    const int wr = wave & 3, wc = wave >> 2;
+    const int fr = lane & 15, fq = lane >> 4;
+    v4i acc[S][2];
+#pragma unroll
+    for (int d = 0; d < S; d++) acc[d][0] = acc[d][1] = v4i{0, 0, 0, 0};
+    for (int kb = 0; kb < nkb; kb++) {
+        __syncthreads();
+        for (int c = tid; c < S * 256; c += 512) {
+            const int sl = c >> 8, row = (c >> 2) & 63, kq = c & 3;
+            const int sw = kq ^ ((row >> 2) & 3);
+            LA[sl][row][sw] = *(const v4i *)(SA + ((((idx_t)sl * nkb + kb) * mpad + r0 + row) * KC + kq * 16));
+            LB[sl][row][sw] = *(const v4i *)(SB + ((((idx_t)sl * nkb + kb) * npad + c0 + row) * KC + kq * 16));
+        }
+        __syncthreads();
+        const int ar = wr * 16 + fr;
+        v4i af[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) af[s] = LA[s][ar][fq ^ ((ar >> 2) & 3)];
+        const int br0 = wc * 32 + fr, br1 = br0 + 16;
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            const v4i b0 = LB[t][br0][fq ^ ((br0 >> 2) & 3)];
+            const v4i b1 = LB[t][br1][fq ^ ((br1 >> 2) & 3)];
+#pragma unroll
+            for (int s = 0; s + t < S; s++) {
+                acc[s + t][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[s], b0, acc[s + t][0], 0, 0, 0);
+                acc[s + t][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[s], b1, acc[s + t][1], 0, 0, 0);
+            }
+        }
+    }
+    // recombine: sum_d acc[d] 2^-7(d+2), Horner from the smallest level
+#pragma unroll
+    for (int ct = 0; ct < 2; ct++) {
+        const idx_t j = c0 + wc * 32 + ct * 16 + fr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const idx_t i = r0 + wr * 16 + fq * 4 + r;
+            if (i >= m || j >= n || (lower && i < j)) continue;
+            dd h = mk((double)acc[S - 1][ct][r]);
+#pragma unroll
+            for (int d = S - 2; d >= 0; d--) {
+                h.hi *= 0.0078125;
+                h.lo *= 0.0078125;
+                h = dd_add_d(h, (double)acc[d][ct][r]);
+            }
+            const int ex = EA[i] + EB[j] - 14;
+            h.hi = ldexp(h.hi, ex);
+            h.lo = ldexp(h.lo, ex);
+            if (alpha < 0) h = dd_neg(h);
+            dd *cp = C + i + j * ldc;
+            *cp = beta ? dd_add(*cp, h) : h;
+        }
+    }
+}
+
+template <int S>
+static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *A, idx_t ars, idx_t acs, const dd *B,
+                   idx_t brs, idx_t bcs, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
+    const int nkb = (int)cdiv(k, KC);
+    const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
+    k_rowexp<<<cdiv(m, 256), 256, 0, s>>>(m, k, A, ars, acs, w.ea);
+    k_slice<S><<<dim3(cdiv(mpad, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea, w.sa, mpad, nkb);
+    k_rowexp<<<cdiv(n, 256), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
+    k_slice<S><<<dim3(cdiv(npad, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, w.sb, npad, nkb);
+    k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, w.sb, npad, w.eb, C, ldc,
+                                                                  alpha, beta, lower ? 1 : 0);
+}
+
+int gemm(hipStream_t s, Work &w, char transa, char transb, idx_t m, idx_t n, idx_t k, int alpha, const dd *A, idx_t lda,
+         const dd *B, idx_t ldb, int beta, dd *C, idx_t ldc, bool lower, int S) {
+    if (m < 0 || n < 0 || k < 0 || k > MAX_K || (alpha != 1 && alpha != -1) || (beta != 0 && beta != 1)) return -1;
+    if (m == 0 || n == 0) return 0;
+    if (k == 0) return beta == 1 ? 0 : -1;  // an empty product only leaves C as it is
+    const bool nt = (transb == 'T' || transb == 't'), ta = (transa == 'T' || transa == 't');
+    if (!nt && transb != 'N' && transb != 'n') return -1;
+    if (!ta && transa != 'N' && transa != 'n') return -1;
+    const idx_t kpad = roundup(k, KC);
+    if (!w.reserve(roundup(m, 64), roundup(n, 64), kpad, S)) return -2;
+    // element (i, kk) of op(A): A[i + kk lda] for 'N', A[kk + i lda] for 'T';
+    // element (j, kk) of op(B)^T: B[kk + j ldb] for 'N', B[j + kk ldb] for 'T'
+    const idx_t ars = ta ? lda : 1, acs = ta ? 1 : lda;
+    const idx_t brs = nt ? 1 : ldb, bcs = nt ? ldb : 1;
+    switch (S) {
+    case 16: launch<16>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
+    case 14: launch<14>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
+    case 12: launch<12>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
+    case 8: launch<8>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
+    default: return -1;
+    }
+    return 0;
+}
+
+}  // namespace oz

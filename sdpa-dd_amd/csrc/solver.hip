@@ -36,6 +36,7 @@ void Context::destroy() {
     if (d_info) (void)hipFree(d_info);
     if (d_rinv) (void)hipFree(d_rinv);
     if (d_max_work) (void)hipFree(d_max_work);
+    ozw.release();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
 }
@@ -226,6 +227,7 @@ class Solver {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (s2) (void)hipStreamDestroy(s2);
+        ozw2.release();
     }
     int run(SolveOutput &out, std::vector<dd> *xo, std::vector<dd> *yo, std::vector<dd> *zo);
 
@@ -256,6 +258,8 @@ class Solver {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int64_t *d_infos = nullptr, *h_infos = nullptr;
     dd *d_rinv2 = nullptr;
+    oz::Work ozw2;            // Ozaki planes for the second stream
+    dd *bSave = nullptr, *rvec = nullptr;  // Ozaki form: unfactored B, residual
     double t_schur = 0, t_chol = 0, t_solve = 0, t_lanczos = 0, t_inverse = 0;
 
     dd *alloc(idx_t n) {
@@ -318,6 +322,15 @@ class Solver {
     }
     // ret = a*b*c per block (Jal::ns_jordan_triple_product)
     void triple(dd *ret, const dd *a, const dd *b, const dd *c, dd *work) {
+        if (ctx.path == 1) {
+            for (int l = 0; l < nB; l++) {
+                int n = bsz(l);
+                idx_t o = dp.boff[l];
+                oz_check(oz::gemm(s, ctx.ozw, 'N', 'N', n, n, n, 1, a + o, n, b + o, n, 0, work + o, n, false, ctx.oz_slices));
+                oz_check(oz::gemm(s, ctx.ozw, 'N', 'N', n, n, n, 1, work + o, n, c + o, n, 0, ret + o, n, false, ctx.oz_slices));
+            }
+            return;
+        }
         for (int l = 0; l < nB; l++) {
             int n = bsz(l);
             idx_t o = dp.boff[l];
@@ -327,6 +340,32 @@ class Solver {
             int n = bsz(l);
             idx_t o = dp.boff[l];
             sdk::gemm(s, 'N', n, n, n, mk(1.0), work + o, n, c + o, n, mk(0.0), ret + o, n);
+        }
+    }
+    bool oz_failed = false;
+    void oz_check(int rc) {
+        if (rc != 0) oz_failed = true;
+    }
+    // Cholesky of the Schur matrix or of an X/Z block in the current form.
+    void potrf(hipStream_t st, idx_t n, dd *A, int64_t *info, dd *rinv, oz::Work &w) {
+        if (ctx.path == 1)
+            oz_check(sdk::potrf_lower_oz(st, n, A, n, info, rinv, w, ctx.oz_slices));
+        else
+            sdk::potrf_lower(st, n, A, n, info, rinv);
+    }
+    // dy = B^{-1} g with the factor L in bMat; in the Ozaki form followed by
+    // refine_steps sweeps of r = g - B dy (dd, with the saved B), dy += L^-T L^-1 r.
+    void schur_solve() {
+        sdk::copy(s, m, g, dy);
+        sdk::trsv_lower(s, 'N', m, bMat, m, dy);
+        sdk::trsv_lower(s, 'T', m, bMat, m, dy);
+        if (ctx.path != 1) return;
+        for (int it = 0; it < ctx.refine_steps; it++) {
+            sdk::copy(s, m, g, rvec);
+            sdk::gemv_n(s, m, m, mk(-1.0), bSave, m, dy, mk(1.0), rvec);
+            sdk::trsv_lower(s, 'N', m, bMat, m, rvec);
+            sdk::trsv_lower(s, 'T', m, bMat, m, rvec);
+            sdk::axpy(s, m, mk(1.0), rvec, dy);
         }
     }
     // ret = a + s*b (Lal::plus) over the whole space
@@ -356,7 +395,7 @@ class Solver {
             int n = bsz(l);
             idx_t o = dp.boff[l];
             sdk::copy(st, (idx_t)n * n, A + o, work + o);
-            sdk::potrf_lower(st, n, work + o, n, infos + l, rinv);
+            potrf(st, n, work + o, infos + l, rinv, st == s ? ctx.ozw : ozw2);
             sdk::zero_upper(st, n, work + o);
             sdk::inv_lower(st, n, work + o, n, invChol + o, n);
         }
@@ -459,6 +498,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     if (hipMalloc((void **)&d_infos, sizeof(int64_t) * 2 * nB) != hipSuccess) return -2;
     mem.push_back(d_infos);
     if (!(d_rinv2 = alloc(64))) return -2;
+    if (ctx.path == 1 && (!(bSave = alloc((idx_t)m * m)) || !(rvec = alloc(m)))) return -2;
     if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess) return -2;
     if (hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
@@ -592,7 +632,8 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             sdp::schur(s, dp, X, invZ, bMat, ctx.dot_T);
             sync();
             double t1 = now();
-            sdk::potrf_lower(s, m, bMat, m, ctx.d_info, ctx.d_rinv);
+            if (ctx.path == 1) sdk::copy(s, (idx_t)m * m, bMat, bSave);
+            potrf(s, m, bMat, ctx.d_info, ctx.d_rinv, ctx.ozw);
             int64_t info = 0;
             (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
             sync();
@@ -601,9 +642,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             t_chol += t2 - t1;
             if (info != 0) return false;
         }
-        sdk::copy(s, m, g, dy);
-        sdk::trsv_lower(s, 'N', m, bMat, m, dy);
-        sdk::trsv_lower(s, 'T', m, bMat, m, dy);
+        schur_solve();
         sdp::plus_A(s, dp, dy, DzMat);
         sdk::scal(s, nn, mk(-1.0), DzMat);
         if (phase == pFEAS || phase == noINFO) sdk::axpy(s, nn, mk(1.0), dualMat, DzMat);
@@ -622,8 +661,8 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             betaV = prm.betaBar;
             if (!switchOn) betaV = ddr(2.0);
         }
-        if (!mehrotra(true)) {
-            res.message = "Schur Cholesky failed";
+        if (!mehrotra(true) || oz_failed) {
+            res.message = oz_failed ? "Ozaki workspace allocation failed" : "Schur Cholesky failed";
             break;
         }
         // StepLength::MehrotraPredictor (sdpa_parts.cpp:307-354)

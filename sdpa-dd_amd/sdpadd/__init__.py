@@ -48,6 +48,7 @@ def load_library(path=None):
         "sdpadd_destroy": (ctypes.c_int, [vp]),
         "sdpadd_last_error": (ctypes.c_char_p, [vp]),
         "sdpadd_set_dot_partition": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sdpadd_set_path": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "sdpadd_malloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
         "sdpadd_stats_enable": (ctypes.c_int, [vp, ctypes.c_int]),
         "sdpadd_stats_reset": (ctypes.c_int, [vp]),
@@ -57,6 +58,9 @@ def load_library(path=None):
         "sdpadd_download": (ctypes.c_int, [vp, vp, vp, ctypes.c_size_t]),
         "sdpadd_Rgemm": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_i64, c_dp, vp, c_i64, vp, c_i64,
                                         c_dp, vp, c_i64]),
+        "sdpadd_ozaki_gemm": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_i64, ctypes.c_int, vp, c_i64,
+                                             vp, c_i64,
+                                             ctypes.c_int, vp, c_i64, ctypes.c_int, ctypes.c_int]),
         "sdpadd_Rsyrk": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_dp, vp, c_i64, c_dp, vp, c_i64]),
         "sdpadd_Rtrsm": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_dp,
                                         vp, c_i64, vp, c_i64]),
@@ -144,6 +148,10 @@ class Device:
         if rc < 0:
             raise SdpaddError("%s failed (%d): %s" % (what, rc, self.lib.sdpadd_last_error(self.h).decode()))
         return rc
+
+    def set_path(self, path, slices=16, refine_steps=2):
+        """path 0: plain dd (the reference's operation order); 1: Ozaki INT8-MFMA form."""
+        self.check(self.lib.sdpadd_set_path(self.h, path, slices, refine_steps), "set_path")
 
     def set_dot_partition(self, t):
         self.dot_partition = t
