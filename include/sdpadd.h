@@ -36,11 +36,15 @@ int sdpadd_set_dot_partition(sdpadd_handle h, int chunks);
 /* Form of the Newton step used by sdpadd_solve (no counterpart in the
  * reference, which has only the first form): path 0 = plain, every dd
  * operation in the reference's order (bit-identical to it for the same dot
- * partition); path 1 = Ozaki, the n x n GEMMs and the Cholesky trailing
- * updates run on INT8 MFMA with `slices` 7-bit digit planes (8, 12, 14 or
- * 16) and the Schur solve is followed by `refine_steps` (0..8) sweeps of dd
- * iterative refinement against the unfactored Schur matrix.  Default: 0, 16,
- * 2. */
+ * partition); path 1 = Ozaki Cholesky, the trailing updates of the Schur
+ * and X/Z Cholesky factorisations run on INT8 MFMA with `slices` 7-bit digit
+ * planes (8, 12, 14 or 16) and the Schur solve is followed by up to
+ * `refine_steps` (0..8) sweeps of dd iterative refinement against the
+ * unfactored Schur matrix; path 2 = as 1, and the n x n GEMMs of the
+ * direction run on INT8 MFMA too (digit planes are scaled per row, so
+ * entries far below their row's largest lose relative precision: fine for
+ * dense iterates such as max-cut's, not for theta-type iterates whose
+ * structurally zero entries shrink with mu).  Default: 0, 16, 2. */
 int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps);
 int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr);
 int sdpadd_free(sdpadd_handle h, void *d_ptr);
@@ -163,6 +167,7 @@ typedef struct {
     double mu[2], objValPrimal[2], objValDual[2], normPrimalVec[2], normDualMat[2], relgap[2];
     double seconds;     /* main loop wall time */
     double t_schur, t_chol, t_lanczos, t_inverse;
+    int refine_sweeps;  /* Ozaki forms: refinement sweeps kept over the solve */
 } sdpadd_result;
 
 /* Runs the predictor-corrector loop of sdpa_main.cpp:294-377 on the device.

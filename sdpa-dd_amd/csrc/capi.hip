@@ -69,7 +69,7 @@ int sdpadd_set_dot_partition(sdpadd_handle h, int chunks) {
 }
 int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps) {
     if (!h) return -1;
-    if (path != 0 && path != 1) return bad(h, "path must be 0 (plain) or 1 (Ozaki)");
+    if (path < 0 || path > 2) return bad(h, "path must be 0 (plain), 1 (Ozaki Cholesky) or 2 (Ozaki Cholesky and GEMMs)");
     if (slices != 8 && slices != 12 && slices != 14 && slices != 16) return bad(h, "slices must be 8, 12, 14 or 16");
     if (refine_steps < 0 || refine_steps > 8) return bad(h, "refine_steps must be in [0, 8]");
     h->c.path = path;
@@ -419,6 +419,7 @@ int sdpadd_solve(sdpadd_problem p, const sdpadd_params *prm, sdpadd_result *res)
     res->t_chol = out.t.chol;
     res->t_lanczos = out.t.lanczos;
     res->t_inverse = out.t.inverse;
+    res->refine_sweeps = out.refine_sweeps;
     return 0;
 }
 int sdpadd_trace(sdpadd_problem p, double *out, int cap) {

@@ -21,9 +21,10 @@ struct Context {
     dd *d_rinv = nullptr;       // Cholesky panel reciprocals (>= 64)
     dd *d_max_work = nullptr;   // max-abs partials (>= 256)
     oz::Work ozw;               // digit planes of the Ozaki products
-    // Solver form: 0 = plain (the reference's dd operation order), 1 = Ozaki
-    // (INT8-MFMA GEMMs and Cholesky trailing updates with oz_slices digit
-    // planes, refine_steps sweeps of dd iterative refinement of dy).
+    // Solver form: 0 = plain (the reference's dd operation order); 1 = Ozaki
+    // Cholesky (trailing updates of the Schur and X/Z Cholesky on INT8 MFMA
+    // with oz_slices digit planes, up to refine_steps sweeps of dd iterative
+    // refinement of dy); 2 = as 1 plus the n x n GEMMs of the direction.
     int path = 0;
     int oz_slices = 16;
     int refine_steps = 2;

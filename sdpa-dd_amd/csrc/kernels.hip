@@ -473,13 +473,15 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
 }
 
 // Cholesky with Ozaki trailing updates (the precision path): panels of
-// OZ_NB columns are factored with the dd kernels above (diagonal blocks,
+// 64 or 256 columns are factored with the dd kernels above (diagonal blocks,
 // panel solves and in-panel dd GEMM updates); the lower triangle right of a
 // panel is updated with the INT8-MFMA product A22 -= L21 L21^T (k = panel
 // width).  Not bit-identical to Rpotrf; its accuracy is that of the Ozaki
 // product (normwise dd for 16 digit planes).
-static constexpr idx_t OZ_NB = 256;
 int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch, oz::Work &w, int S) {
+    // panel width: 256 (four k chunks per Ozaki update) for large matrices,
+    // 64 below 1024 so that small ones still take most updates on MFMA
+    const idx_t OZ_NB = n > 1024 ? 256 : 64;
     k_zero_i64<<<1, 1, 0, s>>>(d_info);
     for (idx_t P = 0; P < n; P += OZ_NB) {
         const idx_t pw = std::min<idx_t>(OZ_NB, n - P);
@@ -503,7 +505,6 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
         const idx_t below = n - P - pw;
         if (below > 0) {
             const dd *L21 = A + (P + pw) + P * lda;
-            StatScope st("ozaki", s, (double)below * below * pw);
             int rc = oz::gemm(s, w, 'N', 'T', below, below, pw, -1, L21, lda, L21, lda, 1, A + (P + pw) * (1 + lda), lda,
                               true, S);
             if (rc != 0) return rc;
@@ -556,15 +557,24 @@ __global__ __launch_bounds__(256) void k_trsv_ln(idx_t n, const dd *L, idx_t lda
                         if (nzv[prv][jj]) xr = dd_sub(xr, dd_mul(v[prv][jj], a[jj]));
                 }
                 __builtin_amdgcn_wave_barrier();
-                for (int jj = 0; jj < nb; jj++) {
-                    int nz = !dd_is_zero(xr);
-                    if (lane == jj && nz) xr = dd_div(xr, D[jj][jj]);
-                    const dd t = shfl_dd(xr, jj);
-                    nz = __shfl(nz, jj, 64);
-                    if (nz && lane > jj && lane < nb) xr = dd_sub(xr, dd_mul(t, D[lane][jj]));
-                    if (lane == jj) {
-                        v[cur][jj] = xr;
-                        nzv[cur][jj] = nz;
+                // this lane's row of the diagonal block in registers (entries
+                // right of the diagonal are never used); the step loop is
+                // unrolled so that register and lane indices are constants
+                dd drow[SB];
+                const int lr = lane < nb ? lane : 0;
+#pragma unroll
+                for (int jj = 0; jj < SB; jj++) drow[jj] = D[lr][jj <= lr ? jj : lr];
+#pragma unroll
+                for (int jj = 0; jj < SB; jj++) {
+                    if (jj < nb) {
+                        const int nz = __builtin_amdgcn_readlane(!dd_is_zero(xr), jj);
+                        if (lane == jj && nz) xr = dd_div(xr, drow[jj]);
+                        const dd t = readlane_dd(xr, jj);
+                        if (nz && lane > jj && lane < nb) xr = dd_sub(xr, dd_mul(t, drow[jj]));
+                        if (lane == jj) {
+                            v[cur][jj] = xr;
+                            nzv[cur][jj] = nz;
+                        }
                     }
                 }
                 if (lane < nb) x[b + lane] = xr;
@@ -616,10 +626,19 @@ __global__ __launch_bounds__(256) void k_trsv_lt(idx_t n, const dd *L, idx_t lda
                     for (int jj = SB - 1; jj >= 0; jj--) xr = dd_sub(xr, dd_mul(a[jj], v[prv][jj]));
                 }
                 __builtin_amdgcn_wave_barrier();
-                for (int jj = nb - 1; jj >= 0; jj--) {
-                    if (lane == jj) xr = dd_div(xr, D[jj][jj]);
-                    const dd t = shfl_dd(xr, jj);
-                    if (lane < jj) xr = dd_sub(xr, dd_mul(D[jj][lane], t));
+                // this lane's column of the diagonal block in registers (dcol[jj]
+                // = L_(jj, lane) for jj >= lane); unrolled step loop
+                dd dcol[SB];
+                const int lc = lane < nb ? lane : 0;
+#pragma unroll
+                for (int jj = 0; jj < SB; jj++) dcol[jj] = D[jj >= lc && jj < nb ? jj : lc][lc];
+#pragma unroll
+                for (int jj = SB - 1; jj >= 0; jj--) {
+                    if (jj < nb) {
+                        if (lane == jj) xr = dd_div(xr, dcol[jj]);
+                        const dd t = readlane_dd(xr, jj);
+                        if (lane < jj) xr = dd_sub(xr, dd_mul(dcol[jj], t));
+                    }
                 }
                 if (lane < nb) {
                     x[b + lane] = xr;

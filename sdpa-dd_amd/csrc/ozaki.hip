@@ -1,6 +1,7 @@
 // Ozaki-scheme dd products on INT8 MFMA: digit-plane slicing, the
 // level-accumulating MFMA kernel and the dd recombination.
 #include "ozaki.hpp"
+#include "kernels.hpp"
 
 #include <algorithm>
 
@@ -22,7 +23,7 @@ bool Work::reserve(idx_t mpad, idx_t npad, idx_t kpad, int S) {
         return true;
     };
     return grow((void **)&sa, cap_a, (size_t)S * mpad * kpad) && grow((void **)&sb, cap_b, (size_t)S * npad * kpad) &&
-           grow((void **)&ea, cap_ea, sizeof(int) * mpad) && grow((void **)&eb, cap_eb, sizeof(int) * npad);
+           grow((void **)&ea, cap_ea, sizeof(unsigned) * mpad) && grow((void **)&eb, cap_eb, sizeof(unsigned) * npad);
 }
 void Work::release() {
     if (sa) (void)hipFree(sa);
@@ -32,24 +33,47 @@ void Work::release() {
     *this = Work();
 }
 
-// exps[r] = e with max_k |M(r,k).hi| = f 2^e, f in [0.5, 1) (0 for a zero
-// row); element (r, k) is M[r*rs + k*cs].  Then |M(r,k)| 2^-e < 1.
-__global__ void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, int *exps) {
+// Row exponents: e_r with max_k |M(r,k).hi| = f 2^e, f in [0.5, 1) (0 for
+// a zero row); element (r, k) is M[r*rs + k*cs].  Then |M(r,k)| 2^-e < 1.
+// The grid matches k_slice: a thread takes 16 k of one row, the four
+// threads of a row reduce in LDS, and chunks combine with an integer
+// atomicMax on e + EXP_BIAS (order-free; 0 = no nonzero seen).
+constexpr int EXP_BIAS = 4096;
+__global__ void k_exp_clear(idx_t rows, unsigned *ex) {
     idx_t r = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    double mx = 0.0;
-    for (idx_t kk = 0; kk < k; kk++) mx = fmax(mx, fabs(M[r * rs + kk * cs].hi));
-    int e = 0;
-    if (mx > 0.0) (void)frexp(mx, &e);
-    exps[r] = e;
+    if (r < rows) ex[r] = 0u;
 }
+__global__ __launch_bounds__(256) void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, unsigned *ex) {
+    __shared__ double part[4][64];
+    const int t = threadIdx.x, kq = t >> 6, rl = t & 63;
+    const idx_t r = (idx_t)blockIdx.x * 64 + rl;
+    double mx = 0.0;
+    if (r < rows) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const idx_t kk = (idx_t)blockIdx.y * KC + kq * 16 + q;
+            if (kk < k) mx = fmax(mx, fabs(M[r * rs + kk * cs].hi));
+        }
+    }
+    part[kq][rl] = mx;
+    __syncthreads();
+    if (kq == 0 && r < rows) {
+        mx = fmax(fmax(part[0][rl], part[1][rl]), fmax(part[2][rl], part[3][rl]));
+        if (mx > 0.0) {
+            int e = 0;
+            (void)frexp(mx, &e);
+            atomicMax(ex + r, (unsigned)(e + EXP_BIAS));
+        }
+    }
+}
+__device__ __forceinline__ int exp_of(unsigned b) { return b ? (int)b - EXP_BIAS : 0; }
 
 // Digit planes of rows [64 bx, 64 bx + 64) over k chunk by: plane s of row r,
 // chunk kb lives at out[((s nkb + kb) rows_pad + r) 64 + (k mod 64)].
 // x = a 2^-e (|x| < 1); digit d = trunc(128 x) in [-127, 127], x <- 128 x - d
 // (exact: the remainder is formed with a TwoSum).  Padded rows and k are 0.
 template <int S>
-__global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const int *exps,
+__global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const unsigned *exps,
                                                int8_t *out, idx_t rows_pad, int nkb) {
     const int t = threadIdx.x, kq = t >> 6, kb = blockIdx.y;
     const idx_t r = (idx_t)blockIdx.x * 64 + (t & 63);
@@ -58,7 +82,7 @@ __global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M,
     for (int s = 0; s < S; s++)
 #pragma unroll
         for (int q = 0; q < 4; q++) wd[s][q] = 0u;
-    const int e = r < rows ? exps[r] : 0;
+    const int e = r < rows ? exp_of(exps[r]) : 0;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const idx_t kk = (idx_t)kb * KC + kq * 16 + q;
@@ -89,8 +113,8 @@ __global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M,
 // chunks XOR-swizzled by row so that a fragment read is conflict-free);
 // level d = s + t of the product accumulates in acc[d] (int32, exact).
 template <int S>
-__global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const int8_t *SA, idx_t mpad, const int *EA,
-                                              const int8_t *SB, idx_t npad, const int *EB, dd *C, idx_t ldc, int alpha,
+__global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const int8_t *SA, idx_t mpad, const unsigned *EA,
+                                              const int8_t *SB, idx_t npad, const unsigned *EB, dd *C, idx_t ldc, int alpha,
                                               int beta, int lower) {
     __shared__ v4i LA[S][64][4];
     __shared__ v4i LB[S][64][4];
@@ -142,7 +166,7 @@ __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const i
                 h.lo *= 0.0078125;
                 h = dd_add_d(h, (double)acc[d][ct][r]);
             }
-            const int ex = EA[i] + EB[j] - 14;
+            const int ex = exp_of(EA[i]) + exp_of(EB[j]) - 14;
             h.hi = ldexp(h.hi, ex);
             h.lo = ldexp(h.lo, ex);
             if (alpha < 0) h = dd_neg(h);
@@ -157,10 +181,15 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
                    idx_t brs, idx_t bcs, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
     const int nkb = (int)cdiv(k, KC);
     const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
-    k_rowexp<<<cdiv(m, 256), 256, 0, s>>>(m, k, A, ars, acs, w.ea);
+    k_exp_clear<<<cdiv(m, 256), 256, 0, s>>>(m, w.ea);
+    k_rowexp<<<dim3(cdiv(m, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea);
     k_slice<S><<<dim3(cdiv(mpad, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea, w.sa, mpad, nkb);
-    k_rowexp<<<cdiv(n, 256), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
+    k_exp_clear<<<cdiv(n, 256), 256, 0, s>>>(n, w.eb);
+    k_rowexp<<<dim3(cdiv(n, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
     k_slice<S><<<dim3(cdiv(npad, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, w.sb, npad, nkb);
+    // timed as family "ozaki" with the product's algorithmic 2mnk dd-flops
+    // (the MFMA work is S(S+1)/2 times that in int8 multiply-adds)
+    sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
     k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, w.sb, npad, w.eb, C, ldc,
                                                                   alpha, beta, lower ? 1 : 0);
 }

@@ -27,7 +27,7 @@ constexpr idx_t MAX_K = 8192;  // int32 level sums stay exact up to this k
 // Workspace for one product: digit planes and exponents of both operands.
 struct Work {
     int8_t *sa = nullptr, *sb = nullptr;
-    int *ea = nullptr, *eb = nullptr;
+    unsigned *ea = nullptr, *eb = nullptr;  // biased row exponents
     size_t cap_a = 0, cap_b = 0, cap_ea = 0, cap_eb = 0;
     bool reserve(idx_t mpad, idx_t npad, idx_t kpad, int S);
     void release();

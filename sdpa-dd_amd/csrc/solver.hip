@@ -259,7 +259,9 @@ class Solver {
     int64_t *d_infos = nullptr, *h_infos = nullptr;
     dd *d_rinv2 = nullptr;
     oz::Work ozw2;            // Ozaki planes for the second stream
-    dd *bSave = nullptr, *rvec = nullptr;  // Ozaki form: unfactored B, residual
+    dd *bSave = nullptr, *rvec = nullptr, *dyTry = nullptr;  // Ozaki form: unfactored B, residual
+    int refine_accepted = 0;
+    ddr bmax = 0.0;
     double t_schur = 0, t_chol = 0, t_solve = 0, t_lanczos = 0, t_inverse = 0;
 
     dd *alloc(idx_t n) {
@@ -322,7 +324,7 @@ class Solver {
     }
     // ret = a*b*c per block (Jal::ns_jordan_triple_product)
     void triple(dd *ret, const dd *a, const dd *b, const dd *c, dd *work) {
-        if (ctx.path == 1) {
+        if (ctx.path == 2) {
             for (int l = 0; l < nB; l++) {
                 int n = bsz(l);
                 idx_t o = dp.boff[l];
@@ -348,24 +350,42 @@ class Solver {
     }
     // Cholesky of the Schur matrix or of an X/Z block in the current form.
     void potrf(hipStream_t st, idx_t n, dd *A, int64_t *info, dd *rinv, oz::Work &w) {
-        if (ctx.path == 1)
+        if (ctx.path >= 1)
             oz_check(sdk::potrf_lower_oz(st, n, A, n, info, rinv, w, ctx.oz_slices));
         else
             sdk::potrf_lower(st, n, A, n, info, rinv);
     }
-    // dy = B^{-1} g with the factor L in bMat; in the Ozaki form followed by
-    // refine_steps sweeps of r = g - B dy (dd, with the saved B), dy += L^-T L^-1 r.
+    // dy = B^{-1} g with the factor L in bMat.  In the Ozaki form this is
+    // followed by up to refine_steps sweeps of dd iterative refinement
+    // against the saved B: r = g - B dy, c = L^-T L^-1 r, dy += c.  A sweep
+    // runs only while max|r| is above the dd rounding floor 64 m 2^-104
+    // max|B| max|dy| (below it a correction only moves dy within rounding
+    // noise, which changes the iterates without improving them) and is kept
+    // only if it at least halves max|r|.
+    void residual(const dd *x, dd *r) {
+        sdk::copy(s, m, g, r);
+        sdk::gemv_n(s, m, m, mk(-1.0), bSave, m, x, mk(1.0), r);
+    }
     void schur_solve() {
         sdk::copy(s, m, g, dy);
         sdk::trsv_lower(s, 'N', m, bMat, m, dy);
         sdk::trsv_lower(s, 'T', m, bMat, m, dy);
-        if (ctx.path != 1) return;
+        if (ctx.path == 0 || ctx.refine_steps == 0) return;
+        residual(dy, rvec);
+        ddr rn = max_abs(rvec, m);
+        const ddr floor_scale = 64.0 * m * ldexp(1.0, -104) * bmax;
         for (int it = 0; it < ctx.refine_steps; it++) {
-            sdk::copy(s, m, g, rvec);
-            sdk::gemv_n(s, m, m, mk(-1.0), bSave, m, dy, mk(1.0), rvec);
+            if (!(rn > floor_scale * max_abs(dy, m))) break;
+            sdk::copy(s, m, dy, dyTry);
             sdk::trsv_lower(s, 'N', m, bMat, m, rvec);
             sdk::trsv_lower(s, 'T', m, bMat, m, rvec);
-            sdk::axpy(s, m, mk(1.0), rvec, dy);
+            sdk::axpy(s, m, mk(1.0), rvec, dyTry);
+            residual(dyTry, rvec);
+            ddr rt = max_abs(rvec, m);
+            if (!(rt * 2.0 <= rn)) break;
+            sdk::copy(s, m, dyTry, dy);
+            rn = rt;
+            refine_accepted++;
         }
     }
     // ret = a + s*b (Lal::plus) over the whole space
@@ -498,7 +518,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     if (hipMalloc((void **)&d_infos, sizeof(int64_t) * 2 * nB) != hipSuccess) return -2;
     mem.push_back(d_infos);
     if (!(d_rinv2 = alloc(64))) return -2;
-    if (ctx.path == 1 && (!(bSave = alloc((idx_t)m * m)) || !(rvec = alloc(m)))) return -2;
+    if (ctx.path >= 1 && (!(bSave = alloc((idx_t)m * m)) || !(rvec = alloc(m)) || !(dyTry = alloc(m)))) return -2;
     if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess) return -2;
     if (hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
@@ -632,7 +652,10 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             sdp::schur(s, dp, X, invZ, bMat, ctx.dot_T);
             sync();
             double t1 = now();
-            if (ctx.path == 1) sdk::copy(s, (idx_t)m * m, bMat, bSave);
+            if (ctx.path >= 1) {
+                sdk::copy(s, (idx_t)m * m, bMat, bSave);
+                bmax = max_abs(bSave, (idx_t)m * m);
+            }
             potrf(s, m, bMat, ctx.d_info, ctx.d_rinv, ctx.ozw);
             int64_t info = 0;
             (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
@@ -809,6 +832,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     res.t.chol = t_chol;
     res.t.lanczos = t_lanczos;
     res.t.inverse = t_inverse;
+    res.refine_sweeps = refine_accepted;
     res.iterations = pIteration;
     res.phase = phase;
     int rep = phase;

@@ -29,6 +29,7 @@ struct SolveOutput {
     std::string message;
     Timers t;
     double schur_ddflops = 0;  // algorithmic dd-flops of Schur assembly + factorisation, summed
+    int refine_sweeps = 0;     // Ozaki forms: dd refinement sweeps kept
 };
 
 struct Context;  // device, stream and scratch of a library handle

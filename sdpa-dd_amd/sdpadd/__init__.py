@@ -111,7 +111,8 @@ class Params(ctypes.Structure):
 class Result(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("phase", ctypes.c_int), ("phase_internal", ctypes.c_int)] + [
         (n, ctypes.c_double * 2) for n in ("mu", "objValPrimal", "objValDual", "normPrimalVec", "normDualMat", "relgap")
-    ] + [(n, ctypes.c_double) for n in ("seconds", "t_schur", "t_chol", "t_lanczos", "t_inverse")]
+    ] + [(n, ctypes.c_double) for n in ("seconds", "t_schur", "t_chol", "t_lanczos", "t_inverse")] + [
+        ("refine_sweeps", ctypes.c_int)]
 
 
 def _dp(a):
@@ -150,7 +151,8 @@ class Device:
         return rc
 
     def set_path(self, path, slices=16, refine_steps=2):
-        """path 0: plain dd (the reference's operation order); 1: Ozaki INT8-MFMA form."""
+        """path 0: plain dd (the reference's operation order); 1: Ozaki INT8-MFMA
+        Cholesky trailing updates; 2: as 1 plus the n x n GEMMs."""
         self.check(self.lib.sdpadd_set_path(self.h, path, slices, refine_steps), "set_path")
 
     def set_dot_partition(self, t):
@@ -271,7 +273,7 @@ class Problem:
         out = {
             "iterations": res.iterations, "phase": PHASES[res.phase], "phase_internal": PHASES[res.phase_internal],
             "seconds": res.seconds, "t_schur": res.t_schur, "t_chol": res.t_chol, "t_lanczos": res.t_lanczos,
-            "t_inverse": res.t_inverse,
+            "t_inverse": res.t_inverse, "refine_sweeps": res.refine_sweeps,
         }
         for k in ("mu", "objValPrimal", "objValDual", "normPrimalVec", "normDualMat", "relgap"):
             out[k] = [getattr(res, k)[0], getattr(res, k)[1]]

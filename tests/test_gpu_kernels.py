@@ -148,15 +148,17 @@ def test_rtrsv_rtrmv(gpu, oracle, n, trans):
 
 
 @pytest.mark.parametrize("m,n", [(1, 1), (33, 17), (64, 16), (65, 33), (130, 801), (300, 300)])
-def test_rgemv(gpu, oracle, m, n):
+@pytest.mark.parametrize("ab", [("gen", "zero"), ("m1", "one"), ("one", "gen")])
+def test_rgemv(gpu, oracle, m, n, ab):
     rng = np.random.default_rng(m + n)
     A = rand_dd((m * n,), rng)
     x = rand_dd((n,), rng)
     y = rand_dd((m,), rng)
-    alpha = rand_dd((1,), rng)[0]
-    ref = oracle.rgemv(m, n, alpha, A, m, x, ZERO, y)
+    alpha = {"gen": rand_dd((1,), rng)[0], "m1": np.array([-1.0, 0.0]), "one": ONE}[ab[0]]
+    beta = {"zero": ZERO, "one": ONE, "gen": rand_dd((1,), rng)[0]}[ab[1]]
+    ref = oracle.rgemv(m, n, alpha, A, m, x, beta, y)
     dA, dx, dy = gpu.to_device(A), gpu.to_device(x), gpu.to_device(y)
-    gpu.check(gpu.lib.sdpadd_Rgemv(gpu.h, b"N", m, n, dp(alpha), dA.ptr, m, dx.ptr, 1, dp(ZERO), dy.ptr, 1), "Rgemv")
+    gpu.check(gpu.lib.sdpadd_Rgemv(gpu.h, b"N", m, n, dp(alpha), dA.ptr, m, dx.ptr, 1, dp(beta), dy.ptr, 1), "Rgemv")
     assert same(dy.get(), ref)
 
 
