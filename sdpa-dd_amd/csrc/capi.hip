@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/sdpadd.h"
+#include "adjoint.hpp"
 #include "context.hpp"
 #include "solver.hpp"
 #include "steqr.hpp"
@@ -20,6 +21,7 @@ struct sdpadd_problem_s {
     bool finalized = false;
     std::vector<sdp::IterRecord> trace;
     std::vector<dd> X, y, Z;
+    sdp::Adjoint adj;
 };
 
 namespace {
@@ -174,7 +176,8 @@ int sdpadd_Rtrsv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, c
                  int64_t incx) {
     if (!h || !up(uplo, 'L') || !up(diag, 'N') || incx != 1 || !(up(trans, 'N') || up(trans, 'T')))
         return bad(h, "Rtrsv: unsupported op");
-    sdk::trsv_lower(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x));
+    if (h->c.trsv(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x)) != 0)
+        return bad(h, "Rtrsv: a wait of the cooperative solve ran out");
     return check(h);
 }
 int sdpadd_Rtrmv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, const double *d_A, int64_t lda, double *d_x,
@@ -324,6 +327,7 @@ int sdpadd_problem_finalize(sdpadd_problem p) {
 }
 int sdpadd_problem_destroy(sdpadd_problem p) {
     if (!p) return -1;
+    p->adj.release();
     sdp::free_problem(p->dp);
     delete p;
     return 0;
@@ -436,6 +440,42 @@ int sdpadd_trace(sdpadd_problem p, double *out, int cap) {
         }
     }
     return n;
+}
+int sdpadd_adjoint_setup(sdpadd_problem p, int64_t *na) {
+    if (!p || !p->finalized) return -1;
+    std::string err;
+    int rc = p->adj.setup(p->h->c, p->hp, p->dp, p->X, p->y, p->Z, err);
+    if (rc != 0) p->h->c.set_error(err);
+    if (rc == 0 && na) *na = p->adj.na;
+    return rc;
+}
+int sdpadd_adjoint_entries(sdpadd_problem p, int *con, int *blk, int *row, int *col) {
+    if (!p || !p->finalized) return -1;
+    idx_t q = 0;
+    for (int i = 0; i < p->hp.m; i++)
+        for (size_t k = 0; k < p->hp.A[i].index.size(); k++) {
+            const sdp::Block &b = p->hp.A[i].blk[k];
+            const idx_t cnt = b.dense ? (idx_t)b.n * b.n : (idx_t)b.nnz_count;
+            for (idx_t e = 0; e < cnt; e++, q++) {
+                if (con) con[q] = i;
+                if (blk) blk[q] = p->hp.A[i].index[k];
+                if (row) row[q] = b.dense ? (int)(e % b.n) : b.row[e];
+                if (col) col[q] = b.dense ? (int)(e / b.n) : b.col[e];
+            }
+        }
+    return 0;
+}
+int sdpadd_adjoint(sdpadd_problem p, int nrhs, const double *d_Xbar, const double *d_ybar, const double *pbar,
+                   double *d_bbar, double *d_Cbar, double *d_Abar) {
+    if (!p || !p->finalized || nrhs < 0) return -1;
+    std::vector<dd> pb;
+    if (pbar)
+        for (int k = 0; k < nrhs; k++) pb.push_back(D(pbar + 2 * k));
+    std::string err;
+    int rc = p->adj.apply(p->h->c, p->dp, nrhs, DP(d_Xbar), DP(d_ybar), pbar ? pb.data() : nullptr, DP(d_bbar),
+                          DP(d_Cbar), DP(d_Abar), err);
+    if (rc != 0) p->h->c.set_error(err);
+    return rc;
 }
 int sdpadd_solution(sdpadd_problem p, double *X, double *y, double *Z) {
     if (!p || p->X.empty()) return -1;

@@ -26,6 +26,21 @@ struct Context {
     // with oz_slices digit planes, up to refine_steps sweeps of dd iterative
     // refinement of dy); 2 = as 1 plus the n x n GEMMs of the direction.
     int path = 0;
+    // workspace of the cooperative triangular solves (grown on demand)
+    void *d_tsw = nullptr;
+    size_t tsw_cap = 0;
+    int *d_tsw_sticky = nullptr;  // set by a cooperative solve whose wait ran out
+    int *h_tsw_err = nullptr;     // pinned copy of it
+    // Rtrsv on stream s: cooperative grid for n >= 512, one workgroup
+    // otherwise or when the cooperative launch is unavailable.  check:
+    // synchronise and return -1 if a wait of the cooperative solve ran out
+    // (the flag is then cleared); otherwise returns 0 without waiting and
+    // the caller reads the flag later through trsv_poll() and a sync.
+    int trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x, bool check = true);
+    // Enqueue the copy of the sticky flag into *h_tsw_err.
+    void trsv_poll(hipStream_t s) {
+        (void)hipMemcpyAsync(h_tsw_err, d_tsw_sticky, sizeof(int), hipMemcpyDeviceToHost, s);
+    }
     int oz_slices = 16;
     int refine_steps = 2;
     std::string last_error;

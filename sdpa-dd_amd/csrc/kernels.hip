@@ -514,13 +514,15 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
 }
 
 // ------------------------------------------------------ triangular solves --
-// Triangular solves with one 256-thread workgroup, in blocks of SB rows
+// Triangular solves with one 512-thread workgroup, in blocks of SB rows
 // with one block of look-ahead.  In round R, wave 0 applies block R-1's
 // solved values to block R (one row per lane) and then solves block R's
 // diagonal part, broadcasting each solved value with readlane, while waves
-// 1-3 apply block R-1's values to every later row.  Each element still sees
+// 1-7 apply block R-1's values to every later row (two rows per thread).  Each element still sees
 // exactly the operation sequence of the column loop of Rtrsv.
 static constexpr int SB = 32;
+static constexpr int TS_NT = 512;          // threads of the triangular-solve kernels
+static constexpr int TS_UPD = TS_NT - 64;  // update threads (waves 1..)
 __device__ __forceinline__ dd shfl_dd(dd v, int src) {
     dd r;
     r.hi = __shfl(v.hi, src, 64);
@@ -530,7 +532,7 @@ __device__ __forceinline__ dd shfl_dd(dd v, int src) {
 
 // x := L^{-1} x ('L','N','N'): for j ascending, if x_j != 0 { x_j /= L_jj;
 // x_i -= x_j * L_ij for i > j }.
-__global__ __launch_bounds__(256) void k_trsv_ln(idx_t n, const dd *L, idx_t lda, dd *x) {
+__global__ __launch_bounds__(TS_NT) void k_trsv_ln(idx_t n, const dd *L, idx_t lda, dd *x) {
     __shared__ dd D[SB][SB + 1];
     __shared__ dd v[2][SB];
     __shared__ int nzv[2][SB];
@@ -580,16 +582,31 @@ __global__ __launch_bounds__(256) void k_trsv_ln(idx_t n, const dd *L, idx_t lda
                 if (lane < nb) x[b + lane] = xr;
             }
         } else if (R > 0) {
-            // rows after this block get block R-1's update
-            for (idx_t i = b + nb + (tid - 64); i < n; i += 192) {
-                dd a[SB];
+            // rows after this block get block R-1's update; two rows at a
+            // time per thread, matrix entries loaded 16 columns at a time
+            for (idx_t i = b + nb + (tid - 64); i < n; i += 2 * TS_UPD) {
+                const idx_t i2 = i + TS_UPD;
+                const bool two = i2 < n;
+                const idx_t i2c = two ? i2 : i;
+                dd x1 = x[i], x2 = x[i2c];
 #pragma unroll
-                for (int jj = 0; jj < SB; jj++) a[jj] = L[i + (bp + jj) * lda];
-                dd xi = x[i];
+                for (int h = 0; h < SB; h += 16) {
+                    dd a1[16], a2[16];
 #pragma unroll
-                for (int jj = 0; jj < SB; jj++)
-                    if (nzv[prv][jj]) xi = dd_sub(xi, dd_mul(v[prv][jj], a[jj]));
-                x[i] = xi;
+                    for (int q = 0; q < 16; q++) {
+                        a1[q] = L[i + (bp + h + q) * lda];
+                        a2[q] = L[i2c + (bp + h + q) * lda];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 16; q++)
+                        if (nzv[prv][h + q]) {
+                            const dd vq = v[prv][h + q];
+                            x1 = dd_sub(x1, dd_mul(vq, a1[q]));
+                            x2 = dd_sub(x2, dd_mul(vq, a2[q]));
+                        }
+                }
+                x[i] = x1;
+                if (two) x[i2] = x2;
             }
         }
         __syncthreads();
@@ -598,7 +615,7 @@ __global__ __launch_bounds__(256) void k_trsv_ln(idx_t n, const dd *L, idx_t lda
 
 // x := L^{-T} x ('L','T','N'): for j descending, x_j = (x_j - sum over
 // i > j, descending, of L_ij x_i) / L_jj.  Blocks from the bottom.
-__global__ __launch_bounds__(256) void k_trsv_lt(idx_t n, const dd *L, idx_t lda, dd *x) {
+__global__ __launch_bounds__(TS_NT) void k_trsv_lt(idx_t n, const dd *L, idx_t lda, dd *x) {
     __shared__ dd D[SB][SB + 1];
     __shared__ dd v[2][SB];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -619,25 +636,26 @@ __global__ __launch_bounds__(256) void k_trsv_lt(idx_t n, const dd *L, idx_t lda
                 }
                 dd xr = lane < nb ? x[b + lane] : mk(0.0);
                 if (R > 0 && lane < nb) {
-                    dd a[SB];
 #pragma unroll
-                    for (int jj = 0; jj < SB; jj++) a[jj] = L[(bp + jj) + (b + lane) * lda];
+                    for (int h = SB - 16; h >= 0; h -= 16) {
+                        dd a[16];
 #pragma unroll
-                    for (int jj = SB - 1; jj >= 0; jj--) xr = dd_sub(xr, dd_mul(a[jj], v[prv][jj]));
+                        for (int q = 0; q < 16; q++) a[q] = L[(bp + h + q) + (b + lane) * lda];
+#pragma unroll
+                        for (int q = 15; q >= 0; q--) xr = dd_sub(xr, dd_mul(a[q], v[prv][h + q]));
+                    }
                 }
                 __builtin_amdgcn_wave_barrier();
-                // this lane's column of the diagonal block in registers (dcol[jj]
-                // = L_(jj, lane) for jj >= lane); unrolled step loop
-                dd dcol[SB];
+                // L_(jj, lane) from LDS (read ahead of the chain: the step loop
+                // is unrolled, so the addresses are known)
                 const int lc = lane < nb ? lane : 0;
-#pragma unroll
-                for (int jj = 0; jj < SB; jj++) dcol[jj] = D[jj >= lc && jj < nb ? jj : lc][lc];
 #pragma unroll
                 for (int jj = SB - 1; jj >= 0; jj--) {
                     if (jj < nb) {
-                        if (lane == jj) xr = dd_div(xr, dcol[jj]);
+                        const dd djj = D[jj >= lc ? jj : lc][lc];
+                        if (lane == jj) xr = dd_div(xr, djj);
                         const dd t = readlane_dd(xr, jj);
-                        if (lane < jj) xr = dd_sub(xr, dd_mul(dcol[jj], t));
+                        if (lane < jj) xr = dd_sub(xr, dd_mul(djj, t));
                     }
                 }
                 if (lane < nb) {
@@ -646,28 +664,341 @@ __global__ __launch_bounds__(256) void k_trsv_lt(idx_t n, const dd *L, idx_t lda
                 }
             }
         } else if (R > 0) {
-            // rows above this block get block R-1's update
-            for (idx_t k = tid - 64; k < b; k += 192) {
-                dd a[SB];
+            // rows above this block get block R-1's update; two rows at a
+            // time per thread, 8 entries of each column segment at a time
+            for (idx_t k = tid - 64; k < b; k += 2 * TS_UPD) {
+                const idx_t k2 = k + TS_UPD;
+                const bool two = k2 < b;
+                const idx_t k2c = two ? k2 : k;
+                dd x1 = x[k], x2 = x[k2c];
 #pragma unroll
-                for (int jj = 0; jj < SB; jj++) a[jj] = L[(bp + jj) + k * lda];
-                dd xk = x[k];
+                for (int h = SB - 8; h >= 0; h -= 8) {
+                    dd a1[8], a2[8];
 #pragma unroll
-                for (int jj = SB - 1; jj >= 0; jj--) xk = dd_sub(xk, dd_mul(a[jj], v[prv][jj]));
-                x[k] = xk;
+                    for (int q = 0; q < 8; q++) {
+                        a1[q] = L[(bp + h + q) + k * lda];
+                        a2[q] = L[(bp + h + q) + k2c * lda];
+                    }
+#pragma unroll
+                    for (int q = 7; q >= 0; q--) {
+                        const dd vq = v[prv][h + q];
+                        x1 = dd_sub(x1, dd_mul(a1[q], vq));
+                        x2 = dd_sub(x2, dd_mul(a2[q], vq));
+                    }
+                }
+                x[k] = x1;
+                if (two) x[k2] = x2;
             }
         }
         __syncthreads();
     }
 }
 
+// ---- triangular solves across workgroups ----
+// The single-workgroup solves above are bound by the update waves'
+// memory latency for large n.  These cooperative forms spread the updates
+// over TC_UPD workgroups that each own 256 rows (one row per thread) and run
+// concurrently with the diagonal chain:
+//   workgroup 0, round R: wait until the owners of block R's rows have
+//     applied blocks <= R-2; apply block R-1 to block R (row per lane); solve
+//     block R's diagonal part; publish its values and the counter R+1;
+//   updater u, round J: wait for block J; apply it to its rows in blocks
+//     >= J+2 (in the reference's column order per row); publish progress J+1.
+// Every element sees the same operation sequence as the column loop of
+// Rtrsv.  All waits are bounded spins: a spin that runs out sets the error
+// word and every workgroup leaves (the caller reports the error), so the
+// grid always drains.  The cooperative launch guarantees co-residency.
+static constexpr int TC_ROWS = 256;           // rows per updater workgroup
+static constexpr unsigned TC_SPIN = 1u << 22;  // spin bound per wait (about a second)
+__device__ __forceinline__ int tc_load(const int *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void tc_store(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
+// Spin until *p >= target; false when the error word is set or the bound
+// runs out (then the error word and the sticky word are set).
+__device__ bool tc_wait(const int *p, int target, int *err, int *sticky) {
+    for (unsigned it = 0; it < TC_SPIN; it++) {
+        if (tc_load(p) >= target) return true;
+        if (tc_load(err) != 0) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    tc_store(err, 1);
+    tc_store(sticky, 1);
+    return false;
+}
+
+// Cooperative L^{-1} x in the Rtrsv('L','N','N') order.  Workgroup 0 solves
+// the 32-row blocks in sequence: wave 0 runs the division chain, waves 1-3
+// stage the next block's diagonal and off-diagonal tiles in LDS meanwhile.
+// Block R takes block R-1's update from wave 0; updater workgroup u owns
+// rows [256u, 256u + 256) and applies each block J, once published, to its
+// rows in blocks >= J+2.  flags: [0] blocks solved, [1 + u] blocks applied
+// by updater u, [1 + nupd] error word, all zero at launch.  vg/nzg: solved
+// values of every block and their nonzero flags.
+__global__ __launch_bounds__(256) void k_trsv_ln_coop(idx_t n, const dd *L, idx_t lda, dd *x, int *flags, int nupd, dd *vg,
+                                                      int *nzg, int *sticky) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nblk = (int)((n + SB - 1) / SB);
+    int *err = flags + 1 + nupd;
+    if (blockIdx.x == 0) {
+        __shared__ dd Dg[2][SB][SB + 1];  // Dg[buf][r][c] = L[b + r, b + c], c <= r
+        __shared__ dd Od[2][SB][SB + 1];  // Od[buf][r][c] = L[b + r, b - SB + c]
+        __shared__ int stop;
+        auto stage = [&](int R, int buf, int t0, int nt) {
+            const idx_t b = (idx_t)R * SB;
+            const int nb = (int)((n - b) < SB ? (n - b) : SB);
+            for (int e = t0; e < SB * SB; e += nt) {
+                const int r = e & (SB - 1), c = e >> 5;
+                if (r < nb && c <= r) Dg[buf][r][c] = L[(b + r) + (b + c) * lda];
+                if (R > 0 && r < nb) Od[buf][r][c] = L[(b + r) + (b - SB + c) * lda];
+            }
+        };
+        if (tid == 0) stop = 0;
+        stage(0, 0, tid, 256);
+        __syncthreads();
+        dd prev = mk(0.0);  // wave 0: lane j holds the solved x of block R-1, row j
+        int pnz = 0;
+        for (int R = 0; R < nblk; R++) {
+            const int buf = R & 1;
+            if (wave != 0) {
+                if (R + 1 < nblk) stage(R + 1, buf ^ 1, tid - 64, 192);
+            } else {
+                const idx_t b = (idx_t)R * SB;
+                const int nb = (int)((n - b) < SB ? (n - b) : SB);
+                // owners of this block's rows must have applied blocks <= R-2
+                int ok = 1;
+                if (lane == 0 && R >= 2) {
+                    const int u0 = (int)(b / TC_ROWS), u1 = (int)((b + nb - 1) / TC_ROWS);
+                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, R - 1, err, sticky);
+                }
+                ok = __builtin_amdgcn_readlane(ok, 0);
+                if (!ok) {
+                    if (lane == 0) stop = 1;
+                } else {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    const int lr = lane < nb ? lane : 0;
+                    dd xr = x[b + lr];
+                    int nzr = 0;
+                    if (R > 0) {
+#pragma unroll
+                        for (int q = 0; q < SB; q++) {
+                            const dd a = Od[buf][lr][q];
+                            const int nz = __builtin_amdgcn_readlane(pnz, q);
+                            const dd t = readlane_dd(prev, q);
+                            if (nz) xr = dd_sub(xr, dd_mul(t, a));
+                        }
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < SB; jj++) {
+                        if (jj < nb) {
+                            const dd djj = Dg[buf][lr][jj <= lr ? jj : lr];
+                            const int nz = __builtin_amdgcn_readlane(!dd_is_zero(xr), jj);
+                            if (lane == jj) {
+                                nzr = nz;
+                                if (nz) xr = dd_div(xr, djj);
+                            }
+                            const dd t = readlane_dd(xr, jj);
+                            if (nz && lane > jj && lane < nb) xr = dd_sub(xr, dd_mul(t, djj));
+                        }
+                    }
+                    // rows skip the entries that were zero before their division
+                    pnz = lane < nb ? nzr : 0;
+                    if (lane < nb) {
+                        x[b + lane] = xr;
+                        vg[b + lane] = xr;
+                        nzg[b + lane] = pnz;
+                    }
+                    prev = xr;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                    if (lane == 0) tc_store(flags, R + 1);
+                }
+            }
+            __syncthreads();
+            if (stop) return;
+        }
+        return;
+    }
+    const int u = blockIdx.x - 1;
+    const idx_t lo = (idx_t)u * TC_ROWS, hi = lo + TC_ROWS < n ? lo + TC_ROWS : n;
+    const idx_t i = lo + tid;
+    __shared__ int go;
+    dd xi = i < hi ? x[i] : mk(0.0);
+    for (int J = 0; J < nblk; J++) {
+        const idx_t first = (idx_t)(J + 2) * SB;  // rows that take block J here
+        if (first >= hi) break;
+        const bool act = i < hi && i >= first;
+        const idx_t bp = (idx_t)J * SB;
+        dd a[SB];
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < SB; q++) a[q] = L[i + (bp + q) * lda];
+        }
+        if (tid == 0) go = tc_wait(flags, J + 1, err, sticky);
+        __syncthreads();
+        if (!go) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < SB; q++)
+                if (nzg[bp + q]) xi = dd_sub(xi, dd_mul(vg[bp + q], a[q]));
+            x[i] = xi;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) tc_store(flags + 1 + u, J + 1);
+    }
+    if (tid == 0) tc_store(flags + 1 + u, nblk);
+}
+
+// Cooperative L^{-T} x in the Rtrsv('L','T','N') order: blocks from the
+// bottom (block R = rows [blo(R), n - 32R)), otherwise as k_trsv_ln_coop.
+__global__ __launch_bounds__(256) void k_trsv_lt_coop(idx_t n, const dd *L, idx_t lda, dd *x, int *flags, int nupd, dd *vg,
+                                                      int *sticky) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nblk = (int)((n + SB - 1) / SB);
+    int *err = flags + 1 + nupd;
+    auto blo = [&](int R) -> idx_t { idx_t t = n - (idx_t)(R + 1) * SB; return t > 0 ? t : 0; };
+    if (blockIdx.x == 0) {
+        __shared__ dd Dg[2][SB][SB + 1];  // Dg[buf][r][c] = L[b + r, b + c], c <= r
+        __shared__ dd Od[2][SB][SB + 1];  // Od[buf][q][c] = L[blo(R-1) + q, b + c]
+        __shared__ int stop;
+        auto stage = [&](int R, int buf, int t0, int nt) {
+            const idx_t b = blo(R);
+            const int nb = (int)(n - (idx_t)R * SB - b);
+            const idx_t bp = R > 0 ? blo(R - 1) : 0;
+            for (int e = t0; e < SB * SB; e += nt) {
+                const int r = e & (SB - 1), c = e >> 5;
+                if (r < nb && c <= r) Dg[buf][r][c] = L[(b + r) + (b + c) * lda];
+                if (R > 0 && c < nb) Od[buf][r][c] = L[(bp + r) + (b + c) * lda];
+            }
+        };
+        if (tid == 0) stop = 0;
+        stage(0, 0, tid, 256);
+        __syncthreads();
+        dd prev = mk(0.0);  // wave 0: lane q holds the solved x of block R-1, row q
+        for (int R = 0; R < nblk; R++) {
+            const int buf = R & 1;
+            if (wave != 0) {
+                if (R + 1 < nblk) stage(R + 1, buf ^ 1, tid - 64, 192);
+            } else {
+                const idx_t b = blo(R);
+                const int nb = (int)(n - (idx_t)R * SB - b);
+                int ok = 1;
+                if (lane == 0 && R >= 2) {
+                    const int u0 = (int)(b / TC_ROWS), u1 = (int)((b + nb - 1) / TC_ROWS);
+                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, R - 1, err, sticky);
+                }
+                ok = __builtin_amdgcn_readlane(ok, 0);
+                if (!ok) {
+                    if (lane == 0) stop = 1;
+                } else {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    const int lc = lane < nb ? lane : 0;
+                    dd xr = x[b + lc];
+                    if (R > 0) {
+#pragma unroll
+                        for (int q = SB - 1; q >= 0; q--) xr = dd_sub(xr, dd_mul(Od[buf][q][lc], readlane_dd(prev, q)));
+                    }
+#pragma unroll
+                    for (int jj = SB - 1; jj >= 0; jj--) {
+                        if (jj < nb) {
+                            const dd djj = Dg[buf][jj >= lc ? jj : lc][lc];
+                            if (lane == jj) xr = dd_div(xr, djj);
+                            const dd t = readlane_dd(xr, jj);
+                            if (lane < jj) xr = dd_sub(xr, dd_mul(djj, t));
+                        }
+                    }
+                    if (lane < nb) {
+                        x[b + lane] = xr;
+                        vg[(idx_t)R * SB + lane] = xr;
+                    }
+                    prev = xr;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                    if (lane == 0) tc_store(flags, R + 1);
+                }
+            }
+            __syncthreads();
+            if (stop) return;
+        }
+        return;
+    }
+    const int u = blockIdx.x - 1;
+    const idx_t lo = (idx_t)u * TC_ROWS, hi = lo + TC_ROWS < n ? lo + TC_ROWS : n;
+    const idx_t k = lo + tid;
+    __shared__ int go;
+    dd xk = k < hi ? x[k] : mk(0.0);
+    for (int J = 0; J < nblk; J++) {
+        const idx_t lim = blo(J + 1);  // rows below lim take block J here (blocks >= J+2)
+        if (J + 1 >= nblk || lo >= lim) break;
+        const bool act = k < hi && k < lim;
+        const idx_t bp = blo(J);
+        dd a[SB];
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < SB; q++) a[q] = L[(bp + q) + k * lda];
+        }
+        if (tid == 0) go = tc_wait(flags, J + 1, err, sticky);
+        __syncthreads();
+        if (!go) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (act) {
+#pragma unroll
+            for (int q = SB - 1; q >= 0; q--) xk = dd_sub(xk, dd_mul(a[q], vg[(idx_t)J * SB + q]));
+            x[k] = xk;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) tc_store(flags + 1 + u, J + 1);
+    }
+    if (tid == 0) tc_store(flags + 1 + u, nblk);
+}
+
+// Workspace of the cooperative solves: flags, then the solved block values.
+size_t trsv_coop_bytes(idx_t n) {
+    const idx_t nblk = (n + SB - 1) / SB, nupd = (n + TC_ROWS - 1) / TC_ROWS;
+    return 256 + sizeof(int) * (2 + nupd) + sizeof(dd) * nblk * SB + sizeof(int) * nblk * SB;
+}
+__global__ void k_zero_ints(int *p, int cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) p[i] = 0;
+}
+void zero_ints(hipStream_t s, int *p, int cnt) {
+    if (cnt > 0) k_zero_ints<<<cdiv(cnt, 256), 256, 0, s>>>(p, cnt);
+}
+// Stream-ordered; returns 0, or 1 when the cooperative launch is not
+// available (the caller then uses the single-workgroup kernel).  A wait
+// that runs out sets *sticky (device) and leaves x incomplete.
+int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, void *work, int *sticky) {
+    if (n <= 0) return 0;
+    const int nupd = (int)((n + TC_ROWS - 1) / TC_ROWS);
+    const idx_t nblk = (n + SB - 1) / SB;
+    char *w = (char *)work;
+    dd *vg = (dd *)(w + 256 - (((uintptr_t)w) & 255));
+    int *nzg = (int *)(vg + nblk * SB);
+    int *flags = nzg + nblk * SB;
+    zero_ints(s, flags, 2 + nupd);
+    StatScope st("trsv", s, (double)n * n);
+    dim3 grid(1 + nupd), block(256);
+    hipError_t e;
+    if (trans == 'N' || trans == 'n') {
+        void *args[] = {&n, &L, &lda, &x, &flags, (void *)&nupd, &vg, &nzg, &sticky};
+        e = hipLaunchCooperativeKernel((const void *)k_trsv_ln_coop, grid, block, args, 0, s);
+    } else {
+        void *args[] = {&n, &L, &lda, &x, &flags, (void *)&nupd, &vg, &sticky};
+        e = hipLaunchCooperativeKernel((const void *)k_trsv_lt_coop, grid, block, args, 0, s);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return 0;
+}
+
 void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x) {
     if (n <= 0) return;
     StatScope st("trsv", s, (double)n * n);
     if (trans == 'N' || trans == 'n')
-        k_trsv_ln<<<1, 256, 0, s>>>(n, L, lda, x);
+        k_trsv_ln<<<1, TS_NT, 0, s>>>(n, L, lda, x);
     else
-        k_trsv_lt<<<1, 256, 0, s>>>(n, L, lda, x);
+        k_trsv_lt<<<1, TS_NT, 0, s>>>(n, L, lda, x);
 }
 
 // ---------------------------------------------- inverse of a lower triangle --

@@ -30,6 +30,15 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
 
 // x := L^{-1} x ('N') or L^{-T} x ('T'), L lower non-unit.
 void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x);
+// The same solve spread over a cooperative grid (large n), stream-ordered
+// with no host synchronisation.  work: at least trsv_coop_bytes(n) bytes of
+// device memory.  A bounded wait that runs out sets *sticky (a device int
+// the caller clears) and leaves x incomplete.  Returns 0, or 1 when the
+// cooperative launch is unavailable (nothing ran).
+size_t trsv_coop_bytes(idx_t n);
+int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, void *work, int *sticky);
+// p[0..cnt) := 0 with a kernel (no memset node in captured graphs).
+void zero_ints(hipStream_t s, int *p, int cnt);
 
 // B := L^{-1} (B enters as anything; set to identity then solved like
 // Rtrsm('L','L','N','N') on the identity).

@@ -27,6 +27,34 @@ int Context::init(int dev) {
     if (hipMalloc((void **)&d_rinv, sizeof(dd) * 64) != hipSuccess) return -1;
     if (hipMalloc((void **)&d_max_work, sizeof(dd) * 256) != hipSuccess) return -1;
     if (hipMemset(d_counter, 0, sizeof(unsigned) * 4) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_tsw_sticky, 256) != hipSuccess) return -1;
+    if (hipMemset(d_tsw_sticky, 0, 256) != hipSuccess) return -1;
+    if (hipHostMalloc((void **)&h_tsw_err, sizeof(int)) != hipSuccess) return -1;
+    *h_tsw_err = 0;
+    return 0;
+}
+int Context::trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x, bool check) {
+    if (n >= 512) {
+        const size_t need = sdk::trsv_coop_bytes(n);
+        bool ok = true;
+        if (need > tsw_cap) {
+            if (d_tsw) (void)hipFree(d_tsw);
+            d_tsw = nullptr;
+            tsw_cap = 0;
+            ok = hipMalloc(&d_tsw, need) == hipSuccess;
+            if (ok) tsw_cap = need;
+        }
+        if (ok && sdk::trsv_lower_coop(s, trans, n, L, lda, x, d_tsw, d_tsw_sticky) == 0) {
+            if (!check) return 0;
+            trsv_poll(s);
+            if (hipStreamSynchronize(s) != hipSuccess) return -1;
+            if (*h_tsw_err == 0) return 0;
+            sdk::zero_ints(s, d_tsw_sticky, 1);
+            *h_tsw_err = 0;
+            return -1;
+        }
+    }
+    sdk::trsv_lower(s, trans, n, L, lda, x);
     return 0;
 }
 void Context::destroy() {
@@ -37,6 +65,13 @@ void Context::destroy() {
     if (d_rinv) (void)hipFree(d_rinv);
     if (d_max_work) (void)hipFree(d_max_work);
     ozw.release();
+    if (d_tsw) (void)hipFree(d_tsw);
+    if (d_tsw_sticky) (void)hipFree(d_tsw_sticky);
+    if (h_tsw_err) (void)hipHostFree(h_tsw_err);
+    d_tsw = nullptr;
+    d_tsw_sticky = nullptr;
+    h_tsw_err = nullptr;
+    tsw_cap = 0;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
 }
@@ -362,14 +397,23 @@ class Solver {
     // max|B| max|dy| (below it a correction only moves dy within rounding
     // noise, which changes the iterates without improving them) and is kept
     // only if it at least halves max|r|.
+    // The Schur solves do not wait for the device: a cooperative solve whose
+    // bounded wait ran out sets a sticky device flag, copied to the host after
+    // every solve and checked after the next synchronisation.
+    void trsv_chk(char trans, idx_t n, const dd *L, idx_t lda, dd *x) { (void)ctx.trsv(s, trans, n, L, lda, x, false); }
+    bool trsv_failed() const { return *ctx.h_tsw_err != 0; }
     void residual(const dd *x, dd *r) {
         sdk::copy(s, m, g, r);
         sdk::gemv_n(s, m, m, mk(-1.0), bSave, m, x, mk(1.0), r);
     }
     void schur_solve() {
+        schur_solve_refined();
+        ctx.trsv_poll(s);
+    }
+    void schur_solve_refined() {
         sdk::copy(s, m, g, dy);
-        sdk::trsv_lower(s, 'N', m, bMat, m, dy);
-        sdk::trsv_lower(s, 'T', m, bMat, m, dy);
+        trsv_chk('N', m, bMat, m, dy);
+        trsv_chk('T', m, bMat, m, dy);
         if (ctx.path == 0 || ctx.refine_steps == 0) return;
         residual(dy, rvec);
         ddr rn = max_abs(rvec, m);
@@ -377,8 +421,8 @@ class Solver {
         for (int it = 0; it < ctx.refine_steps; it++) {
             if (!(rn > floor_scale * max_abs(dy, m))) break;
             sdk::copy(s, m, dy, dyTry);
-            sdk::trsv_lower(s, 'N', m, bMat, m, rvec);
-            sdk::trsv_lower(s, 'T', m, bMat, m, rvec);
+            trsv_chk('N', m, bMat, m, rvec);
+            trsv_chk('T', m, bMat, m, rvec);
             sdk::axpy(s, m, mk(1.0), rvec, dyTry);
             residual(dyTry, rvec);
             ddr rt = max_abs(rvec, m);
@@ -546,6 +590,8 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         if (!(d_partial = alloc((idx_t)3 * nB * ctx.dot_T))) return -2;
     }
     double t_start = now();
+    sdk::zero_ints(s, ctx.d_tsw_sticky, 1);
+    *ctx.h_tsw_err = 0;
     for (dd **b : bufs) sdk::fill(s, nn, mk(0.0), *b);
 
     // Solutions::initialize (sdpa_dataset.cpp:35-60)
@@ -729,7 +775,11 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             const ddr xi = 3.0;
             ddr alphaBD = 100.0;
             ddr mx, mz;
-            min_eigen_xz(mx, mz);
+            min_eigen_xz(mx, mz);  // synchronises: the solves' flag is current
+            if (trsv_failed()) {
+                res.message = "triangular solve did not complete";
+                break;
+            }
             alphaP = (-mx > 1.0 / alphaBD) ? -1.0 / mx : alphaBD;
             alphaD = (-mz > 1.0 / alphaBD) ? -1.0 / mz : alphaBD;
             alphaP = prm.gammaStar * alphaP;
@@ -827,6 +877,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     }
     residuals(curNormP, curNormD);
     sync();
+    if (trsv_failed() && res.message.empty()) res.message = "triangular solve did not complete";
     res.t.total = now() - t_start;
     res.t.schur = t_schur;
     res.t.chol = t_chol;

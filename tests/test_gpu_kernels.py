@@ -130,7 +130,7 @@ def test_rtrmm(gpu, oracle, m, n):
     assert same(dB.get(), ref)
 
 
-@pytest.mark.parametrize("n", [1, 2, 17, 50, 64, 65, 257, 1100])
+@pytest.mark.parametrize("n", [1, 2, 17, 50, 64, 65, 257, 511, 512, 1100, 2049, 4551])
 @pytest.mark.parametrize("trans", ["N", "T"])
 def test_rtrsv_rtrmv(gpu, oracle, n, trans):
     rng = np.random.default_rng(n)
