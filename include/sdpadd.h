@@ -180,6 +180,38 @@ int sdpadd_trace(sdpadd_problem p, double *out, int cap);
 /* Final iterate of the last solve (host arrays: X and Z nn dd, y m dd). */
 int sdpadd_solution(sdpadd_problem p, double *X, double *y, double *Z);
 
+/* ---- Adjoint -------------------------------------------------------------- */
+/* No counterpart in the reference (which has no derivatives): implicit
+ * differentiation of the central-path equations at the final iterate of the
+ * last sdpadd_solve, with the HKM Newton system the solver factors
+ * (Newton::compute_bMat_dense_SDP, sdpa_newton.cpp:979-1068; compute_DxMat,
+ * :1329-1343).  Internal convention: min C.X s.t. A_i.X = b_i; p* = b.y*.
+ * The linearisation is exact only on the central path, so
+ * sdpadd_adjoint_setup first takes up to center_steps (0..16) full Newton
+ * steps toward the central point at mu = X.Z/n, stopping once
+ * max|XZ - mu I| <= 1e-10 mu (8 steps suffice from the solver's last
+ * iterate), then assembles the
+ * Schur matrix there and factors it once.  *na receives the number of
+ * A-gradient entries per cotangent, *centrality max|XZ - mu I| / mu after
+ * centring (either may be NULL).  Returns > 0 when Z or the Schur matrix is
+ * not positive definite. */
+int sdpadd_adjoint_setup(sdpadd_problem p, int center_steps, int64_t *na, double *centrality);
+/* Which entry each A-gradient value belongs to (host arrays of na ints, any
+ * may be NULL): constraint (0-based), block (0-based), row, column.  Sparse
+ * blocks list their stored entries in file order, dense blocks all n*n
+ * entries column-major. */
+int sdpadd_adjoint_entries(sdpadd_problem p, int *con, int *blk, int *row, int *col);
+/* Vector-Jacobian products for nrhs cotangents.  Device arrays (dd pairs):
+ * d_Xbar nrhs x nn (cotangent of X*), d_ybar nrhs x m (of y*); host pbar
+ * nrhs dd (of p*).  Outputs (device): d_bbar nrhs x m, d_Cbar nrhs x nn (the
+ * gradient with respect to the symmetric matrix C), d_Abar nrhs x na (for a
+ * stored sparse entry the derivative with respect to its value, which sets
+ * both (r, c) and (c, r); for a dense block the gradient with respect to the
+ * symmetric matrix).  Inputs and outputs may be NULL (a NULL cotangent is
+ * zero).  Synchronous. */
+int sdpadd_adjoint(sdpadd_problem p, int nrhs, const double *d_Xbar, const double *d_ybar, const double *pbar,
+                   double *d_bbar, double *d_Cbar, double *d_Abar);
+
 #ifdef __cplusplus
 }
 #endif

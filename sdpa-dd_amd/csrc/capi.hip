@@ -441,12 +441,13 @@ int sdpadd_trace(sdpadd_problem p, double *out, int cap) {
     }
     return n;
 }
-int sdpadd_adjoint_setup(sdpadd_problem p, int64_t *na) {
-    if (!p || !p->finalized) return -1;
+int sdpadd_adjoint_setup(sdpadd_problem p, int center_steps, int64_t *na, double *centrality) {
+    if (!p || !p->finalized || center_steps < 0 || center_steps > 16) return -1;
     std::string err;
-    int rc = p->adj.setup(p->h->c, p->hp, p->dp, p->X, p->y, p->Z, err);
+    int rc = p->adj.setup(p->h->c, p->hp, p->dp, p->X, p->y, p->Z, center_steps, err);
     if (rc != 0) p->h->c.set_error(err);
     if (rc == 0 && na) *na = p->adj.na;
+    if (rc == 0 && centrality) *centrality = p->adj.centrality;
     return rc;
 }
 int sdpadd_adjoint_entries(sdpadd_problem p, int *con, int *blk, int *row, int *col) {

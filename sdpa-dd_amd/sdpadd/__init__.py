@@ -93,6 +93,9 @@ def load_library(path=None):
         "sdpadd_solve": (ctypes.c_int, [vp, vp, vp]),
         "sdpadd_trace": (ctypes.c_int, [vp, c_dp, ctypes.c_int]),
         "sdpadd_solution": (ctypes.c_int, [vp, c_dp, c_dp, c_dp]),
+        "sdpadd_adjoint_setup": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(c_i64), c_dp]),
+        "sdpadd_adjoint_entries": (ctypes.c_int, [vp] + [ctypes.POINTER(ctypes.c_int)] * 4),
+        "sdpadd_adjoint": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, c_dp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -234,10 +237,13 @@ def rgemm(dev, transb, A, B, C, alpha=(1.0, 0.0), beta=(0.0, 0.0)):
 class Problem:
     """A problem on the device, loaded from a .dat-s file or built in memory."""
 
-    def __init__(self, dev, path=None):
+    def __init__(self, dev, path=None, _handle=None):
         self.dev = dev
         self.p = ctypes.c_void_p()
-        if path is not None:
+        self.na = None
+        if _handle is not None:
+            self.p = _handle
+        elif path is not None:
             dev.check(dev.lib.sdpadd_problem_load(dev.h, path.encode(), ctypes.byref(self.p)), "problem_load")
         m, nb = ctypes.c_int(), ctypes.c_int()
         bs = (ctypes.c_int * 64)()
@@ -247,6 +253,30 @@ class Problem:
     @property
     def nn(self):
         return sum(n * n for n in self.block_struct)
+
+    @classmethod
+    def from_arrays(cls, dev, m, block_struct, b, entries):
+        """Builds a problem in SDPA's internal form.  b: (m, 2) dd; entries:
+        iterable of (k, l, rows, cols, vals) with k = 0 for C, 1..m for A_k,
+        l the 0-based block, 0-based rows/cols and (nnz, 2) dd values."""
+        h = ctypes.c_void_p()
+        bs = (ctypes.c_int * len(block_struct))(*block_struct)
+        dev.check(dev.lib.sdpadd_problem_create(dev.h, m, len(block_struct), bs, ctypes.byref(h)), "problem_create")
+        try:
+            for k, l, rows, cols, vals in entries:
+                r = np.ascontiguousarray(rows, dtype=np.int32)
+                c = np.ascontiguousarray(cols, dtype=np.int32)
+                v = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1, 2)
+                ip = ctypes.POINTER(ctypes.c_int)
+                dev.check(dev.lib.sdpadd_problem_set_entries(h, int(k), int(l), len(r), r.ctypes.data_as(ip),
+                                                             c.ctypes.data_as(ip), _dp(v)), "problem_set_entries")
+            bb = np.ascontiguousarray(b, dtype=np.float64).reshape(m, 2)
+            dev.check(dev.lib.sdpadd_problem_set_b(h, _dp(bb)), "problem_set_b")
+            dev.check(dev.lib.sdpadd_problem_finalize(h), "problem_finalize")
+        except Exception:
+            dev.lib.sdpadd_problem_destroy(h)
+            raise
+        return cls(dev, _handle=h)
 
     def formula(self):
         out = (ctypes.c_int * (self.m * len(self.block_struct)))()
@@ -296,6 +326,50 @@ class Problem:
         y = np.zeros((self.m, 2))
         self.dev.check(self.dev.lib.sdpadd_solution(self.p, _dp(X), _dp(y), _dp(Z)), "solution")
         return X, y, Z
+
+    # ---- adjoint (implicit differentiation at the final iterate) --------
+    def adjoint_setup(self, center_steps=8):
+        """Centres the final iterate of the last solve (center_steps Newton
+        steps at mu = X.Z/n) and factors the Schur matrix there once; returns
+        the number of A-gradient entries per cotangent.  self.centrality:
+        max|XZ - mu I| / mu after centring."""
+        na = ctypes.c_int64()
+        cen = np.zeros(1)
+        rc = self.dev.lib.sdpadd_adjoint_setup(self.p, center_steps, ctypes.byref(na), _dp(cen))
+        if rc != 0:
+            raise SdpaddError("adjoint_setup failed (%d): %s" % (rc, self.dev.lib.sdpadd_last_error(self.dev.h).decode()))
+        self.na = na.value
+        self.centrality = float(cen[0])
+        return self.na
+
+    def adjoint_entries(self):
+        """(constraint, block, row, col) of every A-gradient entry (0-based)."""
+        if self.na is None:
+            self.adjoint_setup()
+        out = [np.zeros(self.na, dtype=np.int32) for _ in range(4)]
+        ip = ctypes.POINTER(ctypes.c_int)
+        self.dev.check(self.dev.lib.sdpadd_adjoint_entries(self.p, *[o.ctypes.data_as(ip) for o in out]),
+                       "adjoint_entries")
+        return tuple(out)
+
+    def adjoint(self, Xbar=None, ybar=None, pbar=None, nrhs=None):
+        """Vector-Jacobian products of (X*, y*, p* = b.y*) for nrhs cotangents.
+        Xbar (nrhs, nn, 2), ybar (nrhs, m, 2), pbar (nrhs, 2) dd host arrays
+        (None = zero).  Returns (bbar (nrhs, m, 2), Cbar (nrhs, nn, 2),
+        Abar (nrhs, na, 2)) as host arrays."""
+        if self.na is None:
+            self.adjoint_setup()
+        K = nrhs or next(len(a) for a in (Xbar, ybar, pbar) if a is not None)
+        dX = self.dev.to_device(np.asarray(Xbar).reshape(K, self.nn, 2)) if Xbar is not None else None
+        dy = self.dev.to_device(np.asarray(ybar).reshape(K, self.m, 2)) if ybar is not None else None
+        pb = np.ascontiguousarray(pbar, dtype=np.float64).reshape(K, 2) if pbar is not None else None
+        db, dC, dA = (self.dev.empty((K, self.m, 2)), self.dev.empty((K, self.nn, 2)),
+                      self.dev.empty((K, max(1, self.na), 2)))
+        rc = self.dev.lib.sdpadd_adjoint(self.p, K, dX.ptr if dX else None, dy.ptr if dy else None,
+                                         _dp(pb) if pb is not None else None, db.ptr, dC.ptr, dA.ptr)
+        if rc != 0:
+            raise SdpaddError("adjoint failed (%d): %s" % (rc, self.dev.lib.sdpadd_last_error(self.dev.h).decode()))
+        return db.get(), dC.get(), dA.get()[:, :self.na]
 
     def close(self):
         if self.p:

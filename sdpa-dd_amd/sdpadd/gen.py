@@ -167,6 +167,51 @@ def theta(path, n=300, p=0.1, seed=SEED0 + 4):
     return w.write(path, "theta G(%d,%.2f) m=%d seed=%d" % (n, p, m, seed))
 
 
+def read_arrays(path):
+    """Parses an SDPA sparse-format file into SDPA's internal form as float64
+    arrays: (m, block_struct, b, entries) with entries a list of
+    (k, l, rows, cols, vals), k = 0 for C (= -F0), 1..m for A_k, 0-based
+    block l, rows and cols.  Entries of one (k, l) keep file order."""
+    import re
+    toks = []
+    with open(path) as f:
+        for line in f:
+            t = line.strip()
+            if not t or t[0] in '"*':
+                continue
+            toks.extend(x for x in re.split(r"[\s,{}()=]+", t) if x and _isnum(x))
+    pos = 0
+
+    def take():
+        nonlocal pos
+        pos += 1
+        return toks[pos - 1]
+
+    m = int(take())
+    nblock = int(take())
+    bs = [abs(int(take())) for _ in range(nblock)]
+    b = np.array([float(take()) for _ in range(m)])
+    groups = {}
+    while pos + 5 <= len(toks):
+        k, l, i, j, v = int(take()), int(take()), int(take()), int(take()), float(take())
+        if k == 0:
+            v = -v
+        g = groups.setdefault((k, l - 1), ([], [], []))
+        g[0].append(i - 1)
+        g[1].append(j - 1)
+        g[2].append(v)
+    entries = [(k, l, np.array(r), np.array(c), np.array(v)) for (k, l), (r, c, v) in sorted(groups.items())]
+    return m, bs, b, entries
+
+
+def _isnum(t):
+    try:
+        float(t)
+        return True
+    except ValueError:
+        return False
+
+
 BUILDERS = {0: random_dense, 1: maxcut_torus, 2: control, 3: random_sparse, 4: theta}
 
 
