@@ -20,7 +20,13 @@ Schur dd-GFLOP/s of the plain form, the roofline of the dominant kernel of
 each form from HIP-event timing in the timed region, and (max-cut only) the
 CPU oracle's time on a bounded sample (one interior-point iteration).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload maxcut|theta]
+The reference's own algorithm stalls short of gap 1e-25 on the theta
+workload (the plain form, which reproduces it, ends pFEAS near 1e-23; the
+CPU oracle does the same on G(150, 0.1)), so --gap sets the stopping
+tolerance for both forms (epsilonStar = epsilonDash); 1e-20 is the
+reference's published SDPLIB setting (SDPLIB/param.sdpa:2,10).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload maxcut|theta] [--gap G]
 """
 import argparse
 import json
@@ -41,7 +47,7 @@ METRIC = "solve time to rel. gap 1e-25 (Ozaki-MFMA vs plain dd HIP); Schur dd-GF
 FAMILIES = ("gemm", "ozaki", "potrf", "trsm", "trmm", "trsv", "mv", "dot")
 
 
-def workload_files(tmp, workload):
+def workload_files(tmp, workload, gap):
     from sdpadd import gen
     if workload == "theta":
         data = os.path.join(tmp, "theta_G300_0.1.dat-s")
@@ -51,8 +57,8 @@ def workload_files(tmp, workload):
         data = os.path.join(tmp, "maxcut_torus_40x20.dat-s")
         if not os.path.exists(data):
             gen.maxcut_torus(data)
-    param = os.path.join(tmp, "param_gap1e-25.sdpa")
-    gen.param_file(param, epsilon_star=1e-25, epsilon_dash=1e-25)
+    param = os.path.join(tmp, "param_gap%g.sdpa" % gap)
+    gen.param_file(param, epsilon_star=gap, epsilon_dash=gap)
     return data, param
 
 
@@ -100,6 +106,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("maxcut", "theta"), default="maxcut")
+    ap.add_argument("--gap", type=float, default=1e-25,
+                    help="stopping tolerance: relative gap and feasibility (default 1e-25)")
     ap.add_argument("--ozaki-path", type=int, choices=(1, 2), default=None,
                     help="1: Ozaki Cholesky; 2: Ozaki Cholesky and GEMMs (default: 2 for max-cut, 1 for theta)")
     a = ap.parse_args()
@@ -110,7 +118,7 @@ def main():
     import sdpadd
     tmp = os.path.join(tempfile.gettempdir(), "sdpadd_bench")
     os.makedirs(tmp, exist_ok=True)
-    data, param = workload_files(tmp, a.workload)
+    data, param = workload_files(tmp, a.workload, a.gap)
     dev = sdpadd.Device(0)
     prob = sdpadd.Problem(dev, data)
     n, m = prob.block_struct[0], prob.m
@@ -126,10 +134,10 @@ def main():
     # int8 multiply-adds of the Ozaki kernel: S(S+1)/2 plane products per dd multiply-add
     oz_int8_tops = (o["ddflops"] * OZ_SLICES * (OZ_SLICES + 1) / 2) / (o["ms"] * 1e-3) / 1e12 if o["ms"] > 0 else 0.0
     chol_flops = plain["iterations"] * m ** 3 / 3.0
-    ozaki_ok = ozk["phase"] == "pdOPT" and ozk["relgap"][0] <= 1e-25
+    ozaki_ok = ozk["phase"] == "pdOPT" and ozk["relgap"][0] <= a.gap
     value = oz_s if ozaki_ok else plain_s
     line = {
-        "metric": METRIC,
+        "metric": METRIC if a.gap == 1e-25 else METRIC.replace("1e-25", "%g" % a.gap),
         "value": round(value, 4),
         "unit": "s",
         "n_gpus": 1,
@@ -143,7 +151,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": ("Lovasz theta SDP of G(300, 0.1) (n=300, m=%d)" % m) if a.workload == "theta" else
                    "max-cut SDP relaxation, 40x20 toroidal grid, +-1 weights (n=m=800)",
-                   "gap_and_feasibility": 1e-25, "n": n, "m": m,
+                   "gap_and_feasibility": a.gap, "n": n, "m": m,
                    "value_form": "ozaki" if ozaki_ok else "plain (the Ozaki form did not reach the gap)",
                    "ozaki_slices": OZ_SLICES, "ozaki_path": oz_path},
         "plain_solve_s": round(plain_s, 4),

@@ -44,7 +44,10 @@ int sdpadd_set_dot_partition(sdpadd_handle h, int chunks);
  * direction run on INT8 MFMA too (digit planes are scaled per row, so
  * entries far below their row's largest lose relative precision: fine for
  * dense iterates such as max-cut's, not for theta-type iterates whose
- * structurally zero entries shrink with mu).  Default: 0, 16, 2. */
+ * structurally zero entries shrink with mu).  A negative `slices` (-4 .. -7)
+ * selects the FP64-slice variant: |slices| 18-bit digits held in doubles,
+ * accumulated by FP64 MFMA (-5: about 2^-80 normwise, -6 and -7: dd).
+ * Default: 0, 16, 2. */
 int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps);
 int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr);
 int sdpadd_free(sdpadd_handle h, void *d_ptr);
@@ -77,7 +80,10 @@ int sdpadd_Rgemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n
  * (8, 12, 14 or 16) signed 7-bit digit planes under its own power-of-two
  * scale; plane products accumulate exactly in int32 and are recombined in
  * dd.  The error of C_ij is about k * 2^(-7*slices) * max_l |A_il| *
- * max_l |B_lj| (16 slices: dd accuracy normwise).  k <= 8192. */
+ * max_l |B_lj| (16 slices: dd accuracy normwise).  k <= 8192.  slices
+ * -4 .. -7: the FP64-slice variant (|slices| 18-bit digits in doubles,
+ * accumulated exactly by FP64 MFMA; error about k * 2^(10-18|slices|) *
+ * max_l |A_il| * max_l |B_lj| plus dd rounding). */
 int sdpadd_ozaki_gemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n, int64_t k, int alpha,
                       const double *d_A, int64_t lda, const double *d_B, int64_t ldb, int beta, double *d_C, int64_t ldc,
                       int lower, int slices);

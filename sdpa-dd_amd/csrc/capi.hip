@@ -72,7 +72,8 @@ int sdpadd_set_dot_partition(sdpadd_handle h, int chunks) {
 int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps) {
     if (!h) return -1;
     if (path < 0 || path > 2) return bad(h, "path must be 0 (plain), 1 (Ozaki Cholesky) or 2 (Ozaki Cholesky and GEMMs)");
-    if (slices != 8 && slices != 12 && slices != 14 && slices != 16) return bad(h, "slices must be 8, 12, 14 or 16");
+    if (slices != 8 && slices != 12 && slices != 14 && slices != 16 && (slices < -7 || slices > -4))
+        return bad(h, "slices must be 8, 12, 14 or 16 (INT8 planes) or -4 .. -7 (FP64 slices)");
     if (refine_steps < 0 || refine_steps > 8) return bad(h, "refine_steps must be in [0, 8]");
     h->c.path = path;
     h->c.oz_slices = slices;
@@ -133,7 +134,8 @@ int sdpadd_ozaki_gemm(sdpadd_handle h, char transa, char transb, int64_t m, int6
         return bad(h, "ozaki_gemm: bad dimension");
     int rc = oz::gemm(h->c.stream, h->c.ozw, transa, transb, m, n, k, alpha, DP(d_A), lda, DP(d_B), ldb, beta, DP(d_C),
                       ldc, lower != 0, slices);
-    if (rc == -1) return bad(h, "ozaki_gemm: unsupported argument (alpha +-1, beta 0/1, k <= 8192, slices 8/12/14/16)");
+    if (rc == -1)
+        return bad(h, "ozaki_gemm: unsupported argument (alpha +-1, beta 0/1, k <= 8192, slices 8/12/14/16 or -4..-7)");
     if (rc != 0) return bad(h, "ozaki_gemm: out of device memory");
     return check(h);
 }

@@ -963,8 +963,8 @@ __global__ void k_zero_ints(int *p, int cnt) {
 void zero_ints(hipStream_t s, int *p, int cnt) {
     if (cnt > 0) k_zero_ints<<<cdiv(cnt, 256), 256, 0, s>>>(p, cnt);
 }
-// Stream-ordered; returns 0, or 1 when the cooperative launch is not
-// available (the caller then uses the single-workgroup kernel).  A wait
+// Stream-ordered; returns 0, or 1 when the launch fails (the caller then
+// uses the single-workgroup kernel).  A wait
 // that runs out sets *sticky (device) and leaves x incomplete.
 int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, void *work, int *sticky) {
     if (n <= 0) return 0;
@@ -977,8 +977,22 @@ int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, 
     zero_ints(s, flags, 2 + nupd);
     StatScope st("trsv", s, (double)n * n);
     dim3 grid(1 + nupd), block(256);
+    // All workgroups must be resident together.  A grid no larger than the
+    // CU count is (two of these workgroups fit one CU), so it takes a plain
+    // launch; a larger one needs the cooperative launch's guarantee.  The
+    // waits are bounded either way.
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    const bool ln = (trans == 'N' || trans == 'n');
     hipError_t e;
-    if (trans == 'N' || trans == 'n') {
+    if ((int)grid.x <= cus) {
+        if (ln)
+            k_trsv_ln_coop<<<grid, block, 0, s>>>(n, L, lda, x, flags, nupd, vg, nzg, sticky);
+        else
+            k_trsv_lt_coop<<<grid, block, 0, s>>>(n, L, lda, x, flags, nupd, vg, sticky);
+        e = hipGetLastError();
+    } else if (ln) {
         void *args[] = {&n, &L, &lda, &x, &flags, (void *)&nupd, &vg, &nzg, &sticky};
         e = hipLaunchCooperativeKernel((const void *)k_trsv_ln_coop, grid, block, args, 0, s);
     } else {
@@ -1506,6 +1520,37 @@ __global__ __launch_bounds__(256) void k_dot_batch_off(const DotTaskOff *tasks, 
 }
 void dot_batch_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T) {
     if (ntasks > 0) k_dot_batch_off<<<ntasks, 256, sizeof(dd) * T, s>>>(d_tasks, ybase, T);
+}
+// Long offset tasks: one workgroup per (chunk, task) as k_dot_chunks, then
+// the chunk sums of each task in chunk order.
+__global__ __launch_bounds__(256) void k_dot_chunks_off(const DotTaskOff *tasks, const dd *ybase, int T, dd *partial) {
+    __shared__ dd prod[DOT_TILE];
+    const DotTaskOff tk = tasks[blockIdx.y];
+    const dd *y = ybase + tk.yoff;
+    idx_t st, cnt;
+    chunk_range(tk.n, T, blockIdx.x, st, cnt);
+    dd acc = mk(0.0);
+    for (idx_t t0 = 0; t0 < cnt; t0 += DOT_TILE) {
+        const int tl = (int)((cnt - t0) < DOT_TILE ? (cnt - t0) : DOT_TILE);
+        for (int e = threadIdx.x; e < tl; e += 256) prod[e] = dd_mul(tk.x[st + t0 + e], y[st + t0 + e]);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int e = 0; e < tl; e++) acc = dd_add_eq(acc, prod[e]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(idx_t)blockIdx.y * T + blockIdx.x] = acc;
+}
+__global__ void k_dot_combine_off(const DotTaskOff *tasks, int ntasks, int T, const dd *partial) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ntasks) return;
+    dd acc = mk(0.0);
+    for (int c = 0; c < T; c++) acc = dd_add_eq(acc, partial[(idx_t)q * T + c]);
+    *tasks[q].out = acc;
+}
+void dot_tasks_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T, dd *partial) {
+    if (ntasks <= 0) return;
+    k_dot_chunks_off<<<dim3((unsigned)T, (unsigned)ntasks), 256, 0, s>>>(d_tasks, ybase, T, partial);
+    k_dot_combine_off<<<cdiv(ntasks, 64), 64, 0, s>>>(d_tasks, ntasks, T, partial);
 }
 
 // out_i = A_ii for an n x n column-major block.

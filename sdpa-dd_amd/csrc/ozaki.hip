@@ -12,7 +12,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 static inline unsigned cdiv(idx_t a, idx_t b) { return (unsigned)((a + b - 1) / b); }
 static inline idx_t roundup(idx_t a, idx_t b) { return (a + b - 1) / b * b; }
 
-bool Work::reserve(idx_t mpad, idx_t npad, idx_t kpad, int S) {
+bool Work::reserve(idx_t mpad, idx_t npad, idx_t kpad, int S, int esize) {
     auto grow = [](void **p, size_t &cap, size_t want) {
         if (want <= cap) return true;
         if (*p) (void)hipFree(*p);
@@ -22,7 +22,8 @@ bool Work::reserve(idx_t mpad, idx_t npad, idx_t kpad, int S) {
         cap = want;
         return true;
     };
-    return grow((void **)&sa, cap_a, (size_t)S * mpad * kpad) && grow((void **)&sb, cap_b, (size_t)S * npad * kpad) &&
+    return grow((void **)&sa, cap_a, (size_t)esize * S * mpad * kpad) &&
+           grow((void **)&sb, cap_b, (size_t)esize * S * npad * kpad) &&
            grow((void **)&ea, cap_ea, sizeof(unsigned) * mpad) && grow((void **)&eb, cap_eb, sizeof(unsigned) * npad);
 }
 void Work::release() {
@@ -194,6 +195,122 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
                                                                   alpha, beta, lower ? 1 : 0);
 }
 
+
+// ------------------------------------------------------ FP64-slice variant --
+// The same splitting with FB-bit digits held in doubles: d = trunc(2^FB x).
+// Plane products accumulate per level in FP64 MFMA (v_mfma_f64_16x16x4f64);
+// |d| < 2^18, so a level sum over k <= 8192 with at most 7 products per k
+// stays below 2^52 and is exact.  Recombination: sum_d acc[d] 2^-FB(d+2).
+constexpr int FB = 18;
+constexpr int KC2 = 8;  // k per LDS chunk (two MFMA k-steps of 4)
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// plane s of row r, chunk kb: out[((s nkb + kb) rows_pad + r) KC2 + (k mod KC2)]
+template <int S>
+__global__ __launch_bounds__(512) void k_slice_f64(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs,
+                                                   const unsigned *exps, double *out, idx_t rows_pad, int nkb) {
+    const int t = threadIdx.x, kq = t >> 6, kb = blockIdx.y;
+    const idx_t r = (idx_t)blockIdx.x * 64 + (t & 63);
+    if (r >= rows_pad) return;
+    const idx_t kk = (idx_t)kb * KC2 + kq;
+    const int e = r < rows ? exp_of(exps[r]) : 0;
+    const dd a = (r < rows && kk < k) ? M[r * rs + kk * cs] : mk(0.0);
+    double xh = ldexp(a.hi, -e), xl = ldexp(a.lo, -e);
+    constexpr double SC = (double)(1 << FB);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const double th = xh * SC, tl = xl * SC;
+        const double d = trunc(th);
+        const double rh = th - d;
+        const double sm = rh + tl, bb = sm - rh;
+        xl = (rh - (sm - bb)) + (tl - bb);
+        xh = sm;
+        out[(((idx_t)s * nkb + kb) * rows_pad + r) * KC2 + kq] = d;
+    }
+}
+
+// 64 x 64 output tile per 512-thread workgroup, wave layout as k_gemm.
+template <int S>
+__global__ __launch_bounds__(512) void k_gemm_f64(idx_t m, idx_t n, int nkb, const double *SA, idx_t mpad,
+                                                  const unsigned *EA, const double *SB, idx_t npad, const unsigned *EB,
+                                                  dd *C, idx_t ldc, int alpha, int beta, int lower) {
+    __shared__ double LA[S][64][KC2 + 1];
+    __shared__ double LB[S][64][KC2 + 1];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const idx_t r0 = (idx_t)blockIdx.x * 64, c0 = (idx_t)blockIdx.y * 64;
+    if (lower && r0 + 63 < c0) return;
+    const int wr = wave & 3, wc = wave >> 2;
+    const int fr = lane & 15, fq = lane >> 4;
+    v4d acc[S][2];
+#pragma unroll
+    for (int d = 0; d < S; d++) acc[d][0] = acc[d][1] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int kb = 0; kb < nkb; kb++) {
+        __syncthreads();
+        for (int c = tid; c < S * 64 * KC2; c += 512) {
+            const int sl = c / (64 * KC2), row = (c / KC2) & 63, kq = c % KC2;
+            LA[sl][row][kq] = SA[(((idx_t)sl * nkb + kb) * mpad + r0 + row) * KC2 + kq];
+            LB[sl][row][kq] = SB[(((idx_t)sl * nkb + kb) * npad + c0 + row) * KC2 + kq];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k4 = 0; k4 < KC2; k4 += 4) {
+            double af[S];
+#pragma unroll
+            for (int s = 0; s < S; s++) af[s] = LA[s][wr * 16 + fr][k4 + fq];
+#pragma unroll
+            for (int t = 0; t < S; t++) {
+                const double b0 = LB[t][wc * 32 + fr][k4 + fq];
+                const double b1 = LB[t][wc * 32 + 16 + fr][k4 + fq];
+#pragma unroll
+                for (int s = 0; s + t < S; s++) {
+                    acc[s + t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], b0, acc[s + t][0], 0, 0, 0);
+                    acc[s + t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], b1, acc[s + t][1], 0, 0, 0);
+                }
+            }
+        }
+    }
+    constexpr double STEP = 1.0 / (double)(1 << FB);
+#pragma unroll
+    for (int ct = 0; ct < 2; ct++) {
+        const idx_t j = c0 + wc * 32 + ct * 16 + fr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const idx_t i = r0 + wr * 16 + fq + 4 * r;  // v_mfma_f64_16x16x4f64: row g + 4r in lane j + 16g
+            if (i >= m || j >= n || (lower && i < j)) continue;
+            dd h = mk(acc[S - 1][ct][r]);
+#pragma unroll
+            for (int d = S - 2; d >= 0; d--) {
+                h.hi *= STEP;
+                h.lo *= STEP;
+                h = dd_add_d(h, acc[d][ct][r]);
+            }
+            const int ex = exp_of(EA[i]) + exp_of(EB[j]) - 2 * FB;
+            h.hi = ldexp(h.hi, ex);
+            h.lo = ldexp(h.lo, ex);
+            if (alpha < 0) h = dd_neg(h);
+            dd *cp = C + i + j * ldc;
+            *cp = beta ? dd_add(*cp, h) : h;
+        }
+    }
+}
+
+template <int S>
+static void launch_f64(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *A, idx_t ars, idx_t acs,
+                       const dd *B, idx_t brs, idx_t bcs, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
+    const int nkb = (int)cdiv(k, KC2), nkb64 = (int)cdiv(k, KC);
+    const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
+    double *sa = (double *)w.sa, *sb = (double *)w.sb;
+    k_exp_clear<<<cdiv(m, 256), 256, 0, s>>>(m, w.ea);
+    k_rowexp<<<dim3(cdiv(m, 64), nkb64), 256, 0, s>>>(m, k, A, ars, acs, w.ea);
+    k_slice_f64<S><<<dim3(cdiv(mpad, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, sa, mpad, nkb);
+    k_exp_clear<<<cdiv(n, 256), 256, 0, s>>>(n, w.eb);
+    k_rowexp<<<dim3(cdiv(n, 64), nkb64), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
+    k_slice_f64<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, sb, npad, nkb);
+    sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
+    k_gemm_f64<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, sa, mpad, w.ea, sb, npad, w.eb, C,
+                                                                      ldc, alpha, beta, lower ? 1 : 0);
+}
+
 int gemm(hipStream_t s, Work &w, char transa, char transb, idx_t m, idx_t n, idx_t k, int alpha, const dd *A, idx_t lda,
          const dd *B, idx_t ldb, int beta, dd *C, idx_t ldc, bool lower, int S) {
     if (m < 0 || n < 0 || k < 0 || k > MAX_K || (alpha != 1 && alpha != -1) || (beta != 0 && beta != 1)) return -1;
@@ -203,12 +320,18 @@ int gemm(hipStream_t s, Work &w, char transa, char transb, idx_t m, idx_t n, idx
     if (!nt && transb != 'N' && transb != 'n') return -1;
     if (!ta && transa != 'N' && transa != 'n') return -1;
     const idx_t kpad = roundup(k, KC);
-    if (!w.reserve(roundup(m, 64), roundup(n, 64), kpad, S)) return -2;
+    const bool f64 = S < 0;
+    if (f64 && (S < -7 || S > -4)) return -1;
+    if (!w.reserve(roundup(m, 64), roundup(n, 64), kpad, f64 ? -S : S, f64 ? 8 : 1)) return -2;
     // element (i, kk) of op(A): A[i + kk lda] for 'N', A[kk + i lda] for 'T';
     // element (j, kk) of op(B)^T: B[kk + j ldb] for 'N', B[j + kk ldb] for 'T'
     const idx_t ars = ta ? lda : 1, acs = ta ? 1 : lda;
     const idx_t brs = nt ? 1 : ldb, bcs = nt ? ldb : 1;
     switch (S) {
+    case -7: launch_f64<7>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
+    case -6: launch_f64<6>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
+    case -5: launch_f64<5>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
+    case -4: launch_f64<4>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
     case 16: launch<16>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
     case 14: launch<14>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;
     case 12: launch<12>(s, w, m, n, k, A, ars, acs, B, brs, bcs, alpha, beta, C, ldc, lower); break;

@@ -29,7 +29,7 @@ struct Work {
     int8_t *sa = nullptr, *sb = nullptr;
     unsigned *ea = nullptr, *eb = nullptr;  // biased row exponents
     size_t cap_a = 0, cap_b = 0, cap_ea = 0, cap_eb = 0;
-    bool reserve(idx_t mpad, idx_t npad, idx_t kpad, int S);
+    bool reserve(idx_t mpad, idx_t npad, idx_t kpad, int S, int esize = 1);
     void release();
 };
 
@@ -37,8 +37,11 @@ struct Work {
 //   op(A) = A (transa 'N', m x k) or A^T ('T', A is k x m);
 //   op(B) = B (transb 'N', B is k x n) or B^T ('T', B is n x k).
 // lower: only entries with i >= j are written.  k <= MAX_K per call (the
-// caller splits longer products).  Returns 0, -1 for bad arguments, -2 when
-// the workspace cannot grow.
+// caller splits longer products).  S in {8, 12, 14, 16}: INT8 planes of 7
+// bits on INT8 MFMA; S in {-4 .. -7}: the FP64-slice variant, |S| digits of
+// 18 bits held in doubles and accumulated by FP64 MFMA (exact while
+// k <= MAX_K), error about k 2^(10 - 18|S|) max|A_i.| max|B_.j|.  Returns 0,
+// -1 for bad arguments, -2 when the workspace cannot grow.
 int gemm(hipStream_t s, Work &w, char transa, char transb, idx_t m, idx_t n, idx_t k, int alpha, const dd *A, idx_t lda,
          const dd *B, idx_t ldb, int beta, dd *C, idx_t ldc, bool lower, int S);
 

@@ -485,6 +485,8 @@ int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::
             if (cty[t] == 1) dct.push_back({dense_base + coff[t], P.boff[cbl[t]], (idx_t)cnz[t], dp.d_ctask_val + t});
         dp.n_dense_tasks = (int)dt.size();
         dp.n_dense_ctasks = (int)dct.size();
+        for (auto &t : dt) dp.dense_task_maxn = std::max(dp.dense_task_maxn, t.n);
+        for (auto &t : dct) dp.dense_ctask_maxn = std::max(dp.dense_ctask_maxn, t.n);
         dp.d_dense_tasks = dev_copy(dp, dt, s);
         dp.d_dense_ctasks = dev_copy(dp, dct, s);
         dp.d_boff = dev_copy(dp, P.boff, s);
@@ -553,6 +555,17 @@ int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::
         S.npairs = (int)pi.size();
         max_pairs = std::max(max_pairs, S.npairs);
         S.h_pair_form = pf;
+        {
+            std::vector<int> lf2, lot;
+            for (int q = 0; q < S.npairs; q++) {
+                if (type[psj[q]] != 0) continue;  // dense A_j: dot tasks
+                (pf[q] == 1 ? lf2 : lot).push_back(q);
+            }
+            S.n_f2 = (int)lf2.size();
+            S.n_other = (int)lot.size();
+            S.d_list_f2 = dev_copy(dp, lf2, s);
+            S.d_list_other = dev_copy(dp, lot, s);
+        }
         S.d_pair_i = dev_copy(dp, pi, s);
         S.d_pair_j = dev_copy(dp, pj, s);
         S.d_pair_slot_i = dev_copy(dp, psi, s);
@@ -634,6 +647,9 @@ int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::
 void free_problem(DeviceProblem &dp) {
     for (void *p : dp.allocs) hipFree(p);
     dp.allocs.clear();
+    if (dp.d_dot_partial) (void)hipFree(dp.d_dot_partial);
+    dp.d_dot_partial = nullptr;
+    dp.dot_partial_cap = 0;
 }
 
 // ------------------------------------------------------- inner products --
@@ -702,12 +718,34 @@ __global__ void k_combine_tasks(int m, const idx_t *ptr, const dd *tv, dd *out) 
     out[i] = r;
 }
 
+// Dense tasks: one thread per chunk while chunks are short; one workgroup
+// per chunk (coalesced loads) once they are long.  Same sums either way.
+static void dense_dots(hipStream_t s, DeviceProblem &dp, const sdk::DotTaskOff *tasks, int ntasks, idx_t maxn,
+                       const dd *W, int T) {
+    if (ntasks <= 0) return;
+    const size_t need = (size_t)T * ntasks;
+    bool chunked = maxn / T > 64;
+    if (chunked && need > dp.dot_partial_cap) {
+        if (dp.d_dot_partial) (void)hipFree(dp.d_dot_partial);
+        dp.d_dot_partial = nullptr;
+        dp.dot_partial_cap = 0;
+        if (hipMalloc((void **)&dp.d_dot_partial, sizeof(dd) * need) == hipSuccess)
+            dp.dot_partial_cap = need;
+        else
+            chunked = false;
+    }
+    if (chunked)
+        sdk::dot_tasks_off(s, tasks, ntasks, W, T, dp.d_dot_partial);
+    else
+        sdk::dot_batch_off(s, tasks, ntasks, W, T);
+}
+
 void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
     if (dp.ntask > 0)
         k_sparse_tasks<<<dp.ntask, 64, 0, s>>>(dp.ntask, dp.d_task_type, dp.d_task_block, dp.d_task_nnz,
                                                               dp.d_task_off, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val,
                                                               dp.d_boff, dp.d_bsize, W, dp.d_task_val);
-    if (dp.n_dense_tasks > 0) sdk::dot_batch_off(s, dp.d_dense_tasks, dp.n_dense_tasks, W, T);
+    dense_dots(s, dp, dp.d_dense_tasks, dp.n_dense_tasks, dp.dense_task_maxn, W, T);
     k_combine_tasks<<<(dp.m + 255) / 256, 256, 0, s>>>(dp.m, dp.d_task_ptr, dp.d_task_val, out);
 }
 void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
@@ -715,7 +753,7 @@ void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
         k_sparse_tasks<<<dp.nctask, 64, 0, s>>>(dp.nctask, dp.d_ctask_type, dp.d_ctask_block,
                                                                dp.d_ctask_nnz, dp.d_ctask_off, dp.d_sp_row, dp.d_sp_col,
                                                                dp.d_sp_val, dp.d_boff, dp.d_bsize, W, dp.d_ctask_val);
-    if (dp.n_dense_ctasks > 0) sdk::dot_batch_off(s, dp.d_dense_ctasks, dp.n_dense_ctasks, W, T);
+    dense_dots(s, dp, dp.d_dense_ctasks, dp.n_dense_ctasks, dp.dense_ctask_maxn, W, T);
     k_combine_tasks<<<1, 32, 0, s>>>(1, dp.d_ctask_ptr, dp.d_ctask_val, out);
 }
 
@@ -765,11 +803,13 @@ __global__ void k_rowlist_mul(int n, int ns, const int *slot_list, int nslots, c
 
 // Pair values computed by one thread: F1 with sparse A_j, F2 with sparse
 // A_j, and F3 (sdpa_newton.cpp:895-977).
-__global__ void k_pair_thread(int npairs, int n, const int *pform, const int *psi, const int *psj, const int *type,
-                              const int *nnz, const idx_t *off, const int *wslot, const int *row, const int *col,
-                              const dd *val, const dd *X, const dd *Zinv, const dd *work1, const dd *work2, dd *pv) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+__global__ void k_pair_thread(int nlist, const int *list, int n, const int *pform, const int *psi, const int *psj,
+                              const int *type, const int *nnz, const idx_t *off, const int *wslot, const int *row,
+                              const int *col, const dd *val, const dd *X, const dd *Zinv, const dd *work1,
+                              const dd *work2, dd *pv) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlist) return;
+    const int p = list[t];
     int f = pform[p], si = psi[p], sj = psj[p];
     if (type[sj] != 0) return;  // dense A_j: chunked dot kernel
     const int *rj = row + off[sj], *cj = col + off[sj];
@@ -822,6 +862,40 @@ __global__ void k_pair_thread(int npairs, int n, const int *pform, const int *ps
         }
     }
     pv[p] = ret;
+}
+// F2 pairs with a sparse A_j (sdpa_newton.cpp:897-935), one wave per pair:
+// for each stored entry (a, b) of A_j the n products X(a,t) F(t,b) are
+// formed by the lanes into LDS and lane 0 adds them in t order (then the
+// (b, a) term for an off-diagonal entry), as the reference's loop does.
+__global__ __launch_bounds__(64) void k_pair_f2(int nlist, const int *list, int n, const int *psi, const int *psj,
+                                                const int *nnz, const idx_t *off, const int *wslot, const int *row,
+                                                const int *col, const dd *val, const dd *X, const dd *work1, dd *pv) {
+    __shared__ dd prod[256];
+    const int p = list[blockIdx.x], lane = threadIdx.x;
+    const int si = psi[p], sj = psj[p];
+    const int *rj = row + off[sj], *cj = col + off[sj];
+    const dd *vj = val + off[sj];
+    const int nj = nnz[sj];
+    const dd *F = work1 + (idx_t)wslot[si] * n * n;
+    dd ret = mk(0.0);
+    for (int q = 0; q < nj; q++) {
+        const int a = rj[q], b = cj[q];
+        for (int pass = 0; pass < (a != b ? 2 : 1); pass++) {
+            const int ra = pass ? b : a, cb = pass ? a : b;
+            dd v2 = mk(0.0);
+            for (int t0 = 0; t0 < n; t0 += 256) {
+                const int cnt = n - t0 < 256 ? n - t0 : 256;
+                for (int e = lane; e < cnt; e += 64)
+                    prod[e] = dd_mul(X[ra + (idx_t)n * (t0 + e)], F[(t0 + e) + (idx_t)n * cb]);
+                __syncthreads();
+                if (lane == 0)
+                    for (int e = 0; e < cnt; e++) v2 = dd_add_eq(v2, prod[e]);
+                __syncthreads();
+            }
+            if (lane == 0) ret = dd_add_eq(ret, dd_mul(vj[q], v2));
+        }
+    }
+    if (lane == 0) pv[p] = ret;
 }
 __global__ void k_scatter_pairs(int npairs, int m, const int *pi, const int *pj, const dd *pv, dd *B) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -885,10 +959,15 @@ void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B,
             sdk::gemm_ptrs(s, n, n, n, pA, n, pB, n, pC, n, c);
         }
         if (S.npairs > 0) {
-            k_pair_thread<<<(S.npairs + 127) / 128, 128, 0, s>>>(S.npairs, n, S.d_pair_form, S.d_pair_slot_i,
-                                                                  S.d_pair_slot_j, S.d_type, S.d_nnz, S.d_off, S.d_wslot,
-                                                                  dp.d_sp_row, dp.d_sp_col, dp.d_sp_val, Xl, Zl,
-                                                                  dp.d_work1, dp.d_work2, dp.d_pair_val);
+            if (S.n_other > 0)
+                k_pair_thread<<<(S.n_other + 127) / 128, 128, 0, s>>>(
+                    S.n_other, S.d_list_other, n, S.d_pair_form, S.d_pair_slot_i, S.d_pair_slot_j, S.d_type, S.d_nnz,
+                    S.d_off, S.d_wslot, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val, Xl, Zl, dp.d_work1, dp.d_work2,
+                    dp.d_pair_val);
+            if (S.n_f2 > 0)
+                k_pair_f2<<<S.n_f2, 64, 0, s>>>(S.n_f2, S.d_list_f2, n, S.d_pair_slot_i, S.d_pair_slot_j, S.d_nnz,
+                                                S.d_off, S.d_wslot, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val, Xl,
+                                                dp.d_work1, dp.d_pair_val);
             if (S.ndense_pairs > 0) sdk::dot_batch(s, S.d_pair_tasks, S.ndense_pairs, T);
             k_scatter_pairs<<<(S.npairs + 255) / 256, 256, 0, s>>>(S.npairs, m, S.d_pair_i, S.d_pair_j, dp.d_pair_val, B);
         }

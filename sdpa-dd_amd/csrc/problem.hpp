@@ -73,6 +73,10 @@ struct SchurBlock {
     int *d_pair_slot_j = nullptr;                  // position of j in ci
     int *d_pair_form = nullptr;
     std::vector<int> h_pair_form;
+    // pair ids with a sparse A_j, by kernel: F2 rows (one wave per pair) and
+    // F1/F3 rows (one thread per pair)
+    int n_f2 = 0, n_other = 0;
+    int *d_list_f2 = nullptr, *d_list_other = nullptr;
     // per-constraint block data on device (index = slot)
     int *d_type = nullptr;       // 0 sparse, 1 dense
     int *d_nnz = nullptr;        // NonZeroCount
@@ -130,7 +134,10 @@ struct DeviceProblem {
     int *d_bsize = nullptr;
     // dense inner-product tasks (y offset relative to the W passed in)
     int n_dense_tasks = 0, n_dense_ctasks = 0;
+    idx_t dense_task_maxn = 0, dense_ctask_maxn = 0;  // longest dense task
     sdk::DotTaskOff *d_dense_tasks = nullptr, *d_dense_ctasks = nullptr;
+    dd *d_dot_partial = nullptr;  // chunk sums of long dense tasks (grown on demand)
+    size_t dot_partial_cap = 0;
     std::vector<SchurBlock> schur;
     // pointer arrays for batched GEMMs
     void *d_ptrs = nullptr;

@@ -142,6 +142,38 @@ def test_ozaki_gemm_transposed_a(gpu, oracle):
     assert np.all(dd_diff(dC.get(), ref) <= bound)
 
 
+# ------------------------------------------------------ FP64-slice variant --
+# |S| digits of 18 bits per element, held in doubles and accumulated by FP64
+# MFMA: the truncation error is below k 2^(10 - 18|S|) rowmax colmax; at 7
+# slices that is below dd rounding, so the bound adds the two dd products'
+# own rounding, k 2^-100 rowmax colmax.
+@pytest.mark.parametrize("m,n,k", [(7, 5, 3), (65, 63, 17), (130, 70, 200), (200, 129, 801)])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("S", [-5, -6, -7])
+def test_ozaki_fp64_slices_error_bound(gpu, oracle, m, n, k, tb, S):
+    rng = np.random.default_rng(13 * m + n + k - S)
+    A, B, C = rand_dd((m * k,), rng), rand_dd((k * n,), rng), rand_dd((m * n,), rng)
+    ldb = k if tb == "N" else n
+    ref = ref_gemm(oracle, tb, m, n, k, -1, A, B, ldb, 1, C)
+    got = oz(gpu, tb, m, n, k, -1, A, B, ldb, 1, C, S=S)
+    sc = scale_bound(m, n, k, A, B, tb, ldb).T.reshape(-1)
+    err = dd_diff(got, ref)
+    assert np.all(err <= k * (2.0 ** (10 - 18 * -S) + 2.0 ** -100) * sc + 1e-300)
+    if S == -5:
+        # 90 bits: the truncation shows above dd rounding
+        assert np.max(err / sc) > 2.0 ** -110
+
+
+def test_ozaki_fp64_slices_exact_on_small_integers(gpu, oracle):
+    m, n, k = 100, 90, 300
+    rng = np.random.default_rng(4)
+    A = np.stack([rng.integers(-2 ** 20, 2 ** 20, m * k).astype(float), np.zeros(m * k)], -1)
+    B = np.stack([rng.integers(-2 ** 20, 2 ** 20, k * n).astype(float), np.zeros(k * n)], -1)
+    C = np.zeros((m * n, 2))
+    ref = ref_gemm(oracle, "N", m, n, k, 1, A, B, k, 0, C)
+    assert same(oz(gpu, "N", m, n, k, 1, A, B, k, 0, C, S=-4), ref)
+
+
 # ---------------------------------------------------------------- solver --
 def _solve(gpu, path, data, param, slices=16, refine=2):
     gpu.set_path(path, slices, refine)
@@ -188,7 +220,8 @@ def test_ozaki_solve_matches_plain(gpu, tmp_path, builder, kw, eps, path):
 
 
 @pytest.mark.parametrize("name,slices", [("theta1", 16), ("theta2", 16), ("mcp100", 16), ("mcp124-1", 16),
-                                         ("mcp124-3", 16), ("theta1", 14), ("mcp100", 14)])
+                                         ("mcp124-3", 16), ("theta1", 14), ("mcp100", 14), ("theta1", -7),
+                                         ("mcp100", -7)])
 def test_ozaki_solve_sdplib(gpu, name, slices):
     """SDPLIB instances with the reference's SDPLIB parameters, Ozaki
     Cholesky form: same phase and optimal values as the reference's result
