@@ -97,6 +97,19 @@ int sdpadd_Rtrsm(sdpadd_handle h, char side, char uplo, char transa, char diag, 
 /* Rtrmm: (L,L,T,N), alpha = 1 (mplapack/Rtrmm.cpp:31-275). */
 int sdpadd_Rtrmm(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n,
                  const double *alpha, const double *d_A, int64_t lda, double *d_B, int64_t ldb);
+/* The Ozaki form's triangular solve (no counterpart in the reference): x :=
+ * L^{-1} x ('N') or L^{-T} x ('T') for lower non-unit L, by 32-row blocks
+ * with inverted diagonal blocks in a cooperative grid; not Rtrsv's operation
+ * order (the Ozaki form refines its Schur solves against B).  Each call
+ * forms the block operators of L; sdpadd_solve forms them once per factor. */
+int sdpadd_ozaki_trsv(sdpadd_handle h, char trans, int64_t n, const double *d_L, int64_t lda, double *d_x);
+/* The Ozaki form's mat-vecs and dot (no counterpart in the reference): sums
+ * split into independent parts and merged (not Rgemv/Rtrmv/Rdot's order).
+ * op 'G': y = alpha*A*x + beta*y (A m x n); 'L': y = L*x, 'T': y = L^T*x
+ * (L lower n x n, m = n; alpha, beta unused).  result: dd of x.y (host). */
+int sdpadd_ozaki_matvec(sdpadd_handle h, char op, int64_t m, int64_t n, const double *alpha, const double *d_A,
+                        int64_t lda, const double *d_x, const double *beta, double *d_y);
+int sdpadd_ozaki_dot(sdpadd_handle h, int64_t n, const double *d_x, const double *d_y, double *result);
 /* Rtrsv / Rtrmv: uplo 'L', trans 'N' or 'T', diag 'N', incx 1
  * (mplapack/Rtrsv.cpp:31-234, Rtrmv.cpp:31-234). */
 int sdpadd_Rtrsv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, const double *d_A, int64_t lda,

@@ -1,4 +1,5 @@
 // extern "C" entry points of libsdpadd_hip.so (declared in include/sdpadd.h).
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -180,6 +181,45 @@ int sdpadd_Rtrsv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, c
         return bad(h, "Rtrsv: unsupported op");
     if (h->c.trsv(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x)) != 0)
         return bad(h, "Rtrsv: a wait of the cooperative solve ran out");
+    return check(h);
+}
+int sdpadd_ozaki_trsv(sdpadd_handle h, char trans, int64_t n, const double *d_L, int64_t lda, double *d_x) {
+    if (!h || n < 0 || lda < std::max<int64_t>(1, n) || !(up(trans, 'N') || up(trans, 'T')))
+        return bad(h, "ozaki_trsv: bad argument");
+    if (n == 0) return 0;
+    hipStream_t s = h->c.stream;
+    if (h->c.binv_prepare(s, n, DP(d_L), lda) != 0) return bad(h, "ozaki_trsv: workspace allocation failed");
+    sdk::zero_ints(s, h->c.d_tsw_sticky, 1);
+    (void)h->c.trsv_binv(s, up(trans, 'N') ? 'N' : 'T', n, DP(d_L), lda, DP(d_x));
+    h->c.trsv_poll(s);
+    int rc = check(h);
+    if (rc) return rc;
+    if (*h->c.h_tsw_err != 0) {
+        *h->c.h_tsw_err = 0;
+        return bad(h, "ozaki_trsv: a wait of the cooperative solve ran out");
+    }
+    return 0;
+}
+int sdpadd_ozaki_matvec(sdpadd_handle h, char op, int64_t m, int64_t n, const double *alpha, const double *d_A,
+                        int64_t lda, const double *d_x, const double *beta, double *d_y) {
+    if (!h || m < 0 || n < 0 || lda < std::max<int64_t>(1, m)) return bad(h, "ozaki_matvec: bad argument");
+    hipStream_t s = h->c.stream;
+    if (up(op, 'G')) {
+        if (!alpha || !beta) return bad(h, "ozaki_matvec: alpha and beta needed");
+        sdk::gemv_n_fast(s, m, n, D(alpha), DP(d_A), lda, DP(d_x), D(beta), DP(d_y));
+    } else if (up(op, 'L') || up(op, 'T')) {
+        if (m != n) return bad(h, "ozaki_matvec: triangular operand must be square");
+        sdk::trmv_lower_fast(s, up(op, 'T') ? 'T' : 'N', n, DP(d_A), lda, DP(d_x), DP(d_y));
+    } else {
+        return bad(h, "ozaki_matvec: op must be G, L or T");
+    }
+    return check(h);
+}
+int sdpadd_ozaki_dot(sdpadd_handle h, int64_t n, const double *d_x, const double *d_y, double *result) {
+    if (!h || n < 0 || !result) return bad(h, "ozaki_dot: bad argument");
+    hipStream_t s = h->c.stream;
+    sdk::dot_fast(s, n, DP(d_x), DP(d_y), h->c.d_scal, h->c.d_dot_work, h->c.d_counter + 2);
+    (void)hipMemcpyAsync(result, h->c.d_scal, sizeof(dd), hipMemcpyDeviceToHost, s);
     return check(h);
 }
 int sdpadd_Rtrmv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, const double *d_A, int64_t lda, double *d_x,

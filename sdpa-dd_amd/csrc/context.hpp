@@ -37,6 +37,13 @@ struct Context {
     // (the flag is then cleared); otherwise returns 0 without waiting and
     // the caller reads the flag later through trsv_poll() and a sync.
     int trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x, bool check = true);
+    // The Ozaki form's block-inverse solves: operators of the factor L (both
+    // directions) formed by binv_prepare, used by trsv_binv for that n.
+    dd *d_binv = nullptr;
+    size_t binv_cap = 0;
+    sdk::idx_t binv_n = -1;
+    int binv_prepare(hipStream_t s, sdk::idx_t n, const dd *L, sdk::idx_t lda);
+    int trsv_binv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x);
     // Enqueue the copy of the sticky flag into *h_tsw_err.
     void trsv_poll(hipStream_t s) {
         (void)hipMemcpyAsync(h_tsw_err, d_tsw_sticky, sizeof(int), hipMemcpyDeviceToHost, s);

@@ -422,20 +422,26 @@ __global__ __launch_bounds__(256) void k_panel_rltn(dd *B, idx_t ldb, idx_t r0, 
     dd x[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) x[q] = lane < nb ? Tt[4 * wave + q][lane] : mk(0.0);
-    for (int k = 0; k < nb; k++) {
-        if (lane == k) {
-            const dd r = rl[k];
+    // steps in groups of eight so that the LDS reads of a group are issued
+    // ahead of its dependent chain
+    for (int k0 = 0; k0 < nb; k0 += 8) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) x[q] = dd_mul(r, x[q]);
-        }
-        dd xk[4];
+        for (int u = 0; u < 8; u++) {
+            const int k = k0 + u;
+            if (k < nb) {
+                const dd l = Lc[lpack(lane < nb ? lane : k, k)];
+                if (lane == k) {
+                    const dd r = rl[k];
 #pragma unroll
-        for (int q = 0; q < 4; q++) xk[q] = readlane_dd(x[q], k);
-        if (lane > k && lane < nb) {
-            const dd l = Lc[lpack(lane, k)];
-            if (!dd_is_zero(l)) {
+                    for (int q = 0; q < 4; q++) x[q] = dd_mul(r, x[q]);
+                }
+                dd xk[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) x[q] = dd_sub(x[q], dd_mul(l, xk[q]));
+                for (int q = 0; q < 4; q++) xk[q] = readlane_dd(x[q], k);
+                if (lane > k && lane < nb && !dd_is_zero(l)) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) x[q] = dd_sub(x[q], dd_mul(l, xk[q]));
+                }
             }
         }
     }
@@ -1006,6 +1012,274 @@ int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, 
     return 0;
 }
 
+
+// ------------------------------------------ block-inverse triangular solves --
+// The Ozaki form's Schur solves (not Rtrsv's operation order; the Ozaki
+// form refines the result against B).  Blocks of 32 rows are solved in a
+// processing sequence (L x = b: from the top; L^T x = b: from the bottom),
+// block s as x_s = Dinv_s (p_s - sum_{d=2..BI_D} T_{s,s-d} x_{s-d})
+// - Dinv_s T_{s,s-1} x_{s-1}, where Dinv_s inverts the diagonal block,
+// T_{s,t} is the coupling block (L_{s,t}, or L_{t,s}^T) and p_s is b_s
+// after the updater workgroups applied blocks <= s - BI_D - 1.  The
+// operators Dinv_s and N_{s,d} = Dinv_s T_{s,s-d} are formed once per
+// factor.  In workgroup 0, wave 0 runs the one dependent step per block
+// (x_s from x_{s-1}: a 32-term sum per lane) while waves 1-3 prepare block
+// s+1 (the Dinv product of its partial and the d = 2, 3 terms).
+static constexpr int BI_D = 3;
+static constexpr int BI_OPS = (BI_D + 1) * SB * SB;  // dd per block and direction
+
+__host__ __device__ __forceinline__ void seq_block(idx_t n, int trans, int s, idx_t &lo, int &nb) {
+    if (!trans) {
+        lo = (idx_t)s * SB;
+        nb = (int)((n - lo) < SB ? (n - lo) : SB);
+    } else {
+        const idx_t hi = n - (idx_t)s * SB;
+        lo = hi - SB > 0 ? hi - SB : 0;
+        nb = (int)(hi - lo);
+    }
+}
+
+// inv[s] = (diagonal block s)^{-1}, row-major 32 x 32, zero outside; one
+// wave per block, lane c forms column c by forward substitution.
+__global__ __launch_bounds__(64) void k_blk_inv(idx_t n, int trans, const dd *L, idx_t lda, dd *inv) {
+    __shared__ dd T[SB][SB + 1];
+    const int s = blockIdx.x, lane = threadIdx.x;
+    idx_t lo;
+    int nb;
+    seq_block(n, trans, s, lo, nb);
+    for (int e = lane; e < SB * SB; e += 64) {
+        const int i = e % SB, k = e / SB;
+        T[i][k] = (i < nb && k <= i) ? L[(lo + i) + (lo + k) * lda] : mk(0.0);
+    }
+    __syncthreads();
+    const int c = lane;
+    dd y[SB];
+#pragma unroll
+    for (int i = 0; i < SB; i++) {
+        dd acc = mk(i == c ? 1.0 : 0.0);
+#pragma unroll
+        for (int k = 0; k < i; k++)
+            if (k >= c) acc = dd_sub(acc, dd_mul(T[i][k], y[k]));
+        y[i] = (c < nb && i < nb && i >= c) ? dd_div(acc, T[i][i]) : mk(0.0);
+    }
+    if (c < SB)
+#pragma unroll
+        for (int i = 0; i < SB; i++) inv[((idx_t)s * SB + i) * SB + c] = y[i];
+}
+
+// ops[s][0] = Dinv_s (the inverse, transposed for L^T x = b);
+// ops[s][d] = Dinv_s T_{s,s-d} for d = 1..BI_D (zero when s - d < 0).
+__global__ __launch_bounds__(256) void k_blk_ops(idx_t n, int trans, const dd *L, idx_t lda, const dd *inv, dd *ops) {
+    __shared__ dd Iv[SB][SB + 1];
+    __shared__ dd T[SB][SB + 1];
+    const int s = blockIdx.x, d = blockIdx.y, tid = threadIdx.x;
+    idx_t lo;
+    int nb;
+    seq_block(n, trans, s, lo, nb);
+    dd *out = ops + (idx_t)s * BI_OPS + (idx_t)d * SB * SB;
+    for (int e = tid; e < SB * SB; e += 256) {
+        const int i = e % SB, k = e / SB;
+        Iv[i][k] = trans ? inv[((idx_t)s * SB + k) * SB + i] : inv[((idx_t)s * SB + i) * SB + k];
+    }
+    if (d == 0) {
+        __syncthreads();
+        for (int e = tid; e < SB * SB; e += 256) out[e] = Iv[e / SB][e % SB];
+        return;
+    }
+    if (s - d < 0) {
+        for (int e = tid; e < SB * SB; e += 256) out[e] = mk(0.0);
+        return;
+    }
+    idx_t lo2;
+    int nb2;
+    seq_block(n, trans, s - d, lo2, nb2);
+    for (int e = tid; e < SB * SB; e += 256) {
+        const int k = e % SB, q = e / SB;
+        dd v = mk(0.0);
+        if (k < nb && q < nb2) v = trans ? L[(lo2 + q) + (lo + k) * lda] : L[(lo + k) + (lo2 + q) * lda];
+        T[k][q] = v;
+    }
+    __syncthreads();
+    for (int e = tid; e < SB * SB; e += 256) {
+        const int i = e / SB, q = e % SB;
+        dd acc = mk(0.0);
+        for (int k = 0; k < nb; k++) acc = dd_add(acc, dd_mul(Iv[i][k], T[k][q]));
+        out[e] = acc;
+    }
+}
+
+// flags, vg and the owner workgroups as in k_trsv_ln_coop / k_trsv_lt_coop;
+// an owner applies sequence block J to its rows in blocks >= J + BI_D + 1.
+__global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd *L, idx_t lda, dd *x, const dd *ops,
+                                                   int *flags, int nupd, dd *vg, int *sticky) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nblk = (int)((n + SB - 1) / SB);
+    int *err = flags + 1 + nupd;
+    if (blockIdx.x == 0) {
+        __shared__ dd hp[2][3][SB];          // prepared terms of the next block
+        __shared__ dd xs[BI_D + 1][SB];      // solved blocks, ring by s mod (BI_D + 1)
+        __shared__ int stop;
+        const int li = lane < SB ? lane : 0;
+        // each wave holds row li of the operator it applies next (wave 0:
+        // op 1 of the block it solves next; wave 1: op 0 and wave w = 2, 3:
+        // op w of the block it prepares next), loaded one round ahead
+        dd opr[SB];
+        auto load_row = [&](int t, int d) {
+            if (t >= nblk) return;
+            const dd *o = ops + (idx_t)t * BI_OPS + (idx_t)d * SB * SB + (idx_t)li * SB;
+#pragma unroll
+            for (int q = 0; q < SB; q++) opr[q] = o[q];
+        };
+        const int wop = wave == 1 ? 0 : wave;
+        // waves 1-3: terms of block t into hp[t & 1]
+        auto prepare = [&](int t) {
+            if (t >= nblk) return;
+            idx_t lo;
+            int nb;
+            seq_block(n, trans, t, lo, nb);
+            dd a = mk(0.0);
+            if (wave == 1) {
+                int ok = 1;
+                if (lane == 0 && t - BI_D - 1 >= 0) {
+                    const int u0 = (int)(lo / TC_ROWS), u1 = (int)((lo + nb - 1) / TC_ROWS);
+                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, t - BI_D, err, sticky);
+                }
+                ok = __builtin_amdgcn_readlane(ok, 0);
+                if (!ok) {
+                    if (lane == 0) stop = 1;
+                    return;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                const dd p = lane < nb ? x[lo + lane] : mk(0.0);
+#pragma unroll
+                for (int q = 0; q < SB; q++) a = dd_add(a, dd_mul(opr[q], readlane_dd(p, q)));
+            } else if (t - wave >= 0) {
+                const int slot = (t - wave) % (BI_D + 1);
+#pragma unroll
+                for (int q = 0; q < SB; q++) a = dd_add(a, dd_mul(opr[q], xs[slot][q]));
+            }
+            if (lane < SB) hp[t & 1][wave - 1][lane] = a;
+            load_row(t + 1, wop);
+        };
+        if (tid == 0) stop = 0;
+        load_row(wave == 0 ? 1 : 0, wave == 0 ? 1 : wop);
+        __syncthreads();
+        if (wave >= 1) prepare(0);
+        __syncthreads();
+        if (stop) return;
+        dd prev = mk(0.0);
+        for (int s = 0; s < nblk; s++) {
+            if (wave == 0) {
+                idx_t lo;
+                int nb;
+                seq_block(n, trans, s, lo, nb);
+                dd xr = dd_sub(dd_sub(hp[s & 1][0][li], hp[s & 1][1][li]), hp[s & 1][2][li]);
+                if (s >= 1) {
+                    dd a = mk(0.0);
+#pragma unroll
+                    for (int q = 0; q < SB; q++) a = dd_add(a, dd_mul(opr[q], readlane_dd(prev, q)));
+                    xr = dd_sub(xr, a);
+                }
+                if (lane >= nb) xr = mk(0.0);
+                if (lane < nb) {
+                    x[lo + lane] = xr;
+                    vg[(idx_t)s * SB + lane] = xr;
+                }
+                if (lane < SB) xs[s % (BI_D + 1)][lane] = xr;
+                prev = xr;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (lane == 0) tc_store(flags, s + 1);
+                load_row(s + 1, 1);
+            } else {
+                prepare(s + 1);
+            }
+            __syncthreads();
+            if (stop) return;
+        }
+        return;
+    }
+    const int u = blockIdx.x - 1;
+    const idx_t rlo = (idx_t)u * TC_ROWS, rhi = rlo + TC_ROWS < n ? rlo + TC_ROWS : n;
+    const idx_t i = rlo + tid;
+    __shared__ int go;
+    int avail = 0;  // blocks known to be published
+    dd xi = i < rhi ? x[i] : mk(0.0);
+    for (int J = 0; J + BI_D + 1 < nblk; J++) {
+        // rows of sequence blocks >= J + BI_D + 1 take block J here
+        idx_t lo_n;
+        int nb_n;
+        seq_block(n, trans, J + BI_D, lo_n, nb_n);
+        const bool act = i < rhi && (trans ? i < lo_n : i >= lo_n + nb_n);
+        if (trans ? rlo >= lo_n : rhi <= lo_n + nb_n) break;
+        idx_t bp;
+        int nbj;
+        seq_block(n, trans, J, bp, nbj);
+        dd a[SB];
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < SB; q++) a[q] = trans ? L[(bp + q) + i * lda] : L[i + (bp + q) * lda];
+        }
+        if (J + 1 > avail) {
+            if (tid == 0) {
+                go = tc_wait(flags, J + 1, err, sticky) ? tc_load(flags) : 0;
+            }
+            __syncthreads();
+            avail = go;
+            if (avail == 0) return;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < SB; q++) xi = dd_sub(xi, dd_mul(a[q], vg[(idx_t)J * SB + q]));
+            x[i] = xi;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) tc_store(flags + 1 + u, J + 1);
+    }
+    if (tid == 0) tc_store(flags + 1 + u, nblk);
+}
+
+size_t binv_ops_elems(idx_t n) { return (size_t)((n + SB - 1) / SB) * BI_OPS; }
+void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n, dd *ops_t, dd *inv) {
+    if (n <= 0) return;
+    const int nblk = (int)((n + SB - 1) / SB);
+    StatScope st("trsv", s, 2.0 * nblk * (SB * SB * SB / 3.0 + BI_D * SB * SB * SB));
+    k_blk_inv<<<nblk, 64, 0, s>>>(n, 0, L, lda, inv);
+    k_blk_ops<<<dim3(nblk, BI_D + 1), 256, 0, s>>>(n, 0, L, lda, inv, ops_n);
+    k_blk_inv<<<nblk, 64, 0, s>>>(n, 1, L, lda, inv);
+    k_blk_ops<<<dim3(nblk, BI_D + 1), 256, 0, s>>>(n, 1, L, lda, inv, ops_t);
+}
+int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, const dd *ops, void *work,
+              int *sticky) {
+    if (n <= 0) return 0;
+    const int nupd = (int)((n + TC_ROWS - 1) / TC_ROWS);
+    const idx_t nblk = (n + SB - 1) / SB;
+    char *w = (char *)work;
+    dd *vg = (dd *)(w + 256 - (((uintptr_t)w) & 255));
+    int *flags = (int *)(vg + nblk * SB) + nblk * SB;
+    zero_ints(s, flags, 2 + nupd);
+    StatScope st("trsv", s, (double)n * n);
+    const int tr = (trans == 'T' || trans == 't') ? 1 : 0;
+    dim3 grid(1 + nupd), block(256);
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    hipError_t e;
+    if ((int)grid.x <= cus) {
+        k_trsv_binv<<<grid, block, 0, s>>>(n, tr, L, lda, x, ops, flags, nupd, vg, sticky);
+        e = hipGetLastError();
+    } else {
+        void *args[] = {&n, (void *)&tr, &L, &lda, &x, &ops, &flags, (void *)&nupd, &vg, &sticky};
+        e = hipLaunchCooperativeKernel((const void *)k_trsv_binv, grid, block, args, 0, s);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return 0;
+}
+
 void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x) {
     if (n <= 0) return;
     StatScope st("trsv", s, (double)n * n);
@@ -1075,6 +1349,38 @@ void inv_lower(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb)
 void trsm_llnn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, dd *B, idx_t ldb) {
     if (m <= 0 || n <= 0) return;
     trsm_llnn_impl(s, m, n, A, lda, B, ldb, false);
+}
+
+// The Ozaki form's L^{-1}: blocked forward substitution on the identity in
+// panels of 256 rows (64-row diagonal solves and dd updates inside a panel),
+// the rows below a panel updated with the Ozaki product.
+int inv_lower_oz(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb, oz::Work &w, int S) {
+    if (n <= 0) return 0;
+    k_set_identity<<<dim3(cdiv(n, 256), (unsigned)n), 256, 0, s>>>(n, mk(1.0), B, ldb);
+    const idx_t OZ_NB = 256;
+    for (idx_t P = 0; P < n; P += OZ_NB) {
+        const idx_t pw = std::min<idx_t>(OZ_NB, n - P);
+        for (idx_t K = P; K < P + pw; K += PB) {
+            const int nb = (int)std::min<idx_t>(PB, P + pw - K);
+            const idx_t ncols = std::min<idx_t>(n, K + nb);
+            {
+                StatScope st("trsm", s, (double)ncols * nb * nb);
+                k_trsm_llnn_diag<<<std::min<idx_t>(cdiv(ncols, 4), 1024), 256, 0, s>>>(K, nb, ncols, L, lda, B, ldb);
+            }
+            const idx_t rem = P + pw - K - nb;
+            if (rem > 0)
+                gemm(s, 'N', rem, ncols, nb, mk(-1.0), L + (K + nb) + K * lda, lda, B + K, ldb, mk(1.0), B + (K + nb),
+                     ldb);
+        }
+        const idx_t below = n - P - pw;
+        if (below > 0) {
+            const idx_t ncols = std::min<idx_t>(n, P + pw);
+            int rc = oz::gemm(s, w, 'N', 'N', below, ncols, pw, -1, L + (P + pw) + P * lda, lda, B + P, ldb, 1,
+                              B + (P + pw), ldb, false, S);
+            if (rc != 0) return rc;
+        }
+    }
+    return 0;
 }
 
 // B := B * inv(A^T) for lower A (Rtrsm 'R','L','T','N', alpha = 1): the same
@@ -1341,6 +1647,146 @@ void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, c
     }
     StatScope st("mv", s, 2.0 * (double)m * n);
     k_gemv_n<<<cdiv(m, 64), 256, 0, s>>>(m, n, alpha, ak, A, lda, x, beta, bk, y);
+}
+
+
+// ------------------------------------------------- Ozaki form: fast order --
+// Mat-vecs and dots of the Ozaki form, which does not keep the reference's
+// summation order: sums are split into independent parts and merged, so no
+// output waits on an n-long chain of dd additions.
+// 'N'-type (y_i = sum_j A_ij x_j; kind 2: j <= i, the lower triangle): a
+// 1024-thread workgroup serves 64 rows, wave w summing the w-th sixteenth of
+// the columns; wave 0 merges the parts in order.
+__global__ __launch_bounds__(1024) void k_mvf_n(int kind, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *x,
+                                                dd alpha, int ak, dd beta, int bk, dd *y) {
+    __shared__ dd part[16][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const idx_t r = (idx_t)blockIdx.x * 64 + lane;
+    idx_t c0 = n * wave / 16, c1 = n * (wave + 1) / 16;
+    if (kind == 2 && c1 > r + 1) c1 = r + 1;
+    dd acc = mk(0.0);
+    if (r < m)
+        for (idx_t c = c0; c < c1; c++) acc = dd_add(acc, dd_mul(A[r + c * lda], x[c]));
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && r < m) {
+        dd t = part[0][lane];
+#pragma unroll
+        for (int w = 1; w < 16; w++) t = dd_add(t, part[w][lane]);
+        if (kind == 0) {
+            t = (ak == 1) ? t : (ak == 2 ? dd_neg(t) : dd_mul(alpha, t));
+            if (bk == 1)
+                t = dd_add(y[r], t);
+            else if (bk == 2)
+                t = dd_add(dd_mul(beta, y[r]), t);
+        }
+        y[r] = t;
+    }
+}
+// 'T'-type (w_j = sum_{i >= j} L_ij x_i): one wave per output, lanes over i,
+// then a shuffle tree.
+__device__ __forceinline__ dd wave_sum_dd(dd v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = dd_add(v, shfl_dd(v, (threadIdx.x & 63) ^ o));
+    return v;
+}
+__global__ __launch_bounds__(256) void k_mvf_lt(idx_t n, const dd *L, idx_t lda, const dd *x, dd *w) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const idx_t j = (idx_t)blockIdx.x * 4 + wave;
+    if (j >= n) return;
+    dd acc = mk(0.0);
+    for (idx_t i = j + lane; i < n; i += 64) acc = dd_add(acc, dd_mul(L[i + j * lda], x[i]));
+    acc = wave_sum_dd(acc);
+    if (lane == 0) w[j] = acc;
+}
+void trmv_lower_fast(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, const dd *x, dd *out) {
+    if (n <= 0) return;
+    StatScope st("mv", s, (double)n * n);
+    if (trans == 'T' || trans == 't')
+        k_mvf_lt<<<cdiv(n, 4), 256, 0, s>>>(n, L, lda, x, out);
+    else
+        k_mvf_n<<<cdiv(n, 64), 1024, 0, s>>>(2, n, n, L, lda, x, mk(1.0), 1, mk(0.0), 0, out);
+}
+void gemv_n_fast(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y) {
+    if (m <= 0) return;
+    const int ak = kind_alpha(alpha), bk = kind_beta(beta);
+    const bool az = (alpha.hi == 0.0 && alpha.lo == 0.0);
+    if (n == 0 || (az && bk == 1)) return;
+    if (az) {
+        k_gemm_scale_only<<<dim3(cdiv(m, 256), 1, 1), 256, 0, s>>>(m, 1, beta, bk, y, m, 0);
+        return;
+    }
+    StatScope st("mv", s, 2.0 * (double)m * n);
+    k_mvf_n<<<cdiv(m, 64), 1024, 0, s>>>(0, m, n, A, lda, x, alpha, ak, beta, bk, y);
+}
+// Dot with a fixed tree: per-thread sums over a grid stride, a workgroup
+// tree, then the last workgroup to finish merges the workgroup sums in
+// order (the ticket only picks which workgroup does it).
+__global__ __launch_bounds__(256) void k_dot_fast(idx_t n, const dd *x, const dd *y, dd *out, dd *partial,
+                                                  unsigned *counter) {
+    __shared__ dd sm[256];
+    __shared__ unsigned ticket;
+    dd acc = mk(0.0);
+    for (idx_t i = (idx_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (idx_t)gridDim.x * 256)
+        acc = dd_add(acc, dd_mul(x[i], y[i]));
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] = dd_add(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+    __threadfence();
+    if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
+    __syncthreads();
+    if (ticket != gridDim.x - 1) return;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        dd r = mk(0.0);
+        for (unsigned b = 0; b < gridDim.x; b++) r = dd_add(r, partial[b]);
+        *out = r;
+        *counter = 0u;
+    }
+}
+// Many dots in the fixed-tree order: grid (G, ntasks), workgroup g of task
+// t sums a grid stride of its products into partial[t G + g]; a second
+// kernel adds each task's G partials in order.
+__global__ __launch_bounds__(256) void k_dot_tasks_fast(const DotTask *tasks, dd *partial) {
+    __shared__ dd sm[256];
+    const DotTask tk = tasks[blockIdx.y];
+    dd acc = mk(0.0);
+    for (idx_t i = (idx_t)blockIdx.x * 256 + threadIdx.x; i < tk.n; i += (idx_t)gridDim.x * 256)
+        acc = dd_add(acc, dd_mul(tk.x[i], tk.y[i]));
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] = dd_add(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(idx_t)blockIdx.y * gridDim.x + blockIdx.x] = sm[0];
+}
+__global__ void k_dot_tasks_merge(const DotTask *tasks, int ntasks, int G, const dd *partial) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ntasks) return;
+    dd acc = mk(0.0);
+    for (int g = 0; g < G; g++) acc = dd_add(acc, partial[(idx_t)q * G + g]);
+    *tasks[q].out = acc;
+}
+void dot_tasks_fast(hipStream_t s, const DotTask *d_tasks, int ntasks, idx_t maxn, int G, dd *partial) {
+    if (ntasks <= 0) return;
+    unsigned g = cdiv(maxn, 256 * 16);
+    if (g > (unsigned)G) g = (unsigned)G;
+    if (g == 0) g = 1;
+    StatScope st("dot", s, 2.0 * (double)maxn * ntasks);
+    k_dot_tasks_fast<<<dim3(g, (unsigned)ntasks), 256, 0, s>>>(d_tasks, partial);
+    k_dot_tasks_merge<<<cdiv(ntasks, 64), 64, 0, s>>>(d_tasks, ntasks, (int)g, partial);
+}
+void dot_fast(hipStream_t s, idx_t n, const dd *x, const dd *y, dd *out, dd *work, unsigned *counter) {
+    unsigned g = cdiv(n, 256 * 16);
+    if (g > 256) g = 256;
+    if (g == 0) g = 1;
+    StatScope st("dot", s, 2.0 * (double)n);
+    k_dot_fast<<<g, 256, 0, s>>>(n, x, y, out, work, counter);
 }
 
 // ----------------------------------------------------------- elementwise --

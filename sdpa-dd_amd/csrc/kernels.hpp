@@ -27,6 +27,9 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
 namespace oz { struct Work; }
 namespace sdk {
 int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch, oz::Work &w, int S);
+// The Ozaki form's B := L^{-1} (rows below each 256-row panel updated with
+// the Ozaki product).  Returns 0 or the product's error code.
+int inv_lower_oz(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb, oz::Work &w, int S);
 
 // x := L^{-1} x ('N') or L^{-T} x ('T'), L lower non-unit.
 void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x);
@@ -37,6 +40,16 @@ void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *
 // cooperative launch is unavailable (nothing ran).
 size_t trsv_coop_bytes(idx_t n);
 int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, void *work, int *sticky);
+// The Ozaki form's solves: block-inverse cooperative L^{-1} x / L^{-T} x
+// (not Rtrsv's operation order).  trsv_binv_prepare forms the per-block
+// operators of both directions once per factor (ops_n, ops_t:
+// binv_ops_elems(n) dd each; inv: binv_ops_elems(n) dd of scratch);
+// trsv_binv uses the trsv_coop_bytes workspace and the sticky flag as
+// trsv_lower_coop and returns 0, or 1 when the launch fails.
+size_t binv_ops_elems(idx_t n);
+void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n, dd *ops_t, dd *inv);
+int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, const dd *ops, void *work,
+              int *sticky);
 // p[0..cnt) := 0 with a kernel (no memset node in captured graphs).
 void zero_ints(hipStream_t s, int *p, int cnt);
 
@@ -59,6 +72,17 @@ void syrk_ln(hipStream_t s, idx_t n, idx_t k, dd alpha, const dd *A, idx_t lda, 
 void trmv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, const dd *x, dd *out);
 // y := alpha*A*x (+ beta*y), Rgemv 'N'.
 void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y);
+
+// The Ozaki form's mat-vecs and dot (sums split into parts and merged, not
+// the reference's order).  dot_fast: work >= 256 dd, *counter 0 on entry
+// (left 0).
+void trmv_lower_fast(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, const dd *x, dd *out);
+void gemv_n_fast(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y);
+void dot_fast(hipStream_t s, idx_t n, const dd *x, const dd *y, dd *out, dd *work, unsigned *counter);
+struct DotTask;
+// Many dots in that order; partial: >= G * ntasks dd (G <= the partial
+// buffer's per-task share).
+void dot_tasks_fast(hipStream_t s, const DotTask *d_tasks, int ntasks, idx_t maxn, int G, dd *partial);
 
 // Elementwise (length len, flat).
 void fill(hipStream_t s, idx_t len, dd v, dd *x);

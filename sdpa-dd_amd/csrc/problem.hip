@@ -863,39 +863,50 @@ __global__ void k_pair_thread(int nlist, const int *list, int n, const int *pfor
     }
     pv[p] = ret;
 }
-// F2 pairs with a sparse A_j (sdpa_newton.cpp:897-935), one wave per pair:
-// for each stored entry (a, b) of A_j the n products X(a,t) F(t,b) are
-// formed by the lanes into LDS and lane 0 adds them in t order (then the
-// (b, a) term for an off-diagonal entry), as the reference's loop does.
-__global__ __launch_bounds__(64) void k_pair_f2(int nlist, const int *list, int n, const int *psi, const int *psj,
-                                                const int *nnz, const idx_t *off, const int *wslot, const int *row,
-                                                const int *col, const dd *val, const dd *X, const dd *work1, dd *pv) {
-    __shared__ dd prod[256];
-    const int p = list[blockIdx.x], lane = threadIdx.x;
+// F2 pairs with a sparse A_j (sdpa_newton.cpp:897-935), one workgroup per
+// pair.  For each stored entry (a, b) of A_j the reference forms
+// v = sum_t X(a,t) F(t,b) in t order (and the (b, a) sum for an
+// off-diagonal entry) and adds v_j * v to the pair value.  The inner sums of
+// different entries are independent: thread q forms those of entry q (its
+// own chains in t order), and thread 0 then adds the terms in entry order.
+__global__ __launch_bounds__(256) void k_pair_f2(int nlist, const int *list, int n, const int *psi, const int *psj,
+                                                 const int *nnz, const idx_t *off, const int *wslot, const int *row,
+                                                 const int *col, const dd *val, const dd *X, const dd *work1, dd *pv) {
+    __shared__ dd va[256], vb[256];
+    const int p = list[blockIdx.x], tid = threadIdx.x;
     const int si = psi[p], sj = psj[p];
     const int *rj = row + off[sj], *cj = col + off[sj];
     const dd *vj = val + off[sj];
     const int nj = nnz[sj];
     const dd *F = work1 + (idx_t)wslot[si] * n * n;
     dd ret = mk(0.0);
-    for (int q = 0; q < nj; q++) {
-        const int a = rj[q], b = cj[q];
-        for (int pass = 0; pass < (a != b ? 2 : 1); pass++) {
-            const int ra = pass ? b : a, cb = pass ? a : b;
-            dd v2 = mk(0.0);
-            for (int t0 = 0; t0 < n; t0 += 256) {
-                const int cnt = n - t0 < 256 ? n - t0 : 256;
-                for (int e = lane; e < cnt; e += 64)
-                    prod[e] = dd_mul(X[ra + (idx_t)n * (t0 + e)], F[(t0 + e) + (idx_t)n * cb]);
-                __syncthreads();
-                if (lane == 0)
-                    for (int e = 0; e < cnt; e++) v2 = dd_add_eq(v2, prod[e]);
-                __syncthreads();
+    for (int q0 = 0; q0 < nj; q0 += 256) {
+        const int q = q0 + tid;
+        if (q < nj) {
+            const int a = rj[q], b = cj[q];
+            dd v = mk(0.0);
+#pragma unroll 8
+            for (int t = 0; t < n; t++) v = dd_add_eq(v, dd_mul(X[a + (idx_t)n * t], F[t + (idx_t)n * b]));
+            va[tid] = v;
+            if (a != b) {
+                v = mk(0.0);
+#pragma unroll 8
+                for (int t = 0; t < n; t++) v = dd_add_eq(v, dd_mul(X[b + (idx_t)n * t], F[t + (idx_t)n * a]));
+                vb[tid] = v;
             }
-            if (lane == 0) ret = dd_add_eq(ret, dd_mul(vj[q], v2));
         }
+        __syncthreads();
+        if (tid == 0) {
+            const int cnt = nj - q0 < 256 ? nj - q0 : 256;
+            for (int e = 0; e < cnt; e++) {
+                const dd v1 = vj[q0 + e];
+                ret = dd_add_eq(ret, dd_mul(v1, va[e]));
+                if (rj[q0 + e] != cj[q0 + e]) ret = dd_add_eq(ret, dd_mul(v1, vb[e]));
+            }
+        }
+        __syncthreads();
     }
-    if (lane == 0) pv[p] = ret;
+    if (tid == 0) pv[p] = ret;
 }
 __global__ void k_scatter_pairs(int npairs, int m, const int *pi, const int *pj, const dd *pv, dd *B) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -965,7 +976,7 @@ void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B,
                     S.d_off, S.d_wslot, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val, Xl, Zl, dp.d_work1, dp.d_work2,
                     dp.d_pair_val);
             if (S.n_f2 > 0)
-                k_pair_f2<<<S.n_f2, 64, 0, s>>>(S.n_f2, S.d_list_f2, n, S.d_pair_slot_i, S.d_pair_slot_j, S.d_nnz,
+                k_pair_f2<<<S.n_f2, 256, 0, s>>>(S.n_f2, S.d_list_f2, n, S.d_pair_slot_i, S.d_pair_slot_j, S.d_nnz,
                                                 S.d_off, S.d_wslot, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val, Xl,
                                                 dp.d_work1, dp.d_pair_val);
             if (S.ndense_pairs > 0) sdk::dot_batch(s, S.d_pair_tasks, S.ndense_pairs, T);

@@ -57,6 +57,35 @@ int Context::trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx
     sdk::trsv_lower(s, trans, n, L, lda, x);
     return 0;
 }
+int Context::binv_prepare(hipStream_t s, sdk::idx_t n, const dd *L, sdk::idx_t lda) {
+    const size_t e = sdk::binv_ops_elems(n), need_ops = 3 * e;
+    if (need_ops > binv_cap) {
+        if (d_binv) (void)hipFree(d_binv);
+        d_binv = nullptr;
+        binv_cap = 0;
+        if (hipMalloc((void **)&d_binv, sizeof(dd) * need_ops) != hipSuccess) return -1;
+        binv_cap = need_ops;
+    }
+    const size_t need = sdk::trsv_coop_bytes(n);
+    if (need > tsw_cap) {
+        if (d_tsw) (void)hipFree(d_tsw);
+        d_tsw = nullptr;
+        tsw_cap = 0;
+        if (hipMalloc(&d_tsw, need) != hipSuccess) return -1;
+        tsw_cap = need;
+    }
+    binv_n = n;
+    sdk::trsv_binv_prepare(s, n, L, lda, d_binv, d_binv + e, d_binv + 2 * e);
+    return 0;
+}
+int Context::trsv_binv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x) {
+    const size_t e = sdk::binv_ops_elems(n);
+    const bool tr = (trans == 'T' || trans == 't');
+    if (n == binv_n && sdk::trsv_binv(s, trans, n, L, lda, x, d_binv + (tr ? e : 0), d_tsw, d_tsw_sticky) == 0)
+        return 0;
+    sdk::trsv_lower(s, trans, n, L, lda, x);
+    return 0;
+}
 void Context::destroy() {
     if (d_dot_work) (void)hipFree(d_dot_work);
     if (d_counter) (void)hipFree(d_counter);
@@ -66,6 +95,9 @@ void Context::destroy() {
     if (d_max_work) (void)hipFree(d_max_work);
     ozw.release();
     if (d_tsw) (void)hipFree(d_tsw);
+    if (d_binv) (void)hipFree(d_binv);
+    d_binv = nullptr;
+    binv_cap = 0;
     if (d_tsw_sticky) (void)hipFree(d_tsw_sticky);
     if (h_tsw_err) (void)hipHostFree(h_tsw_err);
     d_tsw = nullptr;
@@ -96,6 +128,7 @@ struct Chain {
     ddr alpha, beta, value, min = 1.0e+51, min_old = 1.0e+52, error = 1.0e+10;
     int k = 0, kk = 0;
     bool done = false;
+    bool fast = false;  // the Ozaki form: mat-vecs in split order
     std::vector<ddr> diagVec, diagVec2, out, bv, work;
     ddr result;
 };
@@ -153,9 +186,15 @@ void chain_enqueue(Chain &c, int T) {
     const int nDim = c.n;
     c.value = ddr(1.0) / c.beta;
     sdk::lanczos_q(s, nDim, c.value.v, c.r, c.q, c.qold);   // qold = q; q = r / beta
-    sdk::trmv_lower(s, 'T', nDim, c.L, nDim, c.q, c.w);
-    sdk::gemv_n(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
-    sdk::trmv_lower(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
+    if (c.fast) {
+        sdk::trmv_lower_fast(s, 'T', nDim, c.L, nDim, c.q, c.w);
+        sdk::gemv_n_fast(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
+        sdk::trmv_lower_fast(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
+    } else {
+        sdk::trmv_lower(s, 'T', nDim, c.L, nDim, c.q, c.w);
+        sdk::gemv_n(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
+        sdk::trmv_lower(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
+    }
     // alpha = q.w; r = w - alpha q - beta qold; r.r
     sdk::lanczos_tail(s, nDim, T, c.q, c.qold, c.w, c.r, dd_neg(c.beta.v), c.scal);
     (void)hipMemcpyAsync(c.h, c.scal, sizeof(dd) * 2, hipMemcpyDeviceToHost, s);
@@ -190,13 +229,14 @@ void chain_finish_step(Chain &c) {
 }
 
 // Runs chains[0..cnt) to completion; chain i computes lambda_min(L_i M_i L_i^T).
-void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent) {
+void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) {
     hipEvent_t ready;
     (void)hipEventCreateWithFlags(&ready, hipEventDisableTiming);
     (void)hipEventRecord(ready, parent);
     for (int i = 0; i < cnt; i++) {
         Chain &c = pool.chains[i];
         (void)hipStreamWaitEvent(c.s, ready, 0);
+        c.fast = fast;
         const int nDim = c.n;
         c.min = 1.0e+51;
         c.min_old = 1.0e+52;
@@ -313,7 +353,10 @@ class Solver {
         return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
     }
     void dot(idx_t n, const dd *x, const dd *y, int slot) {
-        sdk::dot(s, n, x, y, d_sc + slot, ctx.dot_T, ctx.d_dot_work, ctx.d_counter);
+        if (ctx.path >= 1)
+            sdk::dot_fast(s, n, x, y, d_sc + slot, ctx.d_dot_work, ctx.d_counter + 2);
+        else
+            sdk::dot(s, n, x, y, d_sc + slot, ctx.dot_T, ctx.d_dot_work, ctx.d_counter);
     }
     // Download scalar slots [first, first + cnt) into h_sc.
     const dd *get_scalars(int first, int cnt) {
@@ -325,8 +368,16 @@ class Solver {
     // ---- dense linear-space helpers (sdpa_linear.cpp Lal::*) ----
     // DenseLinearSpace inner product X . Z: per-block Rdot (all blocks in one
     // launch), summed in block order.
+    // the solver's batched dots: the reference's partition order, or the
+    // Ozaki form's fixed tree (d_partial holds 3 nB T entries)
+    void dots(const sdk::DotTask *tasks, int cnt) {
+        if (ctx.path >= 1)
+            sdk::dot_tasks_fast(s, tasks, cnt, max_blk_elems, 3 * nB * ctx.dot_T / cnt, d_partial);
+        else
+            sdk::dot_tasks(s, tasks, cnt, max_blk_elems, ctx.dot_T, d_partial);
+    }
     ddr inner_xz() {
-        sdk::dot_tasks(s, d_tasks_xz, nB, max_blk_elems, ctx.dot_T, d_partial);
+        dots(d_tasks_xz, nB);
         const dd *h = get_scalars(0, nB);
         ddr r = 0.0;
         for (int l = 0; l < nB; l++) r += ddr(h[l]);
@@ -334,7 +385,7 @@ class Solver {
     }
     // X . dZ, dX . Z and dX . dZ with one launch and one synchronisation.
     void inner_dirs(ddr &r0, ddr &r1, ddr &r2) {
-        sdk::dot_tasks(s, d_tasks3, 3 * nB, max_blk_elems, ctx.dot_T, d_partial);
+        dots(d_tasks3, 3 * nB);
         const dd *h = get_scalars(0, 3 * nB);
         ddr r[3];
         for (int q = 0; q < 3; q++) {
@@ -400,11 +451,17 @@ class Solver {
     // The Schur solves do not wait for the device: a cooperative solve whose
     // bounded wait ran out sets a sticky device flag, copied to the host after
     // every solve and checked after the next synchronisation.
-    void trsv_chk(char trans, idx_t n, const dd *L, idx_t lda, dd *x) { (void)ctx.trsv(s, trans, n, L, lda, x, false); }
+    bool binv = false;
+    void trsv_chk(char trans, idx_t n, const dd *L, idx_t lda, dd *x) {
+        if (binv)
+            (void)ctx.trsv_binv(s, trans, n, L, lda, x);
+        else
+            (void)ctx.trsv(s, trans, n, L, lda, x, false);
+    }
     bool trsv_failed() const { return *ctx.h_tsw_err != 0; }
     void residual(const dd *x, dd *r) {
         sdk::copy(s, m, g, r);
-        sdk::gemv_n(s, m, m, mk(-1.0), bSave, m, x, mk(1.0), r);
+        sdk::gemv_n_fast(s, m, m, mk(-1.0), bSave, m, x, mk(1.0), r);
     }
     void schur_solve() {
         schur_solve_refined();
@@ -459,15 +516,25 @@ class Solver {
             int n = bsz(l);
             idx_t o = dp.boff[l];
             sdk::copy(st, (idx_t)n * n, A + o, work + o);
-            potrf(st, n, work + o, infos + l, rinv, st == s ? ctx.ozw : ozw2);
+            oz::Work &w = st == s ? ctx.ozw : ozw2;
+            potrf(st, n, work + o, infos + l, rinv, w);
             sdk::zero_upper(st, n, work + o);
-            sdk::inv_lower(st, n, work + o, n, invChol + o, n);
+            // path 2 (dense iterates): the inverse factor's updates and
+            // L^-T L^-1 as Ozaki products, like the direction's GEMMs
+            if (ctx.path == 2)
+                oz_check(sdk::inv_lower_oz(st, n, work + o, n, invChol + o, n, w, ctx.oz_slices));
+            else
+                sdk::inv_lower(st, n, work + o, n, invChol + o, n);
         }
         if (inv)
             for (int l = 0; l < nB; l++) {
                 int n = bsz(l);
                 idx_t o = dp.boff[l];
-                sdk::trmm_lltn_self(st, n, invChol + o, n, inv + o, n);
+                if (ctx.path == 2)
+                    oz_check(oz::gemm(st, st == s ? ctx.ozw : ozw2, 'T', 'N', n, n, n, 1, invChol + o, n, invChol + o,
+                                      n, 0, inv + o, n, false, ctx.oz_slices));
+                else
+                    sdk::trmm_lltn_self(st, n, invChol + o, n, inv + o, n);
             }
     }
     // Solutions::update's factorisations: X on the main stream and Z on the
@@ -510,7 +577,7 @@ class Solver {
             cz.L = invCholZ + dp.boff[l];
             cz.M = DzMat + dp.boff[l];
         }
-        run_chains(pool, 2 * nB, ctx.dot_T, s);
+        run_chains(pool, 2 * nB, ctx.dot_T, s, ctx.path >= 1);
         mx = 1.0E50;
         mz = 1.0E50;
         for (int l = 0; l < nB; l++) {
@@ -710,6 +777,8 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             t_schur += t1 - t0;
             t_chol += t2 - t1;
             if (info != 0) return false;
+            // the Ozaki form solves with the block-inverse operators of this factor
+            binv = ctx.path >= 1 && m >= 512 && ctx.binv_prepare(s, m, bMat, m) == 0;
         }
         schur_solve();
         sdp::plus_A(s, dp, dy, DzMat);
@@ -941,7 +1010,7 @@ dd lanczos_min_eigen(Context &ctx, int n, const dd *d_L, const dd *d_M) {
     pool.chains[0].n = n;
     pool.chains[0].L = d_L;
     pool.chains[0].M = d_M;
-    run_chains(pool, 1, ctx.dot_T, ctx.stream);
+    run_chains(pool, 1, ctx.dot_T, ctx.stream, false);
     (void)hipStreamSynchronize(ctx.stream);
     return pool.chains[0].result.v;
 }

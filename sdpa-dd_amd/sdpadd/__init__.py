@@ -67,6 +67,9 @@ def load_library(path=None):
         "sdpadd_Rtrmm": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_dp,
                                         vp, c_i64, vp, c_i64]),
         "sdpadd_Rtrsv": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, vp, c_i64, vp, c_i64]),
+        "sdpadd_ozaki_trsv": (ctypes.c_int, [vp, ctypes.c_char, c_i64, vp, c_i64, vp]),
+        "sdpadd_ozaki_matvec": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_i64, c_dp, vp, c_i64, vp, c_dp, vp]),
+        "sdpadd_ozaki_dot": (ctypes.c_int, [vp, c_i64, vp, vp, c_dp]),
         "sdpadd_Rtrmv": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, vp, c_i64, vp, c_i64]),
         "sdpadd_Rgemv": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_i64, c_dp, vp, c_i64, vp, c_i64, c_dp, vp, c_i64]),
         "sdpadd_Rdot": (ctypes.c_int, [vp, c_i64, vp, c_i64, vp, c_i64, c_dp]),

@@ -147,6 +147,64 @@ def test_rtrsv_rtrmv(gpu, oracle, n, trans):
     assert same(dx.get(), ref)
 
 
+@pytest.mark.parametrize("n", [1, 31, 33, 100, 130, 511, 512, 1100, 2049, 4551])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_ozaki_trsv(gpu, oracle, n, trans):
+    """The Ozaki form's block-inverse solve: not Rtrsv's operation order, so
+    compared with the oracle's dd substitution to a tolerance (a
+    well-conditioned factor: unit-scale diagonal, off-diagonal 0.5/sqrt(n))."""
+    from ddgen import dd_from_hi
+    rng = np.random.default_rng(n + 7)
+    a = np.tril(rng.standard_normal((n, n))) * (0.5 / np.sqrt(n)) + np.diag(1.0 + rng.uniform(0, 1, n))
+    A = dd_from_hi(a, rng)
+    A[np.triu_indices(n, 1)] = 0.0
+    A = np.ascontiguousarray(A.transpose(1, 0, 2)).reshape(-1, 2)
+    x = rand_dd((n,), rng)
+    ref = oracle.rtrsv(trans, n, A, n, x)
+    dA, dx = gpu.to_device(A), gpu.to_device(x)
+    gpu.check(gpu.lib.sdpadd_ozaki_trsv(gpu.h, trans.encode(), n, dA.ptr, n, dx.ptr), "ozaki_trsv")
+    got = dx.get()
+    err = np.abs((got[:, 0] - ref[:, 0]) + (got[:, 1] - ref[:, 1]))
+    assert np.max(err) <= 1e-26 * np.max(np.abs(ref[:, 0])), float(np.max(err) / np.max(np.abs(ref[:, 0])))
+
+
+def _dd_err(got, ref):
+    return np.abs((got[:, 0] - ref[:, 0]) + (got[:, 1] - ref[:, 1]))
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 63, 64, 65, 300, 801, 2000])
+def test_ozaki_matvecs_and_dot(gpu, oracle, n):
+    """The Ozaki form's split-order mat-vecs and dot against the oracle's
+    reference-order results: |diff| <= 4 n 2^-104 sum |terms| (each order's
+    dd rounding)."""
+    rng = np.random.default_rng(n + 100)
+    A = rand_dd((n * n,), rng)
+    L = rand_lower_cm(n, rng)
+    x = rand_dd((n,), rng)
+    y = rand_dd((n,), rng)
+    alpha, beta = rand_dd((1,), rng)[0], rand_dd((1,), rng)[0]
+    tol = (4 * n + 16) * 2.0 ** -104
+    absA = np.abs(A[:, 0]).reshape(n, n).T   # row-major view of column-major A
+    absL = np.abs(L[:, 0]).reshape(n, n).T
+    ax = np.abs(x[:, 0])
+    # y = alpha A x + beta y
+    ref = oracle.rgemv(n, n, alpha, A, n, x, beta, y)
+    dA, dx, dy = gpu.to_device(A), gpu.to_device(x), gpu.to_device(y)
+    gpu.check(gpu.lib.sdpadd_ozaki_matvec(gpu.h, b"G", n, n, dp(alpha), dA.ptr, n, dx.ptr, dp(beta), dy.ptr), "mv G")
+    scale = abs(alpha[0]) * (absA @ ax) + abs(beta[0]) * np.abs(y[:, 0])
+    assert np.all(_dd_err(dy.get(), ref) <= tol * scale + 1e-300)
+    for op, tr in ((b"L", "N"), (b"T", "T")):
+        ref = oracle.rtrmv(tr, n, L, n, x)
+        dL, dx2, dout = gpu.to_device(L), gpu.to_device(x), gpu.empty((n, 2))
+        gpu.check(gpu.lib.sdpadd_ozaki_matvec(gpu.h, op, n, n, None, dL.ptr, n, dx2.ptr, None, dout.ptr), "mv")
+        sc = (absL @ ax) if tr == "N" else (absL.T @ ax)
+        assert np.all(_dd_err(dout.get(), ref) <= tol * sc + 1e-300), op
+    res = np.zeros(2)
+    gpu.check(gpu.lib.sdpadd_ozaki_dot(gpu.h, n, dx.ptr, gpu.to_device(y).ptr, dp(res)), "dot")
+    ref = oracle.rdot(x, y)
+    assert abs((res[0] - ref[0]) + (res[1] - ref[1])) <= tol * float(np.sum(np.abs(x[:, 0] * y[:, 0]))) + 1e-300
+
+
 @pytest.mark.parametrize("m,n", [(1, 1), (33, 17), (64, 16), (65, 33), (130, 801), (300, 300)])
 @pytest.mark.parametrize("ab", [("gen", "zero"), ("m1", "one"), ("one", "gen")])
 def test_rgemv(gpu, oracle, m, n, ab):
