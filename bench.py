@@ -45,6 +45,11 @@ DD_FP64_SLOTS_PER_DDFLOP = 20   # a dd multiply-add is 20 FP64 instructions = 40
 OZ_SLICES = 16
 METRIC = "solve time to rel. gap 1e-25 (Ozaki-MFMA vs plain dd HIP); Schur dd-GFLOP/s"
 FAMILIES = ("gemm", "ozaki", "potrf", "trsm", "trmm", "trsv", "mv", "dot")
+# HBM bytes per launch of the dominant kernels on the max-cut workload, from
+# two rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors
+# a 256 MiB copy needs to read its true size); profiles/gemm_hbm_traffic_maxcut800.md
+TRAFFIC_MAXCUT = {"oz::k_gemm": 31.29e6 + 2.83e6, "sdk::k_gemm": 38.84e6 + 2.45e6}
+TRAFFIC_SOURCE = "profiles/gemm_hbm_traffic_maxcut800.md"
 
 
 def workload_files(tmp, workload, gap):
@@ -167,14 +172,18 @@ def main():
         "roofline": {
             "bound": "mfma", "kernel": "oz::k_gemm (Ozaki dd product, INT8 MFMA v_mfma_i32_16x16x64_i8)",
             "achieved": round(oz_int8_tops, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8 multiply-adds x2)",
-            "frac": round(oz_int8_tops / INT8_PEAK_TOPS, 4), "traffic": None,
+            "frac": round(oz_int8_tops / INT8_PEAK_TOPS, 4),
+            "traffic": TRAFFIC_MAXCUT["oz::k_gemm"] if a.workload == "maxcut" else None,
+            "traffic_source": TRAFFIC_SOURCE if a.workload == "maxcut" else None,
             "avg_launch_ms": round(o["ms"] / max(1, o["launches"]), 4),
         },
         "roofline_plain": {
             "bound": "mfma", "kernel": "sdk::k_gemm (dd GEMM on the FP64 VALU)",
             "achieved": round(gemm_dd_tflops * DD_FP64_SLOTS_PER_DDFLOP, 3), "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(gemm_dd_tflops * DD_FP64_SLOTS_PER_DDFLOP / FP64_PEAK_TFLOPS, 4),
-            "traffic": None, "avg_launch_ms": round(g["ms"] / max(1, g["launches"]), 4),
+            "traffic": TRAFFIC_MAXCUT["sdk::k_gemm"] if a.workload == "maxcut" else None,
+            "traffic_source": TRAFFIC_SOURCE if a.workload == "maxcut" else None,
+            "avg_launch_ms": round(g["ms"] / max(1, g["launches"]), 4),
             "dd_tflops": round(gemm_dd_tflops, 4),
         },
     }

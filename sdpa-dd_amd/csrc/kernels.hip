@@ -1665,8 +1665,10 @@ __global__ __launch_bounds__(1024) void k_mvf_n(int kind, idx_t m, idx_t n, cons
     idx_t c0 = n * wave / 16, c1 = n * (wave + 1) / 16;
     if (kind == 2 && c1 > r + 1) c1 = r + 1;
     dd acc = mk(0.0);
-    if (r < m)
+    if (r < m) {
+#pragma unroll 8
         for (idx_t c = c0; c < c1; c++) acc = dd_add(acc, dd_mul(A[r + c * lda], x[c]));
+    }
     part[wave][lane] = acc;
     __syncthreads();
     if (wave == 0 && r < m) {
@@ -2020,14 +2022,30 @@ __global__ void k_lanczos_q(idx_t n, dd v, const dd *r, dd *q, dd *qold) {
 void lanczos_q(hipStream_t s, idx_t n, dd v, const dd *r, dd *q, dd *qold) {
     if (n > 0) k_lanczos_q<<<cdiv(n, 256), 256, 0, s>>>(n, v, r, q, qold);
 }
+// The same with the scalar on the device: v = sc[3] (the reciprocal of the
+// previous step's beta, formed by k_lanczos_tail_dev).
+__global__ void k_lanczos_q_dev(idx_t n, const dd *sc, const dd *r, dd *q, dd *qold) {
+    idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const dd v = sc[3];
+        qold[i] = q[i];
+        q[i] = dd_mul(v, r[i]);
+    }
+}
+void lanczos_q_dev(hipStream_t s, idx_t n, const dd *sc, const dd *r, dd *q, dd *qold) {
+    if (n > 0) k_lanczos_q_dev<<<cdiv(n, 256), 256, 0, s>>>(n, sc, r, q, qold);
+}
 // alpha = q.w; r = w; r += (-alpha) q; r += (-beta) qold (Raxpy skips a
 // zero scalar); rr = r.r.  Dots in the T-chunk order; out = {alpha, rr}.
 // One workgroup: thread t owns chunk t of both dots and the elements of
-// that chunk in the update.
+// that chunk in the update.  With sc (device scalars) beta is sc[2] and the
+// kernel then leaves the next step's sc[2] = sqrt(rr), sc[3] = 1/sc[2] (the
+// host's Lal::getMinEigen scalar steps, in the same dd operations).
 __global__ __launch_bounds__(256) void k_lanczos_tail(idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd nbeta,
-                                                      dd *out) {
+                                                      dd *out, dd *sc) {
     extern __shared__ dd part[];
     __shared__ dd s_alpha;
+    if (sc) nbeta = dd_neg(sc[2]);
     for (int t = threadIdx.x; t < T; t += 256) {
         idx_t st, cnt;
         chunk_range(n, T, t, st, cnt);
@@ -2062,11 +2080,20 @@ __global__ __launch_bounds__(256) void k_lanczos_tail(idx_t n, int T, const dd *
         dd acc = mk(0.0);
         for (int t = 0; t < T; t++) acc = dd_add_eq(acc, part[t]);
         out[1] = acc;
+        if (sc) {
+            const dd beta = dd_sqrt(acc);
+            sc[2] = beta;
+            sc[3] = dd_div(mk(1.0), beta);
+        }
     }
 }
 void lanczos_tail(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd nbeta, dd *out) {
     StatScope st("dot", s, 6.0 * (double)n);
-    k_lanczos_tail<<<1, 256, sizeof(dd) * T, s>>>(n, T, q, qold, w, r, nbeta, out);
+    k_lanczos_tail<<<1, 256, sizeof(dd) * T, s>>>(n, T, q, qold, w, r, nbeta, out, nullptr);
+}
+void lanczos_tail_dev(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd *sc, dd *out) {
+    StatScope st("dot", s, 6.0 * (double)n);
+    k_lanczos_tail<<<1, 256, sizeof(dd) * T, s>>>(n, T, q, qold, w, r, mk(0.0), out, sc);
 }
 
 // Max of |x_i|: per-thread running max, tree merge, last workgroup merges.

@@ -125,6 +125,11 @@ void gather_diag(hipStream_t s, idx_t n, const dd *A, dd *out);
 // alpha = q.w, r = w - alpha q - beta qold, rr = r.r with out = {alpha, rr}.
 void lanczos_q(hipStream_t s, idx_t n, dd v, const dd *r, dd *q, dd *qold);
 void lanczos_tail(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd nbeta, dd *out);
+// The same steps with the scalars on the device (sc[2] = beta, sc[3] =
+// 1/beta of the previous step; the tail leaves the next step's pair there),
+// so that several steps run without the host.
+void lanczos_q_dev(hipStream_t s, idx_t n, const dd *sc, const dd *r, dd *q, dd *qold);
+void lanczos_tail_dev(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd *sc, dd *out);
 // Batched GEMM over pointer arrays (device arrays of batch pointers):
 // C[b] := A[b]*B[b] (alpha = 1, beta = 0), order as gemm().
 void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx_t lda, const dd *const *B, idx_t ldb,

@@ -185,14 +185,19 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
     k_exp_clear<<<cdiv(m, 256), 256, 0, s>>>(m, w.ea);
     k_rowexp<<<dim3(cdiv(m, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea);
     k_slice<S><<<dim3(cdiv(mpad, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea, w.sa, mpad, nkb);
-    k_exp_clear<<<cdiv(n, 256), 256, 0, s>>>(n, w.eb);
-    k_rowexp<<<dim3(cdiv(n, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
-    k_slice<S><<<dim3(cdiv(npad, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, w.sb, npad, nkb);
+    // a product of a matrix with its own transpose (L L^T, L^T L) has the
+    // same rows in both operands: one set of planes serves both
+    const bool same = (A == B && ars == brs && acs == bcs && m == n);
+    if (!same) {
+        k_exp_clear<<<cdiv(n, 256), 256, 0, s>>>(n, w.eb);
+        k_rowexp<<<dim3(cdiv(n, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
+        k_slice<S><<<dim3(cdiv(npad, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, w.sb, npad, nkb);
+    }
     // timed as family "ozaki" with the product's algorithmic 2mnk dd-flops
     // (the MFMA work is S(S+1)/2 times that in int8 multiply-adds)
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
-    k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, w.sb, npad, w.eb, C, ldc,
-                                                                  alpha, beta, lower ? 1 : 0);
+    k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, same ? w.sa : w.sb, npad,
+                                                                  same ? w.ea : w.eb, C, ldc, alpha, beta, lower ? 1 : 0);
 }
 
 
@@ -303,12 +308,16 @@ static void launch_f64(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const 
     k_exp_clear<<<cdiv(m, 256), 256, 0, s>>>(m, w.ea);
     k_rowexp<<<dim3(cdiv(m, 64), nkb64), 256, 0, s>>>(m, k, A, ars, acs, w.ea);
     k_slice_f64<S><<<dim3(cdiv(mpad, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, sa, mpad, nkb);
-    k_exp_clear<<<cdiv(n, 256), 256, 0, s>>>(n, w.eb);
-    k_rowexp<<<dim3(cdiv(n, 64), nkb64), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
-    k_slice_f64<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, sb, npad, nkb);
+    const bool same = (A == B && ars == brs && acs == bcs && m == n);
+    if (!same) {
+        k_exp_clear<<<cdiv(n, 256), 256, 0, s>>>(n, w.eb);
+        k_rowexp<<<dim3(cdiv(n, 64), nkb64), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
+        k_slice_f64<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, sb, npad, nkb);
+    }
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
-    k_gemm_f64<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, sa, mpad, w.ea, sb, npad, w.eb, C,
-                                                                      ldc, alpha, beta, lower ? 1 : 0);
+    k_gemm_f64<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, sa, mpad, w.ea, same ? sa : sb, npad,
+                                                                      same ? w.ea : w.eb, C, ldc, alpha, beta,
+                                                                      lower ? 1 : 0);
 }
 
 int gemm(hipStream_t s, Work &w, char transa, char transb, idx_t m, idx_t n, idx_t k, int alpha, const dd *A, idx_t lda,

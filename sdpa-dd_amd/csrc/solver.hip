@@ -120,7 +120,10 @@ struct Chain {
     const dd *L = nullptr, *M = nullptr;
     dd *r = nullptr, *q = nullptr, *qold = nullptr, *w = nullptr, *tmp = nullptr;
     dd *scal = nullptr;        // device: [alpha, r.r]
-    dd *dot_work = nullptr;    // partial sums of this chain's dots
+    dd *dot_work = nullptr;    // device scalars (beta, 1/beta at [2], [3]) and the step history
+    dd *hist = nullptr;        // device: {alpha, r.r} per step
+    dd *hh = nullptr;          // pinned host copy of hist
+    int pending = 0;           // steps enqueued and not yet read
     unsigned *counter = nullptr;
     hipStream_t s = nullptr;
     dd *h = nullptr;           // pinned host copy of scal
@@ -132,6 +135,8 @@ struct Chain {
     std::vector<ddr> diagVec, diagVec2, out, bv, work;
     ddr result;
 };
+
+static constexpr int HIST_CAP = 2040;  // steps of one run (at most sqrt(n) + 11)
 
 struct ChainPool {
     std::vector<Chain> chains;
@@ -161,6 +166,9 @@ struct ChainPool {
             (void)hipMemset(c.counter, 0, sizeof(unsigned) * 2);
             if (hipHostMalloc((void **)&c.h, sizeof(dd) * 2) != hipSuccess) return false;
             hmem.push_back(c.h);
+            c.hist = c.dot_work + 8;
+            if (hipHostMalloc((void **)&c.hh, sizeof(dd) * 2 * HIST_CAP) != hipSuccess) return false;
+            hmem.push_back(c.hh);
             hipStream_t s;
             if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return false;
             streams.push_back(s);
@@ -181,28 +189,49 @@ bool chain_active(Chain &c) {
             abs(c.error * c.beta) > (1.0e-2) * abs(c.min) + (1.0e-4));
 }
 
-void chain_enqueue(Chain &c, int T) {
+// Enqueue steps k .. k + cnt - 1: each forms q from the device scalars, the
+// three mat-vecs, then alpha, r and r.r into hist[k + j] and the next
+// scalars (Lal::getMinEigen's loop body, sdpa_linear.cpp:64-82).
+void chain_enqueue(Chain &c, int T, int cnt) {
     hipStream_t s = c.s;
     const int nDim = c.n;
-    c.value = ddr(1.0) / c.beta;
-    sdk::lanczos_q(s, nDim, c.value.v, c.r, c.q, c.qold);   // qold = q; q = r / beta
-    if (c.fast) {
-        sdk::trmv_lower_fast(s, 'T', nDim, c.L, nDim, c.q, c.w);
-        sdk::gemv_n_fast(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
-        sdk::trmv_lower_fast(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
-    } else {
-        sdk::trmv_lower(s, 'T', nDim, c.L, nDim, c.q, c.w);
-        sdk::gemv_n(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
-        sdk::trmv_lower(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
+    dd *sc = c.dot_work;
+    for (int j = 0; j < cnt; j++) {
+        sdk::lanczos_q_dev(s, nDim, sc, c.r, c.q, c.qold);  // qold = q; q = r / beta
+        if (c.fast) {
+            sdk::trmv_lower_fast(s, 'T', nDim, c.L, nDim, c.q, c.w);
+            sdk::gemv_n_fast(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
+            sdk::trmv_lower_fast(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
+        } else {
+            sdk::trmv_lower(s, 'T', nDim, c.L, nDim, c.q, c.w);
+            sdk::gemv_n(s, nDim, nDim, mk(1.0), c.M, nDim, c.w, mk(0.0), c.tmp);
+            sdk::trmv_lower(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
+        }
+        // alpha = q.w; r = w - alpha q - beta qold; r.r; next beta, 1/beta
+        sdk::lanczos_tail_dev(s, nDim, T, c.q, c.qold, c.w, c.r, sc, c.hist + 2 * (size_t)(c.k + j));
     }
-    // alpha = q.w; r = w - alpha q - beta qold; r.r
-    sdk::lanczos_tail(s, nDim, T, c.q, c.qold, c.w, c.r, dd_neg(c.beta.v), c.scal);
-    (void)hipMemcpyAsync(c.h, c.scal, sizeof(dd) * 2, hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(c.hh + 2 * (size_t)c.k, c.hist + 2 * (size_t)c.k, sizeof(dd) * 2 * cnt, hipMemcpyDeviceToHost,
+                         s);
+    c.pending = cnt;
 }
 
-void chain_finish_step(Chain &c) {
+// Steps from c.k on up to and including the next one that runs the
+// tridiagonal eigen solve (whose outcome may end the chain).
+int chain_batch(const Chain &c) {
+    const int nDim = c.n;
+    int k = c.k, kk = c.kk, cnt = 0;
+    while (k < nDim && k < sqrt((ddr)nDim) + 10 && k < HIST_CAP) {
+        cnt++;
+        if (kk >= sqrt((ddr)k) || k == nDim - 1 || k > sqrt((ddr)nDim + 9)) break;
+        k++;
+        kk++;
+    }
+    return cnt;
+}
+
+void chain_finish_step(Chain &c, const dd *rec) {
     const int nDim = c.n, k = c.k;
-    c.alpha = ddr(c.h[0]);
+    c.alpha = ddr(rec[0]);
     c.diagVec[k] = c.alpha;
     if (c.kk >= sqrt((ddr)k) || k == nDim - 1 || k > sqrt((ddr)nDim + 9)) {
         c.kk = 0;
@@ -221,7 +250,7 @@ void chain_finish_step(Chain &c) {
         c.min = c.out[0];
         c.error = z[k];
     }
-    c.value = ddr(c.h[1]);
+    c.value = ddr(rec[1]);
     c.beta = sqrt(c.value);
     c.diagVec2[k] = c.beta;
     ++c.k;
@@ -251,6 +280,11 @@ void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) 
         sdk::fill(c.s, nDim, mk(0.0), c.q);
         sdk::fill(c.s, nDim, mk(1.0), c.r);
         c.beta = sqrt((ddr)nDim);
+        // device scalars of the first step: beta and 1/beta
+        c.h[0] = c.beta.v;
+        c.h[1] = (ddr(1.0) / c.beta).v;
+        (void)hipMemcpyAsync(c.dot_work + 2, c.h, sizeof(dd) * 2, hipMemcpyHostToDevice, c.s);
+        c.pending = 0;
     }
     for (;;) {
         int launched = 0;
@@ -261,15 +295,28 @@ void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) 
                 c.done = true;
                 continue;
             }
-            chain_enqueue(c, T);
+            chain_enqueue(c, T, chain_batch(c));
             launched++;
         }
         if (!launched) break;
         for (int i = 0; i < cnt; i++) {
             Chain &c = pool.chains[i];
-            if (c.done) continue;
+            if (c.done || c.pending == 0) continue;
             (void)hipStreamSynchronize(c.s);
-            chain_finish_step(c);
+            const int first = c.k, np = c.pending;
+            c.pending = 0;
+            for (int j = 0; j < np; j++) {
+                // within a batch the convergence test cannot change (min and
+                // error move only at the eigen solve that closes the batch);
+                // beta can end the chain early, and the later steps the
+                // device already ran are then dropped
+                if (j > 0 && !chain_active(c)) {
+                    c.done = true;
+                    break;
+                }
+                chain_finish_step(c, c.hh + 2 * (size_t)(first + j));
+                if (c.done) break;
+            }
         }
     }
     for (int i = 0; i < cnt; i++) {
