@@ -92,6 +92,29 @@ long orc_Rsteqr_I(long n, double *d, double *e, double *z, long ldz) {
     Rsteqr('I', n, D(d), D(e), D(z), ldz, work.data(), info);
     return (long)info;
 }
+// Eigenvalues (ascending) of the symmetric matrix whose lower triangle is A.
+long orc_Rsyev_NL(long n, double *A, long lda, double *w) {
+    mint info = 0;
+    Rsyev_NL(n, D(A), lda, D(w), info);
+    return (long)info;
+}
+long orc_Rsterf(long n, double *d, double *e) {
+    mint info = 0;
+    Rsterf(n, D(d), D(e), info);
+    return (long)info;
+}
+long orc_qr_min_eigen(long n, const double *L, const double *M, double *out) {
+    DenseMatrix l, mm;
+    l.nRow = l.nCol = (int)n;
+    mm.nRow = mm.nCol = (int)n;
+    l.de_ele.assign(D(L), D(L) + n * n);
+    mm.de_ele.assign(D(M), D(M) + n * n);
+    mint info = 0;
+    dd_real r = qr_min_eigen(l, mm, info);
+    out[0] = r.x[0];
+    out[1] = r.x[1];
+    return (long)info;
+}
 void orc_lanczos(long n, const double *L, const double *M, double *out) {
     DenseMatrix l, mm;
     l.nRow = l.nCol = (int)n;
@@ -115,6 +138,7 @@ int orc_load(const char *path, int *m, int *nblock, int *bs, int bs_cap, char *e
     }
     *m = g_data.m;
     *nblock = (int)g_data.blockStruct.size();
+    if (*nblock < bs_cap) bs[*nblock] = -g_data.LP_nBlock;  // LP entry count after the SDP sizes
     for (int i = 0; i < *nblock && i < bs_cap; i++) bs[i] = g_data.blockStruct[i];
     return 0;
 }
@@ -129,14 +153,18 @@ void orc_schur_forced(const double *X, const double *invZ, double *B, int forced
 void orc_schur(const double *X, const double *invZ, double *B) { orc_schur_forced(X, invZ, B, -1); }
 void orc_schur_forced(const double *X, const double *invZ, double *B, int forced) {
     DenseLinearSpace x, z;
-    x.initialize(g_data.blockStruct);
-    z.initialize(g_data.blockStruct);
+    x.initialize(g_data.blockStruct, g_data.LP_nBlock);
+    z.initialize(g_data.blockStruct, g_data.LP_nBlock);
     size_t off = 0;
     for (size_t l = 0; l < g_data.blockStruct.size(); l++) {
         size_t len = x.SDP_block[l].de_ele.size();
         std::memcpy(x.SDP_block[l].de_ele.data(), X + 2 * off, len * 16);
         std::memcpy(z.SDP_block[l].de_ele.data(), invZ + 2 * off, len * 16);
         off += len;
+    }
+    if (g_data.LP_nBlock > 0) {
+        std::memcpy(x.LP_block.data(), X + 2 * off, (size_t)g_data.LP_nBlock * 16);
+        std::memcpy(z.LP_block.data(), invZ + 2 * off, (size_t)g_data.LP_nBlock * 16);
     }
     std::vector<dd_real> b((size_t)g_data.m * g_data.m, dd_real(0.0));
     std::vector<int> f = compute_formula(g_data, dd_real(1.2));

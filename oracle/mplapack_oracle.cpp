@@ -792,4 +792,377 @@ sorting:
 #undef E
 }
 
+// ---- symmetric eigenvalues (mplapack/Rsyev.cpp with jobz 'N', uplo 'L') ----
+// Rlascl's multiplier sequence (mplapack/Rlascl.cpp:100-185): apply(mul) is
+// called once per pass.
+template <class F>
+static void lascl_passes(dd_real cfrom, dd_real cto, F apply) {
+    const dd_real zero = 0.0, one = 1.0;
+    dd_real smlnum = lamchS();
+    dd_real bignum = one / smlnum;
+    dd_real cfromc = cfrom, ctoc = cto, mul;
+    bool done = false;
+    while (!done) {
+        dd_real cfrom1 = cfromc * smlnum;
+        if (cfrom1 == cfromc) {
+            mul = ctoc / cfromc;
+            done = true;
+        } else {
+            dd_real cto1 = ctoc / bignum;
+            if (cto1 == ctoc) {
+                mul = ctoc;
+                done = true;
+                cfromc = one;
+            } else if (abs(cfrom1) > abs(ctoc) && ctoc != zero) {
+                mul = smlnum;
+                done = false;
+                cfromc = cfrom1;
+            } else if (abs(cto1) > abs(cfromc)) {
+                mul = bignum;
+                done = false;
+                ctoc = cto1;
+            } else {
+                mul = ctoc / cfromc;
+                done = true;
+            }
+        }
+        apply(mul);
+    }
+}
+
+// mplapack/Rnrm2.cpp:32-60 (scaled sum of squares, unit stride)
+static dd_real Rnrm2(mint n, const dd_real *x) {
+    const dd_real zero = 0.0, one = 1.0;
+    if (n < 1) return zero;
+    if (n == 1) return abs(x[0]);
+    dd_real scale = zero, ssq = one;
+    for (mint i = 0; i < n; i++) {
+        if (x[i] != zero) {
+            dd_real absxi = abs(x[i]);
+            if (scale < absxi) {
+                ssq = one + ssq * pow2(scale / absxi);
+                scale = absxi;
+            } else {
+                ssq += pow2(absxi / scale);
+            }
+        }
+    }
+    return scale * sqrt(ssq);
+}
+
+// mplapack/Rlarfg.cpp:32-80 (unit stride): Householder reflector for (alpha, x)
+static void Rlarfg(mint n, dd_real &alpha, dd_real *x, dd_real &tau) {
+    const dd_real zero = 0.0, one = 1.0;
+    if (n <= 1) {
+        tau = zero;
+        return;
+    }
+    dd_real xnorm = Rnrm2(n - 1, x);
+    if (xnorm == zero) {
+        tau = zero;
+        return;
+    }
+    dd_real beta = -sign(Rlapy2(alpha, xnorm), alpha);
+    dd_real safmin = lamchS() / lamchE();
+    mint knt = 0;
+    if (abs(beta) < safmin) {
+        dd_real rsafmn = one / safmin;
+        do {
+            knt++;
+            Rscal(n - 1, rsafmn, x, 1);
+            beta = beta * rsafmn;
+            alpha = alpha * rsafmn;
+        } while (abs(beta) < safmin && knt < 20);
+        xnorm = Rnrm2(n - 1, x);
+        beta = -sign(Rlapy2(alpha, xnorm), alpha);
+    }
+    tau = (beta - alpha) / beta;
+    Rscal(n - 1, one / (alpha - beta), x, 1);
+    for (mint j = 1; j <= knt; j++) beta = beta * safmin;
+    alpha = beta;
+}
+
+// mplapack/Rsymv.cpp, uplo 'L', beta 0, unit strides: y = alpha A x
+static void Rsymv_L(mint n, dd_real alpha, const dd_real *a, mint lda, const dd_real *x, dd_real *y) {
+    const dd_real zero = 0.0;
+    for (mint i = 0; i < n; i++) y[i] = zero;
+    if (alpha == zero) return;
+    for (mint j = 0; j < n; j++) {
+        dd_real temp1 = alpha * x[j], temp2 = zero;
+        y[j] += temp1 * a[j + j * lda];
+        for (mint i = j + 1; i < n; i++) {
+            y[i] += temp1 * a[i + j * lda];
+            temp2 += a[i + j * lda] * x[i];
+        }
+        y[j] += alpha * temp2;
+    }
+}
+
+// mplapack/Rsyr2.cpp, uplo 'L', unit strides: A += alpha (x y^T + y x^T)
+static void Rsyr2_L(mint n, dd_real alpha, const dd_real *x, const dd_real *y, dd_real *a, mint lda) {
+    const dd_real zero = 0.0;
+    if (n == 0 || alpha == zero) return;
+    for (mint j = 0; j < n; j++) {
+        if (x[j] != zero || y[j] != zero) {
+            dd_real temp1 = alpha * y[j], temp2 = alpha * x[j];
+            for (mint i = j; i < n; i++) a[i + j * lda] += x[i] * temp1 + y[i] * temp2;
+        }
+    }
+}
+
+// mplapack/Rsytd2.cpp, uplo 'L' (Rsytrd runs it unblocked for n <= 32)
+static void Rsytd2_L(mint n, dd_real *a, mint lda, dd_real *d, dd_real *e, dd_real *tau) {
+    const dd_real zero = 0.0, one = 1.0, half = 1.0 / 2.0;
+    if (n <= 0) return;
+    for (mint i = 1; i <= n - 1; i++) {
+        dd_real taui;
+        dd_real *col = &a[i + (i - 1) * lda];  // A(i+1, i)
+        Rlarfg(n - i, col[0], &a[std::min(i + 2, n) - 1 + (i - 1) * lda], taui);
+        e[i - 1] = col[0];
+        if (taui != zero) {
+            col[0] = one;
+            dd_real *sub = &a[i + i * lda];  // A(i+1, i+1)
+            Rsymv_L(n - i, taui, sub, lda, col, &tau[i - 1]);
+            dd_real alpha = -half * taui * Rdot(n - i, &tau[i - 1], 1, col, 1);
+            Raxpy(n - i, alpha, col, 1, &tau[i - 1], 1);
+            Rsyr2_L(n - i, -one, col, &tau[i - 1], sub, lda);
+            col[0] = e[i - 1];
+        }
+        d[i - 1] = a[(i - 1) + (i - 1) * lda];
+        tau[i - 1] = taui;
+    }
+    d[n - 1] = a[(n - 1) + (n - 1) * lda];
+}
+
+// mplapack/Rsterf.cpp: eigenvalues of a symmetric tridiagonal by the
+// Pal-Walker-Kahan root-free QL/QR.  Indices are the reference's 1-based
+// ones; d(i) is d[i-1].  The final ascending sort (Rlasrt 'I') is done with
+// std::sort: it only permutes values.
+void Rsterf(mint n, dd_real *dv, dd_real *ev, mint &info) {
+#define D(i) dv[(i) - 1]
+#define E(i) ev[(i) - 1]
+    const dd_real zero = 0.0, one = 1.0, two = 2.0, three = 3.0;
+    const mint maxit = 30;
+    info = 0;
+    if (n <= 1) return;
+    dd_real eps = lamchE();
+    dd_real eps2 = pow2(eps);
+    dd_real safmin = lamchS();
+    dd_real safmax = one / safmin;
+    dd_real ssfmax = sqrt(safmax) / three;
+    dd_real ssfmin = sqrt(safmin) / eps2;
+    mint nmaxit = n * maxit;
+    dd_real sigma = zero;
+    mint jtot = 0;
+    mint l1 = 1;
+    mint m = 0, l = 0, lsv = 0, lend = 0, lendsv = 0, iscale = 0;
+    dd_real anorm, p, rte, rt1, rt2, r, c, s, gamma, bb, oldc, oldgam, alpha;
+    for (;;) {
+        // statement 10
+        if (l1 > n) {
+            std::sort(dv, dv + n);
+            return;
+        }
+        if (l1 > 1) E(l1 - 1) = zero;
+        for (m = l1; m <= n - 1; m++) {
+            if (abs(E(m)) <= (sqrt(abs(D(m))) * sqrt(abs(D(m + 1)))) * eps) {
+                E(m) = zero;
+                goto found30;
+            }
+        }
+        m = n;
+    found30:
+        l = l1;
+        lsv = l;
+        lend = m;
+        lendsv = lend;
+        l1 = m + 1;
+        if (lend == l) continue;
+        anorm = Rlanst_M(lend - l + 1, &D(l), &E(l));
+        iscale = 0;
+        if (anorm == zero) continue;
+        if (anorm > ssfmax) {
+            iscale = 1;
+            Rlascl_G(anorm, ssfmax, lend - l + 1, &D(l));
+            Rlascl_G(anorm, ssfmax, lend - l, &E(l));
+        } else if (anorm < ssfmin) {
+            iscale = 2;
+            Rlascl_G(anorm, ssfmin, lend - l + 1, &D(l));
+            Rlascl_G(anorm, ssfmin, lend - l, &E(l));
+        }
+        for (mint i = l; i <= lend - 1; i++) E(i) = pow2(E(i));
+        if (abs(D(lend)) < abs(D(l))) {
+            lend = lsv;
+            l = lendsv;
+        }
+        if (lend >= l) {
+            // QL iteration
+            for (;;) {
+                if (l != lend) {
+                    for (m = l; m <= lend - 1; m++)
+                        if (abs(E(m)) <= eps2 * abs(D(m) * D(m + 1))) goto found70;
+                }
+                m = lend;
+            found70:
+                if (m < lend) E(m) = zero;
+                p = D(l);
+                if (m == l) {
+                    D(l) = p;
+                    l++;
+                    if (l <= lend) continue;
+                    break;
+                }
+                if (m == l + 1) {
+                    rte = sqrt(E(l));
+                    Rlae2(D(l), rte, D(l + 1), rt1, rt2);
+                    D(l) = rt1;
+                    D(l + 1) = rt2;
+                    E(l) = zero;
+                    l += 2;
+                    if (l <= lend) continue;
+                    break;
+                }
+                if (jtot == nmaxit) break;
+                jtot++;
+                rte = sqrt(E(l));
+                sigma = (D(l + 1) - p) / (two * rte);
+                r = Rlapy2(sigma, one);
+                sigma = p - (rte / (sigma + sign(r, sigma)));
+                c = one;
+                s = zero;
+                gamma = D(m) - sigma;
+                p = gamma * gamma;
+                for (mint i = m - 1; i >= l; i--) {
+                    bb = E(i);
+                    r = p + bb;
+                    if (i != m - 1) E(i + 1) = s * r;
+                    oldc = c;
+                    c = p / r;
+                    s = bb / r;
+                    oldgam = gamma;
+                    alpha = D(i);
+                    gamma = c * (alpha - sigma) - s * oldgam;
+                    D(i + 1) = oldgam + (alpha - gamma);
+                    if (c != zero)
+                        p = (gamma * gamma) / c;
+                    else
+                        p = oldc * bb;
+                }
+                E(l) = s * p;
+                D(l) = sigma + gamma;
+            }
+        } else {
+            // QR iteration
+            for (;;) {
+                for (m = l; m >= lend + 1; m--)
+                    if (abs(E(m - 1)) <= eps2 * abs(D(m) * D(m - 1))) goto found120;
+                m = lend;
+            found120:
+                if (m > lend) E(m - 1) = zero;
+                p = D(l);
+                if (m == l) {
+                    D(l) = p;
+                    l--;
+                    if (l >= lend) continue;
+                    break;
+                }
+                if (m == l - 1) {
+                    rte = sqrt(E(l - 1));
+                    Rlae2(D(l), rte, D(l - 1), rt1, rt2);
+                    D(l) = rt1;
+                    D(l - 1) = rt2;
+                    E(l - 1) = zero;
+                    l -= 2;
+                    if (l >= lend) continue;
+                    break;
+                }
+                if (jtot == nmaxit) break;
+                jtot++;
+                rte = sqrt(E(l - 1));
+                sigma = (D(l - 1) - p) / (two * rte);
+                r = Rlapy2(sigma, one);
+                sigma = p - (rte / (sigma + sign(r, sigma)));
+                c = one;
+                s = zero;
+                gamma = D(m) - sigma;
+                p = gamma * gamma;
+                for (mint i = m; i <= l - 1; i++) {
+                    bb = E(i);
+                    r = p + bb;
+                    if (i != m) E(i - 1) = s * r;
+                    oldc = c;
+                    c = p / r;
+                    s = bb / r;
+                    oldgam = gamma;
+                    alpha = D(i + 1);
+                    gamma = c * (alpha - sigma) - s * oldgam;
+                    D(i) = oldgam + (alpha - gamma);
+                    if (c != zero)
+                        p = (gamma * gamma) / c;
+                    else
+                        p = oldc * bb;
+                }
+                E(l - 1) = s * p;
+                D(l) = sigma + gamma;
+            }
+        }
+        // statement 150
+        if (iscale == 1) Rlascl_G(ssfmax, anorm, lendsv - lsv + 1, &D(lsv));
+        if (iscale == 2) Rlascl_G(ssfmin, anorm, lendsv - lsv + 1, &D(lsv));
+        if (jtot >= nmaxit) {
+            for (mint i = 1; i <= n - 1; i++)
+                if (E(i) != zero) info++;
+            return;
+        }
+    }
+#undef D
+#undef E
+}
+
+// mplapack/Rsyev.cpp with jobz 'N', uplo 'L': eigenvalues of the symmetric
+// matrix whose lower triangle is in a (overwritten) into w, ascending.
+void Rsyev_NL(mint n, dd_real *a, mint lda, dd_real *w, mint &info) {
+    const dd_real zero = 0.0, one = 1.0;
+    info = 0;
+    if (n == 0) return;
+    if (n == 1) {
+        w[0] = a[0];
+        return;
+    }
+    dd_real safmin = lamchS();
+    dd_real eps = lamchE();
+    dd_real smlnum = safmin / eps;
+    dd_real bignum = one / smlnum;
+    dd_real rmin = sqrt(smlnum);
+    dd_real rmax = sqrt(bignum);
+    // Rlansy 'M' on the lower triangle
+    dd_real anrm = zero;
+    for (mint j = 0; j < n; j++)
+        for (mint i = j; i < n; i++) {
+            dd_real sum = abs(a[i + j * lda]);
+            if (anrm < sum || isnan_dd(sum)) anrm = sum;
+        }
+    bool iscale = false;
+    dd_real sigma = zero;
+    if (anrm > zero && anrm < rmin) {
+        iscale = true;
+        sigma = rmin / anrm;
+    } else if (anrm > rmax) {
+        iscale = true;
+        sigma = rmax / anrm;
+    }
+    if (iscale)  // Rlascl type 'L' from one to sigma
+        lascl_passes(one, sigma, [&](const dd_real &mul) {
+            for (mint j = 0; j < n; j++)
+                for (mint i = j; i < n; i++) a[i + j * lda] = a[i + j * lda] * mul;
+        });
+    std::vector<dd_real> e(n), tau(n);
+    Rsytd2_L(n, a, lda, w, e.data(), tau.data());
+    Rsterf(n, w, e.data(), info);
+    if (iscale) {
+        mint imax = (info == 0) ? n : info - 1;
+        Rscal(imax, one / sigma, w, 1);
+    }
+}
+
 }  // namespace orc

@@ -43,5 +43,10 @@ void Rsteqr(char compz, mint n, dd_real *d, dd_real *e, dd_real *z, mint ldz, dd
 void Rlaev2(dd_real a, dd_real b, dd_real c, dd_real &rt1, dd_real &rt2, dd_real &cs1, dd_real &sn1);
 void Rlae2(dd_real a, dd_real b, dd_real c, dd_real &rt1, dd_real &rt2);
 dd_real Rlapy2(dd_real x, dd_real y);
+// Root-free QL/QR eigenvalues of a symmetric tridiagonal, sorted ascending.
+void Rsterf(mint n, dd_real *d, dd_real *e, mint &info);
+// Eigenvalues (ascending) of a symmetric matrix from its lower triangle
+// (Rsyev with jobz 'N', uplo 'L'; a is overwritten).
+void Rsyev_NL(mint n, dd_real *a, mint lda, dd_real *w, mint &info);
 void Rlartg(dd_real f, dd_real g, dd_real &cs, dd_real &sn, dd_real &r);
 }  // namespace orc

@@ -115,23 +115,33 @@ bool read_dat_s(const std::string &path, InputData &data, std::string &err) {
         err = "bad nBLOCK";
         return false;
     }
-    std::vector<int> bs(nBlock);
+    // Block types (sdpa_main.cpp:111-130): size >= 2 is an SDP block, a
+    // negative size -k or the size 1 adds k (or 1) LP entries.
+    std::vector<int> fbs(nBlock), btype(nBlock), bnum(nBlock), bs;
+    int nLP = 0;
     for (int l = 0; l < nBlock; l++) {
-        if (!sc.sep_int(bs[l])) {
+        if (!sc.sep_int(fbs[l])) {
             err = "bad blockStruct";
             return false;
         }
-        if (bs[l] < 2) {
-            err = "LP blocks are not supported";
-            return false;
-        }
-        if (bs[l] <= 20) {
-            err = "SDP blocks of size <= 20 are not supported (the reference uses its QR eigen path for them)";
+        if (fbs[l] >= 2) {
+            btype[l] = 1;
+            bnum[l] = (int)bs.size();
+            bs.push_back(fbs[l]);
+        } else if (fbs[l] < 0 || fbs[l] == 1) {
+            btype[l] = 3;
+            bnum[l] = nLP;
+            fbs[l] = fbs[l] < 0 ? -fbs[l] : 1;
+            nLP += fbs[l];
+        } else {
+            err = "bad blockStruct";
             return false;
         }
     }
+    const int nSDP = (int)bs.size();
     data.m = m;
     data.blockStruct = bs;
+    data.LP_nBlock = nLP;
     data.b.assign(m, dd_real(0.0));
     for (int k = 0; k < m; k++) {
         std::string t;
@@ -165,11 +175,33 @@ bool read_dat_s(const std::string &path, InputData &data, std::string &err) {
         ents.push_back(e);
     }
     // Blocks present per matrix, sorted (setBlockStruct), then elements in
-    // file order (setElement).
-    std::vector<std::vector<int>> cnt(m + 1, std::vector<int>(nBlock, 0));
-    for (auto &e : ents) cnt[e.k][e.l - 1]++;
+    // file order (setElement).  LP entries must be diagonal; their indices
+    // are sorted and deduplicated (sdpa_io.cpp:509-517, 607-617) and a
+    // repeated entry keeps the last value (setElement_LP, sdpa_struct.cpp:1253-1265).
+    std::vector<std::vector<int>> cnt(m + 1, std::vector<int>(nSDP, 0));
+    std::vector<std::vector<int>> lpidx(m + 1);
+    for (auto &e : ents) {
+        if (btype[e.l - 1] == 1) {
+            cnt[e.k][bnum[e.l - 1]]++;
+        } else {
+            if (e.i != e.j) {
+                err = "LP entry off the diagonal";
+                return false;
+            }
+            if (e.i < 1 || e.i > fbs[e.l - 1]) {
+                err = "out of range in input data";
+                return false;
+            }
+            lpidx[e.k].push_back(bnum[e.l - 1] + e.i - 1);
+        }
+    }
     auto init_space = [&](SparseLinearSpace &sp, int k) {
-        for (int l = 0; l < nBlock; l++) {
+        std::vector<int> &li = lpidx[k];
+        std::sort(li.begin(), li.end());
+        li.erase(std::unique(li.begin(), li.end()), li.end());
+        sp.LP_sp_index = li;
+        sp.LP_sp_block.assign(li.size(), dd_real(0.0));
+        for (int l = 0; l < nSDP; l++) {
             if (cnt[k][l] == 0) continue;
             sp.SDP_sp_index.push_back(l);
             SparseMatrix sm;
@@ -187,7 +219,13 @@ bool read_dat_s(const std::string &path, InputData &data, std::string &err) {
     for (int k = 0; k < m; k++) init_space(data.A[k], k + 1);
     for (auto &e : ents) {
         SparseLinearSpace &sp = (e.k == 0) ? data.C : data.A[e.k - 1];
-        int l2 = e.l - 1, kk = 0;
+        if (btype[e.l - 1] == 3) {
+            int idx = bnum[e.l - 1] + e.i - 1, kk = 0;
+            while (sp.LP_sp_index[kk] != idx) kk++;
+            sp.LP_sp_block[kk] = (e.k == 0) ? -e.v : e.v;
+            continue;
+        }
+        int l2 = bnum[e.l - 1], kk = 0;
         while (sp.SDP_sp_index[kk] != l2) kk++;
         SparseMatrix &sm = sp.SDP_sp_block[kk];
         if (e.i - 1 >= sm.nRow || e.j - 1 >= sm.nCol || e.i < 1 || e.j < 1) {
@@ -210,6 +248,7 @@ bool read_dat_s(const std::string &path, InputData &data, std::string &err) {
 }
 
 // sdpa_dataset.cpp:254-330
+// and 395-465 (the LP entries' constraint lists)
 void InputData::initialize_index() {
     int nB = (int)blockStruct.size();
     SDP_constraint.assign(nB, {});
@@ -219,6 +258,14 @@ void InputData::initialize_index() {
             int l = A[i].SDP_sp_index[k];
             SDP_constraint[l].push_back(i);
             SDP_blockIndex[l].push_back((int)k);
+        }
+    LP_constraint.assign(LP_nBlock, {});
+    LP_blockIndex.assign(LP_nBlock, {});
+    for (int i = 0; i < m; i++)
+        for (size_t k = 0; k < A[i].LP_sp_index.size(); k++) {
+            int l = A[i].LP_sp_index[k];
+            LP_constraint[l].push_back(i);
+            LP_blockIndex[l].push_back((int)k);
         }
 }
 
@@ -268,12 +315,14 @@ static dd_real inner(const DenseLinearSpace &a, const DenseLinearSpace &b) {
         const DenseMatrix &x = a.SDP_block[l];
         ret += Rdot((mint)x.nRow * x.nCol, x.de_ele.data(), 1, b.SDP_block[l].de_ele.data(), 1);
     }
+    for (size_t l = 0; l < a.LP_block.size(); l++) ret += a.LP_block[l] * b.LP_block[l];
     return ret;
 }
 // sdpa_linear.cpp:1616-1650 (SparseLinearSpace . DenseLinearSpace).
 static dd_real inner(const SparseLinearSpace &a, const DenseLinearSpace &b) {
     dd_real ret = 0.0;
     for (size_t l = 0; l < a.SDP_sp_index.size(); l++) ret += inner_sparse_dense(a.SDP_sp_block[l], b.SDP_block[a.SDP_sp_index[l]]);
+    for (size_t l = 0; l < a.LP_sp_index.size(); l++) ret += a.LP_sp_block[l] * b.LP_block[a.LP_sp_index[l]];
     return ret;
 }
 static dd_real inner(const std::vector<dd_real> &a, const std::vector<dd_real> &b) {
@@ -286,6 +335,7 @@ static void scale(DenseLinearSpace &ret, const DenseLinearSpace &a, const dd_rea
         if (&ret != &a) Rcopy(len, a.SDP_block[l].de_ele.data(), 1, ret.SDP_block[l].de_ele.data(), 1);
         Rscal(len, s, ret.SDP_block[l].de_ele.data(), 1);
     }
+    for (size_t l = 0; l < a.LP_block.size(); l++) ret.LP_block[l] = a.LP_block[l] * s;
 }
 // ret = a + s*b  (Lal::plus dense, sdpa_linear.cpp:1108-1129, 1694-1737)
 static void plus(DenseLinearSpace &ret, const DenseLinearSpace &a, const DenseLinearSpace &b, const dd_real &s) {
@@ -294,6 +344,7 @@ static void plus(DenseLinearSpace &ret, const DenseLinearSpace &a, const DenseLi
         if (&ret != &a) Rcopy(len, a.SDP_block[l].de_ele.data(), 1, ret.SDP_block[l].de_ele.data(), 1);
         Raxpy(len, s, b.SDP_block[l].de_ele.data(), 1, ret.SDP_block[l].de_ele.data(), 1);
     }
+    for (size_t l = 0; l < a.LP_block.size(); l++) ret.LP_block[l] = a.LP_block[l] + b.LP_block[l] * s;
 }
 // ret += s*A for sparse A (Lal::plus(Dense, Dense, Sparse), sdpa_linear.cpp:1224-1317, 1777-1812);
 // ret aliases a on every call site on the path.
@@ -316,6 +367,10 @@ static void plus_sparse(DenseLinearSpace &ret, const SparseLinearSpace &A, const
             Raxpy((mint)R.de_ele.size(), s, B.de_ele.data(), 1, R.de_ele.data(), 1);
         }
     }
+    for (size_t l = 0; l < A.LP_sp_index.size(); l++) {
+        dd_real &r = ret.LP_block[A.LP_sp_index[l]];
+        r = r + A.LP_sp_block[l] * s;
+    }
 }
 static void gemm_nn(DenseMatrix &C, const DenseMatrix &A, const DenseMatrix &B) {
     Rgemm('N', 'N', C.nRow, C.nCol, A.nCol, MONE, A.de_ele.data(), A.nRow, B.de_ele.data(), B.nRow, 0.0, C.de_ele.data(),
@@ -326,6 +381,10 @@ static void triple(DenseLinearSpace &ret, const DenseLinearSpace &a, const Dense
                    const DenseLinearSpace &c, DenseLinearSpace &work) {
     for (size_t l = 0; l < a.SDP_block.size(); l++) gemm_nn(work.SDP_block[l], a.SDP_block[l], b.SDP_block[l]);
     for (size_t l = 0; l < a.SDP_block.size(); l++) gemm_nn(ret.SDP_block[l], work.SDP_block[l], c.SDP_block[l]);
+    for (size_t l = 0; l < a.LP_block.size(); l++) {
+        work.LP_block[l] = a.LP_block[l] * b.LP_block[l];
+        ret.LP_block[l] = work.LP_block[l] * c.LP_block[l];
+    }
 }
 // sdpa_linear.cpp:832-909: ret = A * B for sparse A via row axpys.
 static void mul_sparse_dense(DenseMatrix &ret, const SparseMatrix &A, const DenseMatrix &B) {
@@ -424,6 +483,18 @@ dd_real lanczos_min_eigen(const DenseMatrix &L, const DenseMatrix &M) {
         ++kk;
     }
     return min - abs(error * beta);
+}
+
+// sdpa_jordan.cpp:62-66 and sdpa_linear.cpp:124-156: for a block of size
+// <= 20, lambda_min(L M L^T) from all eigenvalues (Rsyev).
+dd_real qr_min_eigen(const DenseMatrix &L, const DenseMatrix &M, mint &info) {
+    const int n = M.nRow;
+    std::vector<dd_real> t((size_t)n * n), a((size_t)n * n), w(n);
+    // Lal::let(DLS2, '=', M, 'T', L) = M L^T, then Lal::let(DLS1, '=', L, '*', DLS2)
+    Rgemm('N', 'T', n, n, n, MONE, M.de_ele.data(), n, L.de_ele.data(), n, 0.0, t.data(), n);
+    Rgemm('N', 'N', n, n, n, MONE, L.de_ele.data(), n, t.data(), n, 0.0, a.data(), n);
+    Rsyev_NL(n, a.data(), n, w.data(), info);
+    return w[0];
 }
 
 // --------------------------------------------------------------- Schur --
@@ -582,6 +653,27 @@ void schur_dense(const InputData &data, const DenseLinearSpace &X, const DenseLi
             }
         }
     }
+    // compute_bMat_dense_LP (sdpa_newton.cpp:1162-1197), after the SDP blocks
+    for (int l = 0; l < (int)X.LP_block.size(); l++) {
+        const dd_real &x = X.LP_block[l], &iz = invZ.LP_block[l];
+        const auto &cons = data.LP_constraint[l];
+        const auto &bidx = data.LP_blockIndex[l];
+        for (size_t k1 = 0; k1 < cons.size(); k1++) {
+            int i = cons[k1];
+            const dd_real &Ai = data.A[i].LP_sp_block[bidx[k1]];
+            for (size_t k2 = k1; k2 < cons.size(); k2++) {
+                int j = cons[k2];
+                const dd_real &Aj = data.A[j].LP_sp_block[bidx[k2]];
+                dd_real value = x * iz * Ai * Aj;
+                if (i != j) {
+                    B[i + (size_t)m * j] += value;
+                    B[j + (size_t)m * i] += value;
+                } else {
+                    B[i + (size_t)m * i] += value;
+                }
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- solver --
@@ -629,6 +721,10 @@ dd_real max_norm(const DenseLinearSpace &d) {
             dd_real t = abs(x);
             if (t > ret) ret = t;
         }
+    for (auto &x : d.LP_block) {
+        dd_real t = abs(x);
+        if (t > ret) ret = t;
+    }
     return ret;
 }
 // sdpa_dataset.cpp:625-645
@@ -644,21 +740,20 @@ void residuals_compute(const InputData &d, const Solutions &pt, Residuals &res) 
     res.normDualMat = max_norm(res.dualMat);
 }
 // sdpa_dataset.cpp:96-118
+// Jal::getInvChol / getInvCholAndInv (sdpa_jordan.cpp:123-170): an LP entry
+// fails only when negative; its inverse factor is 1/sqrt(x), its inverse 1/z.
+static bool inv_chol_space(DenseLinearSpace &invChol, const DenseLinearSpace &a, DenseLinearSpace &work) {
+    for (size_t l = 0; l < a.SDP_block.size(); l++)
+        if (!inv_chol(invChol.SDP_block[l], a.SDP_block[l], work.SDP_block[l])) return false;
+    for (size_t l = 0; l < a.LP_block.size(); l++) {
+        if (a.LP_block[l] < 0.0) return false;
+        invChol.LP_block[l] = 1.0 / sqrt(a.LP_block[l]);
+    }
+    return true;
+}
 bool compute_inverse(Solutions &pt, Work &w) {
-    bool ok = true;
-    for (size_t l = 0; l < pt.xMat.SDP_block.size(); l++) {
-        if (!inv_chol(pt.invCholeskyX.SDP_block[l], pt.xMat.SDP_block[l], w.DLS1.SDP_block[l])) {
-            ok = false;
-            break;
-        }
-    }
-    bool okz = true;
-    for (size_t l = 0; l < pt.zMat.SDP_block.size(); l++) {
-        if (!inv_chol(pt.invCholeskyZ.SDP_block[l], pt.zMat.SDP_block[l], w.DLS2.SDP_block[l])) {
-            okz = false;
-            break;
-        }
-    }
+    bool ok = inv_chol_space(pt.invCholeskyX, pt.xMat, w.DLS1);
+    bool okz = inv_chol_space(pt.invCholeskyZ, pt.zMat, w.DLS2);
     if (okz) {
         for (size_t l = 0; l < pt.zMat.SDP_block.size(); l++) {
             DenseMatrix &inv = pt.invzMat.SDP_block[l];
@@ -666,14 +761,28 @@ bool compute_inverse(Solutions &pt, Work &w) {
             inv.de_ele = lc.de_ele;
             Rtrmm_LLTN(lc.nRow, lc.nCol, MONE, lc.de_ele.data(), lc.nRow, inv.de_ele.data(), inv.nRow);
         }
+        for (size_t l = 0; l < pt.zMat.LP_block.size(); l++) pt.invzMat.LP_block[l] = 1.0 / pt.zMat.LP_block[l];
     }
     return ok && okz;
 }
-// sdpa_jordan.cpp:55-88
-dd_real min_eigen(const DenseLinearSpace &L, const DenseLinearSpace &M) {
+// sdpa_jordan.cpp:55-88: Lanczos for SDP blocks of size > 20, Rsyev for
+// the others, x l l for LP entries.  A failed Rsyev ends the reference's
+// process (sdpa_linear.cpp:139-145); ok reports it here.
+dd_real min_eigen(const DenseLinearSpace &L, const DenseLinearSpace &M, bool &ok) {
     dd_real min = 1.0E50;
     for (size_t l = 0; l < M.SDP_block.size(); l++) {
-        dd_real v = lanczos_min_eigen(L.SDP_block[l], M.SDP_block[l]);
+        dd_real v;
+        if (M.SDP_block[l].nRow > 20) {
+            v = lanczos_min_eigen(L.SDP_block[l], M.SDP_block[l]);
+        } else {
+            mint info = 0;
+            v = qr_min_eigen(L.SDP_block[l], M.SDP_block[l], info);
+            if (info != 0) ok = false;
+        }
+        if (v < min) min = v;
+    }
+    for (size_t l = 0; l < M.LP_block.size(); l++) {
+        dd_real v = M.LP_block[l] * L.LP_block[l] * L.LP_block[l];
         if (v < min) min = v;
     }
     return min;
@@ -688,37 +797,36 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
     nw.gVec.assign(m, dd_real(0.0));
     nw.DyVec.assign(m, dd_real(0.0));
     nw.bMat.assign((size_t)m * m, dd_real(0.0));
-    nw.DxMat.initialize(bs);
-    nw.DzMat.initialize(bs);
-    nw.r_zinvMat.initialize(bs);
-    nw.x_rd_zinvMat.initialize(bs);
+    nw.DxMat.initialize(bs, data.LP_nBlock);
+    nw.DzMat.initialize(bs, data.LP_nBlock);
+    nw.r_zinvMat.initialize(bs, data.LP_nBlock);
+    nw.x_rd_zinvMat.initialize(bs, data.LP_nBlock);
     nw.useFormula = compute_formula(data, dd_real(1.2));
     Work work;
-    work.DLS1.initialize(bs);
-    work.DLS2.initialize(bs);
+    work.DLS1.initialize(bs, data.LP_nBlock);
+    work.DLS2.initialize(bs, data.LP_nBlock);
     // Solutions::initialize (sdpa_dataset.cpp:35-60).
     Solutions pt;
     pt.mDim = m;
     for (int n : bs) pt.nDim += n;
+    pt.nDim += data.LP_nBlock;
     dd_real lambda = param.lambdaStar;
-    pt.xMat.initialize(bs);
-    pt.zMat.initialize(bs);
-    pt.invCholeskyX.initialize(bs);
-    pt.invCholeskyZ.initialize(bs);
-    pt.invzMat.initialize(bs);
-    for (size_t l = 0; l < bs.size(); l++) {
-        pt.xMat.SDP_block[l].setIdentity(lambda);
-        pt.zMat.SDP_block[l].setIdentity(lambda);
-        pt.invCholeskyX.SDP_block[l].setIdentity(1.0 / sqrt(lambda));
-        pt.invCholeskyZ.SDP_block[l].setIdentity(1.0 / sqrt(lambda));
-        pt.invzMat.SDP_block[l].setIdentity(1.0 / lambda);
-    }
+    pt.xMat.initialize(bs, data.LP_nBlock);
+    pt.zMat.initialize(bs, data.LP_nBlock);
+    pt.invCholeskyX.initialize(bs, data.LP_nBlock);
+    pt.invCholeskyZ.initialize(bs, data.LP_nBlock);
+    pt.invzMat.initialize(bs, data.LP_nBlock);
+    pt.xMat.setIdentity(lambda);
+    pt.zMat.setIdentity(lambda);
+    pt.invCholeskyX.setIdentity(1.0 / sqrt(lambda));
+    pt.invCholeskyZ.setIdentity(1.0 / sqrt(lambda));
+    pt.invzMat.setIdentity(1.0 / lambda);
     pt.yVec.assign(m, dd_real(0.0));
     const int nDim = pt.nDim;
 
     Residuals initRes;
     initRes.primalVec.assign(m, dd_real(0.0));
-    initRes.dualMat.initialize(bs);
+    initRes.dualMat.initialize(bs, data.LP_nBlock);
     residuals_compute(data, pt, initRes);
     Residuals cur = initRes;
 
@@ -887,9 +995,14 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
         {
             const dd_real xi = 3.0;
             dd_real alphaBD = 100.0;
-            dd_real mx = min_eigen(pt.invCholeskyX, nw.DxMat);
+            bool eig_ok = true;
+            dd_real mx = min_eigen(pt.invCholeskyX, nw.DxMat, eig_ok);
             alphaP = (-mx > 1.0 / alphaBD) ? -1.0 / mx : alphaBD;
-            dd_real mz = min_eigen(pt.invCholeskyZ, nw.DzMat);
+            dd_real mz = min_eigen(pt.invCholeskyZ, nw.DzMat, eig_ok);
+            if (!eig_ok) {
+                res.message = "eigenvalue iteration did not converge";
+                break;
+            }
             alphaD = (-mz > 1.0 / alphaBD) ? -1.0 / mz : alphaBD;
             alphaP = param.gammaStar * alphaP;
             alphaD = param.gammaStar * alphaD;
@@ -979,6 +1092,10 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
                     xMatz0 += lam * pt.xMat.SDP_block[b].de_ele[(size_t)i * dim + i];
                 }
             }
+            for (int b = 0; b < data.LP_nBlock; b++) {
+                x0zMat += lam * pt.zMat.LP_block[b];
+                xMatz0 += lam * pt.xMat.LP_block[b];
+            }
             dd_real accuracy = param.epsilonDash;
             if (cur.normPrimalVec <= accuracy && xMatz0 < etaPrimal) etaPrimal = xMatz0;
             if (cur.normDualMat <= accuracy && x0zMat < etaDual) etaDual = x0zMat;
@@ -1023,6 +1140,8 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
         res.xMat.insert(res.xMat.end(), pt.xMat.SDP_block[l].de_ele.begin(), pt.xMat.SDP_block[l].de_ele.end());
         res.zMat.insert(res.zMat.end(), pt.zMat.SDP_block[l].de_ele.begin(), pt.zMat.SDP_block[l].de_ele.end());
     }
+    res.xMat.insert(res.xMat.end(), pt.xMat.LP_block.begin(), pt.xMat.LP_block.end());
+    res.zMat.insert(res.zMat.end(), pt.zMat.LP_block.begin(), pt.zMat.LP_block.end());
     res.yVec = pt.yVec;
     return true;
 }

@@ -2,10 +2,11 @@
 //
 // CPU restatement of SDPA-DD 7.1.3's dense-Schur SDP path: the .dat-s reader,
 // the parameter file, and the Mehrotra predictor-corrector loop with its HKM
-// Newton step, Schur assembly (F1/F2/F3), Cholesky solve, Lanczos step length
-// and phase logic.  Sparse-Schur ordering, LP/SOCP blocks and blocks of size
-// <= 20 (which use an eigenvalue routine instead of Lanczos) are not restated;
-// inputs needing them are rejected.
+// Newton step, Schur assembly (F1/F2/F3 for SDP blocks, the diagonal form
+// for LP entries), Cholesky solve, step length (Lanczos for SDP blocks of
+// size > 20, Rsyev's tridiagonal reduction and root-free QR for smaller
+// ones, ratios for LP entries) and phase logic.  Sparse-Schur ordering and
+// SOCP blocks are not restated; inputs needing them are rejected.
 #pragma once
 #include <string>
 #include <vector>
@@ -36,19 +37,30 @@ struct DenseMatrix {
         for (int i = 0; i < nRow; i++) de_ele[(size_t)i * nRow + i] = s;
     }
 };
+// sdpa_struct.h: SDP blocks plus the LP part (one scalar per LP entry; the
+// LP entries of a matrix are stored at ascending LP_sp_index).
 struct SparseLinearSpace {
     std::vector<int> SDP_sp_index;
     std::vector<SparseMatrix> SDP_sp_block;
+    std::vector<int> LP_sp_index;
+    std::vector<dd_real> LP_sp_block;
 };
 struct DenseLinearSpace {
     std::vector<DenseMatrix> SDP_block;
-    void initialize(const std::vector<int> &bs) {
+    std::vector<dd_real> LP_block;
+    void initialize(const std::vector<int> &bs, int nLP) {
         SDP_block.resize(bs.size());
         for (size_t l = 0; l < bs.size(); l++) SDP_block[l].initialize(bs[l]);
+        LP_block.assign(nLP, dd_real(0.0));
     }
     void setZero() {
         for (auto &b : SDP_block)
             for (auto &v : b.de_ele) v = 0.0;
+        for (auto &v : LP_block) v = 0.0;
+    }
+    void setIdentity(dd_real s) {
+        for (auto &b : SDP_block) b.setIdentity(s);
+        for (auto &v : LP_block) v = s;
     }
 };
 
@@ -62,10 +74,12 @@ struct Parameter {
 struct InputData {
     int m = 0;
     std::vector<int> blockStruct;  // SDP blocks only
+    int LP_nBlock = 0;             // LP entries (all LP blocks and size-1 blocks)
     std::vector<dd_real> b;
     SparseLinearSpace C;
     std::vector<SparseLinearSpace> A;
     std::vector<std::vector<int>> SDP_constraint, SDP_blockIndex;
+    std::vector<std::vector<int>> LP_constraint, LP_blockIndex;
     void initialize_index();
 };
 
@@ -85,7 +99,7 @@ struct SolveResult {
     int phase = noINFO;  // internal phase before the primal/dual swap
     int phase_reported = noINFO;  // after Phase::reverse (what the result file prints)
     dd_real mu, objValPrimal, objValDual, normPrimalVec, normDualMat, relgap;
-    std::vector<dd_real> xMat, zMat, yVec;  // concatenated blocks, column-major
+    std::vector<dd_real> xMat, zMat, yVec;  // concatenated SDP blocks (column-major), then the LP entries
     std::vector<IterRecord> trace;
     std::string message;
 };
@@ -98,5 +112,7 @@ void schur_dense(const InputData &data, const DenseLinearSpace &X, const DenseLi
                  std::vector<dd_real> &B, const std::vector<int> &useFormula);
 std::vector<int> compute_formula(const InputData &data, dd_real kappa);
 dd_real lanczos_min_eigen(const DenseMatrix &L, const DenseMatrix &M);
+// Smallest eigenvalue of L M L^T by Rsyev ('N', 'L'); info as Rsyev's.
+dd_real qr_min_eigen(const DenseMatrix &L, const DenseMatrix &M, mint &info);
 
 }  // namespace orc

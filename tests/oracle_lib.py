@@ -43,6 +43,12 @@ class Oracle:
         L.orc_Rsteqr_I.argtypes = [ctypes.c_long, c_dp, c_dp, c_dp, ctypes.c_long]
         L.orc_Rsteqr_I.restype = ctypes.c_long
         L.orc_lanczos.argtypes = [ctypes.c_long, c_dp, c_dp, c_dp]
+        L.orc_Rsyev_NL.argtypes = [ctypes.c_long, c_dp, ctypes.c_long, c_dp]
+        L.orc_Rsyev_NL.restype = ctypes.c_long
+        L.orc_Rsterf.argtypes = [ctypes.c_long, c_dp, c_dp]
+        L.orc_Rsterf.restype = ctypes.c_long
+        L.orc_qr_min_eigen.argtypes = [ctypes.c_long, c_dp, c_dp, c_dp]
+        L.orc_qr_min_eigen.restype = ctypes.c_long
         L.orc_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.orc_load.restype = ctypes.c_int
@@ -131,13 +137,35 @@ class Oracle:
         self.lib.orc_lanczos(n, _dp(np.ascontiguousarray(L)), _dp(np.ascontiguousarray(M)), _dp(out))
         return out
 
+    def rsyev(self, A):
+        """Eigenvalues (ascending) from the lower triangle of the n x n dd
+        matrix A (column-major, shape (n*n, 2)); returns (w, info)."""
+        a = np.array(A, dtype=np.float64)
+        n = int(round(np.sqrt(a.shape[0])))
+        w = np.zeros((n, 2))
+        info = self.lib.orc_Rsyev_NL(n, _dp(a), n, _dp(w))
+        return w, info
+
+    def rsterf(self, d, e):
+        d = np.array(d, dtype=np.float64)
+        e = np.array(e, dtype=np.float64)
+        info = self.lib.orc_Rsterf(d.shape[0], _dp(d), _dp(e))
+        return d, info
+
+    def qr_min_eigen(self, n, L, M):
+        out = np.zeros(2)
+        info = self.lib.orc_qr_min_eigen(n, _dp(np.ascontiguousarray(L)), _dp(np.ascontiguousarray(M)), _dp(out))
+        return out, info
+
     def load(self, path):
+        """(m, SDP block sizes); self.n_lp holds the LP entry count."""
         m, nb = ctypes.c_int(), ctypes.c_int()
-        bs = (ctypes.c_int * 64)()
+        bs = (ctypes.c_int * 4096)()
         err = ctypes.create_string_buffer(512)
-        rc = self.lib.orc_load(path.encode(), ctypes.byref(m), ctypes.byref(nb), bs, 64, err, 512)
+        rc = self.lib.orc_load(path.encode(), ctypes.byref(m), ctypes.byref(nb), bs, 4096, err, 512)
         if rc != 0:
             raise RuntimeError(err.value.decode())
+        self.n_lp = -bs[nb.value]
         return m.value, [bs[i] for i in range(nb.value)]
 
     def formula(self, m, nb):

@@ -230,3 +230,78 @@ def _dense_constraints(path, m, n):
         if i != j:
             A[k - 1][j - 1, i - 1] += v
     return A
+
+
+def _mp_sym(A, n):
+    T = mpmath.matrix(n, n)
+    for j in range(n):
+        for i in range(j, n):
+            T[i, j] = T[j, i] = mp(A[i + n * j])
+    return T
+
+
+@pytest.mark.parametrize("n,scale", [(2, 1.0), (5, 1.0), (13, 1.0), (20, 1.0), (9, 1e-140), (9, 1e140)])
+def test_rsyev_eigenvalues(oracle, n, scale):
+    """Rsyev ('N', 'L'): ascending eigenvalues to dd accuracy against
+    40-digit mpmath, including matrices scaled below rmin (~1e-130) and
+    above rmax, which take the Rlascl path; only the lower triangle is read."""
+    rng = np.random.default_rng(n)
+    A = rand_dd((n * n,), rng)
+    A[:, 0] *= scale
+    A[:, 1] *= scale
+    for j in range(n):
+        for i in range(j + 1, n):
+            A[j + n * i] = [7.0, 0.0]  # upper triangle must be ignored
+    ev = sorted(mpmath.eigsy(_mp_sym(A, n))[0])
+    w, info = oracle.rsyev(A)
+    assert info == 0
+    norm = max(abs(x) for x in ev)
+    for i in range(n):
+        assert abs(mp(w[i]) - ev[i]) < 1e-29 * n * norm, (i, w[i], ev[i])
+
+
+@pytest.mark.parametrize("scale", [1.0, 1e150, 1e-90])
+def test_rsterf_tridiagonal(oracle, scale):
+    """Rsterf on a tridiagonal with a zero off-diagonal (two unreduced
+    blocks), plain and scaled beyond ssfmax (~1e145) and below ssfmin
+    (~6e-84): eigenvalues against mpmath."""
+    rng = np.random.default_rng(11)
+    n = 14
+    d = rand_dd((n,), rng) * scale
+    e = rand_dd((n,), rng) * scale
+    e[6] = 0.0
+    T = mpmath.matrix(n, n)
+    for i in range(n):
+        T[i, i] = mp(d[i])
+        if i + 1 < n:
+            T[i, i + 1] = T[i + 1, i] = mp(e[i])
+    ev = sorted(mpmath.eigsy(T)[0])
+    w, info = oracle.rsterf(d, e[: n - 1].copy())
+    assert info == 0
+    norm = max(abs(x) for x in ev)
+    for i in range(n):
+        assert abs(mp(w[i]) - ev[i]) < 1e-29 * n * norm
+
+
+def test_qr_min_eigen_is_lambda_min_of_congruence(oracle):
+    """The small-block step length's eigenvalue: lambda_min(L M L^T) for a
+    lower-triangular L and symmetric M, against mpmath."""
+    rng = np.random.default_rng(13)
+    n = 11
+    L = rand_dd((n * n,), rng)
+    for j in range(n):
+        for i in range(j):
+            L[i + n * j] = 0.0
+    M = rand_dd((n * n,), rng)
+    for j in range(n):
+        for i in range(j):
+            M[i + n * j] = M[j + n * i]
+    Lm = mpmath.matrix(n, n)
+    for j in range(n):
+        for i in range(n):
+            Lm[i, j] = mp(L[i + n * j])
+    S = Lm * _mp_sym(M, n) * Lm.T
+    lmin = min(mpmath.eigsy(S)[0])
+    got, info = oracle.qr_min_eigen(n, L, M)
+    assert info == 0
+    assert abs(mp(got) - lmin) < 1e-28 * mpmath.norm(S)

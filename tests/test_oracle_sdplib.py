@@ -38,10 +38,25 @@ def fmt_trace(rec):
             "%4.1e" % rec["alphaP"][0], "%4.2e" % rec["beta"][0]]
 
 
-@pytest.mark.parametrize("name", ["theta1", "mcp100", "theta2", "mcp124-1", "mcp124-3"])
-def test_oracle_reproduces_reference_results(oracle, name):
+# (problem, dot partition, relative tolerance of the final mu).  truss* and
+# control1 have only SDP blocks of size <= 20 (Rsyev step length) and a
+# size-1 block (an LP entry); arch0 has a 161 block (Lanczos) and 174 LP
+# entries.  arch0's last trace line (a step length at mu ~ 1e-23)
+# reproduces with the 16-way dot partition, not the serial one; every
+# earlier line reproduces with both.  The final mu (~1e-21, after the last
+# step) moves by up to 7e-5 (truss3) and 1.3e-6 (control1) relative between
+# dot partitions 1..64 of the oracle, so those are the tolerances for them;
+# arch0's final mu (1.6e-23) follows its last step length, which moves by a
+# unit in the second printed digit between partitions, so it is held to 5%.
+EXACT = [("theta1", 1, 1e-8), ("mcp100", 1, 1e-8), ("theta2", 1, 1e-8), ("mcp124-1", 1, 1e-8),
+         ("mcp124-3", 1, 1e-8), ("truss1", 1, 1e-8), ("truss3", 1, 1e-4), ("truss4", 1, 1e-4),
+         ("control1", 1, 2e-6), ("arch0", 16, 5e-2)]
+
+
+@pytest.mark.parametrize("name,threads,mu_tol", EXACT)
+def test_oracle_reproduces_reference_results(oracle, name, threads, mu_tol):
     trace, fin = parse_result(name)
-    r = oracle.solve(golden_sdplib(name), SDPLIB_PARAM, dot_threads=1)
+    r = oracle.solve(golden_sdplib(name), SDPLIB_PARAM, dot_threads=threads)
     assert r["iterations"] == int(fin["Iteration"])
     assert r["phase"] == fin["phase"]
     # the result file has one more trace line (printLastInfo repeats the final state)
@@ -55,7 +70,7 @@ def test_oracle_reproduces_reference_results(oracle, name):
     assert "%.16e" % -r["objValPrimal"][0] == fin["objValDual"]
     # mu is a converged quantity ~1e-21: agreement to ~8-12 digits is what the
     # reference's own thread-order variation allows
-    assert abs(r["mu"][0] - float(fin["mu"])) <= 1e-8 * abs(float(fin["mu"]))
+    assert abs(r["mu"][0] - float(fin["mu"])) <= mu_tol * abs(float(fin["mu"]))
 
 
 def test_oracle_chaotic_problems_depend_on_dot_order(oracle):
@@ -64,3 +79,34 @@ def test_oracle_chaotic_problems_depend_on_dot_order(oracle):
     assert its == {26, 28}
     _, fin = parse_result("qap5")
     assert int(fin["Iteration"]) in its
+
+
+def test_oracle_control2_up_to_the_stopping_edge(oracle):
+    """control2 stops in 51-53 iterations depending on the dot partition (its
+    last gap sits at the 1e-20 stopping test); every shared trace line but
+    the last two and the final objectives reproduce the reference's."""
+    trace, fin = parse_result("control2")
+    r = oracle.solve(golden_sdplib("control2"), SDPLIB_PARAM, dot_threads=1)
+    assert r["phase"] == fin["phase"] == "pdOPT"
+    assert abs(r["iterations"] - int(fin["Iteration"])) <= 2
+    n = min(len(r["trace"]), len(trace) - 1) - 2
+    for rec, ref in zip(r["trace"][:n], trace[:n]):
+        got = fmt_trace(rec)
+        ref = [x.replace("-0.00e+00", "+0.00e+00") for x in ref]
+        assert got[:2] + got[4:] == ref[:2] + ref[4:], (rec["it"], got, ref)
+    assert "%.16e" % -r["objValDual"][0] == fin["objValPrimal"]
+    assert "%.16e" % -r["objValPrimal"][0] == fin["objValDual"]
+
+
+def test_oracle_hinf1_is_chaotic_but_close(oracle):
+    """hinf1 (ill-posed, reference ends pFEAS) depends on the dot partition
+    like qap5: the trace up to iteration 28 and the objectives to 5 digits."""
+    trace, fin = parse_result("hinf1")
+    for t in (1, 8):
+        r = oracle.solve(golden_sdplib("hinf1"), SDPLIB_PARAM, dot_threads=t)
+        assert r["phase"] == "pFEAS"
+        for rec, ref in zip(r["trace"][:29], trace[:29]):
+            got = fmt_trace(rec)
+            ref = [x.replace("-0.00e+00", "+0.00e+00") for x in ref]
+            assert got[:2] + got[4:] == ref[:2] + ref[4:], (t, rec["it"], got, ref)
+        assert abs(-r["objValDual"][0] - float(fin["objValPrimal"])) < 1e-4
