@@ -135,6 +135,12 @@ int sdpadd_Rsteqr(sdpadd_handle h, char compz, int64_t n, double *d, double *e, 
  * Lal::getMinEigen computes it (sdpa_linear.cpp:27-122).  d_L, d_M device
  * n x n (lda = n); result to host dd *out. */
 int sdpadd_lanczos_min_eig(sdpadd_handle h, int64_t n, const double *d_L, const double *d_M, double *out);
+/* lambda_min(L M L^T) for a block of size n <= 20 as the reference takes it
+ * there: M L^T and L (M L^T) by Rgemm, then all eigenvalues by Rsyev ('N',
+ * 'L') (Jal::getMinEigen, sdpa_jordan.cpp:62-66; Lal::getMinEigenValue,
+ * sdpa_linear.cpp:124-156).  d_L, d_M device n x n; result to host dd *out;
+ * *info as Rsyev's (> 0: Rsterf did not converge). */
+int sdpadd_small_min_eig(sdpadd_handle h, int64_t n, const double *d_L, const double *d_M, double *out, int *info);
 /* Inverse Cholesky pieces of Solutions::computeInverse (sdpa_dataset.cpp:96-118,
  * sdpa_jordan.cpp:123-170): d_invChol = chol(A)^{-1}; if d_inv != NULL also
  * d_inv = L^{-T} L^{-1}.  d_work: n*n dd.  Returns > 0 (failing pivot) if A is
@@ -143,12 +149,17 @@ int sdpadd_inv_chol(sdpadd_handle h, int64_t n, const double *d_A, double *d_wor
 
 /* ---- Problems ------------------------------------------------------------- */
 /* Reads an SDPA sparse-format file (.dat-s) the way IO::read does
- * (sdpa_io.cpp:124-150, 429-786): C = -F0, A_k = F_k, b = c. */
+ * (sdpa_io.cpp:124-150, 289-426, 429-786): C = -F0, A_k = F_k, b = c.  Block
+ * sizes as sdpa_main.cpp:111-130 reads them: >= 2 an SDP block, -k (or 1) a
+ * diagonal block of k (or 1) LP entries, which must be given on the
+ * diagonal. */
 int sdpadd_problem_load(sdpadd_handle h, const char *path, sdpadd_problem *p);
-/* Builds a problem in SDPA's internal form: block sizes (all > 20, SDP),
- * then entries of matrix k (0 = C, 1..m = A_k) block l (0-based), given as
- * 0-based (row, col) pairs with dd values in file order
- * (SparseLinearSpace::setElement_SDP, sdpa_struct.cpp:1217-1249). */
+/* Builds a problem in SDPA's internal form: the block structure as in a
+ * data file (>= 2 SDP, -k or 1 LP), then entries of matrix k (0 = C, 1..m =
+ * A_k) block l (0-based, in that structure), given as 0-based (row, col)
+ * pairs with dd values in file order (SparseLinearSpace::setElement_SDP and
+ * setElement_LP, sdpa_struct.cpp:1217-1265; LP entries need row == col, a
+ * repeated LP entry keeps its last value). */
 int sdpadd_problem_create(sdpadd_handle h, int m, int nblock, const int *block_struct, sdpadd_problem *p);
 int sdpadd_problem_set_entries(sdpadd_problem p, int k, int l, int64_t nnz, const int *rows, const int *cols,
                                const double *vals);
@@ -156,10 +167,12 @@ int sdpadd_problem_set_b(sdpadd_problem p, const double *b);
 /* Dense conversion (>= 20% stored entries), formula table, device image. */
 int sdpadd_problem_finalize(sdpadd_problem p);
 int sdpadd_problem_destroy(sdpadd_problem p);
-/* m, number of blocks, block sizes (up to cap). */
+/* m, number of blocks, block structure as given (up to cap; LP blocks
+ * negative).  Iterates hold the SDP blocks (each n x n, in order) followed by
+ * the LP entries. */
 int sdpadd_problem_dims(sdpadd_problem p, int *m, int *nblock, int *block_struct, int cap);
 /* useFormula table of Newton::computeFormula_SDP (sdpa_newton.cpp:694-819):
- * out[i*nblock + l] in {0: F1, 1: F2, 2: F3}. */
+ * out[i*nsdp + l] in {0: F1, 1: F2, 2: F3}, l over the SDP blocks only. */
 int sdpadd_problem_formula(sdpadd_problem p, int *out);
 /* Schur complement of Newton::compute_bMat_dense_SDP (sdpa_newton.cpp:979-1068)
  * for device X and Zinv (blocks concatenated, each n_l x n_l): d_B (m x m)
@@ -196,7 +209,8 @@ int sdpadd_solve(sdpadd_problem p, const sdpadd_params *prm, sdpadd_result *res)
  * (it, then dd pairs mu, thetaP, thetaD, objP, objD, alphaP, alphaD, beta).
  * Returns the number of records (copies at most cap). */
 int sdpadd_trace(sdpadd_problem p, double *out, int cap);
-/* Final iterate of the last solve (host arrays: X and Z nn dd, y m dd). */
+/* Final iterate of the last solve (host arrays: X and Z nn + nlp dd -- the
+ * SDP blocks, then the LP entries -- and y m dd). */
 int sdpadd_solution(sdpadd_problem p, double *X, double *y, double *Z);
 
 /* ---- Adjoint -------------------------------------------------------------- */

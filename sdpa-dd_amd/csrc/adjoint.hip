@@ -123,6 +123,10 @@ int Adjoint::setup(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const
     m = hp.m;
     nB = (int)hp.bs.size();
     nn = hp.nn;
+    if (hp.nlp > 0) {
+        err = "adjoint: problems with LP entries are not supported";
+        return -1;
+    }
     if ((idx_t)X.size() != nn || (idx_t)Z.size() != nn || (int)y.size() != m) {
         err = "adjoint: no solution to differentiate (solve first)";
         return -1;

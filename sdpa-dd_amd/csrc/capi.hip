@@ -7,6 +7,7 @@
 #include "../../include/sdpadd.h"
 #include "adjoint.hpp"
 #include "context.hpp"
+#include "smallblk.hpp"
 #include "solver.hpp"
 #include "steqr.hpp"
 
@@ -270,6 +271,32 @@ int sdpadd_lanczos_min_eig(sdpadd_handle h, int64_t n, const double *d_L, const 
     out[1] = r.lo;
     return check(h);
 }
+int sdpadd_small_min_eig(sdpadd_handle h, int64_t n, const double *d_L, const double *d_M, double *out, int *info) {
+    if (!h || n < 1 || n > sdk::SMALL_EIG_MAXN || !out || !info) return bad(h, "small_min_eig: bad arguments");
+    hipStream_t s = h->c.stream;
+    sdk::SmallEigTask tk{DP(d_L), DP(d_M), (int)n};
+    sdk::SmallEigTask *d_tk = nullptr;
+    dd *d_out = nullptr;
+    int *d_info = nullptr;
+    if (hipMalloc((void **)&d_tk, sizeof(tk)) != hipSuccess || hipMalloc((void **)&d_out, sizeof(dd)) != hipSuccess ||
+        hipMalloc((void **)&d_info, sizeof(int)) != hipSuccess) {
+        (void)hipFree(d_tk);
+        (void)hipFree(d_out);
+        return bad(h, "small_min_eig: allocation failed");
+    }
+    (void)hipMemcpyAsync(d_tk, &tk, sizeof(tk), hipMemcpyHostToDevice, s);
+    sdk::small_min_eig(s, d_tk, 1, h->c.dot_T, d_out, d_info);
+    dd r{0.0, 0.0};
+    (void)hipMemcpyAsync(&r, d_out, sizeof(dd), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(info, d_info, sizeof(int), hipMemcpyDeviceToHost, s);
+    int rc = check(h);
+    (void)hipFree(d_tk);
+    (void)hipFree(d_out);
+    (void)hipFree(d_info);
+    out[0] = r.hi;
+    out[1] = r.lo;
+    return rc;
+}
 int sdpadd_inv_chol(sdpadd_handle h, int64_t n, const double *d_A, double *d_work, double *d_invChol, double *d_inv) {
     if (!h || n < 1) return bad(h, "inv_chol: bad dimension");
     hipStream_t s = h->c.stream;
@@ -308,12 +335,13 @@ int sdpadd_problem_load(sdpadd_handle h, const char *path, sdpadd_problem *p) {
 }
 int sdpadd_problem_create(sdpadd_handle h, int m, int nblock, const int *bs, sdpadd_problem *p) {
     if (!h || m < 1 || nblock < 1 || !bs || !p) return bad(h, "problem_create: bad arguments");
-    for (int l = 0; l < nblock; l++)
-        if (bs[l] <= 20) return bad(h, "problem_create: SDP blocks must be larger than 20");
     auto *pr = new sdpadd_problem_s();
     pr->h = h;
     pr->hp.m = m;
-    pr->hp.bs.assign(bs, bs + nblock);
+    if (!pr->hp.set_structure(std::vector<int>(bs, bs + nblock))) {
+        delete pr;
+        return bad(h, "problem_create: a block size is 0");
+    }
     pr->hp.b.assign(m, mk(0.0));
     pr->hp.A.assign(m, sdp::Space());
     *p = pr;
@@ -321,9 +349,19 @@ int sdpadd_problem_create(sdpadd_handle h, int m, int nblock, const int *bs, sdp
 }
 int sdpadd_problem_set_entries(sdpadd_problem p, int k, int l, int64_t nnz, const int *rows, const int *cols,
                                const double *vals) {
-    if (!p || p->finalized || k < 0 || k > p->hp.m || l < 0 || l >= (int)p->hp.bs.size() || nnz < 0)
+    if (!p || p->finalized || k < 0 || k > p->hp.m || l < 0 || l >= (int)p->hp.given.size() || nnz < 0)
         return bad(p ? p->h : nullptr, "set_entries: bad arguments");
     sdp::Space &sp = k ? p->hp.A[k - 1] : p->hp.C;
+    if (p->hp.gtype[l] == 3) {  // LP block: diagonal entries (row == col) of a diagonal block
+        const int len = std::abs(p->hp.given[l]);
+        for (int64_t q = 0; q < nnz; q++) {
+            if (rows[q] != cols[q] || rows[q] < 0 || rows[q] >= len)
+                return bad(p->h, "set_entries: LP entries must be diagonal and in range");
+            sp.lp_raw.push_back({p->hp.gnum[l] + rows[q], D(vals + 2 * q)});
+        }
+        return 0;
+    }
+    l = p->hp.gnum[l];
     int slot = sp.find(l);
     if (slot < 0) {
         // keep the block list sorted by block number
@@ -356,6 +394,8 @@ int sdpadd_problem_finalize(sdpadd_problem p) {
     for (auto &b : p->hp.C.blk) b.change_to_dense();
     for (auto &a : p->hp.A)
         for (auto &b : a.blk) b.change_to_dense();
+    p->hp.C.finish_lp();
+    for (auto &a : p->hp.A) a.finish_lp();
     p->hp.layout();
     p->hp.index();
     p->hp.formula = sdp::compute_formula(p->hp, ddr(1.2));
@@ -377,8 +417,8 @@ int sdpadd_problem_destroy(sdpadd_problem p) {
 int sdpadd_problem_dims(sdpadd_problem p, int *m, int *nblock, int *bs, int cap) {
     if (!p) return -1;
     *m = p->hp.m;
-    *nblock = (int)p->hp.bs.size();
-    for (int l = 0; l < *nblock && l < cap; l++) bs[l] = p->hp.bs[l];
+    *nblock = (int)p->hp.given.size();
+    for (int l = 0; l < *nblock && l < cap; l++) bs[l] = p->hp.given[l];
     return 0;
 }
 int sdpadd_problem_formula(sdpadd_problem p, int *out) {

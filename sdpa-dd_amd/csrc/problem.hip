@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -154,23 +156,19 @@ bool read_dat_s(const std::string &path, HostProblem &P, std::string &err) {
         err = "bad nBLOCK";
         return false;
     }
-    std::vector<int> bs(nb);
+    std::vector<int> given(nb);
     for (int l = 0; l < nb; l++) {
-        if (!c.sep() || !c.integer(bs[l])) {
+        if (!c.sep() || !c.integer(given[l])) {
             err = "bad bLOCKsTRUCT";
-            return false;
-        }
-        if (bs[l] < 2) {
-            err = "LP blocks are not supported by this path";
-            return false;
-        }
-        if (bs[l] <= 20) {
-            err = "SDP blocks of size <= 20 (eigenvalue path) are not supported by this path";
             return false;
         }
     }
     P.m = m;
-    P.bs = bs;
+    if (!P.set_structure(given)) {
+        err = "bad bLOCKsTRUCT";
+        return false;
+    }
+    const std::vector<int> &bs = P.bs;
     P.b.assign(m, mk(0.0));
     for (int k = 0; k < m; k++) {
         std::string t;
@@ -196,17 +194,24 @@ bool read_dat_s(const std::string &path, HostProblem &P, std::string &err) {
         double chk;
         if (std::sscanf(t.c_str(), "%lf", &chk) <= 0) break;
         e.v = value_of(t);
-        if (e.k < 0 || e.k > m || e.l < 1 || e.l > nb || e.i < 1 || e.j < 1 || e.i > bs[e.l - 1] || e.j > bs[e.l - 1]) {
+        if (e.k < 0 || e.k > m || e.l < 1 || e.l > nb || e.i < 1 || e.j < 1 || e.i > std::abs(given[e.l - 1]) ||
+            e.j > std::abs(given[e.l - 1])) {
             err = "entry out of range in input data";
+            return false;
+        }
+        if (P.gtype[e.l - 1] == 3 && e.i != e.j) {  // sdpa_io.cpp:342-345
+            err = "LP entry off the diagonal";
             return false;
         }
         ent.push_back(e);
     }
-    std::vector<int> cnt((size_t)(m + 1) * nb, 0);
-    for (auto &e : ent) cnt[(size_t)e.k * nb + e.l - 1]++;
+    const int nsdp = (int)bs.size();
+    std::vector<int> cnt((size_t)(m + 1) * nsdp, 0);
+    for (auto &e : ent)
+        if (P.gtype[e.l - 1] == 1) cnt[(size_t)e.k * nsdp + P.gnum[e.l - 1]]++;
     auto init = [&](Space &sp, int k) {
-        for (int l = 0; l < nb; l++) {
-            int c0 = cnt[(size_t)k * nb + l];
+        for (int l = 0; l < nsdp; l++) {
+            int c0 = cnt[(size_t)k * nsdp + l];
             if (!c0) continue;
             sp.index.push_back(l);
             Block b;
@@ -222,7 +227,11 @@ bool read_dat_s(const std::string &path, HostProblem &P, std::string &err) {
     for (int k = 0; k < m; k++) init(P.A[k], k + 1);
     for (auto &e : ent) {
         Space &sp = e.k ? P.A[e.k - 1] : P.C;
-        Block &b = sp.blk[sp.find(e.l - 1)];
+        if (P.gtype[e.l - 1] == 3) {
+            sp.lp_raw.push_back({P.gnum[e.l - 1] + e.i - 1, e.k ? e.v : dd_neg(e.v)});
+            continue;
+        }
+        Block &b = sp.blk[sp.find(P.gnum[e.l - 1])];
         b.row.push_back(e.i - 1);
         b.col.push_back(e.j - 1);
         b.val.push_back(e.k ? e.v : dd_neg(e.v));  // C = -F0 (sdpa_io.cpp:692)
@@ -232,9 +241,48 @@ bool read_dat_s(const std::string &path, HostProblem &P, std::string &err) {
     for (auto &b : P.C.blk) b.change_to_dense();
     for (auto &a : P.A)
         for (auto &b : a.blk) b.change_to_dense();
+    P.C.finish_lp();
+    for (auto &a : P.A) a.finish_lp();
     P.layout();
     P.index();
     P.formula = compute_formula(P, ddr(1.2));
+    return true;
+}
+
+// sdpa_io.cpp:607-617 (indices sorted, deduplicated) and
+// SparseLinearSpace::setElement_LP (sdpa_struct.cpp:1253-1265: a repeated
+// index keeps the value set last).
+void Space::finish_lp() {
+    std::map<int, dd> v;
+    for (auto &e : lp_raw) v[e.first] = e.second;
+    lp_index.clear();
+    lp_val.clear();
+    for (auto &e : v) {
+        lp_index.push_back(e.first);
+        lp_val.push_back(e.second);
+    }
+    lp_raw.clear();
+}
+
+bool HostProblem::set_structure(const std::vector<int> &g) {
+    given = g;
+    gtype.assign(g.size(), 0);
+    gnum.assign(g.size(), 0);
+    bs.clear();
+    nlp = 0;
+    for (size_t l = 0; l < g.size(); l++) {
+        if (g[l] >= 2) {
+            gtype[l] = 1;
+            gnum[l] = (int)bs.size();
+            bs.push_back(g[l]);
+        } else if (g[l] < 0 || g[l] == 1) {
+            gtype[l] = 3;
+            gnum[l] = nlp;
+            nlp += g[l] < 0 ? -g[l] : 1;
+        } else {
+            return false;
+        }
+    }
     return true;
 }
 
@@ -253,6 +301,14 @@ void HostProblem::index() {
         for (size_t k = 0; k < A[i].index.size(); k++) {
             cons[A[i].index[k]].push_back(i);
             cidx[A[i].index[k]].push_back((int)k);
+        }
+    // InputData::initialize_index_LP (sdpa_dataset.cpp:395-465)
+    lp_cons.assign(nlp, {});
+    lp_cidx.assign(nlp, {});
+    for (int i = 0; i < m; i++)
+        for (size_t k = 0; k < A[i].lp_index.size(); k++) {
+            lp_cons[A[i].lp_index[k]].push_back(i);
+            lp_cidx[A[i].lp_index[k]].push_back((int)k);
         }
 }
 
@@ -341,6 +397,7 @@ int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::
     dp.bs = P.bs;
     dp.boff = P.boff;
     dp.nn = P.nn;
+    dp.nlp = P.nlp;
     dp.d_b = dev_copy(dp, P.b, s);
     // Shared sparse-entry and dense-value pools.
     std::vector<int> sp_row, sp_col;
@@ -412,10 +469,74 @@ int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::
     dp.d_sp_val = dev_copy(dp, sp_val, s);
     dp.d_dense = dev_copy(dp, dense, s);
 
-    // Element update lists for sum_i y_i A_i (multi_plusToA order: i
-    // ascending, entries in file order, off-diagonals to both positions).
+    // LP entries of each A_i and of C for the inner products.
     {
-        std::vector<std::vector<std::pair<int, dd>>> el(P.nn);
+        std::vector<idx_t> lptr(m + 1, 0), cptr2(2, 0);
+        std::vector<int> lidx, clidx;
+        std::vector<dd> lval, clval;
+        for (int i = 0; i < m; i++) {
+            lptr[i] = (idx_t)lidx.size();
+            lidx.insert(lidx.end(), P.A[i].lp_index.begin(), P.A[i].lp_index.end());
+            lval.insert(lval.end(), P.A[i].lp_val.begin(), P.A[i].lp_val.end());
+        }
+        lptr[m] = (idx_t)lidx.size();
+        clidx = P.C.lp_index;
+        clval = P.C.lp_val;
+        cptr2[1] = (idx_t)clidx.size();
+        dp.d_lp_ptr = dev_copy(dp, lptr, s);
+        dp.d_lp_idx = dev_copy(dp, lidx, s);
+        dp.d_lp_val = dev_copy(dp, lval, s);
+        dp.d_clp_ptr = dev_copy(dp, cptr2, s);
+        dp.d_clp_idx = dev_copy(dp, clidx, s);
+        dp.d_clp_val = dev_copy(dp, clval, s);
+    }
+    // LP part of the Schur matrix (Newton::compute_bMat_dense_LP,
+    // sdpa_newton.cpp:1162-1197): LP entry l adds x_l z_l^-1 a_il a_jl to B_ij
+    // for every pair k1 <= k2 of its constraint list (constraints ascending),
+    // the entries in ascending l.  Grouped here per pair (i, j), i <= j.
+    {
+        std::map<std::pair<int, int>, std::vector<int>> pl;  // pair -> (l, k1, k2) triples
+        for (int l = 0; l < P.nlp; l++) {
+            const auto &cs = P.lp_cons[l];
+            for (size_t k1 = 0; k1 < cs.size(); k1++)
+                for (size_t k2 = k1; k2 < cs.size(); k2++) {
+                    auto &v = pl[{cs[k1], cs[k2]}];
+                    v.push_back(l);
+                    v.push_back((int)k1);
+                    v.push_back((int)k2);
+                }
+        }
+        std::vector<int> pi, pj, ll;
+        std::vector<idx_t> ptr;
+        std::vector<dd> ai, aj;
+        for (auto &e : pl) {
+            pi.push_back(e.first.first);
+            pj.push_back(e.first.second);
+            ptr.push_back((idx_t)ll.size());
+            for (size_t q = 0; q < e.second.size(); q += 3) {
+                int l = e.second[q], k1 = e.second[q + 1], k2 = e.second[q + 2];
+                int i = P.lp_cons[l][k1], j = P.lp_cons[l][k2];
+                ll.push_back(l);
+                ai.push_back(P.A[i].lp_val[P.lp_cidx[l][k1]]);
+                aj.push_back(P.A[j].lp_val[P.lp_cidx[l][k2]]);
+            }
+        }
+        ptr.push_back((idx_t)ll.size());
+        dp.n_lp_pairs = (int)pi.size();
+        dp.d_lpp_i = dev_copy(dp, pi, s);
+        dp.d_lpp_j = dev_copy(dp, pj, s);
+        dp.d_lpp_ptr = dev_copy(dp, ptr, s);
+        dp.d_lpp_l = dev_copy(dp, ll, s);
+        dp.d_lpp_ai = dev_copy(dp, ai, s);
+        dp.d_lpp_aj = dev_copy(dp, aj, s);
+    }
+
+    // Element update lists for sum_i y_i A_i (multi_plusToA order: i
+    // ascending, entries in file order, off-diagonals to both positions;
+    // LP entries at nn + index).
+    {
+        const idx_t nt = P.nt();
+        std::vector<std::vector<std::pair<int, dd>>> el(nt);
         for (int i = 0; i < m; i++)
             for (size_t k = 0; k < P.A[i].index.size(); k++) {
                 int l = P.A[i].index[k], n = P.bs[l];
@@ -432,21 +553,24 @@ int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::
                     }
                 }
             }
-        std::vector<idx_t> ptr(P.nn + 1, 0);
+        for (int i = 0; i < m; i++)
+            for (size_t k = 0; k < P.A[i].lp_index.size(); k++)
+                el[P.nn + P.A[i].lp_index[k]].push_back({i, P.A[i].lp_val[k]});
+        std::vector<idx_t> ptr(nt + 1, 0);
         std::vector<int> idx;
         std::vector<dd> val;
-        for (idx_t e = 0; e < P.nn; e++) {
+        for (idx_t e = 0; e < nt; e++) {
             ptr[e] = (idx_t)idx.size();
             for (auto &pr : el[e]) {
                 idx.push_back(pr.first);
                 val.push_back(pr.second);
             }
         }
-        ptr[P.nn] = (idx_t)idx.size();
+        ptr[nt] = (idx_t)idx.size();
         dp.d_el_ptr = dev_copy(dp, ptr, s);
         dp.d_el_idx = dev_copy(dp, idx, s);
         dp.d_el_val = dev_copy(dp, val, s);
-        std::vector<std::vector<dd>> cel(P.nn);
+        std::vector<std::vector<dd>> cel(nt);
         for (size_t k = 0; k < P.C.index.size(); k++) {
             int l = P.C.index[k], n = P.bs[l];
             idx_t o = P.boff[l];
@@ -462,13 +586,14 @@ int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::
                 }
             }
         }
-        std::vector<idx_t> cp(P.nn + 1, 0);
+        for (size_t k = 0; k < P.C.lp_index.size(); k++) cel[P.nn + P.C.lp_index[k]].push_back(P.C.lp_val[k]);
+        std::vector<idx_t> cp(nt + 1, 0);
         std::vector<dd> cv;
-        for (idx_t e = 0; e < P.nn; e++) {
+        for (idx_t e = 0; e < nt; e++) {
             cp[e] = (idx_t)cv.size();
             for (auto &v : cel[e]) cv.push_back(v);
         }
-        cp[P.nn] = (idx_t)cv.size();
+        cp[nt] = (idx_t)cv.size();
         dp.d_cel_ptr = dev_copy(dp, cp, s);
         dp.d_cel_val = dev_copy(dp, cv, s);
     }
@@ -709,12 +834,15 @@ __global__ __launch_bounds__(64) void k_sparse_tasks(int ntask, const int *type,
     }
     if (lane == 0) out[t] = ret;
 }
-// out[i] = 0 + sum of its task values in block order
-__global__ void k_combine_tasks(int m, const idx_t *ptr, const dd *tv, dd *out) {
+// out[i] = 0 + sum of its task values in block order, then its LP terms
+// a_l w_l in ascending l (Lal::getInnerProduct, sdpa_linear.cpp:1616-1650)
+__global__ void k_combine_tasks(int m, const idx_t *ptr, const dd *tv, const idx_t *lptr, const int *lidx,
+                                const dd *lval, const dd *Wlp, dd *out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     dd r = mk(0.0);
     for (idx_t t = ptr[i]; t < ptr[i + 1]; t++) r = dd_add_eq(r, tv[t]);
+    for (idx_t q = lptr[i]; q < lptr[i + 1]; q++) r = dd_add_eq(r, dd_mul(lval[q], Wlp[lidx[q]]));
     out[i] = r;
 }
 
@@ -746,7 +874,8 @@ void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
                                                               dp.d_task_off, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val,
                                                               dp.d_boff, dp.d_bsize, W, dp.d_task_val);
     dense_dots(s, dp, dp.d_dense_tasks, dp.n_dense_tasks, dp.dense_task_maxn, W, T);
-    k_combine_tasks<<<(dp.m + 255) / 256, 256, 0, s>>>(dp.m, dp.d_task_ptr, dp.d_task_val, out);
+    k_combine_tasks<<<(dp.m + 255) / 256, 256, 0, s>>>(dp.m, dp.d_task_ptr, dp.d_task_val, dp.d_lp_ptr, dp.d_lp_idx,
+                                                       dp.d_lp_val, W + dp.nn, out);
 }
 void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
     if (dp.nctask > 0)
@@ -754,31 +883,42 @@ void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
                                                                dp.d_ctask_nnz, dp.d_ctask_off, dp.d_sp_row, dp.d_sp_col,
                                                                dp.d_sp_val, dp.d_boff, dp.d_bsize, W, dp.d_ctask_val);
     dense_dots(s, dp, dp.d_dense_ctasks, dp.n_dense_ctasks, dp.dense_ctask_maxn, W, T);
-    k_combine_tasks<<<1, 32, 0, s>>>(1, dp.d_ctask_ptr, dp.d_ctask_val, out);
+    k_combine_tasks<<<1, 32, 0, s>>>(1, dp.d_ctask_ptr, dp.d_ctask_val, dp.d_clp_ptr, dp.d_clp_idx, dp.d_clp_val,
+                                     W + dp.nn, out);
 }
 
 // ------------------------------------------------------------ sum y_i A_i --
-__global__ void k_plus_A(idx_t nn, const idx_t *ptr, const int *idx, const dd *val, const dd *y, dd *R) {
+// SDP elements accumulate with +=; LP elements (e >= nn) with r = r + a y
+// (Lal::plus's LP loop, sdpa_linear.cpp:1764-1771), a different rounding.
+__global__ void k_plus_A(idx_t nt, idx_t nn, const idx_t *ptr, const int *idx, const dd *val, const dd *y, dd *R) {
     idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nn) return;
+    if (e >= nt) return;
     dd acc = mk(0.0);
-    for (idx_t q = ptr[e]; q < ptr[e + 1]; q++) acc = dd_add_eq(acc, dd_mul(val[q], y[idx[q]]));
+    if (e < nn)
+        for (idx_t q = ptr[e]; q < ptr[e + 1]; q++) acc = dd_add_eq(acc, dd_mul(val[q], y[idx[q]]));
+    else
+        for (idx_t q = ptr[e]; q < ptr[e + 1]; q++) acc = dd_add(acc, dd_mul(val[q], y[idx[q]]));
     R[e] = acc;
 }
-__global__ void k_plus_C(idx_t nn, const idx_t *ptr, const dd *val, dd *R) {
+__global__ void k_plus_C(idx_t nt, idx_t nn, const idx_t *ptr, const dd *val, dd *R) {
     idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nn) return;
+    if (e >= nt) return;
     idx_t q0 = ptr[e], q1 = ptr[e + 1];
     if (q0 == q1) return;
     dd acc = R[e];
-    for (idx_t q = q0; q < q1; q++) acc = dd_add_eq(acc, val[q]);
+    if (e < nn)
+        for (idx_t q = q0; q < q1; q++) acc = dd_add_eq(acc, val[q]);
+    else
+        for (idx_t q = q0; q < q1; q++) acc = dd_add(acc, val[q]);
     R[e] = acc;
 }
 void plus_A(hipStream_t s, DeviceProblem &dp, const dd *y, dd *R) {
-    k_plus_A<<<(unsigned)((dp.nn + 255) / 256), 256, 0, s>>>(dp.nn, dp.d_el_ptr, dp.d_el_idx, dp.d_el_val, y, R);
+    const idx_t nt = dp.nn + dp.nlp;
+    k_plus_A<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(nt, dp.nn, dp.d_el_ptr, dp.d_el_idx, dp.d_el_val, y, R);
 }
 void plus_C(hipStream_t s, DeviceProblem &dp, dd *R) {
-    k_plus_C<<<(unsigned)((dp.nn + 255) / 256), 256, 0, s>>>(dp.nn, dp.d_cel_ptr, dp.d_cel_val, R);
+    const idx_t nt = dp.nn + dp.nlp;
+    k_plus_C<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(nt, dp.nn, dp.d_cel_ptr, dp.d_cel_val, R);
 }
 
 // ------------------------------------------------------------------ Schur --
@@ -938,6 +1078,26 @@ __global__ void k_fill_ptrs(int cnt, const int *slots, const int *wslot, const i
     }
 }
 
+// LP part of the Schur matrix, after the SDP blocks: one thread per
+// constraint pair i <= j adds x_l z_l^-1 a_i a_j (evaluated left to right,
+// as compute_bMat_dense_LP does) for its LP entries in ascending l, to B_ij
+// and, off the diagonal, to B_ji.
+__global__ void k_schur_lp(int np, int m, const int *pi, const int *pj, const idx_t *ptr, const int *ll, const dd *ai,
+                           const dd *aj, const dd *Xlp, const dd *Zilp, dd *B) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int i = pi[p], j = pj[p];
+    dd bij = B[i + (idx_t)m * j], bji = B[j + (idx_t)m * i];
+    for (idx_t q = ptr[p]; q < ptr[p + 1]; q++) {
+        const int l = ll[q];
+        dd v = dd_mul(dd_mul(dd_mul(Xlp[l], Zilp[l]), ai[q]), aj[q]);
+        bij = dd_add_eq(bij, v);
+        if (i != j) bji = dd_add_eq(bji, v);
+    }
+    B[i + (idx_t)m * j] = bij;
+    if (i != j) B[j + (idx_t)m * i] = bji;
+}
+
 void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B, int T) {
     const int m = dp.m;
     sdk::fill(s, (idx_t)m * m, mk(0.0), B);  // Newton::Make_bMat: bMat.setZero()
@@ -983,6 +1143,10 @@ void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B,
             k_scatter_pairs<<<(S.npairs + 255) / 256, 256, 0, s>>>(S.npairs, m, S.d_pair_i, S.d_pair_j, dp.d_pair_val, B);
         }
     }
+    if (dp.n_lp_pairs > 0)
+        k_schur_lp<<<(dp.n_lp_pairs + 255) / 256, 256, 0, s>>>(dp.n_lp_pairs, m, dp.d_lpp_i, dp.d_lpp_j, dp.d_lpp_ptr,
+                                                              dp.d_lpp_l, dp.d_lpp_ai, dp.d_lpp_aj, X + dp.nn,
+                                                              invZ + dp.nn, B);
 }
 
 }  // namespace sdp

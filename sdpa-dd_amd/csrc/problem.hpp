@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ddr.hpp"
@@ -27,6 +28,13 @@ struct Block {
 struct Space {
     std::vector<int> index;  // SDP block numbers present, ascending
     std::vector<Block> blk;
+    // LP entries (SparseLinearSpace::LP_sp_index / LP_sp_block): ascending
+    // indices into the LP part; raw (index, value) pairs in input order until
+    // finish_lp() sorts them (a repeated index keeps its last value).
+    std::vector<int> lp_index;
+    std::vector<dd> lp_val;
+    std::vector<std::pair<int, dd>> lp_raw;
+    void finish_lp();
     int find(int l) const {
         for (size_t k = 0; k < index.size(); k++)
             if (index[k] == l) return (int)k;
@@ -43,14 +51,24 @@ bool read_params(const std::string &path, Params &p);
 
 struct HostProblem {
     int m = 0;
+    // Block structure as given (sdpa_main.cpp:111-130): a size >= 2 is an SDP
+    // block, a negative size -k (or the size 1) adds k (or 1) LP entries.
+    std::vector<int> given;
+    std::vector<int> gtype;     // 1 SDP, 3 LP
+    std::vector<int> gnum;      // SDP block number, or the first LP index
     std::vector<int> bs;        // SDP block sizes
     std::vector<idx_t> boff;    // offset of block l in a concatenated dense space
     idx_t nn = 0;               // sum of n_l^2
+    int nlp = 0;                // LP entries, stored after the SDP blocks (at nn)
+    idx_t nt() const { return nn + nlp; }
     std::vector<dd> b;
     Space C;
     std::vector<Space> A;
     std::vector<std::vector<int>> cons, cidx;  // SDP_constraint / SDP_blockIndex per block
+    std::vector<std::vector<int>> lp_cons, lp_cidx;  // LP_constraint / LP_blockIndex per LP entry
     std::vector<int> formula;                  // useFormula[i*nBlock + l]
+    // Sets given/gtype/gnum/bs/nlp from a given block structure; false on a 0.
+    bool set_structure(const std::vector<int> &given_bs);
     void layout();
     void index();
 };
@@ -106,6 +124,19 @@ struct DeviceProblem {
     std::vector<int> bs;
     std::vector<idx_t> boff;
     idx_t nn = 0;
+    int nlp = 0;  // LP entries at [nn, nn + nlp) of every iterate
+    // LP entries of each A_i (CSR over constraints, ascending LP index) and
+    // of C, added after the SDP blocks of an inner product
+    idx_t *d_lp_ptr = nullptr, *d_clp_ptr = nullptr;
+    int *d_lp_idx = nullptr, *d_clp_idx = nullptr;
+    dd *d_lp_val = nullptr, *d_clp_val = nullptr;
+    // LP part of the Schur matrix: per constraint pair (i <= j), the LP
+    // entries both touch, ascending, with A_i's and A_j's values
+    int n_lp_pairs = 0;
+    int *d_lpp_i = nullptr, *d_lpp_j = nullptr;
+    idx_t *d_lpp_ptr = nullptr;
+    int *d_lpp_l = nullptr;
+    dd *d_lpp_ai = nullptr, *d_lpp_aj = nullptr;
     dd *d_b = nullptr;
     // inner products A_i . W: tasks per constraint in block order
     int ntask = 0;

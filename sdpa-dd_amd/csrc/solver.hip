@@ -10,6 +10,7 @@
 
 #include "context.hpp"
 #include "solver.hpp"
+#include "smallblk.hpp"
 #include "steqr.hpp"
 
 namespace sdp {
@@ -346,6 +347,9 @@ class Solver {
         if (h_sc) (void)hipHostFree(h_sc);
         if (h_diag) (void)hipHostFree(h_diag);
         if (h_infos) (void)hipHostFree(h_infos);
+        if (h_small_out) (void)hipHostFree(h_small_out);
+        if (h_small_info) (void)hipHostFree(h_small_info);
+        if (h_lp_eig) (void)hipHostFree(h_lp_eig);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (s2) (void)hipStreamDestroy(s2);
@@ -362,6 +366,17 @@ class Solver {
     hipStream_t s = nullptr;
     int m = 0, nB = 0, nDim = 0;
     idx_t nn = 0;
+    // LP entries: the last nlp values of every iterate (after the nn SDP
+    // values); nt = nn + nlp.  Their operations follow the LP loops of the
+    // reference's linear-space routines (different roundings from the SDP
+    // blocks' Raxpy forms), so whole-space updates split at nn.
+    idx_t nlp = 0, nt = 0;
+    int slot_lp = 0;  // 3 nlp scalar slots for the LP terms of inner products
+    std::vector<int> small_blk;  // SDP blocks of size <= 20 (Rsyev step length)
+    sdk::SmallEigTask *d_small_tasks = nullptr;
+    dd *d_small_out = nullptr, *h_small_out = nullptr;
+    int *d_small_info = nullptr, *h_small_info = nullptr;
+    dd *d_lp_eig = nullptr, *h_lp_eig = nullptr;
     // iterates and work spaces (concatenated SDP blocks)
     dd *X, *Z, *invCholX, *invCholZ, *invZ, *DxMat, *DzMat, *r_zinv, *x_rd_zinv, *DLS1, *DLS2, *dualMat;
     dd *y, *dy, *g, *primalVec, *bMat;
@@ -423,21 +438,31 @@ class Solver {
         else
             sdk::dot_tasks(s, tasks, cnt, max_blk_elems, ctx.dot_T, d_partial);
     }
+    // The LP terms a_l b_l follow the block sums in ascending l
+    // (Lal::getInnerProduct, sdpa_linear.cpp:1603-1611).
     ddr inner_xz() {
-        dots(d_tasks_xz, nB);
-        const dd *h = get_scalars(0, nB);
+        if (nB) dots(d_tasks_xz, nB);
+        if (nlp) sdk::lp_prod(s, nlp, X + nn, Z + nn, d_sc + slot_lp);
+        const dd *h = get_scalars(0, slot_lp + (int)nlp);
         ddr r = 0.0;
         for (int l = 0; l < nB; l++) r += ddr(h[l]);
+        for (idx_t l = 0; l < nlp; l++) r += ddr(h[slot_lp + l]);
         return r;
     }
     // X . dZ, dX . Z and dX . dZ with one launch and one synchronisation.
     void inner_dirs(ddr &r0, ddr &r1, ddr &r2) {
-        dots(d_tasks3, 3 * nB);
-        const dd *h = get_scalars(0, 3 * nB);
+        if (nB) dots(d_tasks3, 3 * nB);
+        if (nlp) {
+            sdk::lp_prod(s, nlp, X + nn, DzMat + nn, d_sc + slot_lp);
+            sdk::lp_prod(s, nlp, DxMat + nn, Z + nn, d_sc + slot_lp + nlp);
+            sdk::lp_prod(s, nlp, DxMat + nn, DzMat + nn, d_sc + slot_lp + 2 * nlp);
+        }
+        const dd *h = get_scalars(0, slot_lp + 3 * (int)nlp);
         ddr r[3];
         for (int q = 0; q < 3; q++) {
             r[q] = 0.0;
             for (int l = 0; l < nB; l++) r[q] += ddr(h[q * nB + l]);
+            for (idx_t l = 0; l < nlp; l++) r[q] += ddr(h[slot_lp + q * nlp + l]);
         }
         r0 = r[0];
         r1 = r[1];
@@ -464,6 +489,7 @@ class Solver {
                 oz_check(oz::gemm(s, ctx.ozw, 'N', 'N', n, n, n, 1, a + o, n, b + o, n, 0, work + o, n, false, ctx.oz_slices));
                 oz_check(oz::gemm(s, ctx.ozw, 'N', 'N', n, n, n, 1, work + o, n, c + o, n, 0, ret + o, n, false, ctx.oz_slices));
             }
+            if (nlp) sdk::lp_triple(s, nlp, a + nn, b + nn, c + nn, ret + nn);
             return;
         }
         for (int l = 0; l < nB; l++) {
@@ -476,6 +502,7 @@ class Solver {
             idx_t o = dp.boff[l];
             sdk::gemm(s, 'N', n, n, n, mk(1.0), work + o, n, c + o, n, mk(0.0), ret + o, n);
         }
+        if (nlp) sdk::lp_triple(s, nlp, a + nn, b + nn, c + nn, ret + nn);
     }
     bool oz_failed = false;
     void oz_check(int rc) {
@@ -540,6 +567,12 @@ class Solver {
     void plus(dd *ret, const dd *a, const dd *b, dd sc) {
         sdk::copy(s, nn, a, ret);
         sdk::axpy(s, nn, sc, b, ret);
+        if (nlp) sdk::lp_plus(s, nlp, a + nn, b + nn, sc, ret + nn);
+    }
+    // a += s*b in place: the SDP blocks by Raxpy, the LP entries as Lal::plus
+    void plus_in(dd *a, const dd *b, dd sc) {
+        sdk::axpy(s, nn, sc, b, a);
+        if (nlp) sdk::lp_plus(s, nlp, a + nn, b + nn, sc, a + nn);
     }
 
     // ---- pieces of the algorithm ----
@@ -549,16 +582,17 @@ class Solver {
         sdk::scal(s, m, mk(-1.0), primalVec);
         sdk::axpy(s, m, mk(1.0), dp.d_b, primalVec);
         sdp::plus_A(s, dp, y, dualMat);
-        sdk::scal(s, nn, mk(-1.0), dualMat);
+        sdk::scal(s, nt, mk(-1.0), dualMat);
         sdp::plus_C(s, dp, dualMat);
-        sdk::axpy(s, nn, mk(-1.0), Z, dualMat);
+        plus_in(dualMat, Z, mk(-1.0));
         normP = max_abs(primalVec, m);
-        normD = max_abs(dualMat, nn);
+        normD = max_abs(dualMat, nt);
     }
     // Jal::getInvChol / getInvCholAndInv (sdpa_jordan.cpp:123-170) for every
     // block of A, enqueued on `st` without synchronisation; block l's
     // Cholesky info goes to infos[l].
-    void inv_chol_async(hipStream_t st, dd *A, dd *invChol, dd *work, dd *inv, int64_t *infos, dd *rinv) {
+    void inv_chol_async(hipStream_t st, dd *A, dd *invChol, dd *work, dd *inv, int64_t *infos, int64_t *lp_info,
+                        dd *rinv) {
         for (int l = 0; l < nB; l++) {
             int n = bsz(l);
             idx_t o = dp.boff[l];
@@ -583,6 +617,7 @@ class Solver {
                 else
                     sdk::trmm_lltn_self(st, n, invChol + o, n, inv + o, n);
             }
+        if (nlp) sdk::lp_inv_chol(st, nlp, A + nn, invChol + nn, inv ? inv + nn : nullptr, lp_info);
     }
     // Solutions::update's factorisations: X on the main stream and Z on the
     // second one, concurrently.  Returns false when either Cholesky fails.
@@ -591,47 +626,85 @@ class Solver {
         double t0 = now();
         (void)hipEventRecord(ev_fork, s);
         (void)hipStreamWaitEvent(s2, ev_fork, 0);
-        inv_chol_async(s, X, invCholX, DLS1, nullptr, d_infos, ctx.d_rinv);
-        inv_chol_async(s2, Z, invCholZ, DLS2, invZ, d_infos + nB, d_rinv2);
+        inv_chol_async(s, X, invCholX, DLS1, nullptr, d_infos, d_infos + 2 * nB, ctx.d_rinv);
+        inv_chol_async(s2, Z, invCholZ, DLS2, invZ, d_infos + nB, d_infos + 2 * nB + 1, d_rinv2);
         (void)hipEventRecord(ev_join, s2);
         (void)hipStreamWaitEvent(s, ev_join, 0);
-        (void)hipMemcpyAsync(h_infos, d_infos, sizeof(int64_t) * 2 * nB, hipMemcpyDeviceToHost, s);
+        (void)hipMemcpyAsync(h_infos, d_infos, sizeof(int64_t) * (2 * nB + 2), hipMemcpyDeviceToHost, s);
         sync();
         t_inverse += now() - t0;
-        for (int q = 0; q < 2 * nB; q++)
+        for (int q = 0; q < 2 * nB + (nlp ? 2 : 0); q++)
             if (h_infos[q] != 0) return false;
         return true;
     }
     ChainPool pool;
     // Jal::getMinEigen (sdpa_jordan.cpp:55-88) for (L_X^{-1}, dX) and (L_Z^{-1}, dZ):
-    // all blocks of both pairs at once.
-    void min_eigen_xz(ddr &mx, ddr &mz) {
+    // all blocks of both pairs at once -- Lanczos for blocks of size > 20,
+    // Rsyev for the smaller ones, (m l) l for the LP entries.  The minimum
+    // over blocks does not depend on their order.  Returns false when an
+    // Rsyev did not converge (the reference ends the run there).
+    bool min_eigen_xz(ddr &mx, ddr &mz) {
         double t0 = now();
+        std::vector<int> big;
         int maxn = 0;
-        for (int l = 0; l < nB; l++) maxn = std::max(maxn, bsz(l));
-        if (!pool.ensure(2 * nB, maxn)) {
-            ctx.set_error("Lanczos workspace allocation failed");
-            mx = mz = ddr(NAN);
-            return;
-        }
-        for (int l = 0; l < nB; l++) {
-            Chain &cx = pool.chains[l];
-            cx.n = bsz(l);
-            cx.L = invCholX + dp.boff[l];
-            cx.M = DxMat + dp.boff[l];
-            Chain &cz = pool.chains[nB + l];
-            cz.n = bsz(l);
-            cz.L = invCholZ + dp.boff[l];
-            cz.M = DzMat + dp.boff[l];
-        }
-        run_chains(pool, 2 * nB, ctx.dot_T, s, ctx.path >= 1);
+        for (int l = 0; l < nB; l++)
+            if (bsz(l) > sdk::SMALL_EIG_MAXN) {
+                big.push_back(l);
+                maxn = std::max(maxn, bsz(l));
+            }
+        const int nbig = (int)big.size(), nsmall = (int)small_blk.size();
         mx = 1.0E50;
         mz = 1.0E50;
-        for (int l = 0; l < nB; l++) {
-            if (pool.chains[l].result < mx) mx = pool.chains[l].result;
-            if (pool.chains[nB + l].result < mz) mz = pool.chains[nB + l].result;
+        if (nsmall) sdk::small_min_eig(s, d_small_tasks, 2 * nsmall, ctx.dot_T, d_small_out, d_small_info);
+        if (nlp) {
+            sdk::lp_eig(s, nlp, DxMat + nn, invCholX + nn, d_lp_eig);
+            sdk::lp_eig(s, nlp, DzMat + nn, invCholZ + nn, d_lp_eig + nlp);
+        }
+        if (nbig) {
+            if (!pool.ensure(2 * nbig, maxn)) {
+                ctx.set_error("Lanczos workspace allocation failed");
+                mx = mz = ddr(NAN);
+                return false;
+            }
+            for (int q = 0; q < nbig; q++) {
+                const int l = big[q];
+                Chain &cx = pool.chains[q];
+                cx.n = bsz(l);
+                cx.L = invCholX + dp.boff[l];
+                cx.M = DxMat + dp.boff[l];
+                Chain &cz = pool.chains[nbig + q];
+                cz.n = bsz(l);
+                cz.L = invCholZ + dp.boff[l];
+                cz.M = DzMat + dp.boff[l];
+            }
+            run_chains(pool, 2 * nbig, ctx.dot_T, s, ctx.path >= 1);
+            for (int q = 0; q < nbig; q++) {
+                if (pool.chains[q].result < mx) mx = pool.chains[q].result;
+                if (pool.chains[nbig + q].result < mz) mz = pool.chains[nbig + q].result;
+            }
+        }
+        bool ok = true;
+        if (nsmall || nlp) {
+            if (nsmall) {
+                (void)hipMemcpyAsync(h_small_out, d_small_out, sizeof(dd) * 2 * nsmall, hipMemcpyDeviceToHost, s);
+                (void)hipMemcpyAsync(h_small_info, d_small_info, sizeof(int) * 2 * nsmall, hipMemcpyDeviceToHost, s);
+            }
+            if (nlp) (void)hipMemcpyAsync(h_lp_eig, d_lp_eig, sizeof(dd) * 2 * nlp, hipMemcpyDeviceToHost, s);
+            sync();
+            for (int q = 0; q < nsmall; q++) {
+                if (h_small_info[q] != 0 || h_small_info[nsmall + q] != 0) ok = false;
+                if (ddr(h_small_out[q]) < mx) mx = h_small_out[q];
+                if (ddr(h_small_out[nsmall + q]) < mz) mz = h_small_out[nsmall + q];
+            }
+            for (idx_t q = 0; q < nlp; q++) {
+                if (ddr(h_lp_eig[q]) < mx) mx = h_lp_eig[q];
+                if (ddr(h_lp_eig[nlp + q]) < mz) mz = h_lp_eig[nlp + q];
+            }
+        } else {
+            sync();
         }
         t_lanczos += now() - t0;
+        return ok;
     }
     void diag_sums(double lam, ddr &x0z, ddr &xz0) {
         // x0zMat += lambda * Z_ii, xMatz0 += lambda * X_ii, blocks in order
@@ -641,6 +714,10 @@ class Solver {
             sdk::gather_diag(s, bsz(l), blk(Z, l), d_diag + off);
             sdk::gather_diag(s, bsz(l), blk(X, l), d_diag + nDim + off);
             off += bsz(l);
+        }
+        if (nlp) {  // the LP entries follow the SDP diagonals (sdpa_parts.cpp:767-770)
+            sdk::copy(s, nlp, Z + nn, d_diag + off);
+            sdk::copy(s, nlp, X + nn, d_diag + nDim + off);
         }
         (void)hipMemcpyAsync(h_diag, d_diag, sizeof(dd) * 2 * (size_t)nDim, hipMemcpyDeviceToHost, s);
         sync();
@@ -656,25 +733,52 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     m = hp.m;
     nB = (int)hp.bs.size();
     nn = hp.nn;
+    nlp = hp.nlp;
+    nt = nn + nlp;
     nDim = 0;
     for (int n : hp.bs) nDim += n;
-    int maxn = 0;
-    for (int n : hp.bs) maxn = std::max(maxn, n);
+    nDim += (int)nlp;
     dd **bufs[] = {&X, &Z, &invCholX, &invCholZ, &invZ, &DxMat, &DzMat, &r_zinv, &x_rd_zinv, &DLS1, &DLS2, &dualMat};
     for (dd **b : bufs)
-        if (!(*b = alloc(nn))) return -2;
+        if (!(*b = alloc(nt))) return -2;
     if (!(y = alloc(m)) || !(dy = alloc(m)) || !(g = alloc(m)) || !(primalVec = alloc(m)) ||
         !(bMat = alloc((idx_t)m * m)))
         return -2;
     slot_C = 3 * nB;
     slot_b = slot_C + 1;
     slot_max = slot_C + 2;
-    if (!(d_sc = alloc(slot_C + 3)) || !(d_diag = alloc(2 * (idx_t)nDim))) return -2;
-    if (hipHostMalloc((void **)&h_sc, sizeof(dd) * (slot_C + 3)) != hipSuccess) return -2;
+    slot_lp = slot_C + 3;
+    const int nslots = slot_lp + 3 * (int)nlp;
+    if (!(d_sc = alloc(nslots)) || !(d_diag = alloc(2 * (idx_t)nDim))) return -2;
+    if (hipHostMalloc((void **)&h_sc, sizeof(dd) * nslots) != hipSuccess) return -2;
     if (hipHostMalloc((void **)&h_diag, sizeof(dd) * 2 * (size_t)nDim) != hipSuccess) return -2;
-    if (hipHostMalloc((void **)&h_infos, sizeof(int64_t) * 2 * nB) != hipSuccess) return -2;
-    if (hipMalloc((void **)&d_infos, sizeof(int64_t) * 2 * nB) != hipSuccess) return -2;
+    if (hipHostMalloc((void **)&h_infos, sizeof(int64_t) * (2 * nB + 2)) != hipSuccess) return -2;
+    if (hipMalloc((void **)&d_infos, sizeof(int64_t) * (2 * nB + 2)) != hipSuccess) return -2;
     mem.push_back(d_infos);
+    // Rsyev step-length tasks: (L_X^{-1}, dX) of each small block, then (L_Z^{-1}, dZ)
+    for (int l = 0; l < nB; l++)
+        if (bsz(l) <= sdk::SMALL_EIG_MAXN) small_blk.push_back(l);
+    if (!small_blk.empty()) {
+        const int ns = (int)small_blk.size();
+        std::vector<sdk::SmallEigTask> tk;
+        for (int pass = 0; pass < 2; pass++)
+            for (int l : small_blk)
+                tk.push_back({(pass ? invCholZ : invCholX) + dp.boff[l], (pass ? DzMat : DxMat) + dp.boff[l], bsz(l)});
+        if (hipMalloc((void **)&d_small_tasks, sizeof(sdk::SmallEigTask) * tk.size()) != hipSuccess) return -2;
+        mem.push_back(d_small_tasks);
+        if (hipMemcpy(d_small_tasks, tk.data(), sizeof(sdk::SmallEigTask) * tk.size(), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return -2;
+        if (!(d_small_out = alloc(2 * ns))) return -2;
+        if (hipMalloc((void **)&d_small_info, sizeof(int) * 2 * ns) != hipSuccess) return -2;
+        mem.push_back(d_small_info);
+        if (hipHostMalloc((void **)&h_small_out, sizeof(dd) * 2 * ns) != hipSuccess) return -2;
+        if (hipHostMalloc((void **)&h_small_info, sizeof(int) * 2 * ns) != hipSuccess) return -2;
+    }
+    if (nlp) {
+        if (!(d_lp_eig = alloc(2 * nlp))) return -2;
+        if (hipHostMalloc((void **)&h_lp_eig, sizeof(dd) * 2 * nlp) != hipSuccess) return -2;
+    }
     if (!(d_rinv2 = alloc(64))) return -2;
     if (ctx.path >= 1 && (!(bSave = alloc((idx_t)m * m)) || !(rvec = alloc(m)) || !(dyTry = alloc(m)))) return -2;
     if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess) return -2;
@@ -701,12 +805,12 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         if (hipMemcpy(d_tasks_xz, txz.data(), sizeof(sdk::DotTask) * txz.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(d_tasks3, t3.data(), sizeof(sdk::DotTask) * t3.size(), hipMemcpyHostToDevice) != hipSuccess)
             return -2;
-        if (!(d_partial = alloc((idx_t)3 * nB * ctx.dot_T))) return -2;
+        if (!(d_partial = alloc((idx_t)std::max(1, 3 * nB) * ctx.dot_T))) return -2;
     }
     double t_start = now();
     sdk::zero_ints(s, ctx.d_tsw_sticky, 1);
     *ctx.h_tsw_err = 0;
-    for (dd **b : bufs) sdk::fill(s, nn, mk(0.0), *b);
+    for (dd **b : bufs) sdk::fill(s, nt, mk(0.0), *b);
 
     // Solutions::initialize (sdpa_dataset.cpp:35-60)
     ddr lambda = prm.lambdaStar;
@@ -717,6 +821,13 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         sdk::set_identity(s, n, (1.0 / sqrt(lambda)).v, blk(invCholX, l));
         sdk::set_identity(s, n, (1.0 / sqrt(lambda)).v, blk(invCholZ, l));
         sdk::set_identity(s, n, (1.0 / lambda).v, blk(invZ, l));
+    }
+    if (nlp) {  // DenseLinearSpace::setIdentity sets every LP entry to the scalar
+        sdk::fill(s, nlp, lambda.v, X + nn);
+        sdk::fill(s, nlp, lambda.v, Z + nn);
+        sdk::fill(s, nlp, (1.0 / sqrt(lambda)).v, invCholX + nn);
+        sdk::fill(s, nlp, (1.0 / sqrt(lambda)).v, invCholZ + nn);
+        sdk::fill(s, nlp, (1.0 / lambda).v, invZ + nn);
     }
     sdk::fill(s, m, mk(0.0), y);
 
@@ -790,18 +901,18 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     // Newton::Mehrotra (sdpa_newton.cpp:1345-1384)
     auto mehrotra = [&](bool predictor) -> bool {
         ddr target = betaV * muCurrent;
-        sdk::copy(s, nn, invZ, r_zinv);
-        sdk::scal(s, nn, target.v, r_zinv);
-        sdk::axpy(s, nn, mk(-1.0), X, r_zinv);
+        sdk::copy(s, nt, invZ, r_zinv);
+        sdk::scal(s, nt, target.v, r_zinv);
+        plus_in(r_zinv, X, mk(-1.0));
         if (!predictor) {
             triple(DLS1, DxMat, DzMat, invZ, DLS2);
-            sdk::axpy(s, nn, mk(-1.0), DLS1, r_zinv);
+            plus_in(r_zinv, DLS1, mk(-1.0));
         }
         if (phase == pFEAS || phase == noINFO) {
             if (predictor) triple(x_rd_zinv, X, dualMat, invZ, DLS2);
             plus(DLS1, r_zinv, x_rd_zinv, mk(-1.0));
         } else {
-            sdk::copy(s, nn, r_zinv, DLS1);
+            sdk::copy(s, nt, r_zinv, DLS1);
         }
         sdp::inner_A(s, dp, DLS1, g, ctx.dot_T);
         sdk::scal(s, m, mk(-1.0), g);
@@ -829,8 +940,8 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         }
         schur_solve();
         sdp::plus_A(s, dp, dy, DzMat);
-        sdk::scal(s, nn, mk(-1.0), DzMat);
-        if (phase == pFEAS || phase == noINFO) sdk::axpy(s, nn, mk(1.0), dualMat, DzMat);
+        sdk::scal(s, nt, mk(-1.0), DzMat);
+        if (phase == pFEAS || phase == noINFO) plus_in(DzMat, dualMat, mk(1.0));
         triple(DLS1, X, DzMat, invZ, DLS2);
         plus(DxMat, r_zinv, DLS1, mk(-1.0));
         for (int l = 0; l < nB; l++) sdk::symmetrize(s, bsz(l), blk(DxMat, l));
@@ -891,9 +1002,13 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             const ddr xi = 3.0;
             ddr alphaBD = 100.0;
             ddr mx, mz;
-            min_eigen_xz(mx, mz);  // synchronises: the solves' flag is current
+            bool eig_ok = min_eigen_xz(mx, mz);  // synchronises: the solves' flag is current
             if (trsv_failed()) {
                 res.message = "triangular solve did not complete";
+                break;
+            }
+            if (!eig_ok) {
+                res.message = "eigenvalue iteration did not converge";
                 break;
             }
             alphaP = (-mx > 1.0 / alphaBD) ? -1.0 / mx : alphaBD;
@@ -949,9 +1064,9 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         res.trace.push_back({pIteration, muCurrent.v, thetaP.v, thetaD.v, objValPrimal.v, objValDual.v, alphaP.v, alphaD.v,
                              betaV.v});
         // Solutions::update (sdpa_dataset.cpp:120-143)
-        sdk::axpy(s, nn, alphaP.v, DxMat, X);
+        plus_in(X, DxMat, alphaP.v);
         sdk::axpy(s, m, alphaD.v, dy, y);
-        sdk::axpy(s, nn, alphaD.v, DzMat, Z);
+        plus_in(Z, DzMat, alphaD.v);
         const ddr cannot_move = 1.0e-4;
         if (alphaP < cannot_move && alphaD < cannot_move) {
             res.message = "cannot move";
@@ -1024,12 +1139,12 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         res.relgap = ((mean < 1.0) ? PDgap : PDgap / mean).v;
     }
     if (xo) {
-        xo->resize(nn);
-        (void)hipMemcpy(xo->data(), X, sizeof(dd) * nn, hipMemcpyDeviceToHost);
+        xo->resize(nt);
+        (void)hipMemcpy(xo->data(), X, sizeof(dd) * nt, hipMemcpyDeviceToHost);
     }
     if (zo) {
-        zo->resize(nn);
-        (void)hipMemcpy(zo->data(), Z, sizeof(dd) * nn, hipMemcpyDeviceToHost);
+        zo->resize(nt);
+        (void)hipMemcpy(zo->data(), Z, sizeof(dd) * nt, hipMemcpyDeviceToHost);
     }
     if (yo) {
         yo->resize(m);

@@ -76,6 +76,7 @@ def load_library(path=None):
         "sdpadd_Rpotrf": (ctypes.c_int, [vp, ctypes.c_char, c_i64, vp, c_i64, ctypes.POINTER(c_i64)]),
         "sdpadd_Rsteqr": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_dp, c_dp, c_dp, c_i64, c_dp, ctypes.POINTER(c_i64)]),
         "sdpadd_lanczos_min_eig": (ctypes.c_int, [vp, c_i64, vp, vp, c_dp]),
+        "sdpadd_small_min_eig": (ctypes.c_int, [vp, c_i64, vp, vp, c_dp, ctypes.POINTER(ctypes.c_int)]),
         "sdpadd_inv_chol": (ctypes.c_int, [vp, c_i64, vp, vp, vp, vp]),
         "sdpadd_problem_load": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(vp)]),
         "sdpadd_problem_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -249,13 +250,24 @@ class Problem:
         elif path is not None:
             dev.check(dev.lib.sdpadd_problem_load(dev.h, path.encode(), ctypes.byref(self.p)), "problem_load")
         m, nb = ctypes.c_int(), ctypes.c_int()
-        bs = (ctypes.c_int * 64)()
-        dev.check(dev.lib.sdpadd_problem_dims(self.p, ctypes.byref(m), ctypes.byref(nb), bs, 64), "dims")
+        bs = (ctypes.c_int * 65536)()
+        dev.check(dev.lib.sdpadd_problem_dims(self.p, ctypes.byref(m), ctypes.byref(nb), bs, 65536), "dims")
         self.m, self.block_struct = m.value, [bs[i] for i in range(nb.value)]
 
     @property
+    def sdp_blocks(self):
+        """Sizes of the SDP blocks (block sizes >= 2), in order."""
+        return [n for n in self.block_struct if n >= 2]
+
+    @property
+    def n_lp(self):
+        """LP entries (negative blocks and blocks of size 1)."""
+        return sum(-n if n < 0 else 1 for n in self.block_struct if n < 2)
+
+    @property
     def nn(self):
-        return sum(n * n for n in self.block_struct)
+        """Length of an iterate: the SDP blocks (n^2 each), then the LP entries."""
+        return sum(n * n for n in self.sdp_blocks) + self.n_lp
 
     @classmethod
     def from_arrays(cls, dev, m, block_struct, b, entries):
@@ -282,7 +294,7 @@ class Problem:
         return cls(dev, _handle=h)
 
     def formula(self):
-        out = (ctypes.c_int * (self.m * len(self.block_struct)))()
+        out = (ctypes.c_int * max(1, self.m * len(self.sdp_blocks)))()
         self.dev.check(self.dev.lib.sdpadd_problem_formula(self.p, out), "formula")
         return np.array(out[:], dtype=np.int32)
 

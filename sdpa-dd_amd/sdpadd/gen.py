@@ -149,6 +149,46 @@ def random_sparse(path, n=200, m=300, density=0.20, seed=SEED0 + 3):
     return w.write(path, "random sparse SDP n=%d m=%d seed=%d" % (n, m, seed))
 
 
+def mixed_blocks(path, sdp=(30, 8, 3), lp=12, m=25, density=0.3, seed=SEED0 + 5):
+    """SDP blocks of both step-length kinds (> 20: Lanczos, <= 20: Rsyev), a
+    size-1 block and an LP block of `lp` entries; each A_i touches a random
+    subset of blocks.  Strictly feasible by construction."""
+    rng = np.random.default_rng(seed)
+    nb = len(sdp)
+    blocks = list(sdp) + [1, -lp]
+    X0 = [_pd(n, rng) for n in sdp]
+    Z0 = [_pd(n, rng) for n in sdp]
+    x0, z0 = rng.uniform(0.5, 2.0, lp + 1), rng.uniform(0.5, 2.0, lp + 1)
+    y0 = rng.standard_normal(m)
+    w = Writer(m, blocks)
+    Csdp = [z.copy() for z in Z0]
+    clp = z0.copy()
+    for k in range(m):
+        bk = 0.0
+        for l, n in enumerate(sdp):
+            if rng.random() < 0.6 or l == k % nb:
+                g = np.triu(_sprand((n, n), density, rng))
+                a = g + np.triu(g, 1).T
+                if not a.any():
+                    a[0, 0] = 1.0
+                w.add_dense(k + 1, l + 1, a)
+                bk += np.sum(a * X0[l])
+                Csdp[l] += y0[k] * a
+        v = np.where(rng.random(lp + 1) < density, rng.standard_normal(lp + 1), 0.0)
+        for i in np.nonzero(v)[0]:
+            blk, row = (nb + 1, 0) if i == 0 else (nb + 2, i - 1)
+            w.add(k + 1, blk, row, row, v[i])
+        bk += np.sum(v * x0)
+        clp += y0[k] * v
+        w.b[k] = bk
+    for l in range(nb):
+        w.add_dense(0, l + 1, -Csdp[l])
+    w.add(0, nb + 1, 0, 0, -clp[0])
+    for i in range(lp):
+        w.add(0, nb + 2, i, i, -clp[i + 1])
+    return w.write(path, "mixed SDP/LP blocks %s + 1 + LP %d, m=%d seed=%d" % (sdp, lp, m, seed))
+
+
 def theta(path, n=300, p=0.1, seed=SEED0 + 4):
     """Workload 4: Lovasz theta of G(n, p): C = -J, tr X = 1, X_ij = 0 on edges."""
     rng = np.random.default_rng(seed)

@@ -258,3 +258,30 @@ def test_lanczos(gpu, oracle, n):
     out = np.zeros(2)
     gpu.check(gpu.lib.sdpadd_lanczos_min_eig(gpu.h, n, dL.ptr, dM.ptr, dp(out)), "lanczos")
     assert same(out, ref)
+
+
+@pytest.mark.parametrize("n,scale,threads", [(2, 1.0, None), (3, 1.0, None), (7, 1.0, None), (13, 1.0, None),
+                                             (20, 1.0, None), (20, 1.0, 1), (9, 1e-140, None), (9, 1e140, None)])
+def test_small_min_eig(gpu, oracle, n, scale, threads):
+    """Blocks of size <= 20: lambda_min(L M L^T) by the Rsyev path, bit for
+    bit with the oracle (which mpmath pins), including matrices scaled below
+    rmin and above rmax (Rsyev's Rlascl path) and the serial dot partition."""
+    T = threads or sdpadd.DEFAULT_DOT_PARTITION
+    rng = np.random.default_rng(100 + n)
+    L = rand_lower_cm(n, rng)
+    g = rng.standard_normal((n, n)) * scale
+    Mh = (g + g.T) / 2
+    M = np.ascontiguousarray(np.stack([Mh, np.zeros_like(Mh)], -1).transpose(1, 0, 2)).reshape(-1, 2)
+    oracle.set_dot_threads(T)
+    gpu.set_dot_partition(T)
+    try:
+        ref, rinfo = oracle.qr_min_eigen(n, L, M)
+        dL, dM = gpu.to_device(L), gpu.to_device(M)
+        out = np.zeros(2)
+        info = ctypes.c_int(-1)
+        gpu.check(gpu.lib.sdpadd_small_min_eig(gpu.h, n, dL.ptr, dM.ptr, dp(out), ctypes.byref(info)), "small_min_eig")
+    finally:
+        oracle.set_dot_threads(1)
+        gpu.set_dot_partition(sdpadd.DEFAULT_DOT_PARTITION)
+    assert info.value == 0 and rinfo == 0
+    assert same(out, ref), (out, ref)

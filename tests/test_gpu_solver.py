@@ -129,3 +129,46 @@ def test_small_workloads_bitwise(gpu, oracle, tmp_path, builder, kw):
     for a, b in zip(got["trace"], ref["trace"]):
         for k in KEYS:
             assert same(a[k], b[k]), (a["it"], k)
+
+
+def _mixed(tmp_path):
+    from sdpadd import gen
+    data = gen.mixed_blocks(str(tmp_path / "mixed.dat-s"))
+    return data, gen.param_file(str(tmp_path / "p.sdpa"), epsilon_star=1e-20, epsilon_dash=1e-20)
+
+
+@pytest.mark.parametrize("name", ["truss1", "truss4", "control1", "arch0", "hinf1", "mixed"])
+def test_lp_and_small_blocks_bitwise_vs_oracle(gpu, oracle, tmp_path, name):
+    """SDP blocks of size <= 20 (Rsyev step length), LP entries (negative and
+    size-1 blocks) and their mix with a Lanczos block: the whole trajectory
+    and the final iterate (SDP blocks, then LP entries) are bit-identical to
+    the oracle's with the same dot partition.  hinf1 is ill-posed (its end
+    depends on the partition) and still identical here."""
+    if name == "mixed":
+        path, param = _mixed(tmp_path)
+    else:
+        path, param = golden_sdplib(name), SDPLIB_PARAM
+    ref = oracle.solve(path, param, dot_threads=sdpadd.DEFAULT_DOT_PARTITION, dump=True)
+    p = sdpadd.Problem(gpu, path)
+    try:
+        got = p.solve(param)
+        X, y, Z = p.solution()
+    finally:
+        p.close()
+    assert got["iterations"] == ref["iterations"] and got["phase"] == ref["phase"]
+    assert len(got["trace"]) == len(ref["trace"])
+    for a, b in zip(got["trace"], ref["trace"]):
+        for k in KEYS:
+            assert same(a[k], b[k]), (a["it"], k, a[k], b[k])
+    for k in ("mu", "objValPrimal", "objValDual", "normPrimalVec", "normDualMat"):
+        assert same(got[k], ref[k]), k
+    assert same(X, np.array(ref["xMat"])) and same(Z, np.array(ref["zMat"])) and same(y, np.array(ref["yVec"]))
+
+
+@pytest.mark.parametrize("name", ["truss1", "truss3", "truss4", "control1", "arch0"])
+def test_lp_and_small_blocks_against_reference_results(gpu, name):
+    it, ph, objp, objd = _result_file(name)
+    got = gpu_solve(gpu, golden_sdplib(name), SDPLIB_PARAM)
+    assert got["iterations"] == it and got["phase"] == ph
+    assert "%.16e" % (-got["objValDual"][0]) == "%.16e" % objp
+    assert "%.16e" % (-got["objValPrimal"][0]) == "%.16e" % objd
