@@ -137,7 +137,7 @@ def _mixed(tmp_path):
     return data, gen.param_file(str(tmp_path / "p.sdpa"), epsilon_star=1e-20, epsilon_dash=1e-20)
 
 
-@pytest.mark.parametrize("name", ["truss1", "truss4", "control1", "arch0", "hinf1", "mixed"])
+@pytest.mark.parametrize("name", ["truss1", "truss4", "control1", "arch0", "hinf1", "example1", "mixed"])
 def test_lp_and_small_blocks_bitwise_vs_oracle(gpu, oracle, tmp_path, name):
     """SDP blocks of size <= 20 (Rsyev step length), LP entries (negative and
     size-1 blocks) and their mix with a Lanczos block: the whole trajectory
@@ -146,6 +146,9 @@ def test_lp_and_small_blocks_bitwise_vs_oracle(gpu, oracle, tmp_path, name):
     depends on the partition) and still identical here."""
     if name == "mixed":
         path, param = _mixed(tmp_path)
+    elif name == "example1":
+        g = os.path.join(os.path.dirname(__file__), "golden")
+        path, param = os.path.join(g, "example1.dat-s"), os.path.join(g, "param_default.sdpa")
     else:
         path, param = golden_sdplib(name), SDPLIB_PARAM
     ref = oracle.solve(path, param, dot_threads=sdpadd.DEFAULT_DOT_PARTITION, dump=True)

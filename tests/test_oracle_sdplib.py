@@ -110,3 +110,28 @@ def test_oracle_hinf1_is_chaotic_but_close(oracle):
             ref = [x.replace("-0.00e+00", "+0.00e+00") for x in ref]
             assert got[:2] + got[4:] == ref[:2] + ref[4:], (t, rec["it"], got, ref)
         assert abs(-r["objValDual"][0] - float(fin["objValPrimal"])) < 1e-4
+
+
+def test_oracle_example1_until_the_dual_is_exact():
+    """The reference's example1 (one 2 x 2 block: the Rsyev step length).
+    From iteration 5 on its dual residual is exactly zero up to rounding
+    (thetaD ~ 1e-35) and the dual step length then switches between 1 and
+    gammaStar * 100 on the sign of rounding noise in dZ, so example1.result
+    (written by an older build) is reproduced through iteration 4, and in the
+    outcome: pdOPT, objectives -41.9 to all printed digits, final mu below
+    1e-29 (the reference stops two noise-driven iterations later)."""
+    import os
+    from oracle_lib import GOLDEN, Oracle
+    o = Oracle()
+    txt = open(os.path.join(GOLDEN, "example1.result")).read()
+    trace = [ln.split() for ln in txt.splitlines()
+             if len(ln.split()) == 9 and ln.split()[0].isdigit() and "e" in ln.split()[1]]
+    r = o.solve(os.path.join(GOLDEN, "example1.dat-s"), os.path.join(GOLDEN, "param_default.sdpa"), dot_threads=1)
+    assert r["phase"] == "pdOPT" and abs(r["iterations"] - 44) <= 2
+    for rec, ref in zip(r["trace"][:5], trace[:5]):
+        got = fmt_trace(rec)
+        ref = [x.replace("-0.00e+00", "+0.00e+00") for x in ref]
+        assert got[:2] + got[4:] == ref[:2] + ref[4:], (rec["it"], got, ref)
+    assert "%.16e" % -r["objValDual"][0] == "-4.1899999999999999e+01"
+    assert "%.16e" % -r["objValPrimal"][0] == "-4.1899999999999999e+01"
+    assert r["mu"][0] < 1e-29
