@@ -213,6 +213,30 @@ int sdpadd_trace(sdpadd_problem p, double *out, int cap);
  * SDP blocks, then the LP entries -- and y m dd). */
 int sdpadd_solution(sdpadd_problem p, double *X, double *y, double *Z);
 
+/* ---- Initial points and full-dd solution files ---------------------------- */
+/* Reads an initial-point file for the next solves the way sdpa_dd's -is
+ * (sparse != 0) and -id options do (sdpa_main.cpp:237-251, IO::read at
+ * sdpa_io.cpp:152-285): the file's y negated, its entries set over
+ * X = Z = lambdaStar I (sparse: "target block i j value", target 1 = Z,
+ * otherwise X, blocks numbered over the SDP blocks and one past them for LP
+ * entry i; dense: Z then X in full, each SDP block then the LP entries).  A
+ * value is a decimal (read as QD reads it) or an exact pair HI+LO of two
+ * doubles as sdpadd_write_solution writes.  The solve then starts from
+ * that point: mu0 = X.Z / n and SolveInfo's initial-point form. */
+int sdpadd_read_initial_point(sdpadd_problem p, const char *path, int sparse);
+/* The same from host arrays in the layout and sign of sdpadd_solution
+ * (exact: a solution set back reproduces it bit for bit). */
+int sdpadd_set_initial_point(sdpadd_problem p, const double *X, const double *y, const double *Z);
+/* Back to the default start X = Z = lambdaStar I, y = 0. */
+int sdpadd_clear_initial_point(sdpadd_problem p);
+/* Writes the last solve's final point as a sparse initial-point file with
+ * every value as the exact pair HI+LO (%.17e each): read back with
+ * sdpadd_read_initial_point it restores the point bit for bit; the
+ * reference's reader (QD's, which stops after HI's exponent) reads HI.
+ * This extends the reference's 17-digit text output (sdpa_io.cpp:799-831,
+ * 1015-1203), which cannot carry a dd point. */
+int sdpadd_write_solution(sdpadd_problem p, const char *path);
+
 /* ---- Adjoint -------------------------------------------------------------- */
 /* No counterpart in the reference (which has no derivatives): implicit
  * differentiation of the central-path equations at the final iterate of the

@@ -247,6 +247,107 @@ bool read_dat_s(const std::string &path, InputData &data, std::string &err) {
     return true;
 }
 
+// A value of an initial-point file: the exact pair HI+LO / HI-LO of two
+// doubles, or a decimal read as QD reads it.
+static dd_real parse_init_value(const std::string &tok) {
+    const char *s = tok.c_str();
+    char *e1 = nullptr;
+    double hi = std::strtod(s, &e1);
+    if (e1 != s && (*e1 == '+' || *e1 == '-')) {
+        char *e2 = nullptr;
+        double lo = std::strtod(e1, &e2);
+        if (e2 != e1 && *e2 == '\0') return dd_real(hi, lo);
+    }
+    return parse_value(tok);
+}
+
+bool read_init(const std::string &path, const InputData &data, const Parameter &param, bool sparse, InitPoint &ip,
+               std::string &err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    std::string buf = ss.str();
+    Scan sc(buf);
+    const std::vector<int> &bs = data.blockStruct;
+    const int nB = (int)bs.size(), nLP = data.LP_nBlock;
+    std::vector<size_t> boff(nB + 1, 0);
+    for (int l = 0; l < nB; l++) boff[l + 1] = boff[l] + (size_t)bs[l] * bs[l];
+    const size_t nt = boff[nB] + nLP;
+    // Solutions::initialize: X = Z = lambda I
+    const dd_real lambda = param.lambdaStar;
+    ip.x.assign(nt, dd_real(0.0));
+    for (int l = 0; l < nB; l++)
+        for (int i = 0; i < bs[l]; i++) ip.x[boff[l] + (size_t)i * bs[l] + i] = lambda;
+    for (int i = 0; i < nLP; i++) ip.x[boff[nB] + i] = lambda;
+    ip.z = ip.x;
+    ip.y.assign(data.m, dd_real(0.0));
+    for (int k = 0; k < data.m; k++) {  // yVec is read with the opposite sign
+        std::string t;
+        if (!sc.sep_token(t)) {
+            err = "initial point: short y vector";
+            return false;
+        }
+        ip.y[k] = -parse_init_value(t);
+    }
+    auto set_sdp = [&](std::vector<dd_real> &v, int l, int i, int j, const dd_real &val) {  // setElement_SDP
+        v[boff[l] + i + (size_t)bs[l] * j] = val;
+        v[boff[l] + j + (size_t)bs[l] * i] = val;
+    };
+    if (sparse) {
+        for (;;) {
+            int target, l, i, j;
+            std::string t;
+            if (!sc.sep_int(target) || !sc.sep_int(l) || !sc.sep_int(i) || !sc.sep_int(j) || !sc.sep_token(t)) break;
+            double chk;
+            if (std::sscanf(t.c_str(), "%lf", &chk) <= 0) break;
+            dd_real val = parse_init_value(t);
+            std::vector<dd_real> &v = (target == 1) ? ip.z : ip.x;
+            if (l >= 1 && l <= nB) {
+                if (i < 1 || j < 1 || i > bs[l - 1] || j > bs[l - 1]) {
+                    err = "initial point: entry out of range";
+                    return false;
+                }
+                set_sdp(v, l - 1, i - 1, j - 1, val);
+            } else {
+                if (i != j || i < 1 || i > nLP || l < 1) {
+                    err = "initial point: bad LP entry";
+                    return false;
+                }
+                v[boff[nB] + i - 1] = val;
+            }
+        }
+    } else {
+        for (int pass = 0; pass < 2; pass++) {  // Z, then X
+            std::vector<dd_real> &v = pass ? ip.x : ip.z;
+            for (int l = 0; l < nB; l++)
+                for (int i = 0; i < bs[l]; i++)
+                    for (int j = 0; j < bs[l]; j++) {
+                        std::string t;
+                        if (!sc.sep_token(t)) {
+                            err = "initial point: short dense block";
+                            return false;
+                        }
+                        dd_real val = parse_init_value(t);
+                        if (i <= j && val != 0.0) set_sdp(v, l, i, j, val);
+                    }
+            for (int j = 0; j < nLP; j++) {
+                std::string t;
+                if (!sc.sep_token(t)) {
+                    err = "initial point: short LP part";
+                    return false;
+                }
+                dd_real val = parse_init_value(t);
+                if (val != 0.0) v[boff[nB] + j] = val;
+            }
+        }
+    }
+    return true;
+}
+
 // sdpa_dataset.cpp:254-330
 // and 395-465 (the LP entries' constraint lists)
 void InputData::initialize_index() {
@@ -789,7 +890,7 @@ dd_real min_eigen(const DenseLinearSpace &L, const DenseLinearSpace &M, bool &ok
 }
 }  // namespace
 
-bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
+bool solve(const InputData &data, const Parameter &param, SolveResult &res, const InitPoint *init) {
     const int m = data.m;
     const std::vector<int> &bs = data.blockStruct;
     // Newton and work space (sdpa_newton.cpp:191-223, sdpa_dataset.cpp:668-702).
@@ -823,6 +924,23 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
     pt.invzMat.setIdentity(1.0 / lambda);
     pt.yVec.assign(m, dd_real(0.0));
     const int nDim = pt.nDim;
+    // sdpa_main.cpp:237-251: the initial point, its inverse factors (the
+    // reference does not check computeInverse's result here) and copies
+    DenseLinearSpace initX, initZ;
+    auto load = [&](DenseLinearSpace &d, const std::vector<dd_real> &v) {
+        size_t o = 0;
+        for (auto &b : d.SDP_block)
+            for (auto &e : b.de_ele) e = v[o++];
+        for (auto &e : d.LP_block) e = v[o++];
+    };
+    if (init) {
+        load(pt.xMat, init->x);
+        load(pt.zMat, init->z);
+        pt.yVec = init->y;
+        (void)compute_inverse(pt, work);
+        initX = pt.xMat;
+        initZ = pt.zMat;
+    }
 
     Residuals initRes;
     initRes.primalVec.assign(m, dd_real(0.0));
@@ -835,6 +953,11 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
     dd_real betaV = param.betaStar;                           // DirectionParameter
     bool switchOn = true;                                     // Switch
     dd_real muInitial = dd_real(param.lambdaStar) * dd_real(param.lambdaStar), muCurrent = muInitial;
+    if (init) {  // AverageComplementarity::initialize(Solutions&) (sdpa_parts.cpp:583-588)
+        muInitial = inner(pt.xMat, pt.zMat);
+        muInitial /= nDim;
+        muCurrent = muInitial;
+    }
     dd_real thetaP, thetaD;                                   // RatioInitResCurrentRes
     {
         dd_real accuracy = param.epsilonDash;
@@ -1077,7 +1200,7 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
         // update_exact (sdpa_parts.cpp:639-651)
         thetaP = (initRes.normPrimalVec > 1.0e-10) ? cur.normPrimalVec / initRes.normPrimalVec : dd_real(0.0);
         thetaD = (initRes.normDualMat > 1.0e-10) ? cur.normDualMat / initRes.normDualMat : dd_real(0.0);
-        // SolveInfo::update without an initial point (sdpa_parts.cpp:743-790)
+        // SolveInfo::update (sdpa_parts.cpp:743-790; with an initial point 691-740)
         {
             objValPrimal = inner(data.C, pt.xMat);
             objValDual = inner(data.b, pt.yVec);
@@ -1085,16 +1208,21 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res) {
             rho = 0.0;
             dd_real x0z0 = nDim * muInitial, xMatzMat = nDim * muCurrent, x0zMat = 0.0, xMatz0 = 0.0;
             double lam = param.lambdaStar;
-            for (size_t b = 0; b < bs.size(); b++) {
-                int dim = bs[b];
-                for (int i = 0; i < dim; i++) {
-                    x0zMat += lam * pt.zMat.SDP_block[b].de_ele[(size_t)i * dim + i];
-                    xMatz0 += lam * pt.xMat.SDP_block[b].de_ele[(size_t)i * dim + i];
+            if (init) {  // the initial-point form (sdpa_parts.cpp:691-706)
+                x0zMat = inner(initX, pt.zMat);
+                xMatz0 = inner(pt.xMat, initZ);
+            } else {
+                for (size_t b = 0; b < bs.size(); b++) {
+                    int dim = bs[b];
+                    for (int i = 0; i < dim; i++) {
+                        x0zMat += lam * pt.zMat.SDP_block[b].de_ele[(size_t)i * dim + i];
+                        xMatz0 += lam * pt.xMat.SDP_block[b].de_ele[(size_t)i * dim + i];
+                    }
                 }
-            }
-            for (int b = 0; b < data.LP_nBlock; b++) {
-                x0zMat += lam * pt.zMat.LP_block[b];
-                xMatz0 += lam * pt.xMat.LP_block[b];
+                for (int b = 0; b < data.LP_nBlock; b++) {
+                    x0zMat += lam * pt.zMat.LP_block[b];
+                    xMatz0 += lam * pt.xMat.LP_block[b];
+                }
             }
             dd_real accuracy = param.epsilonDash;
             if (cur.normPrimalVec <= accuracy && xMatz0 < etaPrimal) etaPrimal = xMatz0;

@@ -104,8 +104,23 @@ struct SolveResult {
     std::string message;
 };
 
-// Runs the solver; maxIter < 0 keeps the parameter's value.
-bool solve(const InputData &data, const Parameter &param, SolveResult &res);
+// An initial point (sdpa_main.cpp:237-251): X and Z over the whole space
+// (SDP blocks concatenated column-major, then the LP entries), y in the
+// solver's sign (the file's y negated).
+struct InitPoint {
+    std::vector<dd_real> x, z, y;
+};
+// Reads an initial-point file the way IO::read does (sdpa_io.cpp:152-285):
+// X = Z = lambdaStar I first, then the file's y (negated) and entries
+// (sparse: "target block i j value", target 1 = Z, else X; dense: Z's
+// blocks then X's, full n x n each, then the LP entries).  A value token
+// may also be the exact pair HI+LO / HI-LO of two doubles (the full-dd dump
+// format; QD's reader stops after HI's exponent).
+bool read_init(const std::string &path, const InputData &data, const Parameter &param, bool sparse, InitPoint &ip,
+               std::string &err);
+
+// Runs the solver from lambdaStar I, or from *init.
+bool solve(const InputData &data, const Parameter &param, SolveResult &res, const InitPoint *init = nullptr);
 
 // Pieces exposed to tests.
 void schur_dense(const InputData &data, const DenseLinearSpace &X, const DenseLinearSpace &invZ,

@@ -2,6 +2,7 @@
 // Command-line driver: runs the CPU restatement on a .dat-s file and prints
 // the result as JSON, every dd value as a [hi, lo] pair with 17 digits.
 //   sdpa_oracle <data.dat-s> [param.sdpa] [--dot-threads T] [--max-iter N] [--dump-solution]
+//               [--init file [--init-dense]]  (initial point: sparse format unless --init-dense)
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -20,11 +21,14 @@ int main(int argc, char **argv) {
     }
     std::string data_path = argv[1], param_path;
     int max_iter = -1;
-    bool dump = false;
+    bool dump = false, init_dense = false;
+    std::string init_path;
     for (int i = 2; i < argc; i++) {
         if (!std::strcmp(argv[i], "--dot-threads") && i + 1 < argc) g_dot_threads = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "--max-iter") && i + 1 < argc) max_iter = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "--dump-solution")) dump = true;
+        else if (!std::strcmp(argv[i], "--init") && i + 1 < argc) init_path = argv[++i];
+        else if (!std::strcmp(argv[i], "--init-dense")) init_dense = true;
         else param_path = argv[i];
     }
     InputData data;
@@ -39,9 +43,14 @@ int main(int argc, char **argv) {
         return 1;
     }
     if (max_iter >= 0) param.maxIteration = max_iter;
+    InitPoint ip;
+    if (!init_path.empty() && !read_init(init_path, data, param, !init_dense, ip, err)) {
+        std::printf("{\"error\": \"%s\"}\n", err.c_str());
+        return 1;
+    }
     SolveResult res;
     auto t0 = std::chrono::steady_clock::now();
-    solve(data, param, res);
+    solve(data, param, res, init_path.empty() ? nullptr : &ip);
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     static const char *names[] = {"noINFO", "pFEAS", "dFEAS", "pdFEAS", "pdINF", "pFEAS_dINF", "pINF_dFEAS", "pdOPT", "pUNBD", "dUNBD"};
     std::printf("{\"m\": %d, \"iterations\": %d, \"phase\": \"%s\", \"phase_internal\": \"%s\", \"seconds\": %.6f,\n",

@@ -24,6 +24,8 @@ struct sdpadd_problem_s {
     std::vector<sdp::IterRecord> trace;
     std::vector<dd> X, y, Z;
     sdp::Adjoint adj;
+    sdp::InitPoint init;
+    bool has_init = false;
 };
 
 namespace {
@@ -484,7 +486,7 @@ int sdpadd_solve(sdpadd_problem p, const sdpadd_params *prm, sdpadd_result *res)
     d.gammaStar = prm->gammaStar;
     d.epsilonDash = prm->epsilonDash;
     sdp::SolveOutput out;
-    int rc = sdp::solve(p->h->c, p->hp, p->dp, d, out, &p->X, &p->y, &p->Z);
+    int rc = sdp::solve(p->h->c, p->hp, p->dp, d, out, &p->X, &p->y, &p->Z, p->has_init ? &p->init : nullptr);
     if (rc != 0) return rc;
     p->trace = out.trace;
     res->iterations = out.iterations;
@@ -559,6 +561,45 @@ int sdpadd_adjoint(sdpadd_problem p, int nrhs, const double *d_Xbar, const doubl
                           DP(d_Cbar), DP(d_Abar), err);
     if (rc != 0) p->h->c.set_error(err);
     return rc;
+}
+int sdpadd_read_initial_point(sdpadd_problem p, const char *path, int sparse) {
+    if (!p || !p->finalized || !path) return -1;
+    std::string err;
+    sdp::InitPoint ip;
+    if (!sdp::read_init(path, p->hp, sparse != 0, ip, err)) {
+        p->h->c.set_error(err);
+        return -1;
+    }
+    p->init = std::move(ip);
+    p->has_init = true;
+    return 0;
+}
+int sdpadd_set_initial_point(sdpadd_problem p, const double *X, const double *y, const double *Z) {
+    if (!p || !p->finalized || !X || !y || !Z) return -1;
+    const sdp::idx_t nt = p->hp.nt();
+    sdp::InitPoint ip;
+    ip.full = true;
+    ip.x.assign(DP(X), DP(X) + nt);
+    ip.z.assign(DP(Z), DP(Z) + nt);
+    ip.y.assign(DP(y), DP(y) + p->hp.m);
+    p->init = std::move(ip);
+    p->has_init = true;
+    return 0;
+}
+int sdpadd_clear_initial_point(sdpadd_problem p) {
+    if (!p) return -1;
+    p->init = sdp::InitPoint();
+    p->has_init = false;
+    return 0;
+}
+int sdpadd_write_solution(sdpadd_problem p, const char *path) {
+    if (!p || !path) return -1;
+    std::string err;
+    if (!sdp::write_point(path, p->hp, p->X, p->y, p->Z, err)) {
+        p->h->c.set_error(err);
+        return -1;
+    }
+    return 0;
 }
 int sdpadd_solution(sdpadd_problem p, double *X, double *y, double *Z) {
     if (!p || p->X.empty()) return -1;

@@ -1,7 +1,9 @@
 // sdpa_dd_gpu: command-line front end in the shape of the reference's
-// sdpa_dd (sdpa_main.cpp:523-632): -ds data.dat-s [-p param] [-o out].
-// Prints the result as JSON (dd values as [hi, lo] pairs) on stdout and,
-// with -o, the reference-style final block to the output file.
+// sdpa_dd (sdpa_main.cpp:523-632): -ds data.dat-s [-p param] [-o out]
+// [-is init.ini-s | -id init.ini].  Prints the result as JSON (dd values as
+// [hi, lo] pairs) on stdout and, with -o, the reference-style final block to
+// the output file; --dump file writes the final point as a full-dd sparse
+// initial-point file.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -13,18 +15,25 @@
 static void pdd(FILE *f, const double *v) { std::fprintf(f, "[%.17e, %.17e]", v[0], v[1]); }
 
 int main(int argc, char **argv) {
-    std::string data, param, outp;
+    std::string data, param, outp, initf, dump;
+    bool init_sparse = false;
     int dot_chunks = 1024;
     bool trace = false;
     for (int i = 1; i < argc; i++) {
         if (!std::strcmp(argv[i], "-ds") && i + 1 < argc) data = argv[++i];
         else if (!std::strcmp(argv[i], "-p") && i + 1 < argc) param = argv[++i];
         else if (!std::strcmp(argv[i], "-o") && i + 1 < argc) outp = argv[++i];
+        else if (!std::strcmp(argv[i], "-is") && i + 1 < argc) initf = argv[++i], init_sparse = true;
+        else if (!std::strcmp(argv[i], "-id") && i + 1 < argc) initf = argv[++i], init_sparse = false;
+        else if (!std::strcmp(argv[i], "--dump") && i + 1 < argc) dump = argv[++i];
         else if (!std::strcmp(argv[i], "--dot-partition") && i + 1 < argc) dot_chunks = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "--trace")) trace = true;
     }
     if (data.empty()) {
-        std::fprintf(stderr, "usage: %s -ds data.dat-s [-p param.sdpa] [-o out] [--dot-partition T] [--trace]\n", argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s -ds data.dat-s [-p param.sdpa] [-o out] [-is init.ini-s | -id init.ini] [--dump point]"
+                     " [--dot-partition T] [--trace]\n",
+                     argv[0]);
         return 2;
     }
     sdpadd_handle h;
@@ -41,6 +50,10 @@ int main(int argc, char **argv) {
     }
     sdpadd_problem p;
     if (sdpadd_problem_load(h, data.c_str(), &p) != 0) {
+        std::printf("{\"error\": \"%s\"}\n", sdpadd_last_error(h));
+        return 1;
+    }
+    if (!initf.empty() && sdpadd_read_initial_point(p, initf.c_str(), init_sparse) != 0) {
         std::printf("{\"error\": \"%s\"}\n", sdpadd_last_error(h));
         return 1;
     }
@@ -92,6 +105,10 @@ int main(int argc, char **argv) {
             std::fprintf(f, "    main loop time = %.6f\n", r.seconds);
             std::fclose(f);
         }
+    }
+    if (!dump.empty() && sdpadd_write_solution(p, dump.c_str()) != 0) {
+        std::fprintf(stderr, "%s\n", sdpadd_last_error(h));
+        return 1;
     }
     sdpadd_problem_destroy(p);
     sdpadd_destroy(h);

@@ -312,6 +312,171 @@ void HostProblem::index() {
         }
 }
 
+// ------------------------------------------------------- initial points --
+namespace {
+// A value of an initial-point file: the exact pair HI+LO / HI-LO of two
+// doubles, or a decimal read as QD reads it.
+bool init_value(const std::string &t, dd &v) {
+    const char *s = t.c_str();
+    char *e1 = nullptr;
+    double hi = std::strtod(s, &e1);
+    if (e1 != s && (*e1 == '+' || *e1 == '-')) {
+        char *e2 = nullptr;
+        double lo = std::strtod(e1, &e2);
+        if (e2 != e1 && *e2 == '\0') {
+            v = dd{hi, lo};
+            return true;
+        }
+    }
+    return qd_read(s, v);
+}
+}  // namespace
+
+void InitPoint::build(const HostProblem &P, double lambda, std::vector<dd> &X, std::vector<dd> &Y,
+                      std::vector<dd> &Z) const {
+    if (full) {
+        X = x;
+        Y = y;
+        Z = z;
+        return;
+    }
+    // Solutions::initialize (X = Z = lambda I), then the file's settings
+    X.assign(P.nt(), mk(0.0));
+    for (size_t l = 0; l < P.bs.size(); l++)
+        for (int i = 0; i < P.bs[l]; i++) X[P.boff[l] + (idx_t)i * P.bs[l] + i] = mk(lambda);
+    for (int i = 0; i < P.nlp; i++) X[P.nn + i] = mk(lambda);
+    Z = X;
+    Y = y;
+    for (const Set &st : sets) {
+        std::vector<dd> &v = (st.target == 1) ? Z : X;
+        v[st.a] = st.v;
+        v[st.b] = st.v;
+    }
+}
+
+bool read_init(const std::string &path, const HostProblem &P, bool sparse, InitPoint &ip, std::string &err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        err = "cannot open initial-point file " + path;
+        return false;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string buf = ss.str();
+    Cursor c(buf);
+    ip = InitPoint();
+    ip.y.assign(P.m, mk(0.0));
+    for (int k = 0; k < P.m; k++) {  // yVec is read with the opposite sign
+        std::string t;
+        dd v;
+        if (!c.sep() || !c.token(t) || !init_value(t, v)) {
+            err = "initial point: bad y vector";
+            return false;
+        }
+        ip.y[k] = dd_neg(v);
+    }
+    const int nB = (int)P.bs.size();
+    auto pos = [&](int l, int i, int j) { return P.boff[l] + i + (idx_t)P.bs[l] * j; };
+    if (sparse) {
+        for (;;) {
+            int target, l, i, j;
+            std::string t;
+            if (!c.sep() || !c.integer(target) || !c.sep() || !c.integer(l) || !c.sep() || !c.integer(i) || !c.sep() ||
+                !c.integer(j) || !c.sep() || !c.token(t))
+                break;
+            double chk;
+            if (std::sscanf(t.c_str(), "%lf", &chk) <= 0) break;
+            dd v;
+            if (!init_value(t, v)) {
+                err = "initial point: bad value " + t;
+                return false;
+            }
+            if (l >= 1 && l <= nB) {
+                if (i < 1 || j < 1 || i > P.bs[l - 1] || j > P.bs[l - 1]) {
+                    err = "initial point: entry out of range";
+                    return false;
+                }
+                ip.sets.push_back({target, pos(l - 1, i - 1, j - 1), pos(l - 1, j - 1, i - 1), v});
+            } else {
+                if (l < 1 || i != j || i < 1 || i > P.nlp) {
+                    err = "initial point: bad LP entry";
+                    return false;
+                }
+                ip.sets.push_back({target, P.nn + i - 1, P.nn + i - 1, v});
+            }
+        }
+    } else {
+        for (int pass = 0; pass < 2; pass++) {  // Z, then X
+            const int target = pass ? 2 : 1;
+            for (int l = 0; l < nB; l++)
+                for (int i = 0; i < P.bs[l]; i++)
+                    for (int j = 0; j < P.bs[l]; j++) {
+                        std::string t;
+                        dd v;
+                        if (!c.sep() || !c.token(t) || !init_value(t, v)) {
+                            err = "initial point: short dense block";
+                            return false;
+                        }
+                        if (i <= j && !dd_is_zero(v)) ip.sets.push_back({target, pos(l, i, j), pos(l, j, i), v});
+                    }
+            for (int j = 0; j < P.nlp; j++) {
+                std::string t;
+                dd v;
+                if (!c.sep() || !c.token(t) || !init_value(t, v)) {
+                    err = "initial point: short LP part";
+                    return false;
+                }
+                if (!dd_is_zero(v)) ip.sets.push_back({target, P.nn + j, P.nn + j, v});
+            }
+        }
+    }
+    return true;
+}
+
+bool write_point(const std::string &path, const HostProblem &P, const std::vector<dd> &X, const std::vector<dd> &y,
+                 const std::vector<dd> &Z, std::string &err) {
+    if ((idx_t)X.size() != P.nt() || (idx_t)Z.size() != P.nt() || (int)y.size() != P.m) {
+        err = "write_point: no solution of this problem";
+        return false;
+    }
+    FILE *fp = std::fopen(path.c_str(), "w");
+    if (!fp) {
+        err = "cannot write " + path;
+        return false;
+    }
+    auto val = [&](dd v) {  // exact: HI then the signed LO, each a round-trip double
+        std::fprintf(fp, "%.17e%+.17e", v.hi, v.lo);
+    };
+    for (int k = 0; k < P.m; k++) {  // the file's y is the solver's negated
+        std::fputc(' ', fp);  // the readers skip one or more separators before each value
+        val(dd_neg(y[k]));
+    }
+    std::fputc('\n', fp);
+    const int nB = (int)P.bs.size();
+    for (int target = 1; target <= 2; target++) {
+        const std::vector<dd> &v = (target == 1) ? Z : X;
+        for (int l = 0; l < nB; l++) {
+            const int n = P.bs[l];
+            for (int i = 0; i < n; i++)
+                for (int j = i; j < n; j++) {
+                    dd e = v[P.boff[l] + i + (idx_t)n * j];
+                    if (i != j && dd_is_zero(e)) continue;
+                    std::fprintf(fp, "%d %d %d %d ", target, l + 1, i + 1, j + 1);
+                    val(e);
+                    std::fputc('\n', fp);
+                }
+        }
+        for (int i = 0; i < P.nlp; i++) {
+            std::fprintf(fp, "%d %d %d %d ", target, nB + 1, i + 1, i + 1);
+            val(v[P.nn + i]);
+            std::fputc('\n', fp);
+        }
+    }
+    bool ok = std::fclose(fp) == 0;
+    if (!ok) err = "write failed: " + path;
+    return ok;
+}
+
 bool read_params(const std::string &path, Params &p) {
     FILE *fp = std::fopen(path.c_str(), "r");
     if (!fp) return false;

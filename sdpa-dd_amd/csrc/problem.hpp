@@ -74,6 +74,32 @@ struct HostProblem {
 };
 
 bool read_dat_s(const std::string &path, HostProblem &p, std::string &err);
+
+// An initial point (sdpa_main.cpp:237-251).  Either a full point (X and Z
+// over the whole space -- SDP blocks, then LP entries -- and y in the
+// solver's sign), or the settings an initial-point file makes over
+// X = Z = lambdaStar I, applied when a solve knows its lambdaStar.
+struct InitPoint {
+    bool full = false;
+    std::vector<dd> x, z, y;
+    struct Set {
+        int target;  // 1: Z, otherwise X
+        idx_t a, b;  // positions set (b == a on the diagonal and for LP entries)
+        dd v;
+    };
+    std::vector<Set> sets;
+    void build(const HostProblem &P, double lambda, std::vector<dd> &X, std::vector<dd> &Y, std::vector<dd> &Z) const;
+};
+// IO::read for initial points (sdpa_io.cpp:152-285): y (negated), then
+// sparse entries "target block i j value" (block <= the SDP block count: an
+// SDP block; beyond it: LP entry i) or dense Z then X.  A value may also be
+// the exact pair HI+LO / HI-LO of two doubles, as write_point writes it
+// (QD's reader stops after HI's exponent, so the reference reads HI).
+bool read_init(const std::string &path, const HostProblem &P, bool sparse, InitPoint &ip, std::string &err);
+// Writes (X, y, Z) as a sparse initial-point file with full-dd values:
+// every diagonal entry, the nonzero entries above it and every LP entry.
+bool write_point(const std::string &path, const HostProblem &P, const std::vector<dd> &X, const std::vector<dd> &y,
+                 const std::vector<dd> &Z, std::string &err);
 bool qd_read(const char *s, dd &out);
 std::vector<int> compute_formula(const HostProblem &p, ddr kappa);
 

@@ -340,7 +340,8 @@ struct DevBuf {
 
 class Solver {
   public:
-    Solver(Context &c, const HostProblem &h, DeviceProblem &d, const Params &p) : ctx(c), hp(h), dp(d), prm(p) {}
+    Solver(Context &c, const HostProblem &h, DeviceProblem &d, const Params &p, const InitPoint *ip)
+        : ctx(c), hp(h), dp(d), prm(p), init(ip) {}
     ~Solver() {
         if (s2) (void)hipStreamSynchronize(s2);
         for (void *q : mem) (void)hipFree(q);
@@ -362,6 +363,9 @@ class Solver {
     const HostProblem &hp;
     DeviceProblem &dp;
     const Params &prm;
+    const InitPoint *init;
+    dd *initX = nullptr, *initZ = nullptr;  // the initial point's X and Z (kept for SolveInfo)
+    sdk::DotTask *d_tasks_init = nullptr;   // (initX, Z) and (X, initZ) per block
     std::vector<void *> mem;
     hipStream_t s = nullptr;
     int m = 0, nB = 0, nDim = 0;
@@ -448,6 +452,24 @@ class Solver {
         for (int l = 0; l < nB; l++) r += ddr(h[l]);
         for (idx_t l = 0; l < nlp; l++) r += ddr(h[slot_lp + l]);
         return r;
+    }
+    // initX . Z and X . initZ (SolveInfo::update with an initial point,
+    // sdpa_parts.cpp:691-706)
+    void inner_init(ddr &x0z, ddr &xz0) {
+        if (nB) dots(d_tasks_init, 2 * nB);
+        if (nlp) {
+            sdk::lp_prod(s, nlp, initX + nn, Z + nn, d_sc + slot_lp);
+            sdk::lp_prod(s, nlp, X + nn, initZ + nn, d_sc + slot_lp + nlp);
+        }
+        const dd *h = get_scalars(0, slot_lp + 2 * (int)nlp);
+        ddr r[2];
+        for (int q = 0; q < 2; q++) {
+            r[q] = 0.0;
+            for (int l = 0; l < nB; l++) r[q] += ddr(h[q * nB + l]);
+            for (idx_t l = 0; l < nlp; l++) r[q] += ddr(h[slot_lp + q * nlp + l]);
+        }
+        x0z = r[0];
+        xz0 = r[1];
     }
     // X . dZ, dX . Z and dX . dZ with one launch and one synchronisation.
     void inner_dirs(ddr &r0, ddr &r1, ddr &r2) {
@@ -800,6 +822,22 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             }
         if (hipMalloc((void **)&d_tasks_xz, sizeof(sdk::DotTask) * txz.size()) != hipSuccess) return -2;
         mem.push_back(d_tasks_xz);
+        if (init) {
+            if (!(initX = alloc(nt)) || !(initZ = alloc(nt))) return -2;
+            std::vector<sdk::DotTask> ti;
+            for (int q = 0; q < 2; q++)
+                for (int l = 0; l < nB; l++) {
+                    idx_t ne = (idx_t)bsz(l) * bsz(l), o = dp.boff[l];
+                    ti.push_back({(q ? X : initX) + o, (q ? initZ : Z) + o, ne, d_sc + q * nB + l});
+                }
+            if (!ti.empty()) {
+                if (hipMalloc((void **)&d_tasks_init, sizeof(sdk::DotTask) * ti.size()) != hipSuccess) return -2;
+                mem.push_back(d_tasks_init);
+                if (hipMemcpy(d_tasks_init, ti.data(), sizeof(sdk::DotTask) * ti.size(), hipMemcpyHostToDevice) !=
+                    hipSuccess)
+                    return -2;
+            }
+        }
         if (hipMalloc((void **)&d_tasks3, sizeof(sdk::DotTask) * t3.size()) != hipSuccess) return -2;
         mem.push_back(d_tasks3);
         if (hipMemcpy(d_tasks_xz, txz.data(), sizeof(sdk::DotTask) * txz.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -830,6 +868,28 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         sdk::fill(s, nlp, (1.0 / lambda).v, invZ + nn);
     }
     sdk::fill(s, m, mk(0.0), y);
+    if (init) {
+        // sdpa_main.cpp:237-251: the initial point over lambda I, its inverse
+        // factors and copies.  (The reference goes on with whatever a failed
+        // factorisation leaves; this stops with a message.)
+        std::vector<dd> hx, hy, hz;
+        init->build(hp, prm.lambdaStar, hx, hy, hz);
+        if ((idx_t)hx.size() != nt || (idx_t)hz.size() != nt || (int)hy.size() != m) {
+            ctx.set_error("initial point does not match the problem");
+            return -1;
+        }
+        if (hipMemcpyAsync(X, hx.data(), sizeof(dd) * nt, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(Z, hz.data(), sizeof(dd) * nt, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(y, hy.data(), sizeof(dd) * m, hipMemcpyHostToDevice, s) != hipSuccess)
+            return -2;
+        sync();
+        if (!inv_chol_xz()) {
+            ctx.set_error("initial point: X or Z is not positive definite");
+            return -1;
+        }
+        sdk::copy(s, nt, X, initX);
+        sdk::copy(s, nt, Z, initZ);
+    }
 
     ddr initNormP, initNormD, curNormP, curNormD;
     residuals(initNormP, initNormD);
@@ -839,6 +899,11 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     ddr alphaP = 0.0, alphaD = 0.0, betaV = prm.betaStar;
     bool switchOn = true;
     ddr muInitial = ddr(prm.lambdaStar) * ddr(prm.lambdaStar), muCurrent = muInitial;
+    if (init) {  // AverageComplementarity::initialize(Solutions&) (sdpa_parts.cpp:583-588)
+        muInitial = inner_xz();
+        muInitial /= nDim;
+        muCurrent = muInitial;
+    }
     ddr thetaP, thetaD;
     {
         ddr accuracy = prm.epsilonDash;
@@ -1086,13 +1151,16 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         thetaP = (initNormP > 1.0e-10) ? curNormP / initNormP : ddr(0.0);
         thetaD = (initNormD > 1.0e-10) ? curNormD / initNormD : ddr(0.0);
         {
-            // SolveInfo::update without an initial point (sdpa_parts.cpp:743-790)
+            // SolveInfo::update (sdpa_parts.cpp:743-790; with an initial point 691-740)
             objValPrimal = inner_C(X);
             objValDual = inner_b(y);
             ddr primal = thetaP, dual = thetaD, omega = prm.omegaStar;
             rho = 0.0;
             ddr x0z0 = nDim * muInitial, xMatzMat = nDim * muCurrent, x0zMat = 0.0, xMatz0 = 0.0;
-            diag_sums(prm.lambdaStar, x0zMat, xMatz0);
+            if (init)
+                inner_init(x0zMat, xMatz0);
+            else
+                diag_sums(prm.lambdaStar, x0zMat, xMatz0);
             ddr accuracy = prm.epsilonDash;
             if (curNormP <= accuracy && xMatz0 < etaPrimal) etaPrimal = xMatz0;
             if (curNormD <= accuracy && x0zMat < etaDual) etaDual = x0zMat;
@@ -1161,8 +1229,8 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
 }  // namespace
 
 int solve(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const Params &prm, SolveOutput &out, std::vector<dd> *xMat,
-          std::vector<dd> *yVec, std::vector<dd> *zMat) {
-    Solver sv(ctx, hp, dp, prm);
+          std::vector<dd> *yVec, std::vector<dd> *zMat, const InitPoint *init) {
+    Solver sv(ctx, hp, dp, prm, init);
     return sv.run(out, xMat, yVec, zMat);
 }
 

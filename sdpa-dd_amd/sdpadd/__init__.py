@@ -77,6 +77,10 @@ def load_library(path=None):
         "sdpadd_Rsteqr": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_dp, c_dp, c_dp, c_i64, c_dp, ctypes.POINTER(c_i64)]),
         "sdpadd_lanczos_min_eig": (ctypes.c_int, [vp, c_i64, vp, vp, c_dp]),
         "sdpadd_small_min_eig": (ctypes.c_int, [vp, c_i64, vp, vp, c_dp, ctypes.POINTER(ctypes.c_int)]),
+        "sdpadd_read_initial_point": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
+        "sdpadd_set_initial_point": (ctypes.c_int, [vp, c_dp, c_dp, c_dp]),
+        "sdpadd_clear_initial_point": (ctypes.c_int, [vp]),
+        "sdpadd_write_solution": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "sdpadd_inv_chol": (ctypes.c_int, [vp, c_i64, vp, vp, vp, vp]),
         "sdpadd_problem_load": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(vp)]),
         "sdpadd_problem_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -334,6 +338,27 @@ class Problem:
             tr.append(rec)
         out["trace"] = tr
         return out
+
+    def read_initial_point(self, path, sparse=None):
+        """Next solves start from the point in an initial-point file
+        (sparse format if the name ends in '-s', as sdpa_dd's -is/-id)."""
+        if sparse is None:
+            sparse = path.endswith("-s")
+        if self.dev.lib.sdpadd_read_initial_point(self.p, path.encode(), int(bool(sparse))) != 0:
+            raise SdpaddError("read_initial_point: %s" % self.dev.lib.sdpadd_last_error(self.dev.h).decode())
+
+    def set_initial_point(self, X, y, Z):
+        """Next solves start from (X, y, Z): dd host arrays in the layout and
+        sign of solution()."""
+        X, y, Z = (np.ascontiguousarray(a, dtype=np.float64) for a in (X, y, Z))
+        self.dev.check(self.dev.lib.sdpadd_set_initial_point(self.p, _dp(X), _dp(y), _dp(Z)), "set_initial_point")
+
+    def clear_initial_point(self):
+        self.dev.check(self.dev.lib.sdpadd_clear_initial_point(self.p), "clear_initial_point")
+
+    def write_solution(self, path):
+        """The last solve's final point as a full-dd sparse initial-point file."""
+        self.dev.check(self.dev.lib.sdpadd_write_solution(self.p, path.encode()), "write_solution")
 
     def solution(self):
         X = np.zeros((self.nn, 2))

@@ -179,10 +179,13 @@ class Oracle:
         return B
 
     # whole solver -------------------------------------------------------
-    def solve(self, data, param=None, dot_threads=1, max_iter=None, dump=False, timeout=3000):
+    def solve(self, data, param=None, dot_threads=1, max_iter=None, dump=False, timeout=3000, init=None,
+              init_dense=False):
         cmd = [CLI, data]
         if param:
             cmd.append(param)
+        if init:
+            cmd += ["--init", init] + (["--init-dense"] if init_dense else [])
         cmd += ["--dot-threads", str(dot_threads)]
         if max_iter is not None:
             cmd += ["--max-iter", str(max_iter)]
