@@ -60,6 +60,9 @@ int sdpadd_download(sdpadd_handle h, void *dst, const void *d_src, size_t bytes)
  * counterpart in the reference (its ComputeTime buckets are host wall
  * clocks, sdpa_parts.h:46-83). */
 int sdpadd_stats_enable(sdpadd_handle h, int on);
+/* Limits the timed families to a comma-separated list (NULL or "": all);
+ * a family outside it costs no events. */
+int sdpadd_stats_filter(sdpadd_handle h, const char *families);
 int sdpadd_stats_reset(sdpadd_handle h);
 int sdpadd_stats_get(sdpadd_handle h, const char *family, int64_t *launches, double *ms, double *ddflops);
 

@@ -89,6 +89,11 @@ int sdpadd_stats_enable(sdpadd_handle h, int on) {
     sdk::stats_enable(on != 0);
     return 0;
 }
+int sdpadd_stats_filter(sdpadd_handle h, const char *families) {
+    if (!h) return -1;
+    sdk::stats_filter(families);
+    return 0;
+}
 int sdpadd_stats_reset(sdpadd_handle h) {
     if (!h) return -1;
     sdk::stats_reset();

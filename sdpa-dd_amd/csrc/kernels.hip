@@ -21,16 +21,22 @@
 namespace sdk {
 
 // ------------------------------------------------------------- statistics --
+// Per-family launch counts, algorithmic dd-flops and HIP-event time.  Family
+// names are string literals; entries are found by name in a short vector.
+// An optional filter (comma-separated names) limits the timed families, so
+// that a timed run pays for the events of the kernel it measures only.
 namespace {
 struct StatEntry {
+    std::string name;
     int64_t launches = 0;
     double flops = 0, ms = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
 };
 bool g_stats_on = false;
-std::map<std::string, StatEntry> g_stats;
+std::string g_filter;  // empty: every family
+std::vector<StatEntry> g_stats;
 std::vector<hipEvent_t> g_event_pool;
-std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> g_open;
+std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> g_open;
 hipEvent_t get_event() {
     if (!g_event_pool.empty()) {
         hipEvent_t e = g_event_pool.back();
@@ -51,36 +57,58 @@ void resolve(StatEntry &st) {
     }
     st.pending.clear();
 }
+int find_entry(const char *family) {
+    for (size_t i = 0; i < g_stats.size(); i++)
+        if (g_stats[i].name == family) return (int)i;
+    return -1;
+}
+bool selected(const char *family) {
+    if (g_filter.empty()) return true;
+    const size_t n = std::strlen(family);
+    for (size_t p = 0; p <= g_filter.size();) {
+        size_t q = g_filter.find(',', p);
+        if (q == std::string::npos) q = g_filter.size();
+        if (q - p == n && g_filter.compare(p, n, family) == 0) return true;
+        p = q + 1;
+    }
+    return false;
+}
 }  // namespace
 
 void stats_enable(bool on) { g_stats_on = on; }
+void stats_filter(const char *families) { g_filter = families ? families : ""; }
 void stats_reset() {
-    for (auto &kv : g_stats) resolve(kv.second);
+    for (auto &st : g_stats) resolve(st);
     g_stats.clear();
 }
 bool stats_get(const char *family, int64_t *launches, double *ms, double *ddflops) {
-    auto it = g_stats.find(family);
-    if (it == g_stats.end()) {
+    int i = find_entry(family);
+    if (i < 0) {
         *launches = 0;
         *ms = 0;
         *ddflops = 0;
         return false;
     }
-    resolve(it->second);
-    *launches = it->second.launches;
-    *ms = it->second.ms;
-    *ddflops = it->second.flops;
+    resolve(g_stats[i]);
+    *launches = g_stats[i].launches;
+    *ms = g_stats[i].ms;
+    *ddflops = g_stats[i].flops;
     return true;
 }
 StatScope::StatScope(const char *family, hipStream_t stream, double ddflops) {
-    if (!g_stats_on) return;
+    if (!g_stats_on || !selected(family)) return;
     s = stream;
-    StatEntry &st = g_stats[family];
-    st.launches++;
-    st.flops += ddflops;
+    int i = find_entry(family);
+    if (i < 0) {
+        g_stats.push_back(StatEntry());
+        g_stats.back().name = family;
+        i = (int)g_stats.size() - 1;
+    }
+    g_stats[i].launches++;
+    g_stats[i].flops += ddflops;
     hipEvent_t a = get_event(), b = get_event();
     (void)hipEventRecord(a, s);
-    g_open.push_back({family, {a, b}});
+    g_open.push_back({i, {a, b}});
     slot = (int)g_open.size() - 1;
 }
 StatScope::~StatScope() {
@@ -1657,32 +1685,42 @@ void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, c
 // 'N'-type (y_i = sum_j A_ij x_j; kind 2: j <= i, the lower triangle): a
 // 1024-thread workgroup serves 64 rows, wave w summing the w-th sixteenth of
 // the columns; wave 0 merges the parts in order.
+// 'N'-type mat-vec in a fixed split order: 8 rows per 1024-thread
+// workgroup; thread t takes row t % 8 over column group t / 8 (128 groups,
+// each a contiguous column range summed in order), and the 128 group sums
+// of a row meet in a fixed pairwise LDS tree.  A wave loads 8 rows of 8
+// columns per step (eight 128-byte segments).  kind 0: y = alpha A x +
+// beta y; kind 2: y = L x with L lower triangular (columns <= the row).
+constexpr int MV_R = 8, MV_G = 1024 / MV_R;
 __global__ __launch_bounds__(1024) void k_mvf_n(int kind, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *x,
                                                 dd alpha, int ak, dd beta, int bk, dd *y) {
-    __shared__ dd part[16][64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const idx_t r = (idx_t)blockIdx.x * 64 + lane;
-    idx_t c0 = n * wave / 16, c1 = n * (wave + 1) / 16;
+    __shared__ dd part[MV_G][MV_R];
+    const int t = threadIdx.x, rl = t % MV_R, g = t / MV_R;
+    const idx_t r = (idx_t)blockIdx.x * MV_R + rl;
+    idx_t c0 = n * g / MV_G, c1 = n * (g + 1) / MV_G;
     if (kind == 2 && c1 > r + 1) c1 = r + 1;
     dd acc = mk(0.0);
     if (r < m) {
-#pragma unroll 8
+#pragma unroll 4
         for (idx_t c = c0; c < c1; c++) acc = dd_add(acc, dd_mul(A[r + c * lda], x[c]));
     }
-    part[wave][lane] = acc;
+    part[g][rl] = acc;
     __syncthreads();
-    if (wave == 0 && r < m) {
-        dd t = part[0][lane];
 #pragma unroll
-        for (int w = 1; w < 16; w++) t = dd_add(t, part[w][lane]);
+    for (int half = MV_G / 2; half >= 1; half >>= 1) {
+        if (g < half) part[g][rl] = dd_add(part[g][rl], part[g + half][rl]);
+        __syncthreads();
+    }
+    if (g == 0 && r < m) {
+        dd v = part[0][rl];
         if (kind == 0) {
-            t = (ak == 1) ? t : (ak == 2 ? dd_neg(t) : dd_mul(alpha, t));
+            v = (ak == 1) ? v : (ak == 2 ? dd_neg(v) : dd_mul(alpha, v));
             if (bk == 1)
-                t = dd_add(y[r], t);
+                v = dd_add(y[r], v);
             else if (bk == 2)
-                t = dd_add(dd_mul(beta, y[r]), t);
+                v = dd_add(dd_mul(beta, y[r]), v);
         }
-        y[r] = t;
+        y[r] = v;
     }
 }
 // 'T'-type (w_j = sum_{i >= j} L_ij x_i): one wave per output, lanes over i,
@@ -1707,7 +1745,7 @@ void trmv_lower_fast(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda,
     if (trans == 'T' || trans == 't')
         k_mvf_lt<<<cdiv(n, 4), 256, 0, s>>>(n, L, lda, x, out);
     else
-        k_mvf_n<<<cdiv(n, 64), 1024, 0, s>>>(2, n, n, L, lda, x, mk(1.0), 1, mk(0.0), 0, out);
+        k_mvf_n<<<cdiv(n, MV_R), 1024, 0, s>>>(2, n, n, L, lda, x, mk(1.0), 1, mk(0.0), 0, out);
 }
 void gemv_n_fast(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, const dd *x, dd beta, dd *y) {
     if (m <= 0) return;
@@ -1719,7 +1757,7 @@ void gemv_n_fast(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t l
         return;
     }
     StatScope st("mv", s, 2.0 * (double)m * n);
-    k_mvf_n<<<cdiv(m, 64), 1024, 0, s>>>(0, m, n, A, lda, x, alpha, ak, beta, bk, y);
+    k_mvf_n<<<cdiv(m, MV_R), 1024, 0, s>>>(0, m, n, A, lda, x, alpha, ak, beta, bk, y);
 }
 // Dot with a fixed tree: per-thread sums over a grid stride, a workgroup
 // tree, then the last workgroup to finish merges the workgroup sums in

@@ -142,6 +142,8 @@ void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned 
 // default).  Families: "gemm", "potrf", "trsm", "trmm", "trsv", "mv", "dot",
 // "schur", "elementwise".  flops are algorithmic dd operations.
 void stats_enable(bool on);
+// Times only the listed families (comma-separated); null or "" for all.
+void stats_filter(const char *families);
 void stats_reset();
 bool stats_get(const char *family, int64_t *launches, double *ms, double *ddflops);
 struct StatScope {

@@ -22,9 +22,25 @@ bool Work::reserve(idx_t mpad, idx_t npad, idx_t kpad, int S, int esize) {
         cap = want;
         return true;
     };
-    return grow((void **)&sa, cap_a, (size_t)esize * S * mpad * kpad) &&
-           grow((void **)&sb, cap_b, (size_t)esize * S * npad * kpad) &&
-           grow((void **)&ea, cap_ea, sizeof(unsigned) * mpad) && grow((void **)&eb, cap_eb, sizeof(unsigned) * npad);
+    const size_t old_ea = cap_ea, old_eb = cap_eb;
+    bool ok = grow((void **)&sa, cap_a, (size_t)esize * S * mpad * kpad) &&
+              grow((void **)&sb, cap_b, (size_t)esize * S * npad * kpad) &&
+              grow((void **)&ea, cap_ea, sizeof(unsigned) * mpad) && grow((void **)&eb, cap_eb, sizeof(unsigned) * npad);
+    if (cap_ea != old_ea || cap_eb != old_eb) fresh = true;
+    return ok;
+}
+// Clears both exponent arrays (a kernel, not a memset: the stream's work may
+// be captured into a graph by a caller).
+__global__ void k_exp_clear(idx_t rows, unsigned *ex);
+unsigned Work::next_tag(hipStream_t s) {
+    if (fresh || tag >= (1u << 20) - 1) {
+        const idx_t na = (idx_t)(cap_ea / sizeof(unsigned)), nb = (idx_t)(cap_eb / sizeof(unsigned));
+        if (na > 0) k_exp_clear<<<cdiv(na, 256), 256, 0, s>>>(na, ea);
+        if (nb > 0) k_exp_clear<<<cdiv(nb, 256), 256, 0, s>>>(nb, eb);
+        tag = 0;
+        fresh = false;
+    }
+    return ++tag;
 }
 void Work::release() {
     if (sa) (void)hipFree(sa);
@@ -38,13 +54,14 @@ void Work::release() {
 // a zero row); element (r, k) is M[r*rs + k*cs].  Then |M(r,k)| 2^-e < 1.
 // The grid matches k_slice: a thread takes 16 k of one row, the four
 // threads of a row reduce in LDS, and chunks combine with an integer
-// atomicMax on e + EXP_BIAS (order-free; 0 = no nonzero seen).
-constexpr int EXP_BIAS = 4096;
+// atomicMax on the tagged value (tag << 12) | (e + EXP_BIAS) (order-free).
+constexpr int EXP_BIAS = 2048;
 __global__ void k_exp_clear(idx_t rows, unsigned *ex) {
     idx_t r = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r < rows) ex[r] = 0u;
 }
-__global__ __launch_bounds__(256) void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, unsigned *ex) {
+__global__ __launch_bounds__(256) void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, unsigned *ex,
+                                                unsigned tag) {
     __shared__ double part[4][64];
     const int t = threadIdx.x, kq = t >> 6, rl = t & 63;
     const idx_t r = (idx_t)blockIdx.x * 64 + rl;
@@ -63,11 +80,13 @@ __global__ __launch_bounds__(256) void k_rowexp(idx_t rows, idx_t k, const dd *M
         if (mx > 0.0) {
             int e = 0;
             (void)frexp(mx, &e);
-            atomicMax(ex + r, (unsigned)(e + EXP_BIAS));
+            atomicMax(ex + r, (tag << 12) | (unsigned)(e + EXP_BIAS));
         }
     }
 }
-__device__ __forceinline__ int exp_of(unsigned b) { return b ? (int)b - EXP_BIAS : 0; }
+__device__ __forceinline__ int exp_of(unsigned b, unsigned tag) {
+    return (b >> 12) == tag ? (int)(b & 4095u) - EXP_BIAS : 0;
+}
 
 // Digit planes of rows [64 bx, 64 bx + 64) over k chunk by: plane s of row r,
 // chunk kb lives at out[((s nkb + kb) rows_pad + r) 64 + (k mod 64)].
@@ -75,7 +94,7 @@ __device__ __forceinline__ int exp_of(unsigned b) { return b ? (int)b - EXP_BIAS
 // (exact: the remainder is formed with a TwoSum).  Padded rows and k are 0.
 template <int S>
 __global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const unsigned *exps,
-                                               int8_t *out, idx_t rows_pad, int nkb) {
+                                               unsigned tag, int8_t *out, idx_t rows_pad, int nkb) {
     const int t = threadIdx.x, kq = t >> 6, kb = blockIdx.y;
     const idx_t r = (idx_t)blockIdx.x * 64 + (t & 63);
     unsigned wd[S][4];
@@ -83,7 +102,7 @@ __global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M,
     for (int s = 0; s < S; s++)
 #pragma unroll
         for (int q = 0; q < 4; q++) wd[s][q] = 0u;
-    const int e = r < rows ? exp_of(exps[r]) : 0;
+    const int e = r < rows ? exp_of(exps[r], tag) : 0;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const idx_t kk = (idx_t)kb * KC + kq * 16 + q;
@@ -115,7 +134,8 @@ __global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M,
 // level d = s + t of the product accumulates in acc[d] (int32, exact).
 template <int S>
 __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const int8_t *SA, idx_t mpad, const unsigned *EA,
-                                              const int8_t *SB, idx_t npad, const unsigned *EB, dd *C, idx_t ldc, int alpha,
+                                              unsigned ta, const int8_t *SB, idx_t npad, const unsigned *EB, unsigned tb,
+                                              dd *C, idx_t ldc, int alpha,
                                               int beta, int lower) {
     __shared__ v4i LA[S][64][4];
     __shared__ v4i LB[S][64][4];
@@ -167,7 +187,7 @@ __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const i
                 h.lo *= 0.0078125;
                 h = dd_add_d(h, (double)acc[d][ct][r]);
             }
-            const int ex = exp_of(EA[i]) + exp_of(EB[j]) - 14;
+            const int ex = exp_of(EA[i], ta) + exp_of(EB[j], tb) - 14;
             h.hi = ldexp(h.hi, ex);
             h.lo = ldexp(h.lo, ex);
             if (alpha < 0) h = dd_neg(h);
@@ -182,22 +202,23 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
                    idx_t brs, idx_t bcs, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
     const int nkb = (int)cdiv(k, KC);
     const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
-    k_exp_clear<<<cdiv(m, 256), 256, 0, s>>>(m, w.ea);
-    k_rowexp<<<dim3(cdiv(m, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea);
-    k_slice<S><<<dim3(cdiv(mpad, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea, w.sa, mpad, nkb);
+    const unsigned ta = w.next_tag(s);
+    k_rowexp<<<dim3(cdiv(m, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
+    k_slice<S><<<dim3(cdiv(mpad, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, nkb);
     // a product of a matrix with its own transpose (L L^T, L^T L) has the
     // same rows in both operands: one set of planes serves both
     const bool same = (A == B && ars == brs && acs == bcs && m == n);
+    const unsigned tb = same ? ta : w.next_tag(s);
     if (!same) {
-        k_exp_clear<<<cdiv(n, 256), 256, 0, s>>>(n, w.eb);
-        k_rowexp<<<dim3(cdiv(n, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
-        k_slice<S><<<dim3(cdiv(npad, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, w.sb, npad, nkb);
+        k_rowexp<<<dim3(cdiv(n, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
+        k_slice<S><<<dim3(cdiv(npad, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, nkb);
     }
     // timed as family "ozaki" with the product's algorithmic 2mnk dd-flops
     // (the MFMA work is S(S+1)/2 times that in int8 multiply-adds)
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
-    k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, same ? w.sa : w.sb, npad,
-                                                                  same ? w.ea : w.eb, C, ldc, alpha, beta, lower ? 1 : 0);
+    k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, ta, same ? w.sa : w.sb,
+                                                                  npad, same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
+                                                                  lower ? 1 : 0);
 }
 
 
@@ -213,12 +234,13 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // plane s of row r, chunk kb: out[((s nkb + kb) rows_pad + r) KC2 + (k mod KC2)]
 template <int S>
 __global__ __launch_bounds__(512) void k_slice_f64(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs,
-                                                   const unsigned *exps, double *out, idx_t rows_pad, int nkb) {
+                                                   const unsigned *exps, unsigned tag, double *out, idx_t rows_pad,
+                                                   int nkb) {
     const int t = threadIdx.x, kq = t >> 6, kb = blockIdx.y;
     const idx_t r = (idx_t)blockIdx.x * 64 + (t & 63);
     if (r >= rows_pad) return;
     const idx_t kk = (idx_t)kb * KC2 + kq;
-    const int e = r < rows ? exp_of(exps[r]) : 0;
+    const int e = r < rows ? exp_of(exps[r], tag) : 0;
     const dd a = (r < rows && kk < k) ? M[r * rs + kk * cs] : mk(0.0);
     double xh = ldexp(a.hi, -e), xl = ldexp(a.lo, -e);
     constexpr double SC = (double)(1 << FB);
@@ -237,8 +259,9 @@ __global__ __launch_bounds__(512) void k_slice_f64(idx_t rows, idx_t k, const dd
 // 64 x 64 output tile per 512-thread workgroup, wave layout as k_gemm.
 template <int S>
 __global__ __launch_bounds__(512) void k_gemm_f64(idx_t m, idx_t n, int nkb, const double *SA, idx_t mpad,
-                                                  const unsigned *EA, const double *SB, idx_t npad, const unsigned *EB,
-                                                  dd *C, idx_t ldc, int alpha, int beta, int lower) {
+                                                  const unsigned *EA, unsigned ta, const double *SB, idx_t npad,
+                                                  const unsigned *EB, unsigned tb, dd *C, idx_t ldc, int alpha, int beta,
+                                                  int lower) {
     __shared__ double LA[S][64][KC2 + 1];
     __shared__ double LB[S][64][KC2 + 1];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -289,7 +312,7 @@ __global__ __launch_bounds__(512) void k_gemm_f64(idx_t m, idx_t n, int nkb, con
                 h.lo *= STEP;
                 h = dd_add_d(h, acc[d][ct][r]);
             }
-            const int ex = exp_of(EA[i]) + exp_of(EB[j]) - 2 * FB;
+            const int ex = exp_of(EA[i], ta) + exp_of(EB[j], tb) - 2 * FB;
             h.hi = ldexp(h.hi, ex);
             h.lo = ldexp(h.lo, ex);
             if (alpha < 0) h = dd_neg(h);
@@ -305,18 +328,18 @@ static void launch_f64(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const 
     const int nkb = (int)cdiv(k, KC2), nkb64 = (int)cdiv(k, KC);
     const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
     double *sa = (double *)w.sa, *sb = (double *)w.sb;
-    k_exp_clear<<<cdiv(m, 256), 256, 0, s>>>(m, w.ea);
-    k_rowexp<<<dim3(cdiv(m, 64), nkb64), 256, 0, s>>>(m, k, A, ars, acs, w.ea);
-    k_slice_f64<S><<<dim3(cdiv(mpad, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, sa, mpad, nkb);
+    const unsigned ta = w.next_tag(s);
+    k_rowexp<<<dim3(cdiv(m, 64), nkb64), 256, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
+    k_slice_f64<S><<<dim3(cdiv(mpad, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta, sa, mpad, nkb);
     const bool same = (A == B && ars == brs && acs == bcs && m == n);
+    const unsigned tb = same ? ta : w.next_tag(s);
     if (!same) {
-        k_exp_clear<<<cdiv(n, 256), 256, 0, s>>>(n, w.eb);
-        k_rowexp<<<dim3(cdiv(n, 64), nkb64), 256, 0, s>>>(n, k, B, brs, bcs, w.eb);
-        k_slice_f64<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, sb, npad, nkb);
+        k_rowexp<<<dim3(cdiv(n, 64), nkb64), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
+        k_slice_f64<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, sb, npad, nkb);
     }
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
-    k_gemm_f64<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, sa, mpad, w.ea, same ? sa : sb, npad,
-                                                                      same ? w.ea : w.eb, C, ldc, alpha, beta,
+    k_gemm_f64<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, sa, mpad, w.ea, ta, same ? sa : sb,
+                                                                      npad, same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
                                                                       lower ? 1 : 0);
 }
 

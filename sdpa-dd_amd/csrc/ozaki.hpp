@@ -25,11 +25,18 @@ constexpr int MAX_S = 16;     // digit planes (7 bits each)
 constexpr idx_t MAX_K = 8192;  // int32 level sums stay exact up to this k
 
 // Workspace for one product: digit planes and exponents of both operands.
+// Row exponents are stored tagged, (tag << 12) | (e + 2048), with a tag
+// that grows with every operand sliced: a row without a nonzero keeps an
+// older (smaller) tag and reads as exponent 0, so the arrays need no
+// clearing between products, only when they are new or the tag wraps.
 struct Work {
     int8_t *sa = nullptr, *sb = nullptr;
-    unsigned *ea = nullptr, *eb = nullptr;  // biased row exponents
+    unsigned *ea = nullptr, *eb = nullptr;  // tagged row exponents
     size_t cap_a = 0, cap_b = 0, cap_ea = 0, cap_eb = 0;
+    unsigned tag = 0;
+    bool fresh = true;  // ea/eb hold no tagged values yet
     bool reserve(idx_t mpad, idx_t npad, idx_t kpad, int S, int esize = 1);
+    unsigned next_tag(hipStream_t s);
     void release();
 };
 

@@ -353,6 +353,8 @@ class Solver {
         if (h_lp_eig) (void)hipHostFree(h_lp_eig);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        for (hipEvent_t e : {ev_t0, ev_t1, ev_t2})
+            if (e) (void)hipEventDestroy(e);
         if (s2) (void)hipStreamDestroy(s2);
         ozw2.release();
     }
@@ -375,7 +377,7 @@ class Solver {
     // reference's linear-space routines (different roundings from the SDP
     // blocks' Raxpy forms), so whole-space updates split at nn.
     idx_t nlp = 0, nt = 0;
-    int slot_lp = 0;  // 3 nlp scalar slots for the LP terms of inner products
+    int slot_lp = 0;  // first LP-term slot
     std::vector<int> small_blk;  // SDP blocks of size <= 20 (Rsyev step length)
     sdk::SmallEigTask *d_small_tasks = nullptr;
     dd *d_small_out = nullptr, *h_small_out = nullptr;
@@ -388,7 +390,12 @@ class Solver {
     // results; h_sc is their pinned host mirror.  d_diag gathers the X and Z
     // diagonals (2 nDim) for SolveInfo.
     dd *d_sc = nullptr, *h_sc = nullptr, *d_diag = nullptr, *h_diag = nullptr;
-    int slot_C = 0, slot_b = 0, slot_max = 0;
+    // scalar slots: [0, 3 nB) direction products per block, [3 nB, 4 nB)
+    // X . Z per block, [4 nB, 6 nB) the initial-point products; then named
+    // slots; then 6 nlp LP terms (3 direction, 1 X . Z, 2 initial-point)
+    int slot_xz = 0, slot_init = 0, slot_C = 0, slot_b = 0, slot_C2 = 0, slot_b2 = 0, slot_max = 0, slot_max2 = 0,
+        slot_bmax = 0, nslots = 0;
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr, ev_t2 = nullptr;  // Schur build / Cholesky timing
     // Fixed dense inner-product task lists: (X, Z) per block, and (X, dZ),
     // (dX, Z), (dX, dZ) per block; partial sums of their chunks.
     sdk::DotTask *d_tasks_xz = nullptr, *d_tasks3 = nullptr;
@@ -430,6 +437,69 @@ class Solver {
         sync();
         return h_sc + first;
     }
+    // Scalar batches: enqueue several reductions, then one download and one
+    // synchronisation for all of them (fetch_all) -- or a download enqueued
+    // ahead of a later synchronisation (prefetch_all).
+    void prefetch_all() {
+        (void)hipMemcpyAsync(h_sc, d_sc, sizeof(dd) * nslots, hipMemcpyDeviceToHost, s);
+    }
+    void fetch_all() {
+        prefetch_all();
+        sync();
+    }
+    ddr sc(int slot) const { return ddr(h_sc[slot]); }
+    void enq_inner_C(const dd *W, int slot) { sdp::inner_C(s, dp, W, d_sc + slot, ctx.dot_T); }
+    void enq_inner_b(const dd *v, int slot) { dot(m, dp.d_b, v, slot); }
+    void enq_max(const dd *x, idx_t len, int slot) {
+        sdk::max_abs(s, len, x, d_sc + slot, ctx.d_max_work, ctx.d_counter + 1);
+    }
+    // X . dZ, dX . Z, dX . dZ (block dots, then LP terms)
+    void enq_dirs() {
+        if (nB) dots(d_tasks3, 3 * nB);
+        if (nlp) {
+            sdk::lp_prod(s, nlp, X + nn, DzMat + nn, d_sc + slot_lp);
+            sdk::lp_prod(s, nlp, DxMat + nn, Z + nn, d_sc + slot_lp + nlp);
+            sdk::lp_prod(s, nlp, DxMat + nn, DzMat + nn, d_sc + slot_lp + 2 * nlp);
+        }
+    }
+    void read_dirs(ddr &r0, ddr &r1, ddr &r2) const {
+        ddr r[3];
+        for (int q = 0; q < 3; q++) {
+            r[q] = 0.0;
+            for (int l = 0; l < nB; l++) r[q] += sc(q * nB + l);
+            for (idx_t l = 0; l < nlp; l++) r[q] += sc(slot_lp + q * (int)nlp + (int)l);
+        }
+        r0 = r[0];
+        r1 = r[1];
+        r2 = r[2];
+    }
+    void enq_xz() {
+        if (nB) dots(d_tasks_xz, nB);
+        if (nlp) sdk::lp_prod(s, nlp, X + nn, Z + nn, d_sc + slot_lp + 3 * nlp);
+    }
+    ddr read_xz() const {
+        ddr r = 0.0;
+        for (int l = 0; l < nB; l++) r += sc(slot_xz + l);
+        for (idx_t l = 0; l < nlp; l++) r += sc(slot_lp + 3 * (int)nlp + (int)l);
+        return r;
+    }
+    void enq_init() {
+        if (nB) dots(d_tasks_init, 2 * nB);
+        if (nlp) {
+            sdk::lp_prod(s, nlp, initX + nn, Z + nn, d_sc + slot_lp + 4 * nlp);
+            sdk::lp_prod(s, nlp, X + nn, initZ + nn, d_sc + slot_lp + 5 * nlp);
+        }
+    }
+    void read_init(ddr &x0z, ddr &xz0) const {
+        ddr r[2];
+        for (int q = 0; q < 2; q++) {
+            r[q] = 0.0;
+            for (int l = 0; l < nB; l++) r[q] += sc(slot_init + q * nB + l);
+            for (idx_t l = 0; l < nlp; l++) r[q] += sc(slot_lp + (4 + q) * (int)nlp + (int)l);
+        }
+        x0z = r[0];
+        xz0 = r[1];
+    }
 
     // ---- dense linear-space helpers (sdpa_linear.cpp Lal::*) ----
     // DenseLinearSpace inner product X . Z: per-block Rdot (all blocks in one
@@ -445,50 +515,9 @@ class Solver {
     // The LP terms a_l b_l follow the block sums in ascending l
     // (Lal::getInnerProduct, sdpa_linear.cpp:1603-1611).
     ddr inner_xz() {
-        if (nB) dots(d_tasks_xz, nB);
-        if (nlp) sdk::lp_prod(s, nlp, X + nn, Z + nn, d_sc + slot_lp);
-        const dd *h = get_scalars(0, slot_lp + (int)nlp);
-        ddr r = 0.0;
-        for (int l = 0; l < nB; l++) r += ddr(h[l]);
-        for (idx_t l = 0; l < nlp; l++) r += ddr(h[slot_lp + l]);
-        return r;
-    }
-    // initX . Z and X . initZ (SolveInfo::update with an initial point,
-    // sdpa_parts.cpp:691-706)
-    void inner_init(ddr &x0z, ddr &xz0) {
-        if (nB) dots(d_tasks_init, 2 * nB);
-        if (nlp) {
-            sdk::lp_prod(s, nlp, initX + nn, Z + nn, d_sc + slot_lp);
-            sdk::lp_prod(s, nlp, X + nn, initZ + nn, d_sc + slot_lp + nlp);
-        }
-        const dd *h = get_scalars(0, slot_lp + 2 * (int)nlp);
-        ddr r[2];
-        for (int q = 0; q < 2; q++) {
-            r[q] = 0.0;
-            for (int l = 0; l < nB; l++) r[q] += ddr(h[q * nB + l]);
-            for (idx_t l = 0; l < nlp; l++) r[q] += ddr(h[slot_lp + q * nlp + l]);
-        }
-        x0z = r[0];
-        xz0 = r[1];
-    }
-    // X . dZ, dX . Z and dX . dZ with one launch and one synchronisation.
-    void inner_dirs(ddr &r0, ddr &r1, ddr &r2) {
-        if (nB) dots(d_tasks3, 3 * nB);
-        if (nlp) {
-            sdk::lp_prod(s, nlp, X + nn, DzMat + nn, d_sc + slot_lp);
-            sdk::lp_prod(s, nlp, DxMat + nn, Z + nn, d_sc + slot_lp + nlp);
-            sdk::lp_prod(s, nlp, DxMat + nn, DzMat + nn, d_sc + slot_lp + 2 * nlp);
-        }
-        const dd *h = get_scalars(0, slot_lp + 3 * (int)nlp);
-        ddr r[3];
-        for (int q = 0; q < 3; q++) {
-            r[q] = 0.0;
-            for (int l = 0; l < nB; l++) r[q] += ddr(h[q * nB + l]);
-            for (idx_t l = 0; l < nlp; l++) r[q] += ddr(h[slot_lp + q * nlp + l]);
-        }
-        r0 = r[0];
-        r1 = r[1];
-        r2 = r[2];
+        enq_xz();
+        fetch_all();
+        return read_xz();
     }
     ddr inner_C(const dd *W) {
         sdp::inner_C(s, dp, W, d_sc + slot_C, ctx.dot_T);
@@ -568,20 +597,28 @@ class Solver {
         trsv_chk('N', m, bMat, m, dy);
         trsv_chk('T', m, bMat, m, dy);
         if (ctx.path == 0 || ctx.refine_steps == 0) return;
+        // max|r| and max|dy| of each candidate come down together
         residual(dy, rvec);
-        ddr rn = max_abs(rvec, m);
+        enq_max(rvec, m, slot_max);
+        enq_max(dy, m, slot_max2);
+        const dd *h = get_scalars(slot_max, 2);
+        ddr rn = h[0], mdy = h[1];
         const ddr floor_scale = 64.0 * m * ldexp(1.0, -104) * bmax;
         for (int it = 0; it < ctx.refine_steps; it++) {
-            if (!(rn > floor_scale * max_abs(dy, m))) break;
+            if (!(rn > floor_scale * mdy)) break;
             sdk::copy(s, m, dy, dyTry);
             trsv_chk('N', m, bMat, m, rvec);
             trsv_chk('T', m, bMat, m, rvec);
             sdk::axpy(s, m, mk(1.0), rvec, dyTry);
             residual(dyTry, rvec);
-            ddr rt = max_abs(rvec, m);
+            enq_max(rvec, m, slot_max);
+            enq_max(dyTry, m, slot_max2);
+            h = get_scalars(slot_max, 2);
+            ddr rt = h[0], mtry = h[1];
             if (!(rt * 2.0 <= rn)) break;
             sdk::copy(s, m, dyTry, dy);
             rn = rt;
+            mdy = mtry;
             refine_accepted++;
         }
     }
@@ -598,8 +635,9 @@ class Solver {
     }
 
     // ---- pieces of the algorithm ----
-    void residuals(ddr &normP, ddr &normD) {
-        // sdpa_dataset.cpp:625-645
+    // Residuals::compute (sdpa_dataset.cpp:625-645); the norms go to
+    // slot_max (primal) and slot_max2 (dual)
+    void enq_residuals() {
         sdp::inner_A(s, dp, X, primalVec, ctx.dot_T);
         sdk::scal(s, m, mk(-1.0), primalVec);
         sdk::axpy(s, m, mk(1.0), dp.d_b, primalVec);
@@ -607,8 +645,14 @@ class Solver {
         sdk::scal(s, nt, mk(-1.0), dualMat);
         sdp::plus_C(s, dp, dualMat);
         plus_in(dualMat, Z, mk(-1.0));
-        normP = max_abs(primalVec, m);
-        normD = max_abs(dualMat, nt);
+        enq_max(primalVec, m, slot_max);
+        enq_max(dualMat, nt, slot_max2);
+    }
+    void residuals(ddr &normP, ddr &normD) {
+        enq_residuals();
+        const dd *h = get_scalars(slot_max, 2);
+        normP = h[0];
+        normD = h[1];
     }
     // Jal::getInvChol / getInvCholAndInv (sdpa_jordan.cpp:123-170) for every
     // block of A, enqueued on `st` without synchronisation; block l's
@@ -644,20 +688,34 @@ class Solver {
     // Solutions::update's factorisations: X on the main stream and Z on the
     // second one, concurrently.  Returns false when either Cholesky fails.
     // (The results after a failed block are never used: the loop ends.)
-    bool inv_chol_xz() {
-        double t0 = now();
+    // X's factorisation on the main stream, Z's on the second one; work
+    // enqueued on the main stream between start and join overlaps Z's.
+    void enq_inv_chol_start() {
         (void)hipEventRecord(ev_fork, s);
         (void)hipStreamWaitEvent(s2, ev_fork, 0);
         inv_chol_async(s, X, invCholX, DLS1, nullptr, d_infos, d_infos + 2 * nB, ctx.d_rinv);
         inv_chol_async(s2, Z, invCholZ, DLS2, invZ, d_infos + nB, d_infos + 2 * nB + 1, d_rinv2);
+    }
+    void enq_inv_chol_join() {
         (void)hipEventRecord(ev_join, s2);
         (void)hipStreamWaitEvent(s, ev_join, 0);
         (void)hipMemcpyAsync(h_infos, d_infos, sizeof(int64_t) * (2 * nB + 2), hipMemcpyDeviceToHost, s);
-        sync();
-        t_inverse += now() - t0;
+    }
+    void enq_inv_chol_xz() {
+        enq_inv_chol_start();
+        enq_inv_chol_join();
+    }
+    bool inv_chol_ok() const {  // after a synchronisation
         for (int q = 0; q < 2 * nB + (nlp ? 2 : 0); q++)
             if (h_infos[q] != 0) return false;
         return true;
+    }
+    bool inv_chol_xz() {
+        double t0 = now();
+        enq_inv_chol_xz();
+        sync();
+        t_inverse += now() - t0;
+        return inv_chol_ok();
     }
     ChainPool pool;
     // Jal::getMinEigen (sdpa_jordan.cpp:55-88) for (L_X^{-1}, dX) and (L_Z^{-1}, dZ):
@@ -728,9 +786,9 @@ class Solver {
         t_lanczos += now() - t0;
         return ok;
     }
-    void diag_sums(double lam, ddr &x0z, ddr &xz0) {
-        // x0zMat += lambda * Z_ii, xMatz0 += lambda * X_ii, blocks in order
-        // gather both diagonals on the device, one contiguous download
+    // x0zMat += lambda * Z_ii, xMatz0 += lambda * X_ii, blocks in order:
+    // both diagonals gathered on the device, one contiguous download
+    void enq_diag() {
         idx_t off = 0;
         for (int l = 0; l < nB; l++) {
             sdk::gather_diag(s, bsz(l), blk(Z, l), d_diag + off);
@@ -742,7 +800,8 @@ class Solver {
             sdk::copy(s, nlp, X + nn, d_diag + nDim + off);
         }
         (void)hipMemcpyAsync(h_diag, d_diag, sizeof(dd) * 2 * (size_t)nDim, hipMemcpyDeviceToHost, s);
-        sync();
+    }
+    void read_diag(double lam, ddr &x0z, ddr &xz0) const {
         for (int i = 0; i < nDim; i++) {
             x0z += lam * ddr(h_diag[i]);
             xz0 += lam * ddr(h_diag[nDim + i]);
@@ -766,11 +825,19 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     if (!(y = alloc(m)) || !(dy = alloc(m)) || !(g = alloc(m)) || !(primalVec = alloc(m)) ||
         !(bMat = alloc((idx_t)m * m)))
         return -2;
-    slot_C = 3 * nB;
+    slot_xz = 3 * nB;
+    slot_init = 4 * nB;
+    slot_C = 6 * nB;
     slot_b = slot_C + 1;
-    slot_max = slot_C + 2;
-    slot_lp = slot_C + 3;
-    const int nslots = slot_lp + 3 * (int)nlp;
+    slot_C2 = slot_C + 2;
+    slot_b2 = slot_C + 3;
+    slot_max = slot_C + 4;
+    slot_max2 = slot_C + 5;
+    slot_bmax = slot_C + 6;
+    slot_lp = slot_C + 7;
+    nslots = slot_lp + 6 * (int)nlp;
+    for (hipEvent_t *e : {&ev_t0, &ev_t1, &ev_t2})
+        if (hipEventCreate(e) != hipSuccess) return -2;
     if (!(d_sc = alloc(nslots)) || !(d_diag = alloc(2 * (idx_t)nDim))) return -2;
     if (hipHostMalloc((void **)&h_sc, sizeof(dd) * nslots) != hipSuccess) return -2;
     if (hipHostMalloc((void **)&h_diag, sizeof(dd) * 2 * (size_t)nDim) != hipSuccess) return -2;
@@ -813,7 +880,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         for (int l = 0; l < nB; l++) {
             idx_t ne = (idx_t)bsz(l) * bsz(l), o = dp.boff[l];
             max_blk_elems = std::max(max_blk_elems, ne);
-            txz.push_back({X + o, Z + o, ne, d_sc + l});
+            txz.push_back({X + o, Z + o, ne, d_sc + slot_xz + l});
         }
         for (int q = 0; q < 3; q++)
             for (int l = 0; l < nB; l++) {
@@ -828,7 +895,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             for (int q = 0; q < 2; q++)
                 for (int l = 0; l < nB; l++) {
                     idx_t ne = (idx_t)bsz(l) * bsz(l), o = dp.boff[l];
-                    ti.push_back({(q ? X : initX) + o, (q ? initZ : Z) + o, ne, d_sc + q * nB + l});
+                    ti.push_back({(q ? X : initX) + o, (q ? initZ : Z) + o, ne, d_sc + slot_init + q * nB + l});
                 }
             if (!ti.empty()) {
                 if (hipMalloc((void **)&d_tasks_init, sizeof(sdk::DotTask) * ti.size()) != hipSuccess) return -2;
@@ -983,22 +1050,25 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         sdk::scal(s, m, mk(-1.0), g);
         sdk::axpy(s, m, mk(1.0), primalVec, g);
         if (predictor) {
-            sync();
-            double t0 = now();
+            // Schur build and Cholesky timed with events; one synchronisation
+            // brings back the Cholesky's info (and max|B| for the refinement)
+            (void)hipEventRecord(ev_t0, s);
             sdp::schur(s, dp, X, invZ, bMat, ctx.dot_T);
-            sync();
-            double t1 = now();
+            (void)hipEventRecord(ev_t1, s);
             if (ctx.path >= 1) {
                 sdk::copy(s, (idx_t)m * m, bMat, bSave);
-                bmax = max_abs(bSave, (idx_t)m * m);
+                enq_max(bSave, (idx_t)m * m, slot_bmax);
+                (void)hipMemcpyAsync(h_sc + slot_bmax, d_sc + slot_bmax, sizeof(dd), hipMemcpyDeviceToHost, s);
             }
             potrf(s, m, bMat, ctx.d_info, ctx.d_rinv, ctx.ozw);
+            (void)hipEventRecord(ev_t2, s);
             int64_t info = 0;
             (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
             sync();
-            double t2 = now();
-            t_schur += t1 - t0;
-            t_chol += t2 - t1;
+            float ms1 = 0, ms2 = 0;
+            if (hipEventElapsedTime(&ms1, ev_t0, ev_t1) == hipSuccess) t_schur += 1e-3 * ms1;
+            if (hipEventElapsedTime(&ms2, ev_t1, ev_t2) == hipSuccess) t_chol += 1e-3 * ms2;
+            if (ctx.path >= 1) bmax = sc(slot_bmax);
             if (info != 0) return false;
             // the Ozaki form solves with the block-inverse operators of this factor
             binv = ctx.path >= 1 && m >= 512 && ctx.binv_prepare(s, m, bMat, m) == 0;
@@ -1026,12 +1096,18 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             res.message = oz_failed ? "Ozaki workspace allocation failed" : "Schur Cholesky failed";
             break;
         }
-        // StepLength::MehrotraPredictor (sdpa_parts.cpp:307-354)
+        // StepLength::MehrotraPredictor (sdpa_parts.cpp:307-354) and
+        // DirectionParameter::MehrotraCorrector read C . dX, b . dy and the
+        // three direction products: one batch, one synchronisation
+        enq_inner_C(DxMat, slot_C);
+        enq_inner_b(dy, slot_b);
+        enq_dirs();
+        fetch_all();
         alphaP = alphaD = 0.9;
         if (phase == noINFO || phase == dFEAS) {
             if (alphaP > 1.0) alphaP = 1.0;
         } else {
-            ddr inc = inner_C(DxMat);
+            ddr inc = sc(slot_C);
             if (inc > 0.0) {
                 if (alphaP > alphaD) alphaP = alphaD;
                 if (alphaP > 1.0) alphaP = 1.0;
@@ -1040,7 +1116,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         if (phase == noINFO || phase == pFEAS) {
             if (alphaD > 1.0) alphaD = 1.0;
         } else {
-            ddr inc = inner_b(dy);
+            ddr inc = sc(slot_b);
             if (inc < 0.0) {
                 if (alphaD > alphaP) alphaD = alphaP;
                 if (alphaD > 1.0) alphaD = 1.0;
@@ -1049,7 +1125,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         // DirectionParameter::MehrotraCorrector (sdpa_parts.cpp:499-529)
         {
             ddr xv, uz, uv;
-            inner_dirs(xv, uz, uv);
+            read_dirs(xv, uz, uv);
             ddr muTarget = muCurrent + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nDim;
             betaV = muTarget / muCurrent;
             if (betaV < 1.0) betaV = betaV * betaV;
@@ -1067,7 +1143,15 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             const ddr xi = 3.0;
             ddr alphaBD = 100.0;
             ddr mx, mz;
-            bool eig_ok = min_eigen_xz(mx, mz);  // synchronises: the solves' flag is current
+            // every inner product this step may read, downloaded ahead of the
+            // eigenvalue solves' synchronisation
+            enq_inner_C(DxMat, slot_C);
+            enq_inner_b(dy, slot_b);
+            enq_dirs();
+            enq_inner_b(y, slot_b2);
+            enq_inner_C(X, slot_C2);
+            prefetch_all();
+            bool eig_ok = min_eigen_xz(mx, mz);  // synchronises: the solves' flag and h_sc are current
             if (trsv_failed()) {
                 res.message = "triangular solve did not complete";
                 break;
@@ -1083,7 +1167,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             if (phase == noINFO || phase == dFEAS) {
                 if (alphaP > 1.0) alphaP = 1.0;
             } else {
-                ddr inc = inner_C(DxMat);
+                ddr inc = sc(slot_C);
                 if (inc > 0.0) {
                     if (alphaP > alphaD) alphaP = alphaD;
                     if (alphaP > 1.0) alphaP = 1.0;
@@ -1092,7 +1176,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             if (phase == noINFO || phase == pFEAS) {
                 if (alphaD > 1.0) alphaD = 1.0;
             } else {
-                ddr inc = inner_b(dy);
+                ddr inc = sc(slot_b);
                 if (inc < 0.0) {
                     if (alphaD > alphaP) alphaD = alphaP;
                     if (alphaD > 1.0) alphaD = 1.0;
@@ -1100,7 +1184,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             }
             if (switchOn && (phase == noINFO || phase == pFEAS || phase == dFEAS)) {
                 ddr xv, uz, uv;
-                inner_dirs(xv, uz, uv);
+                read_dirs(xv, uz, uv);
                 ddr thetaMax = max((1.0 - alphaP) * thetaP, (1.0 - alphaD) * thetaD);
                 ddr muNew = muCurrent + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nDim;
                 while (thetaMax * thetaMax * muInitial > xi * muNew) {
@@ -1113,11 +1197,11 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
                 }
             }
             if (phase == pdFEAS) {
-                ddr oD = inner_b(y);
-                ddr oP = inner_C(X);
-                ddr incD = inner_b(dy);
+                ddr oD = sc(slot_b2);
+                ddr oP = sc(slot_C2);
+                ddr incD = sc(slot_b);
                 incD *= alphaD;
-                ddr incP = inner_C(DxMat);
+                ddr incP = sc(slot_C);
                 incP *= alphaP;
                 ddr maxRatio = (oD - oP) / (incP - incD);
                 if ((maxRatio > 0.0) && (maxRatio < 1.0)) {
@@ -1137,7 +1221,25 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             res.message = "cannot move";
             break;
         }
-        if (!inv_chol_xz()) {
+        // the new point's inverse factors, mu, residuals, objectives and
+        // SolveInfo products: one batch, one synchronisation (none of the
+        // values is used when a factorisation failed)
+        {
+            double t0 = now();
+            enq_inv_chol_start();
+            enq_xz();
+            enq_residuals();
+            enq_inner_C(X, slot_C2);
+            enq_inner_b(y, slot_b2);
+            if (init)
+                enq_init();
+            else
+                enq_diag();
+            enq_inv_chol_join();
+            fetch_all();
+            t_inverse += now() - t0;
+        }
+        if (!inv_chol_ok()) {
             res.message = "cannot move";
             break;
         }
@@ -1145,22 +1247,23 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             thetaP = abs((1.0 - alphaP) * thetaP);
             thetaD = abs((1.0 - alphaD) * thetaD);
         }
-        muCurrent = inner_xz();
+        muCurrent = read_xz();
         muCurrent /= nDim;
-        residuals(curNormP, curNormD);
+        curNormP = sc(slot_max);
+        curNormD = sc(slot_max2);
         thetaP = (initNormP > 1.0e-10) ? curNormP / initNormP : ddr(0.0);
         thetaD = (initNormD > 1.0e-10) ? curNormD / initNormD : ddr(0.0);
         {
             // SolveInfo::update (sdpa_parts.cpp:743-790; with an initial point 691-740)
-            objValPrimal = inner_C(X);
-            objValDual = inner_b(y);
+            objValPrimal = sc(slot_C2);
+            objValDual = sc(slot_b2);
             ddr primal = thetaP, dual = thetaD, omega = prm.omegaStar;
             rho = 0.0;
             ddr x0z0 = nDim * muInitial, xMatzMat = nDim * muCurrent, x0zMat = 0.0, xMatz0 = 0.0;
             if (init)
-                inner_init(x0zMat, xMatz0);
+                read_init(x0zMat, xMatz0);
             else
-                diag_sums(prm.lambdaStar, x0zMat, xMatz0);
+                read_diag(prm.lambdaStar, x0zMat, xMatz0);
             ddr accuracy = prm.epsilonDash;
             if (curNormP <= accuracy && xMatz0 < etaPrimal) etaPrimal = xMatz0;
             if (curNormD <= accuracy && x0zMat < etaDual) etaDual = x0zMat;

@@ -52,6 +52,7 @@ def load_library(path=None):
         "sdpadd_malloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
         "sdpadd_stats_enable": (ctypes.c_int, [vp, ctypes.c_int]),
         "sdpadd_stats_reset": (ctypes.c_int, [vp]),
+        "sdpadd_stats_filter": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "sdpadd_stats_get": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.POINTER(c_i64), c_dp, c_dp]),
         "sdpadd_free": (ctypes.c_int, [vp, vp]),
         "sdpadd_upload": (ctypes.c_int, [vp, vp, vp, ctypes.c_size_t]),
@@ -170,8 +171,12 @@ class Device:
         self.dot_partition = t
         self.check(self.lib.sdpadd_set_dot_partition(self.h, t), "set_dot_partition")
 
-    def stats(self, on=True):
+    def stats(self, on=True, families=None):
+        """Per-family kernel timing on/off (and reset); families: an
+        iterable of family names to time, None for all."""
         self.check(self.lib.sdpadd_stats_enable(self.h, 1 if on else 0), "stats_enable")
+        self.check(self.lib.sdpadd_stats_filter(self.h, ",".join(families).encode() if families else None),
+                   "stats_filter")
         self.check(self.lib.sdpadd_stats_reset(self.h), "stats_reset")
 
     def stats_get(self, family):
