@@ -52,7 +52,7 @@ void Work::release() {
 
 // Row exponents: e_r with max_k |M(r,k).hi| = f 2^e, f in [0.5, 1) (0 for
 // a zero row); element (r, k) is M[r*rs + k*cs].  Then |M(r,k)| 2^-e < 1.
-// The grid matches k_slice: a thread takes 16 k of one row, the four
+// The grid matches k_slice: a thread takes 8 k of one row, the eight
 // threads of a row reduce in LDS, and chunks combine with an integer
 // atomicMax on the tagged value (tag << 12) | (e + EXP_BIAS) (order-free).
 constexpr int EXP_BIAS = 2048;
@@ -60,23 +60,24 @@ __global__ void k_exp_clear(idx_t rows, unsigned *ex) {
     idx_t r = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r < rows) ex[r] = 0u;
 }
-__global__ __launch_bounds__(256) void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, unsigned *ex,
+__global__ __launch_bounds__(512) void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, unsigned *ex,
                                                 unsigned tag) {
-    __shared__ double part[4][64];
+    __shared__ double part[8][64];
     const int t = threadIdx.x, kq = t >> 6, rl = t & 63;
     const idx_t r = (idx_t)blockIdx.x * 64 + rl;
     double mx = 0.0;
     if (r < rows) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const idx_t kk = (idx_t)blockIdx.y * KC + kq * 16 + q;
+        for (int q = 0; q < 8; q++) {
+            const idx_t kk = (idx_t)blockIdx.y * KC + kq * 8 + q;
             if (kk < k) mx = fmax(mx, fabs(M[r * rs + kk * cs].hi));
         }
     }
     part[kq][rl] = mx;
     __syncthreads();
     if (kq == 0 && r < rows) {
-        mx = fmax(fmax(part[0][rl], part[1][rl]), fmax(part[2][rl], part[3][rl]));
+#pragma unroll
+        for (int w = 1; w < 8; w++) mx = fmax(mx, part[w][rl]);
         if (mx > 0.0) {
             int e = 0;
             (void)frexp(mx, &e);
@@ -92,20 +93,21 @@ __device__ __forceinline__ int exp_of(unsigned b, unsigned tag) {
 // chunk kb lives at out[((s nkb + kb) rows_pad + r) 64 + (k mod 64)].
 // x = a 2^-e (|x| < 1); digit d = trunc(128 x) in [-127, 127], x <- 128 x - d
 // (exact: the remainder is formed with a TwoSum).  Padded rows and k are 0.
+typedef int v2i __attribute__((ext_vector_type(2)));
 template <int S>
-__global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const unsigned *exps,
+__global__ __launch_bounds__(512) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const unsigned *exps,
                                                unsigned tag, int8_t *out, idx_t rows_pad, int nkb) {
-    const int t = threadIdx.x, kq = t >> 6, kb = blockIdx.y;
+    const int t = threadIdx.x, kq = t >> 6, kb = blockIdx.y;  // thread: row t % 64, k values 8 kq .. 8 kq + 7
     const idx_t r = (idx_t)blockIdx.x * 64 + (t & 63);
-    unsigned wd[S][4];
+    unsigned wd[S][2];
 #pragma unroll
     for (int s = 0; s < S; s++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) wd[s][q] = 0u;
+        for (int q = 0; q < 2; q++) wd[s][q] = 0u;
     const int e = r < rows ? exp_of(exps[r], tag) : 0;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const idx_t kk = (idx_t)kb * KC + kq * 16 + q;
+    for (int q = 0; q < 8; q++) {
+        const idx_t kk = (idx_t)kb * KC + kq * 8 + q;
         const dd a = (r < rows && kk < k) ? M[r * rs + kk * cs] : mk(0.0);
         double xh = ldexp(a.hi, -e), xl = ldexp(a.lo, -e);
 #pragma unroll
@@ -122,8 +124,8 @@ __global__ __launch_bounds__(256) void k_slice(idx_t rows, idx_t k, const dd *M,
     if (r >= rows_pad) return;
 #pragma unroll
     for (int s = 0; s < S; s++) {
-        v4i v = {(int)wd[s][0], (int)wd[s][1], (int)wd[s][2], (int)wd[s][3]};
-        *(v4i *)(out + ((((idx_t)s * nkb + kb) * rows_pad + r) * KC + kq * 16)) = v;
+        v2i v = {(int)wd[s][0], (int)wd[s][1]};
+        *(v2i *)(out + ((((idx_t)s * nkb + kb) * rows_pad + r) * KC + kq * 8)) = v;
     }
 }
 
@@ -203,15 +205,15 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
     const int nkb = (int)cdiv(k, KC);
     const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
     const unsigned ta = w.next_tag(s);
-    k_rowexp<<<dim3(cdiv(m, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
-    k_slice<S><<<dim3(cdiv(mpad, 64), nkb), 256, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, nkb);
+    k_rowexp<<<dim3(cdiv(m, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
+    k_slice<S><<<dim3(cdiv(mpad, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, nkb);
     // a product of a matrix with its own transpose (L L^T, L^T L) has the
     // same rows in both operands: one set of planes serves both
     const bool same = (A == B && ars == brs && acs == bcs && m == n);
     const unsigned tb = same ? ta : w.next_tag(s);
     if (!same) {
-        k_rowexp<<<dim3(cdiv(n, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
-        k_slice<S><<<dim3(cdiv(npad, 64), nkb), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, nkb);
+        k_rowexp<<<dim3(cdiv(n, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
+        k_slice<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, nkb);
     }
     // timed as family "ozaki" with the product's algorithmic 2mnk dd-flops
     // (the MFMA work is S(S+1)/2 times that in int8 multiply-adds)
@@ -329,12 +331,12 @@ static void launch_f64(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const 
     const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
     double *sa = (double *)w.sa, *sb = (double *)w.sb;
     const unsigned ta = w.next_tag(s);
-    k_rowexp<<<dim3(cdiv(m, 64), nkb64), 256, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
+    k_rowexp<<<dim3(cdiv(m, 64), nkb64), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
     k_slice_f64<S><<<dim3(cdiv(mpad, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta, sa, mpad, nkb);
     const bool same = (A == B && ars == brs && acs == bcs && m == n);
     const unsigned tb = same ? ta : w.next_tag(s);
     if (!same) {
-        k_rowexp<<<dim3(cdiv(n, 64), nkb64), 256, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
+        k_rowexp<<<dim3(cdiv(n, 64), nkb64), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
         k_slice_f64<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, sb, npad, nkb);
     }
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));

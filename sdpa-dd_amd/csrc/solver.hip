@@ -344,6 +344,7 @@ class Solver {
         : ctx(c), hp(h), dp(d), prm(p), init(ip) {}
     ~Solver() {
         if (s2) (void)hipStreamSynchronize(s2);
+        if (s3) (void)hipStreamSynchronize(s3);
         for (void *q : mem) (void)hipFree(q);
         if (h_sc) (void)hipHostFree(h_sc);
         if (h_diag) (void)hipHostFree(h_diag);
@@ -353,10 +354,13 @@ class Solver {
         if (h_lp_eig) (void)hipHostFree(h_lp_eig);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_join3) (void)hipEventDestroy(ev_join3);
         for (hipEvent_t e : {ev_t0, ev_t1, ev_t2})
             if (e) (void)hipEventDestroy(e);
         if (s2) (void)hipStreamDestroy(s2);
+        if (s3) (void)hipStreamDestroy(s3);
         ozw2.release();
+        ozw3.release();
     }
     int run(SolveOutput &out, std::vector<dd> *xo, std::vector<dd> *yo, std::vector<dd> *zo);
 
@@ -402,11 +406,13 @@ class Solver {
     dd *d_partial = nullptr;
     idx_t max_blk_elems = 0;
     // second stream for the Z factorisation, per-block Cholesky info slots
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // X's factorisation runs on s3, Z's on s2, so that the main stream's
+    // residual and inner-product work of the same step overlaps both
+    hipStream_t s2 = nullptr, s3 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
     int64_t *d_infos = nullptr, *h_infos = nullptr;
-    dd *d_rinv2 = nullptr;
-    oz::Work ozw2;            // Ozaki planes for the second stream
+    dd *d_rinv2 = nullptr, *d_rinv3 = nullptr;
+    oz::Work ozw2, ozw3;      // Ozaki planes for the second and third streams
     dd *bSave = nullptr, *rvec = nullptr, *dyTry = nullptr;  // Ozaki form: unfactored B, residual
     int refine_accepted = 0;
     ddr bmax = 0.0;
@@ -663,7 +669,7 @@ class Solver {
             int n = bsz(l);
             idx_t o = dp.boff[l];
             sdk::copy(st, (idx_t)n * n, A + o, work + o);
-            oz::Work &w = st == s ? ctx.ozw : ozw2;
+            oz::Work &w = st == s ? ctx.ozw : (st == s2 ? ozw2 : ozw3);
             potrf(st, n, work + o, infos + l, rinv, w);
             sdk::zero_upper(st, n, work + o);
             // path 2 (dense iterates): the inverse factor's updates and
@@ -678,7 +684,7 @@ class Solver {
                 int n = bsz(l);
                 idx_t o = dp.boff[l];
                 if (ctx.path == 2)
-                    oz_check(oz::gemm(st, st == s ? ctx.ozw : ozw2, 'T', 'N', n, n, n, 1, invChol + o, n, invChol + o,
+                    oz_check(oz::gemm(st, st == s ? ctx.ozw : (st == s2 ? ozw2 : ozw3), 'T', 'N', n, n, n, 1, invChol + o, n, invChol + o,
                                       n, 0, inv + o, n, false, ctx.oz_slices));
                 else
                     sdk::trmm_lltn_self(st, n, invChol + o, n, inv + o, n);
@@ -693,12 +699,15 @@ class Solver {
     void enq_inv_chol_start() {
         (void)hipEventRecord(ev_fork, s);
         (void)hipStreamWaitEvent(s2, ev_fork, 0);
-        inv_chol_async(s, X, invCholX, DLS1, nullptr, d_infos, d_infos + 2 * nB, ctx.d_rinv);
+        (void)hipStreamWaitEvent(s3, ev_fork, 0);
+        inv_chol_async(s3, X, invCholX, DLS1, nullptr, d_infos, d_infos + 2 * nB, d_rinv3);
         inv_chol_async(s2, Z, invCholZ, DLS2, invZ, d_infos + nB, d_infos + 2 * nB + 1, d_rinv2);
     }
     void enq_inv_chol_join() {
         (void)hipEventRecord(ev_join, s2);
         (void)hipStreamWaitEvent(s, ev_join, 0);
+        (void)hipEventRecord(ev_join3, s3);
+        (void)hipStreamWaitEvent(s, ev_join3, 0);
         (void)hipMemcpyAsync(h_infos, d_infos, sizeof(int64_t) * (2 * nB + 2), hipMemcpyDeviceToHost, s);
     }
     void enq_inv_chol_xz() {
@@ -868,11 +877,14 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         if (!(d_lp_eig = alloc(2 * nlp))) return -2;
         if (hipHostMalloc((void **)&h_lp_eig, sizeof(dd) * 2 * nlp) != hipSuccess) return -2;
     }
-    if (!(d_rinv2 = alloc(64))) return -2;
+    if (!(d_rinv2 = alloc(64)) || !(d_rinv3 = alloc(64))) return -2;
     if (ctx.path >= 1 && (!(bSave = alloc((idx_t)m * m)) || !(rvec = alloc(m)) || !(dyTry = alloc(m)))) return -2;
-    if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess) return -2;
+    if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s3, hipStreamNonBlocking) != hipSuccess)
+        return -2;
     if (hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming) != hipSuccess)
         return -2;
     {
         std::vector<sdk::DotTask> txz, t3;
