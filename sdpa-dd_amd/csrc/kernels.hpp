@@ -26,7 +26,18 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
 }  // namespace sdk
 namespace oz { struct Work; }
 namespace sdk {
-int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch, oz::Work &w, int S);
+// Look-ahead for potrf_lower_oz: a helper stream with its own Ozaki
+// workspace and two events (all default priority).
+struct OzLookAhead {
+    hipStream_t hs = nullptr;
+    oz::Work *w2 = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+};
+// la: when given and n > 1024, each panel's trailing update is split into
+// the next panel's columns (on s, first) and the rest (on la->hs), so that
+// the next panel's diagonal and panel solves overlap the bulk of the update.
+int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch, oz::Work &w, int S,
+                   const OzLookAhead *la = nullptr);
 // The Ozaki form's B := L^{-1} (rows below each 256-row panel updated with
 // the Ozaki product).  Returns 0 or the product's error code.
 int inv_lower_oz(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb, oz::Work &w, int S);

@@ -355,6 +355,8 @@ class Solver {
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (ev_join3) (void)hipEventDestroy(ev_join3);
+        if (la.ready) (void)hipEventDestroy(la.ready);
+        if (la.done) (void)hipEventDestroy(la.done);
         for (hipEvent_t e : {ev_t0, ev_t1, ev_t2})
             if (e) (void)hipEventDestroy(e);
         if (s2) (void)hipStreamDestroy(s2);
@@ -408,8 +410,11 @@ class Solver {
     // second stream for the Z factorisation, per-block Cholesky info slots
     // X's factorisation runs on s3, Z's on s2, so that the main stream's
     // residual and inner-product work of the same step overlaps both
+    // (the process has few hardware queues: s2 doubles as the look-ahead
+    // helper of the Schur Cholesky, which runs at another time)
     hipStream_t s2 = nullptr, s3 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
+    sdk::OzLookAhead la;
     int64_t *d_infos = nullptr, *h_infos = nullptr;
     dd *d_rinv2 = nullptr, *d_rinv3 = nullptr;
     oz::Work ozw2, ozw3;      // Ozaki planes for the second and third streams
@@ -568,7 +573,7 @@ class Solver {
     // Cholesky of the Schur matrix or of an X/Z block in the current form.
     void potrf(hipStream_t st, idx_t n, dd *A, int64_t *info, dd *rinv, oz::Work &w) {
         if (ctx.path >= 1)
-            oz_check(sdk::potrf_lower_oz(st, n, A, n, info, rinv, w, ctx.oz_slices));
+            oz_check(sdk::potrf_lower_oz(st, n, A, n, info, rinv, w, ctx.oz_slices, st == s ? &la : nullptr));
         else
             sdk::potrf_lower(st, n, A, n, info, rinv);
     }
@@ -884,8 +889,12 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         return -2;
     if (hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&la.ready, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&la.done, hipEventDisableTiming) != hipSuccess)
         return -2;
+    la.hs = s2;
+    la.w2 = &ozw2;
     {
         std::vector<sdk::DotTask> txz, t3;
         const dd *a3[3] = {X, DxMat, DxMat}, *b3[3] = {DzMat, Z, DzMat};
