@@ -47,9 +47,10 @@ METRIC = "solve time to rel. gap 1e-25 (Ozaki-MFMA vs plain dd HIP); Schur dd-GF
 FAMILIES = ("gemm", "ozaki", "potrf", "trsm", "trmm", "trsv", "mv", "dot")
 # HBM bytes per launch of the dominant kernels on the max-cut workload, from
 # two rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors
-# a 256 MiB copy needs to read its true size); profiles/gemm_hbm_traffic_maxcut800.md
-TRAFFIC_MAXCUT = {"oz::k_gemm": 31.29e6 + 2.83e6, "sdk::k_gemm": 38.84e6 + 2.45e6}
-TRAFFIC_SOURCE = "profiles/gemm_hbm_traffic_maxcut800.md"
+# a 256 MiB copy needs to read its true size); profiles/gemm_hbm_traffic_maxcut800_theta300.md
+TRAFFIC = {"maxcut": {"oz::k_gemm": 31.29e6 + 2.83e6, "sdk::k_gemm": 38.84e6 + 2.45e6},
+           "theta": {"oz::k_gemm": 71.20e6 + 23.49e6}}
+TRAFFIC_SOURCE = "profiles/gemm_hbm_traffic_maxcut800_theta300.md"
 
 
 def workload_files(tmp, workload, gap):
@@ -88,11 +89,11 @@ def cpu_baseline(data, param, iterations):
             "wall_s": round(wall, 2)}
 
 
-def run_form(dev, prob, param, path, steps, warmup, family):
+def run_form(dev, prob, param, path, steps, warmup, family, breakdown=True):
     """Mean solve time over `steps` timed solves, with HIP events on the
     launches of the measured kernel's family only (events on every family
-    cost ~15% of a solve); then one untimed solve with every family timed,
-    for the per-family breakdown."""
+    cost ~15% of a solve); then (breakdown) one untimed solve with every
+    family timed, for the per-family breakdown."""
     dev.set_path(path, OZ_SLICES, 2)
     for _ in range(warmup):
         prob.solve(param)
@@ -103,9 +104,11 @@ def run_form(dev, prob, param, path, steps, warmup, family):
         res = prob.solve(param)
         times.append(time.time() - t0)
     timed = dev.stats_get(family)
-    dev.stats(True)
-    prob.solve(param)
-    fam = {k: dev.stats_get(k) for k in FAMILIES}
+    fam = {}
+    if breakdown:
+        dev.stats(True)
+        prob.solve(param)
+        fam = {k: dev.stats_get(k) for k in FAMILIES}
     dev.stats(False)
     dev.set_path(0)
     return sum(times) / len(times), res, timed, fam
@@ -120,6 +123,8 @@ def main():
     ap.add_argument("--workload", choices=("maxcut", "theta"), default="maxcut")
     ap.add_argument("--gap", type=float, default=1e-25,
                     help="stopping tolerance: relative gap and feasibility (default 1e-25)")
+    ap.add_argument("--no-breakdown", action="store_true",
+                    help="skip the extra untimed solve per form that times every kernel family")
     ap.add_argument("--ozaki-path", type=int, choices=(1, 2), default=None,
                     help="1: Ozaki Cholesky; 2: Ozaki Cholesky and GEMMs (default: 2 for max-cut, 1 for theta)")
     a = ap.parse_args()
@@ -134,11 +139,11 @@ def main():
     dev = sdpadd.Device(0)
     prob = sdpadd.Problem(dev, data)
     n, m = prob.block_struct[0], prob.m
-    plain_s, plain, g, fam_p = run_form(dev, prob, param, 0, a.steps, a.warmup, "gemm")
+    plain_s, plain, g, fam_p = run_form(dev, prob, param, 0, a.steps, a.warmup, "gemm", not a.no_breakdown)
     # Ozaki form: Cholesky trailing updates and (max-cut, dense iterates) the
     # n x n GEMMs on INT8 MFMA; theta keeps its n x n GEMMs in dd
     oz_path = a.ozaki_path if a.ozaki_path else (2 if a.workload == "maxcut" else 1)
-    oz_s, ozk, o, fam_o = run_form(dev, prob, param, oz_path, a.steps, a.warmup, "ozaki")
+    oz_s, ozk, o, fam_o = run_form(dev, prob, param, oz_path, a.steps, a.warmup, "ozaki", not a.no_breakdown)
 
     gemm_dd_tflops = g["ddflops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
     # int8 multiply-adds of the Ozaki kernel: S(S+1)/2 plane products per dd multiply-add
@@ -178,16 +183,16 @@ def main():
             "bound": "mfma", "kernel": "oz::k_gemm (Ozaki dd product, INT8 MFMA v_mfma_i32_16x16x64_i8)",
             "achieved": round(oz_int8_tops, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8 multiply-adds x2)",
             "frac": round(oz_int8_tops / INT8_PEAK_TOPS, 4),
-            "traffic": TRAFFIC_MAXCUT["oz::k_gemm"] if a.workload == "maxcut" else None,
-            "traffic_source": TRAFFIC_SOURCE if a.workload == "maxcut" else None,
+            "traffic": TRAFFIC[a.workload].get("oz::k_gemm"),
+            "traffic_source": TRAFFIC_SOURCE,
             "avg_launch_ms": round(o["ms"] / max(1, o["launches"]), 4),
         },
         "roofline_plain": {
             "bound": "mfma", "kernel": "sdk::k_gemm (dd GEMM on the FP64 VALU)",
             "achieved": round(gemm_dd_tflops * DD_FP64_SLOTS_PER_DDFLOP, 3), "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(gemm_dd_tflops * DD_FP64_SLOTS_PER_DDFLOP / FP64_PEAK_TFLOPS, 4),
-            "traffic": TRAFFIC_MAXCUT["sdk::k_gemm"] if a.workload == "maxcut" else None,
-            "traffic_source": TRAFFIC_SOURCE if a.workload == "maxcut" else None,
+            "traffic": TRAFFIC[a.workload].get("sdk::k_gemm"),
+            "traffic_source": TRAFFIC_SOURCE if TRAFFIC[a.workload].get("sdk::k_gemm") else None,
             "avg_launch_ms": round(g["ms"] / max(1, g["launches"]), 4),
             "dd_tflops": round(gemm_dd_tflops, 4),
         },
