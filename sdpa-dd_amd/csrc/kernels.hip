@@ -2161,6 +2161,49 @@ void lanczos_tail_dev(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold
     StatScope st("dot", s, 6.0 * (double)n);
     k_lanczos_tail<<<1, 256, sizeof(dd) * T, s>>>(n, T, q, qold, w, r, mk(0.0), out, sc);
 }
+// The same step in a fixed split order (the Ozaki form): 256 contiguous
+// chunks, each summed in order, then a pairwise LDS tree for each of the
+// two dots.
+__device__ __forceinline__ dd block_tree_sum(dd v, dd *sm) {
+    sm[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int h = 128; h >= 1; h >>= 1) {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] = dd_add(sm[threadIdx.x], sm[threadIdx.x + h]);
+        __syncthreads();
+    }
+    dd r = sm[0];
+    __syncthreads();
+    return r;
+}
+__global__ __launch_bounds__(256) void k_lanczos_tail_fast(idx_t n, const dd *q, const dd *qold, const dd *w, dd *r,
+                                                           dd *out, dd *sc) {
+    __shared__ dd sm[256];
+    const dd nbeta = dd_neg(sc[2]);
+    const idx_t st = n * threadIdx.x / 256, en = n * (threadIdx.x + 1) / 256;
+    dd acc = mk(0.0);
+    for (idx_t i = st; i < en; i++) acc = dd_add(acc, dd_mul(q[i], w[i]));
+    const dd alpha = block_tree_sum(acc, sm);
+    const dd na = dd_neg(alpha);
+    acc = mk(0.0);
+    for (idx_t i = st; i < en; i++) {
+        dd ri = dd_add(dd_add(w[i], dd_mul(na, q[i])), dd_mul(nbeta, qold[i]));
+        r[i] = ri;
+        acc = dd_add(acc, dd_mul(ri, ri));
+    }
+    const dd rr = block_tree_sum(acc, sm);
+    if (threadIdx.x == 0) {
+        out[0] = alpha;
+        out[1] = rr;
+        const dd beta = dd_sqrt(rr);
+        sc[2] = beta;
+        sc[3] = dd_div(mk(1.0), beta);
+    }
+}
+void lanczos_tail_fast(hipStream_t s, idx_t n, const dd *q, const dd *qold, const dd *w, dd *r, dd *sc, dd *out) {
+    StatScope st("dot", s, 6.0 * (double)n);
+    k_lanczos_tail_fast<<<1, 256, 0, s>>>(n, q, qold, w, r, out, sc);
+}
 
 // Max of |x_i|: per-thread running max, tree merge, last workgroup merges.
 __global__ __launch_bounds__(256) void k_max_abs(idx_t len, const dd *x, dd *out, dd *partial, unsigned *counter) {

@@ -141,6 +141,8 @@ void lanczos_tail(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, co
 // so that several steps run without the host.
 void lanczos_q_dev(hipStream_t s, idx_t n, const dd *sc, const dd *r, dd *q, dd *qold);
 void lanczos_tail_dev(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd *sc, dd *out);
+// The same step with its two dots in a fixed split order (the Ozaki form).
+void lanczos_tail_fast(hipStream_t s, idx_t n, const dd *q, const dd *qold, const dd *w, dd *r, dd *sc, dd *out);
 // Batched GEMM over pointer arrays (device arrays of batch pointers):
 // C[b] := A[b]*B[b] (alpha = 1, beta = 0), order as gemm().
 void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx_t lda, const dd *const *B, idx_t ldb,

@@ -209,7 +209,10 @@ void chain_enqueue(Chain &c, int T, int cnt) {
             sdk::trmv_lower(s, 'N', nDim, c.L, nDim, c.tmp, c.w);
         }
         // alpha = q.w; r = w - alpha q - beta qold; r.r; next beta, 1/beta
-        sdk::lanczos_tail_dev(s, nDim, T, c.q, c.qold, c.w, c.r, sc, c.hist + 2 * (size_t)(c.k + j));
+        if (c.fast)
+            sdk::lanczos_tail_fast(s, nDim, c.q, c.qold, c.w, c.r, sc, c.hist + 2 * (size_t)(c.k + j));
+        else
+            sdk::lanczos_tail_dev(s, nDim, T, c.q, c.qold, c.w, c.r, sc, c.hist + 2 * (size_t)(c.k + j));
     }
     (void)hipMemcpyAsync(c.hh + 2 * (size_t)c.k, c.hist + 2 * (size_t)c.k, sizeof(dd) * 2 * cnt, hipMemcpyDeviceToHost,
                          s);
