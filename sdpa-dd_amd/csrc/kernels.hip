@@ -424,10 +424,13 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
 // B := B * L^{-T} on an nb-column panel (Rtrsm 'R','L','T','N' order with
 // reciprocal scaling: b_rk *= rl_k, then b_rj -= l_jk * b_rk for j > k,
 // skipping l_jk == 0), rows [r0, r0 + nrows).  L is the nb x nb lower block
-// at Lb.  16 rows per workgroup are staged through LDS; each wave owns 4
-// rows, lane j column j of each, and the solved column is broadcast with
-// readlane.
-static constexpr int PR = 16;
+// at Lb.  PR rows per workgroup are staged through LDS; each wave owns
+// PR_W rows, lane j column j of each, and the solved column is broadcast
+// with readlane.  A step costs a wave about 55 FP64 instructions per row
+// (4 cycles each at wave64), so one row per wave spreads a panel over four
+// times as many SIMDs as four rows per wave did, at the same chain latency.
+static constexpr int PR_W = 1;
+static constexpr int PR = 4 * PR_W;
 __global__ __launch_bounds__(256) void k_panel_rltn(dd *B, idx_t ldb, idx_t r0, idx_t nrows, const dd *Lb, idx_t lda, int nb,
                                                     const dd *rinv, const int64_t *info) {
     __shared__ dd Lc[PB * (PB + 1) / 2];
@@ -447,9 +450,9 @@ __global__ __launch_bounds__(256) void k_panel_rltn(dd *B, idx_t ldb, idx_t r0, 
         Tt[r][j] = r < nr ? B[(rb + r) + (idx_t)j * ldb] : mk(0.0);
     }
     __syncthreads();
-    dd x[4];
+    dd x[PR_W];
 #pragma unroll
-    for (int q = 0; q < 4; q++) x[q] = lane < nb ? Tt[4 * wave + q][lane] : mk(0.0);
+    for (int q = 0; q < PR_W; q++) x[q] = lane < nb ? Tt[PR_W * wave + q][lane] : mk(0.0);
     // steps in groups of eight so that the LDS reads of a group are issued
     // ahead of its dependent chain
     for (int k0 = 0; k0 < nb; k0 += 8) {
@@ -461,21 +464,21 @@ __global__ __launch_bounds__(256) void k_panel_rltn(dd *B, idx_t ldb, idx_t r0, 
                 if (lane == k) {
                     const dd r = rl[k];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) x[q] = dd_mul(r, x[q]);
+                    for (int q = 0; q < PR_W; q++) x[q] = dd_mul(r, x[q]);
                 }
-                dd xk[4];
+                dd xk[PR_W];
 #pragma unroll
-                for (int q = 0; q < 4; q++) xk[q] = readlane_dd(x[q], k);
+                for (int q = 0; q < PR_W; q++) xk[q] = readlane_dd(x[q], k);
                 if (lane > k && lane < nb && !dd_is_zero(l)) {
 #pragma unroll
-                    for (int q = 0; q < 4; q++) x[q] = dd_sub(x[q], dd_mul(l, xk[q]));
+                    for (int q = 0; q < PR_W; q++) x[q] = dd_sub(x[q], dd_mul(l, xk[q]));
                 }
             }
         }
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++)
-        if (lane < nb) Tt[4 * wave + q][lane] = x[q];
+    for (int q = 0; q < PR_W; q++)
+        if (lane < nb) Tt[PR_W * wave + q][lane] = x[q];
     __syncthreads();
     for (int e = tid; e < PR * nb; e += 256) {
         int r = e % PR, j = e / PR;
