@@ -298,7 +298,7 @@ void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx
 
 // -------------------------------------------------------------- Cholesky --
 static constexpr int PB = 64;  // Cholesky panel width
-static constexpr int PD_NT = 512;                                  // threads of the diagonal-block kernel
+static constexpr int PD_NT = 1024;                                 // threads of the diagonal-block kernel
 static constexpr int PD_UPD = PD_NT - 64;                          // update threads (waves 1..)
 static constexpr int PD_OWN = (PB * (PB + 1) / 2 + PD_UPD - 1) / PD_UPD;  // elements per update thread
 
