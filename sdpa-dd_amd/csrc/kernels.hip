@@ -326,16 +326,16 @@ __device__ __forceinline__ void tri_rc(int e, int &r, int &c) {
 
 // Diagonal block of the blocked Cholesky (Rpotrf2 order per element:
 // a_ij -= l_jk * l_ik for k ascending, skipping l_jk == 0; l_kk =
-// sqrt(a_kk); l_ik = (1/l_kk) * a_ik).  One workgroup.  Thread 0 is the
-// pivot thread (lane 0 of wave 0, the rest of that wave idles): during step
-// k it applies column k to a_{k+1,k+1} and forms l_{k+1,k+1} and its
-// reciprocal, while waves 1-7 scale column k and apply it to the rest of
-// the trailing triangle (each update thread owns a fixed set of elements;
-// it issues all their LDS reads before the arithmetic).  Two barriers per
-// column.
+// sqrt(a_kk); l_ik = (1/l_kk) * a_ik).  One workgroup, one barrier per
+// column: during step k, column k stays unscaled in LDS and every thread
+// that needs l_ik forms (1/l_kk) * a_ik itself (the same product wherever
+// it is formed).  Thread 0 applies column k to a_{k+1,k+1} and forms
+// l_{k+1,k+1} and its reciprocal; lanes 1-63 of wave 0 write column k's
+// scaled values to the matrix; waves 1-15 apply column k to the rest of the
+// trailing triangle (each update thread owns a fixed set of elements and
+// issues all their LDS reads before the arithmetic).
 __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
     __shared__ dd S[PB][PB + 1];
-    __shared__ dd Lk[PB];
     __shared__ dd rl[PB];
     __shared__ int failed;
     const int tid = threadIdx.x;
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
         int i = e % nb, j = e / nb;
         if (i >= j) S[i][j] = A[(K + i) + (K + j) * lda];
     }
-    // Waves 1-7 own the lower-triangle elements round-robin (row-major
+    // Waves 1.. own the lower-triangle elements round-robin (row-major
     // enumeration); own[q] packs (i, j) as i | j << 8.  Unused slots hold
     // (0, 0), which the update skips like every element with j <= k.
     int own[PD_OWN];
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
             *info = K + k + 1;
         } else {
             dd l = dd_sqrt(a);
-            S[k][k] = l;
+            A[(K + k) + (K + k) * lda] = l;
             rl[k] = dd_div(mk(1.0), l);
         }
     };
@@ -381,42 +381,37 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
     for (int k = 0; k < nb; k++) {
         if (failed) return;
         const dd r = rl[k];
-        if (tid >= 64)
-            for (int i = k + 1 + (tid - 64); i < nb; i += PD_UPD) {
-                dd v = dd_mul(r, S[i][k]);
-                Lk[i] = v;
-                S[i][k] = v;
-            }
-        __syncthreads();
-        const int rem = nb - k - 1;
         const int own_pivot = (k + 1) | ((k + 1) << 8);
         if (tid < 64) {
-            if (tid == 0 && rem > 0) {
-                dd l = Lk[k + 1];
-                if (!dd_is_zero(l)) S[k + 1][k + 1] = dd_sub(S[k + 1][k + 1], dd_mul(l, l));
-                pivot(k + 1);
+            if (tid == 0) {
+                if (k + 1 < nb) {
+                    dd l = dd_mul(r, S[k + 1][k]);
+                    if (!dd_is_zero(l)) S[k + 1][k + 1] = dd_sub(S[k + 1][k + 1], dd_mul(l, l));
+                    pivot(k + 1);
+                }
+            } else if (tid > k && tid < nb) {
+                A[(K + tid) + (K + k) * lda] = dd_mul(r, S[tid][k]);
             }
         } else {
-            dd sv[PD_OWN], lj[PD_OWN], li[PD_OWN];
+            dd sv[PD_OWN], aj[PD_OWN], ai[PD_OWN];
 #pragma unroll
             for (int q = 0; q < PD_OWN; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
                 sv[q] = S[i][j];
-                lj[q] = Lk[j];
-                li[q] = Lk[i];
+                aj[q] = S[j][k];
+                ai[q] = S[i][k];
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < PD_OWN; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
-                if (j > k && own[q] != own_pivot && !dd_is_zero(lj[q])) S[i][j] = dd_sub(sv[q], dd_mul(lj[q], li[q]));
+                if (j > k && own[q] != own_pivot) {
+                    const dd lj = dd_mul(r, aj[q]);
+                    if (!dd_is_zero(lj)) S[i][j] = dd_sub(sv[q], dd_mul(lj, dd_mul(r, ai[q])));
+                }
             }
         }
         __syncthreads();
-    }
-    for (int e = tid; e < nb * nb; e += PD_NT) {
-        int i = e % nb, j = e / nb;
-        if (i >= j) A[(K + i) + (K + j) * lda] = S[i][j];
     }
     for (int k = tid; k < nb; k += PD_NT) rinv[k] = rl[k];
 }
