@@ -316,12 +316,15 @@ __device__ __forceinline__ dd readlane_dd(dd v, int k) {
 }
 // Packed column-major lower triangle of a PB x PB block: (j, k), j >= k.
 __device__ __forceinline__ int lpack(int j, int k) { return k * PB - (k * (k - 1)) / 2 + (j - k); }
-// Row r and column c (c <= r) of element e of a lower triangle, row-wise.
-__device__ __forceinline__ void tri_rc(int e, int &r, int &c) {
-    r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
-    while ((r + 1) * (r + 2) / 2 <= e) r++;
-    while (r * (r + 1) / 2 > e) r--;
-    c = e - r * (r + 1) / 2;
+// Row r and column c (c <= r) of element e of the lower triangle of an
+// nb x nb block, column by column.
+__device__ __forceinline__ void tri_cm(int e, int nb, int &r, int &c) {
+    c = 0;
+    while (e >= nb - c) {
+        e -= nb - c;
+        c++;
+    }
+    r = c + e;
 }
 
 // Diagonal block of the blocked Cholesky (Rpotrf2 order per element:
@@ -332,8 +335,10 @@ __device__ __forceinline__ void tri_rc(int e, int &r, int &c) {
 // it is formed).  Thread 0 applies column k to a_{k+1,k+1} and forms
 // l_{k+1,k+1} and its reciprocal; lanes 1-63 of wave 0 write column k's
 // scaled values to the matrix; waves 1-15 apply column k to the rest of the
-// trailing triangle (each update thread owns a fixed set of elements and
-// issues all their LDS reads before the arithmetic).
+// trailing triangle.  Each update thread owns a fixed set of elements,
+// enumerated column by column, so the finished columns j <= k are whole
+// waves that skip their slot and the work per step follows the trailing
+// triangle; a thread issues all its LDS reads before the arithmetic.
 __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
     __shared__ dd S[PB][PB + 1];
     __shared__ dd rl[PB];
@@ -344,7 +349,7 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
         int i = e % nb, j = e / nb;
         if (i >= j) S[i][j] = A[(K + i) + (K + j) * lda];
     }
-    // Waves 1.. own the lower-triangle elements round-robin (row-major
+    // Waves 1.. own the lower-triangle elements round-robin (column-major
     // enumeration); own[q] packs (i, j) as i | j << 8.  Unused slots hold
     // (0, 0), which the update skips like every element with j <= k.
     int own[PD_OWN];
@@ -356,7 +361,7 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
             const int e = t + PD_UPD * q;
             if (t >= 0 && e < nb * (nb + 1) / 2) {
                 int ri, ci;
-                tri_rc(e, ri, ci);
+                tri_cm(e, nb, ri, ci);
                 own[q] = ri | (ci << 8);
             }
         }
@@ -394,18 +399,22 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
             }
         } else {
             dd sv[PD_OWN], aj[PD_OWN], ai[PD_OWN];
+            bool act[PD_OWN];
 #pragma unroll
             for (int q = 0; q < PD_OWN; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
-                sv[q] = S[i][j];
-                aj[q] = S[j][k];
-                ai[q] = S[i][k];
+                act[q] = j > k && own[q] != own_pivot;
+                if (act[q]) {
+                    sv[q] = S[i][j];
+                    aj[q] = S[j][k];
+                    ai[q] = S[i][k];
+                }
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < PD_OWN; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
-                if (j > k && own[q] != own_pivot) {
+                if (act[q]) {
                     const dd lj = dd_mul(r, aj[q]);
                     if (!dd_is_zero(lj)) S[i][j] = dd_sub(sv[q], dd_mul(lj, dd_mul(r, ai[q])));
                 }
