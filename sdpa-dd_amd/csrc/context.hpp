@@ -42,6 +42,8 @@ struct Context {
     dd *d_binv = nullptr;
     size_t binv_cap = 0;
     sdk::idx_t binv_n = -1;
+    dd *d_binv_lt = nullptr;  // L^T of the factor the block operators belong to
+    size_t binv_lt_cap = 0;
     int binv_prepare(hipStream_t s, sdk::idx_t n, const dd *L, sdk::idx_t lda);
     int trsv_binv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x);
     // Enqueue the copy of the sticky flag into *h_tsw_err.

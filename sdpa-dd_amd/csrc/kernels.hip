@@ -782,11 +782,17 @@ static constexpr unsigned TC_SPIN = 1u << 22;  // spin bound per wait (about a s
 __device__ __forceinline__ int tc_load(const int *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ void tc_store(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
 // Spin until *p >= target; false when the error word is set or the bound
-// runs out (then the error word and the sticky word are set).
+// runs out (then the error word and the sticky word are set).  The spin
+// reads with relaxed agent-scope loads and takes the acquire once, after
+// the flag is seen: an acquire per read would invalidate the L2 of the
+// waiting workgroup's XCD on every spin.
 __device__ bool tc_wait(const int *p, int target, int *err, int *sticky) {
     for (unsigned it = 0; it < TC_SPIN; it++) {
-        if (tc_load(p) >= target) return true;
-        if (tc_load(err) != 0) return false;
+        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return true;
+        }
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
         __builtin_amdgcn_s_sleep(1);
     }
     tc_store(err, 1);
@@ -1174,7 +1180,7 @@ __global__ __launch_bounds__(256) void k_blk_ops(idx_t n, int trans, const dd *L
 // flags, vg and the owner workgroups as in k_trsv_ln_coop / k_trsv_lt_coop;
 // an owner applies sequence block J to its rows in blocks >= J + BI_D + 1.
 __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd *L, idx_t lda, dd *x, const dd *ops,
-                                                   int *flags, int nupd, dd *vg, int *sticky) {
+                                                   int *flags, int nupd, dd *vg, int *sticky, const dd *LT) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = (int)((n + SB - 1) / SB);
     int *err = flags + 1 + nupd;
@@ -1280,7 +1286,8 @@ __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd 
         dd a[SB];
         if (act) {
 #pragma unroll
-            for (int q = 0; q < SB; q++) a[q] = trans ? L[(bp + q) + i * lda] : L[i + (bp + q) * lda];
+            for (int q = 0; q < SB; q++)
+                a[q] = !trans ? L[i + (bp + q) * lda] : LT ? LT[i + (bp + q) * n] : L[(bp + q) + i * lda];
         }
         if (J + 1 > avail) {
             if (tid == 0) {
@@ -1303,10 +1310,32 @@ __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd 
     if (tid == 0) tc_store(flags + 1 + u, nblk);
 }
 
+// lt[c + r n] = L[r + c lda] on the 32 x 32 tiles that meet the lower
+// triangle (tile row >= tile column).
+__global__ __launch_bounds__(256) void k_transpose_lower(idx_t n, const dd *L, idx_t lda, dd *lt) {
+    __shared__ dd T[32][33];
+    const int bi = blockIdx.x, bj = blockIdx.y, tid = threadIdx.x;
+    if (bi < bj) return;
+    const idx_t r0 = (idx_t)bi * 32, c0 = (idx_t)bj * 32;
+    for (int e = tid; e < 32 * 32; e += 256) {
+        const int r = e % 32, c = e / 32;
+        if (r0 + r < n && c0 + c < n) T[r][c] = L[(r0 + r) + (c0 + c) * lda];
+    }
+    __syncthreads();
+    for (int e = tid; e < 32 * 32; e += 256) {
+        const int c = e % 32, r = e / 32;
+        if (r0 + r < n && c0 + c < n) lt[(c0 + c) + (r0 + r) * n] = T[r][c];
+    }
+}
+
 size_t binv_ops_elems(idx_t n) { return (size_t)((n + SB - 1) / SB) * BI_OPS; }
-void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n, dd *ops_t, dd *inv) {
+void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n, dd *ops_t, dd *inv, dd *lt) {
     if (n <= 0) return;
     const int nblk = (int)((n + SB - 1) / SB);
+    if (lt) {
+        const unsigned nt = (unsigned)((n + 31) / 32);
+        k_transpose_lower<<<dim3(nt, nt), 256, 0, s>>>(n, L, lda, lt);
+    }
     StatScope st("trsv", s, 2.0 * nblk * (SB * SB * SB / 3.0 + BI_D * SB * SB * SB));
     k_blk_inv<<<nblk, 64, 0, s>>>(n, 0, L, lda, inv);
     k_blk_ops<<<dim3(nblk, BI_D + 1), 256, 0, s>>>(n, 0, L, lda, inv, ops_n);
@@ -1314,7 +1343,7 @@ void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n
     k_blk_ops<<<dim3(nblk, BI_D + 1), 256, 0, s>>>(n, 1, L, lda, inv, ops_t);
 }
 int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, const dd *ops, void *work,
-              int *sticky) {
+              int *sticky, const dd *lt) {
     if (n <= 0) return 0;
     const int nupd = (int)((n + TC_ROWS - 1) / TC_ROWS);
     const idx_t nblk = (n + SB - 1) / SB;
@@ -1330,10 +1359,10 @@ int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x,
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
     hipError_t e;
     if ((int)grid.x <= cus) {
-        k_trsv_binv<<<grid, block, 0, s>>>(n, tr, L, lda, x, ops, flags, nupd, vg, sticky);
+        k_trsv_binv<<<grid, block, 0, s>>>(n, tr, L, lda, x, ops, flags, nupd, vg, sticky, lt);
         e = hipGetLastError();
     } else {
-        void *args[] = {&n, (void *)&tr, &L, &lda, &x, &ops, &flags, (void *)&nupd, &vg, &sticky};
+        void *args[] = {&n, (void *)&tr, &L, &lda, &x, &ops, &flags, (void *)&nupd, &vg, &sticky, (void *)&lt};
         e = hipLaunchCooperativeKernel((const void *)k_trsv_binv, grid, block, args, 0, s);
     }
     if (e != hipSuccess) {

@@ -56,11 +56,14 @@ int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, 
 // operators of both directions once per factor (ops_n, ops_t:
 // binv_ops_elems(n) dd each; inv: binv_ops_elems(n) dd of scratch);
 // trsv_binv uses the trsv_coop_bytes workspace and the sticky flag as
-// trsv_lower_coop and returns 0, or 1 when the launch fails.
+// trsv_lower_coop and returns 0, or 1 when the launch fails.  lt (n x n,
+// leading dimension n, may be null): prepare stores L^T there (the upper
+// triangle of lt is the lower triangle of L, transposed) and the L^T x
+// solve then reads L by columns of lt instead of along rows of L.
 size_t binv_ops_elems(idx_t n);
-void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n, dd *ops_t, dd *inv);
+void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n, dd *ops_t, dd *inv, dd *lt);
 int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, const dd *ops, void *work,
-              int *sticky);
+              int *sticky, const dd *lt);
 // p[0..cnt) := 0 with a kernel (no memset node in captured graphs).
 void zero_ints(hipStream_t s, int *p, int cnt);
 

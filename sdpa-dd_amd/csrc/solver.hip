@@ -75,14 +75,27 @@ int Context::binv_prepare(hipStream_t s, sdk::idx_t n, const dd *L, sdk::idx_t l
         if (hipMalloc(&d_tsw, need) != hipSuccess) return -1;
         tsw_cap = need;
     }
+    // The transposed copy is an optimisation: without it the L^T x solve
+    // reads L along rows.
+    const size_t need_lt = (size_t)n * n;
+    if (need_lt > binv_lt_cap) {
+        if (d_binv_lt) (void)hipFree(d_binv_lt);
+        d_binv_lt = nullptr;
+        binv_lt_cap = 0;
+        if (hipMalloc((void **)&d_binv_lt, sizeof(dd) * need_lt) == hipSuccess)
+            binv_lt_cap = need_lt;
+        else
+            (void)hipGetLastError();
+    }
     binv_n = n;
-    sdk::trsv_binv_prepare(s, n, L, lda, d_binv, d_binv + e, d_binv + 2 * e);
+    sdk::trsv_binv_prepare(s, n, L, lda, d_binv, d_binv + e, d_binv + 2 * e, d_binv_lt);
     return 0;
 }
 int Context::trsv_binv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x) {
     const size_t e = sdk::binv_ops_elems(n);
     const bool tr = (trans == 'T' || trans == 't');
-    if (n == binv_n && sdk::trsv_binv(s, trans, n, L, lda, x, d_binv + (tr ? e : 0), d_tsw, d_tsw_sticky) == 0)
+    if (n == binv_n &&
+        sdk::trsv_binv(s, trans, n, L, lda, x, d_binv + (tr ? e : 0), d_tsw, d_tsw_sticky, d_binv_lt) == 0)
         return 0;
     sdk::trsv_lower(s, trans, n, L, lda, x);
     return 0;
@@ -99,6 +112,9 @@ void Context::destroy() {
     if (d_binv) (void)hipFree(d_binv);
     d_binv = nullptr;
     binv_cap = 0;
+    if (d_binv_lt) (void)hipFree(d_binv_lt);
+    d_binv_lt = nullptr;
+    binv_lt_cap = 0;
     if (d_tsw_sticky) (void)hipFree(d_tsw_sticky);
     if (h_tsw_err) (void)hipHostFree(h_tsw_err);
     d_tsw = nullptr;
