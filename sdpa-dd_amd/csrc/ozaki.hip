@@ -199,6 +199,67 @@ __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const i
     }
 }
 
+// 32 x 32 output tile per 256-thread workgroup (wave w: the 16 x 16 tile at
+// rows 16 (w % 2), columns 16 (w / 2)), for products whose 64 x 64 grid
+// would leave most CUs idle: a quarter of the LDS, so two workgroups share
+// a CU.  Same staging and recombination per element as k_gemm (the level
+// sums are exact integers, so the results are identical).
+template <int S>
+__global__ __launch_bounds__(256) void k_gemm32(idx_t m, idx_t n, int nkb, const int8_t *SA, idx_t mpad,
+                                                const unsigned *EA, unsigned ta, const int8_t *SB, idx_t npad,
+                                                const unsigned *EB, unsigned tb, dd *C, idx_t ldc, int alpha, int beta,
+                                                int lower) {
+    __shared__ v4i LA[S][32][4];
+    __shared__ v4i LB[S][32][4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const idx_t r0 = (idx_t)blockIdx.x * 32, c0 = (idx_t)blockIdx.y * 32;
+    if (lower && r0 + 31 < c0) return;
+    const int wr = wave & 1, wc = wave >> 1;
+    const int fr = lane & 15, fq = lane >> 4;
+    v4i acc[S];
+#pragma unroll
+    for (int d = 0; d < S; d++) acc[d] = v4i{0, 0, 0, 0};
+    for (int kb = 0; kb < nkb; kb++) {
+        __syncthreads();
+        for (int c = tid; c < S * 128; c += 256) {
+            const int sl = c >> 7, row = (c >> 2) & 31, kq = c & 3;
+            const int sw = kq ^ ((row >> 2) & 3);
+            LA[sl][row][sw] = *(const v4i *)(SA + ((((idx_t)sl * nkb + kb) * mpad + r0 + row) * KC + kq * 16));
+            LB[sl][row][sw] = *(const v4i *)(SB + ((((idx_t)sl * nkb + kb) * npad + c0 + row) * KC + kq * 16));
+        }
+        __syncthreads();
+        const int ar = wr * 16 + fr, br = wc * 16 + fr;
+        v4i af[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) af[s] = LA[s][ar][fq ^ ((ar >> 2) & 3)];
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            const v4i b = LB[t][br][fq ^ ((br >> 2) & 3)];
+#pragma unroll
+            for (int s = 0; s + t < S; s++) acc[s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[s], b, acc[s + t], 0, 0, 0);
+        }
+    }
+    const idx_t j = c0 + wc * 16 + fr;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const idx_t i = r0 + wr * 16 + fq * 4 + r;
+        if (i >= m || j >= n || (lower && i < j)) continue;
+        dd h = mk((double)acc[S - 1][r]);
+#pragma unroll
+        for (int d = S - 2; d >= 0; d--) {
+            h.hi *= 0.0078125;
+            h.lo *= 0.0078125;
+            h = dd_add_d(h, (double)acc[d][r]);
+        }
+        const int ex = exp_of(EA[i], ta) + exp_of(EB[j], tb) - 14;
+        h.hi = ldexp(h.hi, ex);
+        h.lo = ldexp(h.lo, ex);
+        if (alpha < 0) h = dd_neg(h);
+        dd *cp = C + i + j * ldc;
+        *cp = beta ? dd_add(*cp, h) : h;
+    }
+}
+
 template <int S>
 static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *A, idx_t ars, idx_t acs, const dd *B,
                    idx_t brs, idx_t bcs, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
@@ -218,9 +279,17 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
     // timed as family "ozaki" with the product's algorithmic 2mnk dd-flops
     // (the MFMA work is S(S+1)/2 times that in int8 multiply-adds)
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
-    k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, ta, same ? w.sa : w.sb,
-                                                                  npad, same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
-                                                                  lower ? 1 : 0);
+    // 64 x 64 tiles hold a CU each (128 KB of LDS at S = 16): below about
+    // two per CU, the 32 x 32 kernel fills the chip better
+    const idx_t t64 = cdiv(mpad, 64) * cdiv(npad, 64) / (lower ? 2 : 1);
+    if (t64 < 512)
+        k_gemm32<S><<<dim3(cdiv(mpad, 32), cdiv(npad, 32)), 256, 0, s>>>(
+            m, n, nkb, w.sa, mpad, w.ea, ta, same ? w.sa : w.sb, npad, same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
+            lower ? 1 : 0);
+    else
+        k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, ta,
+                                                                      same ? w.sa : w.sb, npad, same ? w.ea : w.eb, tb,
+                                                                      C, ldc, alpha, beta, lower ? 1 : 0);
 }
 
 
