@@ -48,8 +48,8 @@ FAMILIES = ("gemm", "ozaki", "potrf", "trsm", "trmm", "trsv", "mv", "dot")
 # HBM bytes per launch of the dominant kernels on the max-cut workload, from
 # two rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors
 # a 256 MiB copy needs to read its true size); profiles/gemm_hbm_traffic_maxcut800_theta300.md
-TRAFFIC = {"maxcut": {"oz::k_gemm": 31.29e6 + 2.83e6, "sdk::k_gemm": 38.84e6 + 2.45e6},
-           "theta": {"oz::k_gemm": 71.20e6 + 23.49e6}}
+TRAFFIC = {"maxcut": {"oz::k_gemm": 30.52e6 + 2.83e6, "sdk::k_gemm": 38.86e6 + 2.45e6},
+           "theta": {"oz::k_gemm": 70.49e6 + 23.54e6}}
 TRAFFIC_SOURCE = "profiles/gemm_hbm_traffic_maxcut800_theta300.md"
 
 
