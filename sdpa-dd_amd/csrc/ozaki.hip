@@ -56,15 +56,19 @@ void Work::release() {
 // threads of a row reduce in LDS, and chunks combine with an integer
 // atomicMax on the tagged value (tag << 12) | (e + EXP_BIAS) (order-free).
 constexpr int EXP_BIAS = 2048;
+// Row exponents and slicing: a workgroup covers SL_R rows of one 64-wide k
+// chunk with 8 threads per row (8 k values each).  Small row blocks keep
+// the grid wide for the k = 64 Cholesky updates (one chunk).
+constexpr int SL_R = 16, SL_T = SL_R * 8;
 __global__ void k_exp_clear(idx_t rows, unsigned *ex) {
     idx_t r = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r < rows) ex[r] = 0u;
 }
-__global__ __launch_bounds__(512) void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, unsigned *ex,
-                                                unsigned tag) {
-    __shared__ double part[8][64];
-    const int t = threadIdx.x, kq = t >> 6, rl = t & 63;
-    const idx_t r = (idx_t)blockIdx.x * 64 + rl;
+__global__ __launch_bounds__(SL_T) void k_rowexp(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, unsigned *ex,
+                                                 unsigned tag) {
+    __shared__ double part[8][SL_R];
+    const int t = threadIdx.x, kq = t / SL_R, rl = t % SL_R;
+    const idx_t r = (idx_t)blockIdx.x * SL_R + rl;
     double mx = 0.0;
     if (r < rows) {
 #pragma unroll
@@ -89,16 +93,16 @@ __device__ __forceinline__ int exp_of(unsigned b, unsigned tag) {
     return (b >> 12) == tag ? (int)(b & 4095u) - EXP_BIAS : 0;
 }
 
-// Digit planes of rows [64 bx, 64 bx + 64) over k chunk by: plane s of row r,
+// Digit planes of rows [SL_R bx, SL_R bx + SL_R) over k chunk by: plane s of row r,
 // chunk kb lives at out[((s nkb + kb) rows_pad + r) 64 + (k mod 64)].
 // x = a 2^-e (|x| < 1); digit d = trunc(128 x) in [-127, 127], x <- 128 x - d
 // (exact: the remainder is formed with a TwoSum).  Padded rows and k are 0.
 typedef int v2i __attribute__((ext_vector_type(2)));
 template <int S>
-__global__ __launch_bounds__(512) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const unsigned *exps,
-                                               unsigned tag, int8_t *out, idx_t rows_pad, int nkb) {
-    const int t = threadIdx.x, kq = t >> 6, kb = blockIdx.y;  // thread: row t % 64, k values 8 kq .. 8 kq + 7
-    const idx_t r = (idx_t)blockIdx.x * 64 + (t & 63);
+__global__ __launch_bounds__(SL_T) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const unsigned *exps,
+                                                unsigned tag, int8_t *out, idx_t rows_pad, int nkb) {
+    const int t = threadIdx.x, kq = t / SL_R, kb = blockIdx.y;  // thread: row t % SL_R, k values 8 kq .. 8 kq + 7
+    const idx_t r = (idx_t)blockIdx.x * SL_R + (t % SL_R);
     unsigned wd[S][2];
 #pragma unroll
     for (int s = 0; s < S; s++)
@@ -266,15 +270,15 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
     const int nkb = (int)cdiv(k, KC);
     const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
     const unsigned ta = w.next_tag(s);
-    k_rowexp<<<dim3(cdiv(m, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
-    k_slice<S><<<dim3(cdiv(mpad, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, nkb);
+    k_rowexp<<<dim3(cdiv(m, SL_R), nkb), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
+    k_slice<S><<<dim3(cdiv(mpad, SL_R), nkb), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, nkb);
     // a product of a matrix with its own transpose (L L^T, L^T L) has the
     // same rows in both operands: one set of planes serves both
     const bool same = (A == B && ars == brs && acs == bcs && m == n);
     const unsigned tb = same ? ta : w.next_tag(s);
     if (!same) {
-        k_rowexp<<<dim3(cdiv(n, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
-        k_slice<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, nkb);
+        k_rowexp<<<dim3(cdiv(n, SL_R), nkb), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
+        k_slice<S><<<dim3(cdiv(npad, SL_R), nkb), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, nkb);
     }
     // timed as family "ozaki" with the product's algorithmic 2mnk dd-flops
     // (the MFMA work is S(S+1)/2 times that in int8 multiply-adds)
@@ -400,12 +404,12 @@ static void launch_f64(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const 
     const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
     double *sa = (double *)w.sa, *sb = (double *)w.sb;
     const unsigned ta = w.next_tag(s);
-    k_rowexp<<<dim3(cdiv(m, 64), nkb64), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
+    k_rowexp<<<dim3(cdiv(m, SL_R), nkb64), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
     k_slice_f64<S><<<dim3(cdiv(mpad, 64), nkb), 512, 0, s>>>(m, k, A, ars, acs, w.ea, ta, sa, mpad, nkb);
     const bool same = (A == B && ars == brs && acs == bcs && m == n);
     const unsigned tb = same ? ta : w.next_tag(s);
     if (!same) {
-        k_rowexp<<<dim3(cdiv(n, 64), nkb64), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
+        k_rowexp<<<dim3(cdiv(n, SL_R), nkb64), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
         k_slice_f64<S><<<dim3(cdiv(npad, 64), nkb), 512, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, sb, npad, nkb);
     }
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
