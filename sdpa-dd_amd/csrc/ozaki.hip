@@ -98,21 +98,44 @@ __device__ __forceinline__ int exp_of(unsigned b, unsigned tag) {
 // x = a 2^-e (|x| < 1); digit d = trunc(128 x) in [-127, 127], x <- 128 x - d
 // (exact: the remainder is formed with a TwoSum).  Padded rows and k are 0.
 typedef int v2i __attribute__((ext_vector_type(2)));
-template <int S>
-__global__ __launch_bounds__(SL_T) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, const unsigned *exps,
+// ONE (k <= 64, a single chunk): the workgroup holds whole rows, so it also
+// forms their exponents (as k_rowexp: the same maximum, written tagged even
+// for a zero row, which decodes to 0 as an untouched entry does).
+template <int S, bool ONE>
+__global__ __launch_bounds__(SL_T) void k_slice(idx_t rows, idx_t k, const dd *M, idx_t rs, idx_t cs, unsigned *exps,
                                                 unsigned tag, int8_t *out, idx_t rows_pad, int nkb) {
+    __shared__ double part[8][SL_R];
     const int t = threadIdx.x, kq = t / SL_R, kb = blockIdx.y;  // thread: row t % SL_R, k values 8 kq .. 8 kq + 7
-    const idx_t r = (idx_t)blockIdx.x * SL_R + (t % SL_R);
+    const int rl = t % SL_R;
+    const idx_t r = (idx_t)blockIdx.x * SL_R + rl;
     unsigned wd[S][2];
 #pragma unroll
     for (int s = 0; s < S; s++)
 #pragma unroll
         for (int q = 0; q < 2; q++) wd[s][q] = 0u;
-    const int e = r < rows ? exp_of(exps[r], tag) : 0;
+    dd av[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const idx_t kk = (idx_t)kb * KC + kq * 8 + q;
-        const dd a = (r < rows && kk < k) ? M[r * rs + kk * cs] : mk(0.0);
+        av[q] = (r < rows && kk < k) ? M[r * rs + kk * cs] : mk(0.0);
+    }
+    int e = 0;
+    if (ONE) {
+        double mx = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) mx = fmax(mx, fabs(av[q].hi));
+        part[kq][rl] = mx;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < 8; w++) mx = fmax(mx, part[w][rl]);
+        if (mx > 0.0) (void)frexp(mx, &e);
+        if (kq == 0 && r < rows) exps[r] = (tag << 12) | (unsigned)(e + EXP_BIAS);
+    } else {
+        e = r < rows ? exp_of(exps[r], tag) : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const dd a = av[q];
         double xh = ldexp(a.hi, -e), xl = ldexp(a.lo, -e);
 #pragma unroll
         for (int s = 0; s < S; s++) {
@@ -270,15 +293,23 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
     const int nkb = (int)cdiv(k, KC);
     const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
     const unsigned ta = w.next_tag(s);
-    k_rowexp<<<dim3(cdiv(m, SL_R), nkb), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
-    k_slice<S><<<dim3(cdiv(mpad, SL_R), nkb), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, nkb);
+    if (nkb == 1) {
+        k_slice<S, true><<<dim3(cdiv(mpad, SL_R), 1), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, 1);
+    } else {
+        k_rowexp<<<dim3(cdiv(m, SL_R), nkb), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
+        k_slice<S, false><<<dim3(cdiv(mpad, SL_R), nkb), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, nkb);
+    }
     // a product of a matrix with its own transpose (L L^T, L^T L) has the
     // same rows in both operands: one set of planes serves both
     const bool same = (A == B && ars == brs && acs == bcs && m == n);
     const unsigned tb = same ? ta : w.next_tag(s);
     if (!same) {
-        k_rowexp<<<dim3(cdiv(n, SL_R), nkb), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
-        k_slice<S><<<dim3(cdiv(npad, SL_R), nkb), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, nkb);
+        if (nkb == 1) {
+            k_slice<S, true><<<dim3(cdiv(npad, SL_R), 1), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, 1);
+        } else {
+            k_rowexp<<<dim3(cdiv(n, SL_R), nkb), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb);
+            k_slice<S, false><<<dim3(cdiv(npad, SL_R), nkb), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, nkb);
+        }
     }
     // timed as family "ozaki" with the product's algorithmic 2mnk dd-flops
     // (the MFMA work is S(S+1)/2 times that in int8 multiply-adds)
