@@ -1403,12 +1403,20 @@ __global__ __launch_bounds__(256) void k_trsm_llnn_diag(idx_t rb, int nb, idx_t 
     for (idx_t c = (idx_t)blockIdx.x * 4 + wave; c < ncols; c += (idx_t)gridDim.x * 4) {
         dd* col = B + rb + c * ldb;
         dd b = lane < nb ? col[lane] : mk(0.0);
+        // step k's operands are read during step k-1, off the division chain
+        // (lanes at or above nb read unused entries inside Lc)
+        dd lnext = Lc[lpack(lane, 0)], dnext = Lc[lpack(0, 0)];
         for (int k = 0; k < nb; k++) {
+            const dd lk = lnext, dk = dnext;
+            if (k + 1 < nb) {
+                lnext = Lc[lpack(lane > k + 1 ? lane : k + 1, k + 1)];
+                dnext = Lc[lpack(k + 1, k + 1)];
+            }
             const int nz = __builtin_amdgcn_readlane(!dd_is_zero(b), k);
             if (!nz) continue;
-            if (lane == k) b = dd_div(b, Lc[lpack(k, k)]);
+            if (lane == k) b = dd_div(b, dk);
             const dd bk = readlane_dd(b, k);
-            if (lane > k && lane < nb) b = dd_sub(b, dd_mul(bk, Lc[lpack(lane, k)]));
+            if (lane > k && lane < nb) b = dd_sub(b, dd_mul(bk, lk));
         }
         if (lane < nb) col[lane] = b;
     }
