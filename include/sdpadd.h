@@ -49,6 +49,12 @@ int sdpadd_set_dot_partition(sdpadd_handle h, int chunks);
  * accumulated by FP64 MFMA (-5: about 2^-80 normwise, -6 and -7: dd).
  * Default: 0, 16, 2. */
 int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps);
+/* Tile shape of the dd GEMM and Ozaki-product kernels (no counterpart in
+ * the reference): 0 = chosen by grid size (default), 1 = always 32 x 32
+ * output tiles, 2 = always 64 x 64.  Both tilings give every output element
+ * the same operation sequence, so results do not depend on it; process-wide
+ * (it exists so tests and tuning can reach either kernel at any size). */
+int sdpadd_set_tile_policy(sdpadd_handle h, int policy);
 int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr);
 int sdpadd_free(sdpadd_handle h, void *d_ptr);
 int sdpadd_upload(sdpadd_handle h, void *d_dst, const void *src, size_t bytes);
@@ -71,6 +77,14 @@ int sdpadd_stats_get(sdpadd_handle h, const char *family, int64_t *launches, dou
  * mplapack/mpblas_dd.h:38-56 and mplapack/mplapack_dd.h:38-92) for the
  * option combinations SDPA-DD's Newton step uses; other combinations return
  * -1.  alpha and beta point to host dd scalars. */
+
+/* QD's dd_real operators elementwise on n device dd pairs: out_i = a_i op
+ * b_i with op 0 a+b (sloppy add), 1 a-b, 2 a*b, 3 a/b (sloppy divide),
+ * 4 sqrt(a) (Karp), 5 a+=b, 6 a-=b, 7 a*=b, 8 a*hi(b), 9 a/hi(b),
+ * 10 a+hi(b), 11 hi(a)/b (qd/include/qd/dd_inline.h:26-393,
+ * qd/src/dd_real.cpp:47-69 with QD 2.3.23's default configuration).  The
+ * scalar arithmetic every kernel is built from. */
+int sdpadd_dd_vec_op(sdpadd_handle h, int op, int64_t n, const double *d_a, const double *d_b, double *d_out);
 
 /* Rgemm: transa 'N'; transb 'N' or 'T' (mplapack/Rgemm_omp.cpp:41-127). */
 int sdpadd_Rgemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n, int64_t k, const double *alpha,
@@ -177,6 +191,13 @@ int sdpadd_problem_dims(sdpadd_problem p, int *m, int *nblock, int *block_struct
 /* useFormula table of Newton::computeFormula_SDP (sdpa_newton.cpp:694-819):
  * out[i*nsdp + l] in {0: F1, 1: F2, 2: F3}, l over the SDP blocks only. */
 int sdpadd_problem_formula(sdpadd_problem p, int *out);
+/* Algorithmic dd operations (multiplies and adds) of one Schur assembly
+ * with this problem's formula table: per row, A_i Z^{-1} (2 nnz_i n, or
+ * 2 n^3 dense) and F1's X (A_i Z^{-1}) (2 n^3); per visited pair, 2 nnz_j
+ * (F1), 2 n nnz_j or 2 n^2 (F2), 3 nnz_i nnz_j (F3), with nnz =
+ * NonZeroEffect (sdpa_newton.cpp:895-1068).  No counterpart in the
+ * reference; it is the numerator of Schur dd-GFLOP/s. */
+int sdpadd_problem_schur_ddflops(sdpadd_problem p, double *ddflops);
 /* Schur complement of Newton::compute_bMat_dense_SDP (sdpa_newton.cpp:979-1068)
  * for device X and Zinv (blocks concatenated, each n_l x n_l): d_B (m x m)
  * is overwritten. */

@@ -50,6 +50,8 @@ void Rsyrk_LN(mint n, mint k, dd_real alpha, const dd_real *A, mint lda, dd_real
             for (mint i = j; i < n; i++) C[i + j * ldc] = (beta == zero) ? zero : beta * C[i + j * ldc];
         return;
     }
+    // Columns are independent: each C element keeps the serial loop's order.
+#pragma omp parallel for schedule(dynamic, 4) if (n * (k + 1) > 8192)
     for (mint j = 0; j < n; j++) {
         if (beta == zero) {
             for (mint i = j; i < n; i++) C[i + j * ldc] = zero;
@@ -75,17 +77,24 @@ void Rtrsm_RLTN(mint m, mint n, dd_real alpha, const dd_real *A, mint lda, dd_re
             for (mint i = 0; i < m; i++) B[i + j * ldb] = zero;
         return;
     }
-    for (mint k = 0; k < n; k++) {
-        dd_real temp = one / A[k + k * lda];
-        for (mint i = 0; i < m; i++) B[i + k * ldb] = temp * B[i + k * ldb];
-        for (mint j = k + 1; j < n; j++) {
-            if (A[j + k * lda] != zero) {
-                temp = A[j + k * lda];
-                for (mint i = 0; i < m; i++) B[i + j * ldb] = B[i + j * ldb] - temp * B[i + k * ldb];
+    // Rows of B are independent: each thread runs the loop nest on its own
+    // range of rows, so every element sees the serial operation sequence.
+    const mint RB = 16;
+#pragma omp parallel for schedule(dynamic, 1) if (m * n > 8192)
+    for (mint i0 = 0; i0 < m; i0 += RB) {
+        const mint i1 = std::min(m, i0 + RB);
+        for (mint k = 0; k < n; k++) {
+            dd_real temp = one / A[k + k * lda];
+            for (mint i = i0; i < i1; i++) B[i + k * ldb] = temp * B[i + k * ldb];
+            for (mint j = k + 1; j < n; j++) {
+                if (A[j + k * lda] != zero) {
+                    temp = A[j + k * lda];
+                    for (mint i = i0; i < i1; i++) B[i + j * ldb] = B[i + j * ldb] - temp * B[i + k * ldb];
+                }
             }
+            if (alpha != one)
+                for (mint i = i0; i < i1; i++) B[i + k * ldb] = alpha * B[i + k * ldb];
         }
-        if (alpha != one)
-            for (mint i = 0; i < m; i++) B[i + k * ldb] = alpha * B[i + k * ldb];
     }
 }
 
@@ -99,6 +108,8 @@ void Rtrsm_LLNN(mint m, mint n, dd_real alpha, const dd_real *A, mint lda, dd_re
             for (mint i = 0; i < m; i++) B[i + j * ldb] = zero;
         return;
     }
+    // Columns of B are independent.
+#pragma omp parallel for schedule(dynamic, 1) if (m * n > 8192)
     for (mint j = 0; j < n; j++) {
         if (alpha != one)
             for (mint i = 0; i < m; i++) B[i + j * ldb] = alpha * B[i + j * ldb];
@@ -121,6 +132,8 @@ void Rtrmm_LLTN(mint m, mint n, dd_real alpha, const dd_real *A, mint lda, dd_re
             for (mint i = 0; i < m; i++) B[i + j * ldb] = zero;
         return;
     }
+    // Columns of B are independent.
+#pragma omp parallel for schedule(dynamic, 1) if (m * n > 8192)
     for (mint j = 0; j < n; j++) {
         for (mint i = 0; i < m; i++) {
             dd_real temp = B[i + j * ldb];

@@ -716,11 +716,17 @@ void schur_dense(const InputData &data, const DenseLinearSpace &X, const DenseLi
     const int m = data.m, nB = (int)data.blockStruct.size();
     for (int l = 0; l < nB; l++) {
         const DenseMatrix &xMat = X.SDP_block[l], &invzMat = invZ.SDP_block[l];
+        const auto &cons = data.SDP_constraint[l];
+        const auto &bidx = data.SDP_blockIndex[l];
+        // Rows are independent: row i writes only the pairs (i, j) it owns,
+        // each exactly once per block, so a parallel loop over rows (each
+        // thread with its own work matrices) gives the serial loop's values.
+#pragma omp parallel
+        {
         DenseMatrix work1, work2;
         work1.initialize(data.blockStruct[l]);
         work2.initialize(data.blockStruct[l]);
-        const auto &cons = data.SDP_constraint[l];
-        const auto &bidx = data.SDP_blockIndex[l];
+#pragma omp for schedule(dynamic, 1)
         for (size_t k1 = 0; k1 < cons.size(); k1++) {
             int i = cons[k1];
             const SparseMatrix &Ai = data.A[i].SDP_sp_block[bidx[k1]];
@@ -752,6 +758,7 @@ void schur_dense(const InputData &data, const DenseLinearSpace &X, const DenseLi
                     B[i + (size_t)m * i] += value;
                 }
             }
+        }
         }
     }
     // compute_bMat_dense_LP (sdpa_newton.cpp:1162-1197), after the SDP blocks

@@ -84,6 +84,16 @@ int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps) {
     h->c.refine_steps = refine_steps;
     return 0;
 }
+int sdpadd_set_tile_policy(sdpadd_handle h, int policy) {
+    if (!h || policy < 0 || policy > 2) return bad(h, "tile policy must be 0 (by size), 1 (32 x 32) or 2 (64 x 64)");
+    sdk::g_tile_policy = policy;
+    return 0;
+}
+int sdpadd_dd_vec_op(sdpadd_handle h, int op, int64_t n, const double *d_a, const double *d_b, double *d_out) {
+    if (!h || op < 0 || op > 11 || n < 0 || (n > 0 && (!d_a || !d_b || !d_out))) return bad(h, "dd_vec_op: bad argument");
+    sdk::dd_vec_op(h->c.stream, op, n, DP(d_a), DP(d_b), DP(d_out));
+    return check(h);
+}
 int sdpadd_stats_enable(sdpadd_handle h, int on) {
     if (!h) return -1;
     sdk::stats_enable(on != 0);
@@ -128,6 +138,7 @@ int sdpadd_Rgemm(sdpadd_handle h, char transa, char transb, int64_t m, int64_t n
                  const double *d_A, int64_t lda, const double *d_B, int64_t ldb, const double *beta, double *d_C,
                  int64_t ldc) {
     if (!h || !up(transa, 'N') || !(up(transb, 'N') || up(transb, 'T'))) return bad(h, "Rgemm: unsupported op");
+    if (!alpha || !beta) return bad(h, "Rgemm: alpha and beta must not be null");
     if (m < 0 || n < 0 || k < 0 || lda < std::max<int64_t>(1, m) || ldc < std::max<int64_t>(1, m) ||
         ldb < std::max<int64_t>(1, up(transb, 'N') ? k : n))
         return bad(h, "Rgemm: bad dimension");
@@ -151,6 +162,7 @@ int sdpadd_ozaki_gemm(sdpadd_handle h, char transa, char transb, int64_t m, int6
 int sdpadd_Rsyrk(sdpadd_handle h, char uplo, char trans, int64_t n, int64_t k, const double *alpha, const double *d_A,
                  int64_t lda, const double *beta, double *d_C, int64_t ldc) {
     if (!h || !up(uplo, 'L') || !up(trans, 'N')) return bad(h, "Rsyrk: unsupported op");
+    if (!alpha || !beta) return bad(h, "Rsyrk: alpha and beta must not be null");
     if (n < 0 || k < 0 || lda < std::max<int64_t>(1, n) || ldc < std::max<int64_t>(1, n)) return bad(h, "Rsyrk: bad dimension");
     // Rsyrk skips columns with a[j,l] == 0; adding those zero products leaves
     // every value unchanged, so the GEMM order gives the same bits.
@@ -159,8 +171,12 @@ int sdpadd_Rsyrk(sdpadd_handle h, char uplo, char trans, int64_t n, int64_t k, c
 }
 int sdpadd_Rtrsm(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n, const double *alpha,
                  const double *d_A, int64_t lda, double *d_B, int64_t ldb) {
-    if (!h || !(alpha[0] == 1.0 && alpha[1] == 0.0) || !up(uplo, 'L') || !up(diag, 'N')) return bad(h, "Rtrsm: unsupported op");
-    if (m < 0 || n < 0 || ldb < std::max<int64_t>(1, m)) return bad(h, "Rtrsm: bad dimension");
+    if (!h) return -1;
+    if (!alpha) return bad(h, "Rtrsm: alpha must not be null");
+    if (!(alpha[0] == 1.0 && alpha[1] == 0.0) || !up(uplo, 'L') || !up(diag, 'N')) return bad(h, "Rtrsm: unsupported op");
+    // A is m x m for side 'L', n x n for side 'R' (Rtrsm.cpp:95-112)
+    if (m < 0 || n < 0 || ldb < std::max<int64_t>(1, m) || lda < std::max<int64_t>(1, up(side, 'L') ? m : n))
+        return bad(h, "Rtrsm: bad dimension");
     if (up(side, 'R') && up(transa, 'T'))
         sdk::trsm_rltn(h->c.stream, m, n, DP(d_A), lda, DP(d_B), ldb);
     else if (up(side, 'L') && up(transa, 'N'))
@@ -171,22 +187,25 @@ int sdpadd_Rtrsm(sdpadd_handle h, char side, char uplo, char transa, char diag, 
 }
 int sdpadd_Rtrmm(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n, const double *alpha,
                  const double *d_A, int64_t lda, double *d_B, int64_t ldb) {
-    if (!h || !(alpha[0] == 1.0 && alpha[1] == 0.0) || !up(side, 'L') || !up(uplo, 'L') || !up(transa, 'T') || !up(diag, 'N'))
+    if (!h) return -1;
+    if (!alpha) return bad(h, "Rtrmm: alpha must not be null");
+    if (!(alpha[0] == 1.0 && alpha[1] == 0.0) || !up(side, 'L') || !up(uplo, 'L') || !up(transa, 'T') || !up(diag, 'N'))
         return bad(h, "Rtrmm: unsupported op");
-    if (m < 0 || n < 0 || ldb < std::max<int64_t>(1, m)) return bad(h, "Rtrmm: bad dimension");
-    dd *tmp = nullptr;
-    if (hipMalloc((void **)&tmp, sizeof(dd) * std::max<int64_t>(1, m * n)) != hipSuccess) return bad(h, "hipMalloc failed");
+    if (m < 0 || n < 0 || ldb < std::max<int64_t>(1, m) || lda < std::max<int64_t>(1, m)) return bad(h, "Rtrmm: bad dimension");
+    if (m == 0 || n == 0) return 0;
+    dd *tmp = static_cast<dd *>(h->c.workspace(sizeof(dd) * (size_t)m * (size_t)n));
+    if (!tmp) return bad(h, "Rtrmm: workspace allocation failed");
     sdk::trmm_lltn(h->c.stream, m, n, DP(d_A), lda, DP(d_B), ldb, tmp, m);
     (void)hipMemcpy2DAsync(d_B, sizeof(dd) * ldb, tmp, sizeof(dd) * m, sizeof(dd) * m, n, hipMemcpyDeviceToDevice,
                            h->c.stream);
-    int rc = check(h);
-    (void)hipFree(tmp);
-    return rc;
+    return check(h);
 }
 int sdpadd_Rtrsv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, const double *d_A, int64_t lda, double *d_x,
                  int64_t incx) {
     if (!h || !up(uplo, 'L') || !up(diag, 'N') || incx != 1 || !(up(trans, 'N') || up(trans, 'T')))
         return bad(h, "Rtrsv: unsupported op");
+    if (n < 0 || lda < std::max<int64_t>(1, n)) return bad(h, "Rtrsv: bad dimension");
+    if (n == 0) return 0;
     if (h->c.trsv(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x)) != 0)
         return bad(h, "Rtrsv: a wait of the cooperative solve ran out");
     return check(h);
@@ -234,23 +253,26 @@ int sdpadd_Rtrmv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, c
                  int64_t incx) {
     if (!h || !up(uplo, 'L') || !up(diag, 'N') || incx != 1 || !(up(trans, 'N') || up(trans, 'T')))
         return bad(h, "Rtrmv: unsupported op");
-    dd *tmp = nullptr;
-    if (hipMalloc((void **)&tmp, sizeof(dd) * std::max<int64_t>(1, n)) != hipSuccess) return bad(h, "hipMalloc failed");
+    if (n < 0 || lda < std::max<int64_t>(1, n)) return bad(h, "Rtrmv: bad dimension");
+    if (n == 0) return 0;
+    dd *tmp = static_cast<dd *>(h->c.workspace(sizeof(dd) * (size_t)n));
+    if (!tmp) return bad(h, "Rtrmv: workspace allocation failed");
     sdk::trmv_lower(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x), tmp);
     sdk::copy(h->c.stream, n, tmp, DP(d_x));
-    int rc = check(h);
-    (void)hipFree(tmp);
-    return rc;
+    return check(h);
 }
 int sdpadd_Rgemv(sdpadd_handle h, char trans, int64_t m, int64_t n, const double *alpha, const double *d_A, int64_t lda,
                  const double *d_x, int64_t incx, const double *beta, double *d_y, int64_t incy) {
     if (!h || !up(trans, 'N') || incx != 1 || incy != 1) return bad(h, "Rgemv: unsupported op");
+    if (!alpha || !beta) return bad(h, "Rgemv: alpha and beta must not be null");
+    if (m < 0 || n < 0 || lda < std::max<int64_t>(1, m)) return bad(h, "Rgemv: bad dimension");
     sdk::gemv_n(h->c.stream, m, n, D(alpha), DP(d_A), lda, DP(d_x), D(beta), DP(d_y));
     return check(h);
 }
 int sdpadd_Rdot(sdpadd_handle h, int64_t n, const double *d_x, int64_t incx, const double *d_y, int64_t incy,
                 double *result) {
     if (!h || incx != 1 || incy != 1) return bad(h, "Rdot: unsupported op");
+    if (n < 0 || !result) return bad(h, "Rdot: bad argument");
     h->c.dot(n, DP(d_x), DP(d_y), 0);
     (void)hipMemcpyAsync(result, h->c.d_scal, sizeof(dd), hipMemcpyDeviceToHost, h->c.stream);
     return check(h);
@@ -282,24 +304,19 @@ int sdpadd_small_min_eig(sdpadd_handle h, int64_t n, const double *d_L, const do
     if (!h || n < 1 || n > sdk::SMALL_EIG_MAXN || !out || !info) return bad(h, "small_min_eig: bad arguments");
     hipStream_t s = h->c.stream;
     sdk::SmallEigTask tk{DP(d_L), DP(d_M), (int)n};
-    sdk::SmallEigTask *d_tk = nullptr;
-    dd *d_out = nullptr;
-    int *d_info = nullptr;
-    if (hipMalloc((void **)&d_tk, sizeof(tk)) != hipSuccess || hipMalloc((void **)&d_out, sizeof(dd)) != hipSuccess ||
-        hipMalloc((void **)&d_info, sizeof(int)) != hipSuccess) {
-        (void)hipFree(d_tk);
-        (void)hipFree(d_out);
-        return bad(h, "small_min_eig: allocation failed");
-    }
+    // task, result and info in the handle's workspace (256-byte slots)
+    char *ws = static_cast<char *>(h->c.workspace(768));
+    if (!ws) return bad(h, "small_min_eig: workspace allocation failed");
+    static_assert(sizeof(sdk::SmallEigTask) <= 256, "task slot");
+    auto *d_tk = reinterpret_cast<sdk::SmallEigTask *>(ws);
+    dd *d_out = reinterpret_cast<dd *>(ws + 256);
+    int *d_info = reinterpret_cast<int *>(ws + 512);
     (void)hipMemcpyAsync(d_tk, &tk, sizeof(tk), hipMemcpyHostToDevice, s);
     sdk::small_min_eig(s, d_tk, 1, h->c.dot_T, d_out, d_info);
     dd r{0.0, 0.0};
     (void)hipMemcpyAsync(&r, d_out, sizeof(dd), hipMemcpyDeviceToHost, s);
     (void)hipMemcpyAsync(info, d_info, sizeof(int), hipMemcpyDeviceToHost, s);
     int rc = check(h);
-    (void)hipFree(d_tk);
-    (void)hipFree(d_out);
-    (void)hipFree(d_info);
     out[0] = r.hi;
     out[1] = r.lo;
     return rc;
@@ -412,6 +429,11 @@ int sdpadd_problem_finalize(sdpadd_problem p) {
         return -3;
     }
     p->finalized = true;
+    return 0;
+}
+int sdpadd_problem_schur_ddflops(sdpadd_problem p, double *ddflops) {
+    if (!p || !p->finalized || !ddflops) return -1;
+    *ddflops = sdp::schur_ddflops(p->hp);
     return 0;
 }
 int sdpadd_problem_destroy(sdpadd_problem p) {

@@ -52,6 +52,11 @@ struct Context {
     }
     int oz_slices = 16;
     int refine_steps = 2;
+    // general workspace of the BLAS-level entry points (grown on demand,
+    // kept until the handle is destroyed)
+    void *d_ws = nullptr;
+    size_t ws_cap = 0;
+    void *workspace(size_t bytes);
     std::string last_error;
 
     int init(int dev);

@@ -224,7 +224,10 @@ __global__ __launch_bounds__(NT) void k_gemm(idx_t m, idx_t n, idx_t k, dd alpha
 }
 
 // 32x32 tiles when 64x64 tiles would leave most of the 256 CUs idle.
+int g_tile_policy = 0;
 static bool use_small_tiles(idx_t m, idx_t n, int batch) {
+    if (g_tile_policy == 1) return true;
+    if (g_tile_policy == 2) return false;
     return (idx_t)cdiv(m, TB) * cdiv(n, TB) * batch < 1024;
 }
 
@@ -1933,6 +1936,34 @@ __global__ void k_symmetrize(idx_t n, dd *A) {
     v = dd_mul(mk(0.5), v);
     A[i + j * n] = v;
     A[j + i * n] = v;
+}
+
+// The QD operator set elementwise (qd/include/qd/dd_inline.h:26-393,
+// qd/src/dd_real.cpp:47-69): 0 a+b, 1 a-b, 2 a*b, 3 a/b, 4 sqrt(a), 5 a+=b,
+// 6 a-=b, 7 a*=b, 8 a*hi(b), 9 a/hi(b), 10 a+hi(b), 11 hi(a)/b.
+__global__ void k_dd_vec_op(int op, idx_t n, const dd *__restrict__ a, const dd *__restrict__ b, dd *__restrict__ out) {
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const dd x = a[i], y = b[i];
+    dd r;
+    switch (op) {
+    case 0: r = dd_add(x, y); break;
+    case 1: r = dd_sub(x, y); break;
+    case 2: r = dd_mul(x, y); break;
+    case 3: r = dd_div(x, y); break;
+    case 4: r = dd_sqrt(x); break;
+    case 5: r = dd_add_eq(x, y); break;
+    case 6: r = dd_sub(x, y); break;
+    case 7: r = dd_mul_eq(x, y); break;
+    case 8: r = dd_mul_d(x, y.hi); break;
+    case 9: r = dd_div_d(x, y.hi); break;
+    case 10: r = dd_add_d(x, y.hi); break;
+    default: r = dd_div(mk(x.hi), y); break;
+    }
+    out[i] = r;
+}
+void dd_vec_op(hipStream_t s, int op, idx_t n, const dd *a, const dd *b, dd *out) {
+    if (n > 0) k_dd_vec_op<<<cdiv(n, 256), 256, 0, s>>>(op, n, a, b, out);
 }
 
 void fill(hipStream_t s, idx_t len, dd v, dd *x) {

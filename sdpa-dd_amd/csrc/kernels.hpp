@@ -98,6 +98,9 @@ struct DotTask;
 // buffer's per-task share).
 void dot_tasks_fast(hipStream_t s, const DotTask *d_tasks, int ntasks, idx_t maxn, int G, dd *partial);
 
+// Elementwise QD operator `op` (0..11: + - * / sqrt += -= *= *double
+// /double +double double/) on n dd pairs.
+void dd_vec_op(hipStream_t s, int op, idx_t n, const dd *a, const dd *b, dd *out);
 // Elementwise (length len, flat).
 void fill(hipStream_t s, idx_t len, dd v, dd *x);
 void copy(hipStream_t s, idx_t len, const dd *x, dd *y);
@@ -153,6 +156,11 @@ void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx
 // Sequential strided dot (Rdot with non-unit stride): one thread per task.
 // Max |x_i| over len values into *out.
 void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned *counter);
+
+// Tile choice of the dd GEMM and the Ozaki product: 0 = by grid size
+// (default), 1 = always the 32 x 32 tiles, 2 = always the 64 x 64 tiles.
+// Both tilings give every output element the same operation sequence.
+extern int g_tile_policy;
 
 // Per-kernel-family timing with HIP events on the launch stream (off by
 // default).  Families: "gemm", "potrf", "trsm", "trmm", "trsv", "mv", "dot",

@@ -317,7 +317,8 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
     // 64 x 64 tiles hold a CU each (128 KB of LDS at S = 16): below about
     // two per CU, the 32 x 32 kernel fills the chip better
     const idx_t t64 = cdiv(mpad, 64) * cdiv(npad, 64) / (lower ? 2 : 1);
-    if (t64 < 512)
+    const bool small = sdk::g_tile_policy == 1 || (sdk::g_tile_policy == 0 && t64 < 512);
+    if (small)
         k_gemm32<S><<<dim3(cdiv(mpad, 32), cdiv(npad, 32)), 256, 0, s>>>(
             m, n, nkb, w.sa, mpad, w.ea, ta, same ? w.sa : w.sb, npad, same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
             lower ? 1 : 0);

@@ -536,6 +536,58 @@ std::vector<int> compute_formula(const HostProblem &P, ddr Kappa) {
     return f;
 }
 
+// Algorithmic dd operations of one Schur assembly (compute_bMat_dense_SDP,
+// sdpa_newton.cpp:979-1068, and compute_bMat_dense_LP, :1162-1197): per row
+// i of block l, A_i Z^{-1} costs 2 nnz_i n (sparse) or 2 n^3 (dense), F1's
+// X (A_i Z^{-1}) another 2 n^3; per pair (i, j) the row visits, F1 costs
+// 2 nnz_j (the inner product with A_j), F2 2 n nnz_j for a sparse A_j
+// (calF2's strided dots) and 2 n^2 (+ 2 n^3 once per row for X F) for a
+// dense one, F3 3 nnz_i nnz_j (calF3: two products and a sum per entry
+// pair); each LP entry a pair shares 3.  nnz = NonZeroEffect.
+double schur_ddflops(const HostProblem &P) {
+    const int nB = (int)P.bs.size();
+    double total = 0.0;
+    for (int l = 0; l < nB; l++) {
+        const auto &cs = P.cons[l];
+        const auto &ci = P.cidx[l];
+        const double n = P.bs[l];
+        for (size_t k1 = 0; k1 < cs.size(); k1++) {
+            const int i = cs[k1];
+            const Block &Ai = P.A[i].blk[ci[k1]];
+            const int inz = Ai.nnz_effect;
+            const int f = P.formula[(size_t)i * nB + l];
+            const double aiz = Ai.dense ? 2.0 * n * n * n : 2.0 * inz * n;
+            if (f == 0) total += aiz + 2.0 * n * n * n;
+            if (f == 1) total += aiz;
+            bool g = false;
+            for (size_t k2 = 0; k2 < cs.size(); k2++) {
+                const int j = cs[k2];
+                const Block &Aj = P.A[j].blk[ci[k2]];
+                const int jnz = Aj.nnz_effect;
+                if (inz < jnz || (inz == jnz && i < j)) continue;
+                if (f == 0)
+                    total += 2.0 * (Aj.dense ? n * n : jnz);
+                else if (f == 1) {
+                    if (Aj.dense) {
+                        if (!g) total += 2.0 * n * n * n;
+                        g = true;
+                        total += 2.0 * n * n;
+                    } else {
+                        total += 2.0 * n * jnz;
+                    }
+                } else {
+                    total += 3.0 * (double)inz * (double)jnz;
+                }
+            }
+        }
+    }
+    for (size_t l = 0; l < P.lp_cons.size(); l++) {
+        const double c = (double)P.lp_cons[l].size();
+        total += 3.0 * c * (c + 1) / 2;
+    }
+    return total;
+}
+
 // ----------------------------------------------------------------- upload --
 template <typename T>
 static T *dev_copy(DeviceProblem &dp, const std::vector<T> &h, hipStream_t s) {

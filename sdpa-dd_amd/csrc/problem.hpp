@@ -102,6 +102,8 @@ bool write_point(const std::string &path, const HostProblem &P, const std::vecto
                  const std::vector<dd> &Z, std::string &err);
 bool qd_read(const char *s, dd &out);
 std::vector<int> compute_formula(const HostProblem &p, ddr kappa);
+// Algorithmic dd operations (multiplies and adds) of one Schur assembly.
+double schur_ddflops(const HostProblem &p);
 
 // ---------------------------------------------------------------- device --
 // Pairs of the Schur assembly for one block and their formula data.

@@ -34,6 +34,20 @@ int Context::init(int dev) {
     *h_tsw_err = 0;
     return 0;
 }
+void *Context::workspace(size_t bytes) {
+    if (bytes > ws_cap) {
+        // the old buffer may still be read by work queued on the stream
+        if (d_ws) {
+            if (hipStreamSynchronize(stream) != hipSuccess) return nullptr;
+            (void)hipFree(d_ws);
+        }
+        d_ws = nullptr;
+        ws_cap = 0;
+        if (hipMalloc(&d_ws, bytes) != hipSuccess) return nullptr;
+        ws_cap = bytes;
+    }
+    return d_ws;
+}
 int Context::trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x, bool check) {
     if (n >= 512) {
         const size_t need = sdk::trsv_coop_bytes(n);
@@ -109,6 +123,9 @@ void Context::destroy() {
     if (d_max_work) (void)hipFree(d_max_work);
     ozw.release();
     if (d_tsw) (void)hipFree(d_tsw);
+    if (d_ws) (void)hipFree(d_ws);
+    d_ws = nullptr;
+    ws_cap = 0;
     if (d_binv) (void)hipFree(d_binv);
     d_binv = nullptr;
     binv_cap = 0;

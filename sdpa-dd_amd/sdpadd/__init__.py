@@ -50,6 +50,8 @@ def load_library(path=None):
         "sdpadd_set_dot_partition": (ctypes.c_int, [vp, ctypes.c_int]),
         "sdpadd_set_path": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "sdpadd_malloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
+        "sdpadd_set_tile_policy": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sdpadd_dd_vec_op": (ctypes.c_int, [vp, ctypes.c_int, c_i64, vp, vp, vp]),
         "sdpadd_stats_enable": (ctypes.c_int, [vp, ctypes.c_int]),
         "sdpadd_stats_reset": (ctypes.c_int, [vp]),
         "sdpadd_stats_filter": (ctypes.c_int, [vp, ctypes.c_char_p]),
@@ -95,6 +97,7 @@ def load_library(path=None):
                                                ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
         "sdpadd_problem_formula": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
         "sdpadd_schur_build": (ctypes.c_int, [vp, vp, vp, vp]),
+        "sdpadd_problem_schur_ddflops": (ctypes.c_int, [vp, c_dp]),
         "sdpadd_inner_A": (ctypes.c_int, [vp, vp, vp]),
         "sdpadd_plus_A": (ctypes.c_int, [vp, vp, vp]),
         "sdpadd_read_params": (ctypes.c_int, [ctypes.c_char_p, vp]),
@@ -166,6 +169,18 @@ class Device:
         """path 0: plain dd (the reference's operation order); 1: Ozaki INT8-MFMA
         Cholesky trailing updates; 2: as 1 plus the n x n GEMMs."""
         self.check(self.lib.sdpadd_set_path(self.h, path, slices, refine_steps), "set_path")
+
+    def set_tile_policy(self, policy):
+        """0: GEMM/Ozaki tile shape by grid size; 1: always 32 x 32; 2: always 64 x 64."""
+        self.check(self.lib.sdpadd_set_tile_policy(self.h, int(policy)), "set_tile_policy")
+
+    def dd_vec_op(self, op, a, b):
+        """QD operator `op` (0..11, see sdpadd_dd_vec_op) elementwise on (n, 2) dd arrays."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        da, db, do = self.to_device(a), self.to_device(b), self.empty(a.shape)
+        self.check(self.lib.sdpadd_dd_vec_op(self.h, int(op), a.shape[0], da.ptr, db.ptr, do.ptr), "dd_vec_op")
+        return do.get()
 
     def set_dot_partition(self, t):
         self.dot_partition = t
@@ -306,6 +321,12 @@ class Problem:
         out = (ctypes.c_int * max(1, self.m * len(self.sdp_blocks)))()
         self.dev.check(self.dev.lib.sdpadd_problem_formula(self.p, out), "formula")
         return np.array(out[:], dtype=np.int32)
+
+    def schur_ddflops(self):
+        """Algorithmic dd operations of one Schur assembly."""
+        out = np.zeros(1)
+        self.dev.check(self.dev.lib.sdpadd_problem_schur_ddflops(self.p, _dp(out)), "schur_ddflops")
+        return float(out[0])
 
     def schur(self, X, Zinv):
         dX, dZ = self.dev.to_device(X), self.dev.to_device(Zinv)

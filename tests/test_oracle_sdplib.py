@@ -135,3 +135,25 @@ def test_oracle_example1_until_the_dual_is_exact():
     assert "%.16e" % -r["objValDual"][0] == "-4.1899999999999999e+01"
     assert "%.16e" % -r["objValPrimal"][0] == "-4.1899999999999999e+01"
     assert r["mu"][0] < 1e-29
+
+
+# The oracle at the benchmark workload shapes (n = m = 800 max-cut; n = 200,
+# m = 1949 theta): 46 iterations each, several minutes on 8 host threads
+# (maxG11: 382 s with 6), so they run only when SDPADD_SLOW_ORACLE=1.
+@pytest.mark.slow
+@pytest.mark.parametrize("name", ["maxG11", "theta4"])
+def test_oracle_large_sdplib_reproduces_reference_results(oracle, name):
+    import os
+    if os.environ.get("SDPADD_SLOW_ORACLE") != "1":
+        pytest.skip("several minutes of CPU: set SDPADD_SLOW_ORACLE=1")
+    trace, fin = parse_result(name)
+    r = oracle.solve(golden_sdplib(name), SDPLIB_PARAM, dot_threads=64, timeout=7200)
+    assert r["iterations"] == int(fin["Iteration"]) and r["phase"] == fin["phase"]
+    assert len(r["trace"]) == len(trace) - 1
+    for rec, ref in zip(r["trace"], trace):
+        got = fmt_trace(rec)
+        ref = [x.replace("-0.00e+00", "+0.00e+00") for x in ref]
+        assert got[:2] + got[4:] == ref[:2] + ref[4:], (rec["it"], got, ref)
+    assert "%.16e" % -r["objValDual"][0] == fin["objValPrimal"]
+    assert "%.16e" % -r["objValPrimal"][0] == fin["objValDual"]
+    assert abs(r["mu"][0] - float(fin["mu"])) <= 1e-4 * abs(float(fin["mu"]))
