@@ -114,6 +114,16 @@ int sdpadd_Rtrsm(sdpadd_handle h, char side, char uplo, char transa, char diag, 
 /* Rtrmm: (L,L,T,N), alpha = 1 (mplapack/Rtrmm.cpp:31-275). */
 int sdpadd_Rtrmm(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n,
                  const double *alpha, const double *d_A, int64_t lda, double *d_B, int64_t ldb);
+/* Rtrmm (L,L,T,N) as sdpadd_Rtrmm, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Its m x n product buffer is a fixed device array of 2^20 dd
+ * (16 MiB) per GPU that the library holds until the process ends;
+ * m * n > 1048576 is rejected (-1).  Calls on one GPU share that array:
+ * calls that overlap on one GPU (other streams, other threads, other
+ * handles) race.  Results are identical to sdpadd_Rtrmm's. */
+int sdpadd_Rtrmm_noalloc(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n,
+                         const double *alpha, const double *d_A, int64_t lda, double *d_B, int64_t ldb);
 /* The Ozaki form's triangular solve (no counterpart in the reference): x :=
  * L^{-1} x ('N') or L^{-T} x ('T') for lower non-unit L, by 32-row blocks
  * with inverted diagonal blocks in a cooperative grid; not Rtrsv's operation

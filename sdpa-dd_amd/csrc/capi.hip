@@ -4,6 +4,8 @@
 #include <memory>
 #include <string>
 
+#include <mutex>
+
 #include "../../include/sdpadd.h"
 #include "adjoint.hpp"
 #include "context.hpp"
@@ -27,6 +29,16 @@ struct sdpadd_problem_s {
     sdp::InitPoint init;
     bool has_init = false;
 };
+
+// Scratch of the allocation-free entry points: a fixed device array
+// (each GPU holds its own copy, kept until the process ends).  Host code
+// uses its address from hipGetSymbolAddress, taken once per device at
+// sdpadd_create with that device current, never the array's name.
+constexpr size_t NOALLOC_DD = size_t(1) << 20;  // 16 MiB of dd
+static __device__ __align__(256) dd scratch[NOALLOC_DD];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static dd *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
 
 namespace {
 inline dd D(const double *p) { return dd{p[0], p[1]}; }
@@ -57,6 +69,17 @@ int sdpadd_create(int device, sdpadd_handle *h) {
         delete ctx;
         *h = nullptr;
         return -3;
+    }
+    if (device >= 0 && device < NOALLOC_MAX_DEVICES) {
+        // init made `device` current
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void *ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess)
+                g_scratch_ptr[device] = static_cast<dd *>(ptr);
+            else
+                (void)hipGetLastError();  // the allocation-free entries then report it
+        }
     }
     *h = ctx;
     return 0;
@@ -195,6 +218,25 @@ int sdpadd_Rtrmm(sdpadd_handle h, char side, char uplo, char transa, char diag, 
     if (m == 0 || n == 0) return 0;
     dd *tmp = static_cast<dd *>(h->c.workspace(sizeof(dd) * (size_t)m * (size_t)n));
     if (!tmp) return bad(h, "Rtrmm: workspace allocation failed");
+    sdk::trmm_lltn(h->c.stream, m, n, DP(d_A), lda, DP(d_B), ldb, tmp, m);
+    (void)hipMemcpy2DAsync(d_B, sizeof(dd) * ldb, tmp, sizeof(dd) * m, sizeof(dd) * m, n, hipMemcpyDeviceToDevice,
+                           h->c.stream);
+    return check(h);
+}
+int sdpadd_Rtrmm_noalloc(sdpadd_handle h, char side, char uplo, char transa, char diag, int64_t m, int64_t n,
+                         const double *alpha, const double *d_A, int64_t lda, double *d_B, int64_t ldb) {
+    if (!h) return -1;
+    if (!alpha) return bad(h, "Rtrmm_noalloc: alpha must not be null");
+    if (!(alpha[0] == 1.0 && alpha[1] == 0.0) || !up(side, 'L') || !up(uplo, 'L') || !up(transa, 'T') || !up(diag, 'N'))
+        return bad(h, "Rtrmm_noalloc: unsupported op");
+    if (m < 0 || n < 0 || ldb < std::max<int64_t>(1, m) || lda < std::max<int64_t>(1, m))
+        return bad(h, "Rtrmm_noalloc: bad dimension");
+    if ((uint64_t)m * (uint64_t)n > NOALLOC_DD) return bad(h, "Rtrmm_noalloc: m*n exceeds the 1048576-element scratch");
+    if (m == 0 || n == 0) return 0;
+    dd *tmp = (h->c.device >= 0 && h->c.device < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[h->c.device] : nullptr;
+    if (!tmp) return bad(h, "Rtrmm_noalloc: no scratch address for this device");
+    // the product is formed in the scratch (every element written before the
+    // copy reads it, so nothing needs clearing) and copied over B
     sdk::trmm_lltn(h->c.stream, m, n, DP(d_A), lda, DP(d_B), ldb, tmp, m);
     (void)hipMemcpy2DAsync(d_B, sizeof(dd) * ldb, tmp, sizeof(dd) * m, sizeof(dd) * m, n, hipMemcpyDeviceToDevice,
                            h->c.stream);

@@ -69,6 +69,8 @@ def load_library(path=None):
                                         vp, c_i64, vp, c_i64]),
         "sdpadd_Rtrmm": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, c_i64, c_dp,
                                         vp, c_i64, vp, c_i64]),
+        "sdpadd_Rtrmm_noalloc": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, c_i64,
+                                                c_dp, vp, c_i64, vp, c_i64]),
         "sdpadd_Rtrsv": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, vp, c_i64, vp, c_i64]),
         "sdpadd_ozaki_trsv": (ctypes.c_int, [vp, ctypes.c_char, c_i64, vp, c_i64, vp]),
         "sdpadd_ozaki_matvec": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_i64, c_dp, vp, c_i64, vp, c_dp, vp]),
