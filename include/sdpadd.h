@@ -60,8 +60,11 @@ int sdpadd_free(sdpadd_handle h, void *d_ptr);
 int sdpadd_upload(sdpadd_handle h, void *d_dst, const void *src, size_t bytes);
 int sdpadd_download(sdpadd_handle h, void *dst, const void *d_src, size_t bytes);
 
-/* Kernel timing with HIP events on the library's stream, per family
- * ("gemm", "potrf", "trsm", "trmm", "trsv", "mv", "dot"): launches, summed
+/* Kernel timing with HIP events on the stream each kernel runs on, per
+ * family ("gemm", "ozaki" (Ozaki products), "potrf" (diagonal Cholesky
+ * blocks), "panel" (Cholesky panel solves), "trsm", "trmm", "trsv",
+ * "trsv_prep" (block operators of the Ozaki form's solves), "mv", "dot"):
+ * launches, summed
  * kernel milliseconds and algorithmic dd-flops since the last reset.  No
  * counterpart in the reference (its ComputeTime buckets are host wall
  * clocks, sdpa_parts.h:46-83). */
@@ -149,6 +152,14 @@ int sdpadd_Rgemv(sdpadd_handle h, char trans, int64_t m, int64_t n, const double
 /* Rdot: unit strides; result to host dd *result (mplapack/Rdot.cpp:38-47). */
 int sdpadd_Rdot(sdpadd_handle h, int64_t n, const double *d_x, int64_t incx, const double *d_y, int64_t incy,
                 double *result);
+/* Raxpy (y += alpha*x), Rscal (x = alpha*x; nothing for incx <= 0) and
+ * Rcopy (y = x) with any nonzero strides, negative strides starting at the
+ * far end (mplapack/Raxpy_omp.cpp:79-112, Rscal.cpp, Rcopy.cpp); x and y
+ * must not overlap. */
+int sdpadd_Raxpy(sdpadd_handle h, int64_t n, const double *alpha, const double *d_x, int64_t incx, double *d_y,
+                 int64_t incy);
+int sdpadd_Rscal(sdpadd_handle h, int64_t n, const double *alpha, double *d_x, int64_t incx);
+int sdpadd_Rcopy(sdpadd_handle h, int64_t n, const double *d_x, int64_t incx, double *d_y, int64_t incy);
 /* Rpotrf: uplo 'L'; *info = 0 or the first non-positive leading minor
  * (mplapack/Rpotrf.cpp:32-154). */
 int sdpadd_Rpotrf(sdpadd_handle h, char uplo, int64_t n, double *d_A, int64_t lda, int64_t *info);

@@ -6,6 +6,7 @@
 #include "sdpa_oracle.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -898,6 +899,7 @@ dd_real min_eigen(const DenseLinearSpace &L, const DenseLinearSpace &M, bool &ok
 }  // namespace
 
 bool solve(const InputData &data, const Parameter &param, SolveResult &res, const InitPoint *init) {
+    const auto t_start = std::chrono::steady_clock::now();
     const int m = data.m;
     const std::vector<int> &bs = data.blockStruct;
     // Newton and work space (sdpa_newton.cpp:191-223, sdpa_dataset.cpp:668-702).
@@ -1181,7 +1183,8 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res, cons
                 }
             }
         }
-        res.trace.push_back({pIteration, muCurrent, thetaP, thetaD, objValPrimal, objValDual, alphaP, alphaD, betaV});
+        res.trace.push_back({pIteration, muCurrent, thetaP, thetaD, objValPrimal, objValDual, alphaP, alphaD, betaV,
+                             std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count()});
         // Solutions::update (sdpa_dataset.cpp:120-143)
         plus(pt.xMat, pt.xMat, nw.DxMat, alphaP);
         Raxpy(m, alphaD, nw.DyVec.data(), 1, pt.yVec.data(), 1);

@@ -92,6 +92,7 @@ enum PhaseType { noINFO, pFEAS, dFEAS, pdFEAS, pdINF, pFEAS_dINF, pINF_dFEAS, pd
 struct IterRecord {
     int it;
     dd_real mu, thetaP, thetaD, objP, objD, alphaP, alphaD, beta;
+    double seconds = 0.0;  // wall time since the solve started, when the record was taken
 };
 
 struct SolveResult {

@@ -73,7 +73,7 @@ int main(int argc, char **argv) {
         std::printf(", \"alphaP\": "); pdd(r.alphaP);
         std::printf(", \"alphaD\": "); pdd(r.alphaD);
         std::printf(", \"beta\": "); pdd(r.beta);
-        std::printf("}");
+        std::printf(", \"seconds\": %.6f}", r.seconds);
     }
     std::printf("]");
     if (dump) {

@@ -1,5 +1,7 @@
-"""Builds libsdpadd_hip.so (dd Newton-step kernels + C ABI) and the
-sdpa_dd_gpu command-line solver for gfx950 with hipcc, in-tree.
+"""Builds libsdpadd_hip.so (dd Newton-step kernels + C ABI), the
+sdpa_dd_gpu command-line solver for gfx950 with hipcc, and
+libmpblas_dd_hip.so (MPLAPACK's dd BLAS/LAPACK symbols forwarded to the C
+ABI), in-tree.
 
 Usage: python sdpa-dd_amd/build.py [--jobs N] [--debug]
 """
@@ -14,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsdpadd_hip.so")
 CLI = os.path.join(HERE, "sdpa_dd_gpu")
+SHIM = os.path.join(HERE, "libmpblas_dd_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["kernels.hip", "ozaki.hip", "smallblk.hip", "problem.hip", "solver.hip", "adjoint.hip", "capi.hip", "steqr.cpp"]
@@ -50,6 +53,9 @@ def build(jobs=8, debug=False):
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB] + objs)
     run([HIPCC] + FLAGS + opt + ["-x", "hip", os.path.join(CSRC, "cli_main.cpp"), "-o", CLI,
                                   "-L" + HERE, "-lsdpadd_hip", "-Wl,-rpath,$ORIGIN"])
+    # MPLAPACK-symbol forwarding library (host C++ over the C ABI)
+    run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", os.path.join(CSRC, "mpblas_shim.cpp"),
+         "-o", SHIM, "-L" + HERE, "-lsdpadd_hip", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

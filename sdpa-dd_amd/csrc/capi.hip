@@ -319,6 +319,25 @@ int sdpadd_Rdot(sdpadd_handle h, int64_t n, const double *d_x, int64_t incx, con
     (void)hipMemcpyAsync(result, h->c.d_scal, sizeof(dd), hipMemcpyDeviceToHost, h->c.stream);
     return check(h);
 }
+int sdpadd_Raxpy(sdpadd_handle h, int64_t n, const double *alpha, const double *d_x, int64_t incx, double *d_y,
+                 int64_t incy) {
+    if (!h) return -1;
+    if (!alpha || incx == 0 || incy == 0) return bad(h, "Raxpy: bad argument");
+    sdk::axpy_strided(h->c.stream, n, D(alpha), DP(d_x), incx, DP(d_y), incy);
+    return check(h);
+}
+int sdpadd_Rscal(sdpadd_handle h, int64_t n, const double *alpha, double *d_x, int64_t incx) {
+    if (!h) return -1;
+    if (!alpha) return bad(h, "Rscal: alpha must not be null");
+    sdk::scal_strided(h->c.stream, n, D(alpha), DP(d_x), incx);
+    return check(h);
+}
+int sdpadd_Rcopy(sdpadd_handle h, int64_t n, const double *d_x, int64_t incx, double *d_y, int64_t incy) {
+    if (!h) return -1;
+    if (incx == 0 || incy == 0) return bad(h, "Rcopy: bad argument");
+    sdk::copy_strided(h->c.stream, n, DP(d_x), incx, DP(d_y), incy);
+    return check(h);
+}
 int sdpadd_Rpotrf(sdpadd_handle h, char uplo, int64_t n, double *d_A, int64_t lda, int64_t *info) {
     if (!h || !up(uplo, 'L')) return bad(h, "Rpotrf: unsupported op");
     if (n < 0 || lda < std::max<int64_t>(1, n)) return bad(h, "Rpotrf: bad dimension");

@@ -501,11 +501,13 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
         int nb = (int)std::min<idx_t>(PB, n - K);
         idx_t below = n - K - nb;
         {
-            StatScope st("potrf", s, (double)nb * nb * nb / 3.0 + (double)below * nb * nb);
+            StatScope st("potrf", s, (double)nb * nb * nb / 3.0);
             k_potrf_diag<<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
-            if (below > 0)
-                k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda, nb,
-                                                             scratch, d_info);
+        }
+        if (below > 0) {
+            StatScope st("panel", s, (double)below * nb * nb);
+            k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda, nb,
+                                                         scratch, d_info);
         }
         if (below > 0) {
             // trailing lower triangle: A22 += (-L21) * L21^T, k ascending
@@ -539,11 +541,13 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
             const int nb = (int)std::min<idx_t>(PB, P + pw - K);
             const idx_t below = n - K - nb;
             {
-                StatScope st("potrf", s, (double)nb * nb * nb / 3.0 + (double)below * nb * nb);
+                StatScope st("potrf", s, (double)nb * nb * nb / 3.0);
                 k_potrf_diag<<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
-                if (below > 0)
-                    k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda,
-                                                                 nb, scratch, d_info);
+            }
+            if (below > 0) {
+                StatScope st("panel", s, (double)below * nb * nb);
+                k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda, nb,
+                                                             scratch, d_info);
             }
             const idx_t rem = P + pw - K - nb;  // panel columns right of this block
             if (rem > 0) {
@@ -1339,7 +1343,7 @@ void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n
         const unsigned nt = (unsigned)((n + 31) / 32);
         k_transpose_lower<<<dim3(nt, nt), 256, 0, s>>>(n, L, lda, lt);
     }
-    StatScope st("trsv", s, 2.0 * nblk * (SB * SB * SB / 3.0 + BI_D * SB * SB * SB));
+    StatScope st("trsv_prep", s, 2.0 * nblk * (SB * SB * SB / 3.0 + BI_D * SB * SB * SB));
     k_blk_inv<<<nblk, 64, 0, s>>>(n, 0, L, lda, inv);
     k_blk_ops<<<dim3(nblk, BI_D + 1), 256, 0, s>>>(n, 0, L, lda, inv, ops_n);
     k_blk_inv<<<nblk, 64, 0, s>>>(n, 1, L, lda, inv);
@@ -1962,6 +1966,41 @@ __global__ void k_dd_vec_op(int op, idx_t n, const dd *__restrict__ a, const dd 
     }
     out[i] = r;
 }
+// Level-1 BLAS with strides (Raxpy_omp.cpp:79-112, Rscal.cpp, Rcopy.cpp):
+// element i is x[x0 + i incx], y[y0 + i incy], the start offsets placing a
+// negative stride's first element at the far end as the reference does.
+// Elements are independent (x and y must not overlap).
+__global__ void k_axpy_strided(idx_t n, dd a, const dd *x, idx_t x0, idx_t incx, dd *y, idx_t y0, idx_t incy) {
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dd &yy = y[y0 + i * incy];
+    yy = dd_add_eq(yy, dd_mul(a, x[x0 + i * incx]));  // dy += da * dx
+}
+__global__ void k_scal_strided(idx_t n, dd a, dd *x, idx_t incx) {
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dd &xx = x[i * incx];
+    xx = dd_mul(a, xx);  // dx = da * dx
+}
+__global__ void k_copy_strided(idx_t n, const dd *x, idx_t x0, idx_t incx, dd *y, idx_t y0, idx_t incy) {
+    const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[y0 + i * incy] = x[x0 + i * incx];
+}
+void axpy_strided(hipStream_t s, idx_t n, dd a, const dd *x, idx_t incx, dd *y, idx_t incy) {
+    if (n <= 0 || dd_is_zero(a)) return;
+    const idx_t x0 = incx < 0 ? (1 - n) * incx : 0, y0 = incy < 0 ? (1 - n) * incy : 0;
+    k_axpy_strided<<<cdiv(n, 256), 256, 0, s>>>(n, a, x, x0, incx, y, y0, incy);
+}
+void scal_strided(hipStream_t s, idx_t n, dd a, dd *x, idx_t incx) {
+    if (n <= 0 || incx <= 0) return;
+    k_scal_strided<<<cdiv(n, 256), 256, 0, s>>>(n, a, x, incx);
+}
+void copy_strided(hipStream_t s, idx_t n, const dd *x, idx_t incx, dd *y, idx_t incy) {
+    if (n <= 0) return;
+    const idx_t x0 = incx < 0 ? (1 - n) * incx : 0, y0 = incy < 0 ? (1 - n) * incy : 0;
+    k_copy_strided<<<cdiv(n, 256), 256, 0, s>>>(n, x, x0, incx, y, y0, incy);
+}
+
 void dd_vec_op(hipStream_t s, int op, idx_t n, const dd *a, const dd *b, dd *out) {
     if (n > 0) k_dd_vec_op<<<cdiv(n, 256), 256, 0, s>>>(op, n, a, b, out);
 }

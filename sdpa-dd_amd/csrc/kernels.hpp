@@ -98,6 +98,12 @@ struct DotTask;
 // buffer's per-task share).
 void dot_tasks_fast(hipStream_t s, const DotTask *d_tasks, int ntasks, idx_t maxn, int G, dd *partial);
 
+// Level-1 BLAS with any strides (negative strides start at the far end, as
+// in the reference): y += a*x (skipped when a == 0), x = a*x (nothing for
+// incx <= 0), y = x.  x and y must not overlap.
+void axpy_strided(hipStream_t s, idx_t n, dd a, const dd *x, idx_t incx, dd *y, idx_t incy);
+void scal_strided(hipStream_t s, idx_t n, dd a, dd *x, idx_t incx);
+void copy_strided(hipStream_t s, idx_t n, const dd *x, idx_t incx, dd *y, idx_t incy);
 // Elementwise QD operator `op` (0..11: + - * / sqrt += -= *= *double
 // /double +double double/) on n dd pairs.
 void dd_vec_op(hipStream_t s, int op, idx_t n, const dd *a, const dd *b, dd *out);
@@ -163,8 +169,8 @@ void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned 
 extern int g_tile_policy;
 
 // Per-kernel-family timing with HIP events on the launch stream (off by
-// default).  Families: "gemm", "potrf", "trsm", "trmm", "trsv", "mv", "dot",
-// "schur", "elementwise".  flops are algorithmic dd operations.
+// default).  Families: "gemm", "ozaki", "potrf", "panel", "trsm", "trmm",
+// "trsv", "trsv_prep", "mv", "dot".  flops are algorithmic dd operations.
 void stats_enable(bool on);
 // Times only the listed families (comma-separated); null or "" for all.
 void stats_filter(const char *families);

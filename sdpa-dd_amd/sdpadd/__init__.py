@@ -78,6 +78,9 @@ def load_library(path=None):
         "sdpadd_Rtrmv": (ctypes.c_int, [vp, ctypes.c_char, ctypes.c_char, ctypes.c_char, c_i64, vp, c_i64, vp, c_i64]),
         "sdpadd_Rgemv": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_i64, c_dp, vp, c_i64, vp, c_i64, c_dp, vp, c_i64]),
         "sdpadd_Rdot": (ctypes.c_int, [vp, c_i64, vp, c_i64, vp, c_i64, c_dp]),
+        "sdpadd_Raxpy": (ctypes.c_int, [vp, c_i64, c_dp, vp, c_i64, vp, c_i64]),
+        "sdpadd_Rscal": (ctypes.c_int, [vp, c_i64, c_dp, vp, c_i64]),
+        "sdpadd_Rcopy": (ctypes.c_int, [vp, c_i64, vp, c_i64, vp, c_i64]),
         "sdpadd_Rpotrf": (ctypes.c_int, [vp, ctypes.c_char, c_i64, vp, c_i64, ctypes.POINTER(c_i64)]),
         "sdpadd_Rsteqr": (ctypes.c_int, [vp, ctypes.c_char, c_i64, c_dp, c_dp, c_dp, c_i64, c_dp, ctypes.POINTER(c_i64)]),
         "sdpadd_lanczos_min_eig": (ctypes.c_int, [vp, c_i64, vp, vp, c_dp]),

@@ -80,3 +80,34 @@ def test_generated_workloads_have_the_stated_shapes(oracle, tmp_path):
     assert bs == [300] and 4000 < m < 5000
     f = oracle.formula(m, 1)
     assert f[0] == 1 and set(f[1:].tolist()) == {2}  # A_0 = I uses F2, edges F3
+
+
+# MPLAPACK's dd signatures (mplapack/mpblas_dd.h:38-56, mplapack_dd.h:40-41)
+# as the demangler prints them for mplapackint = int64_t = long and QD's
+# dd_real passed by value.
+MPLAPACK_SIGNATURES = [
+    "Rgemm(char const*, char const*, long, long, long, dd_real, dd_real*, long, dd_real*, long, dd_real, dd_real*, long)",
+    "Rsyrk(char const*, char const*, long, long, dd_real, dd_real*, long, dd_real, dd_real*, long)",
+    "Rtrsm(char const*, char const*, char const*, char const*, long, long, dd_real, dd_real*, long, dd_real*, long)",
+    "Rtrmm(char const*, char const*, char const*, char const*, long, long, dd_real, dd_real*, long, dd_real*, long)",
+    "Rtrsv(char const*, char const*, char const*, long, dd_real*, long, dd_real*, long)",
+    "Rtrmv(char const*, char const*, char const*, long, dd_real*, long, dd_real*, long)",
+    "Rgemv(char const*, long, long, dd_real, dd_real*, long, dd_real*, long, dd_real, dd_real*, long)",
+    "Rdot(long, dd_real*, long, dd_real*, long)",
+    "Raxpy(long, dd_real, dd_real*, long, dd_real*, long)",
+    "Rscal(long, dd_real, dd_real*, long)",
+    "Rcopy(long, dd_real*, long, dd_real*, long)",
+    "Rpotrf(char const*, long, dd_real*, long, long&)",
+    "Rsteqr(char const*, long, dd_real*, dd_real*, dd_real*, long, dd_real*, long&)",
+]
+
+
+def test_mplapack_shim_exports_the_reference_signatures():
+    """libmpblas_dd_hip.so carries MPLAPACK's C++ symbols (the library seam
+    of configure.ac:47-50), so SDPA-DD links it unchanged."""
+    import subprocess
+    shim = os.path.join(ROOT, "sdpa-dd_amd", "libmpblas_dd_hip.so")
+    out = subprocess.run(["nm", "-DC", "--defined-only", shim], capture_output=True, text=True, check=True).stdout
+    exported = {ln.split(" ", 2)[2] for ln in out.splitlines() if " T " in ln}
+    missing = [s for s in MPLAPACK_SIGNATURES if s not in exported]
+    assert not missing, missing
