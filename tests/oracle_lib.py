@@ -192,7 +192,10 @@ class Oracle:
         if dump:
             cmd.append("--dump-solution")
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
-        return json.loads(r.stdout)
+        out = json.loads(r.stdout)
+        # wall-clock stamps differ from run to run: keep them out of the trace
+        out["iter_seconds"] = [rec.pop("seconds", None) for rec in out.get("trace", [])]
+        return out
 
 
 def golden_sdplib(name):
