@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -317,6 +318,13 @@ __device__ __forceinline__ dd readlane_dd(dd v, int k) {
     r.lo = readlane_d(v.lo, k);
     return r;
 }
+// Lane shuffle of a dd from any lane (ds_bpermute).
+__device__ __forceinline__ dd shfl_dd(dd v, int src) {
+    dd r;
+    r.hi = __shfl(v.hi, src, 64);
+    r.lo = __shfl(v.lo, src, 64);
+    return r;
+}
 // Packed column-major lower triangle of a PB x PB block: (j, k), j >= k.
 __device__ __forceinline__ int lpack(int j, int k) { return k * PB - (k * (k - 1)) / 2 + (j - k); }
 // Row r and column c (c <= r) of element e of the lower triangle of an
@@ -348,9 +356,20 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
     __shared__ int failed;
     const int tid = threadIdx.x;
     if (*info != 0) return;
-    for (int e = tid; e < nb * nb; e += PD_NT) {
-        int i = e % nb, j = e / nb;
-        if (i >= j) S[i][j] = A[(K + i) + (K + j) * lda];
+    {
+        // all global loads issued before the LDS stores
+        constexpr int NL = PB * PB / PD_NT;
+        dd v[NL];
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            const int e = tid + PD_NT * q, i = e % PB, j = e / PB;
+            v[q] = (i >= j && i < nb) ? A[(K + i) + (K + j) * lda] : mk(0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            const int e = tid + PD_NT * q, i = e % PB, j = e / PB;
+            if (i >= j && i < nb) S[i][j] = v[q];
+        }
     }
     // Waves 1.. own the lower-triangle elements round-robin (column-major
     // enumeration); own[q] packs (i, j) as i | j << 8.  Unused slots hold
@@ -431,13 +450,14 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
 // B := B * L^{-T} on an nb-column panel (Rtrsm 'R','L','T','N' order with
 // reciprocal scaling: b_rk *= rl_k, then b_rj -= l_jk * b_rk for j > k,
 // skipping l_jk == 0), rows [r0, r0 + nrows).  L is the nb x nb lower block
-// at Lb.  PR rows per workgroup are staged through LDS; each wave owns
-// PR_W rows, lane j column j of each, and the solved column is broadcast
-// with readlane.  A step costs a wave about 55 FP64 instructions per row
-// (4 cycles each at wave64), so one row per wave spreads a panel over four
-// times as many SIMDs as four rows per wave did, at the same chain latency.
-static constexpr int PR_W = 1;
-static constexpr int PR = 4 * PR_W;
+// at Lb.  PR rows per workgroup are staged through LDS.  A wave holds four
+// rows, 16 lanes each: lane c of a row holds columns c, 16 + c, 32 + c and
+// 48 + c in registers, so every lane does useful work at every step (a
+// step costs a wave one scaling, a shuffle and up to four updates for four
+// rows), and the solved column is broadcast within each 16-lane group.
+// Each element still sees the column loop's operation sequence.
+static constexpr int PW_ROWS = 4;        // rows per wave
+static constexpr int PR = 4 * PW_ROWS;   // rows per 256-thread workgroup
 __global__ __launch_bounds__(256) void k_panel_rltn(dd *B, idx_t ldb, idx_t r0, idx_t nrows, const dd *Lb, idx_t lda, int nb,
                                                     const dd *rinv, const int64_t *info) {
     __shared__ dd Lc[PB * (PB + 1) / 2];
@@ -445,47 +465,64 @@ __global__ __launch_bounds__(256) void k_panel_rltn(dd *B, idx_t ldb, idx_t r0, 
     __shared__ dd Tt[PR][PB + 1];
     if (info && *info != 0) return;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, c = lane & 15;
     const idx_t rb = r0 + (idx_t)blockIdx.x * PR;
     const int nr = (int)((r0 + nrows - rb) < PR ? (r0 + nrows - rb) : PR);
-    for (int e = tid; e < nb * nb; e += 256) {
-        int j = e % nb, k = e / nb;
-        if (j >= k) Lc[lpack(j, k)] = Lb[j + (idx_t)k * lda];
-    }
-    for (int k = tid; k < nb; k += 256) rl[k] = rinv[k];
-    for (int e = tid; e < PR * nb; e += 256) {
-        int r = e % PR, j = e / PR;
-        Tt[r][j] = r < nr ? B[(rb + r) + (idx_t)j * ldb] : mk(0.0);
+    {
+        // every global load of the staging is issued before the LDS stores,
+        // so their latencies overlap instead of adding up
+        constexpr int NL = PB * PB / 256, NT = PR * PB / 256;
+        dd lv[NL], tv[NT];
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            const int e = tid + 256 * q, j = e % PB, k = e / PB;
+            lv[q] = (j >= k && j < nb) ? Lb[j + (idx_t)k * lda] : mk(0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+            const int e = tid + 256 * q, r = e % PR, j = e / PR;
+            tv[q] = (r < nr && j < nb) ? B[(rb + r) + (idx_t)j * ldb] : mk(0.0);
+        }
+        const dd rv = tid < nb ? rinv[tid] : mk(0.0);
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            const int e = tid + 256 * q, j = e % PB, k = e / PB;
+            if (j >= k && j < nb) Lc[lpack(j, k)] = lv[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+            const int e = tid + 256 * q, r = e % PR, j = e / PR;
+            Tt[r][j] = tv[q];
+        }
+        if (tid < PB) rl[tid] = rv;
     }
     __syncthreads();
-    dd x[PR_W];
+    const int row = PW_ROWS * wave + g;
+    dd x[4];
 #pragma unroll
-    for (int q = 0; q < PR_W; q++) x[q] = lane < nb ? Tt[PR_W * wave + q][lane] : mk(0.0);
-    // steps in groups of eight so that the LDS reads of a group are issued
-    // ahead of its dependent chain
-    for (int k0 = 0; k0 < nb; k0 += 8) {
+    for (int b = 0; b < 4; b++) x[b] = (16 * b + c < nb) ? Tt[row][16 * b + c] : mk(0.0);
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int k = k0 + u;
-            if (k < nb) {
-                const dd l = Lc[lpack(lane < nb ? lane : k, k)];
-                if (lane == k) {
-                    const dd r = rl[k];
+    for (int b = 0; b < 4; b++) {
+        if (16 * b >= nb) break;
+        const int kend = (nb - 16 * b) < 16 ? (nb - 16 * b) : 16;
+        for (int kk = 0; kk < kend; kk++) {
+            const int k = 16 * b + kk;
+            dd l[4];
 #pragma unroll
-                    for (int q = 0; q < PR_W; q++) x[q] = dd_mul(r, x[q]);
-                }
-                dd xk[PR_W];
-#pragma unroll
-                for (int q = 0; q < PR_W; q++) xk[q] = readlane_dd(x[q], k);
-                if (lane > k && lane < nb && !dd_is_zero(l)) {
-#pragma unroll
-                    for (int q = 0; q < PR_W; q++) x[q] = dd_sub(x[q], dd_mul(l, xk[q]));
-                }
+            for (int b2 = b; b2 < 4; b2++) {
+                const int j = 16 * b2 + c;
+                l[b2] = (j > k && j < nb) ? Lc[lpack(j, k)] : mk(0.0);
             }
+            if (c == kk) x[b] = dd_mul(rl[k], x[b]);
+            const dd xk = shfl_dd(x[b], (lane & 48) | kk);
+#pragma unroll
+            for (int b2 = b; b2 < 4; b2++)
+                if (!dd_is_zero(l[b2])) x[b2] = dd_sub(x[b2], dd_mul(l[b2], xk));
         }
     }
 #pragma unroll
-    for (int q = 0; q < PR_W; q++)
-        if (lane < nb) Tt[PR_W * wave + q][lane] = x[q];
+    for (int b = 0; b < 4; b++)
+        if (16 * b + c < nb) Tt[row][16 * b + c] = x[b];
     __syncthreads();
     for (int e = tid; e < PR * nb; e += 256) {
         int r = e % PR, j = e / PR;
@@ -528,7 +565,8 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
                    const OzLookAhead *la) {
     // panel width: 256 (four k chunks per Ozaki update) for large matrices,
     // 64 below 1024 so that small ones still take most updates on MFMA
-    const idx_t OZ_NB = n > 1024 ? 256 : 64;
+    idx_t OZ_NB = n > 1024 ? 256 : 64;
+    if (const char *e = std::getenv("SDPADD_OZ_NB_EXPERIMENT")) OZ_NB = std::atoi(e);
     // the split costs a second slicing of the panel and two events: it pays
     // only where the trailing updates are long (measured: m = 4551 gains 5%,
     // m = 800 loses 14% of the Cholesky time)
@@ -599,12 +637,6 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
 static constexpr int SB = 32;
 static constexpr int TS_NT = 512;          // threads of the triangular-solve kernels
 static constexpr int TS_UPD = TS_NT - 64;  // update threads (waves 1..)
-__device__ __forceinline__ dd shfl_dd(dd v, int src) {
-    dd r;
-    r.hi = __shfl(v.hi, src, 64);
-    r.lo = __shfl(v.lo, src, 64);
-    return r;
-}
 
 // x := L^{-1} x ('L','N','N'): for j ascending, if x_j != 0 { x_j /= L_jj;
 // x_i -= x_j * L_ij for i > j }.
@@ -1091,7 +1123,7 @@ int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, 
 
 // ------------------------------------------ block-inverse triangular solves --
 // The Ozaki form's Schur solves (not Rtrsv's operation order; the Ozaki
-// form refines the result against B).  Blocks of 32 rows are solved in a
+// form refines the result against B).  Blocks of BB rows are solved in a
 // processing sequence (L x = b: from the top; L^T x = b: from the bottom),
 // block s as x_s = Dinv_s (p_s - sum_{d=2..BI_D} T_{s,s-d} x_{s-d})
 // - Dinv_s T_{s,s-1} x_{s-1}, where Dinv_s inverts the diagonal block,
@@ -1099,85 +1131,97 @@ int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, 
 // after the updater workgroups applied blocks <= s - BI_D - 1.  The
 // operators Dinv_s and N_{s,d} = Dinv_s T_{s,s-d} are formed once per
 // factor.  In workgroup 0, wave 0 runs the one dependent step per block
-// (x_s from x_{s-1}: a 32-term sum per lane) while waves 1-3 prepare block
+// (x_s from x_{s-1}: a BB-term pairwise sum per lane) while waves 1-3 prepare block
 // s+1 (the Dinv product of its partial and the d = 2, 3 terms).
+static constexpr int BB = 32;  // rows per block of the block-inverse solves
 static constexpr int BI_D = 3;
-static constexpr int BI_OPS = (BI_D + 1) * SB * SB;  // dd per block and direction
+static constexpr int BI_OPS = (BI_D + 1) * BB * BB;  // dd per block and direction
+
+// Pairwise sum of N dd terms (depth log2(N) instead of an N-step chain).
+template <int N>
+__device__ __forceinline__ dd tree_sum(dd *t) {
+#pragma unroll
+    for (int w = N / 2; w >= 1; w >>= 1) {
+#pragma unroll
+        for (int q = 0; q < w; q++) t[q] = dd_add(t[q], t[q + w]);
+    }
+    return t[0];
+}
 
 __host__ __device__ __forceinline__ void seq_block(idx_t n, int trans, int s, idx_t &lo, int &nb) {
     if (!trans) {
-        lo = (idx_t)s * SB;
-        nb = (int)((n - lo) < SB ? (n - lo) : SB);
+        lo = (idx_t)s * BB;
+        nb = (int)((n - lo) < BB ? (n - lo) : BB);
     } else {
-        const idx_t hi = n - (idx_t)s * SB;
-        lo = hi - SB > 0 ? hi - SB : 0;
+        const idx_t hi = n - (idx_t)s * BB;
+        lo = hi - BB > 0 ? hi - BB : 0;
         nb = (int)(hi - lo);
     }
 }
 
-// inv[s] = (diagonal block s)^{-1}, row-major 32 x 32, zero outside; one
+// inv[s] = (diagonal block s)^{-1}, row-major BB x BB, zero outside; one
 // wave per block, lane c forms column c by forward substitution.
 __global__ __launch_bounds__(64) void k_blk_inv(idx_t n, int trans, const dd *L, idx_t lda, dd *inv) {
-    __shared__ dd T[SB][SB + 1];
+    __shared__ dd T[BB][BB + 1];
     const int s = blockIdx.x, lane = threadIdx.x;
     idx_t lo;
     int nb;
     seq_block(n, trans, s, lo, nb);
-    for (int e = lane; e < SB * SB; e += 64) {
-        const int i = e % SB, k = e / SB;
+    for (int e = lane; e < BB * BB; e += 64) {
+        const int i = e % BB, k = e / BB;
         T[i][k] = (i < nb && k <= i) ? L[(lo + i) + (lo + k) * lda] : mk(0.0);
     }
     __syncthreads();
     const int c = lane;
-    dd y[SB];
+    dd y[BB];
 #pragma unroll
-    for (int i = 0; i < SB; i++) {
+    for (int i = 0; i < BB; i++) {
         dd acc = mk(i == c ? 1.0 : 0.0);
 #pragma unroll
         for (int k = 0; k < i; k++)
             if (k >= c) acc = dd_sub(acc, dd_mul(T[i][k], y[k]));
         y[i] = (c < nb && i < nb && i >= c) ? dd_div(acc, T[i][i]) : mk(0.0);
     }
-    if (c < SB)
+    if (c < BB)
 #pragma unroll
-        for (int i = 0; i < SB; i++) inv[((idx_t)s * SB + i) * SB + c] = y[i];
+        for (int i = 0; i < BB; i++) inv[((idx_t)s * BB + i) * BB + c] = y[i];
 }
 
 // ops[s][0] = Dinv_s (the inverse, transposed for L^T x = b);
 // ops[s][d] = Dinv_s T_{s,s-d} for d = 1..BI_D (zero when s - d < 0).
 __global__ __launch_bounds__(256) void k_blk_ops(idx_t n, int trans, const dd *L, idx_t lda, const dd *inv, dd *ops) {
-    __shared__ dd Iv[SB][SB + 1];
-    __shared__ dd T[SB][SB + 1];
+    __shared__ dd Iv[BB][BB + 1];
+    __shared__ dd T[BB][BB + 1];
     const int s = blockIdx.x, d = blockIdx.y, tid = threadIdx.x;
     idx_t lo;
     int nb;
     seq_block(n, trans, s, lo, nb);
-    dd *out = ops + (idx_t)s * BI_OPS + (idx_t)d * SB * SB;
-    for (int e = tid; e < SB * SB; e += 256) {
-        const int i = e % SB, k = e / SB;
-        Iv[i][k] = trans ? inv[((idx_t)s * SB + k) * SB + i] : inv[((idx_t)s * SB + i) * SB + k];
+    dd *out = ops + (idx_t)s * BI_OPS + (idx_t)d * BB * BB;
+    for (int e = tid; e < BB * BB; e += 256) {
+        const int i = e % BB, k = e / BB;
+        Iv[i][k] = trans ? inv[((idx_t)s * BB + k) * BB + i] : inv[((idx_t)s * BB + i) * BB + k];
     }
     if (d == 0) {
         __syncthreads();
-        for (int e = tid; e < SB * SB; e += 256) out[e] = Iv[e / SB][e % SB];
+        for (int e = tid; e < BB * BB; e += 256) out[e] = Iv[e / BB][e % BB];
         return;
     }
     if (s - d < 0) {
-        for (int e = tid; e < SB * SB; e += 256) out[e] = mk(0.0);
+        for (int e = tid; e < BB * BB; e += 256) out[e] = mk(0.0);
         return;
     }
     idx_t lo2;
     int nb2;
     seq_block(n, trans, s - d, lo2, nb2);
-    for (int e = tid; e < SB * SB; e += 256) {
-        const int k = e % SB, q = e / SB;
+    for (int e = tid; e < BB * BB; e += 256) {
+        const int k = e % BB, q = e / BB;
         dd v = mk(0.0);
         if (k < nb && q < nb2) v = trans ? L[(lo2 + q) + (lo + k) * lda] : L[(lo + k) + (lo2 + q) * lda];
         T[k][q] = v;
     }
     __syncthreads();
-    for (int e = tid; e < SB * SB; e += 256) {
-        const int i = e / SB, q = e % SB;
+    for (int e = tid; e < BB * BB; e += 256) {
+        const int i = e / BB, q = e % BB;
         dd acc = mk(0.0);
         for (int k = 0; k < nb; k++) acc = dd_add(acc, dd_mul(Iv[i][k], T[k][q]));
         out[e] = acc;
@@ -1189,22 +1233,23 @@ __global__ __launch_bounds__(256) void k_blk_ops(idx_t n, int trans, const dd *L
 __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd *L, idx_t lda, dd *x, const dd *ops,
                                                    int *flags, int nupd, dd *vg, int *sticky, const dd *LT) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nblk = (int)((n + SB - 1) / SB);
+    const int nblk = (int)((n + BB - 1) / BB);
     int *err = flags + 1 + nupd;
     if (blockIdx.x == 0) {
-        __shared__ dd hp[2][3][SB];          // prepared terms of the next block
-        __shared__ dd xs[BI_D + 1][SB];      // solved blocks, ring by s mod (BI_D + 1)
+        __shared__ dd hp[2][3][BB];          // prepared terms of the next block
+        __shared__ dd xs[BI_D + 1][BB];      // solved blocks, ring by s mod (BI_D + 1)
+        __shared__ dd pw[BB];                // wave 1's partial of the block it prepares
         __shared__ int stop;
-        const int li = lane < SB ? lane : 0;
+        const int li = lane < BB ? lane : 0;
         // each wave holds row li of the operator it applies next (wave 0:
         // op 1 of the block it solves next; wave 1: op 0 and wave w = 2, 3:
         // op w of the block it prepares next), loaded one round ahead
-        dd opr[SB];
+        dd opr[BB];
         auto load_row = [&](int t, int d) {
             if (t >= nblk) return;
-            const dd *o = ops + (idx_t)t * BI_OPS + (idx_t)d * SB * SB + (idx_t)li * SB;
+            const dd *o = ops + (idx_t)t * BI_OPS + (idx_t)d * BB * BB + (idx_t)li * BB;
 #pragma unroll
-            for (int q = 0; q < SB; q++) opr[q] = o[q];
+            for (int q = 0; q < BB; q++) opr[q] = o[q];
         };
         const int wop = wave == 1 ? 0 : wave;
         // waves 1-3: terms of block t into hp[t & 1]
@@ -1226,15 +1271,19 @@ __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd 
                     return;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                const dd p = lane < nb ? x[lo + lane] : mk(0.0);
+                // broadcast through LDS (in-order within the wave), not readlane:
+                // a readlane per term would occupy the scalar registers
+                pw[lane] = lane < nb ? x[lo + lane] : mk(0.0);
 #pragma unroll
-                for (int q = 0; q < SB; q++) a = dd_add(a, dd_mul(opr[q], readlane_dd(p, q)));
+                for (int q = 0; q < BB; q++) opr[q] = dd_mul(opr[q], pw[q]);
+                a = tree_sum<BB>(opr);
             } else if (t - wave >= 0) {
                 const int slot = (t - wave) % (BI_D + 1);
 #pragma unroll
-                for (int q = 0; q < SB; q++) a = dd_add(a, dd_mul(opr[q], xs[slot][q]));
+                for (int q = 0; q < BB; q++) opr[q] = dd_mul(opr[q], xs[slot][q]);
+                a = tree_sum<BB>(opr);
             }
-            if (lane < SB) hp[t & 1][wave - 1][lane] = a;
+            if (lane < BB) hp[t & 1][wave - 1][lane] = a;
             load_row(t + 1, wop);
         };
         if (tid == 0) stop = 0;
@@ -1251,17 +1300,17 @@ __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd 
                 seq_block(n, trans, s, lo, nb);
                 dd xr = dd_sub(dd_sub(hp[s & 1][0][li], hp[s & 1][1][li]), hp[s & 1][2][li]);
                 if (s >= 1) {
-                    dd a = mk(0.0);
+                    // products in place: opr is reloaded for the next block below
 #pragma unroll
-                    for (int q = 0; q < SB; q++) a = dd_add(a, dd_mul(opr[q], readlane_dd(prev, q)));
-                    xr = dd_sub(xr, a);
+                    for (int q = 0; q < BB; q++) opr[q] = dd_mul(opr[q], xs[(s - 1) % (BI_D + 1)][q]);
+                    xr = dd_sub(xr, tree_sum<BB>(opr));
                 }
                 if (lane >= nb) xr = mk(0.0);
                 if (lane < nb) {
                     x[lo + lane] = xr;
-                    vg[(idx_t)s * SB + lane] = xr;
+                    vg[(idx_t)s * BB + lane] = xr;
                 }
-                if (lane < SB) xs[s % (BI_D + 1)][lane] = xr;
+                if (lane < BB) xs[s % (BI_D + 1)][lane] = xr;
                 prev = xr;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                 if (lane == 0) tc_store(flags, s + 1);
@@ -1290,12 +1339,19 @@ __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd 
         idx_t bp;
         int nbj;
         seq_block(n, trans, J, bp, nbj);
-        dd a[SB];
-        if (act) {
+        // the block's terms in two halves (register budget): the
+        // first half's loads are issued before the wait for block J
+        constexpr int H = BB / 2;
+        dd a[H];
+        auto load_half = [&](int h) {
+            if (!act) return;
 #pragma unroll
-            for (int q = 0; q < SB; q++)
-                a[q] = !trans ? L[i + (bp + q) * lda] : LT ? LT[i + (bp + q) * n] : L[(bp + q) + i * lda];
-        }
+            for (int q = 0; q < H; q++) {
+                const idx_t c = bp + H * h + q;
+                a[q] = !trans ? L[i + c * lda] : LT ? LT[i + c * n] : L[c + i * lda];
+            }
+        };
+        load_half(0);
         if (J + 1 > avail) {
             if (tid == 0) {
                 go = tc_wait(flags, J + 1, err, sticky) ? tc_load(flags) : 0;
@@ -1307,7 +1363,12 @@ __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd 
         }
         if (act) {
 #pragma unroll
-            for (int q = 0; q < SB; q++) xi = dd_sub(xi, dd_mul(a[q], vg[(idx_t)J * SB + q]));
+            for (int q = 0; q < H; q++) a[q] = dd_mul(a[q], vg[(idx_t)J * BB + q]);
+            const dd s0 = tree_sum<H>(a);
+            load_half(1);
+#pragma unroll
+            for (int q = 0; q < H; q++) a[q] = dd_mul(a[q], vg[(idx_t)J * BB + H + q]);
+            xi = dd_sub(xi, dd_add(s0, tree_sum<H>(a)));
             x[i] = xi;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -1335,15 +1396,15 @@ __global__ __launch_bounds__(256) void k_transpose_lower(idx_t n, const dd *L, i
     }
 }
 
-size_t binv_ops_elems(idx_t n) { return (size_t)((n + SB - 1) / SB) * BI_OPS; }
+size_t binv_ops_elems(idx_t n) { return (size_t)((n + BB - 1) / BB) * BI_OPS; }
 void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n, dd *ops_t, dd *inv, dd *lt) {
     if (n <= 0) return;
-    const int nblk = (int)((n + SB - 1) / SB);
+    const int nblk = (int)((n + BB - 1) / BB);
     if (lt) {
         const unsigned nt = (unsigned)((n + 31) / 32);
         k_transpose_lower<<<dim3(nt, nt), 256, 0, s>>>(n, L, lda, lt);
     }
-    StatScope st("trsv_prep", s, 2.0 * nblk * (SB * SB * SB / 3.0 + BI_D * SB * SB * SB));
+    StatScope st("trsv_prep", s, 2.0 * nblk * (BB * BB * BB / 3.0 + BI_D * BB * BB * BB));
     k_blk_inv<<<nblk, 64, 0, s>>>(n, 0, L, lda, inv);
     k_blk_ops<<<dim3(nblk, BI_D + 1), 256, 0, s>>>(n, 0, L, lda, inv, ops_n);
     k_blk_inv<<<nblk, 64, 0, s>>>(n, 1, L, lda, inv);
@@ -1353,10 +1414,10 @@ int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x,
               int *sticky, const dd *lt) {
     if (n <= 0) return 0;
     const int nupd = (int)((n + TC_ROWS - 1) / TC_ROWS);
-    const idx_t nblk = (n + SB - 1) / SB;
+    const idx_t nblk = (n + BB - 1) / BB;
     char *w = (char *)work;
     dd *vg = (dd *)(w + 256 - (((uintptr_t)w) & 255));
-    int *flags = (int *)(vg + nblk * SB) + nblk * SB;
+    int *flags = (int *)(vg + nblk * BB) + nblk * BB;
     zero_ints(s, flags, 2 + nupd);
     StatScope st("trsv", s, (double)n * n);
     const int tr = (trans == 'T' || trans == 't') ? 1 : 0;
