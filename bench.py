@@ -288,6 +288,11 @@ def main():
             line["%s_solve_s" % form] = round(f["s"], 4)
     if plain and ozk:
         line["ozaki_speedup"] = round(plain["s"] / ozk["s"], 3)
+    # the solver's own wall-clock split of the last timed solve (Schur
+    # assembly and Cholesky from events; Lanczos step lengths and the X/Z
+    # inverse factors as host wall time around their launches)
+    line["phase_s"] = {k: {q: round(w[k]["res"][q], 4) for q in ("t_schur", "t_chol", "t_lanczos", "t_inverse")}
+                       for k in ("plain", "ozaki") if k in w}
     line["iterations"] = {k: w[k]["res"]["iterations"] for k in ("plain", "ozaki") if k in w}
     line["phase"] = {k: w[k]["res"]["phase"] for k in ("plain", "ozaki") if k in w}
     line["relgap"] = {k: w[k]["res"]["relgap"][0] for k in ("plain", "ozaki") if k in w}

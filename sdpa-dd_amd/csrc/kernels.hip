@@ -556,10 +556,11 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
 }
 
 // Cholesky with Ozaki trailing updates (the precision path): panels of
-// 64 or 256 columns are factored with the dd kernels above (diagonal blocks,
-// panel solves and in-panel dd GEMM updates); the lower triangle right of a
-// panel is updated with the INT8-MFMA product A22 -= L21 L21^T (k = panel
-// width).  Not bit-identical to Rpotrf; its accuracy is that of the Ozaki
+// 64 or 256 columns are factored 64 columns at a time with the dd kernels
+// above (diagonal blocks, panel solves), the rest of a panel updated after
+// each 64 columns with an Ozaki product (k = 64); the lower triangle right
+// of a panel is updated with the INT8-MFMA product A22 -= L21 L21^T (k =
+// panel width).  Not bit-identical to Rpotrf; its accuracy is that of the Ozaki
 // product (normwise dd for 16 digit planes).
 int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch, oz::Work &w, int S,
                    const OzLookAhead *la) {
@@ -589,9 +590,13 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
             }
             const idx_t rem = P + pw - K - nb;  // panel columns right of this block
             if (rem > 0) {
-                dd *A21 = A + (K + nb) + K * lda;
-                gemm(s, 'T', below, rem, nb, mk(-1.0), A21, lda, A21, lda, mk(1.0), A + (K + nb) + (K + nb) * lda, lda, 1,
-                     0, 0, 0, true, 0);
+                // the panel's remaining columns, every row below: an Ozaki
+                // product with k = 64 (on the FP64 VALU this rank-64 update
+                // was the panel's largest cost)
+                const dd *A21 = A + (K + nb) + K * lda;
+                int rc = oz::gemm(s, w, 'N', 'T', below, rem, nb, -1, A21, lda, A21, lda, 1, A + (K + nb) + (K + nb) * lda,
+                                  lda, true, S);
+                if (rc != 0) return rc;
             }
         }
         const idx_t Q = P + pw, below = n - Q;
