@@ -305,3 +305,134 @@ def test_qr_min_eigen_is_lambda_min_of_congruence(oracle):
     got, info = oracle.qr_min_eigen(n, L, M)
     assert info == 0
     assert abs(mp(got) - lmin) < 1e-28 * mpmath.norm(S)
+
+
+# ---- the remaining MPLAPACK routines of the path, against exact products --
+# Each check is an identity of the routine's definition evaluated exactly
+# (Fractions of the dd inputs) or at 60 digits; a transposed operand, the
+# wrong triangle, a dropped term or a sign slip misses it by far more than
+# the dd rounding bound.
+def _exact_cm(buf, rows, cols):
+    """column-major (rows*cols, 2) dd -> list of rows of Fractions."""
+    return [[ddF(buf[i + rows * j]) for j in range(cols)] for i in range(rows)]
+
+
+def _close(got, exact, scale):
+    """|got - exact| <= 8 (terms) 2^-104 scale, elementwise."""
+    for g, e, s in zip(got, exact, scale):
+        assert abs(ddF(g) - e) <= Fraction(8) * Fraction(2) ** -104 * s + Fraction(1, 10 ** 300), (float(ddF(g) - e), float(s))
+
+
+def _lower(n, rng):
+    from ddgen import rand_lower_cm
+    return rand_lower_cm(n, rng)
+
+
+@pytest.mark.parametrize("tb", ["N", "T"])
+def test_rgemv_exact(oracle, tb):
+    """Rgemv 'N': y := alpha A x + beta y (mplapack/Rgemv.cpp:31-204)."""
+    rng = np.random.default_rng(41)
+    m, n = 13, 9
+    A, x, y = rand_dd((m * n,), rng), rand_dd((n,), rng), rand_dd((m,), rng)
+    al, be = rand_dd((1,), rng)[0], rand_dd((1,), rng)[0]
+    got = oracle.rgemv(m, n, al, A, m, x, be, y)
+    Ae = _exact_cm(A, m, n)
+    a, b = ddF(al), ddF(be)
+    exact = [a * sum(Ae[i][j] * ddF(x[j]) for j in range(n)) + b * ddF(y[i]) for i in range(m)]
+    scale = [abs(a) * sum(abs(Ae[i][j] * ddF(x[j])) for j in range(n)) + abs(b * ddF(y[i])) for i in range(m)]
+    _close(got, exact, [s * n for s in scale])
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_rtrmv_exact(oracle, trans):
+    """Rtrmv L: x := L x ('N') or L^T x ('T') (mplapack/Rtrmv.cpp:31-234)."""
+    rng = np.random.default_rng(42 + (trans == "T"))
+    n = 14
+    L, x = _lower(n, rng), rand_dd((n,), rng)
+    got = oracle.rtrmv(trans, n, L, n, x)
+    Le = _exact_cm(L, n, n)
+    op = (lambda i, k: Le[i][k]) if trans == "N" else (lambda i, k: Le[k][i])
+    exact = [sum(op(i, k) * ddF(x[k]) for k in range(n)) for i in range(n)]
+    scale = [n * sum(abs(op(i, k) * ddF(x[k])) for k in range(n)) for i in range(n)]
+    _close(got, exact, scale)
+    # strictly upper entries of L are never read: garbage there changes nothing
+    Lg = L.copy()
+    for j in range(n):
+        for i in range(j):
+            Lg[i + n * j] = [1e300, 0.0]
+    assert np.array_equal(oracle.rtrmv(trans, n, Lg, n, x), got)
+
+
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_rtrsv_residual(oracle, trans):
+    """Rtrsv L: solves L x = b ('N') or L^T x = b ('T'); the exact residual
+    of the computed x is at dd rounding level (mplapack/Rtrsv.cpp:31-234)."""
+    rng = np.random.default_rng(44 + (trans == "T"))
+    n = 16
+    L, b = _lower(n, rng), rand_dd((n,), rng)
+    x = oracle.rtrsv(trans, n, L, n, b)
+    Le = _exact_cm(L, n, n)
+    op = (lambda i, k: Le[i][k]) if trans == "N" else (lambda i, k: Le[k][i])
+    for i in range(n):
+        r = sum(op(i, k) * ddF(x[k]) for k in range(n)) - ddF(b[i])
+        s = sum(abs(op(i, k) * ddF(x[k])) for k in range(n)) + abs(ddF(b[i]))
+        assert abs(r) <= 16 * n * Fraction(2) ** -104 * s, (i, float(r))
+
+
+def test_rtrsm_both_sides_residual(oracle):
+    """Rtrsm (R,L,T,N): X L^T = B, and (L,L,N,N): L X = B
+    (mplapack/Rtrsm.cpp:128-142, 256-283), residuals at dd level."""
+    rng = np.random.default_rng(46)
+    m, n = 7, 11
+    Ln, B = _lower(n, rng), rand_dd((m * n,), rng)
+    X = oracle.rtrsm("RLTN", m, n, Ln, n, B, m)
+    Le, Xe, Be = _exact_cm(Ln, n, n), _exact_cm(X, m, n), _exact_cm(B, m, n)
+    for i in range(m):
+        for j in range(n):
+            r = sum(Xe[i][k] * Le[j][k] for k in range(n)) - Be[i][j]  # (X L^T)_ij
+            s = sum(abs(Xe[i][k] * Le[j][k]) for k in range(n)) + abs(Be[i][j])
+            assert abs(r) <= 16 * n * Fraction(2) ** -104 * s, ("RLTN", i, j, float(r))
+    Lm = _lower(m, rng)
+    X = oracle.rtrsm("LLNN", m, n, Lm, m, B, m)
+    Le, Xe = _exact_cm(Lm, m, m), _exact_cm(X, m, n)
+    for i in range(m):
+        for j in range(n):
+            r = sum(Le[i][k] * Xe[k][j] for k in range(m)) - Be[i][j]  # (L X)_ij
+            s = sum(abs(Le[i][k] * Xe[k][j]) for k in range(m)) + abs(Be[i][j])
+            assert abs(r) <= 16 * m * Fraction(2) ** -104 * s, ("LLNN", i, j, float(r))
+
+
+def test_rtrmm_exact(oracle):
+    """Rtrmm (L,L,T,N): B := L^T B (mplapack/Rtrmm.cpp:157-169)."""
+    rng = np.random.default_rng(47)
+    m, n = 9, 6
+    L, B = _lower(m, rng), rand_dd((m * n,), rng)
+    got = oracle.rtrmm(m, n, L, m, B, m)
+    Le, Be = _exact_cm(L, m, m), _exact_cm(B, m, n)
+    exact, scale = [], []
+    for j in range(n):
+        for i in range(m):
+            exact.append(sum(Le[k][i] * Be[k][j] for k in range(m)))
+            scale.append(m * sum(abs(Le[k][i] * Be[k][j]) for k in range(m)))
+    _close(got, exact, scale)
+
+
+def test_rsyrk_exact_lower_only(oracle):
+    """Rsyrk (L,N): the lower triangle of C := alpha A A^T + beta C, the
+    strict upper triangle untouched (mplapack/Rsyrk.cpp:158-176)."""
+    rng = np.random.default_rng(48)
+    n, k = 10, 7
+    A, C = rand_dd((n * k,), rng), rand_dd((n * n,), rng)
+    al, be = rand_dd((1,), rng)[0], rand_dd((1,), rng)[0]
+    got = oracle.rsyrk(n, k, al, A, n, be, C, n)
+    Ae, Ce = _exact_cm(A, n, k), _exact_cm(C, n, n)
+    a, b = ddF(al), ddF(be)
+    for j in range(n):
+        for i in range(n):
+            g = got[i + n * j]
+            if i < j:
+                assert np.array_equal(g, C[i + n * j]), (i, j)
+                continue
+            e = a * sum(Ae[i][l] * Ae[j][l] for l in range(k)) + b * Ce[i][j]
+            s = k * (abs(a) * sum(abs(Ae[i][l] * Ae[j][l]) for l in range(k)) + abs(b * Ce[i][j]))
+            assert abs(ddF(g) - e) <= 8 * Fraction(2) ** -104 * s + Fraction(1, 10 ** 300), (i, j)
