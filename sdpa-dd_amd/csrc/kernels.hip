@@ -350,6 +350,12 @@ __device__ __forceinline__ void tri_cm(int e, int nb, int &r, int &c) {
 // enumerated column by column, so the finished columns j <= k are whole
 // waves that skip their slot and the work per step follows the trailing
 // triangle; a thread issues all its LDS reads before the arithmetic.
+// FAST (the Ozaki form's large factorisations only): the pivot forms
+// 1/l_kk directly from the double reciprocal square root of a_kk and one
+// Newton step in dd, r = y + y (1 - a y^2) / 2 (about 2^-104 relative, the
+// accuracy of QD's square root), and l_kk = a_kk r off the pivot chain;
+// half the latency of a dd square root followed by a dd division.
+template <bool FAST>
 __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
     __shared__ dd S[PB][PB + 1];
     __shared__ dd rl[PB];
@@ -394,6 +400,14 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
         if (dd_le(a, mk(0.0)) || dd_isnan(a)) {
             failed = 1;
             *info = K + k + 1;
+        } else if (FAST) {
+            const double y = rsqrt(a.hi);
+            const dd e = d_sub_dd(1.0, dd_mul(a, dd_sqr_d(y)));
+            double lo;
+            const double hi = two_sum(y, e.hi * (0.5 * y), lo);
+            const dd r = dd{hi, lo};
+            rl[k] = r;
+            A[(K + k) + (K + k) * lda] = dd_mul(a, r);
         } else {
             dd l = dd_sqrt(a);
             A[(K + k) + (K + k) * lda] = l;
@@ -539,7 +553,7 @@ void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *
         idx_t below = n - K - nb;
         {
             StatScope st("potrf", s, (double)nb * nb * nb / 3.0);
-            k_potrf_diag<<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
+            k_potrf_diag<false><<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
         }
         if (below > 0) {
             StatScope st("panel", s, (double)below * nb * nb);
@@ -572,6 +586,9 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
     // only where the trailing updates are long (measured: m = 4551 gains 5%,
     // m = 800 loses 14% of the Cholesky time)
     if (n <= 1024) la = nullptr;
+    // the pivot chain (m square roots and reciprocals in sequence) is the
+    // critical path of the large factorisations: those take the faster pivot
+    const bool fast_pivot = n > 1024;
     k_zero_i64<<<1, 1, 0, s>>>(d_info);
     bool helper_pending = false;  // an update on la->hs that later work on s must wait for
     for (idx_t P = 0; P < n; P += OZ_NB) {
@@ -581,7 +598,10 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
             const idx_t below = n - K - nb;
             {
                 StatScope st("potrf", s, (double)nb * nb * nb / 3.0);
-                k_potrf_diag<<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
+                if (fast_pivot)
+                    k_potrf_diag<true><<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
+                else
+                    k_potrf_diag<false><<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
             }
             if (below > 0) {
                 StatScope st("panel", s, (double)below * nb * nb);
