@@ -738,9 +738,18 @@ class Solver {
     // X's factorisation on the main stream, Z's on the second one; work
     // enqueued on the main stream between start and join overlaps Z's.
     void enq_inv_chol_start() {
+        enq_inv_chol_fork();
+        enq_inv_chol_launch();
+    }
+    // the fork point (the factorisations wait for the work on s so far) ...
+    void enq_inv_chol_fork() {
         (void)hipEventRecord(ev_fork, s);
         (void)hipStreamWaitEvent(s2, ev_fork, 0);
         (void)hipStreamWaitEvent(s3, ev_fork, 0);
+    }
+    // ... and their launches, which may be enqueued after later work on s
+    // (their many small launches then do not hold that work back)
+    void enq_inv_chol_launch() {
         inv_chol_async(s3, X, invCholX, DLS1, nullptr, d_infos, d_infos + 2 * nB, d_rinv3);
         inv_chol_async(s2, Z, invCholZ, DLS2, invZ, d_infos + nB, d_infos + 2 * nB + 1, d_rinv2);
     }
@@ -1283,7 +1292,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         // values is used when a factorisation failed)
         {
             double t0 = now();
-            enq_inv_chol_start();
+            enq_inv_chol_fork();
             enq_xz();
             enq_residuals();
             enq_inner_C(X, slot_C2);
@@ -1292,6 +1301,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
                 enq_init();
             else
                 enq_diag();
+            enq_inv_chol_launch();
             enq_inv_chol_join();
             fetch_all();
             t_inverse += now() - t0;
