@@ -6,14 +6,25 @@ The forward pass builds the problem through ``sdpadd_problem_create`` and
 solves it with ``sdpadd_solve``; the backward pass calls
 ``sdpadd_adjoint_setup`` (the Schur matrix at the final iterate, factored
 once) and ``sdpadd_adjoint`` (implicit differentiation of the central-path
-equations).  Tensors are float64; the values enter the dd solver exactly
-(low word zero) and the high words of the dd results come back.  The full
-dd gradients are available from ``Problem.adjoint``.
+equations).
+
+Values are float64 tensors of shape (n,), entering the dd solver with a
+zero low word, or dd tensors of shape (n, 2) holding (hi, lo) pairs; with
+``dd=True`` the outputs are dd pairs too (p* of shape (2,), X* (nn, 2), y*
+(m, 2)), so the solution keeps its ~32 digits through torch.  A dd pair
+represents the value hi + lo; for autograd the high word carries the
+derivative and the low word none, so a cotangent is read from the high
+word and a dd input receives the same gradient in both words (the chain
+rule of v = hi + lo).  Gradients are float64 in torch's type; the full-dd
+vector-Jacobian products are ``vjp_dd``.
+
+A solve that does not end pdOPT (or one of the phases passed as ``allow``)
+raises: the implicit differentiation is of the optimality conditions.
 """
 import numpy as np
 import torch
 
-from . import Problem
+from . import Problem, SdpaddError
 
 
 class Structure:
@@ -45,7 +56,9 @@ class Structure:
         return st, t(b), t(cv), t(av)
 
     def problem(self, dev, b, cvals, avals):
-        dd = lambda v: np.stack([np.asarray(v, dtype=np.float64), np.zeros(len(v))], axis=-1)
+        def dd(v):
+            v = np.asarray(v, dtype=np.float64)
+            return v if v.ndim == 2 else np.stack([v, np.zeros(len(v))], axis=-1)
         ents, pos = [], 0
         for k, l, r, c in self.c_groups:
             ents.append((k, l, r, c, dd(cvals[pos:pos + len(r)])))
@@ -81,38 +94,87 @@ class Structure:
         return self._amap
 
 
+def _grads(prob, st, Xbar, ybar, pbar):
+    """Full-dd gradients with respect to b (m, 2), the stored C values
+    (nc, 2) and the stored A values (na, 2) for one dd cotangent."""
+    bbar, Cbar, Abar = prob.adjoint(Xbar=Xbar[None], ybar=ybar[None], pbar=pbar[None])
+    gc = []
+    for _, l, r, c in st.c_groups:
+        n, o = st.bs[l], st.boff[l]
+        gc.append(Cbar[0, o + r + n * c] * np.where(r != c, 2.0, 1.0)[:, None])
+    gc = np.concatenate(gc) if gc else np.zeros((0, 2))
+    idx, fac = st.a_map(prob)
+    return bbar[0], gc, Abar[0, idx] * fac[:, None]
+
+
 class SDPSolve(torch.autograd.Function):
-    """(p*, X*, y*) = SDPSolve.apply(b, cvals, avals, structure, device, params)."""
+    """(p*, X*, y*) = SDPSolve.apply(b, cvals, avals, structure, device, params, dd, allow)."""
 
     @staticmethod
-    def forward(ctx, b, cvals, avals, structure, dev, params):
+    def forward(ctx, b, cvals, avals, structure, dev, params, dd=False, allow=("pdOPT",)):
         prob = structure.problem(dev, b.detach().cpu().numpy(), cvals.detach().cpu().numpy(),
                                  avals.detach().cpu().numpy())
         res = prob.solve(params)
+        if res["phase"] not in allow:
+            prob.close()
+            raise SdpaddError("SDPSolve: the solve ended %s after %d iterations (gap %.3g); the derivative is of "
+                              "the optimality conditions and needs one of %s" %
+                              (res["phase"], res["iterations"], res["relgap"][0], ", ".join(allow)))
         X, y, _ = prob.solution()
-        ctx.prob, ctx.st, ctx.res = prob, structure, res
+        ctx.prob, ctx.st, ctx.res, ctx.dd = prob, structure, res, dd
+        ctx.shapes = (tuple(b.shape), tuple(cvals.shape), tuple(avals.shape))
         dv = b.device
-        p = torch.tensor(res["objValDual"][0], dtype=torch.float64, device=dv)
-        return p, torch.from_numpy(X[:, 0].copy()).to(dv), torch.from_numpy(y[:, 0].copy()).to(dv)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
+        if dd:
+            return t(np.array(res["objValDual"], dtype=np.float64)), t(X), t(y)
+        return (torch.tensor(res["objValDual"][0], dtype=torch.float64, device=dv), t(X[:, 0].copy()),
+                t(y[:, 0].copy()))
 
     @staticmethod
     def backward(ctx, gp, gX, gy):
         prob, st = ctx.prob, ctx.st
-        dd = lambda t, n: np.stack([np.zeros(n) if t is None else t.detach().cpu().numpy().reshape(n),
-                                    np.zeros(n)], axis=-1)[None]
-        bbar, Cbar, Abar = prob.adjoint(Xbar=dd(gX, st.nn), ybar=dd(gy, st.m), pbar=dd(gp, 1)[0])
-        dv = gX.device if gX is not None else gy.device
-        gb = torch.from_numpy(bbar[0, :, 0].copy()).to(dv)
-        gc = []
-        for _, l, r, c in st.c_groups:
-            n, o = st.bs[l], st.boff[l]
-            gc.append(Cbar[0, o + r + n * c, 0] * np.where(r != c, 2.0, 1.0))
-        gc = torch.from_numpy(np.concatenate(gc) if gc else np.zeros(0)).to(dv)
-        idx, fac = st.a_map(prob)
-        ga = torch.from_numpy(Abar[0, idx, 0] * fac).to(dv)
-        return gb, gc, ga, None, None, None
+
+        def cot(g, n):
+            # the cotangent of a value: the high word's (the low word carries no derivative)
+            v = np.zeros(n) if g is None else g.detach().cpu().numpy().reshape(n, -1)[:, 0]
+            return np.stack([v, np.zeros(n)], axis=-1)
+
+        gb, gc, ga = _grads(prob, st, cot(gX, st.nn), cot(gy, st.m), cot(gp, 1)[0])
+        dv = next(g.device for g in (gX, gy, gp) if g is not None)
+
+        def out(g, shape):
+            v = g[:, 0] + g[:, 1]  # float64 value of the dd gradient
+            if len(shape) == 2:  # a dd input: v = hi + lo, the same gradient for both words
+                v = np.stack([v, v], axis=-1)
+            return torch.from_numpy(np.ascontiguousarray(v)).to(dv)
+
+        bs, cs, as_ = ctx.shapes
+        return out(gb, bs), out(gc, cs), out(ga, as_), None, None, None, None, None
 
 
-def solve(b, cvals, avals, structure, dev, params=None):
-    """Differentiable (p*, X*, y*) of the SDP with these values."""
-    return SDPSolve.apply(b, cvals, avals, structure, dev, params)
+def solve(b, cvals, avals, structure, dev, params=None, dd=False, allow=("pdOPT",)):
+    """Differentiable (p*, X*, y*) of the SDP with these values (float64
+    tensors, or dd (hi, lo) pair tensors of shape (n, 2)); dd=True returns
+    the solution as dd pairs."""
+    return SDPSolve.apply(b, cvals, avals, structure, dev, params, dd, tuple(allow))
+
+
+def vjp_dd(b, cvals, avals, structure, dev, Xbar=None, ybar=None, pbar=None, params=None, allow=("pdOPT",)):
+    """Full-dd vector-Jacobian products of (X*, y*, p*) for one dd cotangent
+    (Xbar (nn, 2), ybar (m, 2), pbar (2,); None = zero): the gradients with
+    respect to b (m, 2), the stored C values (nc, 2) and the stored A values
+    (na, 2), each a dd pair.  Returns (gb, gc, ga, solve result)."""
+    as_np = lambda t: t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+    prob = structure.problem(dev, as_np(b), as_np(cvals), as_np(avals))
+    try:
+        res = prob.solve(params)
+        if res["phase"] not in allow:
+            raise SdpaddError("vjp_dd: the solve ended %s" % res["phase"])
+        z = lambda n: np.zeros((n, 2))
+        Xb = z(structure.nn) if Xbar is None else np.asarray(Xbar, dtype=np.float64).reshape(structure.nn, 2)
+        yb = z(structure.m) if ybar is None else np.asarray(ybar, dtype=np.float64).reshape(structure.m, 2)
+        pb = np.zeros(2) if pbar is None else np.asarray(pbar, dtype=np.float64).reshape(2)
+        gb, gc, ga = _grads(prob, structure, Xb, yb, pb)
+        return gb, gc, ga, res
+    finally:
+        prob.close()
