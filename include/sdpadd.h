@@ -55,6 +55,16 @@ int sdpadd_set_path(sdpadd_handle h, int path, int slices, int refine_steps);
  * the same operation sequence, so results do not depend on it; process-wide
  * (it exists so tests and tuning can reach either kernel at any size). */
 int sdpadd_set_tile_policy(sdpadd_handle h, int policy);
+/* Bound on each spin-wait of the cross-workgroup triangular solves (no
+ * counterpart in the reference): `spins` iterations of about 0.25 us, 0 =
+ * the default 2^22 (about a second).  The solves launch their workgroups
+ * plainly when the grid fits the CU count and assume they are resident
+ * together; a wait that runs out makes every workgroup leave and a
+ * single-workgroup solve, enqueued behind each of them, redo the solve from
+ * a saved copy of the right-hand side in Rtrsv's operation order, so results
+ * never depend on co-residency.  Process-wide; tests set a tiny bound to
+ * reach that recovery. */
+int sdpadd_set_wait_bound(sdpadd_handle h, unsigned spins);
 int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr);
 int sdpadd_free(sdpadd_handle h, void *d_ptr);
 int sdpadd_upload(sdpadd_handle h, void *d_dst, const void *src, size_t bytes);

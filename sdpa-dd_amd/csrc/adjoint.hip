@@ -162,8 +162,6 @@ int Adjoint::setup(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const
     (void)hipMemcpyAsync(d_y, y.data(), sizeof(dd) * m, hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(d_tcon, tcon.data(), sizeof(int) * ntask, hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(d_aoff, aoff.data(), sizeof(idx_t) * ntask, hipMemcpyHostToDevice, s);
-    sdk::zero_ints(s, ctx.d_tsw_sticky, 1);
-    *ctx.h_tsw_err = 0;
     const dd one = mk(1.0), zero = mk(0.0), mone = mk(-1.0);
     auto info_of = [&]() -> int64_t {
         int64_t info = 0;
@@ -203,13 +201,13 @@ int Adjoint::setup(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const
     // x := B^{-1} x with two refinement sweeps (r: residual scratch)
     auto solve_b = [&](const dd *rhs, dd *x, dd *r) {
         sdk::copy(s, m, rhs, x);
-        (void)ctx.trsv(s, 'N', m, d_L, m, x, false);
-        (void)ctx.trsv(s, 'T', m, d_L, m, x, false);
+        ctx.trsv(s, 'N', m, d_L, m, x);
+        ctx.trsv(s, 'T', m, d_L, m, x);
         for (int sweep = 0; sweep < 2; sweep++) {
             sdk::copy(s, m, rhs, r);
             sdk::gemv_n(s, m, m, mone, d_B, m, x, one, r);
-            (void)ctx.trsv(s, 'N', m, d_L, m, r, false);
-            (void)ctx.trsv(s, 'T', m, d_L, m, r, false);
+            ctx.trsv(s, 'N', m, d_L, m, r);
+            ctx.trsv(s, 'T', m, d_L, m, r);
             sdk::axpy(s, m, one, r, x);
         }
     };
@@ -286,9 +284,8 @@ int Adjoint::setup(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const
         return -3;
     }
     if ((rc = factor_b()) != 0) return rc;
-    ctx.trsv_poll(s);
-    if (hipStreamSynchronize(s) != hipSuccess || *ctx.h_tsw_err != 0) {
-        err = "adjoint: triangular solve did not complete";
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        err = "adjoint: device error";
         return -3;
     }
     ready = true;
@@ -303,8 +300,6 @@ int Adjoint::apply(Context &ctx, DeviceProblem &dp, int K, const dd *Xbar, const
     }
     if (K < 1) return 0;
     hipStream_t s = ctx.stream;
-    sdk::zero_ints(s, ctx.d_tsw_sticky, 1);
-    *ctx.h_tsw_err = 0;
     if (K > kcap) {
         kcap = 0;
         if (!grow(d_S, (size_t)K * nn) || !grow(d_P, (size_t)K * nn) || !grow(d_W, (size_t)K * nn) ||
@@ -336,13 +331,13 @@ int Adjoint::apply(Context &ctx, DeviceProblem &dp, int K, const dd *Xbar, const
         k_add2<<<(m + 255) / 256, 256, 0, s>>>(m, yh, ybar ? ybar + (idx_t)k * m : nullptr, pbar ? pbar[k] : zero,
                                                  dp.d_b);
         sdk::copy(s, m, yh, v);
-        (void)ctx.trsv(s, 'N', m, d_L, m, v, false);
-        (void)ctx.trsv(s, 'T', m, d_L, m, v, false);
+        ctx.trsv(s, 'N', m, d_L, m, v);
+        ctx.trsv(s, 'T', m, d_L, m, v);
         for (int sweep = 0; sweep < 2; sweep++) {
             sdk::copy(s, m, yh, r);
             sdk::gemv_n(s, m, m, mk(-1.0), d_B, m, v, one, r);
-            (void)ctx.trsv(s, 'N', m, d_L, m, r, false);
-            (void)ctx.trsv(s, 'T', m, d_L, m, r, false);
+            ctx.trsv(s, 'N', m, d_L, m, r);
+            ctx.trsv(s, 'T', m, d_L, m, r);
             sdk::axpy(s, m, one, r, v);
         }
         plus_A(s, dp, v, d_V + (idx_t)k * nn);
@@ -366,13 +361,8 @@ int Adjoint::apply(Context &ctx, DeviceProblem &dp, int K, const dd *Xbar, const
                                          d_aoff, dp.d_sp_row, dp.d_sp_col, dp.d_boff, dp.d_bsize, d_y, v, G, d_X,
                                          Abar + (idx_t)k * na);
     }
-    ctx.trsv_poll(s);
     if (hipStreamSynchronize(s) != hipSuccess) {
         err = "adjoint: device error";
-        return -3;
-    }
-    if (*ctx.h_tsw_err != 0) {
-        err = "adjoint: triangular solve did not complete";
         return -3;
     }
     return 0;

@@ -112,6 +112,11 @@ int sdpadd_set_tile_policy(sdpadd_handle h, int policy) {
     sdk::g_tile_policy = policy;
     return 0;
 }
+int sdpadd_set_wait_bound(sdpadd_handle h, unsigned spins) {
+    if (!h) return bad(h, "set_wait_bound: null handle");
+    sdk::g_trsv_spin = spins;
+    return 0;
+}
 int sdpadd_dd_vec_op(sdpadd_handle h, int op, int64_t n, const double *d_a, const double *d_b, double *d_out) {
     if (!h || op < 0 || op > 11 || n < 0 || (n > 0 && (!d_a || !d_b || !d_out))) return bad(h, "dd_vec_op: bad argument");
     sdk::dd_vec_op(h->c.stream, op, n, DP(d_a), DP(d_b), DP(d_out));
@@ -248,8 +253,7 @@ int sdpadd_Rtrsv(sdpadd_handle h, char uplo, char trans, char diag, int64_t n, c
         return bad(h, "Rtrsv: unsupported op");
     if (n < 0 || lda < std::max<int64_t>(1, n)) return bad(h, "Rtrsv: bad dimension");
     if (n == 0) return 0;
-    if (h->c.trsv(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x)) != 0)
-        return bad(h, "Rtrsv: a wait of the cooperative solve ran out");
+    h->c.trsv(h->c.stream, up(trans, 'N') ? 'N' : 'T', n, DP(d_A), lda, DP(d_x));
     return check(h);
 }
 int sdpadd_ozaki_trsv(sdpadd_handle h, char trans, int64_t n, const double *d_L, int64_t lda, double *d_x) {
@@ -258,16 +262,8 @@ int sdpadd_ozaki_trsv(sdpadd_handle h, char trans, int64_t n, const double *d_L,
     if (n == 0) return 0;
     hipStream_t s = h->c.stream;
     if (h->c.binv_prepare(s, n, DP(d_L), lda) != 0) return bad(h, "ozaki_trsv: workspace allocation failed");
-    sdk::zero_ints(s, h->c.d_tsw_sticky, 1);
-    (void)h->c.trsv_binv(s, up(trans, 'N') ? 'N' : 'T', n, DP(d_L), lda, DP(d_x));
-    h->c.trsv_poll(s);
-    int rc = check(h);
-    if (rc) return rc;
-    if (*h->c.h_tsw_err != 0) {
-        *h->c.h_tsw_err = 0;
-        return bad(h, "ozaki_trsv: a wait of the cooperative solve ran out");
-    }
-    return 0;
+    h->c.trsv_binv(s, up(trans, 'N') ? 'N' : 'T', n, DP(d_L), lda, DP(d_x));
+    return check(h);
 }
 int sdpadd_ozaki_matvec(sdpadd_handle h, char op, int64_t m, int64_t n, const double *alpha, const double *d_A,
                         int64_t lda, const double *d_x, const double *beta, double *d_y) {

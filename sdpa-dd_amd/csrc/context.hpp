@@ -29,14 +29,10 @@ struct Context {
     // workspace of the cooperative triangular solves (grown on demand)
     void *d_tsw = nullptr;
     size_t tsw_cap = 0;
-    int *d_tsw_sticky = nullptr;  // set by a cooperative solve whose wait ran out
-    int *h_tsw_err = nullptr;     // pinned copy of it
-    // Rtrsv on stream s: cooperative grid for n >= 512, one workgroup
-    // otherwise or when the cooperative launch is unavailable.  check:
-    // synchronise and return -1 if a wait of the cooperative solve ran out
-    // (the flag is then cleared); otherwise returns 0 without waiting and
-    // the caller reads the flag later through trsv_poll() and a sync.
-    int trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x, bool check = true);
+    // Rtrsv on stream s, stream-ordered: a grid of workgroups for n >= 512,
+    // one workgroup otherwise or when the workspace or the launch is
+    // unavailable (the same operation order either way).
+    void trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x);
     // The Ozaki form's block-inverse solves: operators of the factor L (both
     // directions) formed by binv_prepare, used by trsv_binv for that n.
     dd *d_binv = nullptr;
@@ -45,11 +41,7 @@ struct Context {
     dd *d_binv_lt = nullptr;  // L^T of the factor the block operators belong to
     size_t binv_lt_cap = 0;
     int binv_prepare(hipStream_t s, sdk::idx_t n, const dd *L, sdk::idx_t lda);
-    int trsv_binv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x);
-    // Enqueue the copy of the sticky flag into *h_tsw_err.
-    void trsv_poll(hipStream_t s) {
-        (void)hipMemcpyAsync(h_tsw_err, d_tsw_sticky, sizeof(int), hipMemcpyDeviceToHost, s);
-    }
+    void trsv_binv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x);
     int oz_slices = 16;
     int refine_steps = 2;
     // general workspace of the BLAS-level entry points (grown on demand,

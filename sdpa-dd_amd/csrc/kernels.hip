@@ -663,12 +663,27 @@ static constexpr int SB = 32;
 static constexpr int TS_NT = 512;          // threads of the triangular-solve kernels
 static constexpr int TS_UPD = TS_NT - 64;  // update threads (waves 1..)
 
+// The recovery of a cross-workgroup solve (below) whose bounded wait ran out:
+// with redo non-null the single-workgroup solves run only when *redo (that
+// solve's error word) is set, and then start from the saved right-hand side
+// x0.  The word was written by an earlier kernel on the stream, so a plain
+// load sees it; the restore is made visible to the workgroup by the barrier.
+__device__ __forceinline__ bool trsv_redo(const int *redo, const dd *x0, dd *x, idx_t n) {
+    if (!redo) return true;
+    if (*redo == 0) return false;
+    for (idx_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = x0[i];
+    __syncthreads();
+    return true;
+}
+
 // x := L^{-1} x ('L','N','N'): for j ascending, if x_j != 0 { x_j /= L_jj;
 // x_i -= x_j * L_ij for i > j }.
-__global__ __launch_bounds__(TS_NT) void k_trsv_ln(idx_t n, const dd *L, idx_t lda, dd *x) {
+__global__ __launch_bounds__(TS_NT) void k_trsv_ln(idx_t n, const dd *L, idx_t lda, dd *x, const int *redo,
+                                                   const dd *x0) {
     __shared__ dd D[SB][SB + 1];
     __shared__ dd v[2][SB];
     __shared__ int nzv[2][SB];
+    if (!trsv_redo(redo, x0, x, n)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int nblk = (int)((n + SB - 1) / SB);
     for (int R = 0; R < nblk; R++) {
@@ -748,9 +763,11 @@ __global__ __launch_bounds__(TS_NT) void k_trsv_ln(idx_t n, const dd *L, idx_t l
 
 // x := L^{-T} x ('L','T','N'): for j descending, x_j = (x_j - sum over
 // i > j, descending, of L_ij x_i) / L_jj.  Blocks from the bottom.
-__global__ __launch_bounds__(TS_NT) void k_trsv_lt(idx_t n, const dd *L, idx_t lda, dd *x) {
+__global__ __launch_bounds__(TS_NT) void k_trsv_lt(idx_t n, const dd *L, idx_t lda, dd *x, const int *redo,
+                                                   const dd *x0) {
     __shared__ dd D[SB][SB + 1];
     __shared__ dd v[2][SB];
+    if (!trsv_redo(redo, x0, x, n)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int nblk = (int)((n + SB - 1) / SB);
     // block R covers rows [lo(R), hi(R)); R = 0 is the bottom block
@@ -838,20 +855,28 @@ __global__ __launch_bounds__(TS_NT) void k_trsv_lt(idx_t n, const dd *L, idx_t l
 //   updater u, round J: wait for block J; apply it to its rows in blocks
 //     >= J+2 (in the reference's column order per row); publish progress J+1.
 // Every element sees the same operation sequence as the column loop of
-// Rtrsv.  All waits are bounded spins: a spin that runs out sets the error
-// word and every workgroup leaves (the caller reports the error), so the
-// grid always drains.  The cooperative launch guarantees co-residency.
+// Rtrsv.  The waits assume every workgroup of the grid is resident at once.
+// A grid no larger than the CU count is launched plainly (cooperative
+// launches of these grids made a whole theta solve about 4% slower); a
+// larger one cooperatively.  Should a workgroup still not
+// be resident (CUs or LDS held by other streams or processes), a bounded
+// spin runs out, sets the error word and every workgroup leaves, so the
+// grid always drains; a single-workgroup solve enqueued behind it then
+// restores the saved right-hand side and solves again, in the same
+// operation order, so the result never depends on co-residency.
 static constexpr int TC_ROWS = 256;           // rows per updater workgroup
-static constexpr unsigned TC_SPIN = 1u << 22;  // spin bound per wait (about a second)
+static constexpr unsigned TC_SPIN = 1u << 22;  // default spin bound per wait (about a second)
+unsigned g_trsv_spin = 0;
+static unsigned trsv_spin() { return g_trsv_spin ? g_trsv_spin : TC_SPIN; }
 __device__ __forceinline__ int tc_load(const int *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ void tc_store(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
 // Spin until *p >= target; false when the error word is set or the bound
-// runs out (then the error word and the sticky word are set).  The spin
+// runs out (then it sets the error word).  The spin
 // reads with relaxed agent-scope loads and takes the acquire once, after
 // the flag is seen: an acquire per read would invalidate the L2 of the
 // waiting workgroup's XCD on every spin.
-__device__ bool tc_wait(const int *p, int target, int *err, int *sticky) {
-    for (unsigned it = 0; it < TC_SPIN; it++) {
+__device__ bool tc_wait(const int *p, int target, int *err, unsigned spin) {
+    for (unsigned it = 0; it < spin; it++) {
         if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             return true;
@@ -860,7 +885,6 @@ __device__ bool tc_wait(const int *p, int target, int *err, int *sticky) {
         __builtin_amdgcn_s_sleep(1);
     }
     tc_store(err, 1);
-    tc_store(sticky, 1);
     return false;
 }
 
@@ -873,7 +897,7 @@ __device__ bool tc_wait(const int *p, int target, int *err, int *sticky) {
 // by updater u, [1 + nupd] error word, all zero at launch.  vg/nzg: solved
 // values of every block and their nonzero flags.
 __global__ __launch_bounds__(256) void k_trsv_ln_coop(idx_t n, const dd *L, idx_t lda, dd *x, int *flags, int nupd, dd *vg,
-                                                      int *nzg, int *sticky) {
+                                                      int *nzg, unsigned spin) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = (int)((n + SB - 1) / SB);
     int *err = flags + 1 + nupd;
@@ -906,7 +930,7 @@ __global__ __launch_bounds__(256) void k_trsv_ln_coop(idx_t n, const dd *L, idx_
                 int ok = 1;
                 if (lane == 0 && R >= 2) {
                     const int u0 = (int)(b / TC_ROWS), u1 = (int)((b + nb - 1) / TC_ROWS);
-                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, R - 1, err, sticky);
+                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, R - 1, err, spin);
                 }
                 ok = __builtin_amdgcn_readlane(ok, 0);
                 if (!ok) {
@@ -970,7 +994,7 @@ __global__ __launch_bounds__(256) void k_trsv_ln_coop(idx_t n, const dd *L, idx_
 #pragma unroll
             for (int q = 0; q < SB; q++) a[q] = L[i + (bp + q) * lda];
         }
-        if (tid == 0) go = tc_wait(flags, J + 1, err, sticky);
+        if (tid == 0) go = tc_wait(flags, J + 1, err, spin);
         __syncthreads();
         if (!go) return;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -989,8 +1013,8 @@ __global__ __launch_bounds__(256) void k_trsv_ln_coop(idx_t n, const dd *L, idx_
 
 // Cooperative L^{-T} x in the Rtrsv('L','T','N') order: blocks from the
 // bottom (block R = rows [blo(R), n - 32R)), otherwise as k_trsv_ln_coop.
-__global__ __launch_bounds__(256) void k_trsv_lt_coop(idx_t n, const dd *L, idx_t lda, dd *x, int *flags, int nupd, dd *vg,
-                                                      int *sticky) {
+__global__ __launch_bounds__(256) void k_trsv_lt_coop(idx_t n, const dd *L, idx_t lda, dd *x, int *flags, int nupd,
+                                                      dd *vg, unsigned spin) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = (int)((n + SB - 1) / SB);
     int *err = flags + 1 + nupd;
@@ -1023,7 +1047,7 @@ __global__ __launch_bounds__(256) void k_trsv_lt_coop(idx_t n, const dd *L, idx_
                 int ok = 1;
                 if (lane == 0 && R >= 2) {
                     const int u0 = (int)(b / TC_ROWS), u1 = (int)((b + nb - 1) / TC_ROWS);
-                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, R - 1, err, sticky);
+                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, R - 1, err, spin);
                 }
                 ok = __builtin_amdgcn_readlane(ok, 0);
                 if (!ok) {
@@ -1074,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_trsv_lt_coop(idx_t n, const dd *L, idx_
 #pragma unroll
             for (int q = 0; q < SB; q++) a[q] = L[(bp + q) + k * lda];
         }
-        if (tid == 0) go = tc_wait(flags, J + 1, err, sticky);
+        if (tid == 0) go = tc_wait(flags, J + 1, err, spin);
         __syncthreads();
         if (!go) return;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1090,10 +1114,25 @@ __global__ __launch_bounds__(256) void k_trsv_lt_coop(idx_t n, const dd *L, idx_
     if (tid == 0) tc_store(flags + 1 + u, nblk);
 }
 
-// Workspace of the cooperative solves: flags, then the solved block values.
+// Workspace of the cross-workgroup solves (both forms): the solved block
+// values, the saved right-hand side, the nonzero flags, then the wait words.
+struct CoopWork {
+    dd *vg, *x0;
+    int *nzg, *flags;
+};
+static CoopWork coop_work(void *work, idx_t n) {
+    const idx_t nblk = (n + SB - 1) / SB;
+    char *w = (char *)work;
+    CoopWork cw;
+    cw.vg = (dd *)(w + 256 - (((uintptr_t)w) & 255));
+    cw.x0 = cw.vg + nblk * SB;
+    cw.nzg = (int *)(cw.x0 + nblk * SB);
+    cw.flags = cw.nzg + nblk * SB;
+    return cw;
+}
 size_t trsv_coop_bytes(idx_t n) {
     const idx_t nblk = (n + SB - 1) / SB, nupd = (n + TC_ROWS - 1) / TC_ROWS;
-    return 256 + sizeof(int) * (2 + nupd) + sizeof(dd) * nblk * SB + sizeof(int) * nblk * SB;
+    return 256 + sizeof(dd) * 2 * nblk * SB + sizeof(int) * nblk * SB + sizeof(int) * (2 + nupd);
 }
 __global__ void k_zero_ints(int *p, int cnt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1102,46 +1141,51 @@ __global__ void k_zero_ints(int *p, int cnt) {
 void zero_ints(hipStream_t s, int *p, int cnt) {
     if (cnt > 0) k_zero_ints<<<cdiv(cnt, 256), 256, 0, s>>>(p, cnt);
 }
-// Stream-ordered; returns 0, or 1 when the launch fails (the caller then
-// uses the single-workgroup kernel).  A wait
-// that runs out sets *sticky (device) and leaves x incomplete.
-int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, void *work, int *sticky) {
-    if (n <= 0) return 0;
-    const int nupd = (int)((n + TC_ROWS - 1) / TC_ROWS);
-    const idx_t nblk = (n + SB - 1) / SB;
-    char *w = (char *)work;
-    dd *vg = (dd *)(w + 256 - (((uintptr_t)w) & 255));
-    int *nzg = (int *)(vg + nblk * SB);
-    int *flags = nzg + nblk * SB;
-    zero_ints(s, flags, 2 + nupd);
-    StatScope st("trsv", s, (double)n * n);
-    dim3 grid(1 + nupd), block(256);
-    // All workgroups must be resident together.  A grid no larger than the
-    // CU count is (two of these workgroups fit one CU), so it takes a plain
-    // launch; a larger one needs the cooperative launch's guarantee.  The
-    // waits are bounded either way.
+// A cross-workgroup solve over `grid` workgroups of 256 threads: a plain
+// launch when the grid fits the CU count, a cooperative one otherwise.
+static hipError_t launch_coop(const void *fn, dim3 grid, void **args, hipStream_t s) {
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    if ((int)grid.x <= cus) return hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
+    return hipLaunchCooperativeKernel(fn, grid, dim3(256), args, 0, s);
+}
+// The recovery enqueued behind every cross-workgroup solve: the
+// single-workgroup solve in Rtrsv's order, a no-op unless the error word
+// `err` of that solve is set.
+static void trsv_recover(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, const int *err,
+                         const dd *x0) {
+    if (trans == 'N' || trans == 'n')
+        k_trsv_ln<<<1, TS_NT, 0, s>>>(n, L, lda, x, err, x0);
+    else
+        k_trsv_lt<<<1, TS_NT, 0, s>>>(n, L, lda, x, err, x0);
+}
+// Stream-ordered; returns 0, or 1 when the launch fails (nothing ran; the
+// caller then uses the single-workgroup kernel).
+int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, void *work) {
+    if (n <= 0) return 0;
+    const int nupd = (int)((n + TC_ROWS - 1) / TC_ROWS);
+    const CoopWork cw = coop_work(work, n);
+    dd *vg = cw.vg;
+    int *nzg = cw.nzg, *flags = cw.flags;
+    zero_ints(s, flags, 2 + nupd);
+    StatScope st("trsv", s, (double)n * n);
+    copy(s, n, x, cw.x0);
     const bool ln = (trans == 'N' || trans == 'n');
+    unsigned spin = trsv_spin();
     hipError_t e;
-    if ((int)grid.x <= cus) {
-        if (ln)
-            k_trsv_ln_coop<<<grid, block, 0, s>>>(n, L, lda, x, flags, nupd, vg, nzg, sticky);
-        else
-            k_trsv_lt_coop<<<grid, block, 0, s>>>(n, L, lda, x, flags, nupd, vg, sticky);
-        e = hipGetLastError();
-    } else if (ln) {
-        void *args[] = {&n, &L, &lda, &x, &flags, (void *)&nupd, &vg, &nzg, &sticky};
-        e = hipLaunchCooperativeKernel((const void *)k_trsv_ln_coop, grid, block, args, 0, s);
+    if (ln) {
+        void *args[] = {&n, &L, &lda, &x, &flags, (void *)&nupd, &vg, &nzg, &spin};
+        e = launch_coop((const void *)k_trsv_ln_coop, dim3(1 + nupd), args, s);
     } else {
-        void *args[] = {&n, &L, &lda, &x, &flags, (void *)&nupd, &vg, &sticky};
-        e = hipLaunchCooperativeKernel((const void *)k_trsv_lt_coop, grid, block, args, 0, s);
+        void *args[] = {&n, &L, &lda, &x, &flags, (void *)&nupd, &vg, &spin};
+        e = launch_coop((const void *)k_trsv_lt_coop, dim3(1 + nupd), args, s);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return 1;
     }
+    trsv_recover(s, trans, n, L, lda, x, flags + 1 + nupd, cw.x0);
     return 0;
 }
 
@@ -1256,7 +1300,7 @@ __global__ __launch_bounds__(256) void k_blk_ops(idx_t n, int trans, const dd *L
 // flags, vg and the owner workgroups as in k_trsv_ln_coop / k_trsv_lt_coop;
 // an owner applies sequence block J to its rows in blocks >= J + BI_D + 1.
 __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd *L, idx_t lda, dd *x, const dd *ops,
-                                                   int *flags, int nupd, dd *vg, int *sticky, const dd *LT) {
+                                                   int *flags, int nupd, dd *vg, const dd *LT, unsigned spin) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = (int)((n + BB - 1) / BB);
     int *err = flags + 1 + nupd;
@@ -1288,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd 
                 int ok = 1;
                 if (lane == 0 && t - BI_D - 1 >= 0) {
                     const int u0 = (int)(lo / TC_ROWS), u1 = (int)((lo + nb - 1) / TC_ROWS);
-                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, t - BI_D, err, sticky);
+                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, t - BI_D, err, spin);
                 }
                 ok = __builtin_amdgcn_readlane(ok, 0);
                 if (!ok) {
@@ -1379,7 +1423,7 @@ __global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd 
         load_half(0);
         if (J + 1 > avail) {
             if (tid == 0) {
-                go = tc_wait(flags, J + 1, err, sticky) ? tc_load(flags) : 0;
+                go = tc_wait(flags, J + 1, err, spin) ? tc_load(flags) : 0;
             }
             __syncthreads();
             avail = go;
@@ -1436,32 +1480,27 @@ void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n
     k_blk_ops<<<dim3(nblk, BI_D + 1), 256, 0, s>>>(n, 1, L, lda, inv, ops_t);
 }
 int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, const dd *ops, void *work,
-              int *sticky, const dd *lt) {
+              const dd *lt) {
+    static_assert(BB == SB, "the block-inverse solves share the cross-workgroup workspace layout");
     if (n <= 0) return 0;
     const int nupd = (int)((n + TC_ROWS - 1) / TC_ROWS);
-    const idx_t nblk = (n + BB - 1) / BB;
-    char *w = (char *)work;
-    dd *vg = (dd *)(w + 256 - (((uintptr_t)w) & 255));
-    int *flags = (int *)(vg + nblk * BB) + nblk * BB;
+    const CoopWork cw = coop_work(work, n);
+    dd *vg = cw.vg;
+    int *flags = cw.flags;
     zero_ints(s, flags, 2 + nupd);
     StatScope st("trsv", s, (double)n * n);
+    copy(s, n, x, cw.x0);
     const int tr = (trans == 'T' || trans == 't') ? 1 : 0;
-    dim3 grid(1 + nupd), block(256);
-    int dev = 0, cus = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
-    hipError_t e;
-    if ((int)grid.x <= cus) {
-        k_trsv_binv<<<grid, block, 0, s>>>(n, tr, L, lda, x, ops, flags, nupd, vg, sticky, lt);
-        e = hipGetLastError();
-    } else {
-        void *args[] = {&n, (void *)&tr, &L, &lda, &x, &ops, &flags, (void *)&nupd, &vg, &sticky, (void *)&lt};
-        e = hipLaunchCooperativeKernel((const void *)k_trsv_binv, grid, block, args, 0, s);
-    }
+    unsigned spin = trsv_spin();
+    void *args[] = {&n, (void *)&tr, &L, &lda, &x, &ops, &flags, (void *)&nupd, &vg, (void *)&lt, &spin};
+    const hipError_t e = launch_coop((const void *)k_trsv_binv, dim3(1 + nupd), args, s);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return 1;
     }
+    // the recovery solves in Rtrsv's order: the Ozaki form refines the
+    // result against B, and the two orders agree to dd rounding
+    trsv_recover(s, tr ? 'T' : 'N', n, L, lda, x, flags + 1 + nupd, cw.x0);
     return 0;
 }
 
@@ -1469,9 +1508,9 @@ void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *
     if (n <= 0) return;
     StatScope st("trsv", s, (double)n * n);
     if (trans == 'N' || trans == 'n')
-        k_trsv_ln<<<1, TS_NT, 0, s>>>(n, L, lda, x);
+        k_trsv_ln<<<1, TS_NT, 0, s>>>(n, L, lda, x, nullptr, nullptr);
     else
-        k_trsv_lt<<<1, TS_NT, 0, s>>>(n, L, lda, x);
+        k_trsv_lt<<<1, TS_NT, 0, s>>>(n, L, lda, x, nullptr, nullptr);
 }
 
 // ---------------------------------------------- inverse of a lower triangle --

@@ -44,18 +44,19 @@ int inv_lower_oz(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ld
 
 // x := L^{-1} x ('N') or L^{-T} x ('T'), L lower non-unit.
 void trsv_lower(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x);
-// The same solve spread over a cooperative grid (large n), stream-ordered
-// with no host synchronisation.  work: at least trsv_coop_bytes(n) bytes of
-// device memory.  A bounded wait that runs out sets *sticky (a device int
-// the caller clears) and leaves x incomplete.  Returns 0, or 1 when the
-// cooperative launch is unavailable (nothing ran).
+// The same solve spread over a grid of workgroups that wait on each other
+// (large n), stream-ordered with no host synchronisation.  work: at least
+// trsv_coop_bytes(n) bytes of device memory.  Should the grid not be
+// resident at once, its bounded waits run out and a single-workgroup solve
+// enqueued behind it redoes the solve from a saved copy of x, so x is
+// always complete.  Returns 0, or 1 when the launch fails (nothing ran).
 size_t trsv_coop_bytes(idx_t n);
-int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, void *work, int *sticky);
+int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, void *work);
 // The Ozaki form's solves: block-inverse cooperative L^{-1} x / L^{-T} x
 // (not Rtrsv's operation order).  trsv_binv_prepare forms the per-block
 // operators of both directions once per factor (ops_n, ops_t:
 // binv_ops_elems(n) dd each; inv: binv_ops_elems(n) dd of scratch);
-// trsv_binv uses the trsv_coop_bytes workspace and the sticky flag as
+// trsv_binv uses the trsv_coop_bytes workspace and the recovery of
 // trsv_lower_coop and returns 0, or 1 when the launch fails.  lt (n x n,
 // leading dimension n, may be null): prepare stores L^T there (the upper
 // triangle of lt is the lower triangle of L, transposed) and the L^T x
@@ -63,7 +64,7 @@ int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, 
 size_t binv_ops_elems(idx_t n);
 void trsv_binv_prepare(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *ops_n, dd *ops_t, dd *inv, dd *lt);
 int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x, const dd *ops, void *work,
-              int *sticky, const dd *lt);
+              const dd *lt);
 // p[0..cnt) := 0 with a kernel (no memset node in captured graphs).
 void zero_ints(hipStream_t s, int *p, int cnt);
 
@@ -167,6 +168,10 @@ void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned 
 // (default), 1 = always the 32 x 32 tiles, 2 = always the 64 x 64 tiles.
 // Both tilings give every output element the same operation sequence.
 extern int g_tile_policy;
+// Bound on each spin-wait of the cross-workgroup triangular solves
+// (iterations of about 0.25 us; 0 = the default of 2^22).  A wait that runs
+// out hands the solve to the single-workgroup recovery.
+extern unsigned g_trsv_spin;
 
 // Per-kernel-family timing with HIP events on the launch stream (off by
 // default).  Families: "gemm", "ozaki", "potrf", "panel", "trsm", "trmm",

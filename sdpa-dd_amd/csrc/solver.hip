@@ -28,10 +28,6 @@ int Context::init(int dev) {
     if (hipMalloc((void **)&d_rinv, sizeof(dd) * 64) != hipSuccess) return -1;
     if (hipMalloc((void **)&d_max_work, sizeof(dd) * 256) != hipSuccess) return -1;
     if (hipMemset(d_counter, 0, sizeof(unsigned) * 4) != hipSuccess) return -1;
-    if (hipMalloc((void **)&d_tsw_sticky, 256) != hipSuccess) return -1;
-    if (hipMemset(d_tsw_sticky, 0, 256) != hipSuccess) return -1;
-    if (hipHostMalloc((void **)&h_tsw_err, sizeof(int)) != hipSuccess) return -1;
-    *h_tsw_err = 0;
     return 0;
 }
 void *Context::workspace(size_t bytes) {
@@ -48,7 +44,7 @@ void *Context::workspace(size_t bytes) {
     }
     return d_ws;
 }
-int Context::trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x, bool check) {
+void Context::trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x) {
     if (n >= 512) {
         const size_t need = sdk::trsv_coop_bytes(n);
         bool ok = true;
@@ -59,18 +55,9 @@ int Context::trsv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx
             ok = hipMalloc(&d_tsw, need) == hipSuccess;
             if (ok) tsw_cap = need;
         }
-        if (ok && sdk::trsv_lower_coop(s, trans, n, L, lda, x, d_tsw, d_tsw_sticky) == 0) {
-            if (!check) return 0;
-            trsv_poll(s);
-            if (hipStreamSynchronize(s) != hipSuccess) return -1;
-            if (*h_tsw_err == 0) return 0;
-            sdk::zero_ints(s, d_tsw_sticky, 1);
-            *h_tsw_err = 0;
-            return -1;
-        }
+        if (ok && sdk::trsv_lower_coop(s, trans, n, L, lda, x, d_tsw) == 0) return;
     }
     sdk::trsv_lower(s, trans, n, L, lda, x);
-    return 0;
 }
 int Context::binv_prepare(hipStream_t s, sdk::idx_t n, const dd *L, sdk::idx_t lda) {
     const size_t e = sdk::binv_ops_elems(n), need_ops = 3 * e;
@@ -105,14 +92,11 @@ int Context::binv_prepare(hipStream_t s, sdk::idx_t n, const dd *L, sdk::idx_t l
     sdk::trsv_binv_prepare(s, n, L, lda, d_binv, d_binv + e, d_binv + 2 * e, d_binv_lt);
     return 0;
 }
-int Context::trsv_binv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x) {
+void Context::trsv_binv(hipStream_t s, char trans, sdk::idx_t n, const dd *L, sdk::idx_t lda, dd *x) {
     const size_t e = sdk::binv_ops_elems(n);
     const bool tr = (trans == 'T' || trans == 't');
-    if (n == binv_n &&
-        sdk::trsv_binv(s, trans, n, L, lda, x, d_binv + (tr ? e : 0), d_tsw, d_tsw_sticky, d_binv_lt) == 0)
-        return 0;
+    if (n == binv_n && sdk::trsv_binv(s, trans, n, L, lda, x, d_binv + (tr ? e : 0), d_tsw, d_binv_lt) == 0) return;
     sdk::trsv_lower(s, trans, n, L, lda, x);
-    return 0;
 }
 void Context::destroy() {
     if (d_dot_work) (void)hipFree(d_dot_work);
@@ -132,11 +116,7 @@ void Context::destroy() {
     if (d_binv_lt) (void)hipFree(d_binv_lt);
     d_binv_lt = nullptr;
     binv_lt_cap = 0;
-    if (d_tsw_sticky) (void)hipFree(d_tsw_sticky);
-    if (h_tsw_err) (void)hipHostFree(h_tsw_err);
     d_tsw = nullptr;
-    d_tsw_sticky = nullptr;
-    h_tsw_err = nullptr;
     tsw_cap = 0;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -620,26 +600,19 @@ class Solver {
     // max|B| max|dy| (below it a correction only moves dy within rounding
     // noise, which changes the iterates without improving them) and is kept
     // only if it at least halves max|r|.
-    // The Schur solves do not wait for the device: a cooperative solve whose
-    // bounded wait ran out sets a sticky device flag, copied to the host after
-    // every solve and checked after the next synchronisation.
+    // The Schur solves do not wait for the device.
     bool binv = false;
     void trsv_chk(char trans, idx_t n, const dd *L, idx_t lda, dd *x) {
         if (binv)
-            (void)ctx.trsv_binv(s, trans, n, L, lda, x);
+            ctx.trsv_binv(s, trans, n, L, lda, x);
         else
-            (void)ctx.trsv(s, trans, n, L, lda, x, false);
+            ctx.trsv(s, trans, n, L, lda, x);
     }
-    bool trsv_failed() const { return *ctx.h_tsw_err != 0; }
     void residual(const dd *x, dd *r) {
         sdk::copy(s, m, g, r);
         sdk::gemv_n_fast(s, m, m, mk(-1.0), bSave, m, x, mk(1.0), r);
     }
     void schur_solve() {
-        schur_solve_refined();
-        ctx.trsv_poll(s);
-    }
-    void schur_solve_refined() {
         sdk::copy(s, m, g, dy);
         trsv_chk('N', m, bMat, m, dy);
         trsv_chk('T', m, bMat, m, dy);
@@ -979,8 +952,6 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         if (!(d_partial = alloc((idx_t)std::max(1, 3 * nB) * ctx.dot_T))) return -2;
     }
     double t_start = now();
-    sdk::zero_ints(s, ctx.d_tsw_sticky, 1);
-    *ctx.h_tsw_err = 0;
     for (dd **b : bufs) sdk::fill(s, nt, mk(0.0), *b);
 
     // Solutions::initialize (sdpa_dataset.cpp:35-60)
@@ -1217,11 +1188,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             enq_inner_b(y, slot_b2);
             enq_inner_C(X, slot_C2);
             prefetch_all();
-            bool eig_ok = min_eigen_xz(mx, mz);  // synchronises: the solves' flag and h_sc are current
-            if (trsv_failed()) {
-                res.message = "triangular solve did not complete";
-                break;
-            }
+            bool eig_ok = min_eigen_xz(mx, mz);  // synchronises: h_sc is current
             if (!eig_ok) {
                 res.message = "eigenvalue iteration did not converge";
                 break;
@@ -1346,7 +1313,6 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     }
     residuals(curNormP, curNormD);
     sync();
-    if (trsv_failed() && res.message.empty()) res.message = "triangular solve did not complete";
     res.t.total = now() - t_start;
     res.t.schur = t_schur;
     res.t.chol = t_chol;

@@ -51,6 +51,7 @@ def load_library(path=None):
         "sdpadd_set_path": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "sdpadd_malloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
         "sdpadd_set_tile_policy": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sdpadd_set_wait_bound": (ctypes.c_int, [vp, ctypes.c_uint]),
         "sdpadd_dd_vec_op": (ctypes.c_int, [vp, ctypes.c_int, c_i64, vp, vp, vp]),
         "sdpadd_stats_enable": (ctypes.c_int, [vp, ctypes.c_int]),
         "sdpadd_stats_reset": (ctypes.c_int, [vp]),
@@ -178,6 +179,12 @@ class Device:
     def set_tile_policy(self, policy):
         """0: GEMM/Ozaki tile shape by grid size; 1: always 32 x 32; 2: always 64 x 64."""
         self.check(self.lib.sdpadd_set_tile_policy(self.h, int(policy)), "set_tile_policy")
+
+    def set_wait_bound(self, spins):
+        """Bound on each spin-wait of the cross-workgroup triangular solves
+        (0 = default); a wait that runs out hands the solve to the
+        single-workgroup recovery (same results)."""
+        self.check(self.lib.sdpadd_set_wait_bound(self.h, int(spins)), "set_wait_bound")
 
     def dd_vec_op(self, op, a, b):
         """QD operator `op` (0..11, see sdpadd_dd_vec_op) elementwise on (n, 2) dd arrays."""

@@ -168,6 +168,43 @@ def test_ozaki_trsv(gpu, oracle, n, trans):
     assert np.max(err) <= 1e-26 * np.max(np.abs(ref[:, 0])), float(np.max(err) / np.max(np.abs(ref[:, 0])))
 
 
+@pytest.mark.parametrize("n", [512, 2049, 4551])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_trsv_wait_recovery(gpu, oracle, n, trans):
+    """The cross-workgroup solves with a spin bound of one iteration (set
+    from the host): waits run out as soon as a flag is not already up, every
+    workgroup leaves, and the single-workgroup solve enqueued behind redoes
+    the solve from the saved right-hand side.  Rtrsv stays bit-identical to
+    the oracle; the Ozaki form's block-inverse solve, which otherwise
+    differs from Rtrsv's order in the last bits, now equals it bit for bit,
+    which shows that the recovery ran."""
+    from ddgen import dd_from_hi
+    rng = np.random.default_rng(n + 7)
+    a = np.tril(rng.standard_normal((n, n))) * (0.5 / np.sqrt(n)) + np.diag(1.0 + rng.uniform(0, 1, n))
+    A = dd_from_hi(a, rng)
+    A[np.triu_indices(n, 1)] = 0.0
+    A = np.ascontiguousarray(A.transpose(1, 0, 2)).reshape(-1, 2)
+    x = rand_dd((n,), rng)
+    x[n // 3] = 0.0
+    ref = oracle.rtrsv(trans, n, A, n, x)
+    dA = gpu.to_device(A)
+    gpu.set_wait_bound(1)
+    try:
+        dx = gpu.to_device(x)
+        gpu.check(gpu.lib.sdpadd_Rtrsv(gpu.h, b"L", trans.encode(), b"N", n, dA.ptr, n, dx.ptr, 1), "Rtrsv")
+        assert same(dx.get(), ref)
+        dx = gpu.to_device(x)
+        gpu.check(gpu.lib.sdpadd_ozaki_trsv(gpu.h, trans.encode(), n, dA.ptr, n, dx.ptr), "ozaki_trsv")
+        assert same(dx.get(), ref)
+    finally:
+        gpu.set_wait_bound(0)
+    # the default bound: the block-inverse order again
+    dx = gpu.to_device(x)
+    gpu.check(gpu.lib.sdpadd_ozaki_trsv(gpu.h, trans.encode(), n, dA.ptr, n, dx.ptr), "ozaki_trsv")
+    got = dx.get()
+    assert np.max(_dd_err(got, ref)) <= 1e-26 * np.max(np.abs(ref[:, 0]))
+
+
 def _dd_err(got, ref):
     return np.abs((got[:, 0] - ref[:, 0]) + (got[:, 1] - ref[:, 1]))
 

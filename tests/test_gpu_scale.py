@@ -171,6 +171,34 @@ def test_full_size_workload_both_forms(gpu, workloads, which):
         assert abs(a - b) <= 1e-18 * max(1, abs(a)), (k, float(abs(a - b)))
 
 
+def test_wait_recovery_full_solves(gpu, workloads):
+    """Solves whose cross-workgroup triangular solves all hand over to the
+    single-workgroup recovery (spin bound of one iteration): the plain form,
+    whose recovery keeps Rtrsv's order, reproduces the default solve bit for
+    bit; the Ozaki form still reaches pdOPT with the same objectives to
+    1e-18."""
+    import mpmath
+    mpmath.mp.dps = 50
+    path, param = workloads["control"], workloads["param"]
+    say("control workload, default spin bound, both forms")
+    ref = _solve(gpu, path, param, 0)
+    ref_oz = _solve(gpu, path, param, 1)
+    gpu.set_wait_bound(1)
+    try:
+        say("control workload, spin bound 1, both forms")
+        got = _solve(gpu, path, param, 0)
+        got_oz = _solve(gpu, path, param, 1)
+    finally:
+        gpu.set_wait_bound(0)
+    for k in ("iterations", "phase", "mu", "objValPrimal", "objValDual", "relgap", "trace"):
+        assert got[k] == ref[k], (k, got[k], ref[k])
+    assert got_oz["phase"] == "pdOPT" and abs(got_oz["iterations"] - ref_oz["iterations"]) <= 2
+    for k in ("objValPrimal", "objValDual"):
+        a = mpmath.mpf(ref_oz[k][0]) + mpmath.mpf(ref_oz[k][1])
+        b = mpmath.mpf(got_oz[k][0]) + mpmath.mpf(got_oz[k][1])
+        assert abs(a - b) <= 1e-18 * max(1, abs(a)), (k, float(abs(a - b)))
+
+
 # ----------------------------------------------------------- tilings --
 def _rgemm(gpu, tb, m, n, k, alpha, A, B, ldb, beta, C):
     import ctypes
