@@ -229,6 +229,25 @@ int sdpadd_problem_formula(sdpadd_problem p, int *out);
  * NonZeroEffect (sdpa_newton.cpp:895-1068).  No counterpart in the
  * reference; it is the numerator of Schur dd-GFLOP/s. */
 int sdpadd_problem_schur_ddflops(sdpadd_problem p, double *ddflops);
+/* The Schur path of the next solves: 0 = the reference's rule
+ * (Chordal::ordering_bMat, sdpa_chordal.cpp:348-440: the sparse path when
+ * m > 100, more than 5 blocks counting LP entries, every block's constraint
+ * count <= m/2, aggregate sparsity pattern <= m^2/4 and factor pattern
+ * 2 nnz - m <= 0.4 m^2), 1 = dense, 2 = sparse whenever the structure
+ * allows (the thresholds 0, 0, 2.0, 2.0 of the reference's debugging switch,
+ * sdpa_chordal.cpp:53-58).  The sparse path replaces Newton::Make_bMat's
+ * dense B and Rpotrf/Rtrsv by the pattern of Newton::initialize_sparse_bMat
+ * (sdpa_newton.cpp:225-336), Lal::getCholesky (sdpa_linear.cpp:351-395) and
+ * Lal::solveSystems (654-790) with permuteVec (sdpa_newton.cpp:1257-1275);
+ * its ordering is an exact minimum degree (ties to the smaller constraint)
+ * in place of the reference's Spooles 2.2.  The Ozaki forms apply to the
+ * dense path only. */
+int sdpadd_problem_set_schur_mode(sdpadd_problem p, int mode);
+/* The path the current mode gives: *sparse 1 or 0, *nnz entries of the
+ * factor pattern (0 when dense), *aggregate entries of the aggregate
+ * sparsity pattern (0 when the rule stopped before forming it), ordering
+ * (m ints, may be null; written when sparse): newToOld. */
+int sdpadd_problem_schur_info(sdpadd_problem p, int *sparse, int64_t *nnz, int64_t *aggregate, int *ordering);
 /* Schur complement of Newton::compute_bMat_dense_SDP (sdpa_newton.cpp:979-1068)
  * for device X and Zinv (blocks concatenated, each n_l x n_l): d_B (m x m)
  * is overwritten. */

@@ -3,6 +3,7 @@
 // dd values are passed as interleaved (hi, lo) doubles, the dd_real layout.
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "sdpa_oracle.hpp"
 
@@ -14,6 +15,64 @@ static const dd_real *D(const double *p) { return reinterpret_cast<const dd_real
 extern "C" {
 
 void orc_set_dot_threads(int t) { g_dot_threads = t < 1 ? 1 : t; }
+
+// The sparse-Schur path's exact minimum degree on a CSR adjacency; order
+// (m ints) gets newToOld.  Returns the factor pattern's entries, diagonal
+// included.
+long orc_min_degree(int m, const int *ptr, const int *idx, int *order) {
+    std::vector<std::vector<int>> adj(m), rows;
+    for (int i = 0; i < m; i++) adj[i].assign(idx + ptr[i], idx + ptr[i + 1]);
+    std::vector<int> o = min_degree_order(m, adj, rows);
+    long nnz = 0;
+    for (int i = 0; i < m; i++) {
+        order[i] = o[i];
+        nnz += 1 + (long)rows[i].size();
+    }
+    return nnz;
+}
+
+// The sparse path's factorisation and solve on a symmetric dd matrix B
+// (m x m, column-major): the pattern of B's off-diagonal nonzeros ordered by
+// min_degree_order, the values gathered from B as Newton::permuteMat does,
+// Lal::getCholesky, then solve_sparse for g.  Writes x (m), order (newToOld)
+// and U (m x m, column-major, new indices): the factor pattern's entries
+// at (row, column), row <= column, its diagonal holding 1/sqrt(pivot) as
+// the reference stores it.  Returns the pattern's entries.
+long orc_sparse_factor_solve(int m, const double *B, const double *g, double *x, double *U, int *order) {
+    std::vector<std::vector<int>> adj(m), rows;
+    const dd_real *b = D(B);
+    for (int j = 0; j < m; j++)
+        for (int i = 0; i < m; i++)
+            if (i != j && (b[i + (size_t)m * j].x[0] != 0.0 || b[i + (size_t)m * j].x[1] != 0.0)) adj[i].push_back(j);
+    SparseSchur S;
+    S.on = true;
+    S.m = m;
+    S.ordering = min_degree_order(m, adj, rows);
+    S.reverse_ordering.assign(m, 0);
+    for (int i = 0; i < m; i++) S.reverse_ordering[S.ordering[i]] = i;
+    S.diagonalIndex.assign(m + 1, 0);
+    for (int p = 0; p < m; p++) S.diagonalIndex[p + 1] = S.diagonalIndex[p] + 1 + (int)rows[p].size();
+    for (int p = 0; p < m; p++) {
+        S.row_index.push_back(p);
+        S.column_index.push_back(p);
+        for (int c : rows[p]) {
+            S.row_index.push_back(p);
+            S.column_index.push_back(c);
+        }
+    }
+    for (size_t k = 0; k < S.row_index.size(); k++)
+        S.sp_ele.push_back(b[S.ordering[S.row_index[k]] + (size_t)m * S.ordering[S.column_index[k]]]);
+    chol_sparse(S);
+    std::vector<dd_real> gv(D(g), D(g) + m), xv;
+    solve_sparse(S, gv, xv);
+    for (int i = 0; i < m; i++) {
+        D(x)[i] = xv[i];
+        order[i] = S.ordering[i];
+    }
+    for (size_t q = 0; q < (size_t)m * m; q++) D(U)[q] = dd_real(0.0);
+    for (size_t k = 0; k < S.row_index.size(); k++) D(U)[S.row_index[k] + (size_t)m * S.column_index[k]] = S.sp_ele[k];
+    return (long)S.row_index.size();
+}
 int orc_get_dot_threads(void) { return g_dot_threads; }
 
 // Elementwise dd ops on n values: op 0 add (a+b), 1 sub, 2 mul, 3 div, 4 sqrt(a),

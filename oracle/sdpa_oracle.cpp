@@ -9,8 +9,11 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <cmath>
 #include <fstream>
+#include <set>
 #include <sstream>
+#include <stdexcept>
 
 namespace orc {
 
@@ -785,6 +788,267 @@ void schur_dense(const InputData &data, const DenseLinearSpace &X, const DenseLi
     }
 }
 
+// ---------------------------------------------------------- sparse Schur --
+// The chordal path of SDPA-C as SDPA-DD carries it: when the aggregate
+// sparsity of B is low (Chordal::ordering_bMat, sdpa_chordal.cpp:348-440),
+// B is held as the upper-triangular pattern of its factor in a
+// fill-reducing order (Newton::initialize_sparse_bMat, sdpa_newton.cpp:
+// 225-336), assembled pair by pair into that pattern
+// (make_aggrigateIndex_SDP/_LP, sdpa_newton.cpp:398-455, 597-665;
+// compute_bMat_sparse_SDP/_LP, 1070-1142, 1199-1219), factored by
+// Lal::getCholesky (sdpa_linear.cpp:351-395) and solved by
+// Lal::solveSystems (sparse, 654-790) between permuteVec and
+// reverse_permuteVec (sdpa_newton.cpp:1257-1275).  The reference orders with
+// Spooles 2.2's minimum degree, which is absent here; the ordering is the
+// exact minimum-degree elimination of min_degree_order.
+int g_schur_mode = 0;
+
+// Chordal::makeGraph (sdpa_chordal.cpp:141-238): vertex i is adjacent to
+// every constraint that shares an SDP block or an LP entry with it, itself
+// included; lists ascending without repeats.
+static std::vector<std::vector<int>> aggregate_graph(const InputData &d) {
+    std::vector<std::vector<int>> adj(d.m);
+    auto add = [&](const std::vector<int> &cons) {
+        for (int i : cons) adj[i].insert(adj[i].end(), cons.begin(), cons.end());
+    };
+    for (const auto &c : d.SDP_constraint) add(c);
+    for (const auto &c : d.LP_constraint) add(c);
+    for (auto &a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    return adj;
+}
+
+// Exact minimum degree: repeatedly eliminate the vertex with the fewest
+// remaining neighbours (ties: the smaller index); its remaining neighbours
+// become a clique.  Returns newToOld; rows[p] holds the new indices of the
+// neighbours of the p-th eliminated vertex at its elimination, ascending
+// (the strict upper part of row p of the factor).
+std::vector<int> min_degree_order(int m, const std::vector<std::vector<int>> &adj, std::vector<std::vector<int>> &rows) {
+    std::vector<std::set<int>> g(m);
+    for (int i = 0; i < m; i++)
+        for (int j : adj[i])
+            if (j != i) g[i].insert(j);
+    std::vector<int> order, newIdx(m, -1);
+    std::vector<bool> gone(m, false);
+    std::vector<std::vector<int>> nb_old(m);
+    for (int step = 0; step < m; step++) {
+        int v = -1;
+        size_t best = 0;
+        for (int i = 0; i < m; i++)
+            if (!gone[i] && (v < 0 || g[i].size() < best)) {
+                v = i;
+                best = g[i].size();
+            }
+        order.push_back(v);
+        newIdx[v] = step;
+        gone[v] = true;
+        nb_old[step].assign(g[v].begin(), g[v].end());
+        for (int u : g[v]) {
+            g[u].erase(v);
+            for (int w : g[v])
+                if (w != u) g[u].insert(w);
+        }
+        g[v].clear();
+    }
+    rows.assign(m, {});
+    for (int p = 0; p < m; p++) {
+        for (int u : nb_old[p]) rows[p].push_back(newIdx[u]);
+        std::sort(rows[p].begin(), rows[p].end());
+    }
+    return order;
+}
+
+// Decision, storage and pair locations.  mode 0: the reference's thresholds
+// (m > 100, more than 5 blocks, every block's constraint count <= m
+// sqrt(0.25), aggregate pattern <= 0.25 m^2, factor pattern 2 nnz - m <=
+// 0.4 m^2); mode 2: the thresholds of its debugging switch (0, 0, 2.0,
+// 2.0, sdpa_chordal.cpp:53-58), which take the sparse path whenever the
+// pattern allows; mode 1: dense.
+bool sparse_schur_setup(const InputData &d, SparseSchur &S, int mode) {
+    S = SparseSchur();
+    if (mode == 1) return false;
+    const int m = d.m;
+    const int nBlock2 = (int)d.blockStruct.size() + d.LP_nBlock;
+    const int m_thr = mode == 2 ? 0 : 100, b_thr = mode == 2 ? 0 : 5;
+    const double agg = mode == 2 ? 2.0 : 0.25, ext = mode == 2 ? 2.0 : 0.4;
+    if (m <= m_thr || nBlock2 <= b_thr) return false;
+    for (const auto &c : d.SDP_constraint)
+        if ((int)c.size() > m * std::sqrt(agg)) return false;
+    for (const auto &c : d.LP_constraint)
+        if ((int)c.size() > m * std::sqrt(agg)) return false;
+    const auto adj = aggregate_graph(d);
+    long long tsize = 0;
+    for (const auto &a : adj) tsize += (long long)a.size();
+    if (tsize > agg * m * m) return false;
+    std::vector<std::vector<int>> rows;
+    std::vector<int> order = min_degree_order(m, adj, rows);
+    long long nonzeros = 0;
+    for (const auto &r : rows) nonzeros += 1 + (long long)r.size();
+    if (nonzeros * 2 - m > ext * m * m) return false;
+    S.on = true;
+    S.m = m;
+    S.ordering = order;
+    S.reverse_ordering.assign(m, 0);
+    for (int i = 0; i < m; i++) S.reverse_ordering[order[i]] = i;
+    S.diagonalIndex.assign(m + 1, 0);
+    for (int p = 0; p < m; p++) S.diagonalIndex[p + 1] = S.diagonalIndex[p] + 1 + (int)rows[p].size();
+    const int nnz = S.diagonalIndex[m];
+    S.row_index.resize(nnz);
+    S.column_index.resize(nnz);
+    S.sp_ele.assign(nnz, dd_real(0.0));
+    for (int p = 0; p < m; p++) {
+        int k = S.diagonalIndex[p];
+        S.row_index[k] = p;
+        S.column_index[k] = p;
+        for (int c : rows[p]) {
+            k++;
+            S.row_index[k] = p;
+            S.column_index[k] = c;
+        }
+    }
+    auto locate = [&](int i, int j) {
+        int ii = S.reverse_ordering[i], jj = S.reverse_ordering[j];
+        if (ii > jj) std::swap(ii, jj);
+        const int *b = S.column_index.data() + S.diagonalIndex[ii], *e = S.column_index.data() + S.diagonalIndex[ii + 1];
+        const int *f = std::lower_bound(b, e, jj);
+        if (f == e || *f != jj) throw std::runtime_error("sparse Schur: pair outside the factor pattern");
+        return (int)(f - S.column_index.data());
+    };
+    const int nB = (int)d.blockStruct.size();
+    S.sdp.resize(nB);
+    for (int l = 0; l < nB; l++) {
+        const auto &cons = d.SDP_constraint[l];
+        const auto &bidx = d.SDP_blockIndex[l];
+        for (size_t k1 = 0; k1 < cons.size(); k1++) {
+            const int i = cons[k1], inz = d.A[i].SDP_sp_block[bidx[k1]].NonZeroEffect;
+            for (size_t k2 = 0; k2 < cons.size(); k2++) {
+                const int j = cons[k2], jnz = d.A[j].SDP_sp_block[bidx[k2]].NonZeroEffect;
+                if ((inz < jnz) || ((inz == jnz) && (i < j))) continue;
+                S.sdp[l].push_back({i, j, (int)k1, (int)k2, locate(i, j)});
+            }
+        }
+    }
+    S.lp.resize(d.LP_nBlock);
+    for (int l = 0; l < d.LP_nBlock; l++) {
+        const auto &cons = d.LP_constraint[l];
+        for (size_t k1 = 0; k1 < cons.size(); k1++)
+            for (size_t k2 = 0; k2 < cons.size(); k2++) {
+                const int i = cons[k1], j = cons[k2];
+                if (i < j) continue;
+                S.lp[l].push_back({i, j, (int)k1, (int)k2, locate(i, j)});
+            }
+    }
+    return true;
+}
+
+// compute_bMat_sparse_SDP then compute_bMat_sparse_LP (the pattern starts at zero).
+void schur_sparse(const InputData &data, const DenseLinearSpace &X, const DenseLinearSpace &invZ, SparseSchur &S,
+                  const std::vector<int> &useFormula) {
+    for (auto &v : S.sp_ele) v = 0.0;
+    const int nB = (int)data.blockStruct.size();
+    for (int l = 0; l < nB; l++) {
+        const DenseMatrix &xMat = X.SDP_block[l], &invzMat = invZ.SDP_block[l];
+        const auto &bidx = data.SDP_blockIndex[l];
+        DenseMatrix work1, work2;
+        work1.initialize(data.blockStruct[l]);
+        work2.initialize(data.blockStruct[l]);
+        int previous_i = -1, formula = F3;
+        bool hasG = false;
+        for (const auto &pr : S.sdp[l]) {
+            const SparseMatrix &Ai = data.A[pr.i].SDP_sp_block[bidx[pr.k1]];
+            if (pr.i != previous_i) {
+                formula = useFormula[pr.i * nB + l];
+                hasG = false;
+                if (formula == F1) {
+                    mul_sparse_dense(work1, Ai, invzMat);
+                    gemm_nn(work2, xMat, work1);
+                } else if (formula == F2) {
+                    mul_sparse_dense(work1, Ai, invzMat);
+                }
+            }
+            const SparseMatrix &Aj = data.A[pr.j].SDP_sp_block[bidx[pr.k2]];
+            dd_real value;
+            if (formula == F1)
+                value = inner_sparse_dense(Aj, work2);
+            else if (formula == F2)
+                value = calF2(work1, work2, xMat, Aj, hasG);
+            else
+                value = calF3(xMat, invzMat, Ai, Aj);
+            S.sp_ele[pr.loc] += value;
+            previous_i = pr.i;
+        }
+    }
+    for (int l = 0; l < (int)X.LP_block.size(); l++) {
+        const dd_real &x = X.LP_block[l], &iz = invZ.LP_block[l];
+        const auto &bidx = data.LP_blockIndex[l];
+        for (const auto &pr : S.lp[l]) {
+            const dd_real &Ai = data.A[pr.i].LP_sp_block[bidx[pr.k1]];
+            const dd_real &Aj = data.A[pr.j].LP_sp_block[bidx[pr.k2]];
+            dd_real value = x * iz * Ai * Aj;
+            S.sp_ele[pr.loc] += value;
+        }
+    }
+}
+
+// Lal::getCholesky(SparseMatrix&, int*) (sdpa_linear.cpp:351-395): row by
+// row, the diagonal replaced by 1/sqrt (0 for a negative pivot, and the run
+// goes on), the row scaled by it, then the row's outer product subtracted
+// from the rows of its columns.
+void chol_sparse(SparseSchur &S) {
+    auto &a = S.sp_ele;
+    const auto &col = S.column_index;
+    for (int i = 0; i < S.m; ++i) {
+        const int indexA1 = S.diagonalIndex[i], indexA2 = S.diagonalIndex[i + 1];
+        if (a[indexA1] < 0.0)
+            a[indexA1] = 0.0;
+        else
+            a[indexA1] = 1.0 / sqrt(a[indexA1]);
+        for (int k1 = indexA1 + 1; k1 < indexA2; ++k1) a[k1] *= a[indexA1];
+        for (int k1 = indexA1 + 1; k1 < indexA2; ++k1) {
+            const dd_real tmp = a[k1];
+            int k3 = S.diagonalIndex[col[k1]];
+            const int indexB2 = S.diagonalIndex[col[k1] + 1];
+            for (int k2 = k1; k2 < indexA2; ++k2) {
+                const dd_real tmp2 = a[k2];
+                const int tmp3 = col[k2];
+                for (; k3 < indexB2; ++k3)
+                    if (col[k3] == tmp3) {
+                        a[k3] -= tmp * tmp2;
+                        k3++;
+                        break;
+                    }
+            }
+        }
+    }
+}
+
+// permuteVec, Lal::solveSystems (sparse) and reverse_permuteVec.
+void solve_sparse(const SparseSchur &S, const std::vector<dd_real> &g, std::vector<dd_real> &dy) {
+    const int m = S.m, nnz = S.diagonalIndex[m];
+    std::vector<dd_real> x(m);
+    for (int k = 0; k < m; k++) x[k] = g[S.ordering[k]];
+    for (int index = 0; index < nnz; ++index) {
+        const int i = S.row_index[index], j = S.column_index[index];
+        const dd_real value = S.sp_ele[index];
+        if (i == j)
+            x[i] *= value;
+        else
+            x[j] -= value * x[i];
+    }
+    for (int index = nnz - 1; index >= 0; --index) {
+        const int i = S.row_index[index], j = S.column_index[index];
+        const dd_real value = S.sp_ele[index];
+        if (i == j)
+            x[i] *= value;
+        else
+            x[i] -= value * x[j];
+    }
+    dy.assign(m, dd_real(0.0));
+    for (int k = 0; k < m; k++) dy[S.ordering[k]] = x[k];
+}
+
 // ---------------------------------------------------------------- solver --
 namespace {
 struct Solutions {
@@ -912,6 +1176,15 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res, cons
     nw.r_zinvMat.initialize(bs, data.LP_nBlock);
     nw.x_rd_zinvMat.initialize(bs, data.LP_nBlock);
     nw.useFormula = compute_formula(data, dd_real(1.2));
+    // Chordal::ordering_bMat + Newton::initialize_bMat (sdpa_main.cpp:217-221)
+    SparseSchur sps;
+    const bool sparse = sparse_schur_setup(data, sps, g_schur_mode);
+    if (sparse) {
+        nw.bMat.clear();
+        res.schur_sparse = true;
+        res.schur_ordering = sps.ordering;
+        res.schur_nnz = sps.diagonalIndex[m];
+    }
     Work work;
     work.DLS1.initialize(bs, data.LP_nBlock);
     work.DLS2.initialize(bs, data.LP_nBlock);
@@ -1053,16 +1326,26 @@ bool solve(const InputData &data, const Parameter &param, SolveResult &res, cons
         Raxpy(m, MONE, cur.primalVec.data(), 1, nw.gVec.data(), 1);
         if (predictor) {
             // Make_bMat (1223-1244) + compute_DyVec's Cholesky (1280-1296)
-            for (auto &v : nw.bMat) v = 0.0;
-            schur_dense(data, pt.xMat, pt.invzMat, nw.bMat, nw.useFormula);
-            mint info = 0;
-            Rpotrf_L(m, nw.bMat.data(), m, info);
-            if (info != 0) return false;
+            if (sparse) {
+                schur_sparse(data, pt.xMat, pt.invzMat, sps, nw.useFormula);
+                chol_sparse(sps);
+            } else {
+                for (auto &v : nw.bMat) v = 0.0;
+                schur_dense(data, pt.xMat, pt.invzMat, nw.bMat, nw.useFormula);
+                mint info = 0;
+                Rpotrf_L(m, nw.bMat.data(), m, info);
+                if (info != 0) return false;
+            }
         }
-        // solveSystems (sdpa_linear.cpp:639-651)
-        nw.DyVec = nw.gVec;
-        Rtrsv_LN(m, nw.bMat.data(), m, nw.DyVec.data());
-        Rtrsv_LT(m, nw.bMat.data(), m, nw.DyVec.data());
+        if (sparse) {
+            // compute_DyVec (sdpa_newton.cpp:1298-1302)
+            solve_sparse(sps, nw.gVec, nw.DyVec);
+        } else {
+            // solveSystems (sdpa_linear.cpp:639-651)
+            nw.DyVec = nw.gVec;
+            Rtrsv_LN(m, nw.bMat.data(), m, nw.DyVec.data());
+            Rtrsv_LT(m, nw.bMat.data(), m, nw.DyVec.data());
+        }
         // compute_DzMat (1318-1327)
         multi_plus(data, nw.DyVec, nw.DzMat);
         scale(nw.DzMat, nw.DzMat, MMONE);

@@ -1,12 +1,15 @@
 // ORACLE — test infrastructure only.
 //
-// CPU restatement of SDPA-DD 7.1.3's dense-Schur SDP path: the .dat-s reader,
-// the parameter file, and the Mehrotra predictor-corrector loop with its HKM
+// CPU restatement of SDPA-DD 7.1.3's SDP path: the .dat-s reader, the
+// parameter file, and the Mehrotra predictor-corrector loop with its HKM
 // Newton step, Schur assembly (F1/F2/F3 for SDP blocks, the diagonal form
-// for LP entries), Cholesky solve, step length (Lanczos for SDP blocks of
-// size > 20, Rsyev's tridiagonal reduction and root-free QR for smaller
-// ones, ratios for LP entries) and phase logic.  Sparse-Schur ordering and
-// SOCP blocks are not restated; inputs needing them are rejected.
+// for LP entries) into a dense B or into the sparse pattern of its factor,
+// Cholesky solve (dense, or the sparse row Cholesky of the chordal path),
+// step length (Lanczos for SDP blocks of size > 20, Rsyev's tridiagonal
+// reduction and root-free QR for smaller ones, ratios for LP entries) and
+// phase logic.  The sparse path's ordering is an exact minimum degree in
+// place of the reference's Spooles 2.2.  SOCP blocks are not restated;
+// inputs with them are rejected.
 #pragma once
 #include <string>
 #include <vector>
@@ -103,6 +106,9 @@ struct SolveResult {
     std::vector<dd_real> xMat, zMat, yVec;  // concatenated SDP blocks (column-major), then the LP entries
     std::vector<IterRecord> trace;
     std::string message;
+    bool schur_sparse = false;          // the sparse-Schur path ran
+    std::vector<int> schur_ordering;    // its newToOld ordering
+    long long schur_nnz = 0;            // entries of its factor pattern
 };
 
 // An initial point (sdpa_main.cpp:237-251): X and Z over the whole space
@@ -119,6 +125,30 @@ struct InitPoint {
 // format; QD's reader stops after HI's exponent).
 bool read_init(const std::string &path, const InputData &data, const Parameter &param, bool sparse, InitPoint &ip,
                std::string &err);
+
+// The sparse-Schur path (the chordal branch of the reference): B in the
+// upper-triangular pattern of its factor under the ordering newToOld
+// (ordering), rows compressed (diagonalIndex), each row's diagonal first
+// and its columns ascending.
+struct SchurPair {
+    int i, j, k1, k2, loc;  // constraints, their positions in the block's constraint list, entry of the pattern
+};
+struct SparseSchur {
+    bool on = false;
+    int m = 0;
+    std::vector<int> ordering, reverse_ordering, diagonalIndex, row_index, column_index;
+    std::vector<dd_real> sp_ele;
+    std::vector<std::vector<SchurPair>> sdp, lp;  // per SDP block, per LP entry
+};
+// 0: the reference's rule; 1: always dense; 2: sparse whenever the
+// pattern allows (the thresholds of the reference's debugging switch).
+extern int g_schur_mode;
+bool sparse_schur_setup(const InputData &d, SparseSchur &S, int mode);
+std::vector<int> min_degree_order(int m, const std::vector<std::vector<int>> &adj, std::vector<std::vector<int>> &rows);
+void schur_sparse(const InputData &data, const DenseLinearSpace &X, const DenseLinearSpace &invZ, SparseSchur &S,
+                  const std::vector<int> &useFormula);
+void chol_sparse(SparseSchur &S);
+void solve_sparse(const SparseSchur &S, const std::vector<dd_real> &g, std::vector<dd_real> &dy);
 
 // Runs the solver from lambdaStar I, or from *init.
 bool solve(const InputData &data, const Parameter &param, SolveResult &res, const InitPoint *init = nullptr);

@@ -3,6 +3,7 @@
 // the result as JSON, every dd value as a [hi, lo] pair with 17 digits.
 //   sdpa_oracle <data.dat-s> [param.sdpa] [--dot-threads T] [--max-iter N] [--dump-solution]
 //               [--init file [--init-dense]]  (initial point: sparse format unless --init-dense)
+//               [--schur auto|dense|sparse]   (the Schur path: the reference's rule, or forced)
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +30,10 @@ int main(int argc, char **argv) {
         else if (!std::strcmp(argv[i], "--dump-solution")) dump = true;
         else if (!std::strcmp(argv[i], "--init") && i + 1 < argc) init_path = argv[++i];
         else if (!std::strcmp(argv[i], "--init-dense")) init_dense = true;
+        else if (!std::strcmp(argv[i], "--schur") && i + 1 < argc) {
+            const char *v = argv[++i];
+            g_schur_mode = !std::strcmp(v, "dense") ? 1 : !std::strcmp(v, "sparse") ? 2 : 0;
+        }
         else param_path = argv[i];
     }
     InputData data;
@@ -56,6 +61,10 @@ int main(int argc, char **argv) {
     std::printf("{\"m\": %d, \"iterations\": %d, \"phase\": \"%s\", \"phase_internal\": \"%s\", \"seconds\": %.6f,\n",
                 data.m, res.iterations, names[res.phase_reported], names[res.phase], secs);
     std::printf(" \"message\": \"%s\", \"dot_threads\": %d,\n", res.message.c_str(), g_dot_threads);
+    std::printf(" \"schur\": \"%s\", \"schur_nnz\": %lld, \"schur_ordering\": [", res.schur_sparse ? "sparse" : "dense",
+                res.schur_nnz);
+    for (size_t i = 0; i < res.schur_ordering.size(); i++) std::printf("%s%d", i ? ", " : "", res.schur_ordering[i]);
+    std::printf("],\n");
     std::printf(" \"mu\": "); pdd(res.mu);
     std::printf(", \"objValPrimal\": "); pdd(res.objValPrimal);
     std::printf(", \"objValDual\": "); pdd(res.objValDual);

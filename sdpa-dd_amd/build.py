@@ -19,7 +19,8 @@ CLI = os.path.join(HERE, "sdpa_dd_gpu")
 SHIM = os.path.join(HERE, "libmpblas_dd_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "ozaki.hip", "smallblk.hip", "problem.hip", "solver.hip", "adjoint.hip", "capi.hip", "steqr.cpp"]
+SOURCES = ["kernels.hip", "ozaki.hip", "smallblk.hip", "problem.hip", "sparse_schur.hip", "solver.hip", "adjoint.hip",
+           "capi.hip", "steqr.cpp"]
 # Every rounding step of the dd arithmetic is part of the result: no
 # contraction into FMAs.  Packed FP32 instructions are disabled (the dd code
 # is FP64; this keeps the compiler from emitting packed forms anywhere).

@@ -28,6 +28,11 @@ struct sdpadd_problem_s {
     sdp::Adjoint adj;
     sdp::InitPoint init;
     bool has_init = false;
+    // the Schur path: mode (0 the reference's rule, 1 dense, 2 sparse when
+    // the structure allows), its analysis and device image (for sps_mode)
+    int schur_mode = 0, sps_mode = -1;
+    sps::Analysis spa;
+    sps::Device spd;
 };
 
 // Scratch of the allocation-free entry points: a fixed device array
@@ -493,8 +498,36 @@ int sdpadd_problem_schur_ddflops(sdpadd_problem p, double *ddflops) {
     *ddflops = sdp::schur_ddflops(p->hp);
     return 0;
 }
+// Analyses the Schur path for the problem's current mode (once per mode).
+static int ensure_schur_path(sdpadd_problem p) {
+    if (p->sps_mode == p->schur_mode) return 0;
+    sps::analyse(p->hp, p->schur_mode, p->spa);
+    std::string err;
+    if (sps::upload(p->spa, p->spd, p->h->c.stream, err) != 0) {
+        p->h->c.set_error(err);
+        return -2;
+    }
+    p->sps_mode = p->schur_mode;
+    return 0;
+}
+int sdpadd_problem_set_schur_mode(sdpadd_problem p, int mode) {
+    if (!p || mode < 0 || mode > 2) return -1;
+    p->schur_mode = mode;
+    return 0;
+}
+int sdpadd_problem_schur_info(sdpadd_problem p, int *sparse, int64_t *nnz, int64_t *aggregate, int *ordering) {
+    if (!p || !p->finalized) return -1;
+    int rc = ensure_schur_path(p);
+    if (rc) return rc;
+    if (sparse) *sparse = p->spa.on ? 1 : 0;
+    if (nnz) *nnz = p->spa.on ? p->spa.diag[p->spa.m] : 0;
+    if (aggregate) *aggregate = p->spa.aggregate;
+    if (ordering && p->spa.on) std::copy(p->spa.ordering.begin(), p->spa.ordering.end(), ordering);
+    return 0;
+}
 int sdpadd_problem_destroy(sdpadd_problem p) {
     if (!p) return -1;
+    sps::release(p->spd);
     p->adj.release();
     sdp::free_problem(p->dp);
     delete p;
@@ -570,7 +603,9 @@ int sdpadd_solve(sdpadd_problem p, const sdpadd_params *prm, sdpadd_result *res)
     d.gammaStar = prm->gammaStar;
     d.epsilonDash = prm->epsilonDash;
     sdp::SolveOutput out;
-    int rc = sdp::solve(p->h->c, p->hp, p->dp, d, out, &p->X, &p->y, &p->Z, p->has_init ? &p->init : nullptr);
+    int rc = ensure_schur_path(p);
+    if (rc) return rc;
+    rc = sdp::solve(p->h->c, p->hp, p->dp, d, out, &p->X, &p->y, &p->Z, p->has_init ? &p->init : nullptr, &p->spd);
     if (rc != 0) return rc;
     p->trace = out.trace;
     res->iterations = out.iterations;

@@ -580,8 +580,7 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
                    const OzLookAhead *la) {
     // panel width: 256 (four k chunks per Ozaki update) for large matrices,
     // 64 below 1024 so that small ones still take most updates on MFMA
-    idx_t OZ_NB = n > 1024 ? 256 : 64;
-    if (const char *e = std::getenv("SDPADD_OZ_NB_EXPERIMENT")) OZ_NB = std::atoi(e);
+    const idx_t OZ_NB = n > 1024 ? 256 : 64;
     // the split costs a second slicing of the panel and two events: it pays
     // only where the trailing updates are long (measured: m = 4551 gains 5%,
     // m = 800 loses 14% of the Cholesky time)

@@ -1315,7 +1315,7 @@ __global__ void k_schur_lp(int np, int m, const int *pi, const int *pj, const id
     if (i != j) B[j + (idx_t)m * i] = bji;
 }
 
-void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B, int T) {
+void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B, int T, bool lp_larger_first) {
     const int m = dp.m;
     sdk::fill(s, (idx_t)m * m, mk(0.0), B);  // Newton::Make_bMat: bMat.setZero()
     for (size_t l = 0; l < dp.schur.size(); l++) {
@@ -1360,10 +1360,14 @@ void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B,
             k_scatter_pairs<<<(S.npairs + 255) / 256, 256, 0, s>>>(S.npairs, m, S.d_pair_i, S.d_pair_j, dp.d_pair_val, B);
         }
     }
+    // compute_bMat_dense_LP forms x z^-1 a_i a_j with i the earlier constraint
+    // of a pair; compute_bMat_sparse_LP (sdpa_newton.cpp:1199-1219, pairs
+    // kept when i >= j) with i the later one
     if (dp.n_lp_pairs > 0)
-        k_schur_lp<<<(dp.n_lp_pairs + 255) / 256, 256, 0, s>>>(dp.n_lp_pairs, m, dp.d_lpp_i, dp.d_lpp_j, dp.d_lpp_ptr,
-                                                              dp.d_lpp_l, dp.d_lpp_ai, dp.d_lpp_aj, X + dp.nn,
-                                                              invZ + dp.nn, B);
+        k_schur_lp<<<(dp.n_lp_pairs + 255) / 256, 256, 0, s>>>(
+            dp.n_lp_pairs, m, dp.d_lpp_i, dp.d_lpp_j, dp.d_lpp_ptr, dp.d_lpp_l,
+            lp_larger_first ? dp.d_lpp_aj : dp.d_lpp_ai, lp_larger_first ? dp.d_lpp_ai : dp.d_lpp_aj, X + dp.nn,
+            invZ + dp.nn, B);
 }
 
 }  // namespace sdp

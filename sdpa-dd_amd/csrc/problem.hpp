@@ -221,7 +221,9 @@ void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T);
 void plus_A(hipStream_t s, DeviceProblem &dp, const dd *y, dd *R);
 // R += C
 void plus_C(hipStream_t s, DeviceProblem &dp, dd *R);
-// B = Schur complement for (X, invZ) (B zeroed first)
-void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B, int T);
+// B = Schur complement for (X, invZ) (B zeroed first).  lp_larger_first:
+// the LP terms x z^-1 a_i a_j with i the later constraint of each pair (the
+// sparse-Schur path's order) instead of the earlier one.
+void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B, int T, bool lp_larger_first = false);
 
 }  // namespace sdp

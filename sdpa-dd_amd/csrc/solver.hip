@@ -356,8 +356,9 @@ struct DevBuf {
 
 class Solver {
   public:
-    Solver(Context &c, const HostProblem &h, DeviceProblem &d, const Params &p, const InitPoint *ip)
-        : ctx(c), hp(h), dp(d), prm(p), init(ip) {}
+    Solver(Context &c, const HostProblem &h, DeviceProblem &d, const Params &p, const InitPoint *ip,
+           const sps::Device *sp)
+        : ctx(c), hp(h), dp(d), prm(p), init(ip), spd(sp) {}
     ~Solver() {
         if (s2) (void)hipStreamSynchronize(s2);
         if (s3) (void)hipStreamSynchronize(s3);
@@ -388,6 +389,10 @@ class Solver {
     DeviceProblem &dp;
     const Params &prm;
     const InitPoint *init;
+    // the sparse-Schur path (null: dense B): the pattern's values and the
+    // solves' work vector
+    const sps::Device *spd;
+    dd *spv = nullptr, *spx = nullptr;
     dd *initX = nullptr, *initZ = nullptr;  // the initial point's X and Z (kept for SolveInfo)
     sdk::DotTask *d_tasks_init = nullptr;   // (initX, Z) and (X, initZ) per block
     std::vector<void *> mem;
@@ -613,6 +618,11 @@ class Solver {
         sdk::gemv_n_fast(s, m, m, mk(-1.0), bSave, m, x, mk(1.0), r);
     }
     void schur_solve() {
+        if (spd) {
+            // compute_DyVec's sparse branch (sdpa_newton.cpp:1298-1302)
+            sps::solve(s, *spd, spv, g, dy, spx);
+            return;
+        }
         sdk::copy(s, m, g, dy);
         trsv_chk('N', m, bMat, m, dy);
         trsv_chk('T', m, bMat, m, dy);
@@ -901,7 +911,9 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         if (hipHostMalloc((void **)&h_lp_eig, sizeof(dd) * 2 * nlp) != hipSuccess) return -2;
     }
     if (!(d_rinv2 = alloc(64)) || !(d_rinv3 = alloc(64))) return -2;
-    if (ctx.path >= 1 && (!(bSave = alloc((idx_t)m * m)) || !(rvec = alloc(m)) || !(dyTry = alloc(m)))) return -2;
+    if (!spd && ctx.path >= 1 && (!(bSave = alloc((idx_t)m * m)) || !(rvec = alloc(m)) || !(dyTry = alloc(m))))
+        return -2;
+    if (spd && (!(spv = alloc(spd->nnz)) || !(spx = alloc(m)))) return -2;
     if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&s3, hipStreamNonBlocking) != hipSuccess)
         return -2;
@@ -1090,25 +1102,33 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             // Schur build and Cholesky timed with events; one synchronisation
             // brings back the Cholesky's info (and max|B| for the refinement)
             (void)hipEventRecord(ev_t0, s);
-            sdp::schur(s, dp, X, invZ, bMat, ctx.dot_T);
+            // the sparse branch assembles in the reference's sparse LP order
+            sdp::schur(s, dp, X, invZ, bMat, ctx.dot_T, spd != nullptr);
+            if (spd) sps::gather(s, *spd, bMat, spv);
             (void)hipEventRecord(ev_t1, s);
-            if (ctx.path >= 1) {
-                sdk::copy(s, (idx_t)m * m, bMat, bSave);
-                enq_max(bSave, (idx_t)m * m, slot_bmax);
-                (void)hipMemcpyAsync(h_sc + slot_bmax, d_sc + slot_bmax, sizeof(dd), hipMemcpyDeviceToHost, s);
-            }
-            potrf(s, m, bMat, ctx.d_info, ctx.d_rinv, ctx.ozw);
-            (void)hipEventRecord(ev_t2, s);
             int64_t info = 0;
-            (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
+            if (spd) {
+                // Lal::getCholesky on the pattern: no failure exit
+                sps::factor(s, *spd, spv);
+                (void)hipEventRecord(ev_t2, s);
+            } else {
+                if (ctx.path >= 1) {
+                    sdk::copy(s, (idx_t)m * m, bMat, bSave);
+                    enq_max(bSave, (idx_t)m * m, slot_bmax);
+                    (void)hipMemcpyAsync(h_sc + slot_bmax, d_sc + slot_bmax, sizeof(dd), hipMemcpyDeviceToHost, s);
+                }
+                potrf(s, m, bMat, ctx.d_info, ctx.d_rinv, ctx.ozw);
+                (void)hipEventRecord(ev_t2, s);
+                (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
+            }
             sync();
             float ms1 = 0, ms2 = 0;
             if (hipEventElapsedTime(&ms1, ev_t0, ev_t1) == hipSuccess) t_schur += 1e-3 * ms1;
             if (hipEventElapsedTime(&ms2, ev_t1, ev_t2) == hipSuccess) t_chol += 1e-3 * ms2;
-            if (ctx.path >= 1) bmax = sc(slot_bmax);
+            if (ctx.path >= 1 && !spd) bmax = sc(slot_bmax);
             if (info != 0) return false;
             // the Ozaki form solves with the block-inverse operators of this factor
-            binv = ctx.path >= 1 && m >= 512 && ctx.binv_prepare(s, m, bMat, m) == 0;
+            binv = !spd && ctx.path >= 1 && m >= 512 && ctx.binv_prepare(s, m, bMat, m) == 0;
         }
         schur_solve();
         sdp::plus_A(s, dp, dy, DzMat);
@@ -1365,8 +1385,8 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
 }  // namespace
 
 int solve(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const Params &prm, SolveOutput &out, std::vector<dd> *xMat,
-          std::vector<dd> *yVec, std::vector<dd> *zMat, const InitPoint *init) {
-    Solver sv(ctx, hp, dp, prm, init);
+          std::vector<dd> *yVec, std::vector<dd> *zMat, const InitPoint *init, const sps::Device *sparse) {
+    Solver sv(ctx, hp, dp, prm, init, sparse && sparse->on ? sparse : nullptr);
     return sv.run(out, xMat, yVec, zMat);
 }
 

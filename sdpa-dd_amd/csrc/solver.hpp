@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "problem.hpp"
+#include "sparse_schur.hpp"
 
 namespace sdp {
 
@@ -35,10 +36,11 @@ struct SolveOutput {
 struct Context;  // device, stream and scratch of a library handle
 
 // Solves the uploaded problem from lambdaStar I, or from *init; host
-// problem needed for dimensions.
+// problem needed for dimensions.  sparse: the sparse-Schur path's device
+// image (null or off: the dense Schur path).
 int solve(Context &ctx, const HostProblem &hp, DeviceProblem &dp, const Params &prm, SolveOutput &out,
           std::vector<dd> *xMat = nullptr, std::vector<dd> *yVec = nullptr, std::vector<dd> *zMat = nullptr,
-          const InitPoint *init = nullptr);
+          const InitPoint *init = nullptr, const sps::Device *sparse = nullptr);
 
 // Truncated-Lanczos minimum eigenvalue estimate of L M L^T (one block).
 dd lanczos_min_eigen(Context &ctx, int n, const dd *d_L, const dd *d_M);

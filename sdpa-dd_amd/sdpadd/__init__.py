@@ -104,6 +104,9 @@ def load_library(path=None):
         "sdpadd_problem_formula": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int)]),
         "sdpadd_schur_build": (ctypes.c_int, [vp, vp, vp, vp]),
         "sdpadd_problem_schur_ddflops": (ctypes.c_int, [vp, c_dp]),
+        "sdpadd_problem_set_schur_mode": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sdpadd_problem_schur_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(c_i64),
+                                                     ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_int)]),
         "sdpadd_inner_A": (ctypes.c_int, [vp, vp, vp]),
         "sdpadd_plus_A": (ctypes.c_int, [vp, vp, vp]),
         "sdpadd_read_params": (ctypes.c_int, [ctypes.c_char_p, vp]),
@@ -339,6 +342,26 @@ class Problem:
         out = np.zeros(1)
         self.dev.check(self.dev.lib.sdpadd_problem_schur_ddflops(self.p, _dp(out)), "schur_ddflops")
         return float(out[0])
+
+    SCHUR_MODES = {"auto": 0, "dense": 1, "sparse": 2}
+
+    def set_schur_mode(self, mode):
+        """The Schur path of the next solves: "auto" (the reference's rule),
+        "dense", or "sparse" (whenever the structure allows)."""
+        self.dev.check(self.dev.lib.sdpadd_problem_set_schur_mode(self.p, self.SCHUR_MODES[mode]), "set_schur_mode")
+
+    def schur_info(self):
+        """{"sparse": bool, "nnz": factor-pattern entries, "aggregate":
+        aggregate-pattern entries, "ordering": newToOld (sparse only)}."""
+        sp = ctypes.c_int()
+        nnz, agg = ctypes.c_int64(), ctypes.c_int64()
+        order = (ctypes.c_int * max(1, self.m))()
+        self.dev.check(self.dev.lib.sdpadd_problem_schur_info(self.p, ctypes.byref(sp), ctypes.byref(nnz),
+                                                              ctypes.byref(agg), order), "schur_info")
+        out = {"sparse": bool(sp.value), "nnz": nnz.value, "aggregate": agg.value}
+        if sp.value:
+            out["ordering"] = [order[i] for i in range(self.m)]
+        return out
 
     def schur(self, X, Zinv):
         dX, dZ = self.dev.to_device(X), self.dev.to_device(Zinv)

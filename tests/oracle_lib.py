@@ -25,6 +25,11 @@ class Oracle:
         L = self.lib
         c_dp = ctypes.POINTER(ctypes.c_double)
         L.orc_set_dot_threads.argtypes = [ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_min_degree.argtypes = [ctypes.c_int, ip, ip, ip]
+        L.orc_min_degree.restype = ctypes.c_long
+        L.orc_sparse_factor_solve.argtypes = [ctypes.c_int, c_dp, c_dp, c_dp, c_dp, ip]
+        L.orc_sparse_factor_solve.restype = ctypes.c_long
         L.orc_dd_op.argtypes = [ctypes.c_int, ctypes.c_long, c_dp, c_dp, c_dp]
         L.orc_read_dd.argtypes = [ctypes.c_char_p, c_dp]
         L.orc_read_dd.restype = ctypes.c_int
@@ -54,6 +59,32 @@ class Oracle:
         L.orc_load.restype = ctypes.c_int
         L.orc_formula.argtypes = [ctypes.POINTER(ctypes.c_int)]
         L.orc_schur.argtypes = [c_dp, c_dp, c_dp]
+
+    def min_degree(self, adj):
+        """The sparse-Schur path's minimum-degree ordering of an adjacency
+        (list of neighbour lists): (newToOld, factor-pattern entries)."""
+        m = len(adj)
+        ptr = np.zeros(m + 1, dtype=np.int32)
+        for i, a in enumerate(adj):
+            ptr[i + 1] = ptr[i] + len(a)
+        idx = np.array([j for a in adj for j in a] or [0], dtype=np.int32)
+        order = np.zeros(max(1, m), dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        nnz = self.lib.orc_min_degree(m, ptr.ctypes.data_as(ip), idx.ctypes.data_as(ip), order.ctypes.data_as(ip))
+        return order[:m].tolist(), int(nnz)
+
+    def sparse_factor_solve(self, B, g):
+        """The sparse path on a symmetric dd matrix B (m*m x 2, column-major):
+        (x, U, order, nnz) with U the factor pattern (new indices, upper,
+        1/sqrt(pivot) on its diagonal) as an (m*m, 2) array."""
+        m = g.shape[0]
+        x = np.zeros((m, 2))
+        U = np.zeros((m * m, 2))
+        order = np.zeros(m, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        nnz = self.lib.orc_sparse_factor_solve(m, _dp(np.ascontiguousarray(B)), _dp(np.ascontiguousarray(g)), _dp(x),
+                                               _dp(U), order.ctypes.data_as(ip))
+        return x, U, order.tolist(), int(nnz)
 
     def set_dot_threads(self, t):
         self.lib.orc_set_dot_threads(int(t))
@@ -180,10 +211,13 @@ class Oracle:
 
     # whole solver -------------------------------------------------------
     def solve(self, data, param=None, dot_threads=1, max_iter=None, dump=False, timeout=3000, init=None,
-              init_dense=False):
+              init_dense=False, schur=None):
+        """schur: None (the reference's rule), "dense" or "sparse" (forced)."""
         cmd = [CLI, data]
         if param:
             cmd.append(param)
+        if schur:
+            cmd += ["--schur", schur]
         if init:
             cmd += ["--init", init] + (["--init-dense"] if init_dense else [])
         cmd += ["--dot-threads", str(dot_threads)]
