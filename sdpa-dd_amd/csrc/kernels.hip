@@ -1688,14 +1688,51 @@ __global__ __launch_bounds__(NT) void k_trmm_lltn(idx_t m, idx_t n, const dd *__
         }
 }
 
+// The same products on 16 x 16 tiles, one output element per thread (for
+// grids of 64 x 64 tiles too small to fill the chip: a 300 x 300 product
+// has 25 such tiles and 361 of these).  Each element keeps its chain:
+// b_ij a_ii, then + a_ki b_kj for k > i ascending.
+static constexpr int T16 = 16;
+__global__ __launch_bounds__(256) void k_trmm_lltn16(idx_t m, idx_t n, const dd *__restrict__ A, idx_t lda,
+                                                     const dd *__restrict__ B, idx_t ldb, dd *__restrict__ C, idx_t ldc,
+                                                     int b_lower) {
+    __shared__ dd As[T16][T16 + 1];
+    __shared__ dd Bs[T16][T16 + 1];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const idx_t row0 = (idx_t)blockIdx.x * T16, col0 = (idx_t)blockIdx.y * T16;
+    const idx_t i = row0 + tx, j = col0 + ty;
+    dd acc = (i < m && j < n) ? dd_mul(B[i + j * ldb], A[i + i * lda]) : mk(0.0);
+    idx_t kbeg = row0 + 1;
+    if (b_lower && col0 > kbeg) kbeg = col0;
+    for (idx_t k0 = kbeg; k0 < m; k0 += T16) {
+        // thread (tx, ty) stages A[k0 + tx, row0 + ty] and B[k0 + tx, col0 + ty]
+        const idx_t k = k0 + tx, ra = row0 + ty, cb = col0 + ty;
+        As[tx][ty] = (k < m && ra < m) ? A[k + ra * lda] : mk(0.0);
+        Bs[tx][ty] = (k < m && cb < n) ? B[k + cb * ldb] : mk(0.0);
+        __syncthreads();
+        const int kc = (int)((m - k0) < T16 ? (m - k0) : T16);
+        for (int kk = 0; kk < kc; kk++)
+            if (k0 + kk > i) acc = dd_add_eq(acc, dd_mul(As[kk][tx], Bs[kk][ty]));
+        __syncthreads();
+    }
+    if (i < m && j < n) C[i + j * ldc] = acc;
+}
+
+static void trmm_launch(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *B, idx_t ldb, dd *C,
+                        idx_t ldc, int b_lower) {
+    if ((idx_t)cdiv(m, TB) * cdiv(n, TB) < 512)
+        k_trmm_lltn16<<<dim3(cdiv(m, T16), cdiv(n, T16)), 256, 0, s>>>(m, n, A, lda, B, ldb, C, ldc, b_lower);
+    else
+        k_trmm_lltn<<<dim3(cdiv(m, TB), cdiv(n, TB)), NT, 0, s>>>(m, n, A, lda, B, ldb, C, ldc, b_lower);
+}
 void trmm_lltn_self(hipStream_t s, idx_t n, const dd *A, idx_t lda, dd *C, idx_t ldc) {
     if (n <= 0) return;
     StatScope st("trmm", s, (double)n * n * n / 3.0);
-    k_trmm_lltn<<<dim3(cdiv(n, TB), cdiv(n, TB)), NT, 0, s>>>(n, n, A, lda, A, lda, C, ldc, 1);
+    trmm_launch(s, n, n, A, lda, A, lda, C, ldc, 1);
 }
 void trmm_lltn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *B, idx_t ldb, dd *C, idx_t ldc) {
     if (m <= 0 || n <= 0) return;
-    k_trmm_lltn<<<dim3(cdiv(m, TB), cdiv(n, TB)), NT, 0, s>>>(m, n, A, lda, B, ldb, C, ldc, 0);
+    trmm_launch(s, m, n, A, lda, B, ldb, C, ldc, 0);
 }
 
 // SYRK lower/no-transpose: C := alpha*A*A^T + beta*C on the lower triangle.
@@ -2444,11 +2481,29 @@ void lanczos_tail_fast(hipStream_t s, idx_t n, const dd *q, const dd *qold, cons
 }
 
 // Max of |x_i|: per-thread running max, tree merge, last workgroup merges.
-__global__ __launch_bounds__(256) void k_max_abs(idx_t len, const dd *x, dd *out, dd *partial, unsigned *counter) {
+// max |x_i| (exact, so independent of the grid) and, with y non-null,
+// y = x in the same pass; four independent loads per thread in flight
+__global__ __launch_bounds__(256) void k_max_abs(idx_t len, const dd *x, dd *y, dd *out, dd *partial,
+                                                 unsigned *counter) {
     __shared__ dd sm[256];
     dd best = mk(0.0);
-    for (idx_t i = (idx_t)blockIdx.x * 256 + threadIdx.x; i < len; i += (idx_t)gridDim.x * 256) {
-        dd t = dd_abs(x[i]);
+    const idx_t stride = (idx_t)gridDim.x * 256;
+    idx_t i = (idx_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < len; i += 4 * stride) {
+        dd v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = x[i + q * stride];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (y) y[i + q * stride] = v[q];
+            const dd t = dd_abs(v[q]);
+            if (dd_gt(t, best)) best = t;
+        }
+    }
+    for (; i < len; i += stride) {
+        const dd v = x[i];
+        if (y) y[i] = v;
+        const dd t = dd_abs(v);
         if (dd_gt(t, best)) best = t;
     }
     sm[threadIdx.x] = best;
@@ -2463,19 +2518,26 @@ __global__ __launch_bounds__(256) void k_max_abs(idx_t len, const dd *x, dd *out
     if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
     __syncthreads();
     if (ticket != gridDim.x - 1) return;
+    // the last workgroup reduces the partials (exact, any order)
+    __threadfence();
+    dd r = mk(0.0);
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += 256)
+        if (dd_gt(partial[b], r)) r = partial[b];
+    sm[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w && dd_gt(sm[threadIdx.x + w], sm[threadIdx.x])) sm[threadIdx.x] = sm[threadIdx.x + w];
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        __threadfence();
-        dd r = mk(0.0);
-        for (unsigned b = 0; b < gridDim.x; b++)
-            if (dd_gt(partial[b], r)) r = partial[b];
-        *out = r;
+        *out = sm[0];
         *counter = 0u;
     }
 }
-void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned *counter) {
-    unsigned g = std::min<unsigned>(cdiv(len, 256), 256u);
+void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned *counter, dd *copy_to) {
+    unsigned g = std::min<unsigned>(cdiv(len, 1024), (unsigned)MAX_ABS_BLOCKS);
     if (g == 0) g = 1;
-    k_max_abs<<<g, 256, 0, s>>>(len, x, out, work, counter);
+    k_max_abs<<<g, 256, 0, s>>>(len, x, copy_to, out, work, counter);
 }
 
 }  // namespace sdk

@@ -162,7 +162,10 @@ void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx
                dd *const *C, idx_t ldc, int batch);
 // Sequential strided dot (Rdot with non-unit stride): one thread per task.
 // Max |x_i| over len values into *out.
-void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned *counter);
+// *out = max |x_i| (device); work: MAX_ABS_BLOCKS dd; copy_to (may be null):
+// also copies x there in the same pass.
+constexpr int MAX_ABS_BLOCKS = 1024;
+void max_abs(hipStream_t s, idx_t len, const dd *x, dd *out, dd *work, unsigned *counter, dd *copy_to = nullptr);
 
 // Tile choice of the dd GEMM and the Ozaki product: 0 = by grid size
 // (default), 1 = always the 32 x 32 tiles, 2 = always the 64 x 64 tiles.

@@ -299,9 +299,12 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
         k_rowexp<<<dim3(cdiv(m, SL_R), nkb), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta);
         k_slice<S, false><<<dim3(cdiv(mpad, SL_R), nkb), SL_T, 0, s>>>(m, k, A, ars, acs, w.ea, ta, w.sa, mpad, nkb);
     }
-    // a product of a matrix with its own transpose (L L^T, L^T L) has the
-    // same rows in both operands: one set of planes serves both
-    const bool same = (A == B && ars == brs && acs == bcs && m == n);
+    // a product of a matrix with (a leading block of rows of) its own
+    // transpose -- L L^T, and the Cholesky panel's A21 A21(0:n, :)^T -- has
+    // B's rows among A's: one set of planes serves both, B's rows read with
+    // A's padding (every B row the tiles touch, < roundup(n, 64) <= mpad,
+    // lies in A's planes; columns beyond n are never stored)
+    const bool same = (A == B && ars == brs && acs == bcs && n <= m);
     const unsigned tb = same ? ta : w.next_tag(s);
     if (!same) {
         if (nkb == 1) {
@@ -320,12 +323,13 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
     const bool small = sdk::g_tile_policy == 1 || (sdk::g_tile_policy == 0 && t64 < 512);
     if (small)
         k_gemm32<S><<<dim3(cdiv(mpad, 32), cdiv(npad, 32)), 256, 0, s>>>(
-            m, n, nkb, w.sa, mpad, w.ea, ta, same ? w.sa : w.sb, npad, same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
-            lower ? 1 : 0);
+            m, n, nkb, w.sa, mpad, w.ea, ta, same ? w.sa : w.sb, same ? mpad : npad, same ? w.ea : w.eb, tb, C, ldc,
+            alpha, beta, lower ? 1 : 0);
     else
         k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, ta,
-                                                                      same ? w.sa : w.sb, npad, same ? w.ea : w.eb, tb,
-                                                                      C, ldc, alpha, beta, lower ? 1 : 0);
+                                                                      same ? w.sa : w.sb, same ? mpad : npad,
+                                                                      same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
+                                                                      lower ? 1 : 0);
 }
 
 

@@ -26,7 +26,7 @@ int Context::init(int dev) {
     if (hipMalloc((void **)&d_scal, sizeof(dd) * 64) != hipSuccess) return -1;
     if (hipMalloc((void **)&d_info, sizeof(int64_t)) != hipSuccess) return -1;
     if (hipMalloc((void **)&d_rinv, sizeof(dd) * 64) != hipSuccess) return -1;
-    if (hipMalloc((void **)&d_max_work, sizeof(dd) * 256) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_max_work, sizeof(dd) * sdk::MAX_ABS_BLOCKS) != hipSuccess) return -1;
     if (hipMemset(d_counter, 0, sizeof(unsigned) * 4) != hipSuccess) return -1;
     return 0;
 }
@@ -1113,8 +1113,8 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
                 (void)hipEventRecord(ev_t2, s);
             } else {
                 if (ctx.path >= 1) {
-                    sdk::copy(s, (idx_t)m * m, bMat, bSave);
-                    enq_max(bSave, (idx_t)m * m, slot_bmax);
+                    // bSave = B and max|B| in one pass over B
+                    sdk::max_abs(s, (idx_t)m * m, bMat, d_sc + slot_bmax, ctx.d_max_work, ctx.d_counter + 1, bSave);
                     (void)hipMemcpyAsync(h_sc + slot_bmax, d_sc + slot_bmax, sizeof(dd), hipMemcpyDeviceToHost, s);
                 }
                 potrf(s, m, bMat, ctx.d_info, ctx.d_rinv, ctx.ozw);
