@@ -959,6 +959,32 @@ int upload_problem(const HostProblem &P, DeviceProblem &dp, hipStream_t s, std::
     dp.d_work1 = dev_alloc<dd>(dp, work_need);
     dp.d_work2 = dev_alloc<dd>(dp, work_need);
     dp.d_pair_val = dev_alloc<dd>(dp, std::max(1, max_pairs));
+    // One SDP block: every element of B takes at most one pair value, so B
+    // is gathered element by element (coalesced) instead of zeroed and
+    // scattered pair by pair (half the scatter's writes stride by m).
+    if (dp.schur.size() == 1 && dp.schur[0].npairs > 0 && m <= 16384) {
+        std::vector<int> pidx((size_t)m * m, -1);
+        std::vector<int> hpi, hpj;
+        {
+            const auto &cs = P.cons[0];
+            std::vector<int> eff(cs.size());
+            for (size_t k = 0; k < cs.size(); k++) eff[k] = P.A[cs[k]].blk[P.cidx[0][k]].nnz_effect;
+            int p = 0;
+            for (size_t k1 = 0; k1 < cs.size(); k1++)
+                for (size_t k2 = 0; k2 < cs.size(); k2++) {
+                    const int i = cs[k1], j = cs[k2];
+                    if (eff[k1] < eff[k2] || (eff[k1] == eff[k2] && i < j)) continue;
+                    pidx[i + (size_t)m * j] = p;
+                    pidx[j + (size_t)m * i] = p;
+                    p++;
+                }
+            if (p != dp.schur[0].npairs) {
+                err = "Schur pair map: pair count mismatch";
+                return -1;
+            }
+        }
+        dp.d_pidx = dev_copy(dp, pidx, s);
+    }
     // Dot tasks for pairs with a dense A_j, now that work2 has an address.
     for (auto &pt : pending) {
         SchurBlock &S = dp.schur[pt.l];
@@ -1277,6 +1303,14 @@ __global__ void k_scatter_pairs(int npairs, int m, const int *pi, const int *pj,
         B[i + (idx_t)m * i] = dd_add_eq(B[i + (idx_t)m * i], v);
     }
 }
+// B[e] = 0 + value of the pair that visits element e (the zeroed matrix and
+// one "+=", as k_scatter_pairs gives), or 0; one SDP block only.
+__global__ void k_gather_pairs(idx_t len, const int *pidx, const dd *pv, dd *B) {
+    for (idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (idx_t)gridDim.x * blockDim.x) {
+        const int p = pidx[e];
+        B[e] = p >= 0 ? dd_add_eq(mk(0.0), pv[p]) : mk(0.0);
+    }
+}
 // Pointer-list GEMM support: fill pointer arrays for batched products.
 __global__ void k_fill_ptrs(int cnt, const int *slots, const int *wslot, const int *type, const idx_t *off,
                             const dd *dense, dd *work1, dd *work2, idx_t nsq, int mode, const dd **A, const dd **B, dd **C,
@@ -1317,7 +1351,8 @@ __global__ void k_schur_lp(int np, int m, const int *pi, const int *pj, const id
 
 void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B, int T, bool lp_larger_first) {
     const int m = dp.m;
-    sdk::fill(s, (idx_t)m * m, mk(0.0), B);  // Newton::Make_bMat: bMat.setZero()
+    // Newton::Make_bMat: bMat.setZero() (the gather below writes every element)
+    if (!dp.d_pidx) sdk::fill(s, (idx_t)m * m, mk(0.0), B);
     for (size_t l = 0; l < dp.schur.size(); l++) {
         SchurBlock &S = dp.schur[l];
         const int n = S.n;
@@ -1357,7 +1392,11 @@ void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B,
                                                 S.d_off, S.d_wslot, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val, Xl,
                                                 dp.d_work1, dp.d_pair_val);
             if (S.ndense_pairs > 0) sdk::dot_batch(s, S.d_pair_tasks, S.ndense_pairs, T);
-            k_scatter_pairs<<<(S.npairs + 255) / 256, 256, 0, s>>>(S.npairs, m, S.d_pair_i, S.d_pair_j, dp.d_pair_val, B);
+            if (dp.d_pidx)
+                k_gather_pairs<<<2048, 256, 0, s>>>((idx_t)m * m, dp.d_pidx, dp.d_pair_val, B);
+            else
+                k_scatter_pairs<<<(S.npairs + 255) / 256, 256, 0, s>>>(S.npairs, m, S.d_pair_i, S.d_pair_j,
+                                                                       dp.d_pair_val, B);
         }
     }
     // compute_bMat_dense_LP forms x z^-1 a_i a_j with i the earlier constraint

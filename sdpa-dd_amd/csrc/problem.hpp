@@ -205,6 +205,8 @@ struct DeviceProblem {
     size_t work_elems = 0;
     dd *d_work1 = nullptr, *d_work2 = nullptr;
     dd *d_pair_val = nullptr;
+    // one SDP block: pair id of each element of B (-1: none), column-major
+    int *d_pidx = nullptr;
     dd *d_task_val = nullptr;
     dd *d_ctask_val = nullptr;
     std::vector<void *> allocs;
