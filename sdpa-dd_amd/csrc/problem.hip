@@ -1246,6 +1246,7 @@ __global__ void k_pair_thread(int nlist, const int *list, int n, const int *pfor
     }
     pv[p] = ret;
 }
+static constexpr int F2_FEW = 4;  // entries of A_j up to which the products are shared out
 // F2 pairs with a sparse A_j (sdpa_newton.cpp:897-935), one workgroup per
 // pair.  For each stored entry (a, b) of A_j the reference forms
 // v = sum_t X(a,t) F(t,b) in t order (and the (b, a) sum for an
@@ -1263,6 +1264,55 @@ __global__ __launch_bounds__(256) void k_pair_f2(int nlist, const int *list, int
     const int nj = nnz[sj];
     const dd *F = work1 + (idx_t)wslot[si] * n * n;
     dd ret = mk(0.0);
+    if (nj <= F2_FEW) {
+        // few entries (the usual case: an edge or a single coupling): every
+        // thread forms products of the entries' sums, 256 t at a time, into
+        // LDS, and thread d adds those of sum d in t order (the same
+        // products and order as a thread forming its sum alone, without
+        // one thread's serial loads along a row of X)
+        __shared__ dd ps[2 * F2_FEW][256];
+        __shared__ int sa[2 * F2_FEW], sb[2 * F2_FEW], snd;
+        if (tid == 0) {
+            int nd = 0;
+            for (int q = 0; q < nj; q++) {
+                const int a = rj[q], b = cj[q];
+                sa[nd] = a;
+                sb[nd++] = b;
+                if (a != b) {
+                    sa[nd] = b;
+                    sb[nd++] = a;
+                }
+            }
+            snd = nd;
+        }
+        __syncthreads();
+        const int nd = snd;
+        dd v = mk(0.0);  // sum tid (tid < nd)
+        for (int t0 = 0; t0 < n; t0 += 256) {
+            const int tc = n - t0 < 256 ? n - t0 : 256;
+            for (int d = 0; d < nd; d++)
+                if (tid < tc) {
+                    const int t = t0 + tid;
+                    ps[d][tid] = dd_mul(X[sa[d] + (idx_t)n * t], F[t + (idx_t)n * sb[d]]);
+                }
+            __syncthreads();
+            if (tid < nd)
+                for (int u = 0; u < tc; u++) v = dd_add_eq(v, ps[tid][u]);
+            __syncthreads();
+        }
+        if (tid < nd) va[tid] = v;
+        __syncthreads();
+        if (tid == 0) {
+            int d = 0;
+            for (int q = 0; q < nj; q++) {
+                const dd v1 = vj[q];
+                ret = dd_add_eq(ret, dd_mul(v1, va[d++]));
+                if (rj[q] != cj[q]) ret = dd_add_eq(ret, dd_mul(v1, va[d++]));
+            }
+            pv[p] = ret;
+        }
+        return;
+    }
     for (int q0 = 0; q0 < nj; q0 += 256) {
         const int q = q0 + tid;
         if (q < nj) {
