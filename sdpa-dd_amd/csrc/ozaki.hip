@@ -161,6 +161,34 @@ __global__ __launch_bounds__(SL_T) void k_slice(idx_t rows, idx_t k, const dd *M
 // Per 64-wide k chunk the planes of both operands are staged in LDS (16-byte
 // chunks XOR-swizzled by row so that a fragment read is conflict-free);
 // level d = s + t of the product accumulates in acc[d] (int32, exact).
+// An output element from its level sums: level d (int32, exact) carries
+// weight 2^-7(d+2).  Levels are combined four at a time in int64, exactly:
+// a group a0 2^21 + a1 2^14 + a2 2^7 + a3 with |a_d| < 2^31 stays below
+// 2^53, so it is an exact double; the groups (unit 2^-7(4q+5)) are then
+// summed in dd from the smallest, three dd additions where a Horner sum
+// over the levels takes fifteen.  The result is in units of 2^-35.
+static constexpr int RECOMB_UNIT = 35;
+template <int S, class L>
+__device__ __forceinline__ dd recombine(L lev) {
+    constexpr int NG = (S + 3) / 4;
+    double g[NG];
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+        long long G = 0;
+#pragma unroll
+        for (int d = 4 * q; d < 4 * q + 4; d++) G = G * 128 + (d < S ? (long long)lev(d) : 0ll);
+        g[q] = (double)G;
+    }
+    dd h = mk(g[NG - 1]);
+#pragma unroll
+    for (int q = NG - 2; q >= 0; q--) {
+        h.hi *= 0x1p-28;
+        h.lo *= 0x1p-28;
+        h = dd_add_d(h, g[q]);
+    }
+    return h;
+}
+
 template <int S>
 __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const int8_t *SA, idx_t mpad, const unsigned *EA,
                                               unsigned ta, const int8_t *SB, idx_t npad, const unsigned *EB, unsigned tb,
@@ -201,7 +229,7 @@ __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const i
             }
         }
     }
-    // recombine: sum_d acc[d] 2^-7(d+2), Horner from the smallest level
+    // recombine: sum_d acc[d] 2^-7(d+2) (recombine<S>)
 #pragma unroll
     for (int ct = 0; ct < 2; ct++) {
         const idx_t j = c0 + wc * 32 + ct * 16 + fr;
@@ -209,14 +237,8 @@ __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const i
         for (int r = 0; r < 4; r++) {
             const idx_t i = r0 + wr * 16 + fq * 4 + r;
             if (i >= m || j >= n || (lower && i < j)) continue;
-            dd h = mk((double)acc[S - 1][ct][r]);
-#pragma unroll
-            for (int d = S - 2; d >= 0; d--) {
-                h.hi *= 0.0078125;
-                h.lo *= 0.0078125;
-                h = dd_add_d(h, (double)acc[d][ct][r]);
-            }
-            const int ex = exp_of(EA[i], ta) + exp_of(EB[j], tb) - 14;
+            dd h = recombine<S>([&](int d) { return acc[d][ct][r]; });
+            const int ex = exp_of(EA[i], ta) + exp_of(EB[j], tb) - RECOMB_UNIT;
             h.hi = ldexp(h.hi, ex);
             h.lo = ldexp(h.lo, ex);
             if (alpha < 0) h = dd_neg(h);
@@ -271,14 +293,8 @@ __global__ __launch_bounds__(256) void k_gemm32(idx_t m, idx_t n, int nkb, const
     for (int r = 0; r < 4; r++) {
         const idx_t i = r0 + wr * 16 + fq * 4 + r;
         if (i >= m || j >= n || (lower && i < j)) continue;
-        dd h = mk((double)acc[S - 1][r]);
-#pragma unroll
-        for (int d = S - 2; d >= 0; d--) {
-            h.hi *= 0.0078125;
-            h.lo *= 0.0078125;
-            h = dd_add_d(h, (double)acc[d][r]);
-        }
-        const int ex = exp_of(EA[i], ta) + exp_of(EB[j], tb) - 14;
+        dd h = recombine<S>([&](int d) { return acc[d][r]; });
+        const int ex = exp_of(EA[i], ta) + exp_of(EB[j], tb) - RECOMB_UNIT;
         h.hi = ldexp(h.hi, ex);
         h.lo = ldexp(h.lo, ex);
         if (alpha < 0) h = dd_neg(h);
