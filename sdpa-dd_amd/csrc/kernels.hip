@@ -864,6 +864,7 @@ __global__ __launch_bounds__(TS_NT) void k_trsv_lt(idx_t n, const dd *L, idx_t l
 // restores the saved right-hand side and solves again, in the same
 // operation order, so the result never depends on co-residency.
 static constexpr int TC_ROWS = 256;           // rows per updater workgroup
+static constexpr int BO_ROWS = 128;           // the same for the block-inverse solves (four lanes per row)
 static constexpr unsigned TC_SPIN = 1u << 22;  // default spin bound per wait (about a second)
 unsigned g_trsv_spin = 0;
 static unsigned trsv_spin() { return g_trsv_spin ? g_trsv_spin : TC_SPIN; }
@@ -1114,9 +1115,10 @@ __global__ __launch_bounds__(256) void k_trsv_lt_coop(idx_t n, const dd *L, idx_
 }
 
 // Workspace of the cross-workgroup solves (both forms): the solved block
-// values, the saved right-hand side, the nonzero flags, then the wait words.
+// values, the saved right-hand side, the block-inverse solve's partials,
+// the nonzero flags, then the wait words.
 struct CoopWork {
-    dd *vg, *x0;
+    dd *vg, *x0, *pg;
     int *nzg, *flags;
 };
 static CoopWork coop_work(void *work, idx_t n) {
@@ -1125,13 +1127,15 @@ static CoopWork coop_work(void *work, idx_t n) {
     CoopWork cw;
     cw.vg = (dd *)(w + 256 - (((uintptr_t)w) & 255));
     cw.x0 = cw.vg + nblk * SB;
-    cw.nzg = (int *)(cw.x0 + nblk * SB);
+    cw.pg = cw.x0 + nblk * SB;
+    cw.nzg = (int *)(cw.pg + nblk * SB);
     cw.flags = cw.nzg + nblk * SB;
     return cw;
 }
 size_t trsv_coop_bytes(idx_t n) {
-    const idx_t nblk = (n + SB - 1) / SB, nupd = (n + TC_ROWS - 1) / TC_ROWS;
-    return 256 + sizeof(dd) * 2 * nblk * SB + sizeof(int) * nblk * SB + sizeof(int) * (2 + nupd);
+    static_assert(BO_ROWS <= TC_ROWS, "the wait words are counted for the smaller updater");
+    const idx_t nblk = (n + SB - 1) / SB, nupd = (n + BO_ROWS - 1) / BO_ROWS;
+    return 256 + sizeof(dd) * 3 * nblk * SB + sizeof(int) * nblk * SB + sizeof(int) * (2 + nupd);
 }
 __global__ void k_zero_ints(int *p, int cnt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1140,14 +1144,14 @@ __global__ void k_zero_ints(int *p, int cnt) {
 void zero_ints(hipStream_t s, int *p, int cnt) {
     if (cnt > 0) k_zero_ints<<<cdiv(cnt, 256), 256, 0, s>>>(p, cnt);
 }
-// A cross-workgroup solve over `grid` workgroups of 256 threads: a plain
+// A cross-workgroup solve over `grid` workgroups of `nt` threads: a plain
 // launch when the grid fits the CU count, a cooperative one otherwise.
-static hipError_t launch_coop(const void *fn, dim3 grid, void **args, hipStream_t s) {
+static hipError_t launch_coop(const void *fn, dim3 grid, void **args, hipStream_t s, int nt = 256) {
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
-    if ((int)grid.x <= cus) return hipLaunchKernel(fn, grid, dim3(256), args, 0, s);
-    return hipLaunchCooperativeKernel(fn, grid, dim3(256), args, 0, s);
+    if ((int)grid.x <= cus) return hipLaunchKernel(fn, grid, dim3(nt), args, 0, s);
+    return hipLaunchCooperativeKernel(fn, grid, dim3(nt), args, 0, s);
 }
 // The recovery enqueued behind every cross-workgroup solve: the
 // single-workgroup solve in Rtrsv's order, a no-op unless the error word
@@ -1193,17 +1197,27 @@ int trsv_lower_coop(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, 
 // The Ozaki form's Schur solves (not Rtrsv's operation order; the Ozaki
 // form refines the result against B).  Blocks of BB rows are solved in a
 // processing sequence (L x = b: from the top; L^T x = b: from the bottom),
-// block s as x_s = Dinv_s (p_s - sum_{d=2..BI_D} T_{s,s-d} x_{s-d})
-// - Dinv_s T_{s,s-1} x_{s-1}, where Dinv_s inverts the diagonal block,
-// T_{s,t} is the coupling block (L_{s,t}, or L_{t,s}^T) and p_s is b_s
-// after the updater workgroups applied blocks <= s - BI_D - 1.  The
-// operators Dinv_s and N_{s,d} = Dinv_s T_{s,s-d} are formed once per
-// factor.  In workgroup 0, wave 0 runs the one dependent step per block
-// (x_s from x_{s-1}: a BB-term pairwise sum per lane) while waves 1-3 prepare block
-// s+1 (the Dinv product of its partial and the d = 2, 3 terms).
+// block s as x_s = Dinv_s p_s - sum_{d=2..BI_D} N_{s,d} x_{s-d}
+// - N_{s,1} x_{s-1}, where Dinv_s inverts the diagonal block, N_{s,d} =
+// Dinv_s T_{s,s-d}, T_{s,t} is the coupling block (L_{s,t}, or L_{t,s}^T)
+// and p_s is b_s after the updater workgroups applied blocks
+// <= s - BI_D - 1.  The operators are formed once per factor.
+// Workgroup 0 runs one round per block.  In round s, wave 0 forms x_s (the
+// one dependent step: a BB-term product with x_{s-1}); waves 1..BI_D
+// prepare block s+1's terms (wave 1: Dinv p, wave d: N_{s+1,d} x_{s+1-d});
+// wave BI_D + 1 sends x_{s-1} to x and to the updaters (its write-through
+// stores would hold up a product wave's operator loads, which wait on the
+// same in-order counter).  Wave 1 issues the read of block s+2's partial
+// after its product and checks it a round later.  A product wave takes each row in two
+// lanes (even and odd terms, then one exchange), which forms the same
+// pairwise tree as one lane summing all BB terms.
 static constexpr int BB = 32;  // rows per block of the block-inverse solves
-static constexpr int BI_D = 3;
+static constexpr int BI_D = 6;
 static constexpr int BI_OPS = (BI_D + 1) * BB * BB;  // dd per block and direction
+static constexpr int BI_NT = 64 * (BI_D + 2);        // threads per workgroup
+static constexpr int BI_RING = 8;                    // solved blocks kept in LDS
+static_assert(BI_RING >= BI_D + 1 && (BI_RING & (BI_RING - 1)) == 0, "ring of solved blocks");
+static_assert(BO_ROWS * 4 == BI_NT, "an updater row takes four lanes");
 
 // Pairwise sum of N dd terms (depth log2(N) instead of an N-step chain).
 template <int N>
@@ -1257,6 +1271,10 @@ __global__ __launch_bounds__(64) void k_blk_inv(idx_t n, int trans, const dd *L,
 
 // ops[s][0] = Dinv_s (the inverse, transposed for L^T x = b);
 // ops[s][d] = Dinv_s T_{s,s-d} for d = 1..BI_D (zero when s - d < 0).
+// Each BB x BB operator is stored in the order the solve's lanes read it:
+// element (i, q) at op_pos(i, q), so that the loads of term pair k (lane
+// h BB + i takes (i, 2k + h)) cover 1 KB of consecutive memory.
+__host__ __device__ __forceinline__ int op_pos(int i, int q) { return (q >> 1) * 2 * BB + (q & 1) * BB + i; }
 __global__ __launch_bounds__(256) void k_blk_ops(idx_t n, int trans, const dd *L, idx_t lda, const dd *inv, dd *ops) {
     __shared__ dd Iv[BB][BB + 1];
     __shared__ dd T[BB][BB + 1];
@@ -1271,7 +1289,7 @@ __global__ __launch_bounds__(256) void k_blk_ops(idx_t n, int trans, const dd *L
     }
     if (d == 0) {
         __syncthreads();
-        for (int e = tid; e < BB * BB; e += 256) out[e] = Iv[e / BB][e % BB];
+        for (int e = tid; e < BB * BB; e += 256) out[op_pos(e / BB, e % BB)] = Iv[e / BB][e % BB];
         return;
     }
     if (s - d < 0) {
@@ -1292,158 +1310,265 @@ __global__ __launch_bounds__(256) void k_blk_ops(idx_t n, int trans, const dd *L
         const int i = e / BB, q = e % BB;
         dd acc = mk(0.0);
         for (int k = 0; k < nb; k++) acc = dd_add(acc, dd_mul(Iv[i][k], T[k][q]));
-        out[e] = acc;
+        out[op_pos(i, q)] = acc;
     }
 }
 
-// flags, vg and the owner workgroups as in k_trsv_ln_coop / k_trsv_lt_coop;
-// an owner applies sequence block J to its rows in blocks >= J + BI_D + 1.
-__global__ __launch_bounds__(256) void k_trsv_binv(idx_t n, int trans, const dd *L, idx_t lda, dd *x, const dd *ops,
-                                                   int *flags, int nupd, dd *vg, const dd *LT, unsigned spin) {
+// Row sums of the product waves: lane l < 32 holds the even terms of row
+// l, lane l + 32 its odd terms; tree_sum<BB> over all BB terms ends with
+// (even sum) + (odd sum), so each half sums its own and the halves swap.
+__device__ __forceinline__ dd shfl_xor_dd(dd v, int m) {
+    dd r;
+    r.hi = __shfl_xor(v.hi, m, 64);
+    r.lo = __shfl_xor(v.lo, m, 64);
+    return r;
+}
+__device__ __forceinline__ dd pair_tree(dd *t, int h) {
+    const dd mine = tree_sum<BB / 2>(t);
+    const dd other = shfl_xor_dd(mine, 32);
+    return h == 0 ? dd_add(mine, other) : dd_add(other, mine);
+}
+// Values passed between the workgroups of the block-inverse solve: each
+// double is an agent-scope atomic, written once per solve (relaxed) into a
+// slot that holds BI_EMPTY until then, and read (relaxed) until both words
+// differ from it.  The value is its own message: no flag, no fence.
+// BI_EMPTY is a signalling-NaN pattern, which no arithmetic result takes.
+static constexpr unsigned long long BI_EMPTY = 0x7FF4A5A5A5A5A5A5ull;
+__device__ __forceinline__ int relaxed_flag(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void put_dd(dd *p, dd v) {
+    __hip_atomic_store((unsigned long long *)&p->hi, (unsigned long long)__double_as_longlong(v.hi), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store((unsigned long long *)&p->lo, (unsigned long long)__double_as_longlong(v.lo), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+// The read is issued now and checked later (raw_ok), so its latency
+// overlaps other work: the check of a value needs the load to be done.
+struct RawDD {
+    unsigned long long hi, lo;
+};
+__device__ __forceinline__ RawDD get_raw(const dd *p) {
+    RawDD r;
+    r.hi = __hip_atomic_load((unsigned long long *)&p->hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r.lo = __hip_atomic_load((unsigned long long *)&p->lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return r;
+}
+__device__ __forceinline__ bool raw_ok(RawDD r) { return r.hi != BI_EMPTY && r.lo != BI_EMPTY; }
+__device__ __forceinline__ dd raw_dd(RawDD r) {
+    return dd{__longlong_as_double((long long)r.hi), __longlong_as_double((long long)r.lo)};
+}
+__device__ __forceinline__ RawDD dd_raw(dd v) {
+    return RawDD{(unsigned long long)__double_as_longlong(v.hi), (unsigned long long)__double_as_longlong(v.lo)};
+}
+__global__ void k_binv_reset(idx_t len, dd *vg, dd *pg, int *flags, int nflags) {
+    const idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const dd empty = {__longlong_as_double((long long)BI_EMPTY), __longlong_as_double((long long)BI_EMPTY)};
+    if (e < len) {
+        vg[e] = empty;
+        pg[e] = empty;
+    }
+    if (e < nflags) flags[e] = 0;
+}
+
+// vg: the solved blocks (x_J at J BB), written by workgroup 0;
+// pg: each block's partial p_t (at t BB), written by the owner of its rows
+// once blocks <= t - BI_D - 1 are applied.  Owner workgroup u (blockIdx
+// u + 1) holds rows [BO_ROWS u, BO_ROWS u + BO_ROWS), four lanes per row,
+// and applies block J to its rows in blocks >= J + BI_D + 1.  flags[1 +
+// nupd] is the error word: a wait that runs out of its bound sets it and
+// every wave leaves (the recovery behind the launch then solves again).
+__global__ __launch_bounds__(BI_NT) void k_trsv_binv(idx_t n, int trans, const dd *L, idx_t lda, dd *x, const dd *ops,
+                                                     int *flags, int nupd, dd *vg, dd *pg, const dd *LT, unsigned spin) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nblk = (int)((n + BB - 1) / BB);
     int *err = flags + 1 + nupd;
     if (blockIdx.x == 0) {
-        __shared__ dd hp[2][3][BB];          // prepared terms of the next block
-        __shared__ dd xs[BI_D + 1][BB];      // solved blocks, ring by s mod (BI_D + 1)
-        __shared__ dd pw[BB];                // wave 1's partial of the block it prepares
+        __shared__ dd hp[2][BI_D][BB];  // prepared terms of a block, by block parity
+        __shared__ dd xs[BI_RING][BB];  // solved blocks, ring by s mod BI_RING
+        __shared__ dd pw[BB];           // wave 1's partial of the block it prepares
         __shared__ int stop;
-        const int li = lane < BB ? lane : 0;
-        // each wave holds row li of the operator it applies next (wave 0:
-        // op 1 of the block it solves next; wave 1: op 0 and wave w = 2, 3:
-        // op w of the block it prepares next), loaded one round ahead
-        dd opr[BB];
+        constexpr int PUB = BI_D + 1;   // the publishing wave
+        const int r = lane & (BB - 1), h = lane >> 5;
+        const int wop = wave == 1 ? 0 : wave;
+        // row r (terms 2k + h) of the operator the wave applies next,
+        // loaded a round ahead
+        dd opr[BB / 2];
         auto load_row = [&](int t, int d) {
             if (t >= nblk) return;
-            const dd *o = ops + (idx_t)t * BI_OPS + (idx_t)d * BB * BB + (idx_t)li * BB;
+            const dd *o = ops + (idx_t)t * BI_OPS + (idx_t)d * BB * BB + lane;
 #pragma unroll
-            for (int q = 0; q < BB; q++) opr[q] = o[q];
+            for (int k = 0; k < BB / 2; k++) opr[k] = o[k * 2 * BB];
         };
-        const int wop = wave == 1 ? 0 : wave;
-        // waves 1-3: terms of block t into hp[t & 1]
-        auto prepare = [&](int t) {
-            if (t >= nblk) return;
+        // wave 1: block t's partial (row lane), read during round t - 2;
+        // blocks <= BI_D take no update and come from x
+        RawDD pnext = {0, 0};
+        auto read_partial = [&](int t) {
             idx_t lo;
             int nb;
             seq_block(n, trans, t, lo, nb);
+            if (lane < nb) pnext = t >= BI_D + 1 ? get_raw(pg + (idx_t)t * BB + lane) : dd_raw(x[lo + lane]);
+        };
+        // waves 1..BI_D: terms of block t (< nblk) into hp[t & 1]
+        auto prepare = [&](int t) {
             dd a = mk(0.0);
             if (wave == 1) {
-                int ok = 1;
-                if (lane == 0 && t - BI_D - 1 >= 0) {
-                    const int u0 = (int)(lo / TC_ROWS), u1 = (int)((lo + nb - 1) / TC_ROWS);
-                    for (int u = u0; u <= u1 && ok; u++) ok = tc_wait(flags + 1 + u, t - BI_D, err, spin);
+                idx_t lo;
+                int nb;
+                seq_block(n, trans, t, lo, nb);
+                bool have = lane >= nb || t < BI_D + 1 || raw_ok(pnext);
+                for (unsigned it = 0; !__all(have); it++) {
+                    if (it >= spin || relaxed_flag(err) != 0) {
+                        if (lane == 0) {
+                            tc_store(err, 1);
+                            stop = 1;
+                        }
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    if (!have) {
+                        pnext = get_raw(pg + (idx_t)t * BB + lane);
+                        have = raw_ok(pnext);
+                    }
                 }
-                ok = __builtin_amdgcn_readlane(ok, 0);
-                if (!ok) {
-                    if (lane == 0) stop = 1;
-                    return;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                // broadcast through LDS (in-order within the wave), not readlane:
-                // a readlane per term would occupy the scalar registers
-                pw[lane] = lane < nb ? x[lo + lane] : mk(0.0);
+                // broadcast through LDS (in order within the wave)
+                if (lane < BB) pw[lane] = lane < nb ? raw_dd(pnext) : mk(0.0);
 #pragma unroll
-                for (int q = 0; q < BB; q++) opr[q] = dd_mul(opr[q], pw[q]);
-                a = tree_sum<BB>(opr);
+                for (int k = 0; k < BB / 2; k++) opr[k] = dd_mul(opr[k], pw[2 * k + h]);
+                a = pair_tree(opr, h);
             } else if (t - wave >= 0) {
-                const int slot = (t - wave) % (BI_D + 1);
+                const dd *xv = xs[(t - wave) & (BI_RING - 1)];
 #pragma unroll
-                for (int q = 0; q < BB; q++) opr[q] = dd_mul(opr[q], xs[slot][q]);
-                a = tree_sum<BB>(opr);
+                for (int k = 0; k < BB / 2; k++) opr[k] = dd_mul(opr[k], xv[2 * k + h]);
+                a = pair_tree(opr, h);
             }
-            if (lane < BB) hp[t & 1][wave - 1][lane] = a;
+            if (h == 0) hp[t & 1][wave - 1][r] = a;
             load_row(t + 1, wop);
+            // last: the compiler waits for every load in flight before
+            // issuing the operator loads above
+            if (wave == 1 && t + 1 < nblk) read_partial(t + 1);
+        };
+        // wave PUB: block b to x, and to the updaters while they need it
+        auto publish = [&](int b) {
+            idx_t lo;
+            int nb;
+            seq_block(n, trans, b, lo, nb);
+            if (lane < nb) {
+                const dd v = xs[b & (BI_RING - 1)][lane];
+                if (b + BI_D + 1 < nblk) put_dd(vg + (idx_t)b * BB + lane, v);
+                x[lo + lane] = v;
+            }
         };
         if (tid == 0) stop = 0;
-        load_row(wave == 0 ? 1 : 0, wave == 0 ? 1 : wop);
+        if (wave == 0) {
+            load_row(1, 1);
+        } else if (wave < PUB) {
+            load_row(0, wop);
+            if (wave == 1) read_partial(0);
+        }
         __syncthreads();
-        if (wave >= 1) prepare(0);
+        if (wave >= 1 && wave < PUB) prepare(0);
         __syncthreads();
         if (stop) return;
-        dd prev = mk(0.0);
         for (int s = 0; s < nblk; s++) {
             if (wave == 0) {
                 idx_t lo;
                 int nb;
                 seq_block(n, trans, s, lo, nb);
-                dd xr = dd_sub(dd_sub(hp[s & 1][0][li], hp[s & 1][1][li]), hp[s & 1][2][li]);
+                dd xr = hp[s & 1][0][r];
+#pragma unroll
+                for (int d = 1; d < BI_D; d++) xr = dd_sub(xr, hp[s & 1][d][r]);
                 if (s >= 1) {
                     // products in place: opr is reloaded for the next block below
+                    const dd *xv = xs[(s - 1) & (BI_RING - 1)];
 #pragma unroll
-                    for (int q = 0; q < BB; q++) opr[q] = dd_mul(opr[q], xs[(s - 1) % (BI_D + 1)][q]);
-                    xr = dd_sub(xr, tree_sum<BB>(opr));
+                    for (int k = 0; k < BB / 2; k++) opr[k] = dd_mul(opr[k], xv[2 * k + h]);
+                    xr = dd_sub(xr, pair_tree(opr, h));
                 }
-                if (lane >= nb) xr = mk(0.0);
-                if (lane < nb) {
-                    x[lo + lane] = xr;
-                    vg[(idx_t)s * BB + lane] = xr;
-                }
-                if (lane < BB) xs[s % (BI_D + 1)][lane] = xr;
-                prev = xr;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                if (lane == 0) tc_store(flags, s + 1);
+                if (r >= nb) xr = mk(0.0);
+                if (h == 0) xs[s & (BI_RING - 1)][r] = xr;
                 load_row(s + 1, 1);
-            } else {
-                prepare(s + 1);
+            } else if (wave < PUB) {
+                if (s + 1 < nblk) prepare(s + 1);
+            } else if (s >= 1) {
+                publish(s - 1);
             }
             __syncthreads();
             if (stop) return;
         }
+        if (wave == PUB) publish(nblk - 1);
         return;
     }
     const int u = blockIdx.x - 1;
-    const idx_t rlo = (idx_t)u * TC_ROWS, rhi = rlo + TC_ROWS < n ? rlo + TC_ROWS : n;
-    const idx_t i = rlo + tid;
-    __shared__ int go;
-    int avail = 0;  // blocks known to be published
+    const int hq = tid & 3;  // terms 4k + hq of the row
+    const idx_t rlo = (idx_t)u * BO_ROWS, rhi = rlo + BO_ROWS < n ? rlo + BO_ROWS : n;
+    const idx_t i = rlo + (tid >> 2);
+    constexpr int Q = BB / 4;
     dd xi = i < rhi ? x[i] : mk(0.0);
+    dd a[Q];
+    RawDD vr[Q];  // block J's values, read an iteration ahead
+    auto load_l = [&](int J) {
+        idx_t lo_n, bp;
+        int nb_n, nbj;
+        seq_block(n, trans, J + BI_D, lo_n, nb_n);
+        seq_block(n, trans, J, bp, nbj);
+        const bool act = i < rhi && (trans ? i < lo_n : i >= lo_n + nb_n);
+#pragma unroll
+        for (int k = 0; k < Q; k++) {
+            const idx_t c = bp + 4 * k + hq;
+            a[k] = !act ? mk(0.0) : !trans ? L[i + c * lda] : LT ? LT[i + c * n] : L[c + i * lda];
+        }
+    };
+    auto read_v = [&](int J) {
+#pragma unroll
+        for (int k = 0; k < Q; k++) vr[k] = get_raw(vg + (idx_t)J * BB + 4 * k + hq);
+    };
+    auto all_ok = [&]() {
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < Q; k++) ok = ok && raw_ok(vr[k]);
+        return ok;
+    };
+    if (BI_D + 1 < nblk) {
+        load_l(0);
+        read_v(0);
+    }
     for (int J = 0; J + BI_D + 1 < nblk; J++) {
         // rows of sequence blocks >= J + BI_D + 1 take block J here
-        idx_t lo_n;
-        int nb_n;
+        idx_t lo_n, lo_f;
+        int nb_n, nb_f;
         seq_block(n, trans, J + BI_D, lo_n, nb_n);
-        const bool act = i < rhi && (trans ? i < lo_n : i >= lo_n + nb_n);
         if (trans ? rlo >= lo_n : rhi <= lo_n + nb_n) break;
-        idx_t bp;
-        int nbj;
-        seq_block(n, trans, J, bp, nbj);
-        // the block's terms in two halves (register budget): the
-        // first half's loads are issued before the wait for block J
-        constexpr int H = BB / 2;
-        dd a[H];
-        auto load_half = [&](int h) {
-            if (!act) return;
-#pragma unroll
-            for (int q = 0; q < H; q++) {
-                const idx_t c = bp + H * h + q;
-                a[q] = !trans ? L[i + c * lda] : LT ? LT[i + c * n] : L[c + i * lda];
+        const bool act = i < rhi && (trans ? i < lo_n : i >= lo_n + nb_n);
+        for (unsigned it = 0; !__all(all_ok()); it++) {
+            if (it >= spin) {
+                if (lane == 0) tc_store(err, 1);
+                return;
             }
-        };
-        load_half(0);
-        if (J + 1 > avail) {
-            if (tid == 0) {
-                go = tc_wait(flags, J + 1, err, spin) ? tc_load(flags) : 0;
-            }
-            __syncthreads();
-            avail = go;
-            if (avail == 0) return;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (relaxed_flag(err) != 0) return;
+            __builtin_amdgcn_s_sleep(1);
+            read_v(J);
         }
+        dd pr[Q];
+#pragma unroll
+        for (int k = 0; k < Q; k++) pr[k] = dd_mul(a[k], raw_dd(vr[k]));
+        // block J + 1: its operands and values, checked next iteration
+        if (J + BI_D + 2 < nblk) {
+            load_l(J + 1);
+            read_v(J + 1);
+        }
+        dd t = tree_sum<Q>(pr);
+        dd o = shfl_xor_dd(t, 2);
+        t = (hq & 2) == 0 ? dd_add(t, o) : dd_add(o, t);
+        o = shfl_xor_dd(t, 1);
+        t = (hq & 1) == 0 ? dd_add(t, o) : dd_add(o, t);
         if (act) {
-#pragma unroll
-            for (int q = 0; q < H; q++) a[q] = dd_mul(a[q], vg[(idx_t)J * BB + q]);
-            const dd s0 = tree_sum<H>(a);
-            load_half(1);
-#pragma unroll
-            for (int q = 0; q < H; q++) a[q] = dd_mul(a[q], vg[(idx_t)J * BB + H + q]);
-            xi = dd_sub(xi, dd_add(s0, tree_sum<H>(a)));
-            x[i] = xi;
+            xi = dd_sub(xi, t);
+            // rows of block J + BI_D + 1 are final now
+            seq_block(n, trans, J + BI_D + 1, lo_f, nb_f);
+            if (hq == 0 && i >= lo_f && i < lo_f + nb_f) put_dd(pg + (idx_t)(J + BI_D + 1) * BB + (i - lo_f), xi);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (tid == 0) tc_store(flags + 1 + u, J + 1);
     }
-    if (tid == 0) tc_store(flags + 1 + u, nblk);
 }
 
 // lt[c + r n] = L[r + c lda] on the 32 x 32 tiles that meet the lower
@@ -1482,17 +1607,18 @@ int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x,
               const dd *lt) {
     static_assert(BB == SB, "the block-inverse solves share the cross-workgroup workspace layout");
     if (n <= 0) return 0;
-    const int nupd = (int)((n + TC_ROWS - 1) / TC_ROWS);
+    const int nupd = (int)((n + BO_ROWS - 1) / BO_ROWS);
     const CoopWork cw = coop_work(work, n);
-    dd *vg = cw.vg;
+    dd *vg = cw.vg, *pg = cw.pg;
     int *flags = cw.flags;
-    zero_ints(s, flags, 2 + nupd);
+    const idx_t len = ((n + BB - 1) / BB) * BB;
+    k_binv_reset<<<cdiv(std::max<idx_t>(len, 2 + nupd), 256), 256, 0, s>>>(len, vg, pg, flags, 2 + nupd);
     StatScope st("trsv", s, (double)n * n);
     copy(s, n, x, cw.x0);
     const int tr = (trans == 'T' || trans == 't') ? 1 : 0;
     unsigned spin = trsv_spin();
-    void *args[] = {&n, (void *)&tr, &L, &lda, &x, &ops, &flags, (void *)&nupd, &vg, (void *)&lt, &spin};
-    const hipError_t e = launch_coop((const void *)k_trsv_binv, dim3(1 + nupd), args, s);
+    void *args[] = {&n, (void *)&tr, &L, &lda, &x, &ops, &flags, (void *)&nupd, &vg, &pg, (void *)&lt, &spin};
+    const hipError_t e = launch_coop((const void *)k_trsv_binv, dim3(1 + nupd), args, s, BI_NT);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return 1;
