@@ -44,8 +44,10 @@ hipEvent_t get_event() {
         g_event_pool.pop_back();
         return e;
     }
+    // timing only: no system-scope fence when the event completes, so a
+    // record between two launches costs no cache write-back
     hipEvent_t e;
-    (void)hipEventCreate(&e);
+    (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
     return e;
 }
 void resolve(StatEntry &st) {
