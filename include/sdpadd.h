@@ -65,6 +65,13 @@ int sdpadd_set_tile_policy(sdpadd_handle h, int policy);
  * never depend on co-residency.  Process-wide; tests set a tiny bound to
  * reach that recovery. */
 int sdpadd_set_wait_bound(sdpadd_handle h, unsigned spins);
+/* Bound on the consecutive empty polls of a Cholesky panel workgroup waiting
+ * for the columns of its diagonal block (no counterpart in the reference):
+ * 0 = the default 2^22 (seconds).  The diagonal workgroup is the first of
+ * its launch to start and waits for nothing, so the wait always ends; a
+ * bound that runs out anyway stores nothing and reports info = -2 (Rpotrf)
+ * or a solve error.  Process-wide; tests set a tiny bound to reach it. */
+int sdpadd_set_panel_wait_bound(sdpadd_handle h, unsigned polls);
 int sdpadd_malloc(sdpadd_handle h, size_t bytes, void **d_ptr);
 int sdpadd_free(sdpadd_handle h, void *d_ptr);
 int sdpadd_upload(sdpadd_handle h, void *d_dst, const void *src, size_t bytes);
@@ -171,7 +178,8 @@ int sdpadd_Raxpy(sdpadd_handle h, int64_t n, const double *alpha, const double *
 int sdpadd_Rscal(sdpadd_handle h, int64_t n, const double *alpha, double *d_x, int64_t incx);
 int sdpadd_Rcopy(sdpadd_handle h, int64_t n, const double *d_x, int64_t incx, double *d_y, int64_t incy);
 /* Rpotrf: uplo 'L'; *info = 0 or the first non-positive leading minor
- * (mplapack/Rpotrf.cpp:32-154). */
+ * (mplapack/Rpotrf.cpp:32-154); -2 (no counterpart in the reference) when a
+ * panel wait ran out of the bound of sdpadd_set_panel_wait_bound. */
 int sdpadd_Rpotrf(sdpadd_handle h, char uplo, int64_t n, double *d_A, int64_t lda, int64_t *info);
 /* Rsteqr on HOST arrays, compz 'I' (mplapack/Rsteqr.cpp:32-503).
  * work: 2*n dd. */

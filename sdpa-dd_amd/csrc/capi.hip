@@ -122,6 +122,11 @@ int sdpadd_set_wait_bound(sdpadd_handle h, unsigned spins) {
     sdk::g_trsv_spin = spins;
     return 0;
 }
+int sdpadd_set_panel_wait_bound(sdpadd_handle h, unsigned polls) {
+    if (!h) return bad(h, "set_panel_wait_bound: null handle");
+    sdk::g_panel_spin = polls;
+    return 0;
+}
 int sdpadd_dd_vec_op(sdpadd_handle h, int op, int64_t n, const double *d_a, const double *d_b, double *d_out) {
     if (!h || op < 0 || op > 11 || n < 0 || (n > 0 && (!d_a || !d_b || !d_out))) return bad(h, "dd_vec_op: bad argument");
     sdk::dd_vec_op(h->c.stream, op, n, DP(d_a), DP(d_b), DP(d_out));

@@ -18,7 +18,7 @@ struct Context {
     unsigned *d_counter = nullptr;
     dd *d_scal = nullptr;       // scalar slots for dot results
     int64_t *d_info = nullptr;
-    dd *d_rinv = nullptr;       // Cholesky panel reciprocals (>= 64)
+    dd *d_rinv = nullptr;       // Cholesky scratch (sdk::POTRF_SCRATCH_DD dd)
     dd *d_max_work = nullptr;   // max-abs partials (sdk::MAX_ABS_BLOCKS)
     oz::Work ozw;               // digit planes of the Ozaki products
     // Solver form: 0 = plain (the reference's dd operation order); 1 = Ozaki

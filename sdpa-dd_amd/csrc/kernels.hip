@@ -302,6 +302,41 @@ void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx
                                                 0, 0, 0, A, B, C);
 }
 
+// --------------------------------------------- values as their own messages --
+// Values passed between the workgroups of one launch (the fused Cholesky
+// step, the block-inverse solve): each double is an agent-scope atomic,
+// written once per launch (relaxed) into a slot that holds BI_EMPTY until then, and read (relaxed) until both words
+// differ from it.  The value is its own message: no flag, no fence.
+// BI_EMPTY is a signalling-NaN pattern, which no arithmetic result takes.
+static constexpr unsigned long long BI_EMPTY = 0x7FF4A5A5A5A5A5A5ull;
+__device__ __forceinline__ int relaxed_flag(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void put_dd(dd *p, dd v) {
+    __hip_atomic_store((unsigned long long *)&p->hi, (unsigned long long)__double_as_longlong(v.hi), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store((unsigned long long *)&p->lo, (unsigned long long)__double_as_longlong(v.lo), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+// The read is issued now and checked later (raw_ok), so its latency
+// overlaps other work: the check of a value needs the load to be done.
+struct RawDD {
+    unsigned long long hi, lo;
+};
+__device__ __forceinline__ RawDD get_raw(const dd *p) {
+    RawDD r;
+    r.hi = __hip_atomic_load((unsigned long long *)&p->hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r.lo = __hip_atomic_load((unsigned long long *)&p->lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return r;
+}
+__device__ __forceinline__ bool raw_ok(RawDD r) { return r.hi != BI_EMPTY && r.lo != BI_EMPTY; }
+__device__ __forceinline__ dd raw_dd(RawDD r) {
+    return dd{__longlong_as_double((long long)r.hi), __longlong_as_double((long long)r.lo)};
+}
+__device__ __forceinline__ RawDD dd_raw(dd v) {
+    return RawDD{(unsigned long long)__double_as_longlong(v.hi), (unsigned long long)__double_as_longlong(v.lo)};
+}
+
 // -------------------------------------------------------------- Cholesky --
 static constexpr int PB = 64;  // Cholesky panel width
 static constexpr int PD_NT = 1024;                                 // threads of the diagonal-block kernel
@@ -548,20 +583,336 @@ __global__ __launch_bounds__(256) void k_panel_rltn(dd *B, idx_t ldb, idx_t r0, 
 
 __global__ void k_zero_i64(int64_t *p) { *p = 0; }
 
+// ------------------------------------ diagonal block and panel in one launch --
+// One launch factors a diagonal block and solves the panel rows below it.
+// The first workgroup to start (a ticket from a counter, so it is resident
+// by construction) runs k_potrf_diag's schedule and publishes, as it goes,
+// each pivot reciprocal r_k and each scaled column l_jk (j > k) into a
+// buffer whose slots hold BI_EMPTY until written (every value is its own
+// message: no flag, no fence).  Every other workgroup takes FR panel rows:
+// its last wave receives the published columns into LDS in order, a few
+// columns per round trip, and advances an LDS counter; the other waves hold
+// two rows each (32 lanes per row, columns c and 32 + c) and apply column k
+// as soon as the counter passes it.  A panel workgroup waits only for the
+// diagonal workgroup, which waits for nothing, so the grid always drains;
+// a wait that still runs out of its bound (or sees a failed pivot) leaves
+// without storing and, unless a pivot failed, sets info to
+// POTRF_WAIT_FAILED.  Each element sees the operation sequence of
+// k_potrf_diag followed by k_panel_rltn: the results are identical.
+// The publication buffers alternate between launches; the diagonal
+// workgroup refills the other one with BI_EMPTY for the next launch.
+static constexpr int FP_WAVES = PD_NT / 64 - 1;  // row waves of a panel workgroup
+static constexpr int FR = 2 * FP_WAVES;          // panel rows per workgroup
+static constexpr int FP_AHEAD = 4;               // columns read per round trip
+static constexpr int PUB_DD = PB * PB + PB;      // one publication buffer: columns, then reciprocals
+static_assert(POTRF_SCRATCH_DD >= PB + 2 * PUB_DD + 1, "Cholesky scratch layout");
+
+// bound on a panel workgroup's consecutive empty polls (0: 2^22, seconds)
+unsigned g_panel_spin = 0;
+static unsigned panel_spin() { return g_panel_spin ? g_panel_spin : (1u << 22); }
+
+struct PotrfPub {
+    dd *rinv, *buf[2];
+    unsigned long long *ctr;
+    explicit PotrfPub(dd *scratch)
+        : rinv(scratch), buf{scratch + PB, scratch + PB + PUB_DD},
+          ctr(reinterpret_cast<unsigned long long *>(scratch + PB + 2 * PUB_DD)) {}
+};
+
+// info = 0, ticket counter = 0, both publication buffers empty.
+__global__ void k_potrf_init(int64_t *info, dd *pub, unsigned long long *ctr) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const dd empty = {__longlong_as_double((long long)BI_EMPTY), __longlong_as_double((long long)BI_EMPTY)};
+    if (e < 2 * PUB_DD) pub[e] = empty;
+    if (e == 0) {
+        *info = 0;
+        *ctr = 0;
+    }
+}
+
+template <bool FAST>
+__global__ __launch_bounds__(PD_NT) void k_potrf_fused(dd *A, idx_t lda, idx_t K, int nb, idx_t below, int64_t *info,
+                                                       dd *rinv, dd *pub, dd *pub_next, unsigned long long *ctr,
+                                                       unsigned long long base, unsigned spin) {
+    struct DiagLds {
+        dd S[PB][PB + 1];
+        dd rl[PB];
+        int failed;
+    };
+    struct PanelLds {
+        dd Lc[PB * (PB + 1) / 2];
+        dd rl[PB];
+        dd Tt[FR][PB + 1];
+        int ready, stop;
+    };
+    __shared__ union {
+        DiagLds d;
+        PanelLds p;
+    } sm;
+    __shared__ unsigned long long ticket;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid == 0) ticket = __hip_atomic_fetch_add(ctr, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - base;
+    __syncthreads();
+    const unsigned long long role = ticket;
+    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    dd *pubr = pub + PB * PB;
+    if (role == 0) {
+        // ---- the diagonal block: k_potrf_diag's schedule, publishing ----
+        auto &S = sm.d.S;
+        auto &rl = sm.d.rl;
+        {
+            constexpr int NL = PB * PB / PD_NT;
+            dd v[NL];
+#pragma unroll
+            for (int q = 0; q < NL; q++) {
+                const int e = tid + PD_NT * q, i = e % PB, j = e / PB;
+                v[q] = (i >= j && i < nb) ? A[(K + i) + (K + j) * lda] : mk(0.0);
+            }
+#pragma unroll
+            for (int q = 0; q < NL; q++) {
+                const int e = tid + PD_NT * q, i = e % PB, j = e / PB;
+                if (i >= j && i < nb) S[i][j] = v[q];
+            }
+        }
+        int own[PD_OWN];
+        {
+            const int t = tid - 64;
+#pragma unroll
+            for (int q = 0; q < PD_OWN; q++) {
+                own[q] = 0;
+                const int e = t + PD_UPD * q;
+                if (t >= 0 && e < nb * (nb + 1) / 2) {
+                    int ri, ci;
+                    tri_cm(e, nb, ri, ci);
+                    own[q] = ri | (ci << 8);
+                }
+            }
+        }
+        __syncthreads();
+        auto pivot = [&](int k) {
+            dd a = S[k][k];
+            if (dd_le(a, mk(0.0)) || dd_isnan(a)) {
+                sm.d.failed = 1;
+                __hip_atomic_store(info, (int64_t)(K + k + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (FAST) {
+                const double y = rsqrt(a.hi);
+                const dd e = d_sub_dd(1.0, dd_mul(a, dd_sqr_d(y)));
+                double lo;
+                const double hi = two_sum(y, e.hi * (0.5 * y), lo);
+                const dd r = dd{hi, lo};
+                rl[k] = r;
+                put_dd(pubr + k, r);
+                A[(K + k) + (K + k) * lda] = dd_mul(a, r);
+            } else {
+                dd l = dd_sqrt(a);
+                A[(K + k) + (K + k) * lda] = l;
+                const dd r = dd_div(mk(1.0), l);
+                rl[k] = r;
+                put_dd(pubr + k, r);
+            }
+        };
+        if (tid == 0) {
+            sm.d.failed = 0;
+            pivot(0);
+        }
+        __syncthreads();
+        for (int k = 0; k < nb; k++) {
+            if (sm.d.failed) return;
+            const dd r = rl[k];
+            const int own_pivot = (k + 1) | ((k + 1) << 8);
+            if (tid < 64) {
+                if (tid == 0) {
+                    if (k + 1 < nb) {
+                        dd l = dd_mul(r, S[k + 1][k]);
+                        if (!dd_is_zero(l)) S[k + 1][k + 1] = dd_sub(S[k + 1][k + 1], dd_mul(l, l));
+                        pivot(k + 1);
+                    }
+                } else if (tid > k && tid < nb) {
+                    const dd l = dd_mul(r, S[tid][k]);
+                    A[(K + tid) + (K + k) * lda] = l;
+                    put_dd(pub + k * PB + tid, l);
+                }
+            } else {
+                dd sv[PD_OWN], aj[PD_OWN], ai[PD_OWN];
+                bool act[PD_OWN];
+#pragma unroll
+                for (int q = 0; q < PD_OWN; q++) {
+                    const int i = own[q] & 0xff, j = own[q] >> 8;
+                    act[q] = j > k && own[q] != own_pivot;
+                    if (act[q]) {
+                        sv[q] = S[i][j];
+                        aj[q] = S[j][k];
+                        ai[q] = S[i][k];
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < PD_OWN; q++) {
+                    const int i = own[q] & 0xff, j = own[q] >> 8;
+                    if (act[q]) {
+                        const dd lj = dd_mul(r, aj[q]);
+                        if (!dd_is_zero(lj)) S[i][j] = dd_sub(sv[q], dd_mul(lj, dd_mul(r, ai[q])));
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        for (int k = tid; k < nb; k += PD_NT) rinv[k] = rl[k];
+        const dd empty = {__longlong_as_double((long long)BI_EMPTY), __longlong_as_double((long long)BI_EMPTY)};
+        for (int e = tid; e < PUB_DD; e += PD_NT) pub_next[e] = empty;
+        return;
+    }
+    // ---- panel rows [rb, rb + nr) ----
+    auto &P = sm.p;
+    const idx_t r0 = K + nb, rb = r0 + (idx_t)(role - 1) * FR;
+    const int nr = (int)((r0 + below - rb) < FR ? (r0 + below - rb) : FR);
+    dd *B = A + K * lda;
+    {
+        constexpr int NT = (FR * PB + PD_NT - 1) / PD_NT;
+        dd tv[NT];
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+            const int e = tid + PD_NT * q, r = e % FR, j = e / FR;
+            tv[q] = (e < FR * PB && r < nr && j < nb) ? B[(rb + r) + (idx_t)j * lda] : mk(0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+            const int e = tid + PD_NT * q, r = e % FR, j = e / FR;
+            if (e < FR * PB) P.Tt[r][j] = tv[q];
+        }
+        if (tid == 0) {
+            P.ready = 0;
+            P.stop = 0;
+        }
+    }
+    __syncthreads();
+    if (wave == FP_WAVES) {
+        // receiver: columns k0 .. k0 + FP_AHEAD - 1 read together, the
+        // valid ones taken in order
+        int k0 = 0;
+        unsigned empty_polls = 0;
+        while (k0 < nb) {
+            RawDD v[FP_AHEAD], rr[FP_AHEAD];
+#pragma unroll
+            for (int d = 0; d < FP_AHEAD; d++) {
+                const int kc = k0 + d;
+                v[d] = RawDD{0, 0};
+                rr[d] = RawDD{0, 0};
+                if (kc < nb) {
+                    if (lane > kc && lane < nb) v[d] = get_raw(pub + kc * PB + lane);
+                    if (lane == 0) rr[d] = get_raw(pubr + kc);
+                }
+            }
+            int got = 0;
+#pragma unroll
+            for (int d = 0; d < FP_AHEAD; d++) {
+                const int kc = k0 + d;
+                if (kc >= nb || got < d) break;
+                const bool ok = (lane <= kc || lane >= nb || raw_ok(v[d])) && (lane != 0 || raw_ok(rr[d]));
+                if (!__all(ok)) break;
+                if (lane > kc && lane < nb) P.Lc[lpack(lane, kc)] = raw_dd(v[d]);
+                if (lane == 0) P.rl[kc] = raw_dd(rr[d]);
+                got++;
+            }
+            if (got) {
+                k0 += got;
+                empty_polls = 0;
+                if (lane == 0) __hip_atomic_store(&P.ready, k0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                continue;
+            }
+            if (++empty_polls >= spin || __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                if (lane == 0) {
+                    unsigned long long zero = 0;
+                    __hip_atomic_compare_exchange_strong(reinterpret_cast<unsigned long long *>(info), &zero,
+                                                         (unsigned long long)POTRF_WAIT_FAILED, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    P.stop = 1;
+                    __hip_atomic_store(&P.ready, PB + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    } else {
+        // two rows per wave; lane c of a row holds columns c and 32 + c
+        const int g = lane >> 5, c = lane & 31, row = 2 * wave + g;
+        dd x[2];
+#pragma unroll
+        for (int b = 0; b < 2; b++) x[b] = (32 * b + c < nb) ? P.Tt[row][32 * b + c] : mk(0.0);
+        int seen = 0;  // columns known to be in LDS
+        bool stopped = false;
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            if (32 * b >= nb || stopped) break;
+            const int kend = (nb - 32 * b) < 32 ? (nb - 32 * b) : 32;
+            for (int kk = 0; kk < kend; kk++) {
+                const int k = 32 * b + kk;
+                if (k >= seen) {
+                    while ((seen = __hip_atomic_load(&P.ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) <= k)
+                        __builtin_amdgcn_s_sleep(1);
+                    if (P.stop) {
+                        stopped = true;
+                        break;
+                    }
+                }
+                dd l[2];
+#pragma unroll
+                for (int b2 = b; b2 < 2; b2++) {
+                    const int j = 32 * b2 + c;
+                    l[b2] = (j > k && j < nb) ? P.Lc[lpack(j, k)] : mk(0.0);
+                }
+                if (c == kk) x[b] = dd_mul(P.rl[k], x[b]);
+                const dd xk = shfl_dd(x[b], (lane & 32) | kk);
+#pragma unroll
+                for (int b2 = b; b2 < 2; b2++)
+                    if (!dd_is_zero(l[b2])) x[b2] = dd_sub(x[b2], dd_mul(l[b2], xk));
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+            if (32 * b + c < nb) P.Tt[row][32 * b + c] = x[b];
+    }
+    __syncthreads();
+    if (P.stop) return;
+    for (int e = tid; e < FR * nb; e += PD_NT) {
+        const int r = e % FR, j = e / FR;
+        if (r < nr) B[(rb + r) + (idx_t)j * lda] = P.Tt[r][j];
+    }
+}
+
+// Diagonal block K (nb columns) and the `below` rows under it: one fused
+// launch when there are rows below, the diagonal kernel alone otherwise.
+// `launches` counts the fused launches of this factorisation (buffer
+// parity and ticket base).
+template <bool FAST>
+static void potrf_step(hipStream_t s, idx_t K, int nb, idx_t below, dd *A, idx_t lda, int64_t *d_info,
+                       const PotrfPub &pp, unsigned long long &base, int &launches) {
+    StatScope st("potrf", s, (double)nb * nb * nb / 3.0 + (double)below * nb * nb);
+    if (below <= 0) {
+        k_potrf_diag<FAST><<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, pp.rinv);
+        return;
+    }
+    const unsigned grid = 1 + cdiv(below, FR);
+    const int par = launches & 1;
+    k_potrf_fused<FAST><<<grid, PD_NT, 0, s>>>(A, lda, K, nb, below, d_info, pp.rinv, pp.buf[par], pp.buf[par ^ 1],
+                                               pp.ctr, base, panel_spin());
+    base += grid;
+    launches++;
+}
+
+static void potrf_begin(hipStream_t s, int64_t *d_info, const PotrfPub &pp) {
+    k_potrf_init<<<cdiv(2 * PUB_DD, 256), 256, 0, s>>>(d_info, pp.buf[0], pp.ctr);
+}
+
 void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch) {
-    k_zero_i64<<<1, 1, 0, s>>>(d_info);
+    const PotrfPub pp(scratch);
+    potrf_begin(s, d_info, pp);
+    unsigned long long base = 0;
+    int launches = 0;
     for (idx_t K = 0; K < n; K += PB) {
         int nb = (int)std::min<idx_t>(PB, n - K);
         idx_t below = n - K - nb;
-        {
-            StatScope st("potrf", s, (double)nb * nb * nb / 3.0);
-            k_potrf_diag<false><<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
-        }
-        if (below > 0) {
-            StatScope st("panel", s, (double)below * nb * nb);
-            k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda, nb,
-                                                         scratch, d_info);
-        }
+        potrf_step<false>(s, K, nb, below, A, lda, d_info, pp, base, launches);
         if (below > 0) {
             // trailing lower triangle: A22 += (-L21) * L21^T, k ascending
             dd *A21 = A + (K + nb) + K * lda;
@@ -590,25 +941,20 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
     // the pivot chain (m square roots and reciprocals in sequence) is the
     // critical path of the large factorisations: those take the faster pivot
     const bool fast_pivot = n > 1024;
-    k_zero_i64<<<1, 1, 0, s>>>(d_info);
+    const PotrfPub pp(scratch);
+    potrf_begin(s, d_info, pp);
+    unsigned long long base = 0;
+    int launches = 0;
     bool helper_pending = false;  // an update on la->hs that later work on s must wait for
     for (idx_t P = 0; P < n; P += OZ_NB) {
         const idx_t pw = std::min<idx_t>(OZ_NB, n - P);
         for (idx_t K = P; K < P + pw; K += PB) {
             const int nb = (int)std::min<idx_t>(PB, P + pw - K);
             const idx_t below = n - K - nb;
-            {
-                StatScope st("potrf", s, (double)nb * nb * nb / 3.0);
-                if (fast_pivot)
-                    k_potrf_diag<true><<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
-                else
-                    k_potrf_diag<false><<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, scratch);
-            }
-            if (below > 0) {
-                StatScope st("panel", s, (double)below * nb * nb);
-                k_panel_rltn<<<cdiv(below, PR), 256, 0, s>>>(A + K * lda, lda, K + nb, below, A + K + K * lda, lda, nb,
-                                                             scratch, d_info);
-            }
+            if (fast_pivot)
+                potrf_step<true>(s, K, nb, below, A, lda, d_info, pp, base, launches);
+            else
+                potrf_step<false>(s, K, nb, below, A, lda, d_info, pp, base, launches);
             const idx_t rem = P + pw - K - nb;  // panel columns right of this block
             if (rem > 0) {
                 // the panel's remaining columns, every row below: an Ozaki
@@ -1329,39 +1675,6 @@ __device__ __forceinline__ dd pair_tree(dd *t, int h) {
     const dd mine = tree_sum<BB / 2>(t);
     const dd other = shfl_xor_dd(mine, 32);
     return h == 0 ? dd_add(mine, other) : dd_add(other, mine);
-}
-// Values passed between the workgroups of the block-inverse solve: each
-// double is an agent-scope atomic, written once per solve (relaxed) into a
-// slot that holds BI_EMPTY until then, and read (relaxed) until both words
-// differ from it.  The value is its own message: no flag, no fence.
-// BI_EMPTY is a signalling-NaN pattern, which no arithmetic result takes.
-static constexpr unsigned long long BI_EMPTY = 0x7FF4A5A5A5A5A5A5ull;
-__device__ __forceinline__ int relaxed_flag(const int *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void put_dd(dd *p, dd v) {
-    __hip_atomic_store((unsigned long long *)&p->hi, (unsigned long long)__double_as_longlong(v.hi), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store((unsigned long long *)&p->lo, (unsigned long long)__double_as_longlong(v.lo), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-// The read is issued now and checked later (raw_ok), so its latency
-// overlaps other work: the check of a value needs the load to be done.
-struct RawDD {
-    unsigned long long hi, lo;
-};
-__device__ __forceinline__ RawDD get_raw(const dd *p) {
-    RawDD r;
-    r.hi = __hip_atomic_load((unsigned long long *)&p->hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    r.lo = __hip_atomic_load((unsigned long long *)&p->lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return r;
-}
-__device__ __forceinline__ bool raw_ok(RawDD r) { return r.hi != BI_EMPTY && r.lo != BI_EMPTY; }
-__device__ __forceinline__ dd raw_dd(RawDD r) {
-    return dd{__longlong_as_double((long long)r.hi), __longlong_as_double((long long)r.lo)};
-}
-__device__ __forceinline__ RawDD dd_raw(dd v) {
-    return RawDD{(unsigned long long)__double_as_longlong(v.hi), (unsigned long long)__double_as_longlong(v.lo)};
 }
 __global__ void k_binv_reset(idx_t len, dd *vg, dd *pg, int *flags, int nflags) {
     const idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;

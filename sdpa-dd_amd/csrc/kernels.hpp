@@ -18,8 +18,14 @@ void gemm(hipStream_t s, char tb, idx_t m, idx_t n, idx_t k, dd alpha, const dd 
           dd beta, dd *C, idx_t ldc, int batch = 1, idx_t sA = 0, idx_t sB = 0, idx_t sC = 0,
           bool lower_only = false, idx_t diag_offset = 0);
 
-// Lower Cholesky in place.  info (device int64) receives 0 or the 1-based
-// index of the first non-positive pivot.  scratch: >= n dd of device memory.
+// Lower Cholesky in place.  info (device int64) receives 0, the 1-based
+// index of the first non-positive pivot, or POTRF_WAIT_FAILED when a panel
+// workgroup's bounded wait for the diagonal block ran out (nothing is
+// stored then; the factor is unusable).  scratch: POTRF_SCRATCH_DD dd of
+// device memory, one per stream that factors concurrently (pivot
+// reciprocals, two publication buffers and a ticket counter).
+constexpr int64_t POTRF_WAIT_FAILED = -2;
+constexpr size_t POTRF_SCRATCH_DD = 64 + 2 * (64 * 64 + 64) + 1;
 void potrf_lower(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd *scratch);
 // Cholesky of the precision path: dd panels, Ozaki INT8-MFMA trailing
 // updates with S digit planes.  Returns 0 or the Ozaki product's error code.
@@ -175,6 +181,7 @@ extern int g_tile_policy;
 // (iterations of about 0.25 us; 0 = the default of 2^22).  A wait that runs
 // out hands the solve to the single-workgroup recovery.
 extern unsigned g_trsv_spin;
+extern unsigned g_panel_spin;  // bound of the Cholesky panel workgroups' polls (0 = default)
 
 // Per-kernel-family timing with HIP events on the launch stream (off by
 // default).  Families: "gemm", "ozaki", "potrf", "panel", "trsm", "trmm",

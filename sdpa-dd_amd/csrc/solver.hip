@@ -25,7 +25,7 @@ int Context::init(int dev) {
     if (hipMalloc((void **)&d_counter, sizeof(unsigned) * 4) != hipSuccess) return -1;
     if (hipMalloc((void **)&d_scal, sizeof(dd) * 64) != hipSuccess) return -1;
     if (hipMalloc((void **)&d_info, sizeof(int64_t)) != hipSuccess) return -1;
-    if (hipMalloc((void **)&d_rinv, sizeof(dd) * 64) != hipSuccess) return -1;
+    if (hipMalloc((void **)&d_rinv, sizeof(dd) * sdk::POTRF_SCRATCH_DD) != hipSuccess) return -1;
     if (hipMalloc((void **)&d_max_work, sizeof(dd) * sdk::MAX_ABS_BLOCKS) != hipSuccess) return -1;
     if (hipMemset(d_counter, 0, sizeof(unsigned) * 4) != hipSuccess) return -1;
     return 0;
@@ -747,9 +747,14 @@ class Solver {
         enq_inv_chol_start();
         enq_inv_chol_join();
     }
-    bool inv_chol_ok() const {  // after a synchronisation
-        for (int q = 0; q < 2 * nB + (nlp ? 2 : 0); q++)
+    // a panel workgroup's wait for its diagonal block ran out (an error of
+    // the run, not a property of the matrix): the solve returns an error
+    bool wait_failed = false;
+    bool inv_chol_ok() {  // after a synchronisation
+        for (int q = 0; q < 2 * nB + (nlp ? 2 : 0); q++) {
+            if (q < 2 * nB && h_infos[q] == sdk::POTRF_WAIT_FAILED) wait_failed = true;
             if (h_infos[q] != 0) return false;
+        }
         return true;
     }
     bool inv_chol_xz() {
@@ -910,7 +915,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         if (!(d_lp_eig = alloc(2 * nlp))) return -2;
         if (hipHostMalloc((void **)&h_lp_eig, sizeof(dd) * 2 * nlp) != hipSuccess) return -2;
     }
-    if (!(d_rinv2 = alloc(64)) || !(d_rinv3 = alloc(64))) return -2;
+    if (!(d_rinv2 = alloc(sdk::POTRF_SCRATCH_DD)) || !(d_rinv3 = alloc(sdk::POTRF_SCRATCH_DD))) return -2;
     if (!spd && ctx.path >= 1 && (!(bSave = alloc((idx_t)m * m)) || !(rvec = alloc(m)) || !(dyTry = alloc(m))))
         return -2;
     if (spd && (!(spv = alloc(spd->nnz)) || !(spx = alloc(m)))) return -2;
@@ -1126,6 +1131,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             if (hipEventElapsedTime(&ms1, ev_t0, ev_t1) == hipSuccess) t_schur += 1e-3 * ms1;
             if (hipEventElapsedTime(&ms2, ev_t1, ev_t2) == hipSuccess) t_chol += 1e-3 * ms2;
             if (ctx.path >= 1 && !spd) bmax = sc(slot_bmax);
+            if (info == sdk::POTRF_WAIT_FAILED) wait_failed = true;
             if (info != 0) return false;
             // the Ozaki form solves with the block-inverse operators of this factor
             binv = !spd && ctx.path >= 1 && m >= 512 && ctx.binv_prepare(s, m, bMat, m) == 0;
@@ -1377,6 +1383,10 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         ctx.set_error(hipGetErrorString(e));
+        return -3;
+    }
+    if (wait_failed) {
+        ctx.set_error("solve: a Cholesky panel wait for its diagonal block ran out of its bound");
         return -3;
     }
     return 0;

@@ -52,6 +52,7 @@ def load_library(path=None):
         "sdpadd_malloc": (ctypes.c_int, [vp, ctypes.c_size_t, ctypes.POINTER(vp)]),
         "sdpadd_set_tile_policy": (ctypes.c_int, [vp, ctypes.c_int]),
         "sdpadd_set_wait_bound": (ctypes.c_int, [vp, ctypes.c_uint]),
+        "sdpadd_set_panel_wait_bound": (ctypes.c_int, [vp, ctypes.c_uint]),
         "sdpadd_dd_vec_op": (ctypes.c_int, [vp, ctypes.c_int, c_i64, vp, vp, vp]),
         "sdpadd_stats_enable": (ctypes.c_int, [vp, ctypes.c_int]),
         "sdpadd_stats_reset": (ctypes.c_int, [vp]),
@@ -188,6 +189,12 @@ class Device:
         (0 = default); a wait that runs out hands the solve to the
         single-workgroup recovery (same results)."""
         self.check(self.lib.sdpadd_set_wait_bound(self.h, int(spins)), "set_wait_bound")
+
+    def set_panel_wait_bound(self, polls):
+        """Bound on a Cholesky panel workgroup's consecutive empty polls for
+        its diagonal block's columns (0 = default); one that runs out makes
+        Rpotrf report info -2 and a solve fail with an error."""
+        self.check(self.lib.sdpadd_set_panel_wait_bound(self.h, int(polls)), "set_panel_wait_bound")
 
     def dd_vec_op(self, op, a, b):
         """QD operator `op` (0..11, see sdpadd_dd_vec_op) elementwise on (n, 2) dd arrays."""

@@ -75,6 +75,35 @@ def test_rpotrf_not_pd(gpu, oracle, n, bad):
     assert info > 0 and gi.value == info
 
 
+@pytest.mark.parametrize("n", [300, 1000])
+def test_rpotrf_panel_wait_bound(gpu, oracle, n):
+    """Each diagonal block and the panel rows below it are one launch whose
+    panel workgroups poll for the diagonal workgroup's published columns.
+    With a bound of one empty poll (set from the host) a panel workgroup
+    gives up as soon as a column is not there yet: Rpotrf then reports -2
+    and no panel rows are stored (or, if every poll found its columns, it
+    completes bit-identical).  With the default bound it is bit-identical."""
+    rng = np.random.default_rng(n + 3)
+    A = rand_spd_cm(n, rng)
+    ref, info = oracle.rpotrf(n, A, n)
+    assert info == 0
+    gpu.set_panel_wait_bound(1)
+    try:
+        dA = gpu.to_device(A)
+        gi = ctypes.c_int64(-7)
+        gpu.check(gpu.lib.sdpadd_Rpotrf(gpu.h, b"L", n, dA.ptr, n, ctypes.byref(gi)), "Rpotrf")
+        print("n = %d, bound 1: info %d" % (n, gi.value))
+        assert gi.value in (0, -2)
+        if gi.value == 0:
+            assert same(dA.get(), ref)
+    finally:
+        gpu.set_panel_wait_bound(0)
+    dA = gpu.to_device(A)
+    gi = ctypes.c_int64(-7)
+    gpu.check(gpu.lib.sdpadd_Rpotrf(gpu.h, b"L", n, dA.ptr, n, ctypes.byref(gi)), "Rpotrf")
+    assert gi.value == 0 and same(dA.get(), ref)
+
+
 @pytest.mark.parametrize("n", [3, 64, 100, 257])
 def test_inverse_cholesky(gpu, oracle, n):
     """chol + zero upper + Rtrsm(L,L,N,N) on I + Rtrmm(L,L,T,N): Jal::getInvCholAndInv."""
