@@ -189,6 +189,52 @@ __device__ __forceinline__ dd recombine(L lev) {
     return h;
 }
 
+// Tile order of the 64 x 64 kernel.  Workgroups are dealt to the eight
+// XCDs in turn (workgroup L runs on XCD L mod 8), and each XCD has its own
+// L2.  The tiles are grouped in super-blocks of XB_R x XB_C = 32 tiles,
+// listed column-major (for a lower-triangular product only those that reach
+// the lower triangle: super-column cs starts at super-row cs / 2), and the
+// list is dealt to the XCDs in turn: XCD x takes super-blocks x, x + 8,
+// x + 16, ..., workgroup L tile (L / 8) mod 32 of list entry
+// x + 8 ((L / 8) / 32).  The 32 CUs of an XCD then work on one super-block
+// at a time, whose XB_R row tiles and XB_C column tiles of digit planes come
+// from HBM or the last-level cache once and from L2 after that.  Tiles
+// outside the grid (partial super-blocks) leave at once.
+static constexpr int XB_R = 8, XB_C = 4, XB = XB_R * XB_C, NXCD = 8;
+__host__ __device__ __forceinline__ idx_t xcd_blocks(idx_t nsr, idx_t nsc, bool lower) {
+    if (!lower) return nsr * nsc;
+    idx_t t = 0;
+    for (idx_t cs = 0; cs < nsc; cs++) t += nsr - cs / 2 > 0 ? nsr - cs / 2 : 0;
+    return t;
+}
+static idx_t xcd_grid(idx_t mt, idx_t nt, bool lower) {
+    const idx_t nsb = xcd_blocks((mt + XB_R - 1) / XB_R, (nt + XB_C - 1) / XB_C, lower);
+    return NXCD * XB * ((nsb + NXCD - 1) / NXCD);
+}
+__device__ __forceinline__ bool xcd_tile(idx_t mt, idx_t nt, bool lower, idx_t &tr, idx_t &tc) {
+    const idx_t L = blockIdx.x, j = L / NXCD, t = j % XB;
+    const idx_t nsr = (mt + XB_R - 1) / XB_R, nsc = (nt + XB_C - 1) / XB_C;
+    const idx_t sb = L % NXCD + NXCD * (j / XB);
+    idx_t cs, rs;
+    if (!lower) {
+        if (sb >= nsr * nsc) return false;
+        cs = sb / nsr;
+        rs = sb % nsr;
+    } else {
+        idx_t base = 0;
+        for (cs = 0;; cs++) {
+            if (cs >= nsc) return false;
+            const idx_t cnt = nsr - cs / 2 > 0 ? nsr - cs / 2 : 0;
+            if (sb < base + cnt) break;
+            base += cnt;
+        }
+        rs = cs / 2 + (sb - base);
+    }
+    tr = rs * XB_R + t % XB_R;
+    tc = cs * XB_C + t / XB_R;
+    return tr < mt && tc < nt;
+}
+
 template <int S>
 __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const int8_t *SA, idx_t mpad, const unsigned *EA,
                                               unsigned ta, const int8_t *SB, idx_t npad, const unsigned *EB, unsigned tb,
@@ -197,7 +243,9 @@ __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const i
     __shared__ v4i LA[S][64][4];
     __shared__ v4i LB[S][64][4];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const idx_t r0 = (idx_t)blockIdx.x * 64, c0 = (idx_t)blockIdx.y * 64;
+    idx_t tr, tc;
+    if (!xcd_tile(mpad / 64, npad / 64, lower != 0, tr, tc)) return;
+    const idx_t r0 = tr * 64, c0 = tc * 64;
     if (lower && r0 + 63 < c0) return;
     const int wr = wave & 3, wc = wave >> 2;
     const int fr = lane & 15, fq = lane >> 4;
@@ -205,12 +253,23 @@ __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const i
 #pragma unroll
     for (int d = 0; d < S; d++) acc[d][0] = acc[d][1] = v4i{0, 0, 0, 0};
     for (int kb = 0; kb < nkb; kb++) {
+        // every global load of the chunk is issued before the LDS stores,
+        // so the chunk pays one memory latency instead of one per load
+        constexpr int NQ = S * 256 / 512;
+        v4i va[NQ], vb[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int c = tid + 512 * q, sl = c >> 8, row = (c >> 2) & 63, kq = c & 3;
+            va[q] = *(const v4i *)(SA + ((((idx_t)sl * nkb + kb) * mpad + r0 + row) * KC + kq * 16));
+            vb[q] = *(const v4i *)(SB + ((((idx_t)sl * nkb + kb) * npad + c0 + row) * KC + kq * 16));
+        }
         __syncthreads();
-        for (int c = tid; c < S * 256; c += 512) {
-            const int sl = c >> 8, row = (c >> 2) & 63, kq = c & 3;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int c = tid + 512 * q, sl = c >> 8, row = (c >> 2) & 63, kq = c & 3;
             const int sw = kq ^ ((row >> 2) & 3);
-            LA[sl][row][sw] = *(const v4i *)(SA + ((((idx_t)sl * nkb + kb) * mpad + r0 + row) * KC + kq * 16));
-            LB[sl][row][sw] = *(const v4i *)(SB + ((((idx_t)sl * nkb + kb) * npad + c0 + row) * KC + kq * 16));
+            LA[sl][row][sw] = va[q];
+            LB[sl][row][sw] = vb[q];
         }
         __syncthreads();
         const int ar = wr * 16 + fr;
@@ -269,12 +328,21 @@ __global__ __launch_bounds__(256) void k_gemm32(idx_t m, idx_t n, int nkb, const
 #pragma unroll
     for (int d = 0; d < S; d++) acc[d] = v4i{0, 0, 0, 0};
     for (int kb = 0; kb < nkb; kb++) {
+        constexpr int NQ = S * 128 / 256;
+        v4i va[NQ], vb[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int c = tid + 256 * q, sl = c >> 7, row = (c >> 2) & 31, kq = c & 3;
+            va[q] = *(const v4i *)(SA + ((((idx_t)sl * nkb + kb) * mpad + r0 + row) * KC + kq * 16));
+            vb[q] = *(const v4i *)(SB + ((((idx_t)sl * nkb + kb) * npad + c0 + row) * KC + kq * 16));
+        }
         __syncthreads();
-        for (int c = tid; c < S * 128; c += 256) {
-            const int sl = c >> 7, row = (c >> 2) & 31, kq = c & 3;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int c = tid + 256 * q, sl = c >> 7, row = (c >> 2) & 31, kq = c & 3;
             const int sw = kq ^ ((row >> 2) & 3);
-            LA[sl][row][sw] = *(const v4i *)(SA + ((((idx_t)sl * nkb + kb) * mpad + r0 + row) * KC + kq * 16));
-            LB[sl][row][sw] = *(const v4i *)(SB + ((((idx_t)sl * nkb + kb) * npad + c0 + row) * KC + kq * 16));
+            LA[sl][row][sw] = va[q];
+            LB[sl][row][sw] = vb[q];
         }
         __syncthreads();
         const int ar = wr * 16 + fr, br = wc * 16 + fr;
@@ -342,7 +410,7 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
             m, n, nkb, w.sa, mpad, w.ea, ta, same ? w.sa : w.sb, same ? mpad : npad, same ? w.ea : w.eb, tb, C, ldc,
             alpha, beta, lower ? 1 : 0);
     else
-        k_gemm<S><<<dim3(cdiv(mpad, 64), cdiv(npad, 64)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, ta,
+        k_gemm<S><<<dim3((unsigned)xcd_grid(mpad / 64, npad / 64, lower)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, ta,
                                                                       same ? w.sa : w.sb, same ? mpad : npad,
                                                                       same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
                                                                       lower ? 1 : 0);
