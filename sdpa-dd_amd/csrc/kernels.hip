@@ -630,10 +630,20 @@ __global__ void k_potrf_init(int64_t *info, dd *pub, unsigned long long *ctr) {
     }
 }
 
-template <bool FAST>
+// Digit planes of the panel rows for the Ozaki update that follows (the
+// Ozaki form only): the 16 planes and tagged row exponents of rows
+// [0, below) of the panel, as oz::gemm's slicing would write them.
+struct PanelPlanes {
+    int8_t *planes = nullptr;  // plane s of panel row r: planes[(s rows_pad + r) 64 + k]
+    unsigned *exps = nullptr;
+    idx_t rows_pad = 0;
+    unsigned tag = 0;
+};
+
+template <bool FAST, bool SL>
 __global__ __launch_bounds__(PD_NT) void k_potrf_fused(dd *A, idx_t lda, idx_t K, int nb, idx_t below, int64_t *info,
                                                        dd *rinv, dd *pub, dd *pub_next, unsigned long long *ctr,
-                                                       unsigned long long base, unsigned spin) {
+                                                       unsigned long long base, unsigned spin, PanelPlanes pl) {
     struct DiagLds {
         dd S[PB][PB + 1];
         dd rl[PB];
@@ -878,6 +888,37 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_fused(dd *A, idx_t lda, idx_t K
         const int r = e % FR, j = e / FR;
         if (r < nr) B[(rb + r) + (idx_t)j * lda] = P.Tt[r][j];
     }
+    if (SL) {
+        // 16 lanes per row, 4 consecutive k each; the row's exponent is the
+        // maximum over its 16 lanes (k_slice's value: frexp of max |hi|)
+        const int r = tid >> 4, q = tid & 15;
+        if (r < FR) {
+            dd av[4];
+            double mx = 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int kk = 4 * q + u;
+                av[u] = (r < nr && kk < nb) ? P.Tt[r][kk] : mk(0.0);
+                mx = fmax(mx, fabs(av[u].hi));
+            }
+#pragma unroll
+            for (int o = 8; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 16));
+            int e = 0;
+            if (mx > 0.0) (void)frexp(mx, &e);
+            if (r < nr) {
+                const idx_t lr = rb - r0 + r;
+                if (q == 0) pl.exps[lr] = oz::exp_tagged(pl.tag, e);
+                unsigned w4[16];
+#pragma unroll
+                for (int t = 0; t < 16; t++) w4[t] = 0u;
+#pragma unroll
+                for (int u = 0; u < 4; u++) oz::put_digits<16>(av[u], e, w4, u);
+#pragma unroll
+                for (int t = 0; t < 16; t++)
+                    *(unsigned *)(pl.planes + (((idx_t)t * pl.rows_pad + lr) * 64 + 4 * q)) = w4[t];
+            }
+        }
+    }
 }
 
 // Diagonal block K (nb columns) and the `below` rows under it: one fused
@@ -886,7 +927,7 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_fused(dd *A, idx_t lda, idx_t K
 // parity and ticket base).
 template <bool FAST>
 static void potrf_step(hipStream_t s, idx_t K, int nb, idx_t below, dd *A, idx_t lda, int64_t *d_info,
-                       const PotrfPub &pp, unsigned long long &base, int &launches) {
+                       const PotrfPub &pp, unsigned long long &base, int &launches, const PanelPlanes *pl = nullptr) {
     StatScope st("potrf", s, (double)nb * nb * nb / 3.0 + (double)below * nb * nb);
     if (below <= 0) {
         k_potrf_diag<FAST><<<1, PD_NT, 0, s>>>(A, lda, K, nb, d_info, pp.rinv);
@@ -894,8 +935,12 @@ static void potrf_step(hipStream_t s, idx_t K, int nb, idx_t below, dd *A, idx_t
     }
     const unsigned grid = 1 + cdiv(below, FR);
     const int par = launches & 1;
-    k_potrf_fused<FAST><<<grid, PD_NT, 0, s>>>(A, lda, K, nb, below, d_info, pp.rinv, pp.buf[par], pp.buf[par ^ 1],
-                                               pp.ctr, base, panel_spin());
+    if (pl)
+        k_potrf_fused<FAST, true><<<grid, PD_NT, 0, s>>>(A, lda, K, nb, below, d_info, pp.rinv, pp.buf[par],
+                                                         pp.buf[par ^ 1], pp.ctr, base, panel_spin(), *pl);
+    else
+        k_potrf_fused<FAST, false><<<grid, PD_NT, 0, s>>>(A, lda, K, nb, below, d_info, pp.rinv, pp.buf[par],
+                                                          pp.buf[par ^ 1], pp.ctr, base, panel_spin(), PanelPlanes());
     base += grid;
     launches++;
 }
@@ -951,18 +996,30 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
         for (idx_t K = P; K < P + pw; K += PB) {
             const int nb = (int)std::min<idx_t>(PB, P + pw - K);
             const idx_t below = n - K - nb;
-            if (fast_pivot)
-                potrf_step<true>(s, K, nb, below, A, lda, d_info, pp, base, launches);
-            else
-                potrf_step<false>(s, K, nb, below, A, lda, d_info, pp, base, launches);
             const idx_t rem = P + pw - K - nb;  // panel columns right of this block
+            // with 16 digit planes the panel workgroups also write the
+            // planes of their rows for the in-panel update below
+            PanelPlanes pl;
+            const bool fuse = rem > 0 && S == 16 && nb == PB;
+            if (fuse) {
+                pl.tag = oz::prepare_planes(s, w, below);
+                if (!pl.tag) return -2;
+                pl.planes = w.sa;
+                pl.exps = w.ea;
+                pl.rows_pad = (below + 63) / 64 * 64;
+            }
+            if (fast_pivot)
+                potrf_step<true>(s, K, nb, below, A, lda, d_info, pp, base, launches, fuse ? &pl : nullptr);
+            else
+                potrf_step<false>(s, K, nb, below, A, lda, d_info, pp, base, launches, fuse ? &pl : nullptr);
             if (rem > 0) {
                 // the panel's remaining columns, every row below: an Ozaki
                 // product with k = 64 (on the FP64 VALU this rank-64 update
                 // was the panel's largest cost)
                 const dd *A21 = A + (K + nb) + K * lda;
-                int rc = oz::gemm(s, w, 'N', 'T', below, rem, nb, -1, A21, lda, A21, lda, 1, A + (K + nb) + (K + nb) * lda,
-                                  lda, true, S);
+                dd *C = A + (K + nb) + (K + nb) * lda;
+                int rc = fuse ? oz::gemm_planes(s, w, pl.tag, below, rem, nb, -1, 1, C, lda, true)
+                              : oz::gemm(s, w, 'N', 'T', below, rem, nb, -1, A21, lda, A21, lda, 1, C, lda, true, S);
                 if (rc != 0) return rc;
             }
         }

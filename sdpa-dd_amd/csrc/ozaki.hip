@@ -55,7 +55,6 @@ void Work::release() {
 // The grid matches k_slice: a thread takes 8 k of one row, the eight
 // threads of a row reduce in LDS, and chunks combine with an integer
 // atomicMax on the tagged value (tag << 12) | (e + EXP_BIAS) (order-free).
-constexpr int EXP_BIAS = 2048;
 // Row exponents and slicing: a workgroup covers SL_R rows of one 64-wide k
 // chunk with 8 threads per row (8 k values each).  Small row blocks keep
 // the grid wide for the k = 64 Cholesky updates (one chunk).
@@ -85,12 +84,9 @@ __global__ __launch_bounds__(SL_T) void k_rowexp(idx_t rows, idx_t k, const dd *
         if (mx > 0.0) {
             int e = 0;
             (void)frexp(mx, &e);
-            atomicMax(ex + r, (tag << 12) | (unsigned)(e + EXP_BIAS));
+            atomicMax(ex + r, exp_tagged(tag, e));
         }
     }
-}
-__device__ __forceinline__ int exp_of(unsigned b, unsigned tag) {
-    return (b >> 12) == tag ? (int)(b & 4095u) - EXP_BIAS : 0;
 }
 
 // Digit planes of rows [SL_R bx, SL_R bx + SL_R) over k chunk by: plane s of row r,
@@ -109,10 +105,6 @@ __global__ __launch_bounds__(SL_T) void k_slice(idx_t rows, idx_t k, const dd *M
     const int rl = t % SL_R;
     const idx_t r = (idx_t)blockIdx.x * SL_R + rl;
     unsigned wd[S][2];
-#pragma unroll
-    for (int s = 0; s < S; s++)
-#pragma unroll
-        for (int q = 0; q < 2; q++) wd[s][q] = 0u;
     dd av[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -129,24 +121,19 @@ __global__ __launch_bounds__(SL_T) void k_slice(idx_t rows, idx_t k, const dd *M
 #pragma unroll
         for (int w = 0; w < 8; w++) mx = fmax(mx, part[w][rl]);
         if (mx > 0.0) (void)frexp(mx, &e);
-        if (kq == 0 && r < rows) exps[r] = (tag << 12) | (unsigned)(e + EXP_BIAS);
+        if (kq == 0 && r < rows) exps[r] = exp_tagged(tag, e);
     } else {
         e = r < rows ? exp_of(exps[r], tag) : 0;
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const dd a = av[q];
-        double xh = ldexp(a.hi, -e), xl = ldexp(a.lo, -e);
+    for (int h = 0; h < 2; h++) {
+        unsigned w4[S];
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            const double th = xh * 128.0, tl = xl * 128.0;
-            const double d = trunc(th);
-            const double rh = th - d;
-            const double sm = rh + tl, bb = sm - rh;
-            xl = (rh - (sm - bb)) + (tl - bb);
-            xh = sm;
-            wd[s][q >> 2] |= ((unsigned)((int)d & 0xff)) << (8 * (q & 3));
-        }
+        for (int s = 0; s < S; s++) w4[s] = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; q++) put_digits<S>(av[4 * h + q], e, w4, q);
+#pragma unroll
+        for (int s = 0; s < S; s++) wd[s][h] = w4[s];
     }
     if (r >= rows_pad) return;
 #pragma unroll
@@ -372,6 +359,10 @@ __global__ __launch_bounds__(256) void k_gemm32(idx_t m, idx_t n, int nkb, const
 }
 
 template <int S>
+static void product(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, int nkb, unsigned ta, unsigned tb, bool same,
+                    idx_t mpad, idx_t npad, int alpha, int beta, dd *C, idx_t ldc, bool lower);
+
+template <int S>
 static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *A, idx_t ars, idx_t acs, const dd *B,
                    idx_t brs, idx_t bcs, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
     const int nkb = (int)cdiv(k, KC);
@@ -398,8 +389,15 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
             k_slice<S, false><<<dim3(cdiv(npad, SL_R), nkb), SL_T, 0, s>>>(n, k, B, brs, bcs, w.eb, tb, w.sb, npad, nkb);
         }
     }
-    // timed as family "ozaki" with the product's algorithmic 2mnk dd-flops
-    // (the MFMA work is S(S+1)/2 times that in int8 multiply-adds)
+    product<S>(s, w, m, n, k, nkb, ta, same ? ta : tb, same, mpad, npad, alpha, beta, C, ldc, lower);
+}
+
+// The tile kernel of a product whose planes are in w (B's in w.sa when
+// `same`), timed as family "ozaki" with the product's algorithmic 2mnk
+// dd-flops (the MFMA work is S(S+1)/2 times that in int8 multiply-adds).
+template <int S>
+static void product(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, int nkb, unsigned ta, unsigned tb, bool same,
+                    idx_t mpad, idx_t npad, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
     // 64 x 64 tiles hold a CU each (128 KB of LDS at S = 16): below about
     // two per CU, the 32 x 32 kernel fills the chip better
@@ -414,6 +412,21 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
                                                                       same ? w.sa : w.sb, same ? mpad : npad,
                                                                       same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
                                                                       lower ? 1 : 0);
+}
+
+unsigned prepare_planes(hipStream_t s, Work &w, idx_t m) {
+    if (!w.reserve(roundup(m, 64), roundup(m, 64), KC, 16, 1)) return 0;
+    return w.next_tag(s);
+}
+
+int gemm_planes(hipStream_t s, Work &w, unsigned tag, idx_t m, idx_t n, idx_t k, int alpha, int beta, dd *C, idx_t ldc,
+                bool lower) {
+    if (m < 0 || n < 0 || n > m || k < 1 || k > KC || (alpha != 1 && alpha != -1) || (beta != 0 && beta != 1) || !tag)
+        return -1;
+    if (m == 0 || n == 0) return 0;
+    const idx_t mpad = roundup(m, 64);
+    product<16>(s, w, m, n, k, 1, tag, tag, true, mpad, roundup(n, 64), alpha, beta, C, ldc, lower);
+    return 0;
 }
 
 

@@ -52,4 +52,41 @@ struct Work {
 int gemm(hipStream_t s, Work &w, char transa, char transb, idx_t m, idx_t n, idx_t k, int alpha, const dd *A, idx_t lda,
          const dd *B, idx_t ldb, int beta, dd *C, idx_t ldc, bool lower, int S);
 
+// C = beta*C + alpha*(M M(0:n, :)^T) for an m x k operand M (k <= KC, S =
+// 16) whose digit planes (rows padded to roundup(m, 64)) and exponents,
+// tagged `tag`, are already in w.sa / w.ea -- written by the kernel that
+// produced M.  The caller reserves the workspace (prepare_planes) before
+// that kernel runs.  Same results as gemm('N', 'T', ...) on M.
+int gemm_planes(hipStream_t s, Work &w, unsigned tag, idx_t m, idx_t n, idx_t k, int alpha, int beta, dd *C, idx_t ldc,
+                bool lower);
+// Reserves planes for an m-row operand with k <= KC and S = 16 and returns
+// the tag its producer writes (0 when the workspace cannot grow).
+unsigned prepare_planes(hipStream_t s, Work &w, idx_t m);
+
+// ---- digit splitting, shared by the slicing kernels and the producers ----
+// Row exponents are stored as (tag << 12) | (e + EXP_BIAS).
+constexpr int EXP_BIAS = 2048;
+__host__ __device__ __forceinline__ unsigned exp_tagged(unsigned tag, int e) { return (tag << 12) | (unsigned)(e + EXP_BIAS); }
+__device__ __forceinline__ int exp_of(unsigned b, unsigned tag) {
+    return (b >> 12) == tag ? (int)(b & 4095u) - EXP_BIAS : 0;
+}
+// The S signed 7-bit digits of a 2^-e (|a| 2^-e < 1), most significant
+// first, packed as bytes into word[s] at byte position `pos` (0..3): digit
+// d = trunc(128 x) in [-127, 127], x <- 128 x - d, the remainder carried
+// exactly by a TwoSum.
+template <int S>
+__device__ __forceinline__ void put_digits(dd a, int e, unsigned *word, int pos) {
+    double xh = ldexp(a.hi, -e), xl = ldexp(a.lo, -e);
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const double th = xh * 128.0, tl = xl * 128.0;
+        const double d = trunc(th);
+        const double rh = th - d;
+        const double sm = rh + tl, bb = sm - rh;
+        xl = (rh - (sm - bb)) + (tl - bb);
+        xh = sm;
+        word[s] |= ((unsigned)((int)d & 0xff)) << (8 * pos);
+    }
+}
+
 }  // namespace oz
