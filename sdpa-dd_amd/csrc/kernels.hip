@@ -378,27 +378,33 @@ __device__ __forceinline__ void tri_cm(int e, int nb, int &r, int &c) {
 // Diagonal block of the blocked Cholesky (Rpotrf2 order per element:
 // a_ij -= l_jk * l_ik for k ascending, skipping l_jk == 0; l_kk =
 // sqrt(a_kk); l_ik = (1/l_kk) * a_ik).  One workgroup, one barrier per
-// column: during step k, column k stays unscaled in LDS and every thread
-// that needs l_ik forms (1/l_kk) * a_ik itself (the same product wherever
-// it is formed).  Thread 0 applies column k to a_{k+1,k+1} and forms
-// l_{k+1,k+1} and its reciprocal; lanes 1-63 of wave 0 write column k's
-// scaled values to the matrix; waves 1-15 apply column k to the rest of the
-// trailing triangle.  Each update thread owns a fixed set of elements,
-// enumerated column by column, so the finished columns j <= k are whole
-// waves that skip their slot and the work per step follows the trailing
-// triangle; a thread issues all its LDS reads before the arithmetic.
+// column.  In step k, wave 0 finishes column k+1 on its own: lane i applies
+// column k to a_{i,k+1} (the l_ik it formed in step k-1 is still in its
+// register; l_{k+1,k} comes by readlane), lane k+1 forms the pivot, and
+// every lane scales its element by the broadcast reciprocal -- no LDS round
+// trip on the pivot chain.  Meanwhile waves 1-15 apply the scaled column k
+// (kept in LDS by wave 0 a step earlier) to columns k+2 and beyond, one
+// product and one subtraction per element.  Each update thread owns a fixed
+// set of elements of columns >= 2, enumerated column by column, so the
+// finished columns are whole waves that skip their slot.
 // FAST (the Ozaki form's large factorisations only): the pivot forms
 // 1/l_kk directly from the double reciprocal square root of a_kk and one
 // Newton step in dd, r = y + y (1 - a y^2) / 2 (about 2^-104 relative, the
 // accuracy of QD's square root), and l_kk = a_kk r off the pivot chain;
 // half the latency of a dd square root followed by a dd division.
-template <bool FAST>
-__global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
-    __shared__ dd S[PB][PB + 1];
-    __shared__ dd rl[PB];
-    __shared__ int failed;
-    const int tid = threadIdx.x;
-    if (*info != 0) return;
+// PUB: every reciprocal and scaled column is also published (put_dd) for
+// the panel workgroups of the fused launch.
+struct DiagLds {
+    dd S[PB][PB + 1];
+    dd Lb[2][PB];  // scaled column k at parity k & 1
+    dd rl[PB];
+    int failed;
+};
+static constexpr int PD_OWN2 = ((PB - 2) * (PB - 1) / 2 + PD_UPD - 1) / PD_UPD;  // update elements per thread
+template <bool FAST, bool PUB>
+__device__ __forceinline__ void diag_block(DiagLds &D, dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv,
+                                           dd *pub) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     {
         // all global loads issued before the LDS stores
         constexpr int NL = PB * PB / PD_NT;
@@ -411,91 +417,115 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K,
 #pragma unroll
         for (int q = 0; q < NL; q++) {
             const int e = tid + PD_NT * q, i = e % PB, j = e / PB;
-            if (i >= j && i < nb) S[i][j] = v[q];
+            if (i >= j && i < nb) D.S[i][j] = v[q];
         }
     }
-    // Waves 1.. own the lower-triangle elements round-robin (column-major
-    // enumeration); own[q] packs (i, j) as i | j << 8.  Unused slots hold
-    // (0, 0), which the update skips like every element with j <= k.
-    int own[PD_OWN];
+    // waves 1..: elements (i, j), 2 <= j <= i < nb, round-robin over the
+    // column-major enumeration; own[q] packs i | j << 8, 0 for none (j = 0
+    // never takes an update)
+    int own[PD_OWN2];
     {
-        const int t = tid - 64;
+        const int t = tid - 64, m2 = nb - 2;
 #pragma unroll
-        for (int q = 0; q < PD_OWN; q++) {
+        for (int q = 0; q < PD_OWN2; q++) {
             own[q] = 0;
             const int e = t + PD_UPD * q;
-            if (t >= 0 && e < nb * (nb + 1) / 2) {
+            if (t >= 0 && m2 > 0 && e < m2 * (m2 + 1) / 2) {
                 int ri, ci;
-                tri_cm(e, nb, ri, ci);
-                own[q] = ri | (ci << 8);
+                tri_cm(e, m2, ri, ci);
+                own[q] = (ri + 2) | ((ci + 2) << 8);
             }
         }
     }
+    if (tid == 0) D.failed = 0;
     __syncthreads();
-    auto pivot = [&](int k) {
-        dd a = S[k][k];
-        if (dd_le(a, mk(0.0)) || dd_isnan(a)) {
-            failed = 1;
-            *info = K + k + 1;
-        } else if (FAST) {
-            const double y = rsqrt(a.hi);
-            const dd e = d_sub_dd(1.0, dd_mul(a, dd_sqr_d(y)));
-            double lo;
-            const double hi = two_sum(y, e.hi * (0.5 * y), lo);
-            const dd r = dd{hi, lo};
-            rl[k] = r;
-            A[(K + k) + (K + k) * lda] = dd_mul(a, r);
-        } else {
-            dd l = dd_sqrt(a);
-            A[(K + k) + (K + k) * lda] = l;
-            rl[k] = dd_div(mk(1.0), l);
+    dd *pubr = pub + PB * PB;
+    // wave 0: finish column c (apply column c-1 unless c == 0, factor the
+    // pivot, scale); li carries l_{lane,c-1} in and l_{lane,c} out
+    dd li = mk(0.0);
+    auto finish_col = [&](int c) {
+        const bool mine = lane >= c && lane < nb;
+        dd a = mine ? D.S[lane][c] : mk(0.0);
+        if (c > 0) {
+            const dd lj = readlane_dd(li, c);
+            if (mine && !dd_is_zero(lj)) a = dd_sub(a, dd_mul(lj, li));
+        }
+        int bad = 0;
+        dd r = mk(0.0);
+        if (lane == c) {
+            if (dd_le(a, mk(0.0)) || dd_isnan(a)) {
+                bad = 1;
+            } else if (FAST) {
+                const double y = rsqrt(a.hi);
+                const dd e = d_sub_dd(1.0, dd_mul(a, dd_sqr_d(y)));
+                double lo;
+                const double hi = two_sum(y, e.hi * (0.5 * y), lo);
+                r = dd{hi, lo};
+                A[(K + c) + (K + c) * lda] = dd_mul(a, r);
+            } else {
+                const dd l = dd_sqrt(a);
+                A[(K + c) + (K + c) * lda] = l;
+                r = dd_div(mk(1.0), l);
+            }
+        }
+        if (__builtin_amdgcn_readlane(bad, c)) {
+            if (lane == 0) {
+                D.failed = 1;
+                __hip_atomic_store(info, (int64_t)(K + c + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return;
+        }
+        r = readlane_dd(r, c);
+        if (lane == c) {
+            D.rl[c] = r;
+            if (PUB) put_dd(pubr + c, r);
+        }
+        li = mine && lane > c ? dd_mul(r, a) : mk(0.0);
+        if (mine && lane > c) {
+            A[(K + lane) + (K + c) * lda] = li;
+            D.Lb[c & 1][lane] = li;
+            if (PUB) put_dd(pub + c * PB + lane, li);
         }
     };
-    if (tid == 0) {
-        failed = 0;
-        pivot(0);
-    }
+    if (wave == 0) finish_col(0);
     __syncthreads();
-    for (int k = 0; k < nb; k++) {
-        if (failed) return;
-        const dd r = rl[k];
-        const int own_pivot = (k + 1) | ((k + 1) << 8);
-        if (tid < 64) {
-            if (tid == 0) {
-                if (k + 1 < nb) {
-                    dd l = dd_mul(r, S[k + 1][k]);
-                    if (!dd_is_zero(l)) S[k + 1][k + 1] = dd_sub(S[k + 1][k + 1], dd_mul(l, l));
-                    pivot(k + 1);
-                }
-            } else if (tid > k && tid < nb) {
-                A[(K + tid) + (K + k) * lda] = dd_mul(r, S[tid][k]);
-            }
+    for (int k = 0; k + 1 < nb; k++) {
+        if (D.failed) return;
+        if (wave == 0) {
+            finish_col(k + 1);
         } else {
-            dd sv[PD_OWN], aj[PD_OWN], ai[PD_OWN];
-            bool act[PD_OWN];
+            // column k (scaled) to the owned elements of columns >= k + 2
+            const dd *lk = D.Lb[k & 1];
+            dd sv[PD_OWN2], lj[PD_OWN2], lv[PD_OWN2];
+            bool act[PD_OWN2];
 #pragma unroll
-            for (int q = 0; q < PD_OWN; q++) {
+            for (int q = 0; q < PD_OWN2; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
-                act[q] = j > k && own[q] != own_pivot;
+                act[q] = j >= k + 2;
                 if (act[q]) {
-                    sv[q] = S[i][j];
-                    aj[q] = S[j][k];
-                    ai[q] = S[i][k];
+                    sv[q] = D.S[i][j];
+                    lj[q] = lk[j];
+                    lv[q] = lk[i];
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int q = 0; q < PD_OWN; q++) {
+            for (int q = 0; q < PD_OWN2; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
-                if (act[q]) {
-                    const dd lj = dd_mul(r, aj[q]);
-                    if (!dd_is_zero(lj)) S[i][j] = dd_sub(sv[q], dd_mul(lj, dd_mul(r, ai[q])));
-                }
+                if (act[q] && !dd_is_zero(lj[q])) D.S[i][j] = dd_sub(sv[q], dd_mul(lj[q], lv[q]));
             }
         }
         __syncthreads();
     }
-    for (int k = tid; k < nb; k += PD_NT) rinv[k] = rl[k];
+    if (D.failed) return;
+    for (int k = tid; k < nb; k += PD_NT) rinv[k] = D.rl[k];
+}
+
+template <bool FAST>
+__global__ __launch_bounds__(PD_NT) void k_potrf_diag(dd *A, idx_t lda, idx_t K, int nb, int64_t *info, dd *rinv) {
+    __shared__ DiagLds D;
+    if (*info != 0) return;
+    diag_block<FAST, false>(D, A, lda, K, nb, info, rinv, nullptr);
 }
 
 // B := B * L^{-T} on an nb-column panel (Rtrsm 'R','L','T','N' order with
@@ -644,11 +674,6 @@ template <bool FAST, bool SL>
 __global__ __launch_bounds__(PD_NT) void k_potrf_fused(dd *A, idx_t lda, idx_t K, int nb, idx_t below, int64_t *info,
                                                        dd *rinv, dd *pub, dd *pub_next, unsigned long long *ctr,
                                                        unsigned long long base, unsigned spin, PanelPlanes pl) {
-    struct DiagLds {
-        dd S[PB][PB + 1];
-        dd rl[PB];
-        int failed;
-    };
     struct PanelLds {
         dd Lc[PB * (PB + 1) / 2];
         dd rl[PB];
@@ -667,107 +692,9 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_fused(dd *A, idx_t lda, idx_t K
     if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     dd *pubr = pub + PB * PB;
     if (role == 0) {
-        // ---- the diagonal block: k_potrf_diag's schedule, publishing ----
-        auto &S = sm.d.S;
-        auto &rl = sm.d.rl;
-        {
-            constexpr int NL = PB * PB / PD_NT;
-            dd v[NL];
-#pragma unroll
-            for (int q = 0; q < NL; q++) {
-                const int e = tid + PD_NT * q, i = e % PB, j = e / PB;
-                v[q] = (i >= j && i < nb) ? A[(K + i) + (K + j) * lda] : mk(0.0);
-            }
-#pragma unroll
-            for (int q = 0; q < NL; q++) {
-                const int e = tid + PD_NT * q, i = e % PB, j = e / PB;
-                if (i >= j && i < nb) S[i][j] = v[q];
-            }
-        }
-        int own[PD_OWN];
-        {
-            const int t = tid - 64;
-#pragma unroll
-            for (int q = 0; q < PD_OWN; q++) {
-                own[q] = 0;
-                const int e = t + PD_UPD * q;
-                if (t >= 0 && e < nb * (nb + 1) / 2) {
-                    int ri, ci;
-                    tri_cm(e, nb, ri, ci);
-                    own[q] = ri | (ci << 8);
-                }
-            }
-        }
-        __syncthreads();
-        auto pivot = [&](int k) {
-            dd a = S[k][k];
-            if (dd_le(a, mk(0.0)) || dd_isnan(a)) {
-                sm.d.failed = 1;
-                __hip_atomic_store(info, (int64_t)(K + k + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else if (FAST) {
-                const double y = rsqrt(a.hi);
-                const dd e = d_sub_dd(1.0, dd_mul(a, dd_sqr_d(y)));
-                double lo;
-                const double hi = two_sum(y, e.hi * (0.5 * y), lo);
-                const dd r = dd{hi, lo};
-                rl[k] = r;
-                put_dd(pubr + k, r);
-                A[(K + k) + (K + k) * lda] = dd_mul(a, r);
-            } else {
-                dd l = dd_sqrt(a);
-                A[(K + k) + (K + k) * lda] = l;
-                const dd r = dd_div(mk(1.0), l);
-                rl[k] = r;
-                put_dd(pubr + k, r);
-            }
-        };
-        if (tid == 0) {
-            sm.d.failed = 0;
-            pivot(0);
-        }
-        __syncthreads();
-        for (int k = 0; k < nb; k++) {
-            if (sm.d.failed) return;
-            const dd r = rl[k];
-            const int own_pivot = (k + 1) | ((k + 1) << 8);
-            if (tid < 64) {
-                if (tid == 0) {
-                    if (k + 1 < nb) {
-                        dd l = dd_mul(r, S[k + 1][k]);
-                        if (!dd_is_zero(l)) S[k + 1][k + 1] = dd_sub(S[k + 1][k + 1], dd_mul(l, l));
-                        pivot(k + 1);
-                    }
-                } else if (tid > k && tid < nb) {
-                    const dd l = dd_mul(r, S[tid][k]);
-                    A[(K + tid) + (K + k) * lda] = l;
-                    put_dd(pub + k * PB + tid, l);
-                }
-            } else {
-                dd sv[PD_OWN], aj[PD_OWN], ai[PD_OWN];
-                bool act[PD_OWN];
-#pragma unroll
-                for (int q = 0; q < PD_OWN; q++) {
-                    const int i = own[q] & 0xff, j = own[q] >> 8;
-                    act[q] = j > k && own[q] != own_pivot;
-                    if (act[q]) {
-                        sv[q] = S[i][j];
-                        aj[q] = S[j][k];
-                        ai[q] = S[i][k];
-                    }
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int q = 0; q < PD_OWN; q++) {
-                    const int i = own[q] & 0xff, j = own[q] >> 8;
-                    if (act[q]) {
-                        const dd lj = dd_mul(r, aj[q]);
-                        if (!dd_is_zero(lj)) S[i][j] = dd_sub(sv[q], dd_mul(lj, dd_mul(r, ai[q])));
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        for (int k = tid; k < nb; k += PD_NT) rinv[k] = rl[k];
+        // ---- the diagonal block, publishing as it goes ----
+        diag_block<FAST, true>(sm.d, A, lda, K, nb, info, rinv, pub);
+        if (sm.d.failed) return;
         const dd empty = {__longlong_as_double((long long)BI_EMPTY), __longlong_as_double((long long)BI_EMPTY)};
         for (int e = tid; e < PUB_DD; e += PD_NT) pub_next[e] = empty;
         return;
