@@ -386,7 +386,9 @@ __device__ __forceinline__ void tri_cm(int e, int nb, int &r, int &c) {
 // (kept in LDS by wave 0 a step earlier) to columns k+2 and beyond, one
 // product and one subtraction per element.  Each update thread owns a fixed
 // set of elements of columns >= 2, enumerated column by column, so the
-// finished columns are whole waves that skip their slot.
+// finished columns are whole waves that skip their slot; it keeps them in
+// registers and stores each to LDS once, when wave 0 is about to finish
+// its column.
 // FAST (the Ozaki form's large factorisations only): the pivot forms
 // 1/l_kk directly from the double reciprocal square root of a_kk and one
 // Newton step in dd, r = y + y (1 - a y^2) / 2 (about 2^-104 relative, the
@@ -439,6 +441,10 @@ __device__ __forceinline__ void diag_block(DiagLds &D, dd *A, idx_t lda, idx_t K
     }
     if (tid == 0) D.failed = 0;
     __syncthreads();
+    // the owned elements, in registers from here on
+    dd sv[PD_OWN2];
+#pragma unroll
+    for (int q = 0; q < PD_OWN2; q++) sv[q] = own[q] ? D.S[own[q] & 0xff][own[q] >> 8] : mk(0.0);
     dd *pubr = pub + PB * PB;
     // wave 0: finish column c (apply column c-1 unless c == 0, factor the
     // pivot, scale); li carries l_{lane,c-1} in and l_{lane,c} out
@@ -494,25 +500,26 @@ __device__ __forceinline__ void diag_block(DiagLds &D, dd *A, idx_t lda, idx_t K
         if (wave == 0) {
             finish_col(k + 1);
         } else {
-            // column k (scaled) to the owned elements of columns >= k + 2
+            // column k (scaled) to the owned elements of columns >= k + 2,
+            // held in registers; an element goes to LDS once, when its
+            // column is the one wave 0 finishes next (j = k + 2)
             const dd *lk = D.Lb[k & 1];
-            dd sv[PD_OWN2], lj[PD_OWN2], lv[PD_OWN2];
+            dd lj[PD_OWN2], lv[PD_OWN2];
             bool act[PD_OWN2];
 #pragma unroll
             for (int q = 0; q < PD_OWN2; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
                 act[q] = j >= k + 2;
                 if (act[q]) {
-                    sv[q] = D.S[i][j];
                     lj[q] = lk[j];
                     lv[q] = lk[i];
                 }
             }
-            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < PD_OWN2; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
-                if (act[q] && !dd_is_zero(lj[q])) D.S[i][j] = dd_sub(sv[q], dd_mul(lj[q], lv[q]));
+                if (act[q] && !dd_is_zero(lj[q])) sv[q] = dd_sub(sv[q], dd_mul(lj[q], lv[q]));
+                if (act[q] && j == k + 2) D.S[i][j] = sv[q];
             }
         }
         __syncthreads();
