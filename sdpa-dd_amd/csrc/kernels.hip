@@ -927,20 +927,26 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
     bool helper_pending = false;  // an update on la->hs that later work on s must wait for
     for (idx_t P = 0; P < n; P += OZ_NB) {
         const idx_t pw = std::min<idx_t>(OZ_NB, n - P);
+        // a one-block panel without look-ahead: its trailing update (k = 64)
+        // reads the same planes as an in-panel update would
+        const bool trail64 = pw == PB && !la;
+        unsigned trail_tag = 0;
         for (idx_t K = P; K < P + pw; K += PB) {
             const int nb = (int)std::min<idx_t>(PB, P + pw - K);
             const idx_t below = n - K - nb;
             const idx_t rem = P + pw - K - nb;  // panel columns right of this block
             // with 16 digit planes the panel workgroups also write the
-            // planes of their rows for the in-panel update below
+            // planes of their rows for the update (in-panel or trailing)
+            // that follows
             PanelPlanes pl;
-            const bool fuse = rem > 0 && S == 16 && nb == PB;
+            const bool fuse = (rem > 0 || trail64) && S == 16 && nb == PB && below > 0;
             if (fuse) {
                 pl.tag = oz::prepare_planes(s, w, below);
                 if (!pl.tag) return -2;
                 pl.planes = w.sa;
                 pl.exps = w.ea;
                 pl.rows_pad = (below + 63) / 64 * 64;
+                if (trail64) trail_tag = pl.tag;
             }
             if (fast_pivot)
                 potrf_step<true>(s, K, nb, below, A, lda, d_info, pp, base, launches, fuse ? &pl : nullptr);
@@ -961,7 +967,9 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
         if (below <= 0) break;
         const dd *L21 = A + Q + P * lda;
         if (!la) {
-            int rc = oz::gemm(s, w, 'N', 'T', below, below, pw, -1, L21, lda, L21, lda, 1, A + Q * (1 + lda), lda, true, S);
+            int rc = trail_tag ? oz::gemm_planes(s, w, trail_tag, below, below, pw, -1, 1, A + Q * (1 + lda), lda, true)
+                               : oz::gemm(s, w, 'N', 'T', below, below, pw, -1, L21, lda, L21, lda, 1, A + Q * (1 + lda),
+                                          lda, true, S);
             if (rc != 0) return rc;
             continue;
         }
