@@ -61,8 +61,8 @@ FAMILIES = ("gemm", "ozaki", "potrf", "panel", "trsm", "trmm", "trsv", "trsv_pre
 MODEL = {
     "gemm": ("mfma", "sdk::k_gemm (dd GEMM on the FP64 VALU)", "fp64"),
     "ozaki": ("mfma", "oz::k_gemm / oz::k_gemm32 (Ozaki dd product, INT8 MFMA v_mfma_i32_16x16x64_i8)", "int8"),
-    "potrf": ("mfma", "sdk::k_potrf_diag (64-column diagonal Cholesky block on the FP64 VALU; a chain of 64 "
-              "square roots and reciprocals)", "fp64"),
+    "potrf": ("mfma", "sdk::k_potrf_fused (a 64-column diagonal Cholesky block and the panel rows below it in one "
+              "launch on the FP64 VALU; the diagonal block's chain of 64 pivots sets its time)", "fp64"),
     "panel": ("mfma", "sdk::k_panel_rltn (Cholesky panel solve, Rtrsm R,L,T,N on the FP64 VALU)", "fp64"),
     "trsm": ("mfma", "sdk::k_trsm_llnn_diag + GEMM (Rtrsm L,L,N,N)", "fp64"),
     "trmm": ("mfma", "sdk::k_trmm_lltn (Rtrmm L,L,T,N)", "fp64"),
@@ -71,15 +71,15 @@ MODEL = {
     "mv": ("hbm", "sdk::k_trmv_* / k_gemv_n / k_mvf_* (Lanczos mat-vecs)", "bytes8"),
     "dot": ("hbm", "sdk::k_dot_* (dot products)", "bytes16"),
 }
-# HBM bytes per launch of a family's dominant kernel on the theta workload,
+# HBM bytes per launch of a kernel family on the theta workload (read +
+# written, averaged over the family's launches in the form it dominates),
 # from two rocprofv3 --pmc passes (FETCH_SIZE and WRITE_SIZE, scaled by the
 # factors a 256 MiB copy needs to read its true size).
 TRAFFIC = {
-    "theta": {"ozaki": 70.49e6 + 23.54e6, "gemm": 164.37e6 + 89.80e6, "potrf": 48.5e3 + 34.0e3,
-              "panel": 6.10e6 + 2.60e6, "trsv": 172.66e6 + 5.12e6},
-    "maxcut": {"ozaki": 30.52e6 + 2.83e6, "gemm": 38.86e6 + 2.45e6},
+    "theta": {"ozaki": 35.23e6 + 12.58e6, "gemm": 78.77e6 + 42.21e6, "potrf": 3.02e6 + 3.88e6,
+              "trsv": 176.91e6 + 0.36e6},
 }
-TRAFFIC_SOURCE = "profiles/hbm_traffic_theta300_maxcut800.md"
+TRAFFIC_SOURCE = "profiles/hbm_traffic_theta300.md"
 
 
 def workload_files(tmp, workload, gap):
