@@ -303,25 +303,33 @@ void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) 
         (void)hipMemcpyAsync(c.dot_work + 2, c.h, sizeof(dd) * 2, hipMemcpyHostToDevice, c.s);
         c.pending = 0;
     }
-    for (;;) {
-        int launched = 0;
-        for (int i = 0; i < cnt; i++) {
-            Chain &c = pool.chains[i];
-            if (c.done) continue;
-            if (!chain_active(c)) {
-                c.done = true;
-                continue;
-            }
-            chain_enqueue(c, T, chain_batch(c));
-            launched++;
+    // a chain's next batch is enqueued as soon as its own checkpoint is
+    // processed, so the device runs one chain while the host works on the
+    // other's tridiagonal eigen solve
+    auto launch = [&](Chain &c) {
+        if (c.done) return false;
+        if (!chain_active(c)) {
+            c.done = true;
+            return false;
         }
-        if (!launched) break;
+        const int cnt_steps = chain_batch(c);
+        if (cnt_steps == 0) {  // no room left in the step history
+            c.done = true;
+            return false;
+        }
+        chain_enqueue(c, T, cnt_steps);
+        return true;
+    };
+    int running = 0;
+    for (int i = 0; i < cnt; i++) running += launch(pool.chains[i]);
+    while (running > 0) {
         for (int i = 0; i < cnt; i++) {
             Chain &c = pool.chains[i];
             if (c.done || c.pending == 0) continue;
             (void)hipStreamSynchronize(c.s);
             const int first = c.k, np = c.pending;
             c.pending = 0;
+            running--;
             for (int j = 0; j < np; j++) {
                 // within a batch the convergence test cannot change (min and
                 // error move only at the eigen solve that closes the batch);
@@ -334,6 +342,7 @@ void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) 
                 chain_finish_step(c, c.hh + 2 * (size_t)(first + j));
                 if (c.done) break;
             }
+            running += launch(c);
         }
     }
     for (int i = 0; i < cnt; i++) {
