@@ -6,6 +6,7 @@
 // scalars and block diagonals cross the PCIe boundary during the loop.
 #include <chrono>
 #include <cmath>
+#include <deque>
 #include <cstdio>
 
 #include "context.hpp"
@@ -137,8 +138,18 @@ struct Chain {
     dd *dot_work = nullptr;    // device scalars (beta, 1/beta at [2], [3]) and the step history
     dd *hist = nullptr;        // device: {alpha, r.r} per step
     dd *hh = nullptr;          // pinned host copy of hist
-    int pending = 0;           // steps enqueued and not yet read
     unsigned *counter = nullptr;
+    // batches of steps on the device, oldest first; a batch ends at the
+    // step that runs the tridiagonal eigen solve
+    struct Batch {
+        int first, cnt;
+        hipEvent_t ev;
+    };
+    std::deque<Batch> inflight;
+    int ek = 0, ekk = 0;        // the next step to enqueue and its kk
+    hipEvent_t evs[3] = {};     // ring of completion events (two in flight + the last one used)
+    int evi = 0;
+    hipEvent_t last_used = nullptr;
     hipStream_t s = nullptr;
     dd *h = nullptr;           // pinned host copy of scal
     // host state
@@ -156,10 +167,17 @@ struct ChainPool {
     std::vector<Chain> chains;
     std::vector<void *> dmem, hmem;
     std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;
     ~ChainPool() {
+        drain();
         for (void *p : dmem) (void)hipFree(p);
         for (void *p : hmem) (void)hipHostFree(p);
         for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+    // waits for the steps a chain ran past its end (they still read L and M)
+    void drain() {
+        for (hipStream_t s : streams) (void)hipStreamSynchronize(s);
     }
     // make sure chains [0, cnt) exist with vectors of length >= n
     bool ensure(int cnt, int n) {
@@ -183,6 +201,10 @@ struct ChainPool {
             c.hist = c.dot_work + 8;
             if (hipHostMalloc((void **)&c.hh, sizeof(dd) * 2 * HIST_CAP) != hipSuccess) return false;
             hmem.push_back(c.hh);
+            for (hipEvent_t &e : c.evs) {
+                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+                events.push_back(e);
+            }
             hipStream_t s;
             if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return false;
             streams.push_back(s);
@@ -203,10 +225,11 @@ bool chain_active(Chain &c) {
             abs(c.error * c.beta) > (1.0e-2) * abs(c.min) + (1.0e-4));
 }
 
-// Enqueue steps k .. k + cnt - 1: each forms q from the device scalars, the
-// three mat-vecs, then alpha, r and r.r into hist[k + j] and the next
-// scalars (Lal::getMinEigen's loop body, sdpa_linear.cpp:64-82).
-void chain_enqueue(Chain &c, int T, int cnt) {
+// Enqueue steps first .. first + cnt - 1: each forms q from the device
+// scalars, the three mat-vecs, then alpha, r and r.r into hist[first + j]
+// and the next scalars (Lal::getMinEigen's loop body, sdpa_linear.cpp:64-82),
+// then the copy of their history to the host and a completion event.
+void chain_enqueue(Chain &c, int T, int first, int cnt) {
     hipStream_t s = c.s;
     const int nDim = c.n;
     dd *sc = c.dot_work;
@@ -223,20 +246,24 @@ void chain_enqueue(Chain &c, int T, int cnt) {
         }
         // alpha = q.w; r = w - alpha q - beta qold; r.r; next beta, 1/beta
         if (c.fast)
-            sdk::lanczos_tail_fast(s, nDim, c.q, c.qold, c.w, c.r, sc, c.hist + 2 * (size_t)(c.k + j));
+            sdk::lanczos_tail_fast(s, nDim, c.q, c.qold, c.w, c.r, sc, c.hist + 2 * (size_t)(first + j));
         else
-            sdk::lanczos_tail_dev(s, nDim, T, c.q, c.qold, c.w, c.r, sc, c.hist + 2 * (size_t)(c.k + j));
+            sdk::lanczos_tail_dev(s, nDim, T, c.q, c.qold, c.w, c.r, sc, c.hist + 2 * (size_t)(first + j));
     }
-    (void)hipMemcpyAsync(c.hh + 2 * (size_t)c.k, c.hist + 2 * (size_t)c.k, sizeof(dd) * 2 * cnt, hipMemcpyDeviceToHost,
-                         s);
-    c.pending = cnt;
+    (void)hipMemcpyAsync(c.hh + 2 * (size_t)first, c.hist + 2 * (size_t)first, sizeof(dd) * 2 * cnt,
+                         hipMemcpyDeviceToHost, s);
+    hipEvent_t ev = c.evs[c.evi];
+    c.evi = (c.evi + 1) % 3;
+    (void)hipEventRecord(ev, s);
+    c.inflight.push_back({first, cnt, ev});
 }
 
-// Steps from c.k on up to and including the next one that runs the
-// tridiagonal eigen solve (whose outcome may end the chain).
-int chain_batch(const Chain &c) {
+// Steps from k on (kk steps since the last eigen solve) up to and including
+// the next one that runs the tridiagonal eigen solve (whose outcome may end
+// the chain).  The schedule depends only on k and kk.
+int chain_batch(const Chain &c, int k, int kk) {
     const int nDim = c.n;
-    int k = c.k, kk = c.kk, cnt = 0;
+    int cnt = 0;
     while (k < nDim && k < sqrt((ddr)nDim) + 10 && k < HIST_CAP) {
         cnt++;
         if (kk >= sqrt((ddr)k) || k == nDim - 1 || k > sqrt((ddr)nDim + 9)) break;
@@ -301,55 +328,57 @@ void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) 
         c.h[0] = c.beta.v;
         c.h[1] = (ddr(1.0) / c.beta).v;
         (void)hipMemcpyAsync(c.dot_work + 2, c.h, sizeof(dd) * 2, hipMemcpyHostToDevice, c.s);
-        c.pending = 0;
     }
-    // a chain's next batch is enqueued as soon as its own checkpoint is
-    // processed, so the device runs one chain while the host works on the
-    // other's tridiagonal eigen solve
-    auto launch = [&](Chain &c) {
-        if (c.done) return false;
-        if (!chain_active(c)) {
-            c.done = true;
-            return false;
-        }
-        const int cnt_steps = chain_batch(c);
-        if (cnt_steps == 0) {  // no room left in the step history
-            c.done = true;
-            return false;
-        }
-        chain_enqueue(c, T, cnt_steps);
+    // Each chain keeps its next batch running while the host processes the
+    // previous batch's eigen solve: the batch schedule depends only on the
+    // step counters and the device forms its own scalars, so only the
+    // decision to stop waits for the host.  Steps past that decision are
+    // dropped (their results are never read, and nothing waits for them).
+    auto enqueue_next = [&](Chain &c) {
+        const int n_steps = chain_batch(c, c.ek, c.ekk);
+        if (n_steps == 0) return false;
+        chain_enqueue(c, T, c.ek, n_steps);
+        c.ek += n_steps;
+        c.ekk = 1;  // a batch ends at an eigen solve, after which kk restarts
         return true;
     };
-    int running = 0;
-    for (int i = 0; i < cnt; i++) running += launch(pool.chains[i]);
-    while (running > 0) {
+    for (int i = 0; i < cnt; i++) {
+        Chain &c = pool.chains[i];
+        c.inflight.clear();
+        c.ek = c.ekk = 0;
+        c.last_used = nullptr;
+        if (enqueue_next(c)) enqueue_next(c);
+    }
+    for (;;) {
+        bool any = false;
         for (int i = 0; i < cnt; i++) {
             Chain &c = pool.chains[i];
-            if (c.done || c.pending == 0) continue;
-            (void)hipStreamSynchronize(c.s);
-            const int first = c.k, np = c.pending;
-            c.pending = 0;
-            running--;
-            for (int j = 0; j < np; j++) {
+            if (c.done || c.inflight.empty()) continue;
+            any = true;
+            const Chain::Batch b = c.inflight.front();
+            c.inflight.pop_front();
+            (void)hipEventSynchronize(b.ev);
+            c.last_used = b.ev;
+            for (int j = 0; j < b.cnt; j++) {
                 // within a batch the convergence test cannot change (min and
                 // error move only at the eigen solve that closes the batch);
                 // beta can end the chain early, and the later steps the
                 // device already ran are then dropped
-                if (j > 0 && !chain_active(c)) {
+                if (!chain_active(c)) {
                     c.done = true;
                     break;
                 }
-                chain_finish_step(c, c.hh + 2 * (size_t)(first + j));
+                chain_finish_step(c, c.hh + 2 * (size_t)(b.first + j));
                 if (c.done) break;
             }
-            running += launch(c);
+            if (!c.done && c.inflight.size() < 2) enqueue_next(c);
         }
+        if (!any) break;
     }
     for (int i = 0; i < cnt; i++) {
         Chain &c = pool.chains[i];
         c.result = c.min - abs(c.error * c.beta);
-        (void)hipEventRecord(ready, c.s);
-        (void)hipStreamWaitEvent(parent, ready, 0);
+        if (c.last_used) (void)hipStreamWaitEvent(parent, c.last_used, 0);
     }
     (void)hipEventDestroy(ready);
 }
@@ -369,6 +398,7 @@ class Solver {
            const sps::Device *sp)
         : ctx(c), hp(h), dp(d), prm(p), init(ip), spd(sp) {}
     ~Solver() {
+        pool.drain();
         if (s2) (void)hipStreamSynchronize(s2);
         if (s3) (void)hipStreamSynchronize(s3);
         for (void *q : mem) (void)hipFree(q);
