@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,9 @@ namespace sdk {
 // names are string literals; entries are found by name in a short vector.
 // An optional filter (comma-separated names) limits the timed families, so
 // that a timed run pays for the events of the kernel it measures only.
+// Launches may come from several host threads (the Lanczos chains): the
+// shared tables are guarded by a mutex, and each thread keeps its own stack
+// of open scopes.
 namespace {
 struct StatEntry {
     std::string name;
@@ -37,7 +41,8 @@ bool g_stats_on = false;
 std::string g_filter;  // empty: every family
 std::vector<StatEntry> g_stats;
 std::vector<hipEvent_t> g_event_pool;
-std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> g_open;
+thread_local std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> g_open;
+std::mutex g_stats_mu;
 hipEvent_t get_event() {
     if (!g_event_pool.empty()) {
         hipEvent_t e = g_event_pool.back();
@@ -81,10 +86,12 @@ bool selected(const char *family) {
 void stats_enable(bool on) { g_stats_on = on; }
 void stats_filter(const char *families) { g_filter = families ? families : ""; }
 void stats_reset() {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
     for (auto &st : g_stats) resolve(st);
     g_stats.clear();
 }
 bool stats_get(const char *family, int64_t *launches, double *ms, double *ddflops) {
+    std::lock_guard<std::mutex> lk(g_stats_mu);
     int i = find_entry(family);
     if (i < 0) {
         *launches = 0;
@@ -101,6 +108,7 @@ bool stats_get(const char *family, int64_t *launches, double *ms, double *ddflop
 StatScope::StatScope(const char *family, hipStream_t stream, double ddflops) {
     if (!g_stats_on || !selected(family)) return;
     s = stream;
+    std::lock_guard<std::mutex> lk(g_stats_mu);
     int i = find_entry(family);
     if (i < 0) {
         g_stats.push_back(StatEntry());
@@ -118,6 +126,7 @@ StatScope::~StatScope() {
     if (slot < 0) return;
     auto pr = g_open[slot];
     (void)hipEventRecord(pr.second.second, s);
+    std::lock_guard<std::mutex> lk(g_stats_mu);
     g_stats[pr.first].pending.push_back(pr.second);
     g_open.pop_back();
 }
@@ -2372,9 +2381,8 @@ void gemv_n(hipStream_t s, idx_t m, idx_t n, dd alpha, const dd *A, idx_t lda, c
 // columns per step (eight 128-byte segments).  kind 0: y = alpha A x +
 // beta y; kind 2: y = L x with L lower triangular (columns <= the row).
 constexpr int MV_R = 8, MV_G = 1024 / MV_R;
-__global__ __launch_bounds__(1024) void k_mvf_n(int kind, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *x,
-                                                dd alpha, int ak, dd beta, int bk, dd *y) {
-    __shared__ dd part[MV_G][MV_R];
+__device__ __forceinline__ void mvf_rows(int kind, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *x, dd alpha,
+                                         int ak, dd beta, int bk, dd *y, dd (*part)[MV_R]) {
     const int t = threadIdx.x, rl = t % MV_R, g = t / MV_R;
     const idx_t r = (idx_t)blockIdx.x * MV_R + rl;
     idx_t c0 = n * g / MV_G, c1 = n * (g + 1) / MV_G;
@@ -2402,6 +2410,11 @@ __global__ __launch_bounds__(1024) void k_mvf_n(int kind, idx_t m, idx_t n, cons
         }
         y[r] = v;
     }
+}
+__global__ __launch_bounds__(1024) void k_mvf_n(int kind, idx_t m, idx_t n, const dd *A, idx_t lda, const dd *x,
+                                                dd alpha, int ak, dd beta, int bk, dd *y) {
+    __shared__ dd part[MV_G][MV_R];
+    mvf_rows(kind, m, n, A, lda, x, alpha, ak, beta, bk, y, part);
 }
 // 'T'-type (w_j = sum_{i >= j} L_ij x_i): one wave per output, lanes over i,
 // then a shuffle tree.
@@ -2891,14 +2904,25 @@ __device__ __forceinline__ dd block_tree_sum(dd v, dd *sm) {
     __syncthreads();
     return r;
 }
-__global__ __launch_bounds__(256) void k_lanczos_tail_fast(idx_t n, const dd *q, const dd *qold, const dd *w, dd *r,
-                                                           dd *out, dd *sc) {
-    __shared__ dd sm[256];
+// The tail's body on threads 0..255 of a workgroup of at least 256 threads
+// (the others only take part in the barriers); sm: 256 dd of LDS.
+__device__ __forceinline__ void lanczos_tail_fast_body(idx_t n, const dd *q, const dd *qold, const dd *w, dd *r, dd *out,
+                                                       dd *sc, dd *sm) {
+    const int t = threadIdx.x;
+    const bool tl = t < 256;
     const dd nbeta = dd_neg(sc[2]);
-    const idx_t st = n * threadIdx.x / 256, en = n * (threadIdx.x + 1) / 256;
+    const idx_t st = tl ? n * t / 256 : 0, en = tl ? n * (t + 1) / 256 : 0;
     dd acc = mk(0.0);
     for (idx_t i = st; i < en; i++) acc = dd_add(acc, dd_mul(q[i], w[i]));
-    const dd alpha = block_tree_sum(acc, sm);
+    if (tl) sm[t] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int h = 128; h >= 1; h >>= 1) {
+        if (t < h) sm[t] = dd_add(sm[t], sm[t + h]);
+        __syncthreads();
+    }
+    const dd alpha = sm[0];
+    __syncthreads();
     const dd na = dd_neg(alpha);
     acc = mk(0.0);
     for (idx_t i = st; i < en; i++) {
@@ -2906,8 +2930,15 @@ __global__ __launch_bounds__(256) void k_lanczos_tail_fast(idx_t n, const dd *q,
         r[i] = ri;
         acc = dd_add(acc, dd_mul(ri, ri));
     }
-    const dd rr = block_tree_sum(acc, sm);
-    if (threadIdx.x == 0) {
+    if (tl) sm[t] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int h = 128; h >= 1; h >>= 1) {
+        if (t < h) sm[t] = dd_add(sm[t], sm[t + h]);
+        __syncthreads();
+    }
+    if (t == 0) {
+        const dd rr = sm[0];
         out[0] = alpha;
         out[1] = rr;
         const dd beta = dd_sqrt(rr);
@@ -2915,9 +2946,76 @@ __global__ __launch_bounds__(256) void k_lanczos_tail_fast(idx_t n, const dd *q,
         sc[3] = dd_div(mk(1.0), beta);
     }
 }
+__global__ __launch_bounds__(256) void k_lanczos_tail_fast(idx_t n, const dd *q, const dd *qold, const dd *w, dd *r,
+                                                           dd *out, dd *sc) {
+    __shared__ dd sm[256];
+    lanczos_tail_fast_body(n, q, qold, w, r, out, sc, sm);
+}
 void lanczos_tail_fast(hipStream_t s, idx_t n, const dd *q, const dd *qold, const dd *w, dd *r, dd *sc, dd *out) {
     StatScope st("dot", s, 6.0 * (double)n);
     k_lanczos_tail_fast<<<1, 256, 0, s>>>(n, q, qold, w, r, out, sc);
+}
+
+// A Lanczos step of the Ozaki form in three launches, with the same
+// operations as lanczos_q_dev, trmv_lower_fast('T'), gemv_n_fast,
+// trmv_lower_fast('N') and lanczos_tail_fast in sequence:
+// k_mvf_lt_q: workgroup b forms q_i = sc[3] r_i for i >= 4b in LDS (the
+// value lanczos_q_dev stores) and moves q to qold and stores the new q for
+// its own four indices, then computes w_j = sum_{i >= j} L_ij q_i as
+// k_mvf_lt does.  n <= LZ_MAXN.
+constexpr idx_t LZ_MAXN = 4096;
+__global__ __launch_bounds__(256) void k_mvf_lt_q(idx_t n, const dd *L, idx_t lda, const dd *r, const dd *sc, dd *q,
+                                                  dd *qold, dd *w) {
+    extern __shared__ dd xq[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const idx_t j0 = (idx_t)blockIdx.x * 4;
+    const dd v = sc[3];
+    for (idx_t i = j0 + threadIdx.x; i < n; i += 256) xq[i - j0] = dd_mul(v, r[i]);
+    if (threadIdx.x < 4 && j0 + threadIdx.x < n) {
+        const idx_t i = j0 + threadIdx.x;
+        qold[i] = q[i];
+        q[i] = dd_mul(v, r[i]);
+    }
+    __syncthreads();
+    const idx_t j = j0 + wave;
+    if (j >= n) return;
+    dd acc = mk(0.0);
+    for (idx_t i = j + lane; i < n; i += 64) acc = dd_add(acc, dd_mul(L[i + j * lda], xq[i - j0]));
+    acc = wave_sum_dd(acc);
+    if (lane == 0) w[j] = acc;
+}
+// k_mvf_ln_tail: w = L x as k_mvf_n (kind 2); the last workgroup to finish
+// (ticket on ctr, left 0) then runs lanczos_tail_fast's body.  No
+// workgroup waits for another.
+__global__ __launch_bounds__(1024) void k_mvf_ln_tail(idx_t n, const dd *L, idx_t lda, const dd *x, const dd *q,
+                                                      const dd *qold, dd *w, dd *r, dd *out, dd *sc, unsigned *ctr) {
+    __shared__ dd part[MV_G][MV_R];
+    __shared__ unsigned ticket;
+    mvf_rows(2, n, n, L, lda, x, mk(1.0), 1, mk(0.0), 0, w, part);
+    // w's stores come from wave 0 only (mvf_rows): its thread 0 publishes
+    // them with the ticket
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        ticket = atomicAdd(ctr, 1u);
+    }
+    __syncthreads();
+    if (ticket != gridDim.x - 1) return;
+    __threadfence();
+    lanczos_tail_fast_body(n, q, qold, w, r, out, sc, &part[0][0]);
+    if (threadIdx.x == 0) *ctr = 0u;
+}
+bool lanczos_step_fast(hipStream_t s, idx_t n, const dd *L, const dd *M, dd *r, dd *q, dd *qold, dd *w, dd *tmp,
+                       dd *sc, dd *out, unsigned *ctr) {
+    if (n <= 0 || n > LZ_MAXN) return false;
+    {
+        StatScope st("mv", s, (double)n * n);
+        k_mvf_lt_q<<<cdiv(n, 4), 256, sizeof(dd) * n, s>>>(n, L, n, r, sc, q, qold, w);
+    }
+    gemv_n_fast(s, n, n, mk(1.0), M, n, w, mk(0.0), tmp);
+    StatScope st("mv", s, (double)n * n + 6.0 * (double)n);
+    k_mvf_ln_tail<<<cdiv(n, MV_R), 1024, 0, s>>>(n, L, n, tmp, q, qold, w, r, out, sc, ctr);
+    return true;
 }
 
 // Max of |x_i|: per-thread running max, tree merge, last workgroup merges.

@@ -162,6 +162,12 @@ void lanczos_q_dev(hipStream_t s, idx_t n, const dd *sc, const dd *r, dd *q, dd 
 void lanczos_tail_dev(hipStream_t s, idx_t n, int T, const dd *q, const dd *qold, const dd *w, dd *r, dd *sc, dd *out);
 // The same step with its two dots in a fixed split order (the Ozaki form).
 void lanczos_tail_fast(hipStream_t s, idx_t n, const dd *q, const dd *qold, const dd *w, dd *r, dd *sc, dd *out);
+// The whole step of the Ozaki form (lanczos_q_dev, the three fast mat-vecs
+// with L and M of leading dimension n, lanczos_tail_fast) in three launches
+// with the same results; ctr: a zeroed device counter, left zeroed.
+// Returns false (nothing enqueued) for n > 4096.
+bool lanczos_step_fast(hipStream_t s, idx_t n, const dd *L, const dd *M, dd *r, dd *q, dd *qold, dd *w, dd *tmp,
+                       dd *sc, dd *out, unsigned *ctr);
 // Batched GEMM over pointer arrays (device arrays of batch pointers):
 // C[b] := A[b]*B[b] (alpha = 1, beta = 0), order as gemm().
 void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx_t lda, const dd *const *B, idx_t ldb,

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <thread>
 #include <cstdio>
 
 #include "context.hpp"
@@ -234,6 +235,10 @@ void chain_enqueue(Chain &c, int T, int first, int cnt) {
     const int nDim = c.n;
     dd *sc = c.dot_work;
     for (int j = 0; j < cnt; j++) {
+        // the Ozaki form: the same step in three launches
+        if (c.fast && sdk::lanczos_step_fast(s, nDim, c.L, c.M, c.r, c.q, c.qold, c.w, c.tmp, sc,
+                                             c.hist + 2 * (size_t)(first + j), c.counter))
+            continue;
         sdk::lanczos_q_dev(s, nDim, sc, c.r, c.q, c.qold);  // qold = q; q = r / beta
         if (c.fast) {
             sdk::trmv_lower_fast(s, 'T', nDim, c.L, nDim, c.q, c.w);
@@ -349,32 +354,49 @@ void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) 
         c.last_used = nullptr;
         if (enqueue_next(c)) enqueue_next(c);
     }
-    for (;;) {
-        bool any = false;
-        for (int i = 0; i < cnt; i++) {
-            Chain &c = pool.chains[i];
-            if (c.done || c.inflight.empty()) continue;
-            any = true;
-            const Chain::Batch b = c.inflight.front();
-            c.inflight.pop_front();
-            (void)hipEventSynchronize(b.ev);
-            c.last_used = b.ev;
-            for (int j = 0; j < b.cnt; j++) {
-                // within a batch the convergence test cannot change (min and
-                // error move only at the eigen solve that closes the batch);
-                // beta can end the chain early, and the later steps the
-                // device already ran are then dropped
-                if (!chain_active(c)) {
-                    c.done = true;
-                    break;
+    // The host work of a chain (reading its batches, the tridiagonal eigen
+    // solves, enqueueing) runs on one of up to LZ_THREADS host threads, the
+    // chains dealt round-robin; each thread visits its chains in turn.
+    auto drive = [&](int i0, int stride) {
+        for (;;) {
+            bool any = false;
+            for (int i = i0; i < cnt; i += stride) {
+                Chain &c = pool.chains[i];
+                if (c.done || c.inflight.empty()) continue;
+                any = true;
+                const Chain::Batch b = c.inflight.front();
+                c.inflight.pop_front();
+                (void)hipEventSynchronize(b.ev);
+                c.last_used = b.ev;
+                for (int j = 0; j < b.cnt; j++) {
+                    // within a batch the convergence test cannot change (min
+                    // and error move only at the eigen solve that closes the
+                    // batch); beta can end the chain early, and the later
+                    // steps the device already ran are then dropped
+                    if (!chain_active(c)) {
+                        c.done = true;
+                        break;
+                    }
+                    chain_finish_step(c, c.hh + 2 * (size_t)(b.first + j));
+                    if (c.done) break;
                 }
-                chain_finish_step(c, c.hh + 2 * (size_t)(b.first + j));
-                if (c.done) break;
+                if (!c.done && c.inflight.size() < 2) enqueue_next(c);
             }
-            if (!c.done && c.inflight.size() < 2) enqueue_next(c);
+            if (!any) break;
         }
-        if (!any) break;
-    }
+    };
+    constexpr int LZ_THREADS = 4;
+    const int nth = cnt < LZ_THREADS ? cnt : LZ_THREADS;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::vector<std::thread> helpers;
+    for (int t = 1; t < nth; t++)
+        helpers.emplace_back([&, t] {
+            (void)hipSetDevice(dev);  // the current device is per thread
+            drive(t, nth);
+        });
+    drive(0, nth);
+    for (std::thread &h : helpers) h.join();
     for (int i = 0; i < cnt; i++) {
         Chain &c = pool.chains[i];
         c.result = c.min - abs(c.error * c.beta);
