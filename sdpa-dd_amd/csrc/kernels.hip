@@ -458,6 +458,7 @@ __device__ __forceinline__ void diag_block(DiagLds &D, dd *A, idx_t lda, idx_t K
     // wave 0: finish column c (apply column c-1 unless c == 0, factor the
     // pivot, scale); li carries l_{lane,c-1} in and l_{lane,c} out
     dd li = mk(0.0);
+    bool failed0 = false;  // wave 0: a pivot failed (D.failed is set too)
     auto finish_col = [&](int c) {
         const bool mine = lane >= c && lane < nb;
         dd a = mine ? D.S[lane][c] : mk(0.0);
@@ -484,6 +485,7 @@ __device__ __forceinline__ void diag_block(DiagLds &D, dd *A, idx_t lda, idx_t K
             }
         }
         if (__builtin_amdgcn_readlane(bad, c)) {
+            failed0 = true;
             if (lane == 0) {
                 D.failed = 1;
                 __hip_atomic_store(info, (int64_t)(K + c + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -504,31 +506,49 @@ __device__ __forceinline__ void diag_block(DiagLds &D, dd *A, idx_t lda, idx_t K
     };
     if (wave == 0) finish_col(0);
     __syncthreads();
+    // The largest column among a wave's q-th elements (wave-uniform): the
+    // wave skips its q-th updates once that column is finished.  The q = 0
+    // and q = 1 updates run branch-free in one block while both are live,
+    // so their dependent chains interleave; q = 2 holds only a few elements.
+    static_assert(PD_OWN2 == 3, "update element layout");
+    int jmax[PD_OWN2];
+#pragma unroll
+    for (int q = 0; q < PD_OWN2; q++) {
+        int m = own[q] >> 8;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+        jmax[q] = __builtin_amdgcn_readfirstlane(m);
+    }
+    // the loop always runs to the end, so every wave passes the same
+    // barriers; after a failed pivot wave 0 does nothing more and the
+    // update waves' further results are never stored
     for (int k = 0; k + 1 < nb; k++) {
-        if (D.failed) return;
         if (wave == 0) {
-            finish_col(k + 1);
+            if (!failed0) finish_col(k + 1);
         } else {
             // column k (scaled) to the owned elements of columns >= k + 2,
             // held in registers; an element goes to LDS once, when its
             // column is the one wave 0 finishes next (j = k + 2)
             const dd *lk = D.Lb[k & 1];
-            dd lj[PD_OWN2], lv[PD_OWN2];
-            bool act[PD_OWN2];
-#pragma unroll
-            for (int q = 0; q < PD_OWN2; q++) {
+            auto upd = [&](int q) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
-                act[q] = j >= k + 2;
-                if (act[q]) {
-                    lj[q] = lk[j];
-                    lv[q] = lk[i];
-                }
+                const dd lj = lk[j], lv = lk[i];
+                const dd u = dd_sub(sv[q], dd_mul(lj, lv));
+                const bool on = j >= k + 2 && !dd_is_zero(lj);
+                sv[q].hi = on ? u.hi : sv[q].hi;
+                sv[q].lo = on ? u.lo : sv[q].lo;
+            };
+            if (k + 2 <= jmax[0]) {
+                upd(0);
+                upd(1);
+            } else if (k + 2 <= jmax[1]) {
+                upd(1);
             }
+            if (k + 2 <= jmax[2]) upd(2);
 #pragma unroll
             for (int q = 0; q < PD_OWN2; q++) {
                 const int i = own[q] & 0xff, j = own[q] >> 8;
-                if (act[q] && !dd_is_zero(lj[q])) sv[q] = dd_sub(sv[q], dd_mul(lj[q], lv[q]));
-                if (act[q] && j == k + 2) D.S[i][j] = sv[q];
+                if (own[q] && j == k + 2) D.S[i][j] = sv[q];
             }
         }
         __syncthreads();
