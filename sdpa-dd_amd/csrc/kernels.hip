@@ -1895,11 +1895,6 @@ __global__ __launch_bounds__(BI_NT) void k_trsv_binv(idx_t n, int trans, const d
         dd pr[Q];
 #pragma unroll
         for (int k = 0; k < Q; k++) pr[k] = dd_mul(a[k], raw_dd(vr[k]));
-        // block J + 1: its operands and values, checked next iteration
-        if (J + BI_D + 2 < nblk) {
-            load_l(J + 1);
-            read_v(J + 1);
-        }
         dd t = tree_sum<Q>(pr);
         dd o = shfl_xor_dd(t, 2);
         t = (hq & 2) == 0 ? dd_add(t, o) : dd_add(o, t);
@@ -1910,6 +1905,13 @@ __global__ __launch_bounds__(BI_NT) void k_trsv_binv(idx_t n, int trans, const d
             // rows of block J + BI_D + 1 are final now
             seq_block(n, trans, J + BI_D + 1, lo_f, nb_f);
             if (hq == 0 && i >= lo_f && i < lo_f + nb_f) put_dd(pg + (idx_t)(J + BI_D + 1) * BB + (i - lo_f), xi);
+        }
+        // block J + 1: its operands and values, checked next iteration;
+        // issued after the partial above, which they would otherwise hold
+        // back in the memory pipeline
+        if (J + BI_D + 2 < nblk) {
+            load_l(J + 1);
+            read_v(J + 1);
         }
     }
 }
