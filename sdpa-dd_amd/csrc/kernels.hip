@@ -700,10 +700,15 @@ __global__ void k_potrf_init(int64_t *info, dd *pub, unsigned long long *ctr) {
 // Ozaki form only): the 16 planes and tagged row exponents of rows
 // [0, below) of the panel, as oz::gemm's slicing would write them.
 struct PanelPlanes {
-    int8_t *planes = nullptr;  // plane s of panel row r: planes[(s rows_pad + r) 64 + k]
+    int8_t *planes = nullptr;  // plane s, chunk kb of row r: planes[((s nkb + kb) rows_pad + r) 64 + k]
     unsigned *exps = nullptr;
     idx_t rows_pad = 0;
     unsigned tag = 0;
+    // nkb = 1: the block's own 64 columns.  nkb > 1 (the last block of a
+    // panel): the whole panel's 64 nkb columns, starting at column pcol,
+    // the earlier blocks' read back from A (final by then)
+    int nkb = 1;
+    idx_t pcol = 0;
 };
 
 template <bool FAST, bool SL>
@@ -851,7 +856,7 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_fused(dd *A, idx_t lda, idx_t K
         const int r = e % FR, j = e / FR;
         if (r < nr) B[(rb + r) + (idx_t)j * lda] = P.Tt[r][j];
     }
-    if (SL) {
+    if (SL && pl.nkb == 1) {
         // 16 lanes per row, 4 consecutive k each; the row's exponent is the
         // maximum over its 16 lanes (k_slice's value: frexp of max |hi|)
         const int r = tid >> 4, q = tid & 15;
@@ -879,6 +884,51 @@ __global__ __launch_bounds__(PD_NT) void k_potrf_fused(dd *A, idx_t lda, idx_t K
 #pragma unroll
                 for (int t = 0; t < 16; t++)
                     *(unsigned *)(pl.planes + (((idx_t)t * pl.rows_pad + lr) * 64 + 4 * q)) = w4[t];
+            }
+        }
+    } else if (SL) {
+        // the whole panel (nkb <= 4 chunks of 64 columns) as k_rowexp and
+        // k_slice form it: thread (r, g) = (tid % 32, tid / 32) takes row r,
+        // columns [8 g, 8 g + 8), so a column's rows are read by consecutive
+        // threads (the earlier blocks' columns from A, this block's from
+        // LDS); the row's maximum meets in LDS over the 32 groups
+        const int nkb = pl.nkb, own0 = 64 * (nkb - 1), ncol = 64 * nkb;
+        const int r = tid & 31, g = tid >> 5;
+        double *part = reinterpret_cast<double *>(&P.Lc[0]);  // [32 groups][32 rows]
+        static_assert(sizeof(P.Lc) >= 32 * 32 * sizeof(double), "row maxima fit over Lc");
+        const bool live = r < nr && 8 * g < ncol;
+        dd av[8];
+        double mx = 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = 8 * g + u;
+            dd v = mk(0.0);
+            if (live) v = c < own0 ? A[(rb + r) + (pl.pcol + c) * lda] : P.Tt[r][c - own0];
+            av[u] = v;
+            mx = fmax(mx, fabs(v.hi));
+        }
+        __syncthreads();  // Lc's last readers are done
+        part[g * 32 + r] = mx;
+        __syncthreads();
+        double rm = 0.0;
+#pragma unroll 8
+        for (int h = 0; h < 32; h++) rm = fmax(rm, part[h * 32 + r]);
+        int e = 0;
+        if (rm > 0.0) (void)frexp(rm, &e);
+        if (live) {
+            const idx_t lr = rb - r0 + r;
+            if (g == 0) pl.exps[lr] = oz::exp_tagged(pl.tag, e);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int c = 8 * g + 4 * h, kb = c / 64;
+                unsigned w4[16];
+#pragma unroll
+                for (int t = 0; t < 16; t++) w4[t] = 0u;
+#pragma unroll
+                for (int u = 0; u < 4; u++) oz::put_digits<16>(av[4 * h + u], e, w4, u);
+#pragma unroll
+                for (int t = 0; t < 16; t++)
+                    *(unsigned *)(pl.planes + ((((idx_t)t * nkb + kb) * pl.rows_pad + lr) * 64 + (c % 64))) = w4[t];
             }
         }
     }
@@ -954,12 +1004,23 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
     unsigned long long base = 0;
     int launches = 0;
     bool helper_pending = false;  // an update on la->hs that later work on s must wait for
-    for (idx_t P = 0; P < n; P += OZ_NB) {
+    // with look-ahead, the last block's launch of each full panel writes the
+    // digit planes of the whole panel (its rows are the rows below the
+    // panel), which the look-ahead and the helper's update then share: one
+    // slicing launch pair less on each stream.  Panels alternate between
+    // two plane buffers, so the helper can still read one panel's planes
+    // while the next panel's are written.
+    const bool panel_planes = la && la->w2 && la->wp && S == 16;
+    int panel_no = 0;
+    for (idx_t P = 0; P < n; P += OZ_NB, panel_no++) {
         const idx_t pw = std::min<idx_t>(OZ_NB, n - P);
         // a one-block panel without look-ahead: its trailing update (k = 64)
         // reads the same planes as an in-panel update would
         const bool trail64 = pw == PB && !la;
         unsigned trail_tag = 0;
+        oz::Work *pbuf = panel_planes ? (panel_no % 2 ? la->wp : la->w2) : nullptr;
+        unsigned panel_tag = 0;
+        idx_t panel_rows_pad = 0;
         for (idx_t K = P; K < P + pw; K += PB) {
             const int nb = (int)std::min<idx_t>(PB, P + pw - K);
             const idx_t below = n - K - nb;
@@ -969,6 +1030,7 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
             // that follows
             PanelPlanes pl;
             const bool fuse = (rem > 0 || trail64) && S == 16 && nb == PB && below > 0;
+            const bool last_planes = panel_planes && rem == 0 && nb == PB && pw % PB == 0 && below > 0;
             if (fuse) {
                 pl.tag = oz::prepare_planes(s, w, below);
                 if (!pl.tag) return -2;
@@ -976,11 +1038,23 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
                 pl.exps = w.ea;
                 pl.rows_pad = (below + 63) / 64 * 64;
                 if (trail64) trail_tag = pl.tag;
+            } else if (last_planes) {
+                pl.nkb = (int)(pw / PB);
+                pl.pcol = P;
+                pl.tag = oz::prepare_panel_planes(s, *pbuf, below, pl.nkb);
+                if (!pl.tag) return -2;
+                pl.planes = pbuf->sa;
+                pl.exps = pbuf->ea;
+                pl.rows_pad = (below + 63) / 64 * 64;
+                panel_tag = pl.tag;
+                panel_rows_pad = pl.rows_pad;
             }
             if (fast_pivot)
-                potrf_step<true>(s, K, nb, below, A, lda, d_info, pp, base, launches, fuse ? &pl : nullptr);
+                potrf_step<true>(s, K, nb, below, A, lda, d_info, pp, base, launches,
+                                 fuse || last_planes ? &pl : nullptr);
             else
-                potrf_step<false>(s, K, nb, below, A, lda, d_info, pp, base, launches, fuse ? &pl : nullptr);
+                potrf_step<false>(s, K, nb, below, A, lda, d_info, pp, base, launches,
+                                  fuse || last_planes ? &pl : nullptr);
             if (rem > 0) {
                 // the panel's remaining columns, every row below: an Ozaki
                 // product with k = 64 (on the FP64 VALU this rank-64 update
@@ -1009,15 +1083,22 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
         }
         const idx_t qw = std::min<idx_t>(OZ_NB, below), R = Q + qw;
         // next panel's columns [Q, R), every row below Q, on s
-        int rc = oz::gemm(s, w, 'N', 'T', below, qw, pw, -1, L21, lda, L21, lda, 1, A + Q * (1 + lda), lda, true, S);
+        const int pnkb = (int)(pw / PB);
+        int rc = panel_tag ? oz::gemm_panel_planes(s, *pbuf, panel_tag, panel_rows_pad, pnkb, 0, below, qw, -1, 1,
+                                                   A + Q * (1 + lda), lda, true)
+                           : oz::gemm(s, w, 'N', 'T', below, qw, pw, -1, L21, lda, L21, lda, 1, A + Q * (1 + lda), lda,
+                                      true, S);
         if (rc != 0) return rc;
         if (R < n) {
             // the rest: rows and columns from R, on the helper stream
             (void)hipEventRecord(la->ready, s);
             (void)hipStreamWaitEvent(la->hs, la->ready, 0);
             const dd *L31 = A + R + P * lda;
-            rc = oz::gemm(la->hs, *la->w2, 'N', 'T', n - R, n - R, pw, -1, L31, lda, L31, lda, 1, A + R * (1 + lda), lda,
-                          true, S);
+            rc = panel_tag && qw % PB == 0
+                     ? oz::gemm_panel_planes(la->hs, *pbuf, panel_tag, panel_rows_pad, pnkb, qw, n - R, n - R, -1, 1,
+                                             A + R * (1 + lda), lda, true)
+                     : oz::gemm(la->hs, *la->w2, 'N', 'T', n - R, n - R, pw, -1, L31, lda, L31, lda, 1,
+                                A + R * (1 + lda), lda, true, S);
             if (rc != 0) return rc;
             (void)hipEventRecord(la->done, la->hs);
             helper_pending = true;

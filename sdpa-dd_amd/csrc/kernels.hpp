@@ -38,6 +38,9 @@ struct OzLookAhead {
     hipStream_t hs = nullptr;
     oz::Work *w2 = nullptr;
     hipEvent_t ready = nullptr, done = nullptr;
+    // optional second workspace: with it, w2 and wp hold the panels' digit
+    // planes in turn (written by each panel's last diagonal-block launch)
+    oz::Work *wp = nullptr;
 };
 // la: when given and n > 1024, each panel's trailing update is split into
 // the next panel's columns (on s, first) and the rest (on la->hs), so that

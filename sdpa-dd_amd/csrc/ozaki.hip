@@ -231,7 +231,8 @@ __global__ __launch_bounds__(512) void k_gemm(idx_t m, idx_t n, int nkb, const i
     __shared__ v4i LB[S][64][4];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     idx_t tr, tc;
-    if (!xcd_tile(mpad / 64, npad / 64, lower != 0, tr, tc)) return;
+    // mpad and npad are the planes' row strides; the tile grid follows m, n
+    if (!xcd_tile((m + 63) / 64, (n + 63) / 64, lower != 0, tr, tc)) return;
     const idx_t r0 = tr * 64, c0 = tc * 64;
     if (lower && r0 + 63 < c0) return;
     const int wr = wave & 3, wc = wave >> 2;
@@ -396,22 +397,50 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
 // `same`), timed as family "ozaki" with the product's algorithmic 2mnk
 // dd-flops (the MFMA work is S(S+1)/2 times that in int8 multiply-adds).
 template <int S>
+static void product_at(hipStream_t s, const int8_t *sa, const unsigned *ea, idx_t astride, unsigned ta, const int8_t *sb,
+                       const unsigned *eb, idx_t bstride, unsigned tb, idx_t m, idx_t n, idx_t k, int nkb, int alpha,
+                       int beta, dd *C, idx_t ldc, bool lower);
+template <int S>
 static void product(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, int nkb, unsigned ta, unsigned tb, bool same,
                     idx_t mpad, idx_t npad, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
+    product_at<S>(s, w.sa, w.ea, mpad, ta, same ? w.sa : w.sb, same ? w.ea : w.eb, same ? mpad : npad, tb, m, n, k, nkb,
+                  alpha, beta, C, ldc, lower);
+}
+// The tile kernel on planes at sa / sb (row strides astride, bstride: the
+// planes of chunk kb of plane s start at row (s nkb + kb) stride), timed as
+// family "ozaki".
+template <int S>
+static void product_at(hipStream_t s, const int8_t *sa, const unsigned *ea, idx_t astride, unsigned ta, const int8_t *sb,
+                       const unsigned *eb, idx_t bstride, unsigned tb, idx_t m, idx_t n, idx_t k, int nkb, int alpha,
+                       int beta, dd *C, idx_t ldc, bool lower) {
     sdk::StatScope st("ozaki", s, 2.0 * (double)m * n * k * (lower ? 0.5 : 1.0));
+    const idx_t mpad = roundup(m, 64), npad = roundup(n, 64);
     // 64 x 64 tiles hold a CU each (128 KB of LDS at S = 16): below about
     // two per CU, the 32 x 32 kernel fills the chip better
     const idx_t t64 = cdiv(mpad, 64) * cdiv(npad, 64) / (lower ? 2 : 1);
     const bool small = sdk::g_tile_policy == 1 || (sdk::g_tile_policy == 0 && t64 < 512);
     if (small)
-        k_gemm32<S><<<dim3(cdiv(mpad, 32), cdiv(npad, 32)), 256, 0, s>>>(
-            m, n, nkb, w.sa, mpad, w.ea, ta, same ? w.sa : w.sb, same ? mpad : npad, same ? w.ea : w.eb, tb, C, ldc,
-            alpha, beta, lower ? 1 : 0);
+        k_gemm32<S><<<dim3(cdiv(mpad, 32), cdiv(npad, 32)), 256, 0, s>>>(m, n, nkb, sa, astride, ea, ta, sb, bstride, eb,
+                                                                          tb, C, ldc, alpha, beta, lower ? 1 : 0);
     else
-        k_gemm<S><<<dim3((unsigned)xcd_grid(mpad / 64, npad / 64, lower)), 512, 0, s>>>(m, n, nkb, w.sa, mpad, w.ea, ta,
-                                                                      same ? w.sa : w.sb, same ? mpad : npad,
-                                                                      same ? w.ea : w.eb, tb, C, ldc, alpha, beta,
-                                                                      lower ? 1 : 0);
+        k_gemm<S><<<dim3((unsigned)xcd_grid(mpad / 64, npad / 64, lower)), 512, 0, s>>>(
+            m, n, nkb, sa, astride, ea, ta, sb, bstride, eb, tb, C, ldc, alpha, beta, lower ? 1 : 0);
+}
+
+int gemm_panel_planes(hipStream_t s, Work &w, unsigned tag, idx_t stride, int nkb, idx_t row0, idx_t m, idx_t n,
+                      int alpha, int beta, dd *C, idx_t ldc, bool lower) {
+    if (m < 0 || n < 0 || n > m || nkb < 1 || (idx_t)nkb * KC > MAX_K || (alpha != 1 && alpha != -1) ||
+        (beta != 0 && beta != 1) || !tag || row0 < 0 || row0 % 64 != 0 || row0 + roundup(m, 64) > stride)
+        return -1;
+    if (m == 0 || n == 0) return 0;
+    const int8_t *pa = w.sa + row0 * KC;
+    const unsigned *pe = w.ea + row0;
+    product_at<16>(s, pa, pe, stride, tag, pa, pe, stride, tag, m, n, (idx_t)nkb * KC, nkb, alpha, beta, C, ldc, lower);
+    return 0;
+}
+unsigned prepare_panel_planes(hipStream_t s, Work &w, idx_t m, int nkb) {
+    if (!w.reserve(roundup(m, 64), roundup(m, 64), (idx_t)KC * nkb, 16, 1)) return 0;
+    return w.next_tag(s);
 }
 
 unsigned prepare_planes(hipStream_t s, Work &w, idx_t m) {

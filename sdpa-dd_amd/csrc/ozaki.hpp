@@ -59,6 +59,17 @@ int gemm(hipStream_t s, Work &w, char transa, char transb, idx_t m, idx_t n, idx
 // that kernel runs.  Same results as gemm('N', 'T', ...) on M.
 int gemm_planes(hipStream_t s, Work &w, unsigned tag, idx_t m, idx_t n, idx_t k, int alpha, int beta, dd *C, idx_t ldc,
                 bool lower);
+// Panel planes: an m-row operand with k = 64 nkb (S = 16), its planes laid
+// out as k_slice lays them (plane s, chunk kb at rows (s nkb + kb) stride,
+// stride = roundup(m, 64)) and written by the kernel that produced it.
+// prepare_panel_planes reserves them and returns their tag (0 when the
+// workspace cannot grow); gemm_panel_planes forms C = beta*C + alpha*(M'
+// M'(0:n, :)^T) for M' = rows [row0, row0 + m) (row0 a multiple of 64,
+// row0 + roundup(m, 64) <= stride).  Same results as gemm('N', 'T', ...)
+// on M'.
+unsigned prepare_panel_planes(hipStream_t s, Work &w, idx_t m, int nkb);
+int gemm_panel_planes(hipStream_t s, Work &w, unsigned tag, idx_t stride, int nkb, idx_t row0, idx_t m, idx_t n,
+                      int alpha, int beta, dd *C, idx_t ldc, bool lower);
 // Reserves planes for an m-row operand with k <= KC and S = 16 and returns
 // the tag its producer writes (0 when the workspace cannot grow).
 unsigned prepare_planes(hipStream_t s, Work &w, idx_t m);

@@ -991,6 +991,10 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         return -2;
     la.hs = s2;
     la.w2 = &ozw2;
+    // the panels' planes alternate between ozw2 and ozw3 (s3's X
+    // factorisation is joined before the Schur factorisation starts and
+    // forked after it ends)
+    la.wp = &ozw3;
     {
         std::vector<sdk::DotTask> txz, t3;
         const dd *a3[3] = {X, DxMat, DxMat}, *b3[3] = {DzMat, Z, DzMat};
