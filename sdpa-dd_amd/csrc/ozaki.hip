@@ -393,13 +393,12 @@ static void launch(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, const dd *
     product<S>(s, w, m, n, k, nkb, ta, same ? ta : tb, same, mpad, npad, alpha, beta, C, ldc, lower);
 }
 
-// The tile kernel of a product whose planes are in w (B's in w.sa when
-// `same`), timed as family "ozaki" with the product's algorithmic 2mnk
-// dd-flops (the MFMA work is S(S+1)/2 times that in int8 multiply-adds).
 template <int S>
 static void product_at(hipStream_t s, const int8_t *sa, const unsigned *ea, idx_t astride, unsigned ta, const int8_t *sb,
                        const unsigned *eb, idx_t bstride, unsigned tb, idx_t m, idx_t n, idx_t k, int nkb, int alpha,
                        int beta, dd *C, idx_t ldc, bool lower);
+// The tile kernel of a product whose planes are in w (B's in w.sa when
+// `same`).
 template <int S>
 static void product(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, int nkb, unsigned ta, unsigned tb, bool same,
                     idx_t mpad, idx_t npad, int alpha, int beta, dd *C, idx_t ldc, bool lower) {
@@ -408,7 +407,8 @@ static void product(hipStream_t s, Work &w, idx_t m, idx_t n, idx_t k, int nkb, 
 }
 // The tile kernel on planes at sa / sb (row strides astride, bstride: the
 // planes of chunk kb of plane s start at row (s nkb + kb) stride), timed as
-// family "ozaki".
+// family "ozaki" with the product's algorithmic 2mnk dd-flops (the MFMA
+// work is S(S+1)/2 times that in int8 multiply-adds).
 template <int S>
 static void product_at(hipStream_t s, const int8_t *sa, const unsigned *ea, idx_t astride, unsigned ta, const int8_t *sb,
                        const unsigned *eb, idx_t bstride, unsigned tb, idx_t m, idx_t n, idx_t k, int nkb, int alpha,
