@@ -296,6 +296,33 @@ void gemm(hipStream_t s, char tb, idx_t m, idx_t n, idx_t k, dd alpha, const dd 
                                                      diag_offset, nullptr, nullptr, nullptr);
 }
 
+// C = A B in GEMM_KS parts over consecutive k ranges (each part summed in
+// ascending k as gemm sums it), the parts then added in order: the Ozaki
+// form's n x n products, whose per-element chains are GEMM_KS times shorter
+// than Rgemm's.  The last part also takes the remainder of k.
+__global__ void k_sum_parts(idx_t m, idx_t n, int parts, const dd *W, dd *C, idx_t ldc) {
+    const idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    dd v = W[e];
+    for (int p = 1; p < parts; p++) v = dd_add(v, W[(idx_t)p * m * n + e]);
+    C[(e % m) + (e / m) * ldc] = v;
+}
+void gemm_split(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *A, idx_t lda, const dd *B, idx_t ldb, dd *C,
+                idx_t ldc, dd *work) {
+    if (m <= 0 || n <= 0) return;
+    const idx_t q = k / GEMM_KS, r = k - q * GEMM_KS;
+    if (q == 0) {
+        gemm(s, 'N', m, n, k, mk(1.0), A, lda, B, ldb, mk(0.0), C, ldc);
+        return;
+    }
+    // parts p = A(:, p q : (p+1) q) B(p q : (p+1) q, :) as one batch
+    gemm(s, 'N', m, n, q, mk(1.0), A, lda, B, ldb, mk(0.0), work, m, GEMM_KS, q * lda, q, m * n);
+    if (r > 0)
+        gemm(s, 'N', m, n, r, mk(1.0), A + GEMM_KS * q * lda, lda, B + GEMM_KS * q, ldb, mk(1.0),
+             work + (GEMM_KS - 1) * m * n, m);
+    k_sum_parts<<<cdiv(m * n, 256), 256, 0, s>>>(m, n, GEMM_KS, work, C, ldc);
+}
+
 void gemm_ptrs(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *const *A, idx_t lda, const dd *const *B, idx_t ldb,
                dd *const *C, idx_t ldc, int batch) {
     if (m <= 0 || n <= 0 || batch <= 0) return;

@@ -18,6 +18,13 @@ void gemm(hipStream_t s, char tb, idx_t m, idx_t n, idx_t k, dd alpha, const dd 
           dd beta, dd *C, idx_t ldc, int batch = 1, idx_t sA = 0, idx_t sB = 0, idx_t sC = 0,
           bool lower_only = false, idx_t diag_offset = 0);
 
+// C := A*B (alpha 1, beta 0, 'N' 'N') in GEMM_KS parts over consecutive k
+// ranges, each summed in ascending k, added in order (not Rgemm's order:
+// the Ozaki form's n x n products).  work: GEMM_KS m n dd.
+constexpr int GEMM_KS = 4;
+void gemm_split(hipStream_t s, idx_t m, idx_t n, idx_t k, const dd *A, idx_t lda, const dd *B, idx_t ldb, dd *C,
+                idx_t ldc, dd *work);
+
 // Lower Cholesky in place.  info (device int64) receives 0, the 1-based
 // index of the first non-positive pivot, or POTRF_WAIT_FAILED when a panel
 // workgroup's bounded wait for the diagonal block ran out (nothing is

@@ -489,6 +489,7 @@ class Solver {
     sdk::DotTask *d_tasks_xz = nullptr, *d_tasks3 = nullptr;
     dd *d_partial = nullptr;
     idx_t max_blk_elems = 0;
+    dd *gsplit = nullptr;  // the Ozaki form's split products: GEMM_KS parts of the largest block
     // second stream for the Z factorisation, per-block Cholesky info slots
     // X's factorisation runs on s3, Z's on s2, so that the main stream's
     // residual and inner-product work of the same step overlaps both
@@ -636,15 +637,22 @@ class Solver {
             if (nlp) sdk::lp_triple(s, nlp, a + nn, b + nn, c + nn, ret + nn);
             return;
         }
+        // the Ozaki form's products in four k parts (shorter dd chains)
         for (int l = 0; l < nB; l++) {
             int n = bsz(l);
             idx_t o = dp.boff[l];
-            sdk::gemm(s, 'N', n, n, n, mk(1.0), a + o, n, b + o, n, mk(0.0), work + o, n);
+            if (gsplit)
+                sdk::gemm_split(s, n, n, n, a + o, n, b + o, n, work + o, n, gsplit);
+            else
+                sdk::gemm(s, 'N', n, n, n, mk(1.0), a + o, n, b + o, n, mk(0.0), work + o, n);
         }
         for (int l = 0; l < nB; l++) {
             int n = bsz(l);
             idx_t o = dp.boff[l];
-            sdk::gemm(s, 'N', n, n, n, mk(1.0), work + o, n, c + o, n, mk(0.0), ret + o, n);
+            if (gsplit)
+                sdk::gemm_split(s, n, n, n, work + o, n, c + o, n, ret + o, n, gsplit);
+            else
+                sdk::gemm(s, 'N', n, n, n, mk(1.0), work + o, n, c + o, n, mk(0.0), ret + o, n);
         }
         if (nlp) sdk::lp_triple(s, nlp, a + nn, b + nn, c + nn, ret + nn);
     }
@@ -1008,6 +1016,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
                 idx_t ne = (idx_t)bsz(l) * bsz(l), o = dp.boff[l];
                 t3.push_back({a3[q] + o, b3[q] + o, ne, d_sc + q * nB + l});
             }
+        if (ctx.path == 1 && !(gsplit = alloc(sdk::GEMM_KS * max_blk_elems))) return -2;
         if (hipMalloc((void **)&d_tasks_xz, sizeof(sdk::DotTask) * txz.size()) != hipSuccess) return -2;
         mem.push_back(d_tasks_xz);
         if (init) {
