@@ -1024,8 +1024,8 @@ int potrf_lower_oz(hipStream_t s, idx_t n, dd *A, idx_t lda, int64_t *d_info, dd
     // m = 800 loses 14% of the Cholesky time)
     if (n <= 1024) la = nullptr;
     // the pivot chain (m square roots and reciprocals in sequence) is the
-    // critical path of the large factorisations: those take the faster pivot
-    const bool fast_pivot = n > 1024;
+    // critical path of every factorisation here: all take the faster pivot
+    const bool fast_pivot = true;
     const PotrfPub pp(scratch);
     potrf_begin(s, d_info, pp);
     unsigned long long base = 0;
