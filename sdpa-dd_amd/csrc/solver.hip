@@ -4,6 +4,7 @@
 // complementarity, residual ratios) is the reference's host logic evaluated
 // in dd on the CPU; every n x n and m x m operation runs on the GPU.  Only
 // scalars and block diagonals cross the PCIe boundary during the loop.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -656,7 +657,7 @@ class Solver {
         }
         if (nlp) sdk::lp_triple(s, nlp, a + nn, b + nn, c + nn, ret + nn);
     }
-    bool oz_failed = false;
+    std::atomic<bool> oz_failed{false};  // set from the X/Z enqueue threads too
     void oz_check(int rc) {
         if (rc != 0) oz_failed = true;
     }
@@ -801,9 +802,18 @@ class Solver {
     }
     // ... and their launches, which may be enqueued after later work on s
     // (their many small launches then do not hold that work back)
+    // X's launches and Z's are enqueued from two host threads: enqueued in
+    // turn, Z's first launch waited for all of X's (hundreds of launches at
+    // n = 800), so the two factorisations barely overlapped on the device
     void enq_inv_chol_launch() {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        std::thread tz([&] {
+            (void)hipSetDevice(dev);  // the current device is per thread
+            inv_chol_async(s2, Z, invCholZ, DLS2, invZ, d_infos + nB, d_infos + 2 * nB + 1, d_rinv2);
+        });
         inv_chol_async(s3, X, invCholX, DLS1, nullptr, d_infos, d_infos + 2 * nB, d_rinv3);
-        inv_chol_async(s2, Z, invCholZ, DLS2, invZ, d_infos + nB, d_infos + 2 * nB + 1, d_rinv2);
+        tz.join();
     }
     void enq_inv_chol_join() {
         (void)hipEventRecord(ev_join, s2);
