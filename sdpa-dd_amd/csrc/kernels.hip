@@ -2101,30 +2101,35 @@ __global__ void k_set_identity(idx_t n, dd v, dd *A, idx_t lda) {
 // updates from rows above already applied (Rtrsm 'L','L','N','N' order: if
 // b_kj != 0 { b_kj = b_kj / a_kk; b_ij -= b_kj * a_ik for i > k }).  A wave
 // owns a column, lane i its row i; the diagonal block is staged in LDS.
+// FAST (the Ozaki form): b_k times the reciprocal 1/l_kk (formed once per
+// launch, 64 divisions in parallel) in place of the division on the chain.
+template <bool FAST>
 __global__ __launch_bounds__(256) void k_trsm_llnn_diag(idx_t rb, int nb, idx_t ncols, const dd *L, idx_t lda, dd *B,
                                                         idx_t ldb) {
     __shared__ dd Lc[PB * (PB + 1) / 2];
+    __shared__ dd rc[FAST ? PB : 1];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (int e = tid; e < nb * nb; e += 256) {
         int j = e % nb, k = e / nb;
         if (j >= k) Lc[lpack(j, k)] = L[(rb + j) + (rb + k) * lda];
     }
+    if (FAST && tid < nb) rc[tid] = dd_div(mk(1.0), L[(rb + tid) + (rb + tid) * lda]);
     __syncthreads();
     for (idx_t c = (idx_t)blockIdx.x * 4 + wave; c < ncols; c += (idx_t)gridDim.x * 4) {
         dd* col = B + rb + c * ldb;
         dd b = lane < nb ? col[lane] : mk(0.0);
         // step k's operands are read during step k-1, off the division chain
         // (lanes at or above nb read unused entries inside Lc)
-        dd lnext = Lc[lpack(lane, 0)], dnext = Lc[lpack(0, 0)];
+        dd lnext = Lc[lpack(lane, 0)], dnext = FAST ? rc[0] : Lc[lpack(0, 0)];
         for (int k = 0; k < nb; k++) {
             const dd lk = lnext, dk = dnext;
             if (k + 1 < nb) {
                 lnext = Lc[lpack(lane > k + 1 ? lane : k + 1, k + 1)];
-                dnext = Lc[lpack(k + 1, k + 1)];
+                dnext = FAST ? rc[k + 1] : Lc[lpack(k + 1, k + 1)];
             }
             const int nz = __builtin_amdgcn_readlane(!dd_is_zero(b), k);
             if (!nz) continue;
-            if (lane == k) b = dd_div(b, dk);
+            if (lane == k) b = FAST ? dd_mul(b, dk) : dd_div(b, dk);
             const dd bk = readlane_dd(b, k);
             if (lane > k && lane < nb) b = dd_sub(b, dd_mul(bk, lk));
         }
@@ -2134,13 +2139,19 @@ __global__ __launch_bounds__(256) void k_trsm_llnn_diag(idx_t rb, int nb, idx_t 
 
 // General B := L^{-1} B (m x n B); for the inverse, B starts as identity and
 // only columns < rb+nb can be non-zero in the rows being solved.
-static void trsm_llnn_impl(hipStream_t s, idx_t m, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb, bool identity_rhs) {
+static void trsm_llnn_impl(hipStream_t s, idx_t m, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb, bool identity_rhs,
+                           bool fast = false) {
     for (idx_t rb = 0; rb < m; rb += PB) {
         int nb = (int)std::min<idx_t>(PB, m - rb);
         idx_t ncols = identity_rhs ? std::min<idx_t>(n, rb + nb) : n;
         {
             StatScope st("trsm", s, (double)ncols * nb * nb);
-            k_trsm_llnn_diag<<<std::min<idx_t>(cdiv(ncols, 4), 1024), 256, 0, s>>>(rb, nb, ncols, L, lda, B, ldb);
+            if (fast)
+                k_trsm_llnn_diag<true><<<std::min<idx_t>(cdiv(ncols, 4), 1024), 256, 0, s>>>(rb, nb, ncols, L, lda, B,
+                                                                                            ldb);
+            else
+                k_trsm_llnn_diag<false><<<std::min<idx_t>(cdiv(ncols, 4), 1024), 256, 0, s>>>(rb, nb, ncols, L, lda, B,
+                                                                                             ldb);
         }
         idx_t below = m - rb - nb;
         if (below > 0) {
@@ -2150,10 +2161,10 @@ static void trsm_llnn_impl(hipStream_t s, idx_t m, idx_t n, const dd *L, idx_t l
     }
 }
 
-void inv_lower(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb) {
+void inv_lower(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb, bool fast) {
     if (n <= 0) return;
     k_set_identity<<<dim3(cdiv(n, 256), (unsigned)n), 256, 0, s>>>(n, mk(1.0), B, ldb);
-    trsm_llnn_impl(s, n, n, L, lda, B, ldb, true);
+    trsm_llnn_impl(s, n, n, L, lda, B, ldb, true, fast);
 }
 
 void trsm_llnn(hipStream_t s, idx_t m, idx_t n, const dd *A, idx_t lda, dd *B, idx_t ldb) {
@@ -2175,7 +2186,7 @@ int inv_lower_oz(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ld
             const idx_t ncols = std::min<idx_t>(n, K + nb);
             {
                 StatScope st("trsm", s, (double)ncols * nb * nb);
-                k_trsm_llnn_diag<<<std::min<idx_t>(cdiv(ncols, 4), 1024), 256, 0, s>>>(K, nb, ncols, L, lda, B, ldb);
+                k_trsm_llnn_diag<true><<<std::min<idx_t>(cdiv(ncols, 4), 1024), 256, 0, s>>>(K, nb, ncols, L, lda, B, ldb);
             }
             const idx_t rem = P + pw - K - nb;
             if (rem > 0)

@@ -85,8 +85,10 @@ int trsv_binv(hipStream_t s, char trans, idx_t n, const dd *L, idx_t lda, dd *x,
 void zero_ints(hipStream_t s, int *p, int cnt);
 
 // B := L^{-1} (B enters as anything; set to identity then solved like
-// Rtrsm('L','L','N','N') on the identity).
-void inv_lower(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb);
+// Rtrsm('L','L','N','N') on the identity).  fast (the Ozaki form): each
+// solved value is multiplied by the reciprocal of its pivot instead of
+// divided by the pivot.
+void inv_lower(hipStream_t s, idx_t n, const dd *L, idx_t lda, dd *B, idx_t ldb, bool fast = false);
 
 // C := A**T * A for lower triangular A (Rtrmm('L','L','T','N') applied to a
 // copy of A); C full n x n.

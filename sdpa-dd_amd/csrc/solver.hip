@@ -771,7 +771,7 @@ class Solver {
             if (ctx.path == 2)
                 oz_check(sdk::inv_lower_oz(st, n, work + o, n, invChol + o, n, w, ctx.oz_slices));
             else
-                sdk::inv_lower(st, n, work + o, n, invChol + o, n);
+                sdk::inv_lower(st, n, work + o, n, invChol + o, n, ctx.path >= 1);
         }
         if (inv)
             for (int l = 0; l < nB; l++) {
