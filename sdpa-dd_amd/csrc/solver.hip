@@ -132,6 +132,7 @@ namespace {
 // synchronisation per step for all of them.  The (k+1)-tridiagonal
 // eigenproblem is solved on the host, forming only the last row of its
 // eigenvector matrix (the one entry the estimate reads).
+static constexpr int LZ_INFLIGHT = 3;  // batches of steps a chain keeps on the device
 struct Chain {
     int n = 0, cap = 0;
     const dd *L = nullptr, *M = nullptr;
@@ -149,7 +150,7 @@ struct Chain {
     };
     std::deque<Batch> inflight;
     int ek = 0, ekk = 0;        // the next step to enqueue and its kk
-    hipEvent_t evs[3] = {};     // ring of completion events (two in flight + the last one used)
+    hipEvent_t evs[LZ_INFLIGHT + 1] = {};  // ring of completion events (in flight + the last one used)
     int evi = 0;
     hipEvent_t last_used = nullptr;
     hipStream_t s = nullptr;
@@ -259,7 +260,7 @@ void chain_enqueue(Chain &c, int T, int first, int cnt) {
     (void)hipMemcpyAsync(c.hh + 2 * (size_t)first, c.hist + 2 * (size_t)first, sizeof(dd) * 2 * cnt,
                          hipMemcpyDeviceToHost, s);
     hipEvent_t ev = c.evs[c.evi];
-    c.evi = (c.evi + 1) % 3;
+    c.evi = (c.evi + 1) % (LZ_INFLIGHT + 1);
     (void)hipEventRecord(ev, s);
     c.inflight.push_back({first, cnt, ev});
 }
@@ -353,7 +354,8 @@ void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) 
         c.inflight.clear();
         c.ek = c.ekk = 0;
         c.last_used = nullptr;
-        if (enqueue_next(c)) enqueue_next(c);
+        for (int b = 0; b < LZ_INFLIGHT && enqueue_next(c); b++) {
+        }
     }
     // The host work of a chain (reading its batches, the tridiagonal eigen
     // solves, enqueueing) runs on one of up to LZ_THREADS host threads, the
@@ -381,7 +383,7 @@ void run_chains(ChainPool &pool, int cnt, int T, hipStream_t parent, bool fast) 
                     chain_finish_step(c, c.hh + 2 * (size_t)(b.first + j));
                     if (c.done) break;
                 }
-                if (!c.done && c.inflight.size() < 2) enqueue_next(c);
+                if (!c.done && c.inflight.size() < LZ_INFLIGHT) enqueue_next(c);
             }
             if (!any) break;
         }
@@ -1210,6 +1212,11 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
                 (void)hipEventRecord(ev_t2, s);
                 (void)hipMemcpyAsync(&info, ctx.d_info, sizeof(info), hipMemcpyDeviceToHost, s);
             }
+            // the Ozaki form solves with the block-inverse operators of this
+            // factor: enqueued before the host reads info, so the device goes
+            // on (after a failed factorisation they are never used)
+            const bool want_binv = !spd && ctx.path >= 1 && m >= 512;
+            const bool binv_ok = want_binv && ctx.binv_prepare(s, m, bMat, m) == 0;
             sync();
             float ms1 = 0, ms2 = 0;
             if (hipEventElapsedTime(&ms1, ev_t0, ev_t1) == hipSuccess) t_schur += 1e-3 * ms1;
@@ -1217,8 +1224,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
             if (ctx.path >= 1 && !spd) bmax = sc(slot_bmax);
             if (info == sdk::POTRF_WAIT_FAILED) wait_failed = true;
             if (info != 0) return false;
-            // the Ozaki form solves with the block-inverse operators of this factor
-            binv = !spd && ctx.path >= 1 && m >= 512 && ctx.binv_prepare(s, m, bMat, m) == 0;
+            binv = binv_ok;
         }
         schur_solve();
         sdp::plus_A(s, dp, dy, DzMat);
