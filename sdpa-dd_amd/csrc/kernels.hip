@@ -2935,6 +2935,31 @@ void dot_tasks_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const d
     k_dot_chunks_off<<<dim3((unsigned)T, (unsigned)ntasks), 256, 0, s>>>(d_tasks, ybase, T, partial);
     k_dot_combine_off<<<cdiv(ntasks, 64), 64, 0, s>>>(d_tasks, ntasks, T, partial);
 }
+// The Ozaki form's order: each chunk as 256 strided partial sums merged by
+// a pairwise LDS tree (no sequential chain over the chunk), the chunks then
+// added in order as above.
+__global__ __launch_bounds__(256) void k_dot_chunks_off_fast(const DotTaskOff *tasks, const dd *ybase, int T,
+                                                             dd *partial) {
+    __shared__ dd sm[256];
+    const DotTaskOff tk = tasks[blockIdx.y];
+    const dd *y = ybase + tk.yoff;
+    idx_t st, cnt;
+    chunk_range(tk.n, T, blockIdx.x, st, cnt);
+    dd acc = mk(0.0);
+    for (idx_t e = threadIdx.x; e < cnt; e += 256) acc = dd_add(acc, dd_mul(tk.x[st + e], y[st + e]));
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sm[threadIdx.x] = dd_add(sm[threadIdx.x], sm[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(idx_t)blockIdx.y * T + blockIdx.x] = sm[0];
+}
+void dot_tasks_off_fast(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T, dd *partial) {
+    if (ntasks <= 0) return;
+    k_dot_chunks_off_fast<<<dim3((unsigned)T, (unsigned)ntasks), 256, 0, s>>>(d_tasks, ybase, T, partial);
+    k_dot_combine_off<<<cdiv(ntasks, 64), 64, 0, s>>>(d_tasks, ntasks, T, partial);
+}
 
 // out_i = A_ii for an n x n column-major block.
 __global__ void k_gather_diag(idx_t n, const dd *A, dd *out) {

@@ -161,6 +161,8 @@ void dot_batch_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const d
 // Same result for long tasks: one workgroup per (chunk, task); partial:
 // >= T * ntasks dd.
 void dot_tasks_off(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T, dd *partial);
+// The same with each chunk summed by a pairwise tree (the Ozaki form).
+void dot_tasks_off_fast(hipStream_t s, const DotTaskOff *d_tasks, int ntasks, const dd *ybase, int T, dd *partial);
 // out_i = A_ii of an n x n column-major block.
 void gather_diag(hipStream_t s, idx_t n, const dd *A, dd *out);
 // Lanczos step pieces (Lal::getMinEigen): qold = q, q = v*r; and the tail

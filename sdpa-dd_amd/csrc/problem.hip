@@ -1092,7 +1092,7 @@ __global__ void k_combine_tasks(int m, const idx_t *ptr, const dd *tv, const idx
 // Dense tasks: one thread per chunk while chunks are short; one workgroup
 // per chunk (coalesced loads) once they are long.  Same sums either way.
 static void dense_dots(hipStream_t s, DeviceProblem &dp, const sdk::DotTaskOff *tasks, int ntasks, idx_t maxn,
-                       const dd *W, int T) {
+                       const dd *W, int T, bool fast) {
     if (ntasks <= 0) return;
     const size_t need = (size_t)T * ntasks;
     bool chunked = maxn / T > 64;
@@ -1105,27 +1105,29 @@ static void dense_dots(hipStream_t s, DeviceProblem &dp, const sdk::DotTaskOff *
         else
             chunked = false;
     }
-    if (chunked)
+    if (chunked && fast)
+        sdk::dot_tasks_off_fast(s, tasks, ntasks, W, T, dp.d_dot_partial);
+    else if (chunked)
         sdk::dot_tasks_off(s, tasks, ntasks, W, T, dp.d_dot_partial);
     else
         sdk::dot_batch_off(s, tasks, ntasks, W, T);
 }
 
-void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
+void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T, bool fast) {
     if (dp.ntask > 0)
         k_sparse_tasks<<<dp.ntask, 64, 0, s>>>(dp.ntask, dp.d_task_type, dp.d_task_block, dp.d_task_nnz,
                                                               dp.d_task_off, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val,
                                                               dp.d_boff, dp.d_bsize, W, dp.d_task_val);
-    dense_dots(s, dp, dp.d_dense_tasks, dp.n_dense_tasks, dp.dense_task_maxn, W, T);
+    dense_dots(s, dp, dp.d_dense_tasks, dp.n_dense_tasks, dp.dense_task_maxn, W, T, fast);
     k_combine_tasks<<<(dp.m + 255) / 256, 256, 0, s>>>(dp.m, dp.d_task_ptr, dp.d_task_val, dp.d_lp_ptr, dp.d_lp_idx,
                                                        dp.d_lp_val, W + dp.nn, out);
 }
-void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T) {
+void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T, bool fast) {
     if (dp.nctask > 0)
         k_sparse_tasks<<<dp.nctask, 64, 0, s>>>(dp.nctask, dp.d_ctask_type, dp.d_ctask_block,
                                                                dp.d_ctask_nnz, dp.d_ctask_off, dp.d_sp_row, dp.d_sp_col,
                                                                dp.d_sp_val, dp.d_boff, dp.d_bsize, W, dp.d_ctask_val);
-    dense_dots(s, dp, dp.d_dense_ctasks, dp.n_dense_ctasks, dp.dense_ctask_maxn, W, T);
+    dense_dots(s, dp, dp.d_dense_ctasks, dp.n_dense_ctasks, dp.dense_ctask_maxn, W, T, fast);
     k_combine_tasks<<<1, 32, 0, s>>>(1, dp.d_ctask_ptr, dp.d_ctask_val, dp.d_clp_ptr, dp.d_clp_idx, dp.d_clp_val,
                                      W + dp.nn, out);
 }

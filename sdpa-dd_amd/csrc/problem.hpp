@@ -216,9 +216,9 @@ int upload_problem(const HostProblem &hp, DeviceProblem &dp, hipStream_t s, std:
 void free_problem(DeviceProblem &dp);
 
 // out[i] = sum over blocks of A_i . W (the reference's multi_InnerProductToA).
-void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T);
+void inner_A(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T, bool fast = false);
 // *out = C . W
-void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T);
+void inner_C(hipStream_t s, DeviceProblem &dp, const dd *W, dd *out, int T, bool fast = false);
 // R = sum_i y_i A_i (multi_plusToA)
 void plus_A(hipStream_t s, DeviceProblem &dp, const dd *y, dd *R);
 // R += C

@@ -545,7 +545,7 @@ class Solver {
         sync();
     }
     ddr sc(int slot) const { return ddr(h_sc[slot]); }
-    void enq_inner_C(const dd *W, int slot) { sdp::inner_C(s, dp, W, d_sc + slot, ctx.dot_T); }
+    void enq_inner_C(const dd *W, int slot) { sdp::inner_C(s, dp, W, d_sc + slot, ctx.dot_T, ctx.path >= 1); }
     void enq_inner_b(const dd *v, int slot) { dot(m, dp.d_b, v, slot); }
     void enq_max(const dd *x, idx_t len, int slot) {
         sdk::max_abs(s, len, x, d_sc + slot, ctx.d_max_work, ctx.d_counter + 1);
@@ -617,7 +617,7 @@ class Solver {
         return read_xz();
     }
     ddr inner_C(const dd *W) {
-        sdp::inner_C(s, dp, W, d_sc + slot_C, ctx.dot_T);
+        sdp::inner_C(s, dp, W, d_sc + slot_C, ctx.dot_T, ctx.path >= 1);
         return ddr(*get_scalars(slot_C, 1));
     }
     ddr inner_b(const dd *v) {
@@ -740,7 +740,7 @@ class Solver {
     // Residuals::compute (sdpa_dataset.cpp:625-645); the norms go to
     // slot_max (primal) and slot_max2 (dual)
     void enq_residuals() {
-        sdp::inner_A(s, dp, X, primalVec, ctx.dot_T);
+        sdp::inner_A(s, dp, X, primalVec, ctx.dot_T, ctx.path >= 1);
         sdk::scal(s, m, mk(-1.0), primalVec);
         sdk::axpy(s, m, mk(1.0), dp.d_b, primalVec);
         sdp::plus_A(s, dp, y, dualMat);
@@ -1186,7 +1186,7 @@ int Solver::run(SolveOutput &res, std::vector<dd> *xo, std::vector<dd> *yo, std:
         } else {
             sdk::copy(s, nt, r_zinv, DLS1);
         }
-        sdp::inner_A(s, dp, DLS1, g, ctx.dot_T);
+        sdp::inner_A(s, dp, DLS1, g, ctx.dot_T, ctx.path >= 1);
         sdk::scal(s, m, mk(-1.0), g);
         sdk::axpy(s, m, mk(1.0), primalVec, g);
         if (predictor) {
