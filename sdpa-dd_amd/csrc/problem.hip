@@ -1357,8 +1357,29 @@ __global__ void k_scatter_pairs(int npairs, int m, const int *pi, const int *pj,
 }
 // B[e] = 0 + value of the pair that visits element e (the zeroed matrix and
 // one "+=", as k_scatter_pairs gives), or 0; one SDP block only.
-__global__ void k_gather_pairs(idx_t len, const int *pidx, const dd *pv, dd *B) {
-    for (idx_t e = (idx_t)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (idx_t)gridDim.x * blockDim.x) {
+// B_ij = pair value pidx_ij (zero for -1), by 64 x 64 tiles: B_ij and B_ji
+// read the same pair, and the pair values run contiguously along one of
+// the two (the pair list is built row by row), so one half of B reads them
+// strided.  Workgroups 16 q + x and 16 q + x + 8 (both on XCD x) take the
+// tiles (a, b) and (b, a) of the u = 8 q + x-th tile pair a >= b, so a
+// strided pair line is read by the workgroup beside the one that reads it
+// contiguously, through the same L2.
+__global__ __launch_bounds__(256) void k_gather_pairs(idx_t m, idx_t ntp, const int *pidx, const dd *pv, dd *B) {
+    const idx_t L = blockIdx.x, u = (L / 16) * 8 + L % 8;
+    const int h = (int)((L / 8) & 1);
+    if (u >= ntp) return;
+    idx_t a = (idx_t)((sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
+    while ((a + 1) * (a + 2) / 2 <= u) a++;
+    while (a * (a + 1) / 2 > u) a--;
+    const idx_t b = u - a * (a + 1) / 2;
+    if (h && a == b) return;
+    const idx_t i = (h ? b : a) * 64 + (threadIdx.x & 63), j0 = (h ? a : b) * 64 + (threadIdx.x >> 6);
+    if (i >= m) return;
+#pragma unroll 4
+    for (int r = 0; r < 16; r++) {
+        const idx_t j = j0 + 4 * r;
+        if (j >= m) break;
+        const idx_t e = i + m * j;
         const int p = pidx[e];
         B[e] = p >= 0 ? dd_add_eq(mk(0.0), pv[p]) : mk(0.0);
     }
@@ -1444,9 +1465,10 @@ void schur(hipStream_t s, DeviceProblem &dp, const dd *X, const dd *invZ, dd *B,
                                                 S.d_off, S.d_wslot, dp.d_sp_row, dp.d_sp_col, dp.d_sp_val, Xl,
                                                 dp.d_work1, dp.d_pair_val);
             if (S.ndense_pairs > 0) sdk::dot_batch(s, S.d_pair_tasks, S.ndense_pairs, T);
-            if (dp.d_pidx)
-                k_gather_pairs<<<2048, 256, 0, s>>>((idx_t)m * m, dp.d_pidx, dp.d_pair_val, B);
-            else
+            if (dp.d_pidx) {
+                const idx_t nt = ((idx_t)m + 63) / 64, ntp = nt * (nt + 1) / 2;
+                k_gather_pairs<<<(unsigned)(16 * ((ntp + 7) / 8)), 256, 0, s>>>(m, ntp, dp.d_pidx, dp.d_pair_val, B);
+            } else
                 k_scatter_pairs<<<(S.npairs + 255) / 256, 256, 0, s>>>(S.npairs, m, S.d_pair_i, S.d_pair_j,
                                                                        dp.d_pair_val, B);
         }
