@@ -76,8 +76,8 @@ MODEL = {
 # from two rocprofv3 --pmc passes (FETCH_SIZE and WRITE_SIZE, scaled by the
 # factors a 256 MiB copy needs to read its true size).
 TRAFFIC = {
-    "theta": {"ozaki": 35.26e6 + 12.57e6, "gemm": 78.77e6 + 42.21e6, "potrf": 4.36e6 + 5.64e6,
-              "trsv": 176.96e6 + 0.36e6},
+    "theta": {"ozaki": 35.26e6 + 12.58e6, "gemm": 78.77e6 + 42.21e6, "potrf": 4.35e6 + 5.64e6,
+              "trsv": 176.97e6 + 0.36e6},
 }
 TRAFFIC_SOURCE = "profiles/hbm_traffic_theta300.md"
 
